@@ -1,0 +1,145 @@
+// Host/device helpers shared by the libcia_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/cia.h"
+#include "point.hpp"
+
+namespace cia {
+
+// Last HIP error message of the calling thread (exposed by cia_last_error).
+void set_error(const char* what, hipError_t err, const char* file, int line);
+void set_error_msg(const std::string& msg);
+
+#define CIA_HIP(x)                                        \
+  do {                                                    \
+    hipError_t e_ = (x);                                  \
+    if (e_ != hipSuccess) {                               \
+      ::cia::set_error(#x, e_, __FILE__, __LINE__);       \
+      return CIA_ERR_HIP;                                 \
+    }                                                     \
+  } while (0)
+
+#define CIA_ARG(cond, msg)                                \
+  do {                                                    \
+    if (!(cond)) {                                        \
+      ::cia::set_error_msg(msg);                          \
+      return CIA_ERR_INVALID_ARGUMENT;                    \
+    }                                                     \
+  } while (0)
+
+inline PhaseConst make_phase(const cia_material& m) {
+  PhaseConst p{};
+  p.kind = m.kind;
+  const double E = m.young_modulus, nu = m.poisson_ratio;
+  p.G = E / (2.0 * (1.0 + nu));
+  p.K = E / (3.0 * (1.0 - 2.0 * nu));
+  p.lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  p.sy = m.yield_stress;
+  p.H = m.hardening_modulus;
+  p.tau = m.time_scale;
+  p.sref = m.reference_stress;
+  p.m = m.rate_exponent;
+  p.hs = 3.0 * p.G + p.H;
+  p.density = m.density;
+  return p;
+}
+
+inline Settings make_settings(const cia_update_settings& s) {
+  Settings o{};
+  o.max_iterations = s.max_iterations;
+  o.tol_rel_yield = s.yield_relative_tolerance;
+  o.res_abs = s.residual_absolute_tolerance;
+  o.res_rel = s.residual_relative_tolerance;
+  return o;
+}
+
+// Per-device scratch kept for the life of the process: phase table and the
+// failure reporting slots of the standalone material update.
+struct DeviceCtx {
+  PhaseTable* tab = nullptr;
+  unsigned long long* first_fail = nullptr;
+  cia_point_failure* rec = nullptr;
+};
+cia_status device_ctx(DeviceCtx** out);
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  return (unsigned)((n + block - 1) / block);
+}
+
+// ---- deterministic block reductions --------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum over a block of up to 1024 threads; result valid in thread 0.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) r += lds[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double* lds) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = -INFINITY;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) r = fmax(r, lds[w]);
+  }
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_min(double v, double* lds) {
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double r = INFINITY;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) r = fmin(r, lds[w]);
+  }
+  __syncthreads();
+  return r;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v);
+
+}  // namespace cia

@@ -1,0 +1,114 @@
+// Hex8 kinematics on uniform box elements (gfx950 device helpers).
+//
+// Shape functions follow src/fem/elements/lagrange.py:50-76 (local node
+// a = ax + 2 ay + 4 az, x fastest) and the 2x2x2 Gauss rule of
+// src/fem/quadrature/rules.py:24-33 (point q = 4 qx + 2 qy + qz, zeta fastest,
+// coordinates -1/sqrt(3), +1/sqrt(3), unit weights).  For a box element of
+// edges (hx, hy, hz) the Jacobian is constant, so
+//     dN_a/dx (q) = s(ax)/hx * W(qy, ay) * W(qz, az),   W(q, a) = (1 + s(a) p(q)) / 2,
+// and w_q det J = hx hy hz / 8.  The 6x24 B matrix of
+// src/fem/assembly/elasticity.py:160-183 is applied without being formed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cia {
+
+constexpr double kGp = 0.57735026918962576451;  // 1/sqrt(3)
+constexpr double kWp = 0.5 * (1.0 + kGp);
+constexpr double kWm = 0.5 * (1.0 - kGp);
+
+__host__ __device__ constexpr double wq(int q, int a) { return q == a ? kWp : kWm; }
+__host__ __device__ constexpr double sgn(int a) { return a ? 1.0 : -1.0; }
+
+// Reference-coordinate gradient factors (without the 1/h scaling).
+__host__ __device__ constexpr double dnx(int q, int a) {
+  return sgn(a & 1) * wq((q >> 1) & 1, (a >> 1) & 1) * wq(q & 1, (a >> 2) & 1);
+}
+__host__ __device__ constexpr double dny(int q, int a) {
+  return sgn((a >> 1) & 1) * wq((q >> 2) & 1, a & 1) * wq(q & 1, (a >> 2) & 1);
+}
+__host__ __device__ constexpr double dnz(int q, int a) {
+  return sgn((a >> 2) & 1) * wq((q >> 2) & 1, a & 1) * wq((q >> 1) & 1, (a >> 1) & 1);
+}
+
+struct BoxGeom {
+  double ihx, ihy, ihz;  // 1 / element edge
+  double jw;             // w_q det J
+};
+
+// Engineering strain at Gauss point Q from nodal values u[a][c].
+template <int Q, class T>
+__device__ __forceinline__ void strain_at(const T u[8][3], const BoxGeom& g, T e[6]) {
+  T gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gx[c] += (T)dnx(Q, a) * u[a][c];
+      gy[c] += (T)dny(Q, a) * u[a][c];
+      gz[c] += (T)dnz(Q, a) * u[a][c];
+    }
+  }
+  const T ix = (T)g.ihx, iy = (T)g.ihy, iz = (T)g.ihz;
+  e[0] = ix * gx[0];
+  e[1] = iy * gy[1];
+  e[2] = iz * gz[2];
+  e[3] = iz * gz[1] + iy * gy[2];
+  e[4] = iz * gz[0] + ix * gx[2];
+  e[5] = iy * gy[0] + ix * gx[1];
+}
+
+// f[a][c] += w detJ (B_Q^T sigma)[a][c].
+template <int Q, class T>
+__device__ __forceinline__ void force_at(const T s[6], const BoxGeom& g, T f[8][3]) {
+  const T jx = (T)(g.jw * g.ihx), jy = (T)(g.jw * g.ihy), jz = (T)(g.jw * g.ihz);
+  const T tx0 = jx * s[0], tx1 = jx * s[5], tx2 = jx * s[4];
+  const T ty0 = jy * s[5], ty1 = jy * s[1], ty2 = jy * s[3];
+  const T tz0 = jz * s[4], tz1 = jz * s[3], tz2 = jz * s[2];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const T bx = (T)dnx(Q, a), by = (T)dny(Q, a), bz = (T)dnz(Q, a);
+    f[a][0] += bx * tx0 + by * ty0 + bz * tz0;
+    f[a][1] += bx * tx1 + by * ty1 + bz * tz1;
+    f[a][2] += bx * tx2 + by * ty2 + bz * tz2;
+  }
+}
+
+// Strain columns of B_Q for node a (unit displacement in x, y, z).
+template <class T>
+__device__ __forceinline__ void b_columns(int q, int a, const BoxGeom& g, T col[3][6]) {
+  const T bx = (T)(dnx(q, a) * g.ihx), by = (T)(dny(q, a) * g.ihy), bz = (T)(dnz(q, a) * g.ihz);
+  // unit u_x: (bx, 0, 0, 0, bz, by); u_y: (0, by, 0, bz, 0, bx); u_z: (0, 0, bz, by, bx, 0)
+  col[0][0] = bx; col[0][1] = 0;  col[0][2] = 0;  col[0][3] = 0;  col[0][4] = bz; col[0][5] = by;
+  col[1][0] = 0;  col[1][1] = by; col[1][2] = 0;  col[1][3] = bz; col[1][4] = 0;  col[1][5] = bx;
+  col[2][0] = 0;  col[2][1] = 0;  col[2][2] = bz; col[2][3] = by; col[2][4] = bx; col[2][5] = 0;
+}
+
+// Element (i, j, k) of a periodic nx x ny x nz grid: node classes of its 8 corners.
+__device__ __forceinline__ void periodic_corners(int64_t e, int nx, int ny, int nz, int64_t c[8],
+                                                 int& i, int& j, int& k) {
+  i = (int)(e % nx);
+  j = (int)((e / nx) % ny);
+  k = (int)(e / ((int64_t)nx * ny));
+  const int ip = i + 1 == nx ? 0 : i + 1;
+  const int jp = j + 1 == ny ? 0 : j + 1;
+  const int kp = k + 1 == nz ? 0 : k + 1;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int ii = (a & 1) ? ip : i, jj = (a & 2) ? jp : j, kk = (a & 4) ? kp : k;
+    c[a] = ii + (int64_t)nx * (jj + (int64_t)ny * kk);
+  }
+}
+
+// Element that holds periodic node (i, j, k) as its corner a.
+__device__ __forceinline__ int64_t periodic_corner_element(int i, int j, int k, int a, int nx,
+                                                           int ny, int nz) {
+  int ii = i - (a & 1), jj = j - ((a >> 1) & 1), kk = k - ((a >> 2) & 1);
+  if (ii < 0) ii += nx;
+  if (jj < 0) jj += ny;
+  if (kk < 0) kk += nz;
+  return ii + (int64_t)nx * (jj + (int64_t)ny * kk);
+}
+
+}  // namespace cia

@@ -1,0 +1,1521 @@
+// Structured periodic Hex8 RVE on one MI355X: Newton microequilibrium with a
+// matrix-free block-Jacobi PCG linear solve, Armijo line search and the
+// homogenised response.
+//
+// Reference: src/fem/rve/solver.py:62-518 (solve_rve_microequilibrium,
+// _build_rve_response, _compute_effective_tangent), src/fem/rve/periodic.py
+// (periodic classes and affine map), src/fem/solvers/line_search.py:40-67,
+// src/fem/assembly/nonlinear_solid.py:113-180,428-493.
+//
+// Periodic structure.  On an nx x ny x nz box the periodic classes are the
+// nodes of an nx x ny x nz periodic grid, class c = i + nx (j + ny k), which is
+// the reference's class numbering (periodic.py:32-37).  Class 0 is the
+// eliminated reference class.  The reduced unknown vector of the reference is
+// w[3:] of the class vector w used here, and T^T f is the class-summed element
+// force, which a periodic node gather produces directly.  u = T w + A eps is
+// never formed: B (A eps) = eps exactly on a box element, so every Gauss-point
+// strain is eps_macro + B_q w_e.
+//
+// Kernels (one thread per element or per node):
+//   assemble   strain -> constitutive update -> element force rows F[24][E]
+//   gather     node sum of F over the 8 elements of each class (fixed order,
+//              no atomics: bitwise deterministic), residual norm partials
+//   pcg_elem   p = z + beta p_old fused into the element gather, C_q eps, F
+//   pcg_node   y = gather(F), p.y partials; last block computes alpha
+//   pcg_update x, r, z = M^-1 r, r.z and r.r partials; last block computes beta
+// Every reduction is a fixed-order block tree plus a last-block pass over the
+// block partials.
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "hex8.hpp"
+
+namespace cia {
+
+struct Scalars {
+  double rz, pq, alpha, beta, rr, tol2;
+  double norm2[4];
+  int32_t done;  // 0 running, 1 converged, 2 breakdown
+  int32_t iters;
+};
+
+struct Red {
+  double* partials;    // [slots][maxblocks]
+  unsigned* counter;
+  int maxblocks;
+};
+
+// Last-block finalisation of NV block partial sums into out[0..NV).
+template <int NV>
+__device__ __forceinline__ void finalize_sums(const double (&v)[NV], double* partials,
+                                              unsigned* counter, double* out, int nblocks,
+                                              double* lds, bool* is_last) {
+  double tot[NV];
+#pragma unroll
+  for (int s = 0; s < NV; ++s) tot[s] = block_sum<kBlock>(v[s], lds);
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int s = 0; s < NV; ++s) partials[s * nblocks + blockIdx.x] = tot[s];
+    __threadfence();
+    const unsigned prev = atomicAdd(counter, 1u);
+    last = prev == (unsigned)(nblocks - 1);
+  }
+  __syncthreads();
+  *is_last = last;
+  if (!last) return;
+  __threadfence();
+#pragma unroll
+  for (int s = 0; s < NV; ++s) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+      acc += __builtin_nontemporal_load(&partials[s * nblocks + b]);
+    acc = block_sum<kBlock>(acc, lds);
+    if (threadIdx.x == 0) out[s] = acc;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+struct Grid {
+  int nx, ny, nz;
+  int pad;
+  int64_t E;  // elements = classes
+  BoxGeom g;
+};
+
+struct Macro {
+  double e[6];   // requested macro strain
+  double ec[6];  // committed macro strain
+};
+
+// -------------------------------------------------------------------------
+// Assembly: strain -> constitutive update -> element forces.
+template <class T, bool EXTRA>
+__global__ void __launch_bounds__(kBlock)
+    k_assemble(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+               Settings S, double dt, const double* __restrict__ w, const double* __restrict__ wc,
+               Macro M, const T* __restrict__ state_c, T* __restrict__ state_t,
+               T* __restrict__ sig, T* __restrict__ tan, T* __restrict__ F,
+               T* __restrict__ strain_out, T* __restrict__ rate_out, T* __restrict__ psi_out,
+               T* __restrict__ diss_out, T* __restrict__ diag_out,
+               unsigned long long* __restrict__ first_fail) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  int64_t c[8];
+  int i, j, k;
+  periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+  T we[8][3], dwe[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double v = w[3 * c[a] + m];
+      we[a][m] = (T)v;
+      if (EXTRA) dwe[a][m] = (T)(v - wc[3 * c[a] + m]);
+    }
+  const PhaseConst P = tab->p[ephase[e]];
+  const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
+  T f[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = (T)0;
+  int status = CIA_PT_OK;
+  int bad_q = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t p = 8 * e + q;
+    T eps[6], rate[6];
+    switch (q) {
+      case 0: strain_at<0>(we, G.g, eps); break;
+      case 1: strain_at<1>(we, G.g, eps); break;
+      case 2: strain_at<2>(we, G.g, eps); break;
+      case 3: strain_at<3>(we, G.g, eps); break;
+      case 4: strain_at<4>(we, G.g, eps); break;
+      case 5: strain_at<5>(we, G.g, eps); break;
+      case 6: strain_at<6>(we, G.g, eps); break;
+      default: strain_at<7>(we, G.g, eps); break;
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
+    if (EXTRA) {
+      switch (q) {
+        case 0: strain_at<0>(dwe, G.g, rate); break;
+        case 1: strain_at<1>(dwe, G.g, rate); break;
+        case 2: strain_at<2>(dwe, G.g, rate); break;
+        case 3: strain_at<3>(dwe, G.g, rate); break;
+        case 4: strain_at<4>(dwe, G.g, rate); break;
+        case 5: strain_at<5>(dwe, G.g, rate); break;
+        case 6: strain_at<6>(dwe, G.g, rate); break;
+        default: strain_at<7>(dwe, G.g, rate); break;
+      }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
+    }
+    T epsp[6];
+    T qb = (T)0, ga = (T)0;
+    if (j2) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * np + p];
+      qb = state_c[6 * np + p];
+      ga = state_c[7 * np + p];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
+    }
+    PointOut<T> o;
+    PointExtra<T> ex;
+    const int st = material_point<T, EXTRA>(P, S, dt, eps, EXTRA ? rate : nullptr, epsp, qb, ga,
+                                            o, EXTRA ? &ex : nullptr, nullptr);
+    if (st != CIA_PT_OK && status == CIA_PT_OK) {
+      status = st;
+      bad_q = q;
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) sig[m * np + p] = o.sig[m];
+    if (j2) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) state_t[m * np + p] = o.epsp[m];
+      state_t[6 * np + p] = o.qbar;
+      state_t[7 * np + p] = o.gam;
+    }
+    tan[0 * np + p] = o.tan.d;
+    tan[1 * np + p] = o.tan.r;
+    if (o.tan.r != (T)0) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tan[(2 + m) * np + p] = o.tan.n[m];
+    }
+    if (EXTRA) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        strain_out[m * np + p] = eps[m];
+        rate_out[m * np + p] = rate[m];
+      }
+      psi_out[p] = ex.psi;
+      diss_out[p] = ex.diss;
+#pragma unroll
+      for (int m = 0; m < CIA_N_DIAG; ++m) diag_out[m * np + p] = ex.d[m];
+    }
+    switch (q) {
+      case 0: force_at<0>(o.sig, G.g, f); break;
+      case 1: force_at<1>(o.sig, G.g, f); break;
+      case 2: force_at<2>(o.sig, G.g, f); break;
+      case 3: force_at<3>(o.sig, G.g, f); break;
+      case 4: force_at<4>(o.sig, G.g, f); break;
+      case 5: force_at<5>(o.sig, G.g, f); break;
+      case 6: force_at<6>(o.sig, G.g, f); break;
+      default: force_at<7>(o.sig, G.g, f); break;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[(3 * a + m) * G.E + e] = f[a][m];
+  if (status != CIA_PT_OK) atomicMin(first_fail, (unsigned long long)(8 * e + bad_q));
+}
+
+// Failure record of point p (re-evaluated with the record attached).
+template <class T>
+__global__ void k_diagnose(Grid G, const PhaseTable* __restrict__ tab,
+                           const uint8_t* __restrict__ ephase, Settings S, double dt,
+                           const double* __restrict__ w, const double* __restrict__ wc, Macro M,
+                           const T* __restrict__ state_c, int64_t p, cia_point_failure* rec) {
+  const int64_t e = p / 8;
+  const int q = (int)(p % 8);
+  const int64_t np = G.E * 8;
+  int64_t c[8];
+  int i, j, k;
+  periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+  T we[8][3], dwe[8][3];
+  for (int a = 0; a < 8; ++a)
+    for (int m = 0; m < 3; ++m) {
+      we[a][m] = (T)w[3 * c[a] + m];
+      dwe[a][m] = (T)(w[3 * c[a] + m] - wc[3 * c[a] + m]);
+    }
+  T eps[6], rate[6];
+  switch (q) {
+    case 0: strain_at<0>(we, G.g, eps); strain_at<0>(dwe, G.g, rate); break;
+    case 1: strain_at<1>(we, G.g, eps); strain_at<1>(dwe, G.g, rate); break;
+    case 2: strain_at<2>(we, G.g, eps); strain_at<2>(dwe, G.g, rate); break;
+    case 3: strain_at<3>(we, G.g, eps); strain_at<3>(dwe, G.g, rate); break;
+    case 4: strain_at<4>(we, G.g, eps); strain_at<4>(dwe, G.g, rate); break;
+    case 5: strain_at<5>(we, G.g, eps); strain_at<5>(dwe, G.g, rate); break;
+    case 6: strain_at<6>(we, G.g, eps); strain_at<6>(dwe, G.g, rate); break;
+    default: strain_at<7>(we, G.g, eps); strain_at<7>(dwe, G.g, rate); break;
+  }
+  for (int m = 0; m < 6; ++m) {
+    eps[m] += (T)M.e[m];
+    rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
+  }
+  const PhaseConst P = tab->p[ephase[e]];
+  T epsp[6] = {0, 0, 0, 0, 0, 0};
+  T qb = 0, ga = 0;
+  if (P.kind == CIA_MAT_J2_VISCOPLASTIC) {
+    for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * np + p];
+    qb = state_c[6 * np + p];
+    ga = state_c[7 * np + p];
+  }
+  cia_point_failure r;
+  memset(&r, 0, sizeof(r));
+  PointOut<T> o;
+  PointExtra<T> ex;
+  r.reason = material_point<T, true>(P, S, dt, eps, rate, epsp, qb, ga, o, &ex, &r);
+  r.point_id = p;
+  *rec = r;
+}
+
+// Periodic node gather of the element force rows.  Class 0 is zeroed after its
+// raw sum is kept in class0[3] (for the class reaction diagnostic).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_gather(Grid G, const T* __restrict__ F, double* __restrict__ r, double* __restrict__ class0,
+             double* partials, unsigned* counter, double* norm2_out) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (c < G.E) {
+    const int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+    double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int64_t e = periodic_corner_element(i, j, k, a, G.nx, G.ny, G.nz);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) s[m] += (double)F[(3 * a + m) * G.E + e];
+    }
+    if (c == 0) {
+      if (class0) {
+        class0[0] = s[0];
+        class0[1] = s[1];
+        class0[2] = s[2];
+      }
+      s[0] = s[1] = s[2] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      r[3 * c + m] = s[m];
+      acc += s[m] * s[m];
+    }
+  }
+  const double v[1] = {acc};
+  bool last;
+  finalize_sums<1>(v, partials, counter, norm2_out, (int)gridDim.x, lds, &last);
+}
+
+// ||x||^2, ||y||^2 (y may be null) -> out[0..1]
+__global__ void __launch_bounds__(kBlock)
+    k_norms(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+            double* partials, unsigned* counter, double* out) {
+  __shared__ double lds[kBlock / 64];
+  double a = 0.0, b = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double xv = x[t];
+    a += xv * xv;
+    if (y) {
+      const double yv = y[t];
+      b += yv * yv;
+    }
+  }
+  const double v[2] = {a, b};
+  bool last;
+  finalize_sums<2>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+}
+
+// out = alpha * x
+__global__ void k_scale(int64_t n, double alpha, const double* __restrict__ x,
+                        double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = alpha * x[t];
+}
+
+// out = x + alpha * y
+__global__ void k_axpy(int64_t n, const double* __restrict__ x, double alpha,
+                       const double* __restrict__ y, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = x[t] + alpha * y[t];
+}
+
+// -------------------------------------------------------------------------
+// Block-Jacobi preconditioner: inverse of the 3x3 diagonal block per class.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_precond(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+              const T* __restrict__ tan, double* __restrict__ minv) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G.E) return;
+  const int64_t np = G.E * 8;
+  const int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+  double kb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a = 0; a < 8; ++a) {
+    const int64_t e = periodic_corner_element(i, j, k, a, G.nx, G.ny, G.nz);
+    const double K = tab->p[ephase[e]].K;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t p = 8 * e + q;
+      Tangent<double> tc;
+      tc.K = K;
+      tc.d = (double)tan[p];
+      tc.r = (double)tan[np + p];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != 0.0 ? (double)tan[(2 + m) * np + p] : 0.0;
+      double col[3][6];
+      b_columns(q, a, G.g, col);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double s[6];
+        tangent_apply(tc, col[m], s);
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          double v = 0.0;
+#pragma unroll
+          for (int z = 0; z < 6; ++z) v += col[l][z] * s[z];
+          kb[l][m] += G.g.jw * v;
+        }
+      }
+    }
+  }
+  double inv[6];
+  if (c == 0) {
+    inv[0] = inv[3] = inv[5] = 1.0;
+    inv[1] = inv[2] = inv[4] = 0.0;
+  } else {
+    const double a00 = kb[0][0], a01 = 0.5 * (kb[0][1] + kb[1][0]), a02 = 0.5 * (kb[0][2] + kb[2][0]);
+    const double a11 = kb[1][1], a12 = 0.5 * (kb[1][2] + kb[2][1]), a22 = kb[2][2];
+    const double c00 = a11 * a22 - a12 * a12, c01 = a02 * a12 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    const double id = 1.0 / det;
+    inv[0] = c00 * id;
+    inv[1] = c01 * id;
+    inv[2] = c02 * id;
+    inv[3] = (a00 * a22 - a02 * a02) * id;
+    inv[4] = (a01 * a02 - a00 * a12) * id;
+    inv[5] = (a00 * a11 - a01 * a01) * id;
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) minv[m * G.E + c] = inv[m];
+}
+
+__device__ __forceinline__ void apply_minv(const double* __restrict__ minv, int64_t C, int64_t c,
+                                           const double r[3], double z[3]) {
+  const double m0 = minv[c], m1 = minv[C + c], m2 = minv[2 * C + c], m3 = minv[3 * C + c],
+               m4 = minv[4 * C + c], m5 = minv[5 * C + c];
+  z[0] = m0 * r[0] + m1 * r[1] + m2 * r[2];
+  z[1] = m1 * r[0] + m3 * r[1] + m4 * r[2];
+  z[2] = m2 * r[0] + m4 * r[1] + m5 * r[2];
+}
+
+// -------------------------------------------------------------------------
+// PCG.  x = 0, r = b, z = M^-1 r, p_old = 0; rz, rr and the tolerance.
+__global__ void __launch_bounds__(kBlock)
+    k_pcg_init(int64_t C, const double* __restrict__ b, const double* __restrict__ minv,
+               double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+               double* __restrict__ p_old, double rtol, double atol, Scalars* sc, double* partials,
+               unsigned* counter) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (c < C) {
+    double rv[3], zv[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) rv[m] = b[3 * c + m];
+    apply_minv(minv, C, c, rv, zv);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      x[3 * c + m] = 0.0;
+      r[3 * c + m] = rv[m];
+      z[3 * c + m] = zv[m];
+      p_old[3 * c + m] = 0.0;
+      rz += rv[m] * zv[m];
+      rr += rv[m] * rv[m];
+    }
+  }
+  const double v[2] = {rz, rr};
+  double out[2];
+  bool last;
+  finalize_sums<2>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  if (last && threadIdx.x == 0) {
+    sc->rz = out[0];
+    sc->rr = out[1];
+    sc->beta = 0.0;
+    const double tol = fmax(rtol * sqrt(out[1]), atol);
+    sc->tol2 = tol * tol;
+    sc->iters = 0;
+    sc->done = (out[1] <= sc->tol2) ? 1 : 0;
+    if (!(out[0] > 0.0) && sc->done == 0) sc->done = 2;
+  }
+}
+
+// Element part of y = K p with p = z + beta p_old fused in.  p_new is written
+// for the class whose (0,0,0) corner is this element (one writer per class).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pcg_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+               const T* __restrict__ tan, const double* __restrict__ z,
+               const double* __restrict__ p_old, double* __restrict__ p_new,
+               T* __restrict__ F, const Scalars* __restrict__ sc) {
+  if (sc->done) return;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= G.E) return;
+  const double beta = sc->beta;
+  const int64_t np = G.E * 8;
+  int64_t c[8];
+  int i, j, k;
+  periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+  T pe[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double v = z[3 * c[a] + m] + beta * p_old[3 * c[a] + m];
+      pe[a][m] = (T)v;
+      if (a == 0) p_new[3 * e + m] = v;
+    }
+  Tangent<T> tc;
+  tc.K = (T)tab->p[ephase[e]].K;
+  T f[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = (T)0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t p = 8 * e + q;
+    tc.d = tan[p];
+    tc.r = tan[np + p];
+    if (tc.r != (T)0) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * np + p];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
+    }
+    T eps[6], s[6];
+    switch (q) {
+      case 0: strain_at<0>(pe, G.g, eps); break;
+      case 1: strain_at<1>(pe, G.g, eps); break;
+      case 2: strain_at<2>(pe, G.g, eps); break;
+      case 3: strain_at<3>(pe, G.g, eps); break;
+      case 4: strain_at<4>(pe, G.g, eps); break;
+      case 5: strain_at<5>(pe, G.g, eps); break;
+      case 6: strain_at<6>(pe, G.g, eps); break;
+      default: strain_at<7>(pe, G.g, eps); break;
+    }
+    tangent_apply(tc, eps, s);
+    switch (q) {
+      case 0: force_at<0>(s, G.g, f); break;
+      case 1: force_at<1>(s, G.g, f); break;
+      case 2: force_at<2>(s, G.g, f); break;
+      case 3: force_at<3>(s, G.g, f); break;
+      case 4: force_at<4>(s, G.g, f); break;
+      case 5: force_at<5>(s, G.g, f); break;
+      case 6: force_at<6>(s, G.g, f); break;
+      default: force_at<7>(s, G.g, f); break;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[(3 * a + m) * G.E + e] = f[a][m];
+}
+
+// y = gather(F) (class 0 zeroed), p.y -> alpha.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_pcg_node(Grid G, const T* __restrict__ F, const double* __restrict__ p,
+               double* __restrict__ y, Scalars* sc, double* partials, unsigned* counter) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double pq = 0.0;
+  if (c < G.E) {
+    const int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+    double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int64_t e = periodic_corner_element(i, j, k, a, G.nx, G.ny, G.nz);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) s[m] += (double)F[(3 * a + m) * G.E + e];
+    }
+    if (c == 0) s[0] = s[1] = s[2] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      y[3 * c + m] = s[m];
+      pq += p[3 * c + m] * s[m];
+    }
+  }
+  const double v[1] = {pq};
+  double out[1];
+  bool last;
+  finalize_sums<1>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  if (last && threadIdx.x == 0) {
+    sc->pq = out[0];
+    if (!(out[0] > 0.0) || !isfinite(out[0])) {
+      sc->done = 2;
+      sc->alpha = 0.0;
+    } else {
+      sc->alpha = sc->rz / out[0];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_pcg_update(int64_t C, const double* __restrict__ minv, const double* __restrict__ p,
+                 const double* __restrict__ y, double* __restrict__ x, double* __restrict__ r,
+                 double* __restrict__ z, Scalars* sc, double* partials, unsigned* counter) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (c < C) {
+    double rv[3], zv[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      x[3 * c + m] += alpha * p[3 * c + m];
+      rv[m] = r[3 * c + m] - alpha * y[3 * c + m];
+      r[3 * c + m] = rv[m];
+    }
+    apply_minv(minv, C, c, rv, zv);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      z[3 * c + m] = zv[m];
+      rz += rv[m] * zv[m];
+      rr += rv[m] * rv[m];
+    }
+  }
+  const double v[2] = {rz, rr};
+  double out[2];
+  bool last;
+  finalize_sums<2>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  if (last && threadIdx.x == 0) {
+    sc->beta = out[0] / sc->rz;
+    sc->rz = out[0];
+    sc->rr = out[1];
+    sc->iters += 1;
+    if (out[1] <= sc->tol2) sc->done = 1;
+    else if (!isfinite(out[1]) || !(out[0] > 0.0)) sc->done = 2;
+  }
+}
+
+// -------------------------------------------------------------------------
+// Effective tangent: element forces of the uniform strain e_j (K A e_j).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_affine_force(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+                   const T* __restrict__ tan, int jdir, T* __restrict__ F) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  Tangent<T> tc;
+  tc.K = (T)tab->p[ephase[e]].K;
+  T f[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = (T)0;
+  T ej[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) ej[m] = (m == jdir) ? (T)1 : (T)0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t p = 8 * e + q;
+    tc.d = tan[p];
+    tc.r = tan[np + p];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != (T)0 ? tan[(2 + m) * np + p] : (T)0;
+    T s[6];
+    tangent_apply(tc, ej, s);
+    switch (q) {
+      case 0: force_at<0>(s, G.g, f); break;
+      case 1: force_at<1>(s, G.g, f); break;
+      case 2: force_at<2>(s, G.g, f); break;
+      case 3: force_at<3>(s, G.g, f); break;
+      case 4: force_at<4>(s, G.g, f); break;
+      case 5: force_at<5>(s, G.g, f); break;
+      case 6: force_at<6>(s, G.g, f); break;
+      default: force_at<7>(s, G.g, f); break;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[(3 * a + m) * G.E + e] = f[a][m];
+}
+
+// Column jdir of sum_q C_q (e_j - B_q s) (times jw, summed per block).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_ceff_column(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+                  const T* __restrict__ tan, const double* __restrict__ s, int jdir,
+                  double* __restrict__ partials) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (e < G.E) {
+    const int64_t np = G.E * 8;
+    int64_t c[8];
+    int i, j, k;
+    periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+    double se[8][3];
+    for (int a = 0; a < 8; ++a)
+      for (int m = 0; m < 3; ++m) se[a][m] = s[3 * c[a] + m];
+    Tangent<double> tc;
+    tc.K = tab->p[ephase[e]].K;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t p = 8 * e + q;
+      tc.d = (double)tan[p];
+      tc.r = (double)tan[np + p];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != 0.0 ? (double)tan[(2 + m) * np + p] : 0.0;
+      double eps[6], sg[6];
+      switch (q) {
+        case 0: strain_at<0>(se, G.g, eps); break;
+        case 1: strain_at<1>(se, G.g, eps); break;
+        case 2: strain_at<2>(se, G.g, eps); break;
+        case 3: strain_at<3>(se, G.g, eps); break;
+        case 4: strain_at<4>(se, G.g, eps); break;
+        case 5: strain_at<5>(se, G.g, eps); break;
+        case 6: strain_at<6>(se, G.g, eps); break;
+        default: strain_at<7>(se, G.g, eps); break;
+      }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) eps[m] = (m == jdir ? 1.0 : 0.0) - eps[m];
+      tangent_apply(tc, eps, sg);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) acc[m] += sg[m];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    const double t = block_sum<kBlock>(acc[m], lds);
+    if (threadIdx.x == 0) partials[m * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// -------------------------------------------------------------------------
+// Response reductions over elements (sums per block, summed on the host).
+enum {
+  R_SIG = 0,       // 6
+  R_PSI = 6,
+  R_DISS = 7,
+  R_MICRO_WORK = 8,
+  R_POS_WORK = 9,
+  R_NEG_WORK = 10,
+  R_UNLOAD = 11,
+  R_ACTIVE = 12,
+  R_MARGIN_N = 13,
+  R_VIRTUAL = 14,  // 6
+  R_PHASE = 20,    // 9 per phase: count, sig(6), psi, diss
+  R_N = R_PHASE + 9 * CIA_MAX_PHASES
+};
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_response(Grid G, const uint8_t* __restrict__ ephase, double dt, const T* __restrict__ sig,
+               const T* __restrict__ rate, const T* __restrict__ psi, const T* __restrict__ diss,
+               const T* __restrict__ diag, const T* __restrict__ F, double* __restrict__ partials,
+               double* __restrict__ mm_partials) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t np = G.E * 8;
+  double v[R_N];
+#pragma unroll
+  for (int s = 0; s < R_N; ++s) v[s] = 0.0;
+  double mn = INFINITY, mx = -INFINITY;
+  if (e < G.E) {
+    const int ph = ephase[e];
+    for (int q = 0; q < 8; ++q) {
+      const int64_t p = 8 * e + q;
+      double mw = 0.0;
+      for (int m = 0; m < 6; ++m) {
+        const double sg = (double)sig[m * np + p];
+        v[R_SIG + m] += sg;
+        v[R_PHASE + 9 * ph + 1 + m] += sg;
+        mw += sg * (dt * (double)rate[m * np + p]);
+      }
+      v[R_MICRO_WORK] += mw;
+      v[R_PSI] += (double)psi[p];
+      v[R_DISS] += (double)diss[p];
+      v[R_PHASE + 9 * ph + 0] += 1.0;
+      v[R_PHASE + 9 * ph + 7] += (double)psi[p];
+      v[R_PHASE + 9 * ph + 8] += (double)diss[p];
+      const double pw = (double)diag[CIA_D_INCREMENTAL_WORK_DENSITY * np + p];
+      v[R_POS_WORK] += fmax(pw, 0.0);
+      v[R_NEG_WORK] += fmin(pw, 0.0);
+      v[R_UNLOAD] += pw < 0.0 ? 1.0 : 0.0;
+      v[R_ACTIVE] += (double)diag[CIA_D_VISCOPLASTIC_ACTIVE * np + p] > 0.5 ? 1.0 : 0.0;
+      const double mg = (double)diag[CIA_D_YIELD_ACTIVATION_MARGIN * np + p];
+      if (isfinite(mg)) {
+        v[R_MARGIN_N] += 1.0;
+        mn = fmin(mn, mg);
+        mx = fmax(mx, mg);
+      }
+    }
+    // virtual work stress: sum_a A(x_a)^T F_a with x_a the unwrapped node position
+    const int i = (int)(e % G.nx), j = (int)((e / G.nx) % G.ny), k = (int)(e / ((int64_t)G.nx * G.ny));
+    const double hx = 1.0 / G.g.ihx, hy = 1.0 / G.g.ihy, hz = 1.0 / G.g.ihz;
+    for (int a = 0; a < 8; ++a) {
+      const double x = (i + (a & 1)) * hx, y = (j + ((a >> 1) & 1)) * hy, z = (k + ((a >> 2) & 1)) * hz;
+      const double fx = (double)F[(3 * a + 0) * G.E + e], fy = (double)F[(3 * a + 1) * G.E + e],
+                   fz = (double)F[(3 * a + 2) * G.E + e];
+      v[R_VIRTUAL + 0] += x * fx;
+      v[R_VIRTUAL + 1] += y * fy;
+      v[R_VIRTUAL + 2] += z * fz;
+      v[R_VIRTUAL + 3] += 0.5 * z * fy + 0.5 * y * fz;
+      v[R_VIRTUAL + 4] += 0.5 * z * fx + 0.5 * x * fz;
+      v[R_VIRTUAL + 5] += 0.5 * y * fx + 0.5 * x * fy;
+    }
+  }
+  for (int s = 0; s < R_N; ++s) {
+    const double t = block_sum<kBlock>(v[s], lds);
+    if (threadIdx.x == 0) partials[s * gridDim.x + blockIdx.x] = t;
+  }
+  const double tmn = block_min<kBlock>(mn, lds);
+  const double tmx = block_max<kBlock>(mx, lds);
+  if (threadIdx.x == 0) {
+    mm_partials[2 * blockIdx.x] = tmn;
+    mm_partials[2 * blockIdx.x + 1] = tmx;
+  }
+}
+
+// Full (non-periodic) node quantities: displacement and internal force of
+// every mesh node, face resultants, ||f||^2 and the periodic fluctuation error.
+enum { N_F2 = 0, N_FACE = 1 /* 18 */, N_N = 19 };
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_full_nodes(Grid G, const T* __restrict__ F, const double* __restrict__ w, Macro M,
+                 double* __restrict__ u_full, double* __restrict__ f_full,
+                 double* __restrict__ partials, double* __restrict__ max_partials) {
+  __shared__ double lds[kBlock / 64];
+  const int nx1 = G.nx + 1, ny1 = G.ny + 1, nz1 = G.nz + 1;
+  const int64_t N = (int64_t)nx1 * ny1 * nz1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[N_N];
+#pragma unroll
+  for (int s = 0; s < N_N; ++s) v[s] = 0.0;
+  double fl_err = 0.0;
+  if (t < N) {
+    const int i = (int)(t % nx1), j = (int)((t / nx1) % ny1), k = (int)(t / ((int64_t)nx1 * ny1));
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 8; ++a) {
+      const int ei = i - (a & 1), ej = j - ((a >> 1) & 1), ek = k - ((a >> 2) & 1);
+      if (ei < 0 || ej < 0 || ek < 0 || ei >= G.nx || ej >= G.ny || ek >= G.nz) continue;
+      const int64_t e = ei + (int64_t)G.nx * (ej + (int64_t)G.ny * ek);
+      for (int m = 0; m < 3; ++m) f[m] += (double)F[(3 * a + m) * G.E + e];
+    }
+    const double hx = 1.0 / G.g.ihx, hy = 1.0 / G.g.ihy, hz = 1.0 / G.g.ihz;
+    const int ci = i % G.nx, cj = j % G.ny, ck = k % G.nz;
+    const int64_t c = ci + (int64_t)G.nx * (cj + (int64_t)G.ny * ck);
+    auto affine = [&](double x, double y, double z, double out[3]) {
+      out[0] = x * M.e[0] + 0.5 * z * M.e[4] + 0.5 * y * M.e[5];
+      out[1] = y * M.e[1] + 0.5 * z * M.e[3] + 0.5 * x * M.e[5];
+      out[2] = z * M.e[2] + 0.5 * y * M.e[3] + 0.5 * x * M.e[4];
+    };
+    double au[3], ar[3];
+    affine(i * hx, j * hy, k * hz, au);
+    affine(ci * hx, cj * hy, ck * hz, ar);
+    double mism = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      const double wc = w[3 * c + m];
+      const double u = wc + au[m];
+      const double ur = wc + ar[m];
+      u_full[3 * t + m] = u;
+      f_full[3 * t + m] = f[m];
+      const double d = (u - au[m]) - (ur - ar[m]);
+      mism += d * d;
+      v[N_F2] += f[m] * f[m];
+      if (i == 0) v[N_FACE + 0 + m] += f[m];
+      if (i == G.nx) v[N_FACE + 3 + m] += f[m];
+      if (j == 0) v[N_FACE + 6 + m] += f[m];
+      if (j == G.ny) v[N_FACE + 9 + m] += f[m];
+      if (k == 0) v[N_FACE + 12 + m] += f[m];
+      if (k == G.nz) v[N_FACE + 15 + m] += f[m];
+    }
+    fl_err = sqrt(mism);
+  }
+  for (int s = 0; s < N_N; ++s) {
+    const double tt = block_sum<kBlock>(v[s], lds);
+    if (threadIdx.x == 0) partials[s * gridDim.x + blockIdx.x] = tt;
+  }
+  const double m = block_max<kBlock>(fl_err, lds);
+  if (threadIdx.x == 0) max_partials[blockIdx.x] = m;
+}
+
+// max over classes of ||sum of element forces|| (class reaction resultant).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_class_max(Grid G, const T* __restrict__ F, double* __restrict__ max_partials) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double nrm = 0.0;
+  if (c < G.E) {
+    const int i = (int)(c % G.nx), j = (int)((c / G.nx) % G.ny), k = (int)(c / ((int64_t)G.nx * G.ny));
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 8; ++a) {
+      const int64_t e = periodic_corner_element(i, j, k, a, G.nx, G.ny, G.nz);
+      for (int m = 0; m < 3; ++m) s[m] += (double)F[(3 * a + m) * G.E + e];
+    }
+    nrm = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  }
+  const double m = block_max<kBlock>(nrm, lds);
+  if (threadIdx.x == 0) max_partials[blockIdx.x] = m;
+}
+
+template <class T>
+__global__ void k_copy_cast(int64_t n, const T* __restrict__ src, double* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = (double)src[t];
+}
+
+}  // namespace cia
+
+// =========================================================================
+// Host object
+// =========================================================================
+
+struct cia_rve {
+  int nx, ny, nz;
+  int64_t E, np, C, N;
+  cia::Grid G;
+  double volume;
+  int precision;
+  size_t tsize;
+  int n_phases;
+  std::vector<cia::PhaseConst> phases;
+  std::vector<int64_t> phase_points;
+  cia_newton_settings newton;
+  cia_linear_settings lin;
+  double hm_scale;
+  // device
+  cia::PhaseTable* d_tab = nullptr;
+  uint8_t* d_ephase = nullptr;
+  void* st_trial[2] = {nullptr, nullptr};
+  void* sig[2] = {nullptr, nullptr};
+  void* tan[2] = {nullptr, nullptr};
+  void* F = nullptr;
+  void* strain_out = nullptr;
+  void* rate_out = nullptr;
+  void* psi_out = nullptr;
+  void* diss_out = nullptr;
+  void* diag_out = nullptr;
+  double* w = nullptr;
+  double* w_cand = nullptr;
+  double* dw = nullptr;
+  double* r = nullptr;
+  double* r_cand = nullptr;
+  double* wc_buf = nullptr;
+  double* pr = nullptr;
+  double* pz = nullptr;
+  double* pp[2] = {nullptr, nullptr};
+  double* py = nullptr;
+  double* minv = nullptr;
+  double* sens = nullptr;
+  double* u_full = nullptr;
+  double* f_full = nullptr;
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  cia::Scalars* sc = nullptr;
+  double* class0 = nullptr;
+  unsigned long long* first_fail = nullptr;
+  cia_point_failure* rec = nullptr;
+  double* red = nullptr;  // response partials
+  int64_t red_len = 0;
+  int pcg_total = 0;
+  int cur = 0;
+  bool have_fields = false;
+};
+
+namespace cia {
+
+static cia_status dmalloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc", e, __FILE__, __LINE__);
+    return CIA_ERR_OUT_OF_MEMORY;
+  }
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMemset", e, __FILE__, __LINE__);
+    return CIA_ERR_HIP;
+  }
+  return CIA_OK;
+}
+
+#define CIA_TRY(x)               \
+  do {                           \
+    cia_status s_ = (x);         \
+    if (s_ != CIA_OK) return s_; \
+  } while (0)
+
+template <class T>
+struct Ops {
+  static cia_status assemble(cia_rve* R, hipStream_t st, bool extra, int slot, const double* w,
+                             const double* wc, const Macro& M, const void* state_c, void* state_t,
+                             const Settings& S, double dt) {
+    const unsigned grid = grid_for(R->E);
+    if (extra)
+      hipLaunchKernelGGL((k_assemble<T, true>), dim3(grid), dim3(kBlock), 0, st, R->G, R->d_tab,
+                         R->d_ephase, S, dt, w, wc, M, (const T*)state_c, (T*)state_t,
+                         (T*)R->sig[slot], (T*)R->tan[slot], (T*)R->F, (T*)R->strain_out,
+                         (T*)R->rate_out, (T*)R->psi_out, (T*)R->diss_out, (T*)R->diag_out,
+                         R->first_fail);
+    else
+      hipLaunchKernelGGL((k_assemble<T, false>), dim3(grid), dim3(kBlock), 0, st, R->G, R->d_tab,
+                         R->d_ephase, S, dt, w, wc, M, (const T*)state_c, (T*)state_t,
+                         (T*)R->sig[slot], (T*)R->tan[slot], (T*)R->F, nullptr, nullptr, nullptr,
+                         nullptr, nullptr, R->first_fail);
+    CIA_HIP(hipGetLastError());
+    return CIA_OK;
+  }
+
+  static cia_status gather(cia_rve* R, hipStream_t st, double* r, double* class0, int slot_norm) {
+    hipLaunchKernelGGL(k_gather<T>, dim3(grid_for(R->C)), dim3(kBlock), 0, st, R->G,
+                       (const T*)R->F, r, class0, R->partials, R->counter,
+                       &R->sc->norm2[slot_norm]);
+    CIA_HIP(hipGetLastError());
+    return CIA_OK;
+  }
+
+  // Solve K x = b (K from tan[slot]); returns iterations.
+  static cia_status pcg(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
+                        int* iters_out, int* flag_out) {
+    const int64_t C = R->C;
+    const unsigned gn = grid_for(C), ge = grid_for(R->E);
+    hipLaunchKernelGGL(k_precond<T>, dim3(gn), dim3(kBlock), 0, st, R->G, R->d_tab, R->d_ephase,
+                       (const T*)R->tan[slot], R->minv);
+    hipLaunchKernelGGL(k_pcg_init, dim3(gn), dim3(kBlock), 0, st, C, b, R->minv, x, R->pr, R->pz,
+                       R->pp[1], R->lin.relative_tolerance, R->lin.absolute_tolerance, R->sc,
+                       R->partials, R->counter);
+    CIA_HIP(hipGetLastError());
+    int done = 0;
+    int it = 0;
+    int pi = 0;  // pp[1 - pi] is p_old at the next element pass
+    const int chunk = std::max(1, R->lin.check_interval);
+    Scalars hs;
+    while (it < R->lin.max_iterations) {
+      const int n = std::min(chunk, R->lin.max_iterations - it);
+      for (int s = 0; s < n; ++s) {
+        double* p_old = R->pp[1 - pi];
+        double* p_new = R->pp[pi];
+        hipLaunchKernelGGL(k_pcg_elem<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
+                           R->d_ephase, (const T*)R->tan[slot], R->pz, p_old, p_new, (T*)R->F,
+                           R->sc);
+        hipLaunchKernelGGL(k_pcg_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F,
+                           p_new, R->py, R->sc, R->partials, R->counter);
+        hipLaunchKernelGGL(k_pcg_update, dim3(gn), dim3(kBlock), 0, st, C, R->minv, p_new, R->py,
+                           x, R->pr, R->pz, R->sc, R->partials, R->counter);
+        pi = 1 - pi;
+      }
+      CIA_HIP(hipGetLastError());
+      it += n;
+      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+      CIA_HIP(hipStreamSynchronize(st));
+      done = hs.done;
+      if (done) break;
+    }
+    if (!done) {
+      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+      CIA_HIP(hipStreamSynchronize(st));
+    }
+    *iters_out = hs.iters;
+    *flag_out = hs.done;
+    R->pcg_total += hs.iters;
+    return CIA_OK;
+  }
+};
+
+static cia_status read_norms(cia_rve* R, hipStream_t st, double out[4]) {
+  Scalars hs;
+  CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < 4; ++i) out[i] = hs.norm2[i];
+  return CIA_OK;
+}
+
+static cia_status norm2_of(cia_rve* R, hipStream_t st, int64_t n, const double* x, double* out) {
+  hipLaunchKernelGGL(k_norms, dim3(std::min<int64_t>(grid_for(n), 1024)), dim3(kBlock), 0, st, n, x,
+                     (const double*)nullptr, R->partials, R->counter, &R->sc->norm2[2]);
+  CIA_HIP(hipGetLastError());
+  double v[4];
+  CIA_TRY(read_norms(R, st, v));
+  *out = v[2];
+  return CIA_OK;
+}
+
+static cia_status check_fail(cia_rve* R, hipStream_t st, unsigned long long* ff) {
+  CIA_HIP(hipMemcpyAsync(ff, R->first_fail, sizeof(*ff), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  return CIA_OK;
+}
+
+template <class T>
+static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, const double* cmacro,
+                             const double* cw, const double* macro, double dt,
+                             const Settings& S, int need_tangent, void* next_state,
+                             double* next_w, cia_rve_result* res) {
+  const int64_t C3 = 3 * R->C;
+  Macro M;
+  for (int m = 0; m < 6; ++m) {
+    M.e[m] = macro[m];
+    M.ec[m] = cmacro[m];
+  }
+  R->pcg_total = 0;
+  R->have_fields = false;
+  res->gp_updates = 0;
+  CIA_HIP(hipMemcpyAsync(R->w, cw, C3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  CIA_HIP(hipMemcpyAsync(R->wc_buf, cw, C3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, R->first_fail, ~0ull);
+  const cia_newton_settings& ns = R->newton;
+  int cur = 0;
+  unsigned long long ff = 0;
+
+  auto point_failure = [&](const double* wv) -> cia_status {
+    hipLaunchKernelGGL(k_diagnose<T>, dim3(1), dim3(1), 0, st, R->G, R->d_tab, R->d_ephase, S, dt,
+                       wv, R->wc_buf, M, (const T*)state_c, (int64_t)ff, R->rec);
+    CIA_HIP(hipGetLastError());
+    CIA_HIP(hipMemcpyAsync(&res->point_failure, R->rec, sizeof(cia_point_failure),
+                           hipMemcpyDeviceToHost, st));
+    CIA_HIP(hipStreamSynchronize(st));
+    res->reason = CIA_RVE_MATERIAL_POINT_FAILURE;
+    return CIA_ERR_POINT_FAILURE;
+  };
+
+  // assembly at the committed fluctuation (iteration 0)
+  CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, R->st_trial[cur], S, dt));
+  res->gp_updates += R->np;
+  CIA_TRY(Ops<T>::gather(R, st, R->r, nullptr, 0));
+  CIA_TRY(check_fail(R, st, &ff));
+  if (ff != ~0ull) return point_failure(R->w);
+  double nv[4];
+  CIA_TRY(read_norms(R, st, nv));
+  double rn = sqrt(nv[0]);
+  double r0 = -1.0;
+  double last_inc = INFINITY;
+  int backtracks = 0;
+  int it = 0;
+  double rtol = 0.0;
+  for (it = 0; it <= ns.max_iterations; ++it) {
+    if (r0 < 0.0) r0 = rn;
+    rtol = std::max(ns.residual_absolute_tolerance, ns.residual_relative_tolerance * r0);
+    double wn2 = 0.0;
+    CIA_TRY(norm2_of(R, st, C3, R->w, &wn2));
+    const double itol = std::max(ns.increment_absolute_tolerance,
+                                 ns.increment_relative_tolerance * std::max(sqrt(wn2), 1.0));
+    if (rn <= rtol && (it == 0 || last_inc <= itol)) break;
+    if (it == ns.max_iterations) {
+      res->reason = CIA_RVE_MAXIMUM_ITERATIONS_EXCEEDED;
+      res->failure_iteration = it;
+      res->failure_residual_norm = rn;
+      res->failure_residual_tolerance = rtol;
+      res->failure_increment_norm = last_inc;
+      res->failure_increment_tolerance = itol;
+      res->newton_iterations = it;
+      res->linear_iterations = R->pcg_total;
+      return CIA_ERR_RVE_FAILURE;
+    }
+    // Newton direction: K dw = -r
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(C3)), dim3(kBlock), 0, st, C3, -1.0, R->r, R->r_cand);
+    int pit = 0, pflag = 0;
+    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag));
+    double dn2 = 0.0;
+    CIA_TRY(norm2_of(R, st, C3, R->dw, &dn2));
+    const double inc_n = sqrt(dn2);
+    if (!std::isfinite(inc_n)) {
+      res->reason = CIA_RVE_NON_FINITE_INCREMENT;
+      res->failure_iteration = it;
+      res->failure_residual_norm = rn;
+      res->failure_residual_tolerance = rtol;
+      res->failure_increment_norm = inc_n;
+      res->failure_increment_tolerance = itol;
+      return CIA_ERR_RVE_FAILURE;
+    }
+    if (rn <= rtol && inc_n <= itol) break;
+    if (ns.armijo_enabled) {
+      const int cand = 1 - cur;
+      bool accepted = false;
+      double alpha = 1.0;
+      for (int bt = 0; bt <= ns.armijo_max_backtracks; ++bt) {
+        alpha = std::pow(ns.armijo_reduction_factor, (double)bt);
+        hipLaunchKernelGGL(k_axpy, dim3(grid_for(C3)), dim3(kBlock), 0, st, C3, R->w, alpha, R->dw,
+                           R->w_cand);
+        CIA_TRY(Ops<T>::assemble(R, st, false, cand, R->w_cand, R->wc_buf, M, state_c,
+                                 R->st_trial[cand], S, dt));
+        res->gp_updates += R->np;
+        CIA_TRY(Ops<T>::gather(R, st, R->r_cand, nullptr, 1));
+        CIA_TRY(check_fail(R, st, &ff));
+        if (ff != ~0ull) return point_failure(R->w_cand);
+        CIA_TRY(read_norms(R, st, nv));
+        const double cn = sqrt(nv[1]);
+        if (0.5 * cn * cn <= 0.5 * rn * rn - ns.armijo_sufficient_decrease * alpha * rn * rn) {
+          std::swap(R->w, R->w_cand);
+          std::swap(R->r, R->r_cand);
+          cur = cand;
+          rn = cn;
+          last_inc = alpha * inc_n;
+          backtracks += bt;
+          accepted = true;
+          break;
+        }
+      }
+      if (!accepted) {
+        res->reason = CIA_RVE_ARMIJO_EXHAUSTED;
+        res->failure_iteration = it;
+        res->failure_residual_norm = rn;
+        res->failure_residual_tolerance = rtol;
+        res->failure_increment_norm = inc_n;
+        res->failure_increment_tolerance = itol;
+        return CIA_ERR_RVE_FAILURE;
+      }
+    } else {
+      hipLaunchKernelGGL(k_axpy, dim3(grid_for(C3)), dim3(kBlock), 0, st, C3, R->w, 1.0, R->dw,
+                         R->w_cand);
+      std::swap(R->w, R->w_cand);
+      CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, R->st_trial[cur], S,
+                               dt));
+      res->gp_updates += R->np;
+      CIA_TRY(Ops<T>::gather(R, st, R->r, nullptr, 0));
+      CIA_TRY(check_fail(R, st, &ff));
+      if (ff != ~0ull) return point_failure(R->w);
+      CIA_TRY(read_norms(R, st, nv));
+      rn = sqrt(nv[0]);
+      last_inc = inc_n;
+    }
+  }
+  res->reason = CIA_RVE_OK;
+  res->newton_iterations = it;
+  res->armijo_backtracks = backtracks;
+  res->micro_residual_norm = rn;
+  res->micro_residual_tolerance = rtol;
+  R->cur = cur;
+
+  // ---- converged: full evaluation at w (energies, diagnostics, next state)
+  CIA_TRY(Ops<T>::assemble(R, st, true, cur, R->w, R->wc_buf, M, state_c, next_state, S, dt));
+  CIA_TRY(Ops<T>::gather(R, st, R->r, R->class0, 0));
+  CIA_HIP(hipMemcpyAsync(next_w, R->w, C3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  {
+    const unsigned ge = grid_for(R->E);
+    double* part = R->red;
+    double* mm = R->red + (int64_t)R_N * ge;
+    hipLaunchKernelGGL(k_response<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_ephase, dt,
+                       (const T*)R->sig[cur], (const T*)R->rate_out, (const T*)R->psi_out,
+                       (const T*)R->diss_out, (const T*)R->diag_out, (const T*)R->F, part, mm);
+    CIA_HIP(hipGetLastError());
+    const unsigned gfull = grid_for(R->N);
+    double* np_part = mm + 2 * ge;
+    double* np_max = np_part + (int64_t)N_N * gfull;
+    double* cl_max = np_max + gfull;
+    hipLaunchKernelGGL(k_full_nodes<T>, dim3(gfull), dim3(kBlock), 0, st, R->G, (const T*)R->F,
+                       R->w, M, R->u_full, R->f_full, np_part, np_max);
+    const unsigned gc = grid_for(R->C);
+    hipLaunchKernelGGL(k_class_max<T>, dim3(gc), dim3(kBlock), 0, st, R->G, (const T*)R->F, cl_max);
+    CIA_HIP(hipGetLastError());
+    const int64_t total = (int64_t)R_N * ge + 2 * ge + (int64_t)N_N * gfull + gfull + gc;
+    std::vector<double> h(total);
+    CIA_HIP(hipMemcpyAsync(h.data(), R->red, total * sizeof(double), hipMemcpyDeviceToHost, st));
+    CIA_HIP(hipStreamSynchronize(st));
+    double sums[R_N];
+    for (int s = 0; s < R_N; ++s) {
+      double a = 0.0;
+      for (unsigned b = 0; b < ge; ++b) a += h[(int64_t)s * ge + b];
+      sums[s] = a;
+    }
+    double mn = INFINITY, mx = -INFINITY;
+    for (unsigned b = 0; b < ge; ++b) {
+      mn = std::min(mn, h[(int64_t)R_N * ge + 2 * b]);
+      mx = std::max(mx, h[(int64_t)R_N * ge + 2 * b + 1]);
+    }
+    const double* hn = h.data() + (int64_t)R_N * ge + 2 * ge;
+    double nsum[N_N];
+    for (int s = 0; s < N_N; ++s) {
+      double a = 0.0;
+      for (unsigned b = 0; b < gfull; ++b) a += hn[(int64_t)s * gfull + b];
+      nsum[s] = a;
+    }
+    double flmax = 0.0;
+    for (unsigned b = 0; b < gfull; ++b) flmax = std::max(flmax, hn[(int64_t)N_N * gfull + b]);
+    double clmax = 0.0;
+    for (unsigned b = 0; b < gc; ++b) clmax = std::max(clmax, hn[(int64_t)N_N * gfull + gfull + b]);
+
+    const double jw = R->G.g.jw, V = R->volume;
+    for (int m = 0; m < 6; ++m) res->macro_stress[m] = sums[R_SIG + m] * jw / V;
+    res->free_energy_density = sums[R_PSI] * jw / V;
+    res->dissipation_density = sums[R_DISS] * jw / V;
+    res->volume = V;
+    double vdiff = 0.0;
+    for (int m = 0; m < 6; ++m) {
+      const double d = sums[R_VIRTUAL + m] / V - res->macro_stress[m];
+      vdiff += d * d;
+    }
+    res->virtual_work_stress_error = sqrt(vdiff);
+    const double micro = sums[R_MICRO_WORK] * jw / V;
+    double macro_w = 0.0;
+    for (int m = 0; m < 6; ++m) macro_w += res->macro_stress[m] * (macro[m] - cmacro[m]);
+    res->macro_work_increment = macro_w;
+    res->micro_work_increment = micro;
+    res->hill_mandel_residual = macro_w - micro;
+    res->hill_mandel_error =
+        fabs(macro_w - micro) / (fabs(macro_w) + fabs(micro) + R->hm_scale * dt);
+    res->positive_work_density = sums[R_POS_WORK] * jw / V;
+    res->negative_work_density = sums[R_NEG_WORK] * jw / V;
+    res->unloading_volume_fraction = sums[R_UNLOAD] * jw / V;
+    res->viscoplastic_active_volume_fraction = sums[R_ACTIVE] * jw / V;
+    res->yield_activation_margin_available = sums[R_MARGIN_N] > 0.0 ? 1 : 0;
+    res->minimum_yield_activation_margin = sums[R_MARGIN_N] > 0.0 ? mn : NAN;
+    res->maximum_yield_activation_margin = sums[R_MARGIN_N] > 0.0 ? mx : NAN;
+    res->periodic_fluctuation_error = flmax;
+    double face_err = 0.0;
+    for (int ax = 0; ax < 3; ++ax) {
+      double a2 = 0.0, b2 = 0.0, s2 = 0.0;
+      for (int m = 0; m < 3; ++m) {
+        const double a = nsum[N_FACE + 6 * ax + m], b = nsum[N_FACE + 6 * ax + 3 + m];
+        a2 += a * a;
+        b2 += b * b;
+        s2 += (a + b) * (a + b);
+      }
+      const double den = std::max(sqrt(a2) + sqrt(b2), 1.0);
+      face_err = std::max(face_err, sqrt(s2) / den);
+    }
+    res->reaction_antiperiodicity_error = face_err;
+    res->periodic_class_reaction_error = clmax / std::max(sqrt(nsum[N_F2]), 1.0);
+    for (int p = 0; p < R->n_phases && p < 8; ++p) {
+      const double cnt = sums[R_PHASE + 9 * p];
+      const double pv = cnt * jw;
+      res->phase_volume[p] = pv;
+      for (int m = 0; m < 6; ++m)
+        res->phase_average_stress[p][m] = pv > 0 ? sums[R_PHASE + 9 * p + 1 + m] * jw / pv : NAN;
+      res->phase_free_energy_density[p] = pv > 0 ? sums[R_PHASE + 9 * p + 7] * jw / pv : NAN;
+      res->phase_dissipation_density[p] = pv > 0 ? sums[R_PHASE + 9 * p + 8] * jw / pv : NAN;
+    }
+  }
+  R->have_fields = true;
+
+  // ---- effective tangent (Schur complement), solver.py:438-461
+  if (need_tangent) {
+    const unsigned ge = grid_for(R->E);
+    std::vector<double> h(6 * ge);
+    for (int j = 0; j < 6; ++j) {
+      hipLaunchKernelGGL(k_affine_force<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
+                         R->d_ephase, (const T*)R->tan[cur], j, (T*)R->F);
+      CIA_TRY(Ops<T>::gather(R, st, R->r_cand, nullptr, 1));
+      int pit = 0, pflag = 0;
+      CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->sens, &pit, &pflag));
+      double sn2 = 0.0;
+      CIA_TRY(norm2_of(R, st, C3, R->sens, &sn2));
+      if (!std::isfinite(sn2)) {
+        res->reason = CIA_RVE_NON_FINITE_EFFECTIVE_TANGENT_SENSITIVITY;
+        return CIA_ERR_RVE_FAILURE;
+      }
+      hipLaunchKernelGGL(k_ceff_column<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
+                         R->d_ephase, (const T*)R->tan[cur], R->sens, j, R->red);
+      CIA_HIP(hipGetLastError());
+      CIA_HIP(hipMemcpyAsync(h.data(), R->red, 6 * ge * sizeof(double), hipMemcpyDeviceToHost, st));
+      CIA_HIP(hipStreamSynchronize(st));
+      for (int m = 0; m < 6; ++m) {
+        double a = 0.0;
+        for (unsigned b = 0; b < ge; ++b) a += h[(int64_t)m * ge + b];
+        res->effective_tangent[m * 6 + j] = a * R->G.g.jw / R->volume;
+      }
+    }
+    // The F rows now hold affine loads; the final internal force is kept in
+    // f_full, which is what CIA_FIELD_INTERNAL_FORCE returns.
+  }
+  res->linear_iterations = R->pcg_total;
+  return CIA_OK;
+}
+
+}  // namespace cia
+
+extern "C" {
+
+cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
+  using namespace cia;
+  CIA_ARG(d && out, "desc/out is NULL");
+  CIA_ARG(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "divisions must be positive");
+  CIA_ARG(d->n_phases >= 1 && d->n_phases <= CIA_MAX_PHASES && d->phases, "invalid phases");
+  CIA_ARG(d->lengths[0] > 0 && d->lengths[1] > 0 && d->lengths[2] > 0, "lengths must be positive");
+  CIA_ARG(d->precision == CIA_PREC_FP64 || d->precision == CIA_PREC_FP32_REFINED, "bad precision");
+  auto* R = new cia_rve();
+  R->nx = d->nx;
+  R->ny = d->ny;
+  R->nz = d->nz;
+  R->E = (int64_t)d->nx * d->ny * d->nz;
+  R->C = R->E;
+  R->np = 8 * R->E;
+  R->N = (int64_t)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+  R->G.nx = d->nx;
+  R->G.ny = d->ny;
+  R->G.nz = d->nz;
+  R->G.E = R->E;
+  const double hx = d->lengths[0] / d->nx, hy = d->lengths[1] / d->ny, hz = d->lengths[2] / d->nz;
+  R->G.g.ihx = 1.0 / hx;
+  R->G.g.ihy = 1.0 / hy;
+  R->G.g.ihz = 1.0 / hz;
+  R->G.g.jw = hx * hy * hz / 8.0;
+  R->volume = R->G.g.jw * (double)R->np;
+  R->precision = d->precision;
+  R->tsize = d->precision == CIA_PREC_FP64 ? sizeof(double) : sizeof(float);
+  R->n_phases = d->n_phases;
+  R->newton = d->newton;
+  R->lin = d->linear;
+  if (R->lin.max_iterations <= 0) R->lin.max_iterations = 20000;
+  if (R->lin.check_interval <= 0) R->lin.check_interval = 16;
+  R->hm_scale = d->hill_mandel_power_scale;
+  PhaseTable tab{};
+  tab.n = d->n_phases;
+  for (int i = 0; i < d->n_phases; ++i) {
+    tab.p[i] = make_phase(d->phases[i]);
+    R->phases.push_back(tab.p[i]);
+  }
+  std::vector<uint8_t> eph(R->E, 0);
+  if (d->element_phase) {
+    for (int64_t e = 0; e < R->E; ++e) {
+      const int32_t v = d->element_phase[e];
+      if (v < 0 || v >= d->n_phases) {
+        delete R;
+        set_error_msg("element_phase out of range");
+        return CIA_ERR_INVALID_ARGUMENT;
+      }
+      eph[e] = (uint8_t)v;
+    }
+  }
+  const int64_t np = R->np, C3 = 3 * R->C;
+  const size_t ts = R->tsize;
+  cia_status s = CIA_OK;
+#define ALLOC(ptr, bytes)                                   \
+  do {                                                      \
+    s = dmalloc((void**)&(ptr), (bytes));                   \
+    if (s != CIA_OK) {                                      \
+      cia_rve_destroy(R);                                   \
+      return s;                                             \
+    }                                                       \
+  } while (0)
+  ALLOC(R->d_tab, sizeof(PhaseTable));
+  ALLOC(R->d_ephase, R->E);
+  for (int k = 0; k < 2; ++k) {
+    ALLOC(R->st_trial[k], 8 * np * ts);
+    ALLOC(R->sig[k], 6 * np * ts);
+    ALLOC(R->tan[k], 8 * np * ts);
+  }
+  ALLOC(R->F, 24 * R->E * ts);
+  ALLOC(R->strain_out, 6 * np * ts);
+  ALLOC(R->rate_out, 6 * np * ts);
+  ALLOC(R->psi_out, np * ts);
+  ALLOC(R->diss_out, np * ts);
+  ALLOC(R->diag_out, (int64_t)CIA_N_DIAG * np * ts);
+  ALLOC(R->w, C3 * 8);
+  ALLOC(R->w_cand, C3 * 8);
+  ALLOC(R->dw, C3 * 8);
+  ALLOC(R->r, C3 * 8);
+  ALLOC(R->r_cand, C3 * 8);
+  ALLOC(R->wc_buf, C3 * 8);
+  ALLOC(R->pr, C3 * 8);
+  ALLOC(R->pz, C3 * 8);
+  ALLOC(R->pp[0], C3 * 8);
+  ALLOC(R->pp[1], C3 * 8);
+  ALLOC(R->py, C3 * 8);
+  ALLOC(R->sens, C3 * 8);
+  ALLOC(R->minv, 6 * R->C * 8);
+  ALLOC(R->u_full, 3 * R->N * 8);
+  ALLOC(R->f_full, 3 * R->N * 8);
+  const int64_t maxb = std::max<int64_t>(grid_for(R->E), grid_for(R->N)) + 64;
+  ALLOC(R->partials, 8 * maxb * 8);
+  ALLOC(R->counter, 64);
+  ALLOC(R->sc, sizeof(Scalars));
+  ALLOC(R->class0, 3 * 8);
+  ALLOC(R->first_fail, 8);
+  ALLOC(R->rec, sizeof(cia_point_failure));
+  R->red_len = (int64_t)R_N * grid_for(R->E) + 2 * grid_for(R->E) + (int64_t)N_N * grid_for(R->N) +
+               grid_for(R->N) + grid_for(R->C) + 64;
+  ALLOC(R->red, R->red_len * 8);
+#undef ALLOC
+  if (hipMemcpy(R->d_tab, &tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(R->d_ephase, eph.data(), R->E, hipMemcpyHostToDevice) != hipSuccess) {
+    cia_rve_destroy(R);
+    set_error_msg("hipMemcpy of RVE tables failed");
+    return CIA_ERR_HIP;
+  }
+  *out = R;
+  return CIA_OK;
+}
+
+cia_status cia_rve_destroy(cia_rve* R) {
+  if (!R) return CIA_OK;
+  void* ptrs[] = {R->d_tab, R->d_ephase, R->st_trial[0], R->st_trial[1], R->sig[0], R->sig[1],
+                  R->tan[0], R->tan[1], R->F, R->strain_out, R->rate_out, R->psi_out, R->diss_out,
+                  R->diag_out, R->w, R->w_cand, R->dw, R->r, R->r_cand, R->wc_buf, R->pr, R->pz,
+                  R->pp[0], R->pp[1], R->py, R->sens, R->minv, R->u_full, R->f_full, R->partials,
+                  R->counter, R->sc, R->class0, R->first_fail, R->rec, R->red};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete R;
+  return CIA_OK;
+}
+
+cia_status cia_rve_sizes(const cia_rve* R, int64_t* s) {
+  if (!R || !s) return CIA_ERR_INVALID_ARGUMENT;
+  s[0] = R->E;
+  s[1] = R->np;
+  s[2] = R->C;
+  s[3] = R->N;
+  return CIA_OK;
+}
+
+cia_status cia_rve_solve(cia_rve* R, void* stream, const void* committed_state,
+                         const double* committed_macro, const double* committed_w,
+                         const double* macro_strain, double time_step,
+                         const cia_update_settings* settings, int32_t need_tangent,
+                         void* next_state, double* next_w, cia_rve_result* result) {
+  using namespace cia;
+  CIA_ARG(R && committed_state && committed_macro && committed_w && macro_strain && settings &&
+              next_state && next_w && result,
+          "NULL argument to cia_rve_solve");
+  CIA_ARG(time_step > 0.0, "RVE time step must be positive.");
+  CIA_ARG((const void*)next_state != (const void*)committed_state, "next_state aliases committed_state");
+  memset(result, 0, sizeof(*result));
+  result->point_failure.point_id = -1;
+  const Settings S = make_settings(*settings);
+  hipStream_t st = (hipStream_t)stream;
+  if (R->precision == CIA_PREC_FP64)
+    return solve_impl<double>(R, st, committed_state, committed_macro, committed_w, macro_strain,
+                              time_step, S, need_tangent, next_state, next_w, result);
+  return solve_impl<float>(R, st, committed_state, committed_macro, committed_w, macro_strain,
+                           time_step, S, need_tangent, next_state, next_w, result);
+}
+
+cia_status cia_rve_copy_field(cia_rve* R, void* stream, int32_t field, double* dst) {
+  using namespace cia;
+  CIA_ARG(R && dst, "NULL argument");
+  CIA_ARG(R->have_fields, "no converged increment to copy from");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t np = R->np;
+  const void* src = nullptr;
+  int64_t n = 0;
+  switch (field) {
+    case CIA_FIELD_MICRO_STRAIN: src = R->strain_out; n = 6 * np; break;
+    case CIA_FIELD_MICRO_STRAIN_RATE: src = R->rate_out; n = 6 * np; break;
+    case CIA_FIELD_MICRO_STRESS: src = R->sig[R->cur]; n = 6 * np; break;
+    case CIA_FIELD_FREE_ENERGY: src = R->psi_out; n = np; break;
+    case CIA_FIELD_DISSIPATION: src = R->diss_out; n = np; break;
+    case CIA_FIELD_DIAGNOSTICS: src = R->diag_out; n = (int64_t)CIA_N_DIAG * np; break;
+    case CIA_FIELD_DISPLACEMENT:
+      CIA_HIP(hipMemcpyAsync(dst, R->u_full, 3 * R->N * 8, hipMemcpyDeviceToDevice, st));
+      return CIA_OK;
+    case CIA_FIELD_INTERNAL_FORCE:
+      CIA_HIP(hipMemcpyAsync(dst, R->f_full, 3 * R->N * 8, hipMemcpyDeviceToDevice, st));
+      return CIA_OK;
+    default:
+      set_error_msg("unknown field");
+      return CIA_ERR_INVALID_ARGUMENT;
+  }
+  if (R->precision == CIA_PREC_FP64) {
+    CIA_HIP(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, st));
+  } else {
+    hipLaunchKernelGGL(k_copy_cast<float>, dim3(grid_for(n)), dim3(kBlock), 0, st, n,
+                       (const float*)src, dst);
+    CIA_HIP(hipGetLastError());
+  }
+  return CIA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,594 @@
+"""Periodic Hex8 RVE: GPU microequilibrium behind the reference RVE interface.
+
+Mirrors ``src/fem/rve/__init__.py:9-84``:
+
+* data types               ``src/fem/rve/data.py:24-113``
+* structured box           ``src/fem/mesh/structured.py:36-80``
+* periodic constraint      ``src/fem/rve/periodic.py:20-169``
+* operator cache, state    ``src/fem/rve/solver.py:46-59,367-388``
+* microequilibrium         ``src/fem/rve/solver.py:62-227`` -> ``cia_rve_solve``
+* response                 ``src/fem/rve/solver.py:253-364``
+* path drivers             ``src/fem/rve/path.py:37-123``
+
+The committed state of an increment lives on the device (``RVEState.device``);
+NumPy views of it are produced only when a caller reads them.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from collections.abc import Iterator
+from dataclasses import dataclass, field
+from time import perf_counter
+from typing import Any
+
+import numpy as np
+import torch
+
+from concrete_impact_ai_amd import _lib
+from concrete_impact_ai_amd.materials import (
+    J2ViscoplasticMaterial,
+    LinearElasticMaterial,
+    MaterialPointConvergenceError,
+    MaterialResponseRequirements,
+    MaterialState,
+    MaterialUpdateSettings,
+    PhasedMaterialModel,
+    cia_material,
+    cia_settings,
+    device,
+    diagnostics_dict,
+    point_failure_error,
+    stream_handle,
+    unpack_state,
+    _pack_state,
+)
+from concrete_impact_ai_amd.solvers import (
+    LinearSolverSettings,
+    NonlinearNewtonSettings,
+    SolverProgressEvent,
+)
+
+
+# ---------------------------------------------------------------------------
+# mesh and periodic constraint
+
+
+def build_structured_hex8_box(lengths, divisions):
+    """Node id = i + (nx+1)(j + (ny+1)k); element i fastest (``structured.py:36-80``)."""
+    nx, ny, nz = (int(d) for d in divisions)
+    xs = np.linspace(0.0, lengths[0], nx + 1)
+    ys = np.linspace(0.0, lengths[1], ny + 1)
+    zs = np.linspace(0.0, lengths[2], nz + 1)
+    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    nodes = np.stack([xs[ii.ravel()], ys[jj.ravel()], zs[kk.ravel()]], axis=1)
+    ek, ej, ei = (a.ravel() for a in np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx),
+                                                 indexing="ij"))
+
+    def nid(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    corners = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
+    elements = np.stack([nid(ei + a, ej + b, ek + c) for a, b, c in corners], axis=1).astype(np.int64)
+    return nodes, elements
+
+
+@dataclass(frozen=True)
+class PeriodicConstraint:
+    """Exact periodic elimination data (``src/fem/rve/data.py:24-38``).
+
+    ``transformation`` (the 0/1 matrix T) is built on first access only; the
+    GPU path uses the class map directly.
+    """
+
+    affine_matrix: np.ndarray
+    equivalence_class_ids: np.ndarray
+    representative_nodes: np.ndarray
+    reference_class: int
+    divisions: tuple
+    opposite_face_nodes: tuple
+
+    @property
+    def transformation(self):
+        from scipy.sparse import coo_matrix
+
+        cls = self.equivalence_class_ids
+        count = self.representative_nodes.size
+        reduced = np.cumsum(np.arange(count) != self.reference_class) - 1
+        keep = np.flatnonzero(cls != self.reference_class)
+        rows = (3 * keep[:, None] + np.arange(3)).ravel()
+        cols = (3 * reduced[cls[keep]][:, None] + np.arange(3)).ravel()
+        n = cls.size
+        return coo_matrix((np.ones(rows.size), (rows, cols)),
+                          shape=(3 * n, 3 * (count - 1))).tocsr()
+
+
+def build_structured_hex8_periodic_constraint(nodes, divisions) -> PeriodicConstraint:
+    """``src/fem/rve/periodic.py:20-47,50-104,107-133,153-169``."""
+    nx, ny, nz = (int(d) for d in divisions)
+    if nodes.shape != ((nx + 1) * (ny + 1) * (nz + 1), 3):
+        raise ValueError("Structured periodic constraint received an incompatible node array.")
+    if min(nx, ny, nz) <= 0:
+        raise ValueError("Structured periodic constraint requires positive divisions.")
+    k, j, i = (a.ravel() for a in np.meshgrid(np.arange(nz + 1), np.arange(ny + 1),
+                                              np.arange(nx + 1), indexing="ij"))
+    cls = ((i % nx) + nx * ((j % ny) + ny * (k % nz))).astype(np.int64)
+    reps = (np.arange(nx * ny * nz) % nx) + (nx + 1) * (
+        (np.arange(nx * ny * nz) // nx % ny) + (ny + 1) * (np.arange(nx * ny * nz) // (nx * ny)))
+    ids = np.arange(cls.size, dtype=np.int64)
+    idx = (i, j, k)
+    faces = tuple((ids[idx[a] == 0], ids[idx[a] == (nx, ny, nz)[a]]) for a in range(3))
+    rel = nodes - np.min(nodes, axis=0)
+    a_mat = np.zeros((3 * nodes.shape[0], 6))
+    x, y, z = rel[:, 0], rel[:, 1], rel[:, 2]
+    a_mat[0::3, 0] = x
+    a_mat[0::3, 4] = 0.5 * z
+    a_mat[0::3, 5] = 0.5 * y
+    a_mat[1::3, 1] = y
+    a_mat[1::3, 3] = 0.5 * z
+    a_mat[1::3, 5] = 0.5 * x
+    a_mat[2::3, 2] = z
+    a_mat[2::3, 3] = 0.5 * y
+    a_mat[2::3, 4] = 0.5 * x
+    return PeriodicConstraint(a_mat, cls, reps.astype(np.int64), int(cls[faces[0][0][0]]),
+                              (nx, ny, nz), faces)
+
+
+# ---------------------------------------------------------------------------
+# data types
+
+
+@dataclass(frozen=True)
+class StructuredHex8RVE:
+    """``src/fem/rve/data.py:88-102`` plus the GPU linear-solver and precision choice."""
+
+    nodes: np.ndarray
+    elements: np.ndarray
+    divisions: tuple
+    material: Any
+    constraint: PeriodicConstraint
+    newton_settings: NonlinearNewtonSettings
+    element_phase_ids: np.ndarray | None = None
+    element_densities: np.ndarray | None = None
+    phase_names: tuple = ()
+    quadrature_order: int = 2
+    hill_mandel_power_scale: float = 1.0
+    linear_settings: LinearSolverSettings = field(default_factory=LinearSolverSettings)
+    precision: str = "fp64"
+
+
+@dataclass(frozen=True)
+class RVERequest:
+    macro_strain: np.ndarray
+    time_step: float
+    material_update_settings: MaterialUpdateSettings
+    requirements: MaterialResponseRequirements = field(
+        default_factory=lambda: MaterialResponseRequirements(tangent=True)
+    )
+
+
+class DeviceMaterialState:
+    """MaterialState whose arrays live on the device until ``variables`` is read."""
+
+    def __init__(self, packed: torch.Tensor, phases, point_phase_ids, phased: bool):
+        self.packed = packed
+        self._phases = phases
+        self._ppid = point_phase_ids
+        self._phased = phased
+        self._vars = None
+
+    @property
+    def variables(self) -> dict:
+        if self._vars is None:
+            self._vars = unpack_state(self._phases, self._ppid,
+                                      self.packed.double().cpu().numpy(), self._phased)
+        return self._vars
+
+
+class RVEState:
+    """Committed microstate (``src/fem/rve/data.py:53-59``), device-backed."""
+
+    def __init__(self, material_state, macro_strain, fluctuation_dofs=None, *, w_device=None):
+        self.material_state = material_state
+        self.macro_strain = np.asarray(macro_strain, dtype=np.float64)
+        self._fluct = None if fluctuation_dofs is None else np.asarray(fluctuation_dofs, dtype=np.float64)
+        self.w_device = w_device
+
+    @property
+    def fluctuation_dofs(self) -> np.ndarray:
+        if self._fluct is None:
+            self._fluct = self.w_device.cpu().numpy()[3:].copy()
+        return self._fluct
+
+
+class RVEResponse:
+    """Homogenised and micro outputs (``src/fem/rve/data.py:62-76``); micro
+    fields are copied out of the device on first access."""
+
+    def __init__(self, macro_stress, effective_tangent, state, fields, free_energy_density,
+                 dissipation_density, phase_diagnostics, diagnostics, diag_builder):
+        self.macro_stress = macro_stress
+        self.effective_tangent = effective_tangent
+        self.state = state
+        self._fields = fields  # dict of device tensors
+        self.free_energy_density = free_energy_density
+        self.dissipation_density = dissipation_density
+        self.phase_diagnostics = phase_diagnostics
+        self.diagnostics = diagnostics
+        self._diag_builder = diag_builder
+        self._cache = {}
+
+    def _host(self, key):
+        if key not in self._cache:
+            self._cache[key] = self._fields[key].double().cpu().numpy()
+        return self._cache[key]
+
+    @property
+    def displacement(self):
+        return self._host("displacement")
+
+    @property
+    def internal_force(self):
+        return self._host("internal_force")
+
+    @property
+    def micro_strains(self):
+        return self._host("strain").T.copy()
+
+    @property
+    def micro_strain_rates(self):
+        return self._host("rate").T.copy()
+
+    @property
+    def micro_stresses(self):
+        return self._host("stress").T.copy()
+
+    @property
+    def micro_free_energy(self):
+        return self._host("psi")
+
+    @property
+    def micro_dissipation(self):
+        return self._host("diss")
+
+    @property
+    def micro_diagnostics(self):
+        if "diag_dict" not in self._cache:
+            self._cache["diag_dict"] = self._diag_builder(self._host("diag"))
+        return self._cache["diag_dict"]
+
+
+@dataclass(frozen=True)
+class RVEPathResult:
+    times: np.ndarray
+    responses: tuple
+    validation_diagnostics: dict = field(default_factory=dict)
+
+
+class RVEEquilibriumError(RuntimeError):
+    """``src/fem/rve/solver.py:36-43``."""
+
+    def __init__(self, reason: str, diagnostics: dict) -> None:
+        super().__init__(f"RVE microequilibrium failed: {reason}.")
+        self.reason = reason
+        self.diagnostics = diagnostics
+
+
+# ---------------------------------------------------------------------------
+# operator cache: the device RVE object
+
+
+def _phases_of(model: StructuredHex8RVE):
+    mat = model.material
+    if isinstance(mat, PhasedMaterialModel):
+        return [(p.name, p.material) for p in mat.phases], True
+    if isinstance(mat, (J2ViscoplasticMaterial, LinearElasticMaterial)):
+        return [("", mat)], False
+    raise TypeError(f"RVE material {type(mat).__name__} has no GPU kernel on this path")
+
+
+def _box_lengths(model: StructuredHex8RVE):
+    lo = np.min(model.nodes, axis=0)
+    hi = np.max(model.nodes, axis=0)
+    lengths = hi - lo
+    ref, _ = build_structured_hex8_box(lengths, model.divisions)
+    if model.nodes.shape != ref.shape or not np.allclose(model.nodes - lo, ref, rtol=0.0,
+                                                          atol=1e-12 * float(np.max(lengths))):
+        raise ValueError("The GPU RVE requires a uniform structured Hex8 box.")
+    if model.quadrature_order != 2:
+        raise ValueError("The GPU RVE integrates with the 2x2x2 Gauss rule only.")
+    return lengths
+
+
+class RVEOperatorCache:
+    """Device operator of one RVE (``build_rve_operator_cache``, solver.py:367-388)."""
+
+    def __init__(self, model: StructuredHex8RVE):
+        self.phases, self.phased = _phases_of(model)
+        nx, ny, nz = (int(d) for d in model.divisions)
+        self.divisions = (nx, ny, nz)
+        self.lengths = _box_lengths(model)
+        n_el = nx * ny * nz
+        ep = (np.zeros(n_el, dtype=np.int64) if model.element_phase_ids is None
+              else np.asarray(model.element_phase_ids, dtype=np.int64))
+        self.element_phase_ids = ep
+        self.point_phase_ids = np.repeat(ep, 8)
+        self.n_points = 8 * n_el
+        self.n_classes = n_el
+        self.n_nodes = (nx + 1) * (ny + 1) * (nz + 1)
+        h = self.lengths / np.asarray(self.divisions, dtype=np.float64)
+        self.quadrature_weights = np.full(self.n_points, float(np.prod(h)) / 8.0)
+        self.volume = float(np.sum(self.quadrature_weights))
+        self.precision = model.precision
+        self.dtype = torch.float64 if model.precision == "fp64" else torch.float32
+        ns = model.newton_settings
+        ls = model.linear_settings
+        mats = (_lib.Material * len(self.phases))(*[cia_material(m) for _, m in self.phases])
+        self._mats = mats
+        eph = np.ascontiguousarray(ep.astype(np.int32))
+        desc = _lib.RVEDesc()
+        desc.nx, desc.ny, desc.nz = nx, ny, nz
+        desc.n_phases = len(self.phases)
+        desc.lengths = (C.c_double * 3)(*[float(v) for v in self.lengths])
+        desc.phases = mats
+        desc.element_phase = eph.ctypes.data_as(C.POINTER(C.c_int32))
+        desc.newton = _lib.NewtonSettings(
+            int(ns.max_iterations), int(bool(ns.armijo.enabled)),
+            float(ns.residual_absolute_tolerance), float(ns.residual_relative_tolerance),
+            float(ns.increment_absolute_tolerance), float(ns.increment_relative_tolerance),
+            float(ns.armijo.sufficient_decrease), float(ns.armijo.reduction_factor),
+            int(ns.armijo.max_backtracks), 0,
+        )
+        desc.linear = _lib.LinearSettings(int(ls.max_iterations), int(ls.check_interval),
+                                          float(ls.relative_tolerance), float(ls.absolute_tolerance))
+        desc.hill_mandel_power_scale = float(model.hill_mandel_power_scale)
+        desc.precision = _lib.CIA_PREC_FP64 if model.precision == "fp64" else _lib.CIA_PREC_FP32_REFINED
+        device()
+        handle = C.c_void_p()
+        _lib.check(_lib.lib().cia_rve_create(C.byref(desc), C.byref(handle)), "cia_rve_create")
+        self.handle = handle
+        self.phase_names = (tuple(n for n, _ in self.phases) if self.phased else ("domain",))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.lib().cia_rve_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_rve_operator_cache(model: StructuredHex8RVE) -> RVEOperatorCache:
+    return RVEOperatorCache(model)
+
+
+def initialize_rve_state(model: StructuredHex8RVE, cache: RVEOperatorCache | None = None) -> RVEState:
+    """``src/fem/rve/solver.py:46-59`` with device-resident zero state."""
+    cache = cache or RVEOperatorCache(model)
+    dev = device()
+    packed = torch.zeros((8, cache.n_points), dtype=cache.dtype, device=dev)
+    w = torch.zeros(3 * cache.n_classes, dtype=torch.float64, device=dev)
+    mstate = DeviceMaterialState(packed, cache.phases,
+                                 cache.point_phase_ids if cache.phased else None, cache.phased)
+    return RVEState(mstate, np.zeros(6), np.zeros(3 * (cache.n_classes - 1)), w_device=w)
+
+
+def _device_state(cache: RVEOperatorCache, state: RVEState):
+    dev = device()
+    ms = state.material_state
+    if isinstance(ms, DeviceMaterialState) and ms.packed.dtype == cache.dtype and \
+            ms.packed.shape[1] == cache.n_points:
+        packed = ms.packed
+    else:
+        packed = torch.as_tensor(
+            _pack_state(cache.phases, cache.point_phase_ids if cache.phased else None, ms,
+                        cache.n_points, cache.phased), device=dev).to(cache.dtype)
+    if state.w_device is not None:
+        w = state.w_device
+    else:
+        w = torch.zeros(3 * cache.n_classes, dtype=torch.float64, device=dev)
+        w[3:] = torch.as_tensor(state.fluctuation_dofs, device=dev)
+    return packed.contiguous(), w.contiguous()
+
+
+def solve_rve_microequilibrium(model: StructuredHex8RVE, request: RVERequest,
+                               committed_state: RVEState,
+                               operator_cache: RVEOperatorCache | None = None,
+                               retain_micro_fields: bool = True) -> RVEResponse:
+    """One strain-driven increment (``src/fem/rve/solver.py:62-227``)."""
+    macro = np.asarray(request.macro_strain, dtype=np.float64)
+    if macro.shape != (6,):
+        raise ValueError("RVE macro strain must have shape (6,).")
+    if request.time_step <= 0.0:
+        raise ValueError("RVE time step must be positive.")
+    cache = operator_cache if operator_cache is not None else RVEOperatorCache(model)
+    packed, w = _device_state(cache, committed_state)
+    next_packed = torch.empty_like(packed)
+    next_w = torch.empty_like(w)
+    res = _lib.RVEResult()
+    cmacro = (C.c_double * 6)(*[float(v) for v in committed_state.macro_strain])
+    rmacro = (C.c_double * 6)(*[float(v) for v in macro])
+    settings = cia_settings(request.material_update_settings)
+    status = _lib.lib().cia_rve_solve(
+        cache.handle, stream_handle(), C.c_void_p(packed.data_ptr()), cmacro,
+        C.c_void_p(w.data_ptr()), rmacro, float(request.time_step), C.byref(settings),
+        int(bool(request.requirements.tangent)), C.c_void_p(next_packed.data_ptr()),
+        C.c_void_p(next_w.data_ptr()), C.byref(res),
+    )
+    _lib.check(status, "cia_rve_solve")
+    if status == _lib.CIA_ERR_POINT_FAILURE:
+        fail = res.point_failure
+        p = int(fail.point_id)
+        mats = [m for _, m in cache.phases]
+        phase = int(cache.point_phase_ids[p])
+        local = int(np.count_nonzero(cache.point_phase_ids[:p] == phase)) if cache.phased else p
+        mreq = type("Req", (), {"time_step": request.time_step, "kinematics": "three_dimensional",
+                                "update_settings": request.material_update_settings})
+        err = point_failure_error(fail, mats[phase], mreq, local, mats[phase])
+        raise err.with_context(element_id=local // 8, quadrature_id=local % 8)
+    if status == _lib.CIA_ERR_RVE_FAILURE:
+        reason = _lib.RVE_REASONS[res.reason]
+        raise RVEEquilibriumError(reason, {
+            "algorithm": "periodic_rve_microequilibrium_newton",
+            "macro_strain": macro.tolist(),
+            "time_step": request.time_step,
+            "iteration": int(res.failure_iteration),
+            "residual_norm": float(res.failure_residual_norm),
+            "residual_tolerance": float(res.failure_residual_tolerance),
+            "increment_norm": float(res.failure_increment_norm),
+            "increment_tolerance": float(res.failure_increment_tolerance),
+        })
+    return _build_response(cache, request, committed_state, res, next_packed, next_w, macro,
+                           retain_micro_fields)
+
+
+def _build_response(cache, request, committed, res, next_packed, next_w, macro, retain):
+    fields = {}
+    if retain:
+        dev = next_w.device
+        n = cache.n_points
+        lib = _lib.lib()
+        st = stream_handle()
+        spec = {
+            "strain": (_lib.FIELD_MICRO_STRAIN, (6, n)),
+            "rate": (_lib.FIELD_MICRO_STRAIN_RATE, (6, n)),
+            "stress": (_lib.FIELD_MICRO_STRESS, (6, n)),
+            "psi": (_lib.FIELD_FREE_ENERGY, (n,)),
+            "diss": (_lib.FIELD_DISSIPATION, (n,)),
+            "diag": (_lib.FIELD_DIAGNOSTICS, (_lib.N_DIAG, n)),
+            "displacement": (_lib.FIELD_DISPLACEMENT, (3 * cache.n_nodes,)),
+            "internal_force": (_lib.FIELD_INTERNAL_FORCE, (3 * cache.n_nodes,)),
+        }
+        for key, (fid, shape) in spec.items():
+            t = torch.empty(shape, dtype=torch.float64, device=dev)
+            _lib.check(lib.cia_rve_copy_field(cache.handle, st, fid, C.c_void_p(t.data_ptr())),
+                       "cia_rve_copy_field")
+            fields[key] = t
+    mstate = DeviceMaterialState(next_packed, cache.phases,
+                                 cache.point_phase_ids if cache.phased else None, cache.phased)
+    state = RVEState(mstate, macro.copy(), None, w_device=next_w)
+    phase_diag = {}
+    for pid, name in enumerate(cache.phase_names):
+        vol = float(res.phase_volume[pid])
+        if cache.phased:
+            prefix = f"phase_{name}__"
+            fields_names = tuple(k for k in ("plastic_strain", "equivalent_plastic_strain",
+                                             "viscoplastic_multiplier")
+                                 if isinstance(cache.phases[pid][1], J2ViscoplasticMaterial))
+            state_fields = tuple(prefix + f for f in fields_names)
+        else:
+            state_fields = (("plastic_strain", "equivalent_plastic_strain", "viscoplastic_multiplier")
+                            if isinstance(cache.phases[0][1], J2ViscoplasticMaterial) else ())
+        phase_diag[name] = {
+            "volume": vol,
+            "volume_fraction": vol / cache.volume,
+            "average_stress": np.asarray(res.phase_average_stress[pid][:], dtype=np.float64),
+            "free_energy_density": float(res.phase_free_energy_density[pid]),
+            "dissipation_density": float(res.phase_dissipation_density[pid]),
+            "state_fields": state_fields,
+        }
+    diagnostics = {
+        "micro_residual_norm": float(res.micro_residual_norm),
+        "micro_residual_tolerance": float(res.micro_residual_tolerance),
+        "newton_iterations": int(res.newton_iterations),
+        "armijo_backtracks": int(res.armijo_backtracks),
+        "volume": float(res.volume),
+        "virtual_work_stress_error": float(res.virtual_work_stress_error),
+        "periodic_fluctuation_error": float(res.periodic_fluctuation_error),
+        "reaction_antiperiodicity_error": float(res.reaction_antiperiodicity_error),
+        "periodic_class_reaction_error": float(res.periodic_class_reaction_error),
+        "macro_work_increment": float(res.macro_work_increment),
+        "micro_work_increment": float(res.micro_work_increment),
+        "hill_mandel_residual": float(res.hill_mandel_residual),
+        "hill_mandel_error": float(res.hill_mandel_error),
+        "positive_work_density": float(res.positive_work_density),
+        "negative_work_density": float(res.negative_work_density),
+        "unloading_volume_fraction": float(res.unloading_volume_fraction),
+        "viscoplastic_active_volume_fraction": float(res.viscoplastic_active_volume_fraction),
+        "yield_activation_margin_available": int(res.yield_activation_margin_available),
+        "minimum_yield_activation_margin": float(res.minimum_yield_activation_margin),
+        "maximum_yield_activation_margin": float(res.maximum_yield_activation_margin),
+        # GPU-only counters
+        "linear_iterations": int(res.linear_iterations),
+        "gauss_point_updates": int(res.gp_updates),
+    }
+    ctan = (np.asarray(res.effective_tangent[:], dtype=np.float64).reshape(6, 6)
+            if request.requirements.tangent else None)
+
+    def diag_builder(diag):
+        return diagnostics_dict(cache.phases, cache.point_phase_ids if cache.phased else None,
+                                diag, True, cache.phased)
+
+    return RVEResponse(
+        macro_stress=np.asarray(res.macro_stress[:], dtype=np.float64),
+        effective_tangent=ctan,
+        state=state,
+        fields=fields,
+        free_energy_density=float(res.free_energy_density),
+        dissipation_density=float(res.dissipation_density),
+        phase_diagnostics=phase_diag,
+        diagnostics=diagnostics,
+        diag_builder=diag_builder,
+    )
+
+
+# ---------------------------------------------------------------------------
+# path drivers (src/fem/rve/path.py)
+
+
+@dataclass(frozen=True)
+class AcceptedRVEPathStep:
+    step_id: int
+    time: float
+    time_step: float
+    previous_state: RVEState
+    response: RVEResponse
+
+
+def iterate_prescribed_rve_path(model, times, macro_strains, update_settings,
+                                progress_callback=None, retain_micro_fields: bool = True,
+                                requirements: MaterialResponseRequirements | None = None
+                                ) -> Iterator[AcceptedRVEPathStep]:
+    """``src/fem/rve/path.py:47-100``."""
+    times = np.asarray(times, dtype=np.float64)
+    macro_strains = np.asarray(macro_strains, dtype=np.float64)
+    if times.ndim != 1 or macro_strains.shape != (times.size, 6):
+        raise ValueError("Prescribed RVE path requires time (n,) and strain (n, 6).")
+    dts = np.diff(times)
+    if times.size < 2 or np.any(dts <= 0.0):
+        raise ValueError("Prescribed RVE path requires at least two increasing times.")
+    cache = RVEOperatorCache(model)
+    start = perf_counter()
+    state = initialize_rve_state(model, cache)
+    req_kw = {} if requirements is None else {"requirements": requirements}
+    for step_id, dt in enumerate(dts, start=1):
+        prev = state
+        resp = solve_rve_microequilibrium(
+            model, RVERequest(macro_strain=macro_strains[step_id], time_step=float(dt),
+                              material_update_settings=update_settings, **req_kw),
+            state, cache, retain_micro_fields)
+        state = resp.state
+        if progress_callback is not None:
+            progress_callback(SolverProgressEvent(
+                stage="rve_path", scheme="quasi_static_microequilibrium", accepted_step=step_id,
+                nominal_total_steps=dts.size, physical_time=float(times[step_id]),
+                final_time=float(times[-1]),
+                newton_iterations=int(resp.diagnostics["newton_iterations"]),
+                armijo_backtracks=int(resp.diagnostics["armijo_backtracks"]),
+                elapsed_seconds=perf_counter() - start,
+                residual_norm=float(resp.diagnostics["micro_residual_norm"]),
+                residual_tolerance=float(resp.diagnostics["micro_residual_tolerance"]),
+            ))
+        yield AcceptedRVEPathStep(step_id, float(times[step_id]), float(dt), prev, resp)
+
+
+def solve_prescribed_rve_path(model, times, macro_strains, update_settings,
+                              progress_callback=None) -> RVEPathResult:
+    """``src/fem/rve/path.py:103-123``."""
+    acc = tuple(iterate_prescribed_rve_path(model, times, macro_strains, update_settings,
+                                            progress_callback))
+    return RVEPathResult(times=np.asarray([s.time for s in acc]),
+                         responses=tuple(s.response for s in acc))

@@ -1,0 +1,339 @@
+/*
+ * cia.h -- C ABI of libcia_hip.so, the MI355X (gfx950) FEM hot path.
+ *
+ * Every pointer documented "device" is device memory owned by the caller
+ * (for example a torch tensor's data_ptr()).  Arrays of per-point data are
+ * structure-of-arrays, component-major: component c of point p lives at
+ * [c * n + p].  Voigt order is xx, yy, zz, yz, xz, xy; strains carry
+ * engineering shear, stresses are tensor components.  The "stream" argument is
+ * a hipStream_t (NULL = the default stream).  Calls return cia_status; a
+ * material-point failure fills the caller's cia_point_failure record.
+ *
+ * The Python host classes that mirror the reference interface bind these
+ * entry points through ctypes (concrete_impact_ai_amd/_lib.py).
+ */
+#ifndef CIA_H
+#define CIA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t cia_status;
+
+enum {
+  CIA_OK = 0,
+  CIA_ERR_INVALID_ARGUMENT = 1,
+  CIA_ERR_HIP = 2,
+  CIA_ERR_POINT_FAILURE = 3, /* a material point failed; see cia_point_failure */
+  CIA_ERR_RVE_FAILURE = 4,   /* RVE Newton failed; see cia_rve_result.reason */
+  CIA_ERR_OUT_OF_MEMORY = 5,
+  CIA_ERR_NOT_CONVERGED = 6
+};
+
+/* Material-point failure reasons.  Each maps 1:1 onto the reason string of
+ * MaterialPointConvergenceError in src/fem/materials/plasticity.py:543-854,1441-1465. */
+enum {
+  CIA_PT_OK = 0,
+  CIA_PT_NON_FINITE_J2_INVARIANT = 1,
+  CIA_PT_NEGATIVE_J2_INVARIANT = 2,
+  CIA_PT_INVALID_YIELD_ACTIVATION_TOLERANCE = 3,
+  CIA_PT_NON_FINITE_CLOSED_FORM = 4,
+  CIA_PT_CLOSED_FORM_SOLUTION_OUT_OF_BOUNDS = 5,
+  CIA_PT_INVALID_ITERATION_LIMIT = 6,
+  CIA_PT_NON_FINITE_PROBLEM_SCALE = 7,
+  CIA_PT_INVALID_CORRECTED_OVERSTRESS = 8,
+  CIA_PT_NON_FINITE_ITERATION_VALUE = 9,
+  CIA_PT_NON_POSITIVE_DERIVATIVE = 10,
+  CIA_PT_CONVERGED_SOLUTION_OUT_OF_BOUNDS = 11,
+  CIA_PT_NON_FINITE_NEWTON_CANDIDATE = 12,
+  CIA_PT_NEWTON_CANDIDATE_OUT_OF_BOUNDS = 13,
+  CIA_PT_MAXIMUM_ITERATIONS_EXCEEDED = 14,
+  CIA_PT_NON_POSITIVE_TANGENT_OVERSTRESS = 15 /* plasticity.py:918 RuntimeError */
+};
+
+/* Material kinds of the phase table. */
+enum { CIA_MAT_LINEAR_ELASTIC = 0, CIA_MAT_J2_VISCOPLASTIC = 1 };
+
+/* One phase: J2ViscoplasticMaterial (plasticity.py:79-91) or
+ * LinearElasticMaterial (linear_elastic.py:25-32). */
+typedef struct {
+  int32_t kind;
+  int32_t pad;
+  double young_modulus;
+  double poisson_ratio;
+  double yield_stress;
+  double hardening_modulus;
+  double time_scale;
+  double reference_stress;
+  double rate_exponent;
+  double density;
+} cia_material;
+
+/* MaterialUpdateSettings (src/fem/materials/data.py:16-24). */
+typedef struct {
+  int32_t max_iterations;
+  int32_t pad;
+  double yield_relative_tolerance;
+  double residual_absolute_tolerance;
+  double residual_relative_tolerance;
+} cia_update_settings;
+
+/* Output selection flags of cia_material_update_*. */
+enum {
+  CIA_OUT_TANGENT = 1,     /* full 6x6 algorithmic tangent, [36][n] row-major per point */
+  CIA_OUT_ENERGY = 2,      /* free energy and dissipation, [n] each */
+  CIA_OUT_DIAGNOSTICS = 4  /* CIA_N_DIAG diagnostic rows, [CIA_N_DIAG][n] */
+};
+
+/* Diagnostic rows written with CIA_OUT_DIAGNOSTICS, in this order.  Names and
+ * meaning follow plasticity.py:578-595,641-678,704-708,1321-1327. */
+enum {
+  CIA_D_YIELD_VALUE = 0,
+  CIA_D_PLASTIC_WORK,
+  CIA_D_LOCAL_ITERATIONS,
+  CIA_D_ACTIVE_SURFACE,
+  CIA_D_PLASTIC_MULTIPLIER,
+  CIA_D_VISCOPLASTIC_MULTIPLIER,
+  CIA_D_EQUIVALENT_STRESS,
+  CIA_D_TRIAL_YIELD_VALUE,
+  CIA_D_YIELD_SCALE,
+  CIA_D_YIELD_ACTIVATION_THRESHOLD,
+  CIA_D_YIELD_ACTIVATION_MARGIN,
+  CIA_D_VISCOPLASTIC_ACTIVE,
+  CIA_D_TIME_STEP,
+  CIA_D_EQUIVALENT_PLASTIC_STRAIN,
+  CIA_D_VISCOPLASTIC_HV,
+  CIA_D_PLASTIC_POWER,
+  CIA_D_HARDENING_STORAGE_RATE,
+  CIA_D_LOCAL_RESIDUAL,
+  CIA_D_ROOT_LOWER_BOUND,
+  CIA_D_ROOT_UPPER_BOUND,
+  CIA_D_INCREMENTAL_WORK_DENSITY,
+  CIA_D_EQUIVALENT_STRESS_INCREMENT,
+  CIA_N_DIAG
+};
+
+/* First failing point of a material update (lowest point id), with the
+ * scalars of the reference's failure diagnostics (plasticity.py:857-895). */
+typedef struct {
+  int64_t point_id;        /* -1 when no point failed */
+  int32_t reason;          /* CIA_PT_* */
+  int32_t iteration;
+  double plastic_multiplier;
+  double corrected_overstress;
+  double residual;
+  double derivative;
+  double newton_candidate;
+  double trial_yield;
+  double physical_upper_bound;
+  double residual_tolerance;
+  double trial_equivalent_stress;
+  double yield_radius;
+  double equivalent_plastic_strain;
+  double radicand;
+  double trial_stress[6];
+  double strain[6];
+} cia_point_failure;
+
+/* Library / device information.  No reference counterpart (host plumbing). */
+int32_t cia_abi_version(void);
+cia_status cia_device_name(char* buffer, int32_t capacity);
+/* Message of the last failed HIP call or rejected argument on this thread. */
+cia_status cia_last_error(char* buffer, int32_t capacity);
+
+/*
+ * Material-point update -- replaces J2ViscoplasticMaterial.update /
+ * LinearElasticMaterial.update / PhasedMaterialModel.update
+ * (src/fem/materials/plasticity.py:110-117,343-431;
+ *  src/fem/materials/linear_elastic.py:44-52,74-100;
+ *  src/fem/materials/phased.py:67-143).
+ *
+ *   n            number of points
+ *   phases       host array of n_phases phase descriptors
+ *   phase_id     device int32 [n] phase of each point, or NULL (all phase 0)
+ *   strain       device [6][n] total strain (engineering shear)
+ *   strain_rate  device [6][n] or NULL (enables the increment diagnostics)
+ *   state_in     device [8][n] committed state rows: plastic_strain (6),
+ *                equivalent_plastic_strain, viscoplastic_multiplier; ignored
+ *                for elastic points; not modified
+ *   stress       device [6][n]
+ *   state_out    device [8][n] next state (elastic points copy state_in)
+ *   tangent      device [36][n] if CIA_OUT_TANGENT, else may be NULL
+ *   free_energy, dissipation  device [n] if CIA_OUT_ENERGY
+ *   diagnostics  device [CIA_N_DIAG][n] if CIA_OUT_DIAGNOSTICS
+ *   failure      host record, filled on CIA_ERR_POINT_FAILURE (point_id = -1 otherwise)
+ *
+ * The call synchronises the stream to report failures.
+ */
+cia_status cia_material_update_f64(
+    void* stream, int64_t n, const cia_material* phases, int32_t n_phases,
+    const int32_t* phase_id, const double* strain, const double* strain_rate,
+    const double* state_in, double time_step, const cia_update_settings* settings,
+    uint32_t flags, double* stress, double* state_out, double* tangent,
+    double* free_energy, double* dissipation, double* diagnostics,
+    cia_point_failure* failure);
+
+/* FP32 variant: strain, stress, state and tangent in float; the
+ * yield-consistency multiplier solve stays FP64 (north-star precision mode). */
+cia_status cia_material_update_f32(
+    void* stream, int64_t n, const cia_material* phases, int32_t n_phases,
+    const int32_t* phase_id, const float* strain, const float* strain_rate,
+    const float* state_in, double time_step, const cia_update_settings* settings,
+    uint32_t flags, float* stress, float* state_out, float* tangent,
+    float* free_energy, float* dissipation, float* diagnostics,
+    cia_point_failure* failure);
+
+/* ------------------------------------------------------------------------ */
+/* Structured periodic Hex8 RVE (src/fem/rve/solver.py, periodic.py, path.py) */
+
+/* NonlinearNewtonSettings + ArmijoSettings (src/fem/solvers/data.py:32-95). */
+typedef struct {
+  int32_t max_iterations;
+  int32_t armijo_enabled;
+  double residual_absolute_tolerance;
+  double residual_relative_tolerance;
+  double increment_absolute_tolerance;
+  double increment_relative_tolerance;
+  double armijo_sufficient_decrease;
+  double armijo_reduction_factor;
+  int32_t armijo_max_backtracks;
+  int32_t pad;
+} cia_newton_settings;
+
+/* Linear solver of the Newton step: block-Jacobi PCG (no reference
+ * counterpart; replaces spsolve, src/fem/solvers/linear.py:15-22). */
+typedef struct {
+  int32_t max_iterations;   /* per solve */
+  int32_t check_interval;   /* iterations between host convergence checks */
+  double relative_tolerance; /* ||r_k|| <= rtol * ||b|| */
+  double absolute_tolerance;
+} cia_linear_settings;
+
+enum { CIA_PREC_FP64 = 0, CIA_PREC_FP32_REFINED = 1 };
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t n_phases;
+  double lengths[3];
+  const cia_material* phases;     /* host [n_phases] */
+  const int32_t* element_phase;   /* host [nx*ny*nz] element phase ids, NULL = phase 0 */
+  cia_newton_settings newton;
+  cia_linear_settings linear;
+  double hill_mandel_power_scale;
+  int32_t precision;              /* CIA_PREC_* */
+  int32_t pad;
+} cia_rve_desc;
+
+/* RVE failure reasons (src/fem/rve/solver.py:129-141,143-158,214-218,455-459). */
+enum {
+  CIA_RVE_OK = 0,
+  CIA_RVE_MAXIMUM_ITERATIONS_EXCEEDED = 1,
+  CIA_RVE_NON_FINITE_INCREMENT = 2,
+  CIA_RVE_ARMIJO_EXHAUSTED = 3,
+  CIA_RVE_NON_FINITE_EFFECTIVE_TANGENT_SENSITIVITY = 4,
+  CIA_RVE_MATERIAL_POINT_FAILURE = 5
+};
+
+/* Homogenised response and the scalar diagnostics of _build_rve_response
+ * (src/fem/rve/solver.py:253-364). */
+typedef struct {
+  int32_t reason;                 /* CIA_RVE_* */
+  int32_t newton_iterations;
+  int32_t armijo_backtracks;
+  int32_t linear_iterations;      /* PCG iterations summed over the increment */
+  int64_t gp_updates;             /* constitutive point evaluations performed */
+  double macro_stress[6];
+  double effective_tangent[36];   /* row-major, valid when requested */
+  double free_energy_density;
+  double dissipation_density;
+  double micro_residual_norm;
+  double micro_residual_tolerance;
+  double volume;
+  double virtual_work_stress_error;
+  double periodic_fluctuation_error;
+  double reaction_antiperiodicity_error;
+  double periodic_class_reaction_error;
+  double macro_work_increment;
+  double micro_work_increment;
+  double hill_mandel_residual;
+  double hill_mandel_error;
+  double positive_work_density;
+  double negative_work_density;
+  double unloading_volume_fraction;
+  double viscoplastic_active_volume_fraction;
+  int32_t yield_activation_margin_available;
+  int32_t pad;
+  double minimum_yield_activation_margin;
+  double maximum_yield_activation_margin;
+  /* failure snapshot (solver.py:497-518) */
+  double failure_residual_norm;
+  double failure_residual_tolerance;
+  double failure_increment_norm;
+  double failure_increment_tolerance;
+  int32_t failure_iteration;
+  int32_t pad2;
+  cia_point_failure point_failure;
+  /* phase diagnostics (solver.py:391-435), up to 8 phases */
+  double phase_volume[8];
+  double phase_average_stress[8][6];
+  double phase_free_energy_density[8];
+  double phase_dissipation_density[8];
+} cia_rve_result;
+
+typedef struct cia_rve cia_rve;
+
+/* Build the device operator of one structured periodic RVE
+ * (StructuredHex8RVE + build_rve_operator_cache, src/fem/rve/data.py:88-113,
+ *  src/fem/rve/solver.py:367-388). */
+cia_status cia_rve_create(const cia_rve_desc* desc, cia_rve** out);
+cia_status cia_rve_destroy(cia_rve* rve);
+/* Sizes: n_elements, n_points (= 8 n_elements), n_classes, n_nodes. */
+cia_status cia_rve_sizes(const cia_rve* rve, int64_t* sizes4);
+
+/*
+ * One strain-driven increment: solve_rve_microequilibrium
+ * (src/fem/rve/solver.py:62-227).
+ *
+ *   committed_state  device [8][n_points] state rows (see cia_material_update_f64);
+ *                    element type double for CIA_PREC_FP64, float for
+ *                    CIA_PREC_FP32_REFINED
+ *   committed_macro  host [6] macro strain of the committed state
+ *   committed_w      device FP64 [3 * n_classes] periodic fluctuation per class
+ *                    (class 0, the eliminated reference class, holds zeros;
+ *                    the reference's reduced vector is entries 3..end)
+ *   macro_strain     host [6] requested macro strain
+ *   next_state       device [8][n_points], same element type (must not alias)
+ *   next_w           device FP64 [3 * n_classes]
+ *   result           host, filled on return
+ *
+ * Micro fields of the converged state stay inside the RVE object and are
+ * copied out with cia_rve_copy_field.
+ */
+cia_status cia_rve_solve(
+    cia_rve* rve, void* stream, const void* committed_state,
+    const double* committed_macro, const double* committed_w,
+    const double* macro_strain, double time_step,
+    const cia_update_settings* settings, int32_t need_tangent,
+    void* next_state, double* next_w, cia_rve_result* result);
+
+/* Micro fields of the last converged increment. */
+enum {
+  CIA_FIELD_MICRO_STRAIN = 0,      /* [6][n_points] */
+  CIA_FIELD_MICRO_STRAIN_RATE = 1, /* [6][n_points] */
+  CIA_FIELD_MICRO_STRESS = 2,      /* [6][n_points] */
+  CIA_FIELD_FREE_ENERGY = 3,       /* [n_points] */
+  CIA_FIELD_DISSIPATION = 4,       /* [n_points] */
+  CIA_FIELD_DIAGNOSTICS = 5,       /* [CIA_N_DIAG][n_points] */
+  CIA_FIELD_DISPLACEMENT = 6,      /* [3 * n_nodes] full node-major displacement */
+  CIA_FIELD_INTERNAL_FORCE = 7     /* [3 * n_nodes] full node-major internal force */
+};
+cia_status cia_rve_copy_field(cia_rve* rve, void* stream, int32_t field, double* dst);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CIA_H */

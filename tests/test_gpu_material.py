@@ -1,0 +1,206 @@
+"""GPU parity of the material-point update against the FP64 oracle.
+
+Calls libcia_hip.so through the C ABI (``cia_material_update_f64/_f32``) via the
+host mirror of ``MaterialModel.update``.  Bar: 1e-10 relative for FP64;
+the FP32 variant is reported against FP64 with the tolerance stated below.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import j2vp as OJ
+
+pytestmark = pytest.mark.gpu
+
+REF_MAT = dict(young_modulus=3339.9160679991464, poisson_ratio=0.36994096308414537,
+               yield_stress=10.0, hardening_modulus=80.0)
+
+
+def _materials(m, tau=0.02):
+    from concrete_impact_ai_amd import materials as M
+
+    gpu = M.J2ViscoplasticMaterial("j2", 1.0, REF_MAT["young_modulus"], REF_MAT["poisson_ratio"],
+                                   REF_MAT["yield_stress"], REF_MAT["hardening_modulus"], tau, 10.0, m)
+    orc = OJ.J2VPParams(REF_MAT["young_modulus"], REF_MAT["poisson_ratio"], REF_MAT["yield_stress"],
+                        REF_MAT["hardening_modulus"], tau, 10.0, m)
+    return gpu, orc
+
+
+def _random_inputs(n, seed, scale=0.01):
+    rng = np.random.default_rng(seed)
+    strain = rng.normal(scale=scale, size=(n, 6))
+    ep = rng.normal(scale=scale * 0.3, size=(n, 6))
+    ep[:, :3] -= ep[:, :3].mean(axis=1, keepdims=True)
+    q = np.abs(rng.normal(scale=scale * 0.5, size=n))
+    g = q * rng.random(n)
+    rate = rng.normal(scale=scale, size=(n, 6)) * 10.0
+    return strain, ep, q, g, rate
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("m", [1.0, 2.0, 3.5])
+def test_j2vp_fp64_matches_oracle(gpu, m):
+    from concrete_impact_ai_amd import materials as M
+
+    gmat, omat = _materials(m)
+    n = 4099  # ragged: not a multiple of the 256-thread block
+    strain, ep, q, g, rate = _random_inputs(n, seed=int(m * 10))
+    settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    state = M.MaterialState({"plastic_strain": ep, "equivalent_plastic_strain": q,
+                             "viscoplastic_multiplier": g})
+    dt = 1.0e-3
+    req = M.MaterialPointRequest(strain, rate, dt, "three_dimensional", settings)
+    resp = gmat.update(req, state, M.MaterialResponseRequirements(True, True, True))
+    ref = OJ.j2vp_update(omat, strain, ep, q, g, dt, OJ.UpdateSettings(50, 1e-10, 1e-12, 1e-10), rate)
+    plastic = ref["plastic_mask"]
+    assert 0.2 < plastic.mean() < 0.98  # both branches exercised
+    assert _rel(resp.stresses, ref["stress"]) < 1e-10
+    assert _rel(resp.tangents, ref["tangent"]) < 1e-10
+    assert _rel(resp.free_energy, ref["free_energy"]) < 1e-10
+    assert _rel(resp.dissipation, ref["dissipation"]) < 1e-9
+    for key in ("plastic_strain", "equivalent_plastic_strain", "viscoplastic_multiplier"):
+        assert _rel(resp.state.variables[key], ref[key]) < 1e-10, key
+    assert set(resp.diagnostics) == set(ref["diagnostics"])
+    for key, val in ref["diagnostics"].items():
+        got = resp.diagnostics[key]
+        if key in ("local_residual",):
+            assert np.max(np.abs(got - val)) < 1e-12 * max(1.0, np.max(np.abs(val))), key
+        elif key == "local_iterations":
+            assert np.array_equal(got, val), key
+        else:
+            assert _rel(got, val) < 1e-9, key
+    # inputs were not mutated (test_plasticity.py:472-494 analogue)
+    assert np.array_equal(state.variables["plastic_strain"], ep)
+
+
+def test_j2vp_pure_shear_closed_form(gpu):
+    """Reference config plastic_j2_viscoplastic_material_point.yaml: 121 steps,
+    gamma_max 0.023, dt 2, tau 30, m = 1, rel 1e-10 / abs 1e-9."""
+    from concrete_impact_ai_amd import materials as M
+
+    gmat, omat = _materials(1.0, tau=30.0)
+    gammas = np.linspace(0.0, 0.023, 121)
+    exact = OJ.pure_shear_recurrence(omat, gammas, 2.0)
+    state = gmat.initialize_state(1)
+    settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    for k, gam in enumerate(gammas):
+        strain = np.zeros((1, 6))
+        strain[0, 5] = gam
+        resp = gmat.update(M.MaterialPointRequest(strain, None, 2.0, "three_dimensional", settings),
+                           state)
+        state = resp.state
+        assert resp.stresses[0, 5] == pytest.approx(exact["shear_stress"][k], rel=1e-10, abs=1e-9)
+        assert resp.diagnostics["plastic_multiplier"][0] == pytest.approx(
+            exact["plastic_multiplier"][k], rel=1e-10, abs=1e-9)
+        assert state.variables["equivalent_plastic_strain"][0] == pytest.approx(
+            exact["equivalent_plastic_strain"][k], rel=1e-10, abs=1e-9)
+
+
+def test_phased_material_matches_oracle(gpu):
+    from concrete_impact_ai_amd import materials as M
+
+    gmat, omat = _materials(1.0)
+    el = M.LinearElasticMaterial("agg", 1.0, 5000.0, 0.2)
+    rng = np.random.default_rng(3)
+    n = 777
+    pid = (rng.random(n) < 0.4).astype(np.int64)
+    strain, ep, q, g, rate = _random_inputs(n, 5)
+    nj = int(np.sum(pid == 0))
+    state = M.MaterialState({"phase_m__plastic_strain": ep[:nj], "phase_m__equivalent_plastic_strain": q[:nj],
+                             "phase_m__viscoplastic_multiplier": g[:nj]})
+    model = M.PhasedMaterialModel("ph", (M.RVEPhase("m", gmat), M.RVEPhase("a", el)), pid, 1.0)
+    settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    resp = model.update(M.MaterialPointRequest(strain, rate, 1e-3, "three_dimensional", settings),
+                        state, M.MaterialResponseRequirements(True, True, True))
+    ref = OJ.phased_update([("m", omat), ("a", OJ.ElasticParams(5000.0, 0.2))], pid, strain,
+                           dict(state.variables), 1e-3, OJ.UpdateSettings(), rate)
+    assert _rel(resp.stresses, ref["stress"]) < 1e-10
+    assert _rel(resp.tangents, ref["tangent"]) < 1e-10
+    assert list(resp.diagnostics) == list(ref["diagnostics"])
+    for key, val in ref["diagnostics"].items():
+        got = resp.diagnostics[key]
+        assert np.array_equal(np.isnan(got), np.isnan(val)), key
+        fin = ~np.isnan(val)
+        if key == "local_residual":
+            assert np.max(np.abs(got[fin] - val[fin])) < 1e-12
+        else:
+            assert _rel(got[fin], val[fin]) < 1e-9, key
+    for key, val in ref["state"].items():
+        assert _rel(resp.state.variables[key], val) < 1e-10
+
+
+@pytest.mark.parametrize(
+    "case,reason",
+    [("maxit", "maximum_iterations_exceeded"), ("nonfinite", "non_finite_iteration_value"),
+     ("nan", "non_finite_j2_invariant")],
+)
+def test_failure_reasons_match_reference(gpu, case, reason):
+    """Reference failure tests, test_plasticity.py:259-345."""
+    from concrete_impact_ai_amd import materials as M
+
+    sref = 1e-300 if case == "nonfinite" else 10.0
+    mat = M.J2ViscoplasticMaterial("f", 1.0, REF_MAT["young_modulus"], REF_MAT["poisson_ratio"], 10.0,
+                                   80.0, 0.02, sref, 2.0)
+    strain = np.asarray([[0.004, -0.0015, -0.001, 0.002, -0.003, 0.02]])
+    if case == "nan":
+        strain = np.zeros((1, 6))
+        strain[0, 5] = np.nan
+    settings = (M.MaterialUpdateSettings(1, 1e-10, 1e-30, 1e-30) if case == "maxit"
+                else M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10))
+    state = mat.initialize_state(1)
+    snap = {k: v.copy() for k, v in state.variables.items()}
+    with pytest.raises(M.MaterialPointConvergenceError) as info:
+        mat.update(M.MaterialPointRequest(strain, np.zeros((1, 6)) if case == "maxit" else None, 1e-3,
+                                          "three_dimensional", settings), state,
+                   M.MaterialResponseRequirements(True, True, True))
+    err = info.value
+    assert err.reason == reason
+    assert err.diagnostics["point_id"] == 0
+    if case == "maxit":
+        assert err.diagnostics["iteration"] == 1
+        assert 0.0 < err.diagnostics["newton_candidate"] < err.diagnostics["physical_upper_bound"]
+    if case == "nonfinite":
+        assert err.diagnostics["residual"] in {"inf", "-inf"}
+    if case == "nan":
+        assert err.diagnostics["algorithm"] == "j2_equivalent_stress"
+    for k, v in state.variables.items():
+        assert np.array_equal(v, snap[k])
+
+
+def test_first_failing_point_is_lowest_index(gpu):
+    from concrete_impact_ai_amd import materials as M
+
+    gmat, _ = _materials(1.0)
+    strain = np.full((5000, 6), 1e-4)
+    strain[4321, 2] = np.nan
+    strain[1234, 0] = np.inf
+    settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    with pytest.raises(M.MaterialPointConvergenceError) as info:
+        gmat.update(M.MaterialPointRequest(strain, None, 1e-3, "three_dimensional", settings),
+                    gmat.initialize_state(5000))
+    assert info.value.diagnostics["point_id"] == 1234
+
+
+def test_fp32_variant_error_against_fp64(gpu):
+    """FP32 storage/arithmetic with the FP64 multiplier solve: stresses within
+    2e-6 relative (of the max stress) of the FP64 oracle."""
+    import torch
+
+    from concrete_impact_ai_amd import materials as M
+
+    gmat, omat = _materials(1.0)
+    n = 100_000
+    strain, ep, q, g, _ = _random_inputs(n, 11)
+    st = np.vstack([ep.T, q[None], g[None]])
+    dev = gpu
+    s32, st32, *_ = M.update_device([gmat], None, torch.tensor(strain.T.copy(), device=dev, dtype=torch.float32),
+                                    None, torch.tensor(st, device=dev, dtype=torch.float32), 1e-3,
+                                    M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10))
+    ref = OJ.j2vp_update(omat, strain.astype(np.float32).astype(np.float64),
+                         ep.astype(np.float32).astype(np.float64), q.astype(np.float32).astype(np.float64),
+                         g.astype(np.float32).astype(np.float64), 1e-3, OJ.UpdateSettings(), None)
+    err = np.max(np.abs(s32.double().cpu().numpy().T - ref["stress"])) / np.max(np.abs(ref["stress"]))
+    assert err < 2e-6, err
