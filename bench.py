@@ -1,0 +1,262 @@
+"""Benchmark: periodic two-phase RVE increments (FE² micro solve) on one MI355X.
+
+Workload (the north-star metric's configuration that fits one GPU): periodic
+RVE of 48 x 48 x 48 Hex8 elements (110,592 elements, 884,736 Gauss points),
+spherical elastic aggregates (E 50 GPa, nu 0.2) at 40 % volume fraction with
+radii U[2, 6] elements in a J2 Perzyna mortar (E 25 GPa, nu 0.2, sigma_y 20 MPa,
+H 0.5 GPa, eta 1e-3 s, m 1), driven along a seeded random 6-component
+piecewise-linear macro strain path (200 steps, |eps| <= 3e-3, rates 10-1000 1/s).
+A step is one converged RVE increment (Newton + Armijo + PCG) with inputs
+resident in HBM; the effective tangent is not requested (explicit FE² use).
+
+Prints one JSON line (see the repository's measurement notes for the
+roofline and CPU-baseline definitions).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Gauss-pt updates/s & RVE solves/s; surrogate vs plain FE² speedup at ≤2% err"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_model(n: int, seed: int, precision: str, pcg_rtol: float):
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd.microstructure import spherical_aggregates
+    from concrete_impact_ai_amd.solvers import (ArmijoSettings, LinearSolverSettings,
+                                                NonlinearNewtonSettings)
+
+    eph = spherical_aggregates((n, n, n), 0.40, (2.0, 6.0), seed=seed)
+    mortar = M.J2ViscoplasticMaterial("mortar", 2400.0, 25.0e9, 0.2, 20.0e6, 0.5e9, 1.0e-3, 20.0e6, 1.0)
+    agg = M.LinearElasticMaterial("aggregate", 2400.0, 50.0e9, 0.2)
+    nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (n, n, n))
+    # Newton tolerance 1e-10 (relative); absolute tolerances force-scaled for SI units:
+    # 1e-10 * sigma_y * L^2 for the residual, 1e-12 m for the increment.
+    newton = NonlinearNewtonSettings(25, 1.0e-10 * 20.0e6, 1.0e-10, 1.0e-12, 1.0e-10,
+                                     ArmijoSettings(True, 1.0e-4, 0.5, 20))
+    lin = LinearSolverSettings(method="spectral", max_iterations=5000, relative_tolerance=pcg_rtol,
+                               check_interval=8)
+    model = R.StructuredHex8RVE(
+        nodes, elements, (n, n, n),
+        M.PhasedMaterialModel("c2_rve", (M.RVEPhase("mortar", mortar), M.RVEPhase("aggregate", agg)),
+                              np.repeat(eph, 8), 2400.0),
+        R.build_structured_hex8_periodic_constraint(nodes, (n, n, n)), newton,
+        element_phase_ids=eph, phase_names=("mortar", "aggregate"), linear_settings=lin,
+        precision=precision)
+    return model, eph
+
+
+def cpu_baseline(n: int, seed: int, newton_per_step: float, backtracks_per_step: float) -> dict:
+    """Oracle (FP64 NumPy/SciPy port of the reference algorithm) on the host.
+
+    Sample: one full Newton assembly (strain fields, J2/elastic point updates,
+    internal force, tangent COO -> CSR) of the same 48^3 RVE.  A reference
+    increment needs at least 1 + newton (1 + candidates) such assemblies plus one
+    SuperLU factorisation per Newton iteration; the SuperLU time is excluded,
+    so the reported rate is an upper bound on the CPU RVE solves/s.
+    """
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    from concrete_impact_ai_amd.microstructure import random_strain_path, spherical_aggregates
+    from oracle import j2vp as OJ
+    from oracle import rve as OR
+
+    eph = spherical_aggregates((n, n, n), 0.40, (2.0, 6.0), seed=seed)
+    ctx = threadpool_limits(limits=1) if threadpool_limits else None
+    try:
+        model = OR.Model((1.0, 1.0, 1.0), (n, n, n),
+                         [("mortar", OJ.J2VPParams(25e9, 0.2, 20e6, 0.5e9, 1e-3, 20e6, 1.0)),
+                          ("aggregate", OJ.ElasticParams(50e9, 0.2))],
+                         OR.Newton(25, 2e-3, 1e-10, 1e-12, 1e-10, OR.Armijo(True)),
+                         element_phase_ids=eph)
+        times, strains = random_strain_path(200, 8, 3.0e-3, (10.0, 1000.0), seed=seed)
+        st = model.initial_state()
+        u = model.constraint.affine @ strains[2]
+        up = model.constraint.affine @ strains[1]
+        t0 = time.perf_counter()
+        model.assemble(u, up, times[2] - times[1], OJ.UpdateSettings(), st["material_state"])
+        t_asm = time.perf_counter() - t0
+    finally:
+        if ctx is not None:
+            ctx.__exit__(None, None, None)
+    # iterations 0..N each assemble once; each of the N Armijo searches evaluates 1 + bt candidates
+    assemblies = 1.0 + 2.0 * newton_per_step + backtracks_per_step
+    per_step = assemblies * t_asm
+    return {
+        "value": 1.0 / per_step,
+        "unit": "RVE solves/s (upper bound)",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle FP64 assembly of the same {n}^3 RVE timed once ({t_asm:.2f} s); "
+                   f"x {assemblies:.2f} assemblies per increment at the GPU's Newton/Armijo "
+                   "counts; SuperLU factorisations excluded (lower bound on CPU time)"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=48)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--pcg-rtol", type=float, default=1.0e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of this path is design-only).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from concrete_impact_ai_amd import _lib
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd.materials import MaterialResponseRequirements
+    from concrete_impact_ai_amd.microstructure import random_strain_path
+
+    torch.cuda.set_device(0)
+    lib = _lib.lib()
+    n = args.size
+    model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol)
+    times, strains = random_strain_path(200, 8, 3.0e-3, (10.0, 1000.0), seed=args.seed)
+    settings = M.MaterialUpdateSettings(50, 1.0e-10, 1.0e-12, 1.0e-10)
+    req_no_tan = MaterialResponseRequirements(tangent=False)
+    cache = R.build_rve_operator_cache(model)
+    state = R.initialize_rve_state(model, cache)
+    total = args.warmup + args.steps
+    if total > 200:
+        raise SystemExit("the path has 200 increments")
+
+    def step(k, st):
+        return R.solve_rve_microequilibrium(
+            model, R.RVERequest(strains[k], float(times[k] - times[k - 1]), settings, req_no_tan),
+            st, cache, retain_micro_fields=False)
+
+    for k in range(1, args.warmup + 1):
+        state = step(k, state).state
+        print(f"warmup step {k} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    lib.cia_profile_enable(1)
+    tot_ms = (C.c_double * 6)()
+    cnt = (C.c_int64 * 6)()
+    lib.cia_profile_read(tot_ms, cnt, 6)  # clear
+    newton = backtracks = lin_its = 0
+    gp_updates = 0
+    t0 = time.perf_counter()
+    for k in range(args.warmup + 1, total + 1):
+        resp = step(k, state)
+        state = resp.state
+        d = resp.diagnostics
+        newton += d["newton_iterations"]
+        backtracks += d["armijo_backtracks"]
+        lin_its += d["linear_iterations"]
+        gp_updates += d["gauss_point_updates"]
+        print(f"step {k}: newton {d['newton_iterations']} pcg {d['linear_iterations']} "
+              f"active {d['viscoplastic_active_volume_fraction']:.3f}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.cia_profile_read(tot_ms, cnt, 6)
+    lib.cia_profile_enable(0)
+
+    K = args.steps
+    n_el = n**3
+    n_gp = 8 * n_el
+    n_dof = 3 * n_el
+    plastic_frac = float(d["viscoplastic_active_volume_fraction"])
+    bpe = 8 if args.precision == "fp64" else 4
+    # Algorithmic bytes per launch (unique HBM bytes the operation must move):
+    #  pcg_elem: compact tangent (d, r per point; flow direction n[6] on plastic
+    #            points), p_old + z read and p_new written once per class, the
+    #            24 element force rows written.
+    #  assemble: committed state read + trial state written (J2 points), stress and
+    #            compact tangent written, w read once per class, force rows written.
+    mortar_gp = float(np.mean(eph == 0)) * n_gp
+    alg = {
+        "pcg_elem": n_gp * bpe * (2 + 6 * plastic_frac) + n_dof * 8 * 3 + n_el * 24 * bpe,
+        "pcg_node": n_el * 24 * bpe + n_dof * 8 * 2,
+        "pcg_update": n_dof * 8 * 6,
+        "assemble": mortar_gp * bpe * 16 + n_gp * bpe * (6 + 2 + 6 * plastic_frac) + n_dof * 8
+                    + n_el * 24 * bpe,
+    }
+    kernels = {}
+    for i, name in enumerate(_lib.PROFILE_SLOTS):
+        if cnt[i] == 0:
+            continue
+        avg_s = tot_ms[i] / cnt[i] / 1e3
+        entry = {"launches": int(cnt[i]), "avg_us": avg_s * 1e6, "total_ms": tot_ms[i]}
+        if name in alg:
+            entry["alg_bytes"] = alg[name]
+            entry["achieved_GBs"] = alg[name] / avg_s / 1e9
+        kernels[name] = entry
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    ach = kernels[dom].get("achieved_GBs")
+    roofline = {
+        "kernel": f"k_{dom}",
+        "bound": "hbm",
+        "achieved": ach,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": (ach / HBM_PEAK_GBS) if ach else None,
+        "traffic": None,
+    }
+    value = K / elapsed
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "RVE solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 element work, f64 residual/dots",
+        "data": "synthetic (seeded sphere microstructure and strain path)",
+        "config": {
+            "workload": f"periodic two-phase RVE {n}^3 Hex8, increments {args.warmup + 1}-{total} "
+                        "of a 200-step random 6-component path, no effective tangent",
+            "elements": n_el,
+            "gauss_points": n_gp,
+            "reduced_dofs": n_dof - 3,
+            "aggregate_fraction": float(np.mean(eph == 1)),
+            "seed": args.seed,
+            "precision": args.precision,
+            "pcg_rtol": args.pcg_rtol,
+        },
+        "gp_updates_per_s": gp_updates / elapsed,
+        "newton_iterations_per_step": newton / K,
+        "armijo_backtracks_per_step": backtracks / K,
+        "pcg_iterations_per_step": lin_its / K,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(n, args.seed, newton / K, backtracks / K)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -167,7 +167,15 @@ class LinearSettings(C.Structure):
         ("check_interval", C.c_int32),
         ("relative_tolerance", C.c_double),
         ("absolute_tolerance", C.c_double),
+        ("preconditioner", C.c_int32),
+        ("pad", C.c_int32),
+        ("reference_bulk", C.c_double),
+        ("reference_shear", C.c_double),
     ]
+
+
+CIA_PCG_SPECTRAL = 0
+CIA_PCG_BLOCK_JACOBI = 1
 
 
 class RVEDesc(C.Structure):
@@ -257,7 +265,11 @@ _SIGNATURES = {
          C.POINTER(UpdateSettings), C.c_int32, _vp, _vp, C.POINTER(RVEResult)],
     ),
     "cia_rve_copy_field": (C.c_int32, [_vp, _vp, C.c_int32, _vp]),
+    "cia_profile_enable": (C.c_int32, [C.c_int32]),
+    "cia_profile_read": (C.c_int32, [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
 }
+
+PROFILE_SLOTS = ["assemble", "pcg_elem", "pcg_node", "pcg_update", "precond", "gather"]
 
 # Entry points whose prototypes are declared by optional translation units.
 _OPTIONAL: dict[str, tuple] = {}

@@ -57,7 +57,8 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=max(jobs, 1)) as pool:
         objs = list(pool.map(compile_one, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")

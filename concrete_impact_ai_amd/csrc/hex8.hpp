@@ -77,7 +77,7 @@ __device__ __forceinline__ void force_at(const T s[6], const BoxGeom& g, T f[8][
 
 // Strain columns of B_Q for node a (unit displacement in x, y, z).
 template <class T>
-__device__ __forceinline__ void b_columns(int q, int a, const BoxGeom& g, T col[3][6]) {
+__host__ __device__ __forceinline__ void b_columns(int q, int a, const BoxGeom& g, T col[3][6]) {
   const T bx = (T)(dnx(q, a) * g.ihx), by = (T)(dny(q, a) * g.ihy), bz = (T)(dnz(q, a) * g.ihz);
   // unit u_x: (bx, 0, 0, 0, bz, by); u_y: (0, by, 0, bz, 0, bx); u_z: (0, 0, bz, by, bx, 0)
   col[0][0] = bx; col[0][1] = 0;  col[0][2] = 0;  col[0][3] = 0;  col[0][4] = bz; col[0][5] = by;

@@ -70,10 +70,10 @@ struct PointExtra {
   T d[CIA_N_DIAG];
 };
 
-__device__ __forceinline__ bool finite(double x) { return isfinite(x); }
+__host__ __device__ __forceinline__ bool finite(double x) { return isfinite(x); }
 
 template <class T>
-__device__ __forceinline__ void elastic_stress(const PhaseConst& P, const T e[6], T s[6]) {
+__host__ __device__ __forceinline__ void elastic_stress(const PhaseConst& P, const T e[6], T s[6]) {
   const T lam = (T)P.lam, g = (T)P.G, g2 = (T)(2.0 * P.G);
   const T tr = e[0] + e[1] + e[2];
   s[0] = lam * tr + g2 * e[0];
@@ -86,7 +86,7 @@ __device__ __forceinline__ void elastic_stress(const PhaseConst& P, const T e[6]
 
 // 1.5 s:s with stress weights (1,1,1,2,2,2) of the deviator of s.
 template <class T>
-__device__ __forceinline__ T j2_radicand(const T s[6], T dev[6]) {
+__host__ __device__ __forceinline__ T j2_radicand(const T s[6], T dev[6]) {
   const T p = (s[0] + s[1] + s[2]) / (T)3;
   dev[0] = s[0] - p;
   dev[1] = s[1] - p;
@@ -99,7 +99,7 @@ __device__ __forceinline__ T j2_radicand(const T s[6], T dev[6]) {
 }
 
 template <class T>
-__device__ __forceinline__ void tangent_apply(const Tangent<T>& c, const T e[6], T s[6]) {
+__host__ __device__ __forceinline__ void tangent_apply(const Tangent<T>& c, const T e[6], T s[6]) {
   const T tr = e[0] + e[1] + e[2];
   const T t3 = tr / (T)3;
   const T kt = c.K * tr;
@@ -116,7 +116,7 @@ __device__ __forceinline__ void tangent_apply(const Tangent<T>& c, const T e[6],
 
 // Entry (i, j) of the compact tangent as a 6x6 matrix.
 template <class T>
-__device__ __forceinline__ T tangent_entry(const Tangent<T>& c, int i, int j) {
+__host__ __device__ __forceinline__ T tangent_entry(const Tangent<T>& c, int i, int j) {
   T v = c.r * c.n[i] * c.n[j];
   if (i < 3 && j < 3) v += c.K + c.d * ((i == j ? (T)1 : (T)0) - (T)1 / (T)3);
   if (i >= 3 && i == j) v += c.d * (T)0.5;
@@ -124,7 +124,7 @@ __device__ __forceinline__ T tangent_entry(const Tangent<T>& c, int i, int j) {
 }
 
 // Yield-consistency scalar solve, plasticity.py:711-854, always FP64.
-__device__ inline int solve_multiplier(const PhaseConst& P, const Settings& S, double dt,
+__host__ __device__ inline int solve_multiplier(const PhaseConst& P, const Settings& S, double dt,
                                        double f, double& dg, int& iters, double& res_out,
                                        cia_point_failure* rec) {
   const double hs = P.hs;
@@ -203,7 +203,7 @@ __device__ inline int solve_multiplier(const PhaseConst& P, const Settings& S, d
 // Full J2 viscoplastic point update.  Returns a CIA_PT_* status.
 // rate == nullptr disables the increment diagnostics (strain_rates None).
 template <class T, bool EXTRA>
-__device__ inline int j2vp_point(const PhaseConst& P, const Settings& S, double dt, const T eps[6],
+__host__ __device__ inline int j2vp_point(const PhaseConst& P, const Settings& S, double dt, const T eps[6],
                                  const T* rate, const T epsp0[6], T qbar0, T gam0,
                                  PointOut<T>& o, PointExtra<T>* ex, cia_point_failure* rec) {
   T e[6], trial[6], dev[6];
@@ -352,7 +352,7 @@ __device__ inline int j2vp_point(const PhaseConst& P, const Settings& S, double 
 // the elastic-phase diagnostics of phased.py:171-190 (NaN elsewhere, as the
 // NaN-filled union of phased.py:125-134).
 template <class T, bool EXTRA>
-__device__ inline int elastic_point(const PhaseConst& P, double dt, const T eps[6], const T* rate,
+__host__ __device__ inline int elastic_point(const PhaseConst& P, double dt, const T eps[6], const T* rate,
                                     const T epsp0[6], T qbar0, T gam0, PointOut<T>& o,
                                     PointExtra<T>* ex) {
   elastic_stress(P, eps, o.sig);
@@ -396,7 +396,7 @@ __device__ inline int elastic_point(const PhaseConst& P, double dt, const T eps[
 }
 
 template <class T, bool EXTRA>
-__device__ __forceinline__ int material_point(const PhaseConst& P, const Settings& S, double dt,
+__host__ __device__ __forceinline__ int material_point(const PhaseConst& P, const Settings& S, double dt,
                                               const T eps[6], const T* rate, const T epsp0[6],
                                               T qbar0, T gam0, PointOut<T>& o, PointExtra<T>* ex,
                                               cia_point_failure* rec) {

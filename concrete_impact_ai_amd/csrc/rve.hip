@@ -27,8 +27,11 @@
 // block partials.
 #include <string.h>
 
+#include <hipfft/hipfft.h>
+
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <vector>
 
 #include "common.hpp"
@@ -39,8 +42,11 @@ namespace cia {
 struct Scalars {
   double rz, pq, alpha, beta, rr, tol2;
   double norm2[4];
-  int32_t done;  // 0 running, 1 converged, 2 breakdown
+  double sumr[3];  // per-component sum of the PCG residual (spectral preconditioner)
+  int32_t done;    // 0 running, 1 converged, 2 breakdown
   int32_t iters;
+  int32_t first;   // next spectral apply is the initial one (beta = 0)
+  int32_t pad;
 };
 
 struct Red {
@@ -450,7 +456,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // Element part of y = K p with p = z + beta p_old fused in.  p_new is written
 // for the class whose (0,0,0) corner is this element (one writer per class).
-template <class T>
+template <class T, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     k_pcg_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
                const T* __restrict__ tan, const double* __restrict__ z,
@@ -460,6 +466,8 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= G.E) return;
   const double beta = sc->beta;
+  // spectral preconditioner: z is defined up to the shift that pins class 0
+  const double z0[3] = {SHIFT ? z[0] : 0.0, SHIFT ? z[1] : 0.0, SHIFT ? z[2] : 0.0};
   const int64_t np = G.E * 8;
   int64_t c[8];
   int i, j, k;
@@ -469,7 +477,7 @@ __global__ void __launch_bounds__(kBlock)
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      const double v = z[3 * c[a] + m] + beta * p_old[3 * c[a] + m];
+      const double v = (z[3 * c[a] + m] - z0[m]) + beta * p_old[3 * c[a] + m];
       pe[a][m] = (T)v;
       if (a == 0) p_new[3 * e + m] = v;
     }
@@ -595,6 +603,149 @@ __global__ void __launch_bounds__(kBlock)
     sc->iters += 1;
     if (out[1] <= sc->tol2) sc->done = 1;
     else if (!isfinite(out[1]) || !(out[0] > 0.0)) sc->done = 2;
+  }
+}
+
+
+// -------------------------------------------------------------------------
+// Spectral (FFT) reference-medium preconditioner.
+//
+// K0 is the Hex8 stiffness of a homogeneous isotropic reference medium on the
+// periodic grid: a block-circulant operator, diagonalised by the 3-D DFT into
+// 3x3 Hermitian blocks Khat(k).  For the pinned-class system (class 0 fixed)
+// its exact inverse is  z = P Kinv0 (r - (sum r) delta_0),  P = shift so that
+// z(class 0) = 0.  The spectrum of r - (sum r) delta_0 is rhat(k) - sum r, the
+// shift is applied by the consumers (z(c) - z(0)), and r.z is evaluated by
+// Parseval on the half spectrum.
+
+// r = b, x = 0, p_old = 0; r.r and sum(r) -> scalars.
+__global__ void __launch_bounds__(kBlock)
+    k_spec_init(int64_t C, const double* __restrict__ b, double* __restrict__ x,
+                double* __restrict__ r, double* __restrict__ p_old, double rtol, double atol,
+                Scalars* sc, double* partials, unsigned* counter) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (c < C) {
+    double rv[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      rv[m] = b[3 * c + m];
+      x[3 * c + m] = 0.0;
+      r[3 * c + m] = rv[m];
+      p_old[3 * c + m] = 0.0;
+      rr += rv[m] * rv[m];
+    }
+    s0 = rv[0];
+    s1 = rv[1];
+    s2 = rv[2];
+  }
+  const double v[4] = {rr, s0, s1, s2};
+  double out[4];
+  bool last;
+  finalize_sums<4>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  if (last && threadIdx.x == 0) {
+    sc->rr = out[0];
+    sc->sumr[0] = out[1];
+    sc->sumr[1] = out[2];
+    sc->sumr[2] = out[3];
+    const double tol = fmax(rtol * sqrt(out[0]), atol);
+    sc->tol2 = tol * tol;
+    sc->iters = 0;
+    sc->beta = 0.0;
+    sc->first = 1;
+    sc->done = (out[0] <= sc->tol2) ? 1 : 0;
+  }
+}
+
+// rhat (3 x nmodes, component-major) -> zhat = Kinv0 (rhat - sum r) / N in place;
+// r.z by Parseval; beta = rz_new / rz_old (0 on the first apply).
+__global__ void __launch_bounds__(kBlock)
+    k_spec_apply(int64_t nmodes, int nxh, int nx, double inv_n,
+                 const hipDoubleComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ rhat,
+                 Scalars* sc, double* partials, unsigned* counter) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0;
+  if (k < nmodes) {
+    const int kx = (int)(k % nxh);
+    const double w = (kx == 0 || (2 * kx == nx)) ? 1.0 : 2.0;
+    double ar[3], ai[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const hipDoubleComplex v = rhat[m * nmodes + k];
+      ar[m] = v.x - sc->sumr[m];
+      ai[m] = v.y;
+    }
+    // Hermitian inverse stored as (00, 01, 02, 11, 12, 22)
+    const hipDoubleComplex* q = kinv + 6 * k;
+    const double h[3][3][2] = {
+        {{q[0].x, q[0].y}, {q[1].x, q[1].y}, {q[2].x, q[2].y}},
+        {{q[1].x, -q[1].y}, {q[3].x, q[3].y}, {q[4].x, q[4].y}},
+        {{q[2].x, -q[2].y}, {q[4].x, -q[4].y}, {q[5].x, q[5].y}}};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double zr = 0.0, zi = 0.0;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        zr += h[m][l][0] * ar[l] - h[m][l][1] * ai[l];
+        zi += h[m][l][0] * ai[l] + h[m][l][1] * ar[l];
+      }
+      zr *= inv_n;
+      zi *= inv_n;
+      rz += w * (ar[m] * zr + ai[m] * zi);
+      rhat[m * nmodes + k] = make_hipDoubleComplex(zr, zi);
+    }
+  }
+  const double v[1] = {rz};
+  double out[1];
+  bool last;
+  finalize_sums<1>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  if (last && threadIdx.x == 0) {
+    if (sc->first) {
+      sc->beta = 0.0;
+      sc->first = 0;
+    } else {
+      sc->beta = out[0] / sc->rz;
+    }
+    sc->rz = out[0];
+    if (!(out[0] > 0.0) || !isfinite(out[0])) sc->done = 2;
+  }
+}
+
+// x += alpha p, r -= alpha y; r.r and sum(r).
+__global__ void __launch_bounds__(kBlock)
+    k_spec_update(int64_t C, const double* __restrict__ p, const double* __restrict__ y,
+                  double* __restrict__ x, double* __restrict__ r, Scalars* sc, double* partials,
+                  unsigned* counter) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0, s[3] = {0.0, 0.0, 0.0};
+  if (c < C) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      x[3 * c + m] += alpha * p[3 * c + m];
+      const double rv = r[3 * c + m] - alpha * y[3 * c + m];
+      r[3 * c + m] = rv;
+      rr += rv * rv;
+      s[m] = rv;
+    }
+  }
+  const double v[4] = {rr, s[0], s[1], s[2]};
+  double out[4];
+  bool last;
+  finalize_sums<4>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  if (last && threadIdx.x == 0) {
+    sc->rr = out[0];
+    sc->sumr[0] = out[1];
+    sc->sumr[1] = out[2];
+    sc->sumr[2] = out[3];
+    sc->iters += 1;
+    if (out[0] <= sc->tol2) sc->done = 1;
+    else if (!isfinite(out[0])) sc->done = 2;
   }
 }
 
@@ -868,6 +1019,54 @@ __global__ void k_copy_cast(int64_t n, const T* __restrict__ src, double* __rest
   if (t < n) dst[t] = (double)src[t];
 }
 
+
+// -------------------------------------------------------------------------
+// Optional kernel timing with HIP events on the launch stream (used by the
+// benchmark to report the average duration of the dominant kernel).
+namespace prof {
+enum { ASSEMBLE = 0, PCG_ELEM, PCG_NODE, PCG_UPDATE, PRECOND, GATHER, N_IDS };
+struct State {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ids;
+  size_t used = 0;
+  double total_ms[N_IDS] = {0};
+  int64_t count[N_IDS] = {0};
+};
+static State g;
+static void begin(int id, hipStream_t st) {
+  if (!g.on) return;
+  if (g.ev.size() < 2 * (g.used + 1)) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    g.ev.push_back(a);
+    g.ev.push_back(b);
+    g.ids.push_back(0);
+  }
+  g.ids[g.used] = id;
+  (void)hipEventRecord(g.ev[2 * g.used], st);
+}
+static void end(hipStream_t st) {
+  if (!g.on) return;
+  (void)hipEventRecord(g.ev[2 * g.used + 1], st);
+  ++g.used;
+}
+// Call only after the stream has been synchronised.
+static void flush() {
+  if (!g.on) return;
+  for (size_t i = 0; i < g.used; ++i) {
+    float ms = 0.f;
+    if (hipEventSynchronize(g.ev[2 * i + 1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, g.ev[2 * i], g.ev[2 * i + 1]) == hipSuccess) {
+      g.total_ms[g.ids[i]] += ms;
+      g.count[g.ids[i]] += 1;
+    }
+  }
+  g.used = 0;
+}
+}  // namespace prof
+
 }  // namespace cia
 
 // =========================================================================
@@ -924,6 +1123,14 @@ struct cia_rve {
   int pcg_total = 0;
   int cur = 0;
   bool have_fields = false;
+  // spectral preconditioner
+  int prec = CIA_PCG_BLOCK_JACOBI;
+  hipfftHandle fft_fwd = 0, fft_inv = 0;
+  bool fft_ready = false;
+  int64_t nmodes = 0;
+  hipDoubleComplex* kinv = nullptr;
+  hipDoubleComplex* rhat = nullptr;
+  double ref_bulk = 0.0, ref_shear = 0.0;
 };
 
 namespace cia {
@@ -954,6 +1161,7 @@ struct Ops {
                              const double* wc, const Macro& M, const void* state_c, void* state_t,
                              const Settings& S, double dt) {
     const unsigned grid = grid_for(R->E);
+    prof::begin(prof::ASSEMBLE, st);
     if (extra)
       hipLaunchKernelGGL((k_assemble<T, true>), dim3(grid), dim3(kBlock), 0, st, R->G, R->d_tab,
                          R->d_ephase, S, dt, w, wc, M, (const T*)state_c, (T*)state_t,
@@ -965,25 +1173,110 @@ struct Ops {
                          R->d_ephase, S, dt, w, wc, M, (const T*)state_c, (T*)state_t,
                          (T*)R->sig[slot], (T*)R->tan[slot], (T*)R->F, nullptr, nullptr, nullptr,
                          nullptr, nullptr, R->first_fail);
+    prof::end(st);
     CIA_HIP(hipGetLastError());
     return CIA_OK;
   }
 
   static cia_status gather(cia_rve* R, hipStream_t st, double* r, double* class0, int slot_norm) {
+    prof::begin(prof::GATHER, st);
     hipLaunchKernelGGL(k_gather<T>, dim3(grid_for(R->C)), dim3(kBlock), 0, st, R->G,
                        (const T*)R->F, r, class0, R->partials, R->counter,
                        &R->sc->norm2[slot_norm]);
+    prof::end(st);
     CIA_HIP(hipGetLastError());
+    return CIA_OK;
+  }
+
+  // Solve K x = b with the spectral reference-medium preconditioner.
+  static cia_status pcg_spectral(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
+                                 int* iters_out, int* flag_out) {
+    const int64_t C = R->C;
+    const unsigned gn = grid_for(C), ge = grid_for(R->E), gm = grid_for(R->nmodes);
+    const int nxh = R->nx / 2 + 1;
+    const double inv_n = 1.0 / (double)C;
+    if (hipfftSetStream(R->fft_fwd, st) != HIPFFT_SUCCESS ||
+        hipfftSetStream(R->fft_inv, st) != HIPFFT_SUCCESS) {
+      set_error_msg("hipfftSetStream failed");
+      return CIA_ERR_HIP;
+    }
+    auto precondition = [&]() -> cia_status {
+      if (hipfftExecD2Z(R->fft_fwd, (hipfftDoubleReal*)R->pr, (hipfftDoubleComplex*)R->rhat) !=
+          HIPFFT_SUCCESS) {
+        set_error_msg("hipfftExecD2Z failed");
+        return CIA_ERR_HIP;
+      }
+      hipLaunchKernelGGL(k_spec_apply, dim3(gm), dim3(kBlock), 0, st, R->nmodes, nxh, R->nx, inv_n,
+                         (const hipDoubleComplex*)R->kinv, R->rhat, R->sc, R->partials, R->counter);
+      if (hipfftExecZ2D(R->fft_inv, (hipfftDoubleComplex*)R->rhat, (hipfftDoubleReal*)R->pz) !=
+          HIPFFT_SUCCESS) {
+        set_error_msg("hipfftExecZ2D failed");
+        return CIA_ERR_HIP;
+      }
+      return CIA_OK;
+    };
+    prof::begin(prof::PRECOND, st);
+    hipLaunchKernelGGL(k_spec_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1],
+                       R->lin.relative_tolerance, R->lin.absolute_tolerance, R->sc, R->partials,
+                       R->counter);
+    CIA_TRY(precondition());
+    prof::end(st);
+    CIA_HIP(hipGetLastError());
+    int done = 0, it = 0, pi = 0;
+    const int chunk = std::max(1, R->lin.check_interval);
+    Scalars hs;
+    while (it < R->lin.max_iterations) {
+      const int n = std::min(chunk, R->lin.max_iterations - it);
+      for (int s = 0; s < n; ++s) {
+        double* p_old = R->pp[1 - pi];
+        double* p_new = R->pp[pi];
+        prof::begin(prof::PCG_ELEM, st);
+        hipLaunchKernelGGL((k_pcg_elem<T, true>), dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
+                           R->d_ephase, (const T*)R->tan[slot], R->pz, p_old, p_new, (T*)R->F,
+                           R->sc);
+        prof::end(st);
+        prof::begin(prof::PCG_NODE, st);
+        hipLaunchKernelGGL(k_pcg_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F,
+                           p_new, R->py, R->sc, R->partials, R->counter);
+        prof::end(st);
+        prof::begin(prof::PCG_UPDATE, st);
+        hipLaunchKernelGGL(k_spec_update, dim3(gn), dim3(kBlock), 0, st, C, p_new, R->py, x, R->pr,
+                           R->sc, R->partials, R->counter);
+        prof::end(st);
+        prof::begin(prof::PRECOND, st);
+        CIA_TRY(precondition());
+        prof::end(st);
+        pi = 1 - pi;
+      }
+      CIA_HIP(hipGetLastError());
+      it += n;
+      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+      CIA_HIP(hipStreamSynchronize(st));
+      prof::flush();
+      done = hs.done;
+      if (done) break;
+    }
+    if (!done) {
+      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+      CIA_HIP(hipStreamSynchronize(st));
+    }
+    *iters_out = hs.iters;
+    *flag_out = hs.done;
+    R->pcg_total += hs.iters;
     return CIA_OK;
   }
 
   // Solve K x = b (K from tan[slot]); returns iterations.
   static cia_status pcg(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
                         int* iters_out, int* flag_out) {
+    if (R->prec == CIA_PCG_SPECTRAL && R->fft_ready)
+      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out);
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->E);
+    prof::begin(prof::PRECOND, st);
     hipLaunchKernelGGL(k_precond<T>, dim3(gn), dim3(kBlock), 0, st, R->G, R->d_tab, R->d_ephase,
                        (const T*)R->tan[slot], R->minv);
+    prof::end(st);
     hipLaunchKernelGGL(k_pcg_init, dim3(gn), dim3(kBlock), 0, st, C, b, R->minv, x, R->pr, R->pz,
                        R->pp[1], R->lin.relative_tolerance, R->lin.absolute_tolerance, R->sc,
                        R->partials, R->counter);
@@ -998,19 +1291,26 @@ struct Ops {
       for (int s = 0; s < n; ++s) {
         double* p_old = R->pp[1 - pi];
         double* p_new = R->pp[pi];
-        hipLaunchKernelGGL(k_pcg_elem<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
+        prof::begin(prof::PCG_ELEM, st);
+        hipLaunchKernelGGL((k_pcg_elem<T, false>), dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
                            R->d_ephase, (const T*)R->tan[slot], R->pz, p_old, p_new, (T*)R->F,
                            R->sc);
+        prof::end(st);
+        prof::begin(prof::PCG_NODE, st);
         hipLaunchKernelGGL(k_pcg_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F,
                            p_new, R->py, R->sc, R->partials, R->counter);
+        prof::end(st);
+        prof::begin(prof::PCG_UPDATE, st);
         hipLaunchKernelGGL(k_pcg_update, dim3(gn), dim3(kBlock), 0, st, C, R->minv, p_new, R->py,
                            x, R->pr, R->pz, R->sc, R->partials, R->counter);
+        prof::end(st);
         pi = 1 - pi;
       }
       CIA_HIP(hipGetLastError());
       it += n;
       CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
       CIA_HIP(hipStreamSynchronize(st));
+      prof::flush();
       done = hs.done;
       if (done) break;
     }
@@ -1029,6 +1329,7 @@ static cia_status read_norms(cia_rve* R, hipStream_t st, double out[4]) {
   Scalars hs;
   CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
   CIA_HIP(hipStreamSynchronize(st));
+  prof::flush();
   for (int i = 0; i < 4; ++i) out[i] = hs.norm2[i];
   return CIA_OK;
 }
@@ -1047,6 +1348,76 @@ static cia_status check_fail(cia_rve* R, hipStream_t st, unsigned long long* ff)
   CIA_HIP(hipMemcpyAsync(ff, R->first_fail, sizeof(*ff), hipMemcpyDeviceToHost, st));
   CIA_HIP(hipStreamSynchronize(st));
   return CIA_OK;
+}
+
+// Inverse Fourier symbol of the reference-medium Hex8 stiffness on the
+// half spectrum (kx = 0..nx/2 fastest, then ky, kz), Hermitian 3x3 blocks
+// stored as (00, 01, 02, 11, 12, 22); the zero mode maps to 0.
+static void spectral_symbol(int nx, int ny, int nz, const BoxGeom& g, double K0, double G0,
+                            std::vector<hipDoubleComplex>& out) {
+  Tangent<double> t;
+  t.K = K0;
+  t.d = 2.0 * G0;
+  t.r = 0.0;
+  for (int m = 0; m < 6; ++m) t.n[m] = 0.0;
+  double ke[8][3][8][3] = {};
+  for (int q = 0; q < 8; ++q) {
+    for (int b = 0; b < 8; ++b) {
+      double cb[3][6];
+      b_columns(q, b, g, cb);
+      double sb[3][6];
+      for (int m = 0; m < 3; ++m) tangent_apply(t, cb[m], sb[m]);
+      for (int a = 0; a < 8; ++a) {
+        double ca[3][6];
+        b_columns(q, a, g, ca);
+        for (int l = 0; l < 3; ++l)
+          for (int m = 0; m < 3; ++m) {
+            double v = 0.0;
+            for (int z = 0; z < 6; ++z) v += ca[l][z] * sb[m][z];
+            ke[a][l][b][m] += g.jw * v;
+          }
+      }
+    }
+  }
+  double S[3][3][3][3][3] = {};  // [ox+1][oy+1][oz+1][l][m]
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b) {
+      const int ox = (b & 1) - (a & 1), oy = ((b >> 1) & 1) - ((a >> 1) & 1),
+                oz = ((b >> 2) & 1) - ((a >> 2) & 1);
+      for (int l = 0; l < 3; ++l)
+        for (int m = 0; m < 3; ++m) S[ox + 1][oy + 1][oz + 1][l][m] += ke[a][l][b][m];
+    }
+  const int nxh = nx / 2 + 1;
+  out.assign((size_t)6 * nxh * ny * nz, make_hipDoubleComplex(0.0, 0.0));
+  const double two_pi = 6.283185307179586476925286766559;
+  typedef std::complex<double> cd;
+  for (int kz = 0; kz < nz; ++kz)
+    for (int ky = 0; ky < ny; ++ky)
+      for (int kx = 0; kx < nxh; ++kx) {
+        const size_t mode = kx + (size_t)nxh * (ky + (size_t)ny * kz);
+        if (kx == 0 && ky == 0 && kz == 0) continue;
+        const double th[3] = {two_pi * kx / nx, two_pi * ky / ny, two_pi * kz / nz};
+        cd A[3][3] = {};
+        for (int ox = -1; ox <= 1; ++ox)
+          for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+              const cd ph = std::exp(cd(0.0, th[0] * ox + th[1] * oy + th[2] * oz));
+              for (int l = 0; l < 3; ++l)
+                for (int m = 0; m < 3; ++m) A[l][m] += S[ox + 1][oy + 1][oz + 1][l][m] * ph;
+            }
+        const cd c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+        const cd c01 = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+        const cd c02 = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+        const cd det = A[0][0] * c00 + A[1][0] * c01 + A[2][0] * c02;
+        const cd inv[6] = {c00 / det,
+                           c01 / det,
+                           c02 / det,
+                           (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det,
+                           (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det,
+                           (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det};
+        for (int u = 0; u < 6; ++u)
+          out[6 * mode + u] = make_hipDoubleComplex(inv[u].real(), inv[u].imag());
+      }
 }
 
 template <class T>
@@ -1428,6 +1799,41 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
   R->red_len = (int64_t)R_N * grid_for(R->E) + 2 * grid_for(R->E) + (int64_t)N_N * grid_for(R->N) +
                grid_for(R->N) + grid_for(R->C) + 64;
   ALLOC(R->red, R->red_len * 8);
+  R->prec = d->linear.preconditioner;
+  if (R->prec == CIA_PCG_SPECTRAL && R->C > 1) {
+    double kmin = INFINITY, kmax = 0.0, gmin = INFINITY, gmax = 0.0;
+    for (int i = 0; i < d->n_phases; ++i) {
+      kmin = std::min(kmin, tab.p[i].K);
+      kmax = std::max(kmax, tab.p[i].K);
+      gmin = std::min(gmin, tab.p[i].G);
+      gmax = std::max(gmax, tab.p[i].G);
+    }
+    R->ref_bulk = d->linear.reference_bulk > 0 ? d->linear.reference_bulk : std::sqrt(kmin * kmax);
+    R->ref_shear = d->linear.reference_shear > 0 ? d->linear.reference_shear : std::sqrt(gmin * gmax);
+    R->nmodes = (int64_t)(R->nx / 2 + 1) * R->ny * R->nz;
+    ALLOC(R->kinv, 6 * R->nmodes * sizeof(hipDoubleComplex));
+    ALLOC(R->rhat, 3 * R->nmodes * sizeof(hipDoubleComplex));
+    std::vector<hipDoubleComplex> sym;
+    spectral_symbol(R->nx, R->ny, R->nz, R->G.g, R->ref_bulk, R->ref_shear, sym);
+    if (hipMemcpy(R->kinv, sym.data(), sym.size() * sizeof(hipDoubleComplex),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      cia_rve_destroy(R);
+      set_error_msg("hipMemcpy of the spectral symbol failed");
+      return CIA_ERR_HIP;
+    }
+    int dims[3] = {R->nz, R->ny, R->nx};
+    int rem[3] = {R->nz, R->ny, R->nx};
+    int cem[3] = {R->nz, R->ny, R->nx / 2 + 1};
+    if (hipfftPlanMany(&R->fft_fwd, 3, dims, rem, 3, 1, cem, 1, (int)R->nmodes, HIPFFT_D2Z, 3) !=
+            HIPFFT_SUCCESS ||
+        hipfftPlanMany(&R->fft_inv, 3, dims, cem, 1, (int)R->nmodes, rem, 3, 1, HIPFFT_Z2D, 3) !=
+            HIPFFT_SUCCESS) {
+      cia_rve_destroy(R);
+      set_error_msg("hipfftPlanMany failed");
+      return CIA_ERR_HIP;
+    }
+    R->fft_ready = true;
+  }
 #undef ALLOC
   if (hipMemcpy(R->d_tab, &tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(R->d_ephase, eph.data(), R->E, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1448,6 +1854,10 @@ cia_status cia_rve_destroy(cia_rve* R) {
                   R->counter, R->sc, R->class0, R->first_fail, R->rec, R->red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (R->kinv) (void)hipFree(R->kinv);
+  if (R->rhat) (void)hipFree(R->rhat);
+  if (R->fft_fwd) (void)hipfftDestroy(R->fft_fwd);
+  if (R->fft_inv) (void)hipfftDestroy(R->fft_inv);
   delete R;
   return CIA_OK;
 }
@@ -1481,6 +1891,25 @@ cia_status cia_rve_solve(cia_rve* R, void* stream, const void* committed_state,
                               time_step, S, need_tangent, next_state, next_w, result);
   return solve_impl<float>(R, st, committed_state, committed_macro, committed_w, macro_strain,
                            time_step, S, need_tangent, next_state, next_w, result);
+}
+
+cia_status cia_profile_enable(int32_t on) {
+  cia::prof::g.on = on != 0;
+  return CIA_OK;
+}
+
+cia_status cia_profile_read(double* total_ms, int64_t* counts, int32_t n) {
+  using namespace cia;
+  CIA_ARG(total_ms && counts && n >= 0, "NULL argument");
+  CIA_HIP(hipDeviceSynchronize());
+  prof::flush();
+  for (int i = 0; i < n && i < prof::N_IDS; ++i) {
+    total_ms[i] = prof::g.total_ms[i];
+    counts[i] = prof::g.count[i];
+    prof::g.total_ms[i] = 0.0;
+    prof::g.count[i] = 0;
+  }
+  return CIA_OK;
 }
 
 cia_status cia_rve_copy_field(cia_rve* R, void* stream, int32_t field, double* dst) {

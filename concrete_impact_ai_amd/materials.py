@@ -355,6 +355,17 @@ def update_device(
     dtype = strain.dtype
     n = strain.shape[1]
     dev = strain.device
+    if strain.shape[0] != 6 or not strain.is_contiguous():
+        raise ValueError("strain must be a contiguous [6][n] tensor")
+    if state_in is not None and (state_in.shape != (8, n) or state_in.dtype != dtype):
+        raise ValueError("state must be an [8][n] tensor of the strain dtype")
+    if strain_rate is not None and strain_rate.shape != (6, n):
+        raise ValueError("strain_rate must be a [6][n] tensor")
+    if phase_id is not None:
+        if phase_id.dtype != torch.int32 or phase_id.shape != (n,):
+            raise ValueError("phase_id must be an int32 [n] tensor")
+        if n and (int(phase_id.min()) < 0 or int(phase_id.max()) >= len(phases)):
+            raise ValueError("phase_id out of range of the phase table")
     mats = (_lib.Material * len(phases))(*[cia_material(m) for m in phases])
     flags = (
         (_lib.CIA_OUT_TANGENT if tangent else 0)

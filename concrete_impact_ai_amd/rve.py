@@ -339,8 +339,13 @@ class RVEOperatorCache:
             float(ns.armijo.sufficient_decrease), float(ns.armijo.reduction_factor),
             int(ns.armijo.max_backtracks), 0,
         )
-        desc.linear = _lib.LinearSettings(int(ls.max_iterations), int(ls.check_interval),
-                                          float(ls.relative_tolerance), float(ls.absolute_tolerance))
+        if ls.method not in ("spectral", "block_jacobi"):
+            raise ValueError(f"unknown PCG preconditioner {ls.method!r}")
+        desc.linear = _lib.LinearSettings(
+            int(ls.max_iterations), int(ls.check_interval), float(ls.relative_tolerance),
+            float(ls.absolute_tolerance),
+            _lib.CIA_PCG_SPECTRAL if ls.method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI, 0,
+            float(ls.reference_bulk), float(ls.reference_shear))
         desc.hill_mandel_power_scale = float(model.hill_mandel_power_scale)
         desc.precision = _lib.CIA_PREC_FP64 if model.precision == "fp64" else _lib.CIA_PREC_FP32_REFINED
         device()

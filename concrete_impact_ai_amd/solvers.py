@@ -69,12 +69,14 @@ class LinearSolverSettings:
     residual well below the Newton tolerances of the reference configs.
     """
 
-    backend: str = "pcg_block_jacobi"
-    method: str = "pcg"
+    backend: str = "pcg"
+    method: str = "spectral"  # "spectral" (FFT reference medium) or "block_jacobi"
     max_iterations: int = 20000
     relative_tolerance: float = 1.0e-12
     absolute_tolerance: float = 0.0
-    check_interval: int = 16
+    check_interval: int = 8
+    reference_bulk: float = 0.0  # <= 0: geometric mean over the phases
+    reference_shear: float = 0.0
 
 
 @dataclass(frozen=True)

@@ -210,7 +210,16 @@ typedef struct {
   int32_t check_interval;   /* iterations between host convergence checks */
   double relative_tolerance; /* ||r_k|| <= rtol * ||b|| */
   double absolute_tolerance;
+  int32_t preconditioner;   /* CIA_PCG_* */
+  int32_t pad;
+  double reference_bulk;    /* spectral reference medium (<= 0: geometric mean of phases) */
+  double reference_shear;
 } cia_linear_settings;
+
+/* PCG preconditioners.  CIA_PCG_SPECTRAL is the exact inverse of the Hex8
+ * stiffness of a homogeneous reference medium on the periodic grid, applied
+ * with 3-D FFTs (hipFFT); CIA_PCG_BLOCK_JACOBI inverts the 3x3 nodal blocks. */
+enum { CIA_PCG_SPECTRAL = 0, CIA_PCG_BLOCK_JACOBI = 1 };
 
 enum { CIA_PREC_FP64 = 0, CIA_PREC_FP32_REFINED = 1 };
 
@@ -331,6 +340,13 @@ enum {
   CIA_FIELD_INTERNAL_FORCE = 7     /* [3 * n_nodes] full node-major internal force */
 };
 cia_status cia_rve_copy_field(cia_rve* rve, void* stream, int32_t field, double* dst);
+
+/* Kernel timing with HIP events on the launch stream (no reference
+ * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element
+ * pass, 2 PCG node gather, 3 PCG vector update, 4 preconditioner (spectral: FFT,
+ * symbol solve, inverse FFT; block Jacobi: block inversion), 5 residual gather.  cia_profile_read returns and clears the accumulated totals. */
+cia_status cia_profile_enable(int32_t on);
+cia_status cia_profile_read(double* total_ms, int64_t* counts, int32_t n);
 
 #ifdef __cplusplus
 }

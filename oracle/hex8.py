@@ -121,8 +121,8 @@ def strain_fields(elements, b, u, u_prev, dt):
     dofs = element_dofs(elements)
     ue = u[dofs]
     upe = u_prev[dofs]
-    eps = np.einsum("eqij,ej->eqi", b, ue).reshape(-1, 6)
-    rate = (np.einsum("eqij,ej->eqi", b, ue - upe) / dt).reshape(-1, 6)
+    eps = np.matmul(b, ue[:, None, :, None])[..., 0].reshape(-1, 6)
+    rate = (np.matmul(b, (ue - upe)[:, None, :, None])[..., 0] / dt).reshape(-1, 6)
     return eps, rate
 
 
@@ -130,7 +130,7 @@ def internal_force(elements, b, jw, stress, n_dof):
     """Force part of ``_assemble_force_and_tangent`` (element-ordered accumulation)."""
     ne, nq = jw.shape
     s = stress.reshape(ne, nq, 6)
-    fe = np.einsum("eqij,eqi,eq->ej", b, s, jw)
+    fe = np.sum(np.matmul(np.swapaxes(b, -1, -2), (s * jw[..., None])[..., None])[..., 0], axis=1)
     f = np.zeros(n_dof)
     np.add.at(f, element_dofs(elements).ravel(), fe.ravel())
     return f
@@ -140,7 +140,8 @@ def tangent_matrix(elements, b, jw, tangents, n_dof):
     """Tangent part of ``_assemble_force_and_tangent``: COO list -> CSR."""
     ne, nq = jw.shape
     c = tangents.reshape(ne, nq, 6, 6)
-    ke = np.einsum("eqia,eqij,eqjb,eq->eab", b, c, b, jw)
+    bt = np.swapaxes(b, -1, -2)
+    ke = np.einsum("eqab,eq->eab", np.matmul(np.matmul(bt, c), b), jw)
     dofs = element_dofs(elements)
     rows = np.repeat(dofs[:, :, None], 24, axis=2).ravel()
     cols = np.repeat(dofs[:, None, :], 24, axis=1).ravel()

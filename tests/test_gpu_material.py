@@ -26,7 +26,7 @@ def _materials(m, tau=0.02):
     return gpu, orc
 
 
-def _random_inputs(n, seed, scale=0.01):
+def _random_inputs(n, seed, scale=0.002):
     rng = np.random.default_rng(seed)
     strain = rng.normal(scale=scale, size=(n, 6))
     ep = rng.normal(scale=scale * 0.3, size=(n, 6))
