@@ -355,12 +355,15 @@ def update_device(
     dtype = strain.dtype
     n = strain.shape[1]
     dev = strain.device
+    # The C ABI reads raw row-major SoA buffers: every tensor must be contiguous.
     if strain.shape[0] != 6 or not strain.is_contiguous():
         raise ValueError("strain must be a contiguous [6][n] tensor")
-    if state_in is not None and (state_in.shape != (8, n) or state_in.dtype != dtype):
-        raise ValueError("state must be an [8][n] tensor of the strain dtype")
-    if strain_rate is not None and strain_rate.shape != (6, n):
-        raise ValueError("strain_rate must be a [6][n] tensor")
+    if state_in is not None and (state_in.shape != (8, n) or state_in.dtype != dtype
+                                 or not state_in.is_contiguous()):
+        raise ValueError("state must be a contiguous [8][n] tensor of the strain dtype")
+    if strain_rate is not None and (strain_rate.shape != (6, n) or strain_rate.dtype != dtype
+                                    or not strain_rate.is_contiguous()):
+        raise ValueError("strain_rate must be a contiguous [6][n] tensor of the strain dtype")
     if phase_id is not None:
         if phase_id.dtype != torch.int32 or phase_id.shape != (n,):
             raise ValueError("phase_id must be an int32 [n] tensor")

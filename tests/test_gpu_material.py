@@ -194,7 +194,7 @@ def test_fp32_variant_error_against_fp64(gpu):
     gmat, omat = _materials(1.0)
     n = 100_000
     strain, ep, q, g, _ = _random_inputs(n, 11)
-    st = np.vstack([ep.T, q[None], g[None]])
+    st = np.ascontiguousarray(np.vstack([ep.T, q[None], g[None]]))
     dev = gpu
     s32, st32, *_ = M.update_device([gmat], None, torch.tensor(strain.T.copy(), device=dev, dtype=torch.float32),
                                     None, torch.tensor(st, device=dev, dtype=torch.float32), 1e-3,
@@ -204,3 +204,19 @@ def test_fp32_variant_error_against_fp64(gpu):
                          g.astype(np.float32).astype(np.float64), 1e-3, OJ.UpdateSettings(), None)
     err = np.max(np.abs(s32.double().cpu().numpy().T - ref["stress"])) / np.max(np.abs(ref["stress"]))
     assert err < 2e-6, err
+
+
+def test_device_api_rejects_non_contiguous_state(gpu):
+    """An F-ordered [8][n] view must be refused, not read as row-major."""
+    import torch
+
+    from concrete_impact_ai_amd import materials as M
+
+    gmat, _ = _materials(1.0)
+    strain, ep, q, g, _ = _random_inputs(64, 1)
+    st_f = np.vstack([ep.T, q[None], g[None]])  # vstack of a transposed array keeps F order
+    assert not st_f.flags.c_contiguous
+    with pytest.raises(ValueError):
+        M.update_device([gmat], None, torch.as_tensor(np.ascontiguousarray(strain.T), device=gpu), None,
+                        torch.as_tensor(st_f, device=gpu), 1e-3,
+                        M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10))
