@@ -85,6 +85,96 @@ __host__ __device__ __forceinline__ void b_columns(int q, int a, const BoxGeom& 
   col[2][0] = 0;  col[2][1] = 0;  col[2][2] = bz; col[2][3] = by; col[2][4] = bx; col[2][5] = 0;
 }
 
+// Gradient factors of the 8 corner shape functions at a runtime Gauss point q
+// (already scaled by 1/h): dN_a/dx = bx[a], dN_a/dy = by[a], dN_a/dz = bz[a].
+template <class T>
+__device__ __forceinline__ void gp_factors(int q, const BoxGeom& g, T bx[8], T by[8], T bz[8]) {
+  const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+  const T wx[2] = {(T)(qx == 0 ? kWp : kWm), (T)(qx == 1 ? kWp : kWm)};
+  const T wy[2] = {(T)(qy == 0 ? kWp : kWm), (T)(qy == 1 ? kWp : kWm)};
+  const T wz[2] = {(T)(qz == 0 ? kWp : kWm), (T)(qz == 1 ? kWp : kWm)};
+  const T ix = (T)g.ihx, iy = (T)g.ihy, iz = (T)g.ihz;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+    bx[a] = (ax ? ix : -ix) * wy[ay] * wz[az];
+    by[a] = (ay ? iy : -iy) * wx[ax] * wz[az];
+    bz[a] = (az ? iz : -iz) * wx[ax] * wy[ay];
+  }
+}
+
+// Engineering strain at the point whose factors are given.
+template <class T, class U>
+__device__ __forceinline__ void strain_from(const U u[8][3], const T bx[8], const T by[8],
+                                            const T bz[8], T e[6]) {
+  T g[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // g[c][d] = du_c / dx_d
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v = (T)u[a][c];
+      g[c][0] += bx[a] * v;
+      g[c][1] += by[a] * v;
+      g[c][2] += bz[a] * v;
+    }
+  }
+  e[0] = g[0][0];
+  e[1] = g[1][1];
+  e[2] = g[2][2];
+  e[3] = g[1][2] + g[2][1];
+  e[4] = g[0][2] + g[2][0];
+  e[5] = g[0][1] + g[1][0];
+}
+
+// f[a][c] = w detJ (B^T sigma)[a][c] at one point.
+template <class T>
+__device__ __forceinline__ void force_from(const T s[6], T jw, const T bx[8], const T by[8],
+                                           const T bz[8], T f[8][3]) {
+  const T s0 = jw * s[0], s1 = jw * s[1], s2 = jw * s[2], s3 = jw * s[3], s4 = jw * s[4],
+          s5 = jw * s[5];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    f[a][0] = bx[a] * s0 + by[a] * s5 + bz[a] * s4;
+    f[a][1] = by[a] * s1 + bz[a] * s3 + bx[a] * s5;
+    f[a][2] = bz[a] * s2 + by[a] * s3 + bx[a] * s4;
+  }
+}
+
+// Sum f over the 8 lanes of an element (lanes q = 0..7 of an aligned 8-lane
+// group, one per Gauss point) by recursive halving: 12 + 6 + 3 shuffles, after
+// which lane q holds the element force at its corner q.  Fixed order, so the
+// result is bitwise reproducible.
+template <class T>
+__device__ __forceinline__ void element_reduce8(int q, const T f[8][3], T out[3]) {
+  T h1[4][3];
+  const bool b4 = (q & 4) != 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const T send = b4 ? f[j][m] : f[4 + j][m];
+      const T recv = __shfl_xor(send, 4, 8);
+      h1[j][m] = (b4 ? f[4 + j][m] : f[j][m]) + recv;
+    }
+  T h2[2][3];
+  const bool b2 = (q & 2) != 0;
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const T send = b2 ? h1[j][m] : h1[2 + j][m];
+      const T recv = __shfl_xor(send, 2, 8);
+      h2[j][m] = (b2 ? h1[2 + j][m] : h1[j][m]) + recv;
+    }
+  const bool b1 = (q & 1) != 0;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const T send = b1 ? h2[0][m] : h2[1][m];
+    const T recv = __shfl_xor(send, 1, 8);
+    out[m] = (b1 ? h2[1][m] : h2[0][m]) + recv;
+  }
+}
+
 // Element (i, j, k) of a periodic nx x ny x nz grid: node classes of its 8 corners.
 __device__ __forceinline__ void periodic_corners(int64_t e, int nx, int ny, int nz, int64_t c[8],
                                                  int& i, int& j, int& k) {

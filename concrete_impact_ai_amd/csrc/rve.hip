@@ -99,7 +99,9 @@ struct Macro {
 };
 
 // -------------------------------------------------------------------------
-// Assembly: strain -> constitutive update -> element forces.
+// Assembly: strain -> constitutive update -> element forces.  One lane per
+// Gauss point (8 lanes per element): every per-point row is read and written
+// by consecutive lanes, and the element force is reduced across the 8 lanes.
 template <class T, bool EXTRA>
 __global__ void __launch_bounds__(kBlock)
     k_assemble(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
@@ -109,118 +111,87 @@ __global__ void __launch_bounds__(kBlock)
                T* __restrict__ strain_out, T* __restrict__ rate_out, T* __restrict__ psi_out,
                T* __restrict__ diss_out, T* __restrict__ diag_out,
                unsigned long long* __restrict__ first_fail) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= G.E) return;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  if (e >= G.E) return;  // whole 8-lane groups leave together
   const int64_t np = G.E * 8;
   int64_t c[8];
   int i, j, k;
   periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
-  T we[8][3], dwe[8][3];
+  T bx[8], by[8], bz[8];
+  gp_factors<T>(q, G.g, bx, by, bz);
+  T eps[6], rate[6];
+  {
+    double we[8][3];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double v = w[3 * c[a] + m];
-      we[a][m] = (T)v;
-      if (EXTRA) dwe[a][m] = (T)(v - wc[3 * c[a] + m]);
-    }
-  const PhaseConst P = tab->p[ephase[e]];
-  const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
-  T f[8][3];
-#pragma unroll
-  for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = (T)0;
-  int status = CIA_PT_OK;
-  int bad_q = 0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int64_t p = 8 * e + q;
-    T eps[6], rate[6];
-    switch (q) {
-      case 0: strain_at<0>(we, G.g, eps); break;
-      case 1: strain_at<1>(we, G.g, eps); break;
-      case 2: strain_at<2>(we, G.g, eps); break;
-      case 3: strain_at<3>(we, G.g, eps); break;
-      case 4: strain_at<4>(we, G.g, eps); break;
-      case 5: strain_at<5>(we, G.g, eps); break;
-      case 6: strain_at<6>(we, G.g, eps); break;
-      default: strain_at<7>(we, G.g, eps); break;
-    }
-#pragma unroll
-    for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
-    if (EXTRA) {
-      switch (q) {
-        case 0: strain_at<0>(dwe, G.g, rate); break;
-        case 1: strain_at<1>(dwe, G.g, rate); break;
-        case 2: strain_at<2>(dwe, G.g, rate); break;
-        case 3: strain_at<3>(dwe, G.g, rate); break;
-        case 4: strain_at<4>(dwe, G.g, rate); break;
-        case 5: strain_at<5>(dwe, G.g, rate); break;
-        case 6: strain_at<6>(dwe, G.g, rate); break;
-        default: strain_at<7>(dwe, G.g, rate); break;
-      }
-#pragma unroll
-      for (int m = 0; m < 6; ++m) rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
-    }
-    T epsp[6];
-    T qb = (T)0, ga = (T)0;
-    if (j2) {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * np + p];
-      qb = state_c[6 * np + p];
-      ga = state_c[7 * np + p];
-    } else {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
-    }
-    PointOut<T> o;
-    PointExtra<T> ex;
-    const int st = material_point<T, EXTRA>(P, S, dt, eps, EXTRA ? rate : nullptr, epsp, qb, ga,
-                                            o, EXTRA ? &ex : nullptr, nullptr);
-    if (st != CIA_PT_OK && status == CIA_PT_OK) {
-      status = st;
-      bad_q = q;
-    }
-#pragma unroll
-    for (int m = 0; m < 6; ++m) sig[m * np + p] = o.sig[m];
-    if (j2) {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) state_t[m * np + p] = o.epsp[m];
-      state_t[6 * np + p] = o.qbar;
-      state_t[7 * np + p] = o.gam;
-    }
-    tan[0 * np + p] = o.tan.d;
-    tan[1 * np + p] = o.tan.r;
-    if (o.tan.r != (T)0) {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) tan[(2 + m) * np + p] = o.tan.n[m];
-    }
+      for (int m = 0; m < 3; ++m) we[a][m] = w[3 * c[a] + m];
+    strain_from(we, bx, by, bz, eps);
     if (EXTRA) {
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        strain_out[m * np + p] = eps[m];
-        rate_out[m * np + p] = rate[m];
-      }
-      psi_out[p] = ex.psi;
-      diss_out[p] = ex.diss;
+      for (int a = 0; a < 8; ++a)
 #pragma unroll
-      for (int m = 0; m < CIA_N_DIAG; ++m) diag_out[m * np + p] = ex.d[m];
-    }
-    switch (q) {
-      case 0: force_at<0>(o.sig, G.g, f); break;
-      case 1: force_at<1>(o.sig, G.g, f); break;
-      case 2: force_at<2>(o.sig, G.g, f); break;
-      case 3: force_at<3>(o.sig, G.g, f); break;
-      case 4: force_at<4>(o.sig, G.g, f); break;
-      case 5: force_at<5>(o.sig, G.g, f); break;
-      case 6: force_at<6>(o.sig, G.g, f); break;
-      default: force_at<7>(o.sig, G.g, f); break;
+        for (int m = 0; m < 3; ++m) we[a][m] -= wc[3 * c[a] + m];
+      strain_from(we, bx, by, bz, rate);
     }
   }
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
+  if (EXTRA) {
 #pragma unroll
-    for (int m = 0; m < 3; ++m) F[(3 * a + m) * G.E + e] = f[a][m];
-  if (status != CIA_PT_OK) atomicMin(first_fail, (unsigned long long)(8 * e + bad_q));
+    for (int m = 0; m < 6; ++m) rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
+  }
+  const PhaseConst P = tab->p[ephase[e]];
+  const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
+  T epsp[6];
+  T qb = (T)0, ga = (T)0;
+  if (j2) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * np + p];
+    qb = state_c[6 * np + p];
+    ga = state_c[7 * np + p];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
+  }
+  PointOut<T> o;
+  PointExtra<T> ex;
+  const int st = material_point<T, EXTRA>(P, S, dt, eps, EXTRA ? rate : nullptr, epsp, qb, ga, o,
+                                          EXTRA ? &ex : nullptr, nullptr);
+  if (st != CIA_PT_OK) atomicMin(first_fail, (unsigned long long)p);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) sig[m * np + p] = o.sig[m];
+  if (j2) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) state_t[m * np + p] = o.epsp[m];
+    state_t[6 * np + p] = o.qbar;
+    state_t[7 * np + p] = o.gam;
+  }
+  tan[0 * np + p] = o.tan.d;
+  tan[1 * np + p] = o.tan.r;
+  if (o.tan.r != (T)0) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) tan[(2 + m) * np + p] = o.tan.n[m];
+  }
+  if (EXTRA) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      strain_out[m * np + p] = eps[m];
+      rate_out[m * np + p] = rate[m];
+    }
+    psi_out[p] = ex.psi;
+    diss_out[p] = ex.diss;
+#pragma unroll
+    for (int m = 0; m < CIA_N_DIAG; ++m) diag_out[m * np + p] = ex.d[m];
+  }
+  T f[8][3];
+  force_from(o.sig, (T)G.g.jw, bx, by, bz, f);
+  T fin[3];
+  element_reduce8(q, f, fin);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
 }
 
 // Failure record of point p (re-evaluated with the record attached).
@@ -454,8 +425,9 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Element part of y = K p with p = z + beta p_old fused in.  p_new is written
-// for the class whose (0,0,0) corner is this element (one writer per class).
+// Element part of y = K p with p = z + beta p_old fused in.  One lane per
+// Gauss point; p_new is written for the class whose (0,0,0) corner is this
+// element (lane 0 of the group: one writer per class).
 template <class T, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     k_pcg_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
@@ -463,68 +435,69 @@ __global__ void __launch_bounds__(kBlock)
                const double* __restrict__ p_old, double* __restrict__ p_new,
                T* __restrict__ F, const Scalars* __restrict__ sc) {
   if (sc->done) return;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
   if (e >= G.E) return;
   const double beta = sc->beta;
-  // spectral preconditioner: z is defined up to the shift that pins class 0
-  const double z0[3] = {SHIFT ? z[0] : 0.0, SHIFT ? z[1] : 0.0, SHIFT ? z[2] : 0.0};
   const int64_t np = G.E * 8;
+  // Spectral preconditioner: z arrives packed as complex (z_x + i z_y | z_z + i 0)
+  // and is defined up to the translation that pins class 0.
+  const int64_t C = G.E;
+  const hipDoubleComplex* zc = reinterpret_cast<const hipDoubleComplex*>(z);
+  double z0[3] = {0.0, 0.0, 0.0};
+  if (SHIFT) {
+    z0[0] = zc[0].x;
+    z0[1] = zc[0].y;
+    z0[2] = zc[C].x;
+  }
   int64_t c[8];
   int i, j, k;
   periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
-  T pe[8][3];
+  T eps[6];
+  T bx[8], by[8], bz[8];
+  gp_factors<T>(q, G.g, bx, by, bz);
+  {
+    double pe[8][3];
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < 8; ++a) {
+      double zv[3];
+      if (SHIFT) {
+        const hipDoubleComplex t01 = zc[c[a]], t2 = zc[C + c[a]];
+        zv[0] = t01.x - z0[0];
+        zv[1] = t01.y - z0[1];
+        zv[2] = t2.x - z0[2];
+      } else {
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const double v = (z[3 * c[a] + m] - z0[m]) + beta * p_old[3 * c[a] + m];
-      pe[a][m] = (T)v;
-      if (a == 0) p_new[3 * e + m] = v;
+        for (int m = 0; m < 3; ++m) zv[m] = z[3 * c[a] + m];
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) pe[a][m] = zv[m] + beta * p_old[3 * c[a] + m];
     }
+    if (q == 0) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) p_new[3 * e + m] = pe[0][m];
+    }
+    strain_from(pe, bx, by, bz, eps);
+  }
   Tangent<T> tc;
   tc.K = (T)tab->p[ephase[e]].K;
-  T f[8][3];
+  tc.d = tan[p];
+  tc.r = tan[np + p];
 #pragma unroll
-  for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = (T)0;
+  for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
+  if (tc.r != (T)0) {
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int64_t p = 8 * e + q;
-    tc.d = tan[p];
-    tc.r = tan[np + p];
-    if (tc.r != (T)0) {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * np + p];
-    } else {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
-    }
-    T eps[6], s[6];
-    switch (q) {
-      case 0: strain_at<0>(pe, G.g, eps); break;
-      case 1: strain_at<1>(pe, G.g, eps); break;
-      case 2: strain_at<2>(pe, G.g, eps); break;
-      case 3: strain_at<3>(pe, G.g, eps); break;
-      case 4: strain_at<4>(pe, G.g, eps); break;
-      case 5: strain_at<5>(pe, G.g, eps); break;
-      case 6: strain_at<6>(pe, G.g, eps); break;
-      default: strain_at<7>(pe, G.g, eps); break;
-    }
-    tangent_apply(tc, eps, s);
-    switch (q) {
-      case 0: force_at<0>(s, G.g, f); break;
-      case 1: force_at<1>(s, G.g, f); break;
-      case 2: force_at<2>(s, G.g, f); break;
-      case 3: force_at<3>(s, G.g, f); break;
-      case 4: force_at<4>(s, G.g, f); break;
-      case 5: force_at<5>(s, G.g, f); break;
-      case 6: force_at<6>(s, G.g, f); break;
-      default: force_at<7>(s, G.g, f); break;
-    }
+    for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * np + p];
   }
+  T s[6];
+  tangent_apply(tc, eps, s);
+  T f[8][3];
+  force_from(s, (T)G.g.jw, bx, by, bz, f);
+  T fin[3];
+  element_reduce8(q, f, fin);
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int m = 0; m < 3; ++m) F[(3 * a + m) * G.E + e] = f[a][m];
+  for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
 }
 
 // y = gather(F) (class 0 zeroed), p.y -> alpha.
@@ -621,8 +594,9 @@ __global__ void __launch_bounds__(kBlock)
 // r = b, x = 0, p_old = 0; r.r and sum(r) -> scalars.
 __global__ void __launch_bounds__(kBlock)
     k_spec_init(int64_t C, const double* __restrict__ b, double* __restrict__ x,
-                double* __restrict__ r, double* __restrict__ p_old, double rtol, double atol,
-                Scalars* sc, double* partials, unsigned* counter) {
+                double* __restrict__ r, double* __restrict__ p_old,
+                hipDoubleComplex* __restrict__ uab, double rtol, double atol, Scalars* sc,
+                double* partials, unsigned* counter) {
   __shared__ double lds[kBlock / 64];
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -639,6 +613,8 @@ __global__ void __launch_bounds__(kBlock)
     s0 = rv[0];
     s1 = rv[1];
     s2 = rv[2];
+    uab[c] = make_hipDoubleComplex(rv[0], rv[1]);
+    uab[C + c] = make_hipDoubleComplex(rv[2], 0.0);
   }
   const double v[4] = {rr, s0, s1, s2};
   double out[4];
@@ -658,44 +634,75 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// rhat (3 x nmodes, component-major) -> zhat = Kinv0 (rhat - sum r) / N in place;
-// r.z by Parseval; beta = rz_new / rz_old (0 on the first apply).
+// Packed spectra: ua = FFT(r_x + i r_y), ub = FFT(r_z) over the full grid of N
+// modes.  For each pair (k, -k) one thread splits the component spectra
+// (Rx = (ua(k) + conj ua(-k)) / 2, Ry = (ua(k) - conj ua(-k)) / 2i, Rz = ub(k)),
+// removes sum(r) (the reaction at the pinned class), applies Khat^-1 / N and
+// writes va = Zx + i Zy, vb = Zz in place, so that the inverse transforms give
+// z_x + i z_y and z_z.  r.z by Parseval over the full spectrum.
+__device__ __forceinline__ int64_t neg_mode(int64_t k, int nx, int ny, int nz) {
+  const int kx = (int)(k % nx), ky = (int)((k / nx) % ny), kz = (int)(k / ((int64_t)nx * ny));
+  const int mx = kx ? nx - kx : 0, my = ky ? ny - ky : 0, mz = kz ? nz - kz : 0;
+  return mx + (int64_t)nx * (my + (int64_t)ny * mz);
+}
+
+__device__ __forceinline__ void spec_solve(const hipFloatComplex* __restrict__ q, bool conj_sym,
+                                           const double ar[3], const double ai[3], double zr[3],
+                                           double zi[3]) {
+  // Hermitian inverse stored as (00, 01, 02, 11, 12, 22); K^-1(-k) = conj K^-1(k).
+  const double s = conj_sym ? -1.0 : 1.0;
+  const double h[3][3][2] = {
+      {{q[0].x, s * q[0].y}, {q[1].x, s * q[1].y}, {q[2].x, s * q[2].y}},
+      {{q[1].x, -s * q[1].y}, {q[3].x, s * q[3].y}, {q[4].x, s * q[4].y}},
+      {{q[2].x, -s * q[2].y}, {q[4].x, -s * q[4].y}, {q[5].x, s * q[5].y}}};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double xr = 0.0, xi = 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      xr += h[m][l][0] * ar[l] - h[m][l][1] * ai[l];
+      xi += h[m][l][0] * ai[l] + h[m][l][1] * ar[l];
+    }
+    zr[m] = xr;
+    zi[m] = xi;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
-    k_spec_apply(int64_t nmodes, int nxh, int nx, double inv_n,
-                 const hipDoubleComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ rhat,
-                 Scalars* sc, double* partials, unsigned* counter) {
+    k_spec_apply(int64_t N, int nx, int ny, int nz, double inv_n,
+                 const hipFloatComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ ua,
+                 hipDoubleComplex* __restrict__ ub, Scalars* sc, double* partials,
+                 unsigned* counter) {
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0;
-  if (k < nmodes) {
-    const int kx = (int)(k % nxh);
-    const double w = (kx == 0 || (2 * kx == nx)) ? 1.0 : 2.0;
-    double ar[3], ai[3];
+  if (k < N) {
+    const int64_t mk = neg_mode(k, nx, ny, nz);
+    if (k <= mk) {
+      const hipDoubleComplex a = ua[k], am = ua[mk], b = ub[k], bm = ub[mk];
+      const double s0 = sc->sumr[0], s1 = sc->sumr[1], s2 = sc->sumr[2];
+      // component spectra at k and -k (minus the pinned-class reaction sum r)
+      double ar[3] = {0.5 * (a.x + am.x) - s0, 0.5 * (a.y + am.y) - s1, b.x - s2};
+      double ai[3] = {0.5 * (a.y - am.y), -0.5 * (a.x - am.x), b.y};
+      const hipFloatComplex* q = kinv + 6 * k;
+      double zr[3], zi[3];
+      spec_solve(q, false, ar, ai, zr, zi);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const hipDoubleComplex v = rhat[m * nmodes + k];
-      ar[m] = v.x - sc->sumr[m];
-      ai[m] = v.y;
-    }
-    // Hermitian inverse stored as (00, 01, 02, 11, 12, 22)
-    const hipDoubleComplex* q = kinv + 6 * k;
-    const double h[3][3][2] = {
-        {{q[0].x, q[0].y}, {q[1].x, q[1].y}, {q[2].x, q[2].y}},
-        {{q[1].x, -q[1].y}, {q[3].x, q[3].y}, {q[4].x, q[4].y}},
-        {{q[2].x, -q[2].y}, {q[4].x, -q[4].y}, {q[5].x, q[5].y}}};
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      double zr = 0.0, zi = 0.0;
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        zr += h[m][l][0] * ar[l] - h[m][l][1] * ai[l];
-        zi += h[m][l][0] * ai[l] + h[m][l][1] * ar[l];
+      for (int m = 0; m < 3; ++m) {
+        zr[m] *= inv_n;
+        zi[m] *= inv_n;
       }
-      zr *= inv_n;
-      zi *= inv_n;
-      rz += w * (ar[m] * zr + ai[m] * zi);
-      rhat[m * nmodes + k] = make_hipDoubleComplex(zr, zi);
+      const double wk = (k == mk) ? 1.0 : 2.0;
+      rz += wk * (ar[0] * zr[0] + ai[0] * zi[0] + ar[1] * zr[1] + ai[1] * zi[1] +
+                  ar[2] * zr[2] + ai[2] * zi[2]);
+      // Z(-k) = conj Z(k) for real z
+      ua[k] = make_hipDoubleComplex(zr[0] - zi[1], zi[0] + zr[1]);
+      ub[k] = make_hipDoubleComplex(zr[2], zi[2]);
+      if (mk != k) {
+        ua[mk] = make_hipDoubleComplex(zr[0] + zi[1], -zi[0] + zr[1]);
+        ub[mk] = make_hipDoubleComplex(zr[2], -zi[2]);
+      }
     }
   }
   const double v[1] = {rz};
@@ -717,8 +724,8 @@ __global__ void __launch_bounds__(kBlock)
 // x += alpha p, r -= alpha y; r.r and sum(r).
 __global__ void __launch_bounds__(kBlock)
     k_spec_update(int64_t C, const double* __restrict__ p, const double* __restrict__ y,
-                  double* __restrict__ x, double* __restrict__ r, Scalars* sc, double* partials,
-                  unsigned* counter) {
+                  double* __restrict__ x, double* __restrict__ r, hipDoubleComplex* __restrict__ uab,
+                  Scalars* sc, double* partials, unsigned* counter) {
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
   const double alpha = sc->alpha;
@@ -733,6 +740,8 @@ __global__ void __launch_bounds__(kBlock)
       rr += rv * rv;
       s[m] = rv;
     }
+    uab[c] = make_hipDoubleComplex(s[0], s[1]);
+    uab[C + c] = make_hipDoubleComplex(s[2], 0.0);
   }
   const double v[4] = {rr, s[0], s[1], s[2]};
   double out[4];
@@ -1125,11 +1134,10 @@ struct cia_rve {
   bool have_fields = false;
   // spectral preconditioner
   int prec = CIA_PCG_BLOCK_JACOBI;
-  hipfftHandle fft_fwd = 0, fft_inv = 0;
+  hipfftHandle fft = 0;
   bool fft_ready = false;
-  int64_t nmodes = 0;
-  hipDoubleComplex* kinv = nullptr;
-  hipDoubleComplex* rhat = nullptr;
+  hipFloatComplex* kinv = nullptr;   // [N][6] inverse symbol
+  hipDoubleComplex* uab = nullptr;   // [2][N] packed residual / preconditioned spectra
   double ref_bulk = 0.0, ref_shear = 0.0;
 };
 
@@ -1160,7 +1168,7 @@ struct Ops {
   static cia_status assemble(cia_rve* R, hipStream_t st, bool extra, int slot, const double* w,
                              const double* wc, const Macro& M, const void* state_c, void* state_t,
                              const Settings& S, double dt) {
-    const unsigned grid = grid_for(R->E);
+    const unsigned grid = grid_for(R->np);
     prof::begin(prof::ASSEMBLE, st);
     if (extra)
       hipLaunchKernelGGL((k_assemble<T, true>), dim3(grid), dim3(kBlock), 0, st, R->G, R->d_tab,
@@ -1192,31 +1200,30 @@ struct Ops {
   static cia_status pcg_spectral(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
                                  int* iters_out, int* flag_out) {
     const int64_t C = R->C;
-    const unsigned gn = grid_for(C), ge = grid_for(R->E), gm = grid_for(R->nmodes);
-    const int nxh = R->nx / 2 + 1;
+    const unsigned gn = grid_for(C), ge = grid_for(R->np);
     const double inv_n = 1.0 / (double)C;
-    if (hipfftSetStream(R->fft_fwd, st) != HIPFFT_SUCCESS ||
-        hipfftSetStream(R->fft_inv, st) != HIPFFT_SUCCESS) {
+    if (hipfftSetStream(R->fft, st) != HIPFFT_SUCCESS) {
       set_error_msg("hipfftSetStream failed");
       return CIA_ERR_HIP;
     }
     auto precondition = [&]() -> cia_status {
-      if (hipfftExecD2Z(R->fft_fwd, (hipfftDoubleReal*)R->pr, (hipfftDoubleComplex*)R->rhat) !=
-          HIPFFT_SUCCESS) {
-        set_error_msg("hipfftExecD2Z failed");
+      if (hipfftExecZ2Z(R->fft, (hipfftDoubleComplex*)R->uab, (hipfftDoubleComplex*)R->uab,
+                        HIPFFT_FORWARD) != HIPFFT_SUCCESS) {
+        set_error_msg("hipfftExecZ2Z (forward) failed");
         return CIA_ERR_HIP;
       }
-      hipLaunchKernelGGL(k_spec_apply, dim3(gm), dim3(kBlock), 0, st, R->nmodes, nxh, R->nx, inv_n,
-                         (const hipDoubleComplex*)R->kinv, R->rhat, R->sc, R->partials, R->counter);
-      if (hipfftExecZ2D(R->fft_inv, (hipfftDoubleComplex*)R->rhat, (hipfftDoubleReal*)R->pz) !=
-          HIPFFT_SUCCESS) {
-        set_error_msg("hipfftExecZ2D failed");
+      hipLaunchKernelGGL(k_spec_apply, dim3(gn), dim3(kBlock), 0, st, C, R->nx, R->ny, R->nz, inv_n,
+                         (const hipFloatComplex*)R->kinv, R->uab, R->uab + C, R->sc, R->partials,
+                         R->counter);
+      if (hipfftExecZ2Z(R->fft, (hipfftDoubleComplex*)R->uab, (hipfftDoubleComplex*)R->uab,
+                        HIPFFT_BACKWARD) != HIPFFT_SUCCESS) {
+        set_error_msg("hipfftExecZ2Z (inverse) failed");
         return CIA_ERR_HIP;
       }
       return CIA_OK;
     };
     prof::begin(prof::PRECOND, st);
-    hipLaunchKernelGGL(k_spec_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1],
+    hipLaunchKernelGGL(k_spec_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1], R->uab,
                        R->lin.relative_tolerance, R->lin.absolute_tolerance, R->sc, R->partials,
                        R->counter);
     CIA_TRY(precondition());
@@ -1232,8 +1239,8 @@ struct Ops {
         double* p_new = R->pp[pi];
         prof::begin(prof::PCG_ELEM, st);
         hipLaunchKernelGGL((k_pcg_elem<T, true>), dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
-                           R->d_ephase, (const T*)R->tan[slot], R->pz, p_old, p_new, (T*)R->F,
-                           R->sc);
+                           R->d_ephase, (const T*)R->tan[slot], (const double*)R->uab, p_old,
+                           p_new, (T*)R->F, R->sc);
         prof::end(st);
         prof::begin(prof::PCG_NODE, st);
         hipLaunchKernelGGL(k_pcg_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F,
@@ -1241,7 +1248,7 @@ struct Ops {
         prof::end(st);
         prof::begin(prof::PCG_UPDATE, st);
         hipLaunchKernelGGL(k_spec_update, dim3(gn), dim3(kBlock), 0, st, C, p_new, R->py, x, R->pr,
-                           R->sc, R->partials, R->counter);
+                           R->uab, R->sc, R->partials, R->counter);
         prof::end(st);
         prof::begin(prof::PRECOND, st);
         CIA_TRY(precondition());
@@ -1272,7 +1279,7 @@ struct Ops {
     if (R->prec == CIA_PCG_SPECTRAL && R->fft_ready)
       return pcg_spectral(R, st, slot, b, x, iters_out, flag_out);
     const int64_t C = R->C;
-    const unsigned gn = grid_for(C), ge = grid_for(R->E);
+    const unsigned gn = grid_for(C), ge = grid_for(R->np);
     prof::begin(prof::PRECOND, st);
     hipLaunchKernelGGL(k_precond<T>, dim3(gn), dim3(kBlock), 0, st, R->G, R->d_tab, R->d_ephase,
                        (const T*)R->tan[slot], R->minv);
@@ -1350,11 +1357,12 @@ static cia_status check_fail(cia_rve* R, hipStream_t st, unsigned long long* ff)
   return CIA_OK;
 }
 
-// Inverse Fourier symbol of the reference-medium Hex8 stiffness on the
-// half spectrum (kx = 0..nx/2 fastest, then ky, kz), Hermitian 3x3 blocks
-// stored as (00, 01, 02, 11, 12, 22); the zero mode maps to 0.
+// Inverse Fourier symbol of the reference-medium Hex8 stiffness on the full
+// spectrum (kx fastest, then ky, kz), Hermitian 3x3 blocks stored in FP32 as
+// (00, 01, 02, 11, 12, 22); the zero mode maps to 0.  FP32 storage keeps every
+// block Hermitian positive definite, so the preconditioner stays SPD.
 static void spectral_symbol(int nx, int ny, int nz, const BoxGeom& g, double K0, double G0,
-                            std::vector<hipDoubleComplex>& out) {
+                            std::vector<hipFloatComplex>& out) {
   Tangent<double> t;
   t.K = K0;
   t.d = 2.0 * G0;
@@ -1387,14 +1395,13 @@ static void spectral_symbol(int nx, int ny, int nz, const BoxGeom& g, double K0,
       for (int l = 0; l < 3; ++l)
         for (int m = 0; m < 3; ++m) S[ox + 1][oy + 1][oz + 1][l][m] += ke[a][l][b][m];
     }
-  const int nxh = nx / 2 + 1;
-  out.assign((size_t)6 * nxh * ny * nz, make_hipDoubleComplex(0.0, 0.0));
+  out.assign((size_t)6 * nx * ny * nz, make_hipFloatComplex(0.f, 0.f));
   const double two_pi = 6.283185307179586476925286766559;
   typedef std::complex<double> cd;
   for (int kz = 0; kz < nz; ++kz)
     for (int ky = 0; ky < ny; ++ky)
-      for (int kx = 0; kx < nxh; ++kx) {
-        const size_t mode = kx + (size_t)nxh * (ky + (size_t)ny * kz);
+      for (int kx = 0; kx < nx; ++kx) {
+        const size_t mode = kx + (size_t)nx * (ky + (size_t)ny * kz);
         if (kx == 0 && ky == 0 && kz == 0) continue;
         const double th[3] = {two_pi * kx / nx, two_pi * ky / ny, two_pi * kz / nz};
         cd A[3][3] = {};
@@ -1416,7 +1423,7 @@ static void spectral_symbol(int nx, int ny, int nz, const BoxGeom& g, double K0,
                            (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det,
                            (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det};
         for (int u = 0; u < 6; ++u)
-          out[6 * mode + u] = make_hipDoubleComplex(inv[u].real(), inv[u].imag());
+          out[6 * mode + u] = make_hipFloatComplex((float)inv[u].real(), (float)inv[u].imag());
       }
 }
 
@@ -1810,24 +1817,19 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
     }
     R->ref_bulk = d->linear.reference_bulk > 0 ? d->linear.reference_bulk : std::sqrt(kmin * kmax);
     R->ref_shear = d->linear.reference_shear > 0 ? d->linear.reference_shear : std::sqrt(gmin * gmax);
-    R->nmodes = (int64_t)(R->nx / 2 + 1) * R->ny * R->nz;
-    ALLOC(R->kinv, 6 * R->nmodes * sizeof(hipDoubleComplex));
-    ALLOC(R->rhat, 3 * R->nmodes * sizeof(hipDoubleComplex));
-    std::vector<hipDoubleComplex> sym;
+    ALLOC(R->kinv, 6 * R->C * sizeof(hipFloatComplex));
+    ALLOC(R->uab, 2 * R->C * sizeof(hipDoubleComplex));
+    std::vector<hipFloatComplex> sym;
     spectral_symbol(R->nx, R->ny, R->nz, R->G.g, R->ref_bulk, R->ref_shear, sym);
-    if (hipMemcpy(R->kinv, sym.data(), sym.size() * sizeof(hipDoubleComplex),
+    if (hipMemcpy(R->kinv, sym.data(), sym.size() * sizeof(hipFloatComplex),
                   hipMemcpyHostToDevice) != hipSuccess) {
       cia_rve_destroy(R);
       set_error_msg("hipMemcpy of the spectral symbol failed");
       return CIA_ERR_HIP;
     }
     int dims[3] = {R->nz, R->ny, R->nx};
-    int rem[3] = {R->nz, R->ny, R->nx};
-    int cem[3] = {R->nz, R->ny, R->nx / 2 + 1};
-    if (hipfftPlanMany(&R->fft_fwd, 3, dims, rem, 3, 1, cem, 1, (int)R->nmodes, HIPFFT_D2Z, 3) !=
-            HIPFFT_SUCCESS ||
-        hipfftPlanMany(&R->fft_inv, 3, dims, cem, 1, (int)R->nmodes, rem, 3, 1, HIPFFT_Z2D, 3) !=
-            HIPFFT_SUCCESS) {
+    if (hipfftPlanMany(&R->fft, 3, dims, dims, 1, (int)R->C, dims, 1, (int)R->C, HIPFFT_Z2Z, 2) !=
+        HIPFFT_SUCCESS) {
       cia_rve_destroy(R);
       set_error_msg("hipfftPlanMany failed");
       return CIA_ERR_HIP;
@@ -1855,9 +1857,8 @@ cia_status cia_rve_destroy(cia_rve* R) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (R->kinv) (void)hipFree(R->kinv);
-  if (R->rhat) (void)hipFree(R->rhat);
-  if (R->fft_fwd) (void)hipfftDestroy(R->fft_fwd);
-  if (R->fft_inv) (void)hipfftDestroy(R->fft_inv);
+  if (R->uab) (void)hipFree(R->uab);
+  if (R->fft) (void)hipfftDestroy(R->fft);
   delete R;
   return CIA_OK;
 }
