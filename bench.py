@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def build_model(n: int, seed: int, precision: str, pcg_rtol: float):
+def build_model(n: int, seed: int, precision: str, pcg_rtol: float, forcing: bool = True):
     from concrete_impact_ai_amd import materials as M
     from concrete_impact_ai_amd import rve as R
     from concrete_impact_ai_amd.microstructure import spherical_aggregates
@@ -49,7 +49,7 @@ def build_model(n: int, seed: int, precision: str, pcg_rtol: float):
     newton = NonlinearNewtonSettings(25, 1.0e-10 * 20.0e6, 1.0e-10, 1.0e-12, 1.0e-10,
                                      ArmijoSettings(True, 1.0e-4, 0.5, 20))
     lin = LinearSolverSettings(method="spectral", max_iterations=5000, relative_tolerance=pcg_rtol,
-                               check_interval=8)
+                               check_interval=8, adaptive_forcing=forcing)
     model = R.StructuredHex8RVE(
         nodes, elements, (n, n, n),
         M.PhasedMaterialModel("c2_rve", (M.RVEPhase("mortar", mortar), M.RVEPhase("aggregate", agg)),
@@ -118,6 +118,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--pcg-rtol", type=float, default=1.0e-10)
+    ap.add_argument("--fixed-forcing", action="store_true",
+                    help="solve every Newton step to --pcg-rtol instead of the inexact-Newton forcing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -136,7 +138,7 @@ def main() -> int:
     torch.cuda.set_device(0)
     lib = _lib.lib()
     n = args.size
-    model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol)
+    model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol, not args.fixed_forcing)
     times, strains = random_strain_path(200, 8, 3.0e-3, (10.0, 1000.0), seed=args.seed)
     settings = M.MaterialUpdateSettings(50, 1.0e-10, 1.0e-12, 1.0e-10)
     req_no_tan = MaterialResponseRequirements(tangent=False)
@@ -242,6 +244,7 @@ def main() -> int:
             "seed": args.seed,
             "precision": args.precision,
             "pcg_rtol": args.pcg_rtol,
+            "inexact_newton_forcing": not args.fixed_forcing,
         },
         "gp_updates_per_s": gp_updates / elapsed,
         "newton_iterations_per_step": newton / K,

@@ -168,9 +168,10 @@ class LinearSettings(C.Structure):
         ("relative_tolerance", C.c_double),
         ("absolute_tolerance", C.c_double),
         ("preconditioner", C.c_int32),
-        ("pad", C.c_int32),
+        ("adaptive_forcing", C.c_int32),
         ("reference_bulk", C.c_double),
         ("reference_shear", C.c_double),
+        ("forcing_max", C.c_double),
     ]
 
 

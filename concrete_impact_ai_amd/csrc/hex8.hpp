@@ -176,11 +176,14 @@ __device__ __forceinline__ void element_reduce8(int q, const T f[8][3], T out[3]
 }
 
 // Element (i, j, k) of a periodic nx x ny x nz grid: node classes of its 8 corners.
-__device__ __forceinline__ void periodic_corners(int64_t e, int nx, int ny, int nz, int64_t c[8],
+// (32-bit index arithmetic: an RVE holds fewer than 2^31 elements.)
+__device__ __forceinline__ void periodic_corners(int64_t e64, int nx, int ny, int nz, int64_t c[8],
                                                  int& i, int& j, int& k) {
-  i = (int)(e % nx);
-  j = (int)((e / nx) % ny);
-  k = (int)(e / ((int64_t)nx * ny));
+  const unsigned e = (unsigned)e64;
+  const unsigned r = e / (unsigned)nx;
+  i = (int)(e - r * (unsigned)nx);
+  k = (int)(r / (unsigned)ny);
+  j = (int)(r - (unsigned)k * (unsigned)ny);
   const int ip = i + 1 == nx ? 0 : i + 1;
   const int jp = j + 1 == ny ? 0 : j + 1;
   const int kp = k + 1 == nz ? 0 : k + 1;

@@ -41,6 +41,7 @@ namespace cia {
 
 struct Scalars {
   double rz, pq, alpha, beta, rr, tol2;
+  double rzr[2];   // r.z of the last two spectral PCG iterations (ring by parity)
   double norm2[4];
   double sumr[3];  // per-component sum of the PCG residual (spectral preconditioner)
   int32_t done;    // 0 running, 1 converged, 2 breakdown
@@ -758,6 +759,264 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+
+// -------------------------------------------------------------------------
+// Spectral PCG with deferred reductions.  A kernel that produces a dot product
+// writes one partial per block; every block of the next kernel sums those
+// partials itself, in the same fixed order, so all blocks see bitwise-identical
+// scalars and no kernel needs a grid-wide fence (the kernel boundary makes the
+// partials visible across XCDs).  Block 0 records the scalars for the host.
+
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int P,
+                                                double out[NV], double* lds) {
+  __shared__ double bc[NV];
+#pragma unroll
+  for (int s = 0; s < NV; ++s) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < P; b += blockDim.x) a += part[s * P + b];
+    a = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) bc[s] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NV; ++s) out[s] = bc[s];
+}
+
+template <int NV>
+__device__ __forceinline__ void write_partials(const double (&v)[NV], double* __restrict__ part,
+                                               double* lds) {
+#pragma unroll
+  for (int s = 0; s < NV; ++s) {
+    const double t = block_sum<kBlock>(v[s], lds);
+    if (threadIdx.x == 0) part[s * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// r = b, x = 0, p_old = 0, packed r; partials (r.r, sum r).
+__global__ void __launch_bounds__(kBlock)
+    k_sp_init(int64_t C, const double* __restrict__ b, double* __restrict__ x,
+              double* __restrict__ r, double* __restrict__ p_old,
+              hipDoubleComplex* __restrict__ uab, double* __restrict__ part_upd,
+              Scalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (c < C) {
+    double rv[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      rv[m] = b[3 * c + m];
+      x[3 * c + m] = 0.0;
+      r[3 * c + m] = rv[m];
+      p_old[3 * c + m] = 0.0;
+      v[0] += rv[m] * rv[m];
+      v[1 + m] = rv[m];
+    }
+    uab[c] = make_hipDoubleComplex(rv[0], rv[1]);
+    uab[C + c] = make_hipDoubleComplex(rv[2], 0.0);
+  }
+  write_partials<4>(v, part_upd, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->done = 0;
+    sc->iters = 0;
+  }
+}
+
+// Convergence test on the partials of the last update, then the symbol solve
+// of the packed spectra (see k_spec_apply above) and r.z partials.
+__global__ void __launch_bounds__(kBlock)
+    k_sp_apply(int64_t N, int nx, int ny, int nz, double inv_n,
+               const hipFloatComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ ua,
+               hipDoubleComplex* __restrict__ ub, const double* __restrict__ part_upd, int P,
+               double* __restrict__ part_rz, Scalars* __restrict__ sc, int first, double rtol,
+               double atol) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  double u[4];
+  reduce_partials<4>(part_upd, P, u, lds);
+  double tol2;
+  if (first) {
+    const double tol = fmax(rtol * sqrt(u[0]), atol);
+    tol2 = tol * tol;
+  } else {
+    tol2 = sc->tol2;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rr = u[0];
+    if (first) sc->tol2 = tol2;
+  }
+  if (u[0] <= tol2 || !isfinite(u[0])) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = isfinite(u[0]) ? 1 : 2;
+    return;
+  }
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0;
+  if (k < N) {
+    const int64_t mk = neg_mode(k, nx, ny, nz);
+    if (k <= mk) {
+      const hipDoubleComplex a = ua[k], am = ua[mk], b = ub[k];
+      double ar[3] = {0.5 * (a.x + am.x) - u[1], 0.5 * (a.y + am.y) - u[2], b.x - u[3]};
+      double ai[3] = {0.5 * (a.y - am.y), -0.5 * (a.x - am.x), b.y};
+      double zr[3], zi[3];
+      spec_solve(kinv + 6 * k, false, ar, ai, zr, zi);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        zr[m] *= inv_n;
+        zi[m] *= inv_n;
+      }
+      const double wk = (k == mk) ? 1.0 : 2.0;
+      rz += wk * (ar[0] * zr[0] + ai[0] * zi[0] + ar[1] * zr[1] + ai[1] * zi[1] +
+                  ar[2] * zr[2] + ai[2] * zi[2]);
+      ua[k] = make_hipDoubleComplex(zr[0] - zi[1], zi[0] + zr[1]);
+      ub[k] = make_hipDoubleComplex(zr[2], zi[2]);
+      if (mk != k) {
+        ua[mk] = make_hipDoubleComplex(zr[0] + zi[1], -zi[0] + zr[1]);
+        ub[mk] = make_hipDoubleComplex(zr[2], -zi[2]);
+      }
+    }
+  }
+  const double v[1] = {rz};
+  write_partials<1>(v, part_rz, lds);
+}
+
+// y = gather(F) (class 0 zeroed); p.y partials.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sp_node(Grid G, const T* __restrict__ F, const double* __restrict__ p, double* __restrict__ y,
+              double* __restrict__ part_pq, const Scalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double pq = 0.0;
+  if (c < G.E) {
+    const unsigned cu = (unsigned)c, rr = cu / (unsigned)G.nx;
+    const int i = (int)(cu - rr * (unsigned)G.nx);
+    const int k = (int)(rr / (unsigned)G.ny);
+    const int j = (int)(rr - (unsigned)k * (unsigned)G.ny);
+    double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int64_t e = periodic_corner_element(i, j, k, a, G.nx, G.ny, G.nz);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) s[m] += (double)F[(3 * a + m) * G.E + e];
+    }
+    if (c == 0) s[0] = s[1] = s[2] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      y[3 * c + m] = s[m];
+      pq += p[3 * c + m] * s[m];
+    }
+  }
+  const double v[1] = {pq};
+  write_partials<1>(v, part_pq, lds);
+}
+
+// alpha = r.z / p.y; x += alpha p, r -= alpha y, packed r; partials (r.r, sum r).
+__global__ void __launch_bounds__(kBlock)
+    k_sp_update(int64_t C, const double* __restrict__ p, const double* __restrict__ y,
+                double* __restrict__ x, double* __restrict__ r, hipDoubleComplex* __restrict__ uab,
+                const double* __restrict__ part_pq, int P, double* __restrict__ part_upd,
+                Scalars* __restrict__ sc, int it) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  double pq[1];
+  reduce_partials<1>(part_pq, P, pq, lds);
+  if (!(pq[0] > 0.0) || !isfinite(pq[0])) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
+    return;
+  }
+  const double alpha = sc->rzr[it & 1] / pq[0];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (c < C) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      x[3 * c + m] += alpha * p[3 * c + m];
+      const double rv = r[3 * c + m] - alpha * y[3 * c + m];
+      r[3 * c + m] = rv;
+      v[0] += rv * rv;
+      v[1 + m] = rv;
+    }
+    uab[c] = make_hipDoubleComplex(v[1], v[2]);
+    uab[C + c] = make_hipDoubleComplex(v[3], 0.0);
+  }
+  write_partials<4>(v, part_upd, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->iters = it + 1;
+    sc->alpha = alpha;
+  }
+}
+
+// Element pass: beta from the r.z partials, p = (z - z0) + beta p_old, forces.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_sp_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+              const T* __restrict__ tan, const hipDoubleComplex* __restrict__ zc,
+              const double* __restrict__ p_old, double* __restrict__ p_new, T* __restrict__ F,
+              const double* __restrict__ part_rz, int P, Scalars* __restrict__ sc, int it) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  double rz[1];
+  reduce_partials<1>(part_rz, P, rz, lds);
+  if (!(rz[0] > 0.0) || !isfinite(rz[0])) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
+    return;
+  }
+  const double beta = it == 0 ? 0.0 : rz[0] / sc->rzr[(it + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rzr[it & 1] = rz[0];
+    sc->rz = rz[0];
+    sc->beta = beta;
+  }
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  const int64_t C = G.E;
+  const double z0[3] = {zc[0].x, zc[0].y, zc[C].x};
+  int64_t c[8];
+  int i, j, k;
+  periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+  T eps[6];
+  T bx[8], by[8], bz[8];
+  gp_factors<T>(q, G.g, bx, by, bz);
+  {
+    double pe[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const hipDoubleComplex t01 = zc[c[a]], t2 = zc[C + c[a]];
+      pe[a][0] = (t01.x - z0[0]) + beta * p_old[3 * c[a] + 0];
+      pe[a][1] = (t01.y - z0[1]) + beta * p_old[3 * c[a] + 1];
+      pe[a][2] = (t2.x - z0[2]) + beta * p_old[3 * c[a] + 2];
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) p_new[3 * e + m] = pe[0][m];
+    }
+    strain_from(pe, bx, by, bz, eps);
+  }
+  Tangent<T> tc;
+  tc.K = (T)tab->p[ephase[e]].K;
+  tc.d = tan[p];
+  tc.r = tan[np + p];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
+  if (tc.r != (T)0) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * np + p];
+  }
+  T s[6];
+  tangent_apply(tc, eps, s);
+  T f[8][3];
+  force_from(s, (T)G.g.jw, bx, by, bz, f);
+  T fin[3];
+  element_reduce8(q, f, fin);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
+}
+
 // -------------------------------------------------------------------------
 // Effective tangent: element forces of the uniform strain e_j (K A e_j).
 template <class T>
@@ -1198,35 +1457,38 @@ struct Ops {
 
   // Solve K x = b with the spectral reference-medium preconditioner.
   static cia_status pcg_spectral(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
-                                 int* iters_out, int* flag_out) {
+                                 int* iters_out, int* flag_out, double rtol) {
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
+    const int P = (int)gn;
     const double inv_n = 1.0 / (double)C;
+    double* part_pq = R->partials;
+    double* part_upd = R->partials + P;
+    double* part_rz = R->partials + 5 * P;
     if (hipfftSetStream(R->fft, st) != HIPFFT_SUCCESS) {
       set_error_msg("hipfftSetStream failed");
       return CIA_ERR_HIP;
     }
-    auto precondition = [&]() -> cia_status {
-      if (hipfftExecZ2Z(R->fft, (hipfftDoubleComplex*)R->uab, (hipfftDoubleComplex*)R->uab,
-                        HIPFFT_FORWARD) != HIPFFT_SUCCESS) {
-        set_error_msg("hipfftExecZ2Z (forward) failed");
-        return CIA_ERR_HIP;
-      }
-      hipLaunchKernelGGL(k_spec_apply, dim3(gn), dim3(kBlock), 0, st, C, R->nx, R->ny, R->nz, inv_n,
-                         (const hipFloatComplex*)R->kinv, R->uab, R->uab + C, R->sc, R->partials,
-                         R->counter);
-      if (hipfftExecZ2Z(R->fft, (hipfftDoubleComplex*)R->uab, (hipfftDoubleComplex*)R->uab,
-                        HIPFFT_BACKWARD) != HIPFFT_SUCCESS) {
-        set_error_msg("hipfftExecZ2Z (inverse) failed");
+    auto fft = [&](int dir) -> cia_status {
+      if (hipfftExecZ2Z(R->fft, (hipfftDoubleComplex*)R->uab, (hipfftDoubleComplex*)R->uab, dir) !=
+          HIPFFT_SUCCESS) {
+        set_error_msg("hipfftExecZ2Z failed");
         return CIA_ERR_HIP;
       }
       return CIA_OK;
     };
+    auto apply = [&](int first) {
+      hipLaunchKernelGGL(k_sp_apply, dim3(gn), dim3(kBlock), 0, st, C, R->nx, R->ny, R->nz, inv_n,
+                         (const hipFloatComplex*)R->kinv, R->uab, R->uab + C,
+                         (const double*)part_upd, P, part_rz, R->sc, first, rtol,
+                         R->lin.absolute_tolerance);
+    };
     prof::begin(prof::PRECOND, st);
-    hipLaunchKernelGGL(k_spec_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1], R->uab,
-                       R->lin.relative_tolerance, R->lin.absolute_tolerance, R->sc, R->partials,
-                       R->counter);
-    CIA_TRY(precondition());
+    hipLaunchKernelGGL(k_sp_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1], R->uab,
+                       part_upd, R->sc);
+    CIA_TRY(fft(HIPFFT_FORWARD));
+    apply(1);
+    CIA_TRY(fft(HIPFFT_BACKWARD));
     prof::end(st);
     CIA_HIP(hipGetLastError());
     int done = 0, it = 0, pi = 0;
@@ -1234,29 +1496,30 @@ struct Ops {
     Scalars hs;
     while (it < R->lin.max_iterations) {
       const int n = std::min(chunk, R->lin.max_iterations - it);
-      for (int s = 0; s < n; ++s) {
+      for (int s = 0; s < n; ++s, ++it) {
         double* p_old = R->pp[1 - pi];
         double* p_new = R->pp[pi];
         prof::begin(prof::PCG_ELEM, st);
-        hipLaunchKernelGGL((k_pcg_elem<T, true>), dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab,
-                           R->d_ephase, (const T*)R->tan[slot], (const double*)R->uab, p_old,
-                           p_new, (T*)R->F, R->sc);
+        hipLaunchKernelGGL(k_sp_elem<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab, R->d_ephase,
+                           (const T*)R->tan[slot], (const hipDoubleComplex*)R->uab, p_old, p_new,
+                           (T*)R->F, (const double*)part_rz, P, R->sc, it);
         prof::end(st);
         prof::begin(prof::PCG_NODE, st);
-        hipLaunchKernelGGL(k_pcg_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F,
-                           p_new, R->py, R->sc, R->partials, R->counter);
+        hipLaunchKernelGGL(k_sp_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F, p_new,
+                           R->py, part_pq, R->sc);
         prof::end(st);
         prof::begin(prof::PCG_UPDATE, st);
-        hipLaunchKernelGGL(k_spec_update, dim3(gn), dim3(kBlock), 0, st, C, p_new, R->py, x, R->pr,
-                           R->uab, R->sc, R->partials, R->counter);
+        hipLaunchKernelGGL(k_sp_update, dim3(gn), dim3(kBlock), 0, st, C, p_new, R->py, x, R->pr,
+                           R->uab, (const double*)part_pq, P, part_upd, R->sc, it);
         prof::end(st);
         prof::begin(prof::PRECOND, st);
-        CIA_TRY(precondition());
+        CIA_TRY(fft(HIPFFT_FORWARD));
+        apply(0);
+        CIA_TRY(fft(HIPFFT_BACKWARD));
         prof::end(st);
         pi = 1 - pi;
       }
       CIA_HIP(hipGetLastError());
-      it += n;
       CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
       CIA_HIP(hipStreamSynchronize(st));
       prof::flush();
@@ -1275,9 +1538,10 @@ struct Ops {
 
   // Solve K x = b (K from tan[slot]); returns iterations.
   static cia_status pcg(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
-                        int* iters_out, int* flag_out) {
+                        int* iters_out, int* flag_out, double rtol = -1.0) {
+    if (rtol <= 0.0) rtol = R->lin.relative_tolerance;
     if (R->prec == CIA_PCG_SPECTRAL && R->fft_ready)
-      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out);
+      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out, rtol);
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
     prof::begin(prof::PRECOND, st);
@@ -1285,7 +1549,7 @@ struct Ops {
                        (const T*)R->tan[slot], R->minv);
     prof::end(st);
     hipLaunchKernelGGL(k_pcg_init, dim3(gn), dim3(kBlock), 0, st, C, b, R->minv, x, R->pr, R->pz,
-                       R->pp[1], R->lin.relative_tolerance, R->lin.absolute_tolerance, R->sc,
+                       R->pp[1], rtol, R->lin.absolute_tolerance, R->sc,
                        R->partials, R->counter);
     CIA_HIP(hipGetLastError());
     int done = 0;
@@ -1495,7 +1759,14 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
     // Newton direction: K dw = -r
     hipLaunchKernelGGL(k_scale, dim3(grid_for(C3)), dim3(kBlock), 0, st, C3, -1.0, R->r, R->r_cand);
     int pit = 0, pflag = 0;
-    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag));
+    double lin_rtol = R->lin.relative_tolerance;
+    if (R->lin.adaptive_forcing) {
+      // inexact Newton: solve only as accurately as the current residual warrants
+      const double eta = std::min(R->lin.forcing_max > 0 ? R->lin.forcing_max : 0.1, rn / r0);
+      lin_rtol = std::max(lin_rtol, std::max(eta, 0.5 * rtol / std::max(rn, 1e-300)));
+      lin_rtol = std::min(lin_rtol, 0.5);
+    }
+    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag, lin_rtol));
     double dn2 = 0.0;
     CIA_TRY(norm2_of(R, st, C3, R->dw, &dn2));
     const double inc_n = sqrt(dn2);

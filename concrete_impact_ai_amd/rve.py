@@ -344,8 +344,9 @@ class RVEOperatorCache:
         desc.linear = _lib.LinearSettings(
             int(ls.max_iterations), int(ls.check_interval), float(ls.relative_tolerance),
             float(ls.absolute_tolerance),
-            _lib.CIA_PCG_SPECTRAL if ls.method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI, 0,
-            float(ls.reference_bulk), float(ls.reference_shear))
+            _lib.CIA_PCG_SPECTRAL if ls.method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI,
+            int(bool(ls.adaptive_forcing)), float(ls.reference_bulk), float(ls.reference_shear),
+            float(ls.forcing_max))
         desc.hill_mandel_power_scale = float(model.hill_mandel_power_scale)
         desc.precision = _lib.CIA_PREC_FP64 if model.precision == "fp64" else _lib.CIA_PREC_FP32_REFINED
         device()

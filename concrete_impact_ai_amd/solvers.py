@@ -77,6 +77,11 @@ class LinearSolverSettings:
     check_interval: int = 8
     reference_bulk: float = 0.0  # <= 0: geometric mean over the phases
     reference_shear: float = 0.0
+    # Inexact Newton: the k-th linear solve stops at
+    # rtol_k = max(relative_tolerance, 0.5 tol_newton / ||r_k||, min(forcing_max, ||r_k|| / ||r_0||));
+    # the Newton residual and increment tests are unchanged.
+    adaptive_forcing: bool = False
+    forcing_max: float = 0.1
 
 
 @dataclass(frozen=True)

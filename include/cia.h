@@ -211,9 +211,10 @@ typedef struct {
   double relative_tolerance; /* ||r_k|| <= rtol * ||b|| */
   double absolute_tolerance;
   int32_t preconditioner;   /* CIA_PCG_* */
-  int32_t pad;
+  int32_t adaptive_forcing; /* 1: inexact Newton, rtol_k = clamp(||r_k|| / ||r_0||) */
   double reference_bulk;    /* spectral reference medium (<= 0: geometric mean of phases) */
   double reference_shear;
+  double forcing_max;       /* upper clamp of the adaptive relative tolerance (e.g. 0.1) */
 } cia_linear_settings;
 
 /* PCG preconditioners.  CIA_PCG_SPECTRAL is the exact inverse of the Hex8
