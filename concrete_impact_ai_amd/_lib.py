@@ -270,6 +270,36 @@ _SIGNATURES = {
     "cia_profile_read": (C.c_int32, [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
 }
 
+class BoxDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("n_phases", C.c_int32),
+        ("lengths", C.c_double * 3),
+        ("phases", C.POINTER(Material)),
+        ("element_phase", C.POINTER(C.c_int32)),
+        ("precision", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+N_HISTORY = 10
+HISTORY_COLUMNS = ["kinetic", "free_energy", "dissipation_rate", "ext_du", "positive_work",
+                   "negative_work", "unloading_volume", "active_volume", "margin_min",
+                   "margin_max"]
+
+_SIGNATURES.update({
+    "cia_box_create": (C.c_int32, [C.POINTER(BoxDesc), C.POINTER(_vp)]),
+    "cia_box_destroy": (C.c_int32, [_vp]),
+    "cia_box_explicit_steps": (
+        C.c_int32,
+        [_vp, _vp, C.c_int32, C.c_double, C.c_int32, C.POINTER(C.c_double),
+         C.POINTER(UpdateSettings), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+         C.POINTER(PointFailure)],
+    ),
+})
+
 PROFILE_SLOTS = ["assemble", "pcg_elem", "pcg_node", "pcg_update", "precond", "gather"]
 
 # Entry points whose prototypes are declared by optional translation units.

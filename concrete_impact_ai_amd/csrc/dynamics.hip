@@ -1,0 +1,475 @@
+// Explicit velocity-Verlet material dynamics on a structured Hex8 box.
+//
+// Reference: solve_material_explicit (src/fem/dynamics/material.py:89-297) with
+// the energy bookkeeping of _record_step (material.py:781-848), the lumped
+// kinetic energy (:851-856) and homogeneous Dirichlet zeroing (:1154-1162).
+//
+// One step is two kernels:
+//   k_dyn_elem  one lane per Gauss point: Verlet predictor u+ = u + dt v + dt^2/2 a
+//               evaluated for the 8 corners on the fly (Dirichlet dofs zeroed),
+//               strain and strain rate (u+ - u)/dt, constitutive update from the
+//               committed state written back in place (explicit: every step is
+//               committed), element force rows F[24][E], energy partials.
+//   k_dyn_node  one lane per node: f_int = gather(F), f_ext = s(t+) f_unit,
+//               a+ = (f_ext - f_int)/m, v+ = v + dt/2 (a + a+), Dirichlet zeroing,
+//               kinetic-energy and external-work partials.
+// Each kernel's block 0 folds the previous kernel's partials into the per-step
+// history rows, so the time loop needs no host synchronisation.
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "hex8.hpp"
+
+namespace cia {
+
+struct BoxGrid {
+  int nx, ny, nz;
+  int pad;
+  int64_t E, N;  // elements, nodes
+  BoxGeom g;
+};
+
+// energy history columns
+enum {
+  H_KINETIC = 0,
+  H_FREE = 1,
+  H_DISS_RATE = 2,    // sum D w detJ at the step (times dt = dissipated increment)
+  H_EXT_DU = 3,       // f_unit . (u+ - u)
+  H_POS_WORK = 4,
+  H_NEG_WORK = 5,
+  H_UNLOAD = 6,       // volume with negative incremental work
+  H_ACTIVE = 7,       // viscoplastic-active volume
+  H_MARGIN_MIN = 8,
+  H_MARGIN_MAX = 9,
+  H_N = 10
+};
+enum { PE_N = 8, PN_N = 2 };  // element partial sums (incl. min/max), node partial sums
+
+__device__ __forceinline__ void box_corners(int64_t e64, int nx, int ny, int64_t nd[8]) {
+  const unsigned e = (unsigned)e64;
+  const unsigned r = e / (unsigned)nx;
+  const int i = (int)(e - r * (unsigned)nx);
+  const int k = (int)(r / (unsigned)ny);
+  const int j = (int)(r - (unsigned)k * (unsigned)ny);
+  const int64_t sx = 1, sy = nx + 1, sz = (int64_t)(nx + 1) * (ny + 1);
+  const int64_t base = i + sy * j + sz * k;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) nd[a] = base + ((a & 1) ? sx : 0) + ((a & 2) ? sy : 0) + ((a & 4) ? sz : 0);
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void block_partials(const double* v, int nsum, const double* mins,
+                                               const double* maxs, double* part, int P,
+                                               double* lds) {
+  for (int s = 0; s < nsum; ++s) {
+    const double t = block_sum<BLOCK>(v[s], lds);
+    if (threadIdx.x == 0) part[s * P + blockIdx.x] = t;
+  }
+  if (mins) {
+    const double a = block_min<BLOCK>(mins[0], lds);
+    const double b = block_max<BLOCK>(maxs[0], lds);
+    if (threadIdx.x == 0) {
+      part[nsum * P + blockIdx.x] = a;
+      part[(nsum + 1) * P + blockIdx.x] = b;
+    }
+  }
+}
+
+// Fold the element partials of step `row` into hist[row][...] (block 0 only).
+__device__ __forceinline__ void fold_elem(const double* pe, int P, double jw, double* hist_row,
+                                          double* lds) {
+  __shared__ double bc[PE_N];
+  for (int s = 0; s < PE_N; ++s) {
+    double a = (s == 6) ? INFINITY : (s == 7 ? -INFINITY : 0.0);
+    for (int b = threadIdx.x; b < P; b += blockDim.x) {
+      const double v = pe[s * P + b];
+      a = (s == 6) ? fmin(a, v) : (s == 7 ? fmax(a, v) : a + v);
+    }
+    if (s == 6) a = block_min<kBlock>(a, lds);
+    else if (s == 7) a = block_max<kBlock>(a, lds);
+    else a = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) bc[s] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    hist_row[H_FREE] = bc[0] * jw;
+    hist_row[H_DISS_RATE] = bc[1] * jw;
+    hist_row[H_POS_WORK] = bc[2] * jw;
+    hist_row[H_NEG_WORK] = bc[3] * jw;
+    hist_row[H_UNLOAD] = bc[4] * jw;
+    hist_row[H_ACTIVE] = bc[5] * jw;
+    hist_row[H_MARGIN_MIN] = bc[6];
+    hist_row[H_MARGIN_MAX] = bc[7];
+  }
+}
+
+__device__ __forceinline__ void fold_node(const double* pn, int P, double* hist_row, double* lds) {
+  __shared__ double bc[PN_N];
+  for (int s = 0; s < PN_N; ++s) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < P; b += blockDim.x) a += pn[s * P + b];
+    a = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) bc[s] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    hist_row[H_KINETIC] = bc[0];
+    hist_row[H_EXT_DU] = bc[1];
+  }
+}
+
+// predictor = false: evaluate at u itself (initial assembly, rate 0).
+template <class T, bool PREDICT>
+__global__ void __launch_bounds__(kBlock)
+    k_dyn_elem(BoxGrid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
+               Settings S, double dt, const double* __restrict__ u, const double* __restrict__ v,
+               const double* __restrict__ a, const uint8_t* __restrict__ fixed,
+               T* __restrict__ state, T* __restrict__ sig_out, double* __restrict__ F,
+               double* __restrict__ part_e, const double* __restrict__ part_n_prev, int Pn,
+               double* __restrict__ hist_prev_row, unsigned long long* __restrict__ first_fail) {
+  __shared__ double lds[kBlock / 64];
+  if (hist_prev_row && blockIdx.x == 0) fold_node(part_n_prev, Pn, hist_prev_row, lds);
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  double vals[6] = {0, 0, 0, 0, 0, 0};
+  double mn = INFINITY, mx = -INFINITY;
+  if (e < G.E) {
+    const int64_t np = G.E * 8;
+    int64_t nd[8];
+    box_corners(e, G.nx, G.ny, nd);
+    T bx[8], by[8], bz[8];
+    gp_factors<T>(q, G.g, bx, by, bz);
+    T eps[6], rate[6];
+    {
+      double un[8][3], du[8][3];
+      const double h2 = 0.5 * dt * dt;
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          const int64_t d = 3 * nd[c] + m;
+          const double u0 = u[d];
+          double u1 = u0;
+          if (PREDICT) u1 = fixed[d] ? 0.0 : u0 + dt * v[d] + h2 * a[d];
+          un[c][m] = u1;
+          du[c][m] = u1 - u0;
+        }
+      strain_from(un, bx, by, bz, eps);
+      strain_from(du, bx, by, bz, rate);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rate[m] /= (T)dt;
+    }
+    const PhaseConst P = tab->p[ephase[e]];
+    const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
+    T epsp[6];
+    T qb = (T)0, ga = (T)0;
+    if (j2) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) epsp[m] = state[m * np + p];
+      qb = state[6 * np + p];
+      ga = state[7 * np + p];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
+    }
+    PointOut<T> o;
+    PointExtra<T> ex;
+    const int st = material_point<T, true>(P, S, dt, eps, rate, epsp, qb, ga, o, &ex, nullptr);
+    if (st != CIA_PT_OK) atomicMin(first_fail, (unsigned long long)p);
+    if (j2) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) state[m * np + p] = o.epsp[m];
+      state[6 * np + p] = o.qbar;
+      state[7 * np + p] = o.gam;
+    }
+    if (sig_out) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) sig_out[m * np + p] = o.sig[m];
+    }
+    vals[0] = (double)ex.psi;
+    vals[1] = (double)ex.diss;
+    // increment diagnostics exist for J2 points and for elastic points of a phased model
+    const double pw = (double)ex.d[CIA_D_INCREMENTAL_WORK_DENSITY];
+    if (isfinite(pw)) {
+      vals[2] = fmax(pw, 0.0);
+      vals[3] = fmin(pw, 0.0);
+      vals[4] = pw < 0.0 ? 1.0 : 0.0;
+    }
+    const double act = (double)ex.d[CIA_D_VISCOPLASTIC_ACTIVE];
+    vals[5] = act > 0.5 ? 1.0 : 0.0;
+    const double mg = (double)ex.d[CIA_D_YIELD_ACTIVATION_MARGIN];
+    if (isfinite(mg)) {
+      mn = mg;
+      mx = mg;
+    }
+    T f[8][3];
+    force_from(o.sig, (T)G.g.jw, bx, by, bz, f);
+    T fin[3];
+    element_reduce8(q, f, fin);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = (double)fin[m];
+  }
+  block_partials<kBlock>(vals, 6, &mn, &mx, part_e, gridDim.x, lds);
+}
+
+// Node pass.  INIT: a = (f_ext - f_int)/m at the initial state only.
+template <bool INIT>
+__global__ void __launch_bounds__(kBlock)
+    k_dyn_node(BoxGrid G, double dt, const double* __restrict__ F, const double* __restrict__ mass,
+               const uint8_t* __restrict__ fixed, const double* __restrict__ f_unit, double s_ext,
+               double* __restrict__ u, double* __restrict__ v, double* __restrict__ a,
+               double* __restrict__ fint_out, double* __restrict__ part_n,
+               const double* __restrict__ part_e, int Pe, double* __restrict__ hist_row,
+               double jw) {
+  __shared__ double lds[kBlock / 64];
+  if (blockIdx.x == 0) fold_elem(part_e, Pe, jw, hist_row, lds);
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double vals[2] = {0.0, 0.0};
+  if (n < G.N) {
+    const int nx1 = G.nx + 1, ny1 = G.ny + 1;
+    const unsigned nu = (unsigned)n, r = nu / (unsigned)nx1;
+    const int i = (int)(nu - r * (unsigned)nx1);
+    const int k = (int)(r / (unsigned)ny1);
+    const int j = (int)(r - (unsigned)k * (unsigned)ny1);
+    double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int ei = i - (c & 1), ej = j - ((c >> 1) & 1), ek = k - ((c >> 2) & 1);
+      if (ei < 0 || ej < 0 || ek < 0 || ei >= G.nx || ej >= G.ny || ek >= G.nz) continue;
+      const int64_t e = ei + (int64_t)G.nx * (ej + (int64_t)G.ny * ek);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) f[m] += F[(3 * c + m) * G.E + e];
+    }
+    const double h2 = 0.5 * dt * dt;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * n + m;
+      const bool fx = fixed[d] != 0;
+      const double fe = s_ext * f_unit[d];
+      const double an = fx ? 0.0 : (fe - f[m]) / mass[d];
+      if (fint_out) fint_out[d] = f[m];
+      if (INIT) {
+        a[d] = an;
+        const double vv = v[d];
+        vals[0] += 0.5 * mass[d] * vv * vv;
+      } else {
+        const double u0 = u[d];
+        const double u1 = fx ? 0.0 : u0 + dt * v[d] + h2 * a[d];
+        const double v1 = fx ? 0.0 : v[d] + 0.5 * dt * (a[d] + an);
+        vals[0] += 0.5 * mass[d] * v1 * v1;
+        vals[1] += f_unit[d] * (u1 - u0);
+        u[d] = u1;
+        v[d] = v1;
+        a[d] = an;
+      }
+    }
+  }
+  block_partials<kBlock>(vals, 2, nullptr, nullptr, part_n, gridDim.x, lds);
+}
+
+__global__ void k_fold_node_only(const double* __restrict__ part_n, int P, double* hist_row) {
+  __shared__ double lds[kBlock / 64];
+  fold_node(part_n, P, hist_row, lds);
+}
+
+}  // namespace cia
+
+struct cia_box {
+  cia::BoxGrid G;
+  int precision;
+  int n_phases;
+  cia::PhaseTable* d_tab = nullptr;
+  uint8_t* d_ephase = nullptr;
+  double* F = nullptr;
+  double* part_e = nullptr;
+  double* part_n = nullptr;
+  double* hist = nullptr;  // [steps+1][H_N], grown on demand
+  int64_t hist_rows = 0;
+  unsigned long long* first_fail = nullptr;
+};
+
+namespace cia {
+static cia_status dyn_alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc", e, __FILE__, __LINE__);
+    return CIA_ERR_OUT_OF_MEMORY;
+  }
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMemset", e, __FILE__, __LINE__);
+    return CIA_ERR_HIP;
+  }
+  return CIA_OK;
+}
+}  // namespace cia
+
+extern "C" {
+
+cia_status cia_box_create(const cia_box_desc* d, cia_box** out) {
+  using namespace cia;
+  CIA_ARG(d && out, "NULL argument");
+  CIA_ARG(d->nx >= 1 && d->ny >= 1 && d->nz >= 1, "divisions must be positive");
+  CIA_ARG(d->n_phases >= 1 && d->n_phases <= CIA_MAX_PHASES && d->phases, "invalid phases");
+  CIA_ARG(d->precision == CIA_PREC_FP64 || d->precision == CIA_PREC_FP32_REFINED, "bad precision");
+  auto* B = new cia_box();
+  B->G.nx = d->nx;
+  B->G.ny = d->ny;
+  B->G.nz = d->nz;
+  B->G.E = (int64_t)d->nx * d->ny * d->nz;
+  B->G.N = (int64_t)(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+  const double hx = d->lengths[0] / d->nx, hy = d->lengths[1] / d->ny, hz = d->lengths[2] / d->nz;
+  B->G.g.ihx = 1.0 / hx;
+  B->G.g.ihy = 1.0 / hy;
+  B->G.g.ihz = 1.0 / hz;
+  B->G.g.jw = hx * hy * hz / 8.0;
+  B->precision = d->precision;
+  B->n_phases = d->n_phases;
+  PhaseTable tab{};
+  tab.n = d->n_phases;
+  for (int i = 0; i < d->n_phases; ++i) tab.p[i] = make_phase(d->phases[i]);
+  std::vector<uint8_t> eph(B->G.E, 0);
+  if (d->element_phase) {
+    for (int64_t e = 0; e < B->G.E; ++e) {
+      const int32_t v = d->element_phase[e];
+      if (v < 0 || v >= d->n_phases) {
+        delete B;
+        set_error_msg("element_phase out of range");
+        return CIA_ERR_INVALID_ARGUMENT;
+      }
+      eph[e] = (uint8_t)v;
+    }
+  }
+  const int64_t Pe = grid_for(8 * B->G.E), Pn = grid_for(B->G.N);
+  cia_status s;
+#define ALLOC(ptr, bytes)                    \
+  do {                                       \
+    s = dyn_alloc((void**)&(ptr), (bytes));  \
+    if (s != CIA_OK) {                       \
+      cia_box_destroy(B);                    \
+      return s;                              \
+    }                                        \
+  } while (0)
+  ALLOC(B->d_tab, sizeof(PhaseTable));
+  ALLOC(B->d_ephase, B->G.E);
+  ALLOC(B->F, 24 * B->G.E * sizeof(double));
+  ALLOC(B->part_e, PE_N * Pe * sizeof(double));
+  ALLOC(B->part_n, PN_N * Pn * sizeof(double));
+  ALLOC(B->first_fail, 8);
+#undef ALLOC
+  if (hipMemcpy(B->d_tab, &tab, sizeof(tab), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B->d_ephase, eph.data(), B->G.E, hipMemcpyHostToDevice) != hipSuccess) {
+    cia_box_destroy(B);
+    set_error_msg("hipMemcpy of box tables failed");
+    return CIA_ERR_HIP;
+  }
+  *out = B;
+  return CIA_OK;
+}
+
+cia_status cia_box_destroy(cia_box* B) {
+  if (!B) return CIA_OK;
+  void* ptrs[] = {B->d_tab, B->d_ephase, B->F, B->part_e, B->part_n, B->hist, B->first_fail};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete B;
+  return CIA_OK;
+}
+
+cia_status cia_box_explicit_steps(cia_box* B, void* stream, int32_t initialize, double time_step,
+                                  int32_t n_steps, const double* load_scale,
+                                  const cia_update_settings* settings, double* u, double* v,
+                                  double* a, void* state, const double* mass,
+                                  const uint8_t* fixed, const double* f_unit, double* stress,
+                                  double* internal_force, double* history,
+                                  cia_point_failure* failure) {
+  using namespace cia;
+  CIA_ARG(B && settings && u && v && a && mass && fixed && f_unit && history && load_scale,
+          "NULL argument to cia_box_explicit_steps");
+  CIA_ARG(time_step > 0.0 && n_steps >= 0, "invalid time stepping");
+  hipStream_t st = (hipStream_t)stream;
+  if (failure) {
+    memset(failure, 0, sizeof(*failure));
+    failure->point_id = -1;
+  }
+  const Settings S = make_settings(*settings);
+  const int64_t rows = (int64_t)n_steps + 1;
+  if (B->hist_rows < rows) {
+    if (B->hist) (void)hipFree(B->hist);
+    B->hist = nullptr;
+    cia_status s = dyn_alloc((void**)&B->hist, rows * H_N * sizeof(double));
+    if (s != CIA_OK) return s;
+    B->hist_rows = rows;
+  }
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, B->first_fail, ~0ull);
+  const unsigned ge = grid_for(8 * B->G.E), gn = grid_for(B->G.N);
+  const double jw = B->G.g.jw;
+  const bool f64 = B->precision == CIA_PREC_FP64;
+  auto elem = [&](bool predict, double* hist_prev, bool last) {
+    double* sig = last ? stress : nullptr;
+    if (f64) {
+      if (predict)
+        hipLaunchKernelGGL((k_dyn_elem<double, true>), dim3(ge), dim3(kBlock), 0, st, B->G, B->d_tab,
+                           B->d_ephase, S, time_step, u, v, a, fixed, (double*)state, (double*)sig,
+                           B->F, B->part_e, B->part_n, (int)gn, hist_prev, B->first_fail);
+      else
+        hipLaunchKernelGGL((k_dyn_elem<double, false>), dim3(ge), dim3(kBlock), 0, st, B->G,
+                           B->d_tab, B->d_ephase, S, time_step, u, v, a, fixed, (double*)state,
+                           (double*)sig, B->F, B->part_e, B->part_n, (int)gn, hist_prev,
+                           B->first_fail);
+    } else {
+      if (predict)
+        hipLaunchKernelGGL((k_dyn_elem<float, true>), dim3(ge), dim3(kBlock), 0, st, B->G, B->d_tab,
+                           B->d_ephase, S, time_step, u, v, a, fixed, (float*)state, (float*)sig,
+                           B->F, B->part_e, B->part_n, (int)gn, hist_prev, B->first_fail);
+      else
+        hipLaunchKernelGGL((k_dyn_elem<float, false>), dim3(ge), dim3(kBlock), 0, st, B->G,
+                           B->d_tab, B->d_ephase, S, time_step, u, v, a, fixed, (float*)state,
+                           (float*)sig, B->F, B->part_e, B->part_n, (int)gn, hist_prev,
+                           B->first_fail);
+    }
+  };
+  // history row of step s is base + s; row 0 is the initial state when initialising
+  const int64_t base = initialize ? 1 : 0;
+  if (initialize) {
+    elem(false, nullptr, n_steps == 0);
+    hipLaunchKernelGGL((k_dyn_node<true>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
+                       mass, fixed, f_unit, load_scale[0], u, v, a,
+                       n_steps == 0 ? internal_force : nullptr, B->part_n, B->part_e, (int)ge,
+                       B->hist, jw);
+  }
+  for (int32_t s = 0; s < n_steps; ++s) {
+    const bool last = s + 1 == n_steps;
+    double* prev_row = (initialize || s > 0) ? B->hist + (base + s - 1) * H_N : nullptr;
+    elem(true, prev_row, last);
+    hipLaunchKernelGGL((k_dyn_node<false>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
+                       mass, fixed, f_unit, load_scale[s + 1], u, v, a,
+                       last ? internal_force : nullptr, B->part_n, B->part_e, (int)ge,
+                       B->hist + (base + s) * H_N, jw);
+  }
+  if (initialize || n_steps > 0)
+    hipLaunchKernelGGL(k_fold_node_only, dim3(1), dim3(kBlock), 0, st, B->part_n, (int)gn,
+                       B->hist + (base + n_steps - 1) * H_N);
+  CIA_HIP(hipGetLastError());
+  const int64_t nrows = base + n_steps;
+  if (nrows > 0)
+    CIA_HIP(hipMemcpyAsync(history, B->hist, nrows * H_N * sizeof(double), hipMemcpyDeviceToHost, st));
+  unsigned long long ff = 0;
+  CIA_HIP(hipMemcpyAsync(&ff, B->first_fail, sizeof(ff), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  if (ff != ~0ull) {
+    if (failure) {
+      failure->point_id = (int64_t)ff;
+      failure->reason = -1;  // reason re-derived by the host on the last committed state
+    }
+    return CIA_ERR_POINT_FAILURE;
+  }
+  return CIA_OK;
+}
+
+}  // extern "C"

@@ -342,6 +342,63 @@ enum {
 };
 cia_status cia_rve_copy_field(cia_rve* rve, void* stream, int32_t field, double* dst);
 
+
+/* ------------------------------------------------------------------------ */
+/* Explicit material dynamics on a structured Hex8 box
+ * (solve_material_explicit, src/fem/dynamics/material.py:89-297). */
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t n_phases;
+  double lengths[3];
+  const cia_material* phases;     /* host [n_phases] */
+  const int32_t* element_phase;   /* host [nx*ny*nz], NULL = phase 0 */
+  int32_t precision;              /* CIA_PREC_*: element/material work and state type */
+  int32_t pad;
+} cia_box_desc;
+
+typedef struct cia_box cia_box;
+
+cia_status cia_box_create(const cia_box_desc* desc, cia_box** out);
+cia_status cia_box_destroy(cia_box* box);
+
+/* Energy-history columns written per step by cia_box_explicit_steps. */
+enum {
+  CIA_H_KINETIC = 0,        /* 1/2 sum m v^2 (material.py:851-856) */
+  CIA_H_FREE_ENERGY = 1,    /* sum psi w detJ */
+  CIA_H_DISSIPATION_RATE = 2, /* sum D w detJ (times dt: dissipated increment) */
+  CIA_H_EXT_DU = 3,         /* f_unit . (u_{n+1} - u_n) (external work = mean load scale times this) */
+  CIA_H_POSITIVE_WORK = 4,
+  CIA_H_NEGATIVE_WORK = 5,
+  CIA_H_UNLOADING_VOLUME = 6,
+  CIA_H_ACTIVE_VOLUME = 7,
+  CIA_H_MARGIN_MIN = 8,
+  CIA_H_MARGIN_MAX = 9,
+  CIA_N_HISTORY = 10
+};
+
+/*
+ * Velocity-Verlet steps of solve_material_explicit (material.py:164-284), all on
+ * the device.  With initialize = 1 the initial assembly at u (zero rate) sets
+ * a = (f_ext(0) - f_int)/m and commits the state first (material.py:124-148).
+ *   load_scale   host [n_steps + 1]: f_ext(t_k) = load_scale[k] * f_unit
+ *   u, v, a      device [3 n_nodes] (node-major dofs), updated in place
+ *   state        device [8][8 n_elements] (element type by precision), in place
+ *   mass         device [3 n_nodes] lumped mass
+ *   fixed        device uint8 [3 n_nodes]: homogeneous Dirichlet dofs
+ *   f_unit       device [3 n_nodes]
+ *   stress       device [6][8 n_elements] of the final state, or NULL
+ *   internal_force device [3 n_nodes] of the final state, or NULL
+ *   history      host [(initialize + n_steps)][CIA_N_HISTORY]
+ */
+cia_status cia_box_explicit_steps(cia_box* box, void* stream, int32_t initialize,
+                                  double time_step, int32_t n_steps, const double* load_scale,
+                                  const cia_update_settings* settings, double* u, double* v,
+                                  double* a, void* state, const double* mass,
+                                  const uint8_t* fixed, const double* f_unit, double* stress,
+                                  double* internal_force, double* history,
+                                  cia_point_failure* failure);
+
 /* Kernel timing with HIP events on the launch stream (no reference
  * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element
  * pass, 2 PCG node gather, 3 PCG vector update, 4 preconditioner (spectral: FFT,

@@ -1,0 +1,150 @@
+"""Oracle: explicit velocity-Verlet material dynamics (FP64).
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Restates ``solve_material_explicit`` (``src/fem/dynamics/material.py:89-297``):
+initial assembly at u0 with zero rate and state commit (``:124-148``), the step
+loop (``:164-284``: predictor, assembly from the committed state, acceleration,
+velocity, Dirichlet zeroing), dissipation and external-work bookkeeping
+(``:221-229``), ``_record_step`` energies (``:781-848``) and the lumped kinetic
+energy (``:851-856``); CFL check ``check_explicit_stability``
+(``src/fem/dynamics/stability.py:34-67``); uniform boundary pressure
+(``src/fem/assembly/load.py:17-58``) and the half-sine pulse of
+``src/concrete_impact/benchmarks/dynamic_viscoplastic.py:147-170``.
+
+Pinning (tests/test_oracle_dynamics.py): an elastic bar's energy balance
+(kinetic + free - external work = 0 to the Verlet truncation order), momentum
+balance of the free-free bar, the reference's explicit acceptance bound
+(energy residual < 0.05 of the external work, ``test_plasticity_benchmarks.py:58-75``),
+dissipation >= 0 and the CFL rejection (``test_beam_response_benchmarks.py:70``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import hex8, j2vp
+
+
+def box_faces(divisions):
+    """Node ids of the six faces of a structured box (node id = i + (nx+1)(j + (ny+1)k))."""
+    nx, ny, nz = divisions
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    ids = np.arange(i.size)
+    return {"left": ids[i == 0], "right": ids[i == nx], "front": ids[j == 0], "back": ids[j == ny],
+            "bottom": ids[k == 0], "top": ids[k == nz]}
+
+
+def top_patch_unit_pressure(lengths, divisions, patch):
+    """Equivalent nodal force of a unit pressure on the z = top face quads whose
+    element indices lie in ``patch = ((i0, i1), (j0, j1))`` (outward normal +z,
+    traction = -p n, area / 4 to each corner: ``load.py:17-58``)."""
+    nx, ny, nz = divisions
+    hx, hy = lengths[0] / nx, lengths[1] / ny
+    f = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    (i0, i1), (j0, j1) = patch
+    for j in range(j0, j1):
+        for i in range(i0, i1):
+            for di, dj in ((0, 0), (1, 0), (0, 1), (1, 1)):
+                node = (i + di) + (nx + 1) * ((j + dj) + (ny + 1) * nz)
+                f[3 * node + 2] += -1.0 * hx * hy / 4.0
+    return f
+
+
+def half_sine(amplitude, duration):
+    def scale(t):
+        return amplitude * np.sin(np.pi * t / duration) if 0.0 <= t <= duration else 0.0
+    return scale
+
+
+class Box:
+    def __init__(self, lengths, divisions, phases, element_phase_ids=None):
+        self.lengths, self.divisions = lengths, divisions
+        self.nodes, self.elements = hex8.structured_box(lengths, divisions)
+        self.b, self.jw = hex8.geometry_cache(self.nodes, self.elements)
+        self.phases = phases
+        ne = self.elements.shape[0]
+        self.eph = np.zeros(ne, dtype=np.int64) if element_phase_ids is None else np.asarray(element_phase_ids)
+        self.ppid = np.repeat(self.eph, 8)
+        self.phased = element_phase_ids is not None
+        rho = np.asarray([m.density for _, m in phases])[self.eph]
+        self.mass = hex8.lumped_mass(self.nodes, self.elements, rho)
+        self.n_dof = 3 * self.nodes.shape[0]
+
+    def init_state(self):
+        st = {}
+        for pid, (name, mat) in enumerate(self.phases):
+            if isinstance(mat, j2vp.J2VPParams):
+                cnt = int(np.sum(self.ppid == pid))
+                pre = f"phase_{name}__" if self.phased else ""
+                st[pre + "plastic_strain"] = np.zeros((cnt, 6))
+                st[pre + "equivalent_plastic_strain"] = np.zeros(cnt)
+                st[pre + "viscoplastic_multiplier"] = np.zeros(cnt)
+        return st
+
+    def assemble(self, u, u_prev, dt, settings, state):
+        eps, rate = hex8.strain_fields(self.elements, self.b, u, u_prev, dt)
+        if self.phased:
+            # The reference's phased model raises for elastic phases without a tangent
+            # (phased.py:178-179), so explicit dynamics of phased media needs the tangent
+            # for the elastic-phase diagnostics; the GPU build evaluates them directly.
+            r = j2vp.phased_update(self.phases, self.ppid, eps, state, dt, settings, rate, True)
+        else:
+            mat = self.phases[0][1]
+            if isinstance(mat, j2vp.J2VPParams):
+                r = j2vp.j2vp_update(mat, eps, state["plastic_strain"], state["equivalent_plastic_strain"],
+                                     state["viscoplastic_multiplier"], dt, settings, rate, False)
+                r["state"] = {k: r[k] for k in ("plastic_strain", "equivalent_plastic_strain",
+                                                "viscoplastic_multiplier")}
+            else:
+                r = j2vp.elastic_update(mat, eps, dt, rate, False, False)
+                r["state"] = {}
+        r["internal_force"] = hex8.internal_force(self.elements, self.b, self.jw, r["stress"], self.n_dof)
+        return r
+
+
+def explicit(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, cfl=None, wave_speed=None):
+    """Velocity Verlet; returns histories like ``MaterialDynamicSolution``."""
+    if cfl is not None:
+        h = min(box.lengths[i] / box.divisions[i] for i in range(3))
+        if dt > cfl * h / wave_speed:
+            raise ValueError("Explicit time step violates the CFL stability bound")
+    n = box.n_dof
+    u = np.zeros(n)
+    v = np.zeros(n)
+    state = box.init_state()
+    asm = box.assemble(u, u, dt, settings, state)
+    state = asm["state"]
+    ext = scale(0.0) * f_unit
+    a = (ext - asm["internal_force"]) / box.mass
+    a[fixed_dofs] = 0.0
+    w = box.jw.reshape(-1)
+    hist = {"u": [u.copy()], "v": [v.copy()], "a": [a.copy()], "kinetic": [0.5 * np.sum(box.mass * v**2)],
+            "free": [np.sum(asm["free_energy"] * w)], "dissipated": [0.0], "external_work": [0.0],
+            "stress": [asm["stress"].copy()], "internal_force": [asm["internal_force"].copy()]}
+    for s in range(n_steps):
+        t1 = (s + 1) * dt
+        ext1 = scale(t1) * f_unit
+        un = u + dt * v + 0.5 * dt**2 * a
+        un[fixed_dofs] = 0.0
+        asm = box.assemble(un, u, dt, settings, state)
+        an = (ext1 - asm["internal_force"]) / box.mass
+        an[fixed_dofs] = 0.0
+        vn = v + 0.5 * dt * (a + an)
+        vn[fixed_dofs] = 0.0
+        hist["dissipated"].append(hist["dissipated"][-1] + dt * np.sum(asm["dissipation"] * w))
+        hist["external_work"].append(hist["external_work"][-1] + 0.5 * (ext + ext1) @ (un - u))
+        u, v, a, ext, state = un, vn, an, ext1, asm["state"]
+        hist["u"].append(u.copy())
+        hist["v"].append(v.copy())
+        hist["a"].append(a.copy())
+        hist["kinetic"].append(0.5 * np.sum(box.mass * v**2))
+        hist["free"].append(np.sum(asm["free_energy"] * w))
+        hist["stress"].append(asm["stress"].copy())
+        hist["internal_force"].append(asm["internal_force"].copy())
+    out = {k: np.asarray(v) for k, v in hist.items()}
+    out["energy_residual"] = (out["kinetic"] + out["free"] + out["dissipated"] - out["external_work"]
+                              - (out["kinetic"][0] + out["free"][0]))
+    out["state"] = state
+    return out
