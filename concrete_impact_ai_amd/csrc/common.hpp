@@ -74,6 +74,16 @@ inline unsigned grid_for(int64_t n, int block = kBlock) {
   return (unsigned)((n + block - 1) / block);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD
+// b mod 8), and each XCD has its own L2.  Stencil kernels renumber their blocks so
+// that each XCD walks one contiguous range of the grid: the neighbour rows and
+// planes that adjacent elements share then come from that XCD's own L2.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 // ---- deterministic block reductions --------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(kBlock)
                double* __restrict__ hist_prev_row, unsigned long long* __restrict__ first_fail) {
   __shared__ double lds[kBlock / 64];
   if (hist_prev_row && blockIdx.x == 0) fold_node(part_n_prev, Pn, hist_prev_row, lds);
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t e = p >> 3;
   const int q = (int)(p & 7);
   double vals[6] = {0, 0, 0, 0, 0, 0};
@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kBlock)
                double jw) {
   __shared__ double lds[kBlock / 64];
   if (blockIdx.x == 0) fold_elem(part_e, Pe, jw, hist_row, lds);
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   double vals[2] = {0.0, 0.0};
   if (n < G.N) {
     const int nx1 = G.nx + 1, ny1 = G.ny + 1;

@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kBlock)
                T* __restrict__ strain_out, T* __restrict__ rate_out, T* __restrict__ psi_out,
                T* __restrict__ diss_out, T* __restrict__ diag_out,
                unsigned long long* __restrict__ first_fail) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t e = p >> 3;
   const int q = (int)(p & 7);
   if (e >= G.E) return;  // whole 8-lane groups leave together
@@ -887,7 +887,7 @@ __global__ void __launch_bounds__(kBlock)
               double* __restrict__ part_pq, const Scalars* __restrict__ sc) {
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   double pq = 0.0;
   if (c < G.E) {
     const unsigned cu = (unsigned)c, rr = cu / (unsigned)G.nx;
@@ -969,7 +969,7 @@ __global__ void __launch_bounds__(kBlock)
     sc->rz = rz[0];
     sc->beta = beta;
   }
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t e = p >> 3;
   const int q = (int)(p & 7);
   if (e >= G.E) return;
@@ -1297,12 +1297,13 @@ struct State {
   bool on = false;
   std::vector<hipEvent_t> ev;
   std::vector<int> ids;
+  std::vector<int> tags;  // PCG iteration index of the launch, -1 for other launches
   size_t used = 0;
   double total_ms[N_IDS] = {0};
   int64_t count[N_IDS] = {0};
 };
 static State g;
-static void begin(int id, hipStream_t st) {
+static void begin(int id, hipStream_t st, int tag = -1) {
   if (!g.on) return;
   if (g.ev.size() < 2 * (g.used + 1)) {
     hipEvent_t a, b;
@@ -1311,8 +1312,10 @@ static void begin(int id, hipStream_t st) {
     g.ev.push_back(a);
     g.ev.push_back(b);
     g.ids.push_back(0);
+    g.tags.push_back(-1);
   }
   g.ids[g.used] = id;
+  g.tags[g.used] = tag;
   (void)hipEventRecord(g.ev[2 * g.used], st);
 }
 static void end(hipStream_t st) {
@@ -1320,11 +1323,14 @@ static void end(hipStream_t st) {
   (void)hipEventRecord(g.ev[2 * g.used + 1], st);
   ++g.used;
 }
-// Call only after the stream has been synchronised.
-static void flush() {
+// Call only after the stream has been synchronised.  Launches tagged with a PCG
+// iteration index >= active_iters exited at once on the converged flag and are
+// not counted.
+static void flush(int active_iters = 1 << 30) {
   if (!g.on) return;
   for (size_t i = 0; i < g.used; ++i) {
     float ms = 0.f;
+    if (g.tags[i] >= active_iters) continue;
     if (hipEventSynchronize(g.ev[2 * i + 1]) == hipSuccess &&
         hipEventElapsedTime(&ms, g.ev[2 * i], g.ev[2 * i + 1]) == hipSuccess) {
       g.total_ms[g.ids[i]] += ms;
@@ -1389,6 +1395,8 @@ struct cia_rve {
   double* red = nullptr;  // response partials
   int64_t red_len = 0;
   int pcg_total = 0;
+  static constexpr int kHints = 32;
+  int its_hint[kHints] = {0};  // PCG iterations of Newton iteration k in the last increment
   int cur = 0;
   bool have_fields = false;
   // spectral preconditioner
@@ -1457,7 +1465,7 @@ struct Ops {
 
   // Solve K x = b with the spectral reference-medium preconditioner.
   static cia_status pcg_spectral(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
-                                 int* iters_out, int* flag_out, double rtol) {
+                                 int* iters_out, int* flag_out, double rtol, int hint) {
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
     const int P = (int)gn;
@@ -1492,23 +1500,29 @@ struct Ops {
     prof::end(st);
     CIA_HIP(hipGetLastError());
     int done = 0, it = 0, pi = 0;
-    const int chunk = std::max(1, R->lin.check_interval);
+    // Launches after convergence exit at once but the library FFTs still run, so
+    // the first batch is sized by the iteration count of the same Newton index in
+    // the previous increment; later batches are short.
+    const int interval = std::max(1, R->lin.check_interval);
+    const bool hinted = hint >= 0 && hint < cia_rve::kHints && R->its_hint[hint] > 0;
+    int chunk = hinted ? R->its_hint[hint] : interval;
     Scalars hs;
     while (it < R->lin.max_iterations) {
       const int n = std::min(chunk, R->lin.max_iterations - it);
+      chunk = hinted ? std::max(1, interval / 4) : interval;
       for (int s = 0; s < n; ++s, ++it) {
         double* p_old = R->pp[1 - pi];
         double* p_new = R->pp[pi];
-        prof::begin(prof::PCG_ELEM, st);
+        prof::begin(prof::PCG_ELEM, st, it);
         hipLaunchKernelGGL(k_sp_elem<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab, R->d_ephase,
                            (const T*)R->tan[slot], (const hipDoubleComplex*)R->uab, p_old, p_new,
                            (T*)R->F, (const double*)part_rz, P, R->sc, it);
         prof::end(st);
-        prof::begin(prof::PCG_NODE, st);
+        prof::begin(prof::PCG_NODE, st, it);
         hipLaunchKernelGGL(k_sp_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F, p_new,
                            R->py, part_pq, R->sc);
         prof::end(st);
-        prof::begin(prof::PCG_UPDATE, st);
+        prof::begin(prof::PCG_UPDATE, st, it);
         hipLaunchKernelGGL(k_sp_update, dim3(gn), dim3(kBlock), 0, st, C, p_new, R->py, x, R->pr,
                            R->uab, (const double*)part_pq, P, part_upd, R->sc, it);
         prof::end(st);
@@ -1522,8 +1536,8 @@ struct Ops {
       CIA_HIP(hipGetLastError());
       CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
       CIA_HIP(hipStreamSynchronize(st));
-      prof::flush();
       done = hs.done;
+      prof::flush(done ? hs.iters : 1 << 30);
       if (done) break;
     }
     if (!done) {
@@ -1533,15 +1547,17 @@ struct Ops {
     *iters_out = hs.iters;
     *flag_out = hs.done;
     R->pcg_total += hs.iters;
+    if (hint >= 0 && hint < cia_rve::kHints && hs.done == 1) R->its_hint[hint] = hs.iters;
     return CIA_OK;
   }
 
-  // Solve K x = b (K from tan[slot]); returns iterations.
+  // Solve K x = b (K from tan[slot]); returns iterations.  `hint` indexes the
+  // per-Newton-iteration history of PCG counts (-1: none).
   static cia_status pcg(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
-                        int* iters_out, int* flag_out, double rtol = -1.0) {
+                        int* iters_out, int* flag_out, double rtol = -1.0, int hint = -1) {
     if (rtol <= 0.0) rtol = R->lin.relative_tolerance;
     if (R->prec == CIA_PCG_SPECTRAL && R->fft_ready)
-      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out, rtol);
+      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out, rtol, hint);
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
     prof::begin(prof::PRECOND, st);
@@ -1766,7 +1782,7 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
       lin_rtol = std::max(lin_rtol, std::max(eta, 0.5 * rtol / std::max(rn, 1e-300)));
       lin_rtol = std::min(lin_rtol, 0.5);
     }
-    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag, lin_rtol));
+    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag, lin_rtol, it));
     double dn2 = 0.0;
     CIA_TRY(norm2_of(R, st, C3, R->dw, &dn2));
     const double inc_n = sqrt(dn2);
