@@ -298,6 +298,12 @@ _SIGNATURES.update({
          C.POINTER(UpdateSettings), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
          C.POINTER(PointFailure)],
     ),
+    "cia_box_generic_strains": (
+        C.c_int32, [_vp, _vp, C.c_int32, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cia_box_generic_correct": (
+        C.c_int32,
+        [_vp, _vp, C.c_int32, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+         _vp, _vp, _vp, _vp, _vp, _vp]),
 })
 
 PROFILE_SLOTS = ["assemble", "pcg_elem", "pcg_node", "pcg_update", "precond", "gather"]

@@ -277,6 +277,100 @@ __global__ void k_fold_node_only(const double* __restrict__ part_n, int P, doubl
   fold_node(part_n, P, hist_row, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Generic-material step: the caller evaluates the material between the strain
+// pass and the force pass (assemble_bundle_material_response,
+// src/fem/assembly/nonlinear_solid.py:222-300, with any MaterialModel).
+
+// Strains (and rates) at the Verlet predictor u+ (PREDICT) or at u itself.
+template <bool PREDICT>
+__global__ void __launch_bounds__(kBlock)
+    k_box_strain(BoxGrid G, double dt, const double* __restrict__ u, const double* __restrict__ v,
+                 const double* __restrict__ a, const uint8_t* __restrict__ fixed,
+                 double* __restrict__ strain, double* __restrict__ rate) {
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  int64_t nd[8];
+  box_corners(e, G.nx, G.ny, nd);
+  double bx[8], by[8], bz[8];
+  gp_factors<double>(q, G.g, bx, by, bz);
+  double un[8][3], du[8][3];
+  const double h2 = 0.5 * dt * dt;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * nd[c] + m;
+      const double u0 = u[d];
+      double u1 = u0;
+      if (PREDICT) u1 = fixed[d] ? 0.0 : u0 + dt * v[d] + h2 * a[d];
+      un[c][m] = u1;
+      du[c][m] = u1 - u0;
+    }
+  double eps[6], rt[6];
+  strain_from(un, bx, by, bz, eps);
+  strain_from(du, bx, by, bz, rt);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    strain[m * np + p] = eps[m];
+    if (rate) rate[m * np + p] = rt[m] / dt;
+  }
+}
+
+// Element force rows from caller stresses [6][n_gp] and the energy partials of
+// k_dyn_elem from caller point arrays (each may be NULL: contributes nothing).
+__global__ void __launch_bounds__(kBlock)
+    k_box_force(BoxGrid G, const double* __restrict__ stress, const double* __restrict__ psi,
+                const double* __restrict__ diss, const double* __restrict__ work,
+                const double* __restrict__ active, const double* __restrict__ margin,
+                double* __restrict__ F, double* __restrict__ part_e) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  double vals[6] = {0, 0, 0, 0, 0, 0};
+  double mn = INFINITY, mx = -INFINITY;
+  if (e < G.E) {
+    const int64_t np = G.E * 8;
+    double bx[8], by[8], bz[8];
+    gp_factors<double>(q, G.g, bx, by, bz);
+    double sg[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) sg[m] = stress[m * np + p];
+    if (psi) vals[0] = psi[p];
+    if (diss) vals[1] = diss[p];
+    if (work) {
+      const double pw = work[p];
+      if (isfinite(pw)) {
+        vals[2] = fmax(pw, 0.0);
+        vals[3] = fmin(pw, 0.0);
+        vals[4] = pw < 0.0 ? 1.0 : 0.0;
+      }
+    }
+    if (active) {
+      const double ac = active[p];  // a flag for point materials, a volume fraction for RVEs
+      if (isfinite(ac)) vals[5] = ac;
+    }
+    if (margin) {
+      const double mg = margin[p];
+      if (isfinite(mg)) {
+        mn = mg;
+        mx = mg;
+      }
+    }
+    double f[8][3];
+    force_from(sg, G.g.jw, bx, by, bz, f);
+    double fin[3];
+    element_reduce8(q, f, fin);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
+  }
+  block_partials<kBlock>(vals, 6, &mn, &mx, part_e, gridDim.x, lds);
+}
+
 }  // namespace cia
 
 struct cia_box {
@@ -469,6 +563,53 @@ cia_status cia_box_explicit_steps(cia_box* B, void* stream, int32_t initialize, 
     }
     return CIA_ERR_POINT_FAILURE;
   }
+  return CIA_OK;
+}
+
+cia_status cia_box_generic_strains(cia_box* B, void* stream, int32_t predict, double time_step,
+                                   const double* u, const double* v, const double* a,
+                                   const uint8_t* fixed, double* strain, double* rate) {
+  using namespace cia;
+  CIA_ARG(B && u && strain, "NULL argument to cia_box_generic_strains");
+  CIA_ARG(!predict || (v && a && fixed), "the predictor needs v, a and the fixed mask");
+  CIA_ARG(time_step > 0.0, "time step must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned ge = grid_for(8 * B->G.E);
+  if (predict)
+    hipLaunchKernelGGL((k_box_strain<true>), dim3(ge), dim3(kBlock), 0, st, B->G, time_step, u, v,
+                       a, fixed, strain, rate);
+  else
+    hipLaunchKernelGGL((k_box_strain<false>), dim3(ge), dim3(kBlock), 0, st, B->G, time_step, u,
+                       v, a, fixed, strain, rate);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_box_generic_correct(cia_box* B, void* stream, int32_t initialize, double time_step,
+                                   double load_scale, const double* stress, const double* psi,
+                                   const double* diss, const double* work, const double* active,
+                                   const double* margin, double* u, double* v, double* a,
+                                   const double* mass, const uint8_t* fixed, const double* f_unit,
+                                   double* internal_force, double* history_row) {
+  using namespace cia;
+  CIA_ARG(B && stress && u && v && a && mass && fixed && f_unit && history_row,
+          "NULL argument to cia_box_generic_correct");
+  CIA_ARG(time_step > 0.0, "time step must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned ge = grid_for(8 * B->G.E), gn = grid_for(B->G.N);
+  hipLaunchKernelGGL(k_box_force, dim3(ge), dim3(kBlock), 0, st, B->G, stress, psi, diss, work,
+                     active, margin, B->F, B->part_e);
+  if (initialize)
+    hipLaunchKernelGGL((k_dyn_node<true>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
+                       mass, fixed, f_unit, load_scale, u, v, a, internal_force, B->part_n,
+                       B->part_e, (int)ge, history_row, B->G.g.jw);
+  else
+    hipLaunchKernelGGL((k_dyn_node<false>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
+                       mass, fixed, f_unit, load_scale, u, v, a, internal_force, B->part_n,
+                       B->part_e, (int)ge, history_row, B->G.g.jw);
+  hipLaunchKernelGGL(k_fold_node_only, dim3(1), dim3(kBlock), 0, st, B->part_n, (int)gn,
+                     history_row);
+  CIA_HIP(hipGetLastError());
   return CIA_OK;
 }
 

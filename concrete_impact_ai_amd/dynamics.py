@@ -216,9 +216,12 @@ def _phases(material):
 def lumped_mass(bundle) -> np.ndarray:
     """Row-sum lumped mass (``assembly/elasticity.py:73-77``): each box corner
     receives rho V_e / 8 (exact for trilinear shape functions)."""
-    phases, phased = _phases(bundle.material)
     n_el = bundle.mesh_info.elements.shape[0]
-    eph = (np.zeros(n_el, dtype=np.int64) if bundle.element_phase_ids is None
+    try:
+        phases, phased = _phases(bundle.material)
+    except TypeError:  # any other MaterialModel: one density
+        phases = [("", bundle.material)]
+    eph = (np.zeros(n_el, dtype=np.int64) if bundle.element_phase_ids is None or len(phases) == 1
            else np.asarray(bundle.element_phase_ids))
     rho = np.asarray([m.density for _, m in phases])[eph]
     ve = float(np.prod(np.asarray(bundle.lengths) / np.asarray(bundle.divisions)))
@@ -240,12 +243,15 @@ def check_explicit_stability(bundle, time_settings):
 
 
 class _BoxHandle:
-    def __init__(self, bundle, precision):
-        phases, phased = _phases(bundle.material)
+    def __init__(self, bundle, precision, geometry_only=False):
+        if geometry_only:  # generic materials: the phase table is unused
+            phases, phased = [("", LinearElasticMaterial("geometry", 1.0, 1.0, 0.25))], False
+        else:
+            phases, phased = _phases(bundle.material)
         self.phases, self.phased = phases, phased
         mats = (_lib.Material * len(phases))(*[cia_material(m) for _, m in phases])
         n_el = bundle.mesh_info.elements.shape[0]
-        eph = (np.zeros(n_el, dtype=np.int32) if bundle.element_phase_ids is None
+        eph = (np.zeros(n_el, dtype=np.int32) if bundle.element_phase_ids is None or not phased
                else np.ascontiguousarray(np.asarray(bundle.element_phase_ids, dtype=np.int32)))
         self._eph = eph
         desc = _lib.BoxDesc()
@@ -277,8 +283,14 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
                             update_settings, progress_callback=None, residual_corrector=None,
                             correction_controls=None, record_material_state_history=False, *,
                             history_stride: int = 1, record_fields: bool = True,
-                            precision: str = "fp64") -> MaterialDynamicSolution:
-    """Explicit velocity Verlet (``src/fem/dynamics/material.py:89-297``)."""
+                            precision: str = "fp64", generic: bool = False) -> MaterialDynamicSolution:
+    """Explicit velocity Verlet (``src/fem/dynamics/material.py:89-297``).
+
+    Point materials (J2 Perzyna, linear elastic, phased) run the fused device time
+    loop; any other material with a ``device_kernel`` (the FE² RVE adapter, the
+    surrogate) runs the generic loop, which evaluates the material between the
+    device strain pass and force/node pass of every step (``generic=True`` forces
+    it for point materials too)."""
     if residual_corrector is not None or correction_controls is not None:
         raise NotImplementedError("The velocity-Verlet residual corrector is not part of this path.")
     if kinematics != "three_dimensional":
@@ -289,6 +301,14 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
         raise TypeError("GPU explicit dynamics needs a load of the form scale(t) * unit_force "
                         "(SeparableLoad / build_half_sine_pressure_load).")
     check_explicit_stability(bundle, time_settings)
+    point_material = isinstance(bundle.material, (J2ViscoplasticMaterial, LinearElasticMaterial,
+                                                  PhasedMaterialModel))
+    if generic or not point_material:
+        if precision != "fp64":
+            raise ValueError("The generic-material explicit loop runs in FP64.")
+        return _solve_explicit_generic(bundle, time_settings, initial_displacement,
+                                       initial_velocity, dirichlet, load_function, update_settings,
+                                       progress_callback, history_stride, record_fields)
     start = perf_counter()
     dt = float(time_settings.time_step)
     n_steps = int(time_settings.num_steps)
@@ -412,6 +432,139 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
         material_diagnostics={},
         integrator_diagnostics={},
         final_state=final_state,
+        solve_time=perf_counter() - start,
+        material_state_history=None,
+        recorded_steps=np.asarray(rec["steps"]) if record_fields else None,
+    )
+
+
+def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial_velocity,
+                            dirichlet, load_function, update_settings, progress_callback,
+                            history_stride, record_fields) -> MaterialDynamicSolution:
+    """Velocity Verlet with the material evaluated by the caller-side kernel each
+    step (``material.py:124-284``): strain pass at the predictor, material update
+    from the committed state (explicit: every step is committed), force and node
+    pass, energy terms into a device history row."""
+    material = bundle.material
+    if not hasattr(material, "device_kernel"):
+        raise TypeError(f"{type(material).__name__} has no device kernel")
+    start = perf_counter()
+    dt = float(time_settings.time_step)
+    n_steps = int(time_settings.num_steps)
+    times = np.arange(n_steps + 1) * dt
+    scales = np.asarray([float(load_function.scale(t)) for t in times])
+    dev = device()
+    box = _BoxHandle(bundle, "fp64", geometry_only=True)
+    n_dof = bundle.mesh_info.dof_map.size
+    n_gp = 8 * bundle.mesh_info.elements.shape[0]
+    ppid = (None if bundle.element_phase_ids is None
+            else np.repeat(np.asarray(bundle.element_phase_ids), 8))
+    kernel = material.device_kernel(ppid)
+    fixed_np = np.zeros(n_dof, dtype=np.uint8)
+    fixed_np[np.asarray(dirichlet.dofs, dtype=np.int64)] = 1
+    u0 = np.asarray(initial_displacement, dtype=np.float64).copy()
+    v0 = np.asarray(initial_velocity, dtype=np.float64).copy()
+    u0[fixed_np == 1] = 0.0
+    v0[fixed_np == 1] = 0.0
+    u = torch.as_tensor(u0, device=dev)
+    v = torch.as_tensor(v0, device=dev)
+    a = torch.zeros(n_dof, dtype=torch.float64, device=dev)
+    mass = torch.as_tensor(lumped_mass(bundle), device=dev)
+    fixed = torch.as_tensor(fixed_np, device=dev)
+    funit = torch.as_tensor(load_function.unit_force, device=dev)
+    strain = torch.empty((6, n_gp), dtype=torch.float64, device=dev)
+    rate = torch.empty((6, n_gp), dtype=torch.float64, device=dev)
+    fint = torch.zeros(n_dof, dtype=torch.float64, device=dev)
+    hist = torch.zeros((n_steps + 1, _lib.N_HISTORY), dtype=torch.float64, device=dev)
+    lib = _lib.lib()
+    st = stream_handle()
+    stride = max(1, int(history_stride)) if record_fields else max(n_steps, 1)
+    rec = {"u": [], "v": [], "a": [], "fint": [], "stress": [], "q": [], "steps": []}
+    state = kernel.initial_state(n_gp)
+    P = C.c_void_p
+
+    def ptr(t):
+        return P(None) if t is None else P(t.data_ptr())
+
+    def step(k, initialize):
+        _lib.check(lib.cia_box_generic_strains(
+            box.handle, st, 0 if initialize else 1, dt, ptr(u), ptr(v), ptr(a), ptr(fixed),
+            ptr(strain), ptr(rate)), "cia_box_generic_strains")
+        r = kernel.update(strain, rate, state, dt, update_settings, False)
+        _lib.check(lib.cia_box_generic_correct(
+            box.handle, st, 1 if initialize else 0, dt, float(scales[k]), ptr(r.stress),
+            ptr(r.free_energy), ptr(r.dissipation), ptr(r.work), ptr(r.active), ptr(r.margin),
+            ptr(u), ptr(v), ptr(a), ptr(mass), ptr(fixed), ptr(funit), ptr(fint),
+            P(hist[k].data_ptr())), "cia_box_generic_correct")
+        return r
+
+    def record(k, r):
+        if not record_fields or (k % stride and k != n_steps):
+            return
+        rec["u"].append(u.cpu().numpy().copy())
+        rec["v"].append(v.cpu().numpy().copy())
+        rec["a"].append(a.cpu().numpy().copy())
+        rec["fint"].append(fint.cpu().numpy().copy())
+        rec["stress"].append(r.stress.cpu().numpy().T.copy())
+        q = kernel.equivalent_plastic_strain(r.state) if hasattr(kernel, "equivalent_plastic_strain") else None
+        rec["q"].append(None if q is None else q.double().cpu().numpy().copy())
+        rec["steps"].append(k)
+
+    r = step(0, True)
+    state = r.state
+    record(0, r)
+    has_diag = r.work is not None and r.active is not None and r.margin is not None
+    for k in range(1, n_steps + 1):
+        r = step(k, False)
+        state = r.state
+        record(k, r)
+        if progress_callback is not None and (k % stride == 0 or k == n_steps):
+            progress_callback(SolverProgressEvent(
+                stage="material_dynamics", scheme="explicit", accepted_step=k,
+                nominal_total_steps=n_steps, physical_time=float(times[k]),
+                final_time=float(times[-1]), newton_iterations=0, armijo_backtracks=0,
+                elapsed_seconds=perf_counter() - start))
+    H = hist.cpu().numpy()
+    col = {name: H[:, i] for i, name in enumerate(_lib.HISTORY_COLUMNS)}
+    volume = float(np.prod(bundle.lengths))
+    ext_work = np.concatenate([[0.0], np.cumsum(0.5 * (scales[:-1] + scales[1:]) * col["ext_du"][1:])])
+    dissipated = np.concatenate([[0.0], np.cumsum(dt * col["dissipation_rate"][1:])])
+    kinetic, free = col["kinetic"], col["free_energy"]
+    residual = kinetic + free + dissipated - ext_work - (kinetic[0] + free[0])
+    zeros = np.zeros(n_steps + 1)
+    q_hist = None
+    if record_fields and all(q is not None for q in rec["q"]):
+        q_hist = np.asarray(rec["q"])
+    f_ext_hist = (np.asarray([scales[s] * load_function.unit_force for s in rec["steps"]])
+                  if record_fields else None)
+    return MaterialDynamicSolution(
+        times=times,
+        displacement=np.asarray(rec["u"]) if record_fields else None,
+        velocity=np.asarray(rec["v"]) if record_fields else None,
+        acceleration=np.asarray(rec["a"]) if record_fields else None,
+        internal_force=np.asarray(rec["fint"]) if record_fields else None,
+        external_force=f_ext_hist,
+        stresses=np.asarray(rec["stress"]) if record_fields else None,
+        equivalent_plastic_strain=q_hist,
+        kinetic_energy=kinetic,
+        free_energy=free,
+        dissipated_energy=dissipated,
+        external_work=ext_work,
+        energy_residual=residual,
+        positive_incremental_work=col["positive_work"] if has_diag else zeros,
+        negative_incremental_work=col["negative_work"] if has_diag else zeros,
+        unloading_volume_fraction=col["unloading_volume"] / volume if has_diag else zeros,
+        viscoplastic_active_volume_fraction=col["active_volume"] / volume if has_diag else zeros,
+        minimum_yield_activation_margin=col["margin_min"] if has_diag else zeros,
+        maximum_yield_activation_margin=col["margin_max"] if has_diag else zeros,
+        newton_iterations=np.zeros(n_steps + 1, dtype=np.int64),
+        accepted_time_steps=np.full(n_steps, dt),
+        rejected_step_counts=np.zeros(n_steps, dtype=np.int64),
+        armijo_backtracks=np.zeros(n_steps + 1, dtype=np.int64),
+        rejected_reasons=tuple(() for _ in range(n_steps)),
+        material_diagnostics={},
+        integrator_diagnostics={},
+        final_state=kernel.material_state(state),
         solve_time=perf_counter() - start,
         material_state_history=None,
         recorded_steps=np.asarray(rec["steps"]) if record_fields else None,

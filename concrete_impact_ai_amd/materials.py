@@ -158,6 +158,9 @@ class J2ViscoplasticMaterial:
             }
         )
 
+    def device_kernel(self, point_phase_ids=None) -> "PointMaterialKernel":
+        return PointMaterialKernel(self, point_phase_ids)
+
     def update(self, request, state, requirements=DEFAULT_RESPONSE_REQUIREMENTS):
         return _update_host([("", self)], None, request, state, requirements, phased=False)
 
@@ -179,6 +182,9 @@ class LinearElasticMaterial:
 
     def initialize_state(self, n_points: int) -> MaterialState:
         return MaterialState(variables={})
+
+    def device_kernel(self, point_phase_ids=None) -> "PointMaterialKernel":
+        return PointMaterialKernel(self, point_phase_ids)
 
     def update(self, request, state, requirements=DEFAULT_RESPONSE_REQUIREMENTS):
         return _update_host([("", self)], None, request, state, requirements, phased=False)
@@ -216,6 +222,9 @@ class PhasedMaterialModel:
             for key, values in phase.material.initialize_state(count).variables.items():
                 variables[encode_phase_state_key(phase.name, key)] = values
         return MaterialState(variables)
+
+    def device_kernel(self, point_phase_ids=None) -> "PointMaterialKernel":
+        return PointMaterialKernel(self, point_phase_ids)
 
     def update(self, request, state, requirements=DEFAULT_RESPONSE_REQUIREMENTS):
         if request.strains.shape[0] != self.point_phase_ids.size:
@@ -573,3 +582,79 @@ def _update_host(phases, point_phase_ids, request, state, requirements, phased: 
         diagnostics={} if single_elastic else diagnostics_dict(
             phases, point_phase_ids, diag, request.strain_rates is not None, phased),
     )
+
+
+# ---------------------------------------------------------------------------
+# Device-resident material protocol used by the generic macro drivers
+# (explicit dynamics with any MaterialModel, Newmark).  A material exposes
+# ``device_kernel(point_phase_ids)`` returning an object with
+#   initial_state(n) -> opaque committed state
+#   update(strain [6][n], rate [6][n] | None, state, dt, settings, tangent) -> DeviceResponse
+#   material_state(state) -> MaterialState (host view with the reference keys)
+
+
+@dataclass
+class DeviceResponse:
+    """Point responses on the device (FP64 SoA): stress [6][n], tangent [36][n]
+    (row-major 6x6 per point) or None, and the [n] point arrays that the energy
+    histories integrate (free energy, dissipation, incremental work density,
+    viscoplastic-active value, yield activation margin), each possibly None."""
+
+    stress: torch.Tensor
+    state: Any
+    tangent: torch.Tensor | None = None
+    free_energy: torch.Tensor | None = None
+    dissipation: torch.Tensor | None = None
+    work: torch.Tensor | None = None
+    active: torch.Tensor | None = None
+    margin: torch.Tensor | None = None
+
+
+class PointMaterialKernel:
+    """Device kernel of the point materials (J2 Perzyna, linear elastic, phased)."""
+
+    def __init__(self, material, point_phase_ids=None):
+        if isinstance(material, PhasedMaterialModel):
+            self.phases = [(p.name, p.material) for p in material.phases]
+            self.phased = True
+            ppid = np.asarray(material.point_phase_ids if point_phase_ids is None else point_phase_ids)
+        elif isinstance(material, (J2ViscoplasticMaterial, LinearElasticMaterial)):
+            self.phases = [("", material)]
+            self.phased = False
+            ppid = None
+        else:
+            raise TypeError(f"{type(material).__name__} is not a point material")
+        self.mats = [m for _, m in self.phases]
+        self.has_j2 = any(isinstance(m, J2ViscoplasticMaterial) for m in self.mats)
+        self.ppid = ppid
+        self._pid_t = None if ppid is None else torch.as_tensor(ppid.astype(np.int32), device=device())
+
+    def initial_state(self, n: int):
+        return torch.zeros((8, n), dtype=torch.float64, device=device()) if self.has_j2 else None
+
+    def update(self, strain, rate, state, dt, settings, tangent=False) -> DeviceResponse:
+        stress, st_out, tan, fe, di, dg, fail = update_device(
+            self.mats, self._pid_t, strain, rate, state, dt, settings,
+            tangent=tangent, energy=True, diagnostics=True)
+        if fail is not None:
+            p = int(fail.point_id)
+            ph = self.mats[0] if self.ppid is None else self.mats[int(self.ppid[p])]
+            req = type("Req", (), {"time_step": dt, "kinematics": "three_dimensional",
+                                   "update_settings": settings})
+            raise point_failure_error(fail, ph, req, p, ph)
+        d = {n: i for i, n in enumerate(_lib.DIAG_NAMES)}
+        has_diag = self.has_j2 or self.phased
+        return DeviceResponse(
+            stress=stress, state=st_out if self.has_j2 else None, tangent=tan,
+            free_energy=fe, dissipation=di,
+            work=dg[d["incremental_work_density"]].contiguous() if has_diag else None,
+            active=dg[d["viscoplastic_active"]].contiguous() if has_diag else None,
+            margin=dg[d["yield_activation_margin"]].contiguous() if has_diag else None)
+
+    def material_state(self, state) -> MaterialState:
+        if state is None:
+            return MaterialState({})
+        return MaterialState(unpack_state(self.phases, self.ppid, state.cpu().numpy(), self.phased))
+
+    def equivalent_plastic_strain(self, state):
+        return None if state is None else state[6]

@@ -414,15 +414,29 @@ def solve_rve_microequilibrium(model: StructuredHex8RVE, request: RVERequest,
     packed, w = _device_state(cache, committed_state)
     next_packed = torch.empty_like(packed)
     next_w = torch.empty_like(w)
+    res = solve_increment_raw(cache, stream_handle(), packed.data_ptr(), committed_state.macro_strain,
+                              w.data_ptr(), macro, float(request.time_step),
+                              request.material_update_settings, bool(request.requirements.tangent),
+                              next_packed.data_ptr(), next_w.data_ptr())
+    return _build_response(cache, request, committed_state, res, next_packed, next_w, macro,
+                           retain_micro_fields)
+
+
+def solve_increment_raw(cache: RVEOperatorCache, stream, packed_ptr: int, committed_macro,
+                        w_ptr: int, macro, time_step: float, update_settings, need_tangent: bool,
+                        next_packed_ptr: int, next_w_ptr: int) -> _lib.RVEResult:
+    """One increment on raw device buffers (the committed state [8][n_points] and
+    fluctuation w [3 n_classes] in, the trial state and w out); raises the
+    reference's exceptions.  Shared by the single-RVE API and the FE² adapter."""
     res = _lib.RVEResult()
-    cmacro = (C.c_double * 6)(*[float(v) for v in committed_state.macro_strain])
+    macro = np.asarray(macro, dtype=np.float64)
+    cmacro = (C.c_double * 6)(*[float(v) for v in committed_macro])
     rmacro = (C.c_double * 6)(*[float(v) for v in macro])
-    settings = cia_settings(request.material_update_settings)
+    settings = cia_settings(update_settings)
     status = _lib.lib().cia_rve_solve(
-        cache.handle, stream_handle(), C.c_void_p(packed.data_ptr()), cmacro,
-        C.c_void_p(w.data_ptr()), rmacro, float(request.time_step), C.byref(settings),
-        int(bool(request.requirements.tangent)), C.c_void_p(next_packed.data_ptr()),
-        C.c_void_p(next_w.data_ptr()), C.byref(res),
+        cache.handle, stream, C.c_void_p(packed_ptr), cmacro, C.c_void_p(w_ptr), rmacro,
+        float(time_step), C.byref(settings), int(bool(need_tangent)),
+        C.c_void_p(next_packed_ptr), C.c_void_p(next_w_ptr), C.byref(res),
     )
     _lib.check(status, "cia_rve_solve")
     if status == _lib.CIA_ERR_POINT_FAILURE:
@@ -431,8 +445,8 @@ def solve_rve_microequilibrium(model: StructuredHex8RVE, request: RVERequest,
         mats = [m for _, m in cache.phases]
         phase = int(cache.point_phase_ids[p])
         local = int(np.count_nonzero(cache.point_phase_ids[:p] == phase)) if cache.phased else p
-        mreq = type("Req", (), {"time_step": request.time_step, "kinematics": "three_dimensional",
-                                "update_settings": request.material_update_settings})
+        mreq = type("Req", (), {"time_step": time_step, "kinematics": "three_dimensional",
+                                "update_settings": update_settings})
         err = point_failure_error(fail, mats[phase], mreq, local, mats[phase])
         raise err.with_context(element_id=local // 8, quadrature_id=local % 8)
     if status == _lib.CIA_ERR_RVE_FAILURE:
@@ -440,15 +454,14 @@ def solve_rve_microequilibrium(model: StructuredHex8RVE, request: RVERequest,
         raise RVEEquilibriumError(reason, {
             "algorithm": "periodic_rve_microequilibrium_newton",
             "macro_strain": macro.tolist(),
-            "time_step": request.time_step,
+            "time_step": time_step,
             "iteration": int(res.failure_iteration),
             "residual_norm": float(res.failure_residual_norm),
             "residual_tolerance": float(res.failure_residual_tolerance),
             "increment_norm": float(res.failure_increment_norm),
             "increment_tolerance": float(res.failure_increment_tolerance),
         })
-    return _build_response(cache, request, committed_state, res, next_packed, next_w, macro,
-                           retain_micro_fields)
+    return res
 
 
 def _build_response(cache, request, committed, res, next_packed, next_w, macro, retain):

@@ -399,6 +399,32 @@ cia_status cia_box_explicit_steps(cia_box* box, void* stream, int32_t initialize
                                   double* internal_force, double* history,
                                   cia_point_failure* failure);
 
+/*
+ * Generic-material explicit step (solve_material_explicit with any
+ * MaterialModel, material.py:164-284; the assembly pieces of
+ * assemble_bundle_material_response, nonlinear_solid.py:222-300).  The caller
+ * evaluates the material between the two calls.  FP64 throughout.
+ *   cia_box_generic_strains: strain (and rate, may be NULL) [6][8 n_elements] at
+ *     the Verlet predictor u + dt v + dt^2/2 a with fixed dofs zeroed (predict = 1)
+ *     or at u itself with zero rate (predict = 0, the initial assembly).
+ *   cia_box_generic_correct: internal force of the caller's stresses, then the
+ *     node pass of cia_box_explicit_steps (a, v, u updated in place; with
+ *     initialize = 1 only a = (f_ext - f_int)/m).  psi, diss, work (incremental
+ *     work density), active (viscoplastic-active flag) and margin (yield
+ *     activation margin) are [8 n_elements] device arrays or NULL; history_row is
+ *     a device row of CIA_N_HISTORY doubles receiving the step's energy terms.
+ */
+cia_status cia_box_generic_strains(cia_box* box, void* stream, int32_t predict, double time_step,
+                                   const double* u, const double* v, const double* a,
+                                   const uint8_t* fixed, double* strain, double* rate);
+cia_status cia_box_generic_correct(cia_box* box, void* stream, int32_t initialize,
+                                   double time_step, double load_scale, const double* stress,
+                                   const double* psi, const double* diss, const double* work,
+                                   const double* active, const double* margin, double* u,
+                                   double* v, double* a, const double* mass,
+                                   const uint8_t* fixed, const double* f_unit,
+                                   double* internal_force, double* history_row);
+
 /* Kernel timing with HIP events on the launch stream (no reference
  * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element
  * pass, 2 PCG node gather, 3 PCG vector update, 4 preconditioner (spectral: FFT,

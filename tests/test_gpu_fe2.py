@@ -1,0 +1,208 @@
+"""FE² macro adapter (RVEHomogenizedMaterial) and the generic-material explicit loop.
+
+* Single-phase FE² embedding: one homogeneous RVE per macro Gauss point must reproduce
+  direct J2 macro dynamics within the reference's embedding tolerance, 1e-9 of fixed
+  scales (configs/benchmarks/rve/single_phase_dynamic_embedding.yaml,
+  src/concrete_impact/benchmarks/rve_dynamics.py:38-144), and the oracle's direct
+  explicit run.
+* The generic loop with a point material equals the fused device loop (1e-12).
+* Concurrent stream execution equals serial execution bit for bit (the reference
+  asserts serial == pool at 1e-12, test_rve_dynamics.py:61-127).
+* The adapter equals per-point single-RVE solves, and the host state codec round-trips.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import dynamics as OD
+from oracle import j2vp as OJ
+
+pytestmark = pytest.mark.gpu
+
+E, NU = 3339.9160679991464, 0.36994096308414537
+SETTINGS_ARGS = (50, 1e-10, 1e-12, 1e-10)
+
+
+def _j2(M):
+    return M.J2ViscoplasticMaterial("embedding_j2", 1.0, E, NU, 10.0, 80.0, 0.02, 10.0, 2.0)
+
+
+def _newton(S):
+    return S.NonlinearNewtonSettings(20, 1e-9, 1e-8, 1e-12, 1e-8, S.ArmijoSettings(True, 1e-4, 0.5, 20))
+
+
+def _embedding_case(backend="serial", workers=1):
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import fe2
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import solvers as S
+
+    j2 = _j2(M)
+    lengths, div = (0.02, 0.01, 0.01), (2, 1, 1)
+    direct = D.build_structured_box_bundle(lengths, div, j2)
+    nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (1, 1, 1))
+    model = R.StructuredHex8RVE(nodes, elements, (1, 1, 1), j2,
+                                R.build_structured_hex8_periodic_constraint(nodes, (1, 1, 1)),
+                                _newton(S), hill_mandel_power_scale=10.0)
+    mat = fe2.RVEHomogenizedMaterial(model, "single_phase_rve_material",
+                                     fe2.RVEExecutionSettings(backend, workers))
+    rve_bundle = D.build_structured_box_bundle(lengths, div, mat)
+    left = direct.mesh_info.boundary_groups["left"].nodes
+    dofs = (3 * left[:, None] + np.arange(3)).ravel()
+    dirichlet = S.DirichletDofSet(dofs, np.zeros(dofs.size))
+    load = D.build_half_sine_pressure_load(direct, 20.0, 2.0e-4, group="right",
+                                           outward_normal=(1.0, 0.0, 0.0))
+    ts = S.TimeIntegrationSettings("explicit", 2.0e-5, 20, cfl_safety_factor=0.8)
+    settings = M.MaterialUpdateSettings(*SETTINGS_ARGS)
+    n = direct.mesh_info.dof_map.size
+    args = (ts, np.zeros(n), np.zeros(n), dirichlet, load, "three_dimensional", settings)
+    return D, direct, rve_bundle, args, mat, dofs, load
+
+
+def _scaled(a, b, scale):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / scale)
+
+
+def test_fe2_explicit_embedding_matches_direct_dynamics(gpu):
+    D, direct, rve_bundle, args, mat, dofs, load = _embedding_case()
+    try:
+        ref = D.solve_material_explicit(direct, *args)
+        fe = D.solve_material_explicit(rve_bundle, *args)
+    finally:
+        mat.close()
+    assert np.max(ref.equivalent_plastic_strain) > 0.0  # the pulse yields the bar
+    tol = 1e-9
+    assert _scaled(ref.displacement, fe.displacement, 0.01) <= tol
+    assert _scaled(ref.velocity, fe.velocity, 1.0) <= tol
+    assert _scaled(ref.stresses, fe.stresses, 10.0) <= tol
+    assert _scaled(ref.energy_residual, fe.energy_residual, 0.01) <= tol
+    # macro q: RVE average of the homogeneous RVE == the direct Gauss-point value
+    assert _scaled(ref.equivalent_plastic_strain, fe.equivalent_plastic_strain, 0.01) <= tol
+    # and against the oracle's direct explicit run
+    obox = OD.Box((0.02, 0.01, 0.01), (2, 1, 1), [("m", OJ.J2VPParams(E, NU, 10.0, 80.0, 0.02, 10.0, 2.0, 1.0))])
+    orc = OD.explicit(obox, 2.0e-5, 20, dofs, load.unit_force, OD.half_sine(20.0, 2.0e-4),
+                      OJ.UpdateSettings(*SETTINGS_ARGS))
+    assert _scaled(orc["u"], fe.displacement, 0.01) <= tol
+    assert _scaled(orc["stress"], fe.stresses, 10.0) <= tol
+    assert _scaled(orc["energy_residual"], fe.energy_residual, 0.01) <= tol
+
+
+def test_generic_loop_equals_fused_loop_phased(gpu):
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import solvers as S
+
+    lengths, div = (0.6, 0.3, 0.3), (6, 3, 3)
+    eph = (np.random.default_rng(4).random(54) < 0.35).astype(np.int64)
+    mat = M.PhasedMaterialModel("slab", (M.RVEPhase("mortar", _j2(M)),
+                                         M.RVEPhase("agg", M.LinearElasticMaterial("agg", 2.0, 2 * E, 0.2))),
+                                np.repeat(eph, 8), 1.5)
+    bundle = D.build_structured_box_bundle(lengths, div, mat, eph)
+    load = D.build_half_sine_pressure_load(bundle, 60.0, 40 * 5e-4, patch=((1, 5), (0, 3)))
+    bottom = bundle.mesh_info.boundary_groups["bottom"].nodes
+    dofs = (3 * bottom[:, None] + np.arange(3)).ravel()
+    n = bundle.mesh_info.dof_map.size
+    args = (S.TimeIntegrationSettings("explicit", 5e-4, 60, cfl_safety_factor=0.8), np.zeros(n),
+            np.zeros(n), S.DirichletDofSet(dofs, np.zeros(dofs.size)), load, "three_dimensional",
+            M.MaterialUpdateSettings(*SETTINGS_ARGS))
+    fused = D.solve_material_explicit(bundle, *args)
+    gen = D.solve_material_explicit(bundle, *args, generic=True)
+    assert fused.dissipated_energy[-1] > 0.0
+    for name in ("displacement", "velocity", "acceleration", "stresses", "internal_force"):
+        a, b = getattr(fused, name), getattr(gen, name)
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a)), name
+    for name in ("kinetic_energy", "free_energy", "dissipated_energy", "external_work",
+                 "positive_incremental_work", "negative_incremental_work",
+                 "viscoplastic_active_volume_fraction"):
+        a, b = getattr(fused, name), getattr(gen, name)
+        assert np.max(np.abs(a - b)) <= 1e-11 * max(np.max(np.abs(a)), 1e-300), name
+    for k, v in fused.final_state.variables.items():
+        np.testing.assert_allclose(gen.final_state.variables[k], v, rtol=0, atol=1e-14)
+
+
+def _two_phase_adapter(M, R, S, fe2, backend, workers, n=4):
+    eph = np.zeros(n ** 3, dtype=np.int64)
+    eph[: n ** 3 // 3] = 1
+    eph = np.random.default_rng(7).permutation(eph)
+    mortar = M.J2ViscoplasticMaterial("mortar", 2400.0, 25e9, 0.2, 20e6, 0.5e9, 1e-3, 20e6, 1.0)
+    agg = M.LinearElasticMaterial("aggregate", 2400.0, 50e9, 0.2)
+    nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (n, n, n))
+    newton = S.NonlinearNewtonSettings(25, 2e-3, 1e-10, 1e-12, 1e-10, S.ArmijoSettings(True))
+    model = R.StructuredHex8RVE(
+        nodes, elements, (n, n, n),
+        M.PhasedMaterialModel("rve", (M.RVEPhase("mortar", mortar), M.RVEPhase("aggregate", agg)),
+                              np.repeat(eph, 8), 2400.0),
+        R.build_structured_hex8_periodic_constraint(nodes, (n, n, n)), newton,
+        element_phase_ids=eph, phase_names=("mortar", "aggregate"),
+        linear_settings=S.LinearSolverSettings(relative_tolerance=1e-12))
+    return model, fe2.RVEHomogenizedMaterial(model, "fe2", fe2.RVEExecutionSettings(backend, workers))
+
+
+def _request(M, strains, dt=1e-5):
+    return M.MaterialPointRequest(strains, None, dt, "three_dimensional",
+                                  M.MaterialUpdateSettings(*SETTINGS_ARGS))
+
+
+def test_fe2_streams_equal_serial_and_single_rve(gpu):
+    from concrete_impact_ai_amd import fe2
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import solvers as S
+
+    rng = np.random.default_rng(3)
+    npts = 12
+    model, serial = _two_phase_adapter(M, R, S, fe2, "serial", 1)
+    _, par = _two_phase_adapter(M, R, S, fe2, "streams", 4)
+    req_t = M.MaterialResponseRequirements(tangent=True, free_energy=True, dissipation=True)
+    try:
+        st_s, st_p = serial.initialize_state(npts), par.initialize_state(npts)
+        for step in range(3):
+            strains = rng.uniform(-1.0, 1.0, (npts, 6)) * 1.5e-3 * (step + 1)
+            rs = serial.update(_request(M, strains), st_s, req_t)
+            rp = par.update(_request(M, strains), st_p, req_t)
+            assert np.array_equal(rs.stresses, rp.stresses)
+            assert np.array_equal(rs.tangents, rp.tangents)
+            assert np.array_equal(rs.free_energy, rp.free_energy)
+            for k in rs.diagnostics:
+                assert np.array_equal(rs.diagnostics[k], rp.diagnostics[k], equal_nan=True), k
+            assert np.max(rs.diagnostics["viscoplastic_active"]) >= 0.0
+            # each point equals a single-RVE solve from the decoded committed state
+            cache = R.build_rve_operator_cache(model)
+            for i in (0, npts - 1):
+                one = R.solve_rve_microequilibrium(
+                    model, R.RVERequest(strains[i], 1e-5, M.MaterialUpdateSettings(*SETTINGS_ARGS), req_t),
+                    fe2.decode_rve_point_state(st_s, i), cache, retain_micro_fields=False)
+                assert np.array_equal(one.macro_stress, rs.stresses[i])
+                assert np.array_equal(one.effective_tangent, rs.tangents[i])
+            cache.close()
+            st_s, st_p = rs.state, rp.state
+        assert np.max(rs.diagnostics["phase_mortar__maximum_equivalent_plastic_strain"]) > 0.0
+    finally:
+        serial.close()
+        par.close()
+
+
+def test_fe2_host_state_codec_round_trip(gpu):
+    from concrete_impact_ai_amd import fe2
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import solvers as S
+
+    rng = np.random.default_rng(11)
+    model, mat = _two_phase_adapter(M, R, S, fe2, "serial", 1, n=3)
+    try:
+        st = mat.initialize_state(5)
+        strains = rng.uniform(-1.0, 1.0, (5, 6)) * 2e-3
+        r1 = mat.update(_request(M, strains), st)
+        host = M.MaterialState({k: np.array(v) for k, v in r1.state.variables.items()})
+        assert host.variables["rve_fluctuation_dofs"].shape == (5, 3 * (27 - 1))
+        assert host.variables["rve_material__phase_mortar__plastic_strain"].shape[0] == 5
+        strains2 = strains * 1.5
+        a = mat.update(_request(M, strains2), r1.state)
+        b = mat.update(_request(M, strains2), host)
+        assert np.array_equal(a.stresses, b.stresses)
+        for k, v in a.state.variables.items():
+            assert np.array_equal(v, b.state.variables[k]), k
+    finally:
+        mat.close()
