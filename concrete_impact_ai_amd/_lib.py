@@ -22,6 +22,7 @@ CIA_ERR_HIP = 2
 CIA_ERR_POINT_FAILURE = 3
 CIA_ERR_RVE_FAILURE = 4
 CIA_ERR_OUT_OF_MEMORY = 5
+CIA_ERR_NOT_CONVERGED = 6
 
 CIA_MAT_LINEAR_ELASTIC = 0
 CIA_MAT_J2_VISCOPLASTIC = 1
@@ -304,6 +305,16 @@ _SIGNATURES.update({
         C.c_int32,
         [_vp, _vp, C.c_int32, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
          _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cia_box_strains_between": (C.c_int32, [_vp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
+    "cia_box_newmark_residual": (
+        C.c_int32,
+        [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_double, _vp, _vp,
+         _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]),
+    "cia_box_mass_apply": (C.c_int32, [_vp, _vp, C.c_double, _vp, _vp]),
+    "cia_box_newmark_solve": (
+        C.c_int32,
+        [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_double,
+         C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
 })
 
 PROFILE_SLOTS = ["assemble", "pcg_elem", "pcg_node", "pcg_update", "precond", "gather"]

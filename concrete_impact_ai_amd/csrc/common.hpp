@@ -68,6 +68,12 @@ struct DeviceCtx {
 };
 cia_status device_ctx(DeviceCtx** out);
 
+#define CIA_TRY(x)               \
+  do {                           \
+    cia_status s_ = (x);         \
+    if (s_ != CIA_OK) return s_; \
+  } while (0)
+
 constexpr int kBlock = 256;
 
 inline unsigned grid_for(int64_t n, int block = kBlock) {

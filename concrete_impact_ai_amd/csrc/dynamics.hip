@@ -33,6 +33,12 @@ struct BoxGrid {
   BoxGeom g;
 };
 
+struct NmScalars {
+  double rr;
+  int32_t done;  // 0 running, 1 converged, 2 breakdown
+  int32_t iters;
+};
+
 // energy history columns
 enum {
   H_KINETIC = 0,
@@ -371,6 +377,399 @@ __global__ void __launch_bounds__(kBlock)
   block_partials<kBlock>(vals, 6, &mn, &mx, part_e, gridDim.x, lds);
 }
 
+
+// ---------------------------------------------------------------------------
+// Implicit Newmark pieces (solve_material_implicit_newmark, material.py:300-701):
+// the trial residual R = M a + f_int - f_ext with a = (u - u*)/(beta dt^2), and a
+// Jacobi-preconditioned CG for (K_t + c M) x = b on the free dofs.  The mass is
+// the consistent Hex8 mass sum_q rho N^T N w detJ (elasticity.py:109-126), applied
+// per Gauss-point lane like the stiffness.
+
+// Corner shape values at Gauss point q: N_a = Wx[ax] Wy[ay] Wz[az].
+__device__ __forceinline__ void gp_shape(int q, double N[8]) {
+  const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+    N[a] = wq(qx, a & 1) * wq(qy, (a >> 1) & 1) * wq(qz, (a >> 2) & 1);
+}
+
+// Sum of P partials, identical in every block (fixed order), broadcast to all threads.
+__device__ __forceinline__ double sum_parts(const double* part, int P, double* lds) {
+  __shared__ double bc;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < P; b += blockDim.x) a += part[b];
+  a = block_sum<kBlock>(a, lds);
+  if (threadIdx.x == 0) bc = a;
+  __syncthreads();
+  return bc;
+}
+
+// Gather element rows F[24][E] to the node of lane n (non-periodic box).
+__device__ __forceinline__ void gather_node(const BoxGrid& G, const double* F, int64_t n,
+                                            double f[3]) {
+  const int nx1 = G.nx + 1, ny1 = G.ny + 1;
+  const unsigned nu = (unsigned)n, r = nu / (unsigned)nx1;
+  const int i = (int)(nu - r * (unsigned)nx1);
+  const int k = (int)(r / (unsigned)ny1);
+  const int j = (int)(r - (unsigned)k * (unsigned)ny1);
+  f[0] = f[1] = f[2] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int ei = i - (c & 1), ej = j - ((c >> 1) & 1), ek = k - ((c >> 2) & 1);
+    if (ei < 0 || ej < 0 || ek < 0 || ei >= G.nx || ej >= G.ny || ek >= G.nz) continue;
+    const int64_t e = ei + (int64_t)G.nx * (ej + (int64_t)G.ny * ek);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) f[m] += F[(3 * c + m) * G.E + e];
+  }
+}
+
+// Strains at u_new and rates (u_new - u_old)/dt.
+__global__ void __launch_bounds__(kBlock)
+    k_box_strain_between(BoxGrid G, double dt, const double* __restrict__ un,
+                         const double* __restrict__ uo, double* __restrict__ strain,
+                         double* __restrict__ rate) {
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  int64_t nd[8];
+  box_corners(e, G.nx, G.ny, nd);
+  double bx[8], by[8], bz[8];
+  gp_factors<double>(q, G.g, bx, by, bz);
+  double u1[8][3], du[8][3];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * nd[c] + m;
+      u1[c][m] = un[d];
+      du[c][m] = un[d] - uo[d];
+    }
+  double eps[6], rt[6];
+  strain_from(u1, bx, by, bz, eps);
+  strain_from(du, bx, by, bz, rt);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    strain[m * np + p] = eps[m];
+    if (rate) rate[m * np + p] = rt[m] / dt;
+  }
+}
+
+// Element rows of f_int (from stresses) and of M a with a = (u - u*) * inv_b.
+__global__ void __launch_bounds__(kBlock)
+    k_nm_residual_elem(BoxGrid G, double rho, double inv_b, const double* __restrict__ stress,
+                       const double* __restrict__ u, const double* __restrict__ ustar,
+                       double* __restrict__ F, double* __restrict__ Fm) {
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  int64_t nd[8];
+  box_corners(e, G.nx, G.ny, nd);
+  double bx[8], by[8], bz[8], N[8];
+  gp_factors<double>(q, G.g, bx, by, bz);
+  gp_shape(q, N);
+  double sg[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) sg[m] = stress[m * np + p];
+  double f[8][3], fin[3];
+  force_from(sg, G.g.jw, bx, by, bz, f);
+  element_reduce8(q, f, fin);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * nd[b] + m;
+      acc[m] += N[b] * ((u[d] - ustar[d]) * inv_b);
+    }
+  const double s = rho * G.g.jw;
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) f[a][m] = s * N[a] * acc[m];
+  element_reduce8(q, f, fin);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) Fm[(3 * q + m) * G.E + e] = fin[m];
+}
+
+// Node pass: a, v, f_int, residual (fixed dofs zero), partials of |R_free|^2, |u_free|^2.
+__global__ void __launch_bounds__(kBlock)
+    k_nm_residual_node(BoxGrid G, const double* __restrict__ F, const double* __restrict__ Fm,
+                       double s_ext, const double* __restrict__ f_unit,
+                       const uint8_t* __restrict__ fixed, const double* __restrict__ u,
+                       const double* __restrict__ ustar, const double* __restrict__ vstar,
+                       double inv_b, double gdt, double* __restrict__ a_out,
+                       double* __restrict__ v_out, double* __restrict__ fint_out,
+                       double* __restrict__ r_out, double* __restrict__ part) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  double vals[2] = {0.0, 0.0};
+  if (n < G.N) {
+    double fi[3], fm[3];
+    gather_node(G, F, n, fi);
+    gather_node(G, Fm, n, fm);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * n + m;
+      const double acc = (u[d] - ustar[d]) * inv_b;
+      a_out[d] = acc;
+      v_out[d] = vstar[d] + gdt * acc;
+      fint_out[d] = fi[m];
+      const double r = fixed[d] ? 0.0 : fm[m] + fi[m] - s_ext * f_unit[d];
+      r_out[d] = r;
+      vals[0] += r * r;
+      if (!fixed[d]) vals[1] += u[d] * u[d];
+    }
+  }
+  block_partials<kBlock>(vals, 2, nullptr, nullptr, part, gridDim.x, lds);
+}
+
+// y rows of (K_t + cm M) p per Gauss-point lane; tangent [36][n_gp] row-major per
+// point, or NULL for the mass term alone.
+__global__ void __launch_bounds__(kBlock)
+    k_nm_apply_elem(BoxGrid G, const double* __restrict__ tan, double rho_cm,
+                    const double* __restrict__ pv, double* __restrict__ F,
+                    const NmScalars* __restrict__ sc) {
+  if (sc && sc->done) return;
+  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t e = p >> 3;
+  const int q = (int)(p & 7);
+  if (e >= G.E) return;
+  const int64_t np = G.E * 8;
+  int64_t nd[8];
+  box_corners(e, G.nx, G.ny, nd);
+  double bx[8], by[8], bz[8], N[8];
+  gp_factors<double>(q, G.g, bx, by, bz);
+  gp_shape(q, N);
+  double pe[8][3];
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) pe[c][m] = pv[3 * nd[c] + m];
+  double f[8][3];
+  if (tan) {
+    double eps[6], sg[6];
+    strain_from(pe, bx, by, bz, eps);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) t += tan[(6 * i + j) * np + p] * eps[j];
+      sg[i] = t;
+    }
+    force_from(sg, G.g.jw, bx, by, bz, f);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = 0.0;
+  }
+  double mp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int b = 0; b < 8; ++b)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mp[m] += N[b] * pe[b][m];
+  const double s = rho_cm * G.g.jw;
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) f[a][m] += s * N[a] * mp[m];
+  double fin[3];
+  element_reduce8(q, f, fin);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
+}
+
+// Jacobi diagonal of K_t + cm M at every free dof (1 at fixed dofs), inverted.
+__global__ void __launch_bounds__(kBlock)
+    k_nm_diag(BoxGrid G, const double* __restrict__ tan, double rho_cm,
+              const uint8_t* __restrict__ fixed, double* __restrict__ dinv) {
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  const int64_t np = G.E * 8;
+  const int nx1 = G.nx + 1, ny1 = G.ny + 1;
+  const unsigned nu = (unsigned)n, r = nu / (unsigned)nx1;
+  const int i = (int)(nu - r * (unsigned)nx1);
+  const int k = (int)(r / (unsigned)ny1);
+  const int j = (int)(r - (unsigned)k * (unsigned)ny1);
+  double dg[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 8; ++c) {
+    const int ei = i - (c & 1), ej = j - ((c >> 1) & 1), ek = k - ((c >> 2) & 1);
+    if (ei < 0 || ej < 0 || ek < 0 || ei >= G.nx || ej >= G.ny || ek >= G.nz) continue;
+    const int64_t e = ei + (int64_t)G.nx * (ej + (int64_t)G.ny * ek);
+    for (int q = 0; q < 8; ++q) {
+      const int64_t p = 8 * e + q;
+      double col[3][6], N[8];
+      b_columns<double>(q, c, G.g, col);
+      gp_shape(q, N);
+      for (int m = 0; m < 3; ++m) {
+        double kd = 0.0;
+        if (tan) {
+          for (int a = 0; a < 6; ++a) {
+            double t = 0.0;
+            for (int b = 0; b < 6; ++b) t += tan[(6 * a + b) * np + p] * col[m][b];
+            kd += col[m][a] * t;
+          }
+        }
+        dg[m] += G.g.jw * (kd + rho_cm * N[c] * N[c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int64_t d = 3 * n + m;
+    dinv[d] = (fixed[d] || !(dg[m] > 0.0)) ? 1.0 : 1.0 / dg[m];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_nm_residual_node_free(BoxGrid G, const double* __restrict__ F, double* __restrict__ y) {
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (n >= G.N) return;
+  double f[3];
+  gather_node(G, F, n, f);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) y[3 * n + m] = f[m];
+}
+
+// PCG state.  Partials: rz[2][P] (double-buffered by iteration parity), rr[P],
+// rr0[P], py[P].
+__global__ void __launch_bounds__(kBlock)
+    k_nm_init(BoxGrid G, const double* __restrict__ b, const double* __restrict__ dinv,
+              const uint8_t* __restrict__ fixed, double* __restrict__ x, double* __restrict__ r,
+              double* __restrict__ z, double* __restrict__ pv, double* __restrict__ part_rz,
+              double* __restrict__ part_rr0, NmScalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  double vals[2] = {0.0, 0.0};
+  if (n < G.N) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * n + m;
+      const double rv = fixed[d] ? 0.0 : b[d];
+      const double zv = rv * dinv[d];
+      x[d] = 0.0;
+      r[d] = rv;
+      z[d] = zv;
+      pv[d] = zv;
+      vals[0] += rv * zv;
+      vals[1] += rv * rv;
+    }
+  }
+  const double a0 = block_sum<kBlock>(vals[0], lds);
+  const double a1 = block_sum<kBlock>(vals[1], lds);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = a0;
+    part_rr0[blockIdx.x] = a1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->done = 0;
+    sc->iters = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_nm_apply_node(BoxGrid G, const double* __restrict__ F, const uint8_t* __restrict__ fixed,
+                    const double* __restrict__ pv, double* __restrict__ y,
+                    double* __restrict__ part_py, const NmScalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  double py = 0.0;
+  if (n < G.N) {
+    double f[3];
+    gather_node(G, F, n, f);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * n + m;
+      const double yv = fixed[d] ? 0.0 : f[m];
+      y[d] = yv;
+      py += pv[d] * yv;
+    }
+  }
+  py = block_sum<kBlock>(py, lds);
+  if (threadIdx.x == 0) part_py[blockIdx.x] = py;
+}
+
+// alpha = rz_it / p.y; x += alpha p; r -= alpha y; z = D^-1 r; partials rz_{it+1}, rr.
+__global__ void __launch_bounds__(kBlock)
+    k_nm_update(BoxGrid G, int it, const double* __restrict__ dinv, const double* __restrict__ pv,
+                const double* __restrict__ y, double* __restrict__ x, double* __restrict__ r,
+                double* __restrict__ z, const double* __restrict__ part_py,
+                double* __restrict__ part_rz, double* __restrict__ part_rr, int P,
+                NmScalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const double py = sum_parts(part_py, P, lds);
+  const double rz = sum_parts(part_rz + (it & 1) * P, P, lds);
+  if (rz == 0.0) {  // r = 0: x is exact
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 1;
+    return;
+  }
+  if (!(py > 0.0) || !isfinite(py)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
+    return;
+  }
+  const double alpha = rz / py;
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  double vals[2] = {0.0, 0.0};
+  if (n < G.N) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * n + m;
+      x[d] += alpha * pv[d];
+      const double rv = r[d] - alpha * y[d];
+      const double zv = rv * dinv[d];
+      r[d] = rv;
+      z[d] = zv;
+      vals[0] += rv * zv;
+      vals[1] += rv * rv;
+    }
+  }
+  const double a0 = block_sum<kBlock>(vals[0], lds);
+  const double a1 = block_sum<kBlock>(vals[1], lds);
+  if (threadIdx.x == 0) {
+    part_rz[((it + 1) & 1) * P + blockIdx.x] = a0;
+    part_rr[blockIdx.x] = a1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->iters = it + 1;
+}
+
+// Convergence test |r| <= max(rtol |b|, atol); else p = z + (rz_{it+1}/rz_it) p.
+__global__ void __launch_bounds__(kBlock)
+    k_nm_pdir(BoxGrid G, int it, double rtol, double atol, const double* __restrict__ z,
+              double* __restrict__ pv, const double* __restrict__ part_rz,
+              const double* __restrict__ part_rr, const double* __restrict__ part_rr0, int P,
+              NmScalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  if (sc->done) return;
+  const double rr = sum_parts(part_rr, P, lds);
+  const double rr0 = sum_parts(part_rr0, P, lds);
+  const double rz0 = sum_parts(part_rz + (it & 1) * P, P, lds);
+  const double rz1 = sum_parts(part_rz + ((it + 1) & 1) * P, P, lds);
+  const double tol = fmax(rtol * sqrt(rr0), atol);
+  if (rr <= tol * tol || !isfinite(rr)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = isfinite(rr) ? 1 : 2;
+      sc->rr = rr;
+    }
+    return;
+  }
+  const double beta = rz1 / rz0;
+  const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (n < G.N) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * n + m;
+      pv[d] = z[d] + beta * pv[d];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->rr = rr;
+}
+
 }  // namespace cia
 
 struct cia_box {
@@ -385,6 +784,11 @@ struct cia_box {
   double* hist = nullptr;  // [steps+1][H_N], grown on demand
   int64_t hist_rows = 0;
   unsigned long long* first_fail = nullptr;
+  // Newmark (allocated on first use)
+  double* Fm = nullptr;          // [24][E] mass rows
+  double* nm_vec = nullptr;      // r, z, p, y, dinv: 5 x 3N
+  double* nm_part = nullptr;     // 5 x P partials
+  cia::NmScalars* nm_sc = nullptr;
 };
 
 namespace cia {
@@ -468,7 +872,8 @@ cia_status cia_box_create(const cia_box_desc* d, cia_box** out) {
 
 cia_status cia_box_destroy(cia_box* B) {
   if (!B) return CIA_OK;
-  void* ptrs[] = {B->d_tab, B->d_ephase, B->F, B->part_e, B->part_n, B->hist, B->first_fail};
+  void* ptrs[] = {B->d_tab, B->d_ephase, B->F, B->part_e, B->part_n, B->hist, B->first_fail,
+                  B->Fm, B->nm_vec, B->nm_part, B->nm_sc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete B;
@@ -610,6 +1015,135 @@ cia_status cia_box_generic_correct(cia_box* B, void* stream, int32_t initialize,
   hipLaunchKernelGGL(k_fold_node_only, dim3(1), dim3(kBlock), 0, st, B->part_n, (int)gn,
                      history_row);
   CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+static cia_status nm_buffers(cia_box* B) {
+  using namespace cia;
+  if (B->nm_sc) return CIA_OK;
+  const int64_t n3 = 3 * B->G.N, P = grid_for(B->G.N);
+  cia_status s;
+  if ((s = dyn_alloc((void**)&B->Fm, 24 * B->G.E * sizeof(double))) != CIA_OK) return s;
+  if ((s = dyn_alloc((void**)&B->nm_vec, 5 * n3 * sizeof(double))) != CIA_OK) return s;
+  if ((s = dyn_alloc((void**)&B->nm_part, 5 * P * sizeof(double))) != CIA_OK) return s;
+  if ((s = dyn_alloc((void**)&B->nm_sc, sizeof(NmScalars))) != CIA_OK) return s;
+  return CIA_OK;
+}
+
+cia_status cia_box_strains_between(cia_box* B, void* stream, double time_step, const double* u_new,
+                                   const double* u_old, double* strain, double* rate) {
+  using namespace cia;
+  CIA_ARG(B && u_new && u_old && strain, "NULL argument to cia_box_strains_between");
+  CIA_ARG(time_step > 0.0, "time step must be positive");
+  hipLaunchKernelGGL(k_box_strain_between, dim3(grid_for(8 * B->G.E)), dim3(kBlock), 0,
+                     (hipStream_t)stream, B->G, time_step, u_new, u_old, strain, rate);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_box_newmark_residual(cia_box* B, void* stream, double density, double beta,
+                                    double gamma, double time_step, const double* stress,
+                                    double load_scale, const double* f_unit, const uint8_t* fixed,
+                                    const double* u, const double* u_pred, const double* v_pred,
+                                    double* a, double* v, double* internal_force,
+                                    double* residual, double* norms) {
+  using namespace cia;
+  CIA_ARG(B && stress && f_unit && fixed && u && u_pred && v_pred && a && v && internal_force &&
+              residual && norms,
+          "NULL argument to cia_box_newmark_residual");
+  CIA_ARG(beta > 0.0 && time_step > 0.0, "invalid Newmark parameters");
+  CIA_TRY(nm_buffers(B));
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned ge = grid_for(8 * B->G.E), gn = grid_for(B->G.N);
+  const double inv_b = 1.0 / (beta * time_step * time_step);
+  hipLaunchKernelGGL(k_nm_residual_elem, dim3(ge), dim3(kBlock), 0, st, B->G, density, inv_b,
+                     stress, u, u_pred, B->F, B->Fm);
+  hipLaunchKernelGGL(k_nm_residual_node, dim3(gn), dim3(kBlock), 0, st, B->G, B->F, B->Fm,
+                     load_scale, f_unit, fixed, u, u_pred, v_pred, inv_b, gamma * time_step, a, v,
+                     internal_force, residual, B->nm_part);
+  CIA_HIP(hipGetLastError());
+  std::vector<double> h(2 * (size_t)gn);
+  CIA_HIP(hipMemcpyAsync(h.data(), B->nm_part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+  CIA_HIP(hipStreamSynchronize(st));
+  double s0 = 0.0, s1 = 0.0;
+  for (unsigned b = 0; b < gn; ++b) {
+    s0 += h[b];
+    s1 += h[gn + b];
+  }
+  norms[0] = sqrt(s0);
+  norms[1] = sqrt(s1);
+  return CIA_OK;
+}
+
+cia_status cia_box_mass_apply(cia_box* B, void* stream, double density, const double* x,
+                              double* y) {
+  using namespace cia;
+  CIA_ARG(B && x && y, "NULL argument to cia_box_mass_apply");
+  CIA_TRY(nm_buffers(B));
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned ge = grid_for(8 * B->G.E), gn = grid_for(B->G.N);
+  hipLaunchKernelGGL(k_nm_apply_elem, dim3(ge), dim3(kBlock), 0, st, B->G, (const double*)nullptr,
+                     density, x, B->F, (const NmScalars*)nullptr);
+  hipLaunchKernelGGL(k_nm_residual_node_free, dim3(gn), dim3(kBlock), 0, st, B->G, B->F, y);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_box_newmark_solve(cia_box* B, void* stream, const double* tangent, double density,
+                                 double mass_coefficient, const uint8_t* fixed, const double* rhs,
+                                 double* x, int32_t max_iterations, int32_t check_interval,
+                                 double relative_tolerance, double absolute_tolerance,
+                                 int32_t* iterations, double* residual_norm) {
+  using namespace cia;
+  CIA_ARG(B && fixed && rhs && x && iterations && residual_norm,
+          "NULL argument to cia_box_newmark_solve");
+  CIA_ARG(max_iterations > 0, "max_iterations must be positive");
+  CIA_TRY(nm_buffers(B));
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n3 = 3 * B->G.N;
+  const unsigned ge = grid_for(8 * B->G.E), gn = grid_for(B->G.N);
+  const int P = (int)gn;
+  double* r = B->nm_vec;
+  double* z = r + n3;
+  double* pv = z + n3;
+  double* y = pv + n3;
+  double* dinv = y + n3;
+  double* part_rz = B->nm_part;  // [2][P]
+  double* part_rr = part_rz + 2 * P;
+  double* part_rr0 = part_rr + P;
+  double* part_py = part_rr0 + P;
+  const double rho_cm = density * mass_coefficient;
+  hipLaunchKernelGGL(k_nm_diag, dim3(gn), dim3(kBlock), 0, st, B->G, tangent, rho_cm, fixed, dinv);
+  hipLaunchKernelGGL(k_nm_init, dim3(gn), dim3(kBlock), 0, st, B->G, rhs, dinv, fixed, x, r, z, pv,
+                     part_rz, part_rr0, B->nm_sc);
+  CIA_HIP(hipGetLastError());
+  const int chunk = std::max(1, (int)check_interval);
+  NmScalars hs{};
+  int it = 0;
+  while (it < max_iterations) {
+    const int n = std::min(chunk, max_iterations - it);
+    for (int s = 0; s < n; ++s, ++it) {
+      hipLaunchKernelGGL(k_nm_apply_elem, dim3(ge), dim3(kBlock), 0, st, B->G, tangent, rho_cm,
+                         (const double*)pv, B->F, (const NmScalars*)B->nm_sc);
+      hipLaunchKernelGGL(k_nm_apply_node, dim3(gn), dim3(kBlock), 0, st, B->G, (const double*)B->F,
+                         fixed, (const double*)pv, y, part_py, (const NmScalars*)B->nm_sc);
+      hipLaunchKernelGGL(k_nm_update, dim3(gn), dim3(kBlock), 0, st, B->G, it, dinv, pv, y, x, r, z,
+                         part_py, part_rz, part_rr, P, B->nm_sc);
+      hipLaunchKernelGGL(k_nm_pdir, dim3(gn), dim3(kBlock), 0, st, B->G, it, relative_tolerance,
+                         absolute_tolerance, z, pv, part_rz, part_rr, part_rr0, P, B->nm_sc);
+    }
+    CIA_HIP(hipGetLastError());
+    CIA_HIP(hipMemcpyAsync(&hs, B->nm_sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+    CIA_HIP(hipStreamSynchronize(st));
+    if (hs.done) break;
+  }
+  *iterations = hs.iters;
+  *residual_norm = sqrt(hs.rr);
+  if (hs.done != 1) {
+    set_error_msg(hs.done == 2 ? "Newmark PCG breakdown" : "Newmark PCG did not converge");
+    return CIA_ERR_NOT_CONVERGED;
+  }
   return CIA_OK;
 }
 

@@ -1424,11 +1424,6 @@ static cia_status dmalloc(void** p, size_t bytes) {
   return CIA_OK;
 }
 
-#define CIA_TRY(x)               \
-  do {                           \
-    cia_status s_ = (x);         \
-    if (s_ != CIA_OK) return s_; \
-  } while (0)
 
 template <class T>
 struct Ops {

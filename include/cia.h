@@ -425,6 +425,41 @@ cia_status cia_box_generic_correct(cia_box* box, void* stream, int32_t initializ
                                    const uint8_t* fixed, const double* f_unit,
                                    double* internal_force, double* history_row);
 
+/*
+ * Implicit Newmark pieces (solve_material_implicit_newmark,
+ * src/fem/dynamics/material.py:300-701), FP64, with the consistent Hex8 mass
+ * rho sum_q N^T N w detJ of the bundle material's density (elasticity.py:109-126).
+ *   cia_box_strains_between: strain at u_new, rate (u_new - u_old)/dt, [6][8 n_el].
+ *   cia_box_newmark_residual (_evaluate_implicit_trial, material.py:665-701):
+ *     a = (u - u_pred)/(beta dt^2), v = v_pred + gamma dt a, f_int from the
+ *     caller's stresses, R = M a + f_int - load_scale f_unit with fixed dofs zero;
+ *     norms (host [2]) = |R_free|, |u_free|.
+ *   cia_box_mass_apply: y = M x (all dofs).
+ *   cia_box_newmark_solve: Jacobi-preconditioned CG for
+ *     (K_t + mass_coefficient M) x = rhs (M of the given density) on the free dofs (x = 0 on fixed dofs),
+ *     K_t = sum_q B^T C_q B w detJ from tangent [36][8 n_el] (row-major 6x6 per
+ *     point; NULL = mass only); stops at |r| <= max(rtol |rhs|, atol); returns
+ *     CIA_ERR_NOT_CONVERGED on breakdown or iteration limit.  Replaces the
+ *     SuperLU solve of the effective tangent (material.py:540-543).
+ */
+cia_status cia_box_strains_between(cia_box* box, void* stream, double time_step,
+                                   const double* u_new, const double* u_old, double* strain,
+                                   double* rate);
+cia_status cia_box_newmark_residual(cia_box* box, void* stream, double density, double beta,
+                                    double gamma, double time_step, const double* stress,
+                                    double load_scale, const double* f_unit,
+                                    const uint8_t* fixed, const double* u, const double* u_pred,
+                                    const double* v_pred, double* a, double* v,
+                                    double* internal_force, double* residual, double* norms);
+cia_status cia_box_mass_apply(cia_box* box, void* stream, double density, const double* x,
+                              double* y);
+cia_status cia_box_newmark_solve(cia_box* box, void* stream, const double* tangent,
+                                 double density, double mass_coefficient, const uint8_t* fixed,
+                                 const double* rhs, double* x, int32_t max_iterations,
+                                 int32_t check_interval, double relative_tolerance,
+                                 double absolute_tolerance, int32_t* iterations,
+                                 double* residual_norm);
+
 /* Kernel timing with HIP events on the launch stream (no reference
  * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element
  * pass, 2 PCG node gather, 3 PCG vector update, 4 preconditioner (spectral: FFT,

@@ -59,7 +59,9 @@ def half_sine(amplitude, duration):
 
 
 class Box:
-    def __init__(self, lengths, divisions, phases, element_phase_ids=None):
+    def __init__(self, lengths, divisions, phases, element_phase_ids=None, density=None):
+        """``density``: the bundle material's density used by the mass matrices for
+        every element (``elasticity.py:109-126``); defaults to the single phase's."""
         self.lengths, self.divisions = lengths, divisions
         self.nodes, self.elements = hex8.structured_box(lengths, divisions)
         self.b, self.jw = hex8.geometry_cache(self.nodes, self.elements)
@@ -68,7 +70,8 @@ class Box:
         self.eph = np.zeros(ne, dtype=np.int64) if element_phase_ids is None else np.asarray(element_phase_ids)
         self.ppid = np.repeat(self.eph, 8)
         self.phased = element_phase_ids is not None
-        rho = np.asarray([m.density for _, m in phases])[self.eph]
+        self.density = float(phases[0][1].density if density is None else density)
+        rho = np.full(ne, self.density)
         self.mass = hex8.lumped_mass(self.nodes, self.elements, rho)
         self.n_dof = 3 * self.nodes.shape[0]
 
@@ -83,7 +86,7 @@ class Box:
                 st[pre + "viscoplastic_multiplier"] = np.zeros(cnt)
         return st
 
-    def assemble(self, u, u_prev, dt, settings, state):
+    def assemble(self, u, u_prev, dt, settings, state, need_tangent=False):
         eps, rate = hex8.strain_fields(self.elements, self.b, u, u_prev, dt)
         if self.phased:
             # The reference's phased model raises for elastic phases without a tangent
@@ -94,13 +97,16 @@ class Box:
             mat = self.phases[0][1]
             if isinstance(mat, j2vp.J2VPParams):
                 r = j2vp.j2vp_update(mat, eps, state["plastic_strain"], state["equivalent_plastic_strain"],
-                                     state["viscoplastic_multiplier"], dt, settings, rate, False)
+                                     state["viscoplastic_multiplier"], dt, settings, rate, need_tangent)
                 r["state"] = {k: r[k] for k in ("plastic_strain", "equivalent_plastic_strain",
                                                 "viscoplastic_multiplier")}
             else:
-                r = j2vp.elastic_update(mat, eps, dt, rate, False, False)
+                r = j2vp.elastic_update(mat, eps, dt, rate, need_tangent, False)
                 r["state"] = {}
         r["internal_force"] = hex8.internal_force(self.elements, self.b, self.jw, r["stress"], self.n_dof)
+        if need_tangent:
+            r["tangent_matrix"] = hex8.tangent_matrix(self.elements, self.b, self.jw, r["tangent"],
+                                                      self.n_dof)
         return r
 
 
@@ -143,6 +149,114 @@ def explicit(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, cfl=Non
         hist["free"].append(np.sum(asm["free_energy"] * w))
         hist["stress"].append(asm["stress"].copy())
         hist["internal_force"].append(asm["internal_force"].copy())
+    out = {k: np.asarray(v) for k, v in hist.items()}
+    out["energy_residual"] = (out["kinetic"] + out["free"] + out["dissipated"] - out["external_work"]
+                              - (out["kinetic"][0] + out["free"][0]))
+    out["state"] = state
+    return out
+
+
+class NewmarkStepError(RuntimeError):
+    def __init__(self, reason, diagnostics):
+        super().__init__(f"Implicit Newmark step failed: {reason}.")
+        self.reason = reason
+        self.diagnostics = diagnostics
+
+
+def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, beta=0.25,
+            gamma=0.5):
+    """Implicit Newmark with full-step Newton and Armijo (``solve_material_implicit_newmark``,
+    ``material.py:300-443``; ``_solve_implicit_step`` ``:446-662``; ``_evaluate_implicit_trial``
+    ``:665-701``), fixed step (adaptivity disabled), consistent mass, SuperLU solves.
+
+    ``newton`` is an ``oracle.rve.Newton``.  Returns histories like
+    ``MaterialDynamicSolution`` (consistent kinetic energy ``:859-864``)."""
+    from scipy.sparse.linalg import spsolve
+
+    from oracle.rve import ArmijoExhausted, armijo_search
+
+    n = box.n_dof
+    M = hex8.consistent_mass(box.nodes, box.elements, box.density)
+    fixed = np.zeros(n, dtype=bool)
+    fixed[np.asarray(fixed_dofs)] = True
+    free = np.flatnonzero(~fixed)
+    Mff = M[free][:, free].tocsc()
+    w = box.jw.reshape(-1)
+    u = np.zeros(n)
+    v = np.zeros(n)
+    a = np.zeros(n)
+    state = box.init_state()
+    asm = box.assemble(u, u, dt, settings, state)
+    state = asm["state"]
+    a[free] = spsolve(Mff, (scale(0.0) * f_unit - asm["internal_force"])[free])
+    hist = {"u": [u.copy()], "v": [v.copy()], "a": [a.copy()], "kinetic": [0.5 * v @ (M @ v)],
+            "free": [np.sum(asm["free_energy"] * w)], "dissipated": [0.0], "external_work": [0.0],
+            "stress": [asm["stress"].copy()], "internal_force": [asm["internal_force"].copy()],
+            "newton_iterations": [0], "armijo_backtracks": [0]}
+
+    def trial(disp, up, ustar, vstar, t_dt, ext, committed):
+        acc = (disp - ustar) / (beta * t_dt**2)
+        vel = vstar + gamma * t_dt * acc
+        r = box.assemble(disp, up, t_dt, settings, committed, need_tangent=True)
+        res = M @ acc + r["internal_force"] - ext
+        return acc, vel, r, res
+
+    for s in range(n_steps):
+        t0, t1 = s * dt, (s + 1) * dt
+        ustar = u + dt * v + dt**2 * (0.5 - beta) * a
+        vstar = v + dt * (1.0 - gamma) * a
+        disp = ustar.copy()
+        disp[fixed] = 0.0
+        ext = scale(t1) * f_unit
+        r0 = None
+        last_inc = np.inf
+        backtracks = 0
+        for it in range(newton.max_iterations + 1):
+            acc, vel, r, res = trial(disp, u, ustar, vstar, dt, ext, state)
+            rn = float(np.linalg.norm(res[free]))
+            r0 = rn if r0 is None else r0
+            rtol = max(newton.residual_absolute_tolerance, newton.residual_relative_tolerance * r0)
+            itol = max(newton.increment_absolute_tolerance,
+                       newton.increment_relative_tolerance * max(float(np.linalg.norm(disp[free])), 1.0))
+            if rn <= rtol and (it == 0 or last_inc <= itol):
+                break
+            if it == newton.max_iterations:
+                raise NewmarkStepError("maximum_iterations_exceeded", {"iteration": it})
+            keff = (r["tangent_matrix"] + M / (beta * dt**2))[free][:, free].tocsc()
+            inc = spsolve(keff, -res[free])
+            if not np.all(np.isfinite(inc)):
+                raise NewmarkStepError("non_finite_increment", {"iteration": it})
+            inc_n = float(np.linalg.norm(inc))
+            if rn <= rtol and inc_n <= itol:
+                break
+            if newton.armijo.enabled:
+                def evaluate(alpha, base=disp, d=inc):
+                    cand = base.copy()
+                    cand[free] += alpha * d
+                    return trial(cand, u, ustar, vstar, dt, ext, state)[3][free]
+                try:
+                    alpha, bt, _ = armijo_search(res[free], evaluate, newton.armijo)
+                except ArmijoExhausted as exc:
+                    raise NewmarkStepError("armijo_exhausted", exc.diagnostics) from exc
+                disp[free] += alpha * inc
+                last_inc = float(np.linalg.norm(alpha * inc))
+                backtracks += bt
+            else:
+                disp[free] += inc
+                last_inc = inc_n
+        hist["dissipated"].append(hist["dissipated"][-1] + dt * np.sum(r["dissipation"] * w))
+        hist["external_work"].append(hist["external_work"][-1]
+                                     + 0.5 * (scale(t0) * f_unit + ext) @ (disp - u))
+        u, v, a, state = disp, vel, acc, r["state"]
+        hist["u"].append(u.copy())
+        hist["v"].append(v.copy())
+        hist["a"].append(a.copy())
+        hist["kinetic"].append(0.5 * v @ (M @ v))
+        hist["free"].append(np.sum(r["free_energy"] * w))
+        hist["stress"].append(r["stress"].copy())
+        hist["internal_force"].append(r["internal_force"].copy())
+        hist["newton_iterations"].append(it)
+        hist["armijo_backtracks"].append(backtracks)
     out = {k: np.asarray(v) for k, v in hist.items()}
     out["energy_residual"] = (out["kinetic"] + out["free"] + out["dissipated"] - out["external_work"]
                               - (out["kinetic"][0] + out["free"][0]))

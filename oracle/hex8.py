@@ -169,3 +169,24 @@ def lumped_mass(nodes, elements, density, order: int = 2):
     m_node = np.zeros(nodes.shape[0])
     np.add.at(m_node, elements.ravel(), me.ravel())
     return np.repeat(m_node, 3)
+
+
+def consistent_mass(nodes, elements, density, order: int = 2):
+    """``assemble_consistent_mass_matrix`` (``assembly/elasticity.py:50-70,109-126``):
+    M_e = sum_q rho N^T N w detJ with the bundle material's density, 3 dofs per
+    node; CSR (3 n_nodes)^2."""
+    pts, wts = gauss_rule(order)
+    vals, gref = shape_functions(pts)
+    x = nodes[elements]
+    jac = np.einsum("eai,qaj->eqij", x, gref)
+    det = np.linalg.det(jac)
+    me = float(density) * np.einsum("qa,qb,eq->eab", vals, vals, det * wts[None, :])
+    ne = elements.shape[0]
+    dofs = element_dofs(elements)  # (ne, 24) node-major 3 a + c
+    blk = np.zeros((ne, 24, 24))
+    for c in range(3):
+        blk[:, c::3, c::3] = me
+    rows = np.repeat(dofs[:, :, None], 24, axis=2).ravel()
+    cols = np.repeat(dofs[:, None, :], 24, axis=1).ravel()
+    n_dof = 3 * nodes.shape[0]
+    return coo_matrix((blk.ravel(), (rows, cols)), shape=(n_dof, n_dof)).tocsr()

@@ -34,7 +34,8 @@ def _setup(phased, precision="fp64", amplitude=60.0):
         agg = M.LinearElasticMaterial("agg", 2.0, 2.0 * E, 0.2)
         mat = M.PhasedMaterialModel("slab", (M.RVEPhase("mortar", j2), M.RVEPhase("agg", agg)),
                                     np.repeat(eph, 8), 1.5)
-        obox = OD.Box(LEN, DIV, [("mortar", oj2), ("agg", OJ.ElasticParams(2.0 * E, 0.2, 2.0))], eph)
+        obox = OD.Box(LEN, DIV, [("mortar", oj2), ("agg", OJ.ElasticParams(2.0 * E, 0.2, 2.0))], eph,
+                      density=1.5)
     else:
         eph = None
         mat = j2

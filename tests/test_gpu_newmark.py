@@ -1,0 +1,140 @@
+"""GPU parity of implicit Newmark material dynamics (solve_material_implicit_newmark)
+against the FP64 oracle restatement (consistent mass, Newton + Armijo, SuperLU):
+displacement / velocity / acceleration / stress / energy histories at 1e-9 relative
+and identical Newton iteration counts; the FE² implicit embedding equals direct
+dynamics within the reference's 1e-9 scaled tolerance."""
+
+import numpy as np
+import pytest
+
+from oracle import dynamics as OD
+from oracle import j2vp as OJ
+from oracle.rve import Armijo, Newton
+
+pytestmark = pytest.mark.gpu
+
+E, NU = 3339.9160679991464, 0.36994096308414537
+LEN, DIV = (0.6, 0.3, 0.3), (6, 3, 3)
+
+
+def _rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _case(phased, amplitude=60.0, dt=1e-3, steps=30):
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import solvers as S
+
+    j2 = M.J2ViscoplasticMaterial("mortar", 1.0, E, NU, 10.0, 80.0, 0.02, 10.0, 2.0)
+    oj2 = OJ.J2VPParams(E, NU, 10.0, 80.0, 0.02, 10.0, 2.0, 1.0)
+    n_el = DIV[0] * DIV[1] * DIV[2]
+    if phased:
+        eph = (np.random.default_rng(4).random(n_el) < 0.35).astype(np.int64)
+        agg = M.LinearElasticMaterial("agg", 2.0, 2.0 * E, 0.2)
+        mat = M.PhasedMaterialModel("slab", (M.RVEPhase("mortar", j2), M.RVEPhase("agg", agg)),
+                                    np.repeat(eph, 8), 1.5)
+        obox = OD.Box(LEN, DIV, [("mortar", oj2), ("agg", OJ.ElasticParams(2.0 * E, 0.2, 2.0))], eph,
+                      density=1.5)
+    else:
+        eph, mat = None, j2
+        obox = OD.Box(LEN, DIV, [("m", oj2)])
+    bundle = D.build_structured_box_bundle(LEN, DIV, mat, eph)
+    load = D.build_half_sine_pressure_load(bundle, amplitude, 0.02, patch=((1, 5), (0, 3)))
+    bottom = bundle.mesh_info.boundary_groups["bottom"].nodes
+    dofs = (3 * bottom[:, None] + np.arange(3)).ravel()
+    newton = S.NonlinearNewtonSettings(20, 1e-9, 1e-10, 1e-14, 1e-10, S.ArmijoSettings(True, 1e-4, 0.5, 20))
+    n = bundle.mesh_info.dof_map.size
+    sol = D.solve_material_implicit_newmark(
+        bundle, S.TimeIntegrationSettings("implicit_newmark", dt, steps), np.zeros(n), np.zeros(n),
+        S.DirichletDofSet(dofs, np.zeros(dofs.size)), load, "three_dimensional",
+        M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10), newton,
+        linear_settings=S.LinearSolverSettings(relative_tolerance=1e-13, max_iterations=20000))
+    ref = OD.newmark(obox, dt, steps, dofs, load.unit_force, OD.half_sine(amplitude, 0.02),
+                     OJ.UpdateSettings(50, 1e-10, 1e-12, 1e-10),
+                     Newton(20, 1e-9, 1e-10, 1e-14, 1e-10, Armijo(True, 1e-4, 0.5, 20)))
+    return sol, ref
+
+
+@pytest.mark.parametrize("phased", [False, True])
+def test_newmark_matches_oracle(gpu, phased):
+    sol, ref = _case(phased)
+    assert ref["dissipated"][-1] > 0.0
+    np.testing.assert_array_equal(sol.newton_iterations, ref["newton_iterations"])
+    assert _rel(sol.displacement, ref["u"]) < 1e-9
+    assert _rel(sol.velocity, ref["v"]) < 1e-9
+    assert _rel(sol.acceleration, ref["a"]) < 1e-8
+    assert _rel(sol.stresses, ref["stress"]) < 1e-9
+    assert _rel(sol.internal_force, ref["internal_force"]) < 1e-9
+    assert _rel(sol.kinetic_energy, ref["kinetic"]) < 1e-9
+    assert _rel(sol.free_energy, ref["free"]) < 1e-9
+    assert _rel(sol.dissipated_energy, ref["dissipated"]) < 1e-9
+    assert _rel(sol.external_work, ref["external_work"]) < 1e-9
+    assert np.max(np.abs(sol.energy_residual)) < 0.01 * np.max(np.abs(sol.external_work))
+
+
+def test_newmark_elastic_energy_conservation(gpu):
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import solvers as S
+
+    mat = M.LinearElasticMaterial("m", 1.0, E, NU)
+    bundle = D.build_structured_box_bundle(LEN, DIV, mat)
+    load = D.build_half_sine_pressure_load(bundle, 5.0, 0.02, patch=((1, 5), (0, 3)))
+    bottom = bundle.mesh_info.boundary_groups["bottom"].nodes
+    dofs = (3 * bottom[:, None] + np.arange(3)).ravel()
+    n = bundle.mesh_info.dof_map.size
+    sol = D.solve_material_implicit_newmark(
+        bundle, S.TimeIntegrationSettings("implicit_newmark", 2e-3, 30), np.zeros(n), np.zeros(n),
+        S.DirichletDofSet(dofs, np.zeros(dofs.size)), load, "three_dimensional",
+        M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10),
+        S.NonlinearNewtonSettings(20, 1e-9, 1e-10, 1e-14, 1e-10, S.ArmijoSettings(True)),
+        linear_settings=S.LinearSolverSettings(relative_tolerance=1e-13, max_iterations=20000))
+    w = np.max(np.abs(sol.external_work))
+    assert w > 0.0
+    assert np.max(np.abs(sol.energy_residual)) < 1e-8 * w
+
+
+def test_fe2_implicit_embedding_matches_direct(gpu):
+    """configs/benchmarks/rve/single_phase_dynamic_embedding.yaml, implicit part
+    (dt 1e-4, 4 steps), rve_embedding_tolerance 1e-9 of fixed scales."""
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import fe2
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import solvers as S
+
+    j2 = M.J2ViscoplasticMaterial("embedding_j2", 1.0, E, NU, 10.0, 80.0, 0.02, 10.0, 2.0)
+    lengths, div = (0.02, 0.01, 0.01), (2, 1, 1)
+    newton = S.NonlinearNewtonSettings(20, 1e-9, 1e-8, 1e-12, 1e-8, S.ArmijoSettings(True, 1e-4, 0.5, 20))
+    direct = D.build_structured_box_bundle(lengths, div, j2)
+    nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (1, 1, 1))
+    model = R.StructuredHex8RVE(nodes, elements, (1, 1, 1), j2,
+                                R.build_structured_hex8_periodic_constraint(nodes, (1, 1, 1)),
+                                newton, hill_mandel_power_scale=10.0)
+    mat = fe2.RVEHomogenizedMaterial(model, "single_phase_rve_material")
+    rve_bundle = D.build_structured_box_bundle(lengths, div, mat)
+    left = direct.mesh_info.boundary_groups["left"].nodes
+    dofs = (3 * left[:, None] + np.arange(3)).ravel()
+    load = D.build_half_sine_pressure_load(direct, 20.0, 2.0e-4, group="right",
+                                           outward_normal=(1.0, 0.0, 0.0))
+    n = direct.mesh_info.dof_map.size
+    args = (S.TimeIntegrationSettings("implicit_newmark", 1.0e-4, 4), np.zeros(n), np.zeros(n),
+            S.DirichletDofSet(dofs, np.zeros(dofs.size)), load, "three_dimensional",
+            M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10), newton)
+    lin = S.LinearSolverSettings(relative_tolerance=1e-14, max_iterations=20000)
+    try:
+        a = D.solve_material_implicit_newmark(direct, *args, linear_settings=lin)
+        b = D.solve_material_implicit_newmark(rve_bundle, *args, linear_settings=lin)
+    finally:
+        mat.close()
+    assert np.max(a.equivalent_plastic_strain) > 0.0
+
+    def sc(x, y, s):
+        return float(np.max(np.abs(np.asarray(x) - np.asarray(y))) / s)
+
+    assert sc(a.displacement, b.displacement, 0.01) <= 1e-9
+    assert sc(a.velocity, b.velocity, 1.0) <= 1e-9
+    assert sc(a.stresses, b.stresses, 10.0) <= 1e-9
+    assert sc(a.energy_residual, b.energy_residual, 0.01) <= 1e-9
+    assert sc(a.equivalent_plastic_strain, b.equivalent_plastic_strain, 0.01) <= 1e-9

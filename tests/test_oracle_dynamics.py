@@ -74,3 +74,64 @@ def test_cfl_rejection():
     with pytest.raises(ValueError):
         OD.explicit(box, 10.0 * 0.125 / c, 1, np.arange(3), np.zeros(box.n_dof), lambda t: 0.0,
                     OJ.UpdateSettings(), cfl=0.5, wave_speed=c)
+
+
+def _slab(mat, density=None, eph=None, phases=None):
+    phases = phases or [("m", mat)]
+    return OD.Box((0.6, 0.3, 0.3), (6, 3, 3), phases, eph, density=density)
+
+
+def _slab_load(box):
+    faces = OD.box_faces(box.divisions)
+    fixed = (3 * faces["bottom"][:, None] + np.arange(3)).ravel()
+    funit = OD.top_patch_unit_pressure(box.lengths, box.divisions, ((1, 5), (0, 3)))
+    return fixed, funit
+
+
+def test_consistent_mass_row_sums_and_total():
+    """Row sums of the consistent mass are the lumped mass (elasticity.py:73-77) and the
+    total is 3 rho V (one rigid translation per component)."""
+    from oracle import hex8
+
+    box = _slab(OJ.ElasticParams(1.0, 0.25, density=2.5))
+    M = hex8.consistent_mass(box.nodes, box.elements, 2.5)
+    np.testing.assert_allclose(np.asarray(M.sum(axis=1)).ravel(), box.mass, rtol=1e-14)
+    assert abs(M.sum() - 3 * 2.5 * 0.6 * 0.3 * 0.3) < 1e-13
+    # symmetric positive definite: smallest eigenvalue of a 1-element mass is rho V/216
+    one = OD.Box((1.0, 1.0, 1.0), (1, 1, 1), [("m", OJ.ElasticParams(1.0, 0.25, density=1.0))])
+    Me = hex8.consistent_mass(one.nodes, one.elements, 1.0).toarray()
+    assert np.allclose(Me, Me.T)
+    assert abs(np.linalg.eigvalsh(Me).min() - 1.0 / 216.0) < 1e-15
+
+
+def test_newmark_average_acceleration_conserves_elastic_energy():
+    """Average-acceleration Newmark (beta 1/4, gamma 1/2) on a linear elastic body
+    satisfies K + psi - W_ext = 0 to solver round-off with the trapezoidal work
+    measure of material.py:497-499; the reference bound is 0.01 of the work
+    (test_plasticity_benchmarks.py:58-75).  A wrong beta breaks it."""
+    from oracle.rve import Armijo, Newton
+
+    mat = OJ.ElasticParams(3339.9160679991464, 0.36994096308414537, density=1.0)
+    box = _slab(mat)
+    fixed, funit = _slab_load(box)
+    newton = Newton(20, 1e-9, 1e-10, 1e-14, 1e-10, Armijo(True))
+    out = OD.newmark(box, 2e-3, 30, fixed, funit, OD.half_sine(5.0, 0.02), OJ.UpdateSettings(), newton)
+    w = np.max(np.abs(out["external_work"]))
+    assert w > 0.0
+    assert np.max(np.abs(out["energy_residual"])) < 1e-9 * w
+    bad = OD.newmark(box, 2e-3, 30, fixed, funit, OD.half_sine(5.0, 0.02), OJ.UpdateSettings(),
+                     newton, beta=0.3, gamma=0.6)
+    assert np.max(np.abs(bad["energy_residual"])) > 1e-4 * w
+
+
+def test_newmark_viscoplastic_bound_and_dissipation():
+    from oracle.rve import Armijo, Newton
+
+    mat = OJ.J2VPParams(3339.9160679991464, 0.36994096308414537, 10.0, 80.0, 0.02, 10.0, 2.0, 1.0)
+    box = _slab(mat)
+    fixed, funit = _slab_load(box)
+    newton = Newton(20, 1e-9, 1e-10, 1e-14, 1e-10, Armijo(True))
+    out = OD.newmark(box, 1e-3, 40, fixed, funit, OD.half_sine(60.0, 0.02), OJ.UpdateSettings(), newton)
+    assert out["dissipated"][-1] > 0.0
+    assert np.all(np.diff(out["dissipated"]) >= -1e-14)
+    assert np.max(np.abs(out["energy_residual"])) < 0.01 * np.max(np.abs(out["external_work"]))
