@@ -315,6 +315,19 @@ _SIGNATURES.update({
         C.c_int32,
         [_vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_double,
          C.c_double, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "cia_mlp_linear_forward": (
+        C.c_int32, [_vp, C.c_int64, C.c_int64, C.c_int64, _vp, _vp, _vp, C.c_int32, _vp, _vp, _vp]),
+    "cia_mlp_linear_backward_data": (
+        C.c_int32, [_vp, C.c_int64, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp]),
+    "cia_mlp_linear_backward_weight": (
+        C.c_int32, [_vp, C.c_int64, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp, C.c_int64,
+                    C.c_int32]),
+    "cia_mlp_backward_weight_workspace": (C.c_int64, [C.c_int64, C.c_int64]),
+    "cia_mlp_adam_step": (
+        C.c_int32, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float,
+                    C.c_float, C.c_int32, C.c_float]),
+    "cia_f32_to_bf16": (C.c_int32, [_vp, C.c_int64, _vp, _vp]),
+    "cia_mlp_forward_fused": (C.c_int32, [_vp, C.c_int64, _vp, _vp, _vp, _vp]),
 })
 
 PROFILE_SLOTS = ["assemble", "pcg_elem", "pcg_node", "pcg_update", "precond", "gather"]

@@ -1,0 +1,431 @@
+// Recurrent MLP material-point surrogate: BF16 MFMA GEMMs with FP32 accumulation
+// and fused bias / GELU / GELU' epilogues, split-K weight gradients, Adam, and a
+// fused all-layer inference kernel.
+//
+// The surrogate plays the role of the reference's recurrent material-point
+// network (EnergyDissipationRVERNO, src/concrete_impact/nn/models/rve_rno.py:162-476,
+// trained by BPTT in src/concrete_impact/nn/training/rve_rno.py:113-415) with the
+// architecture 13 -> 256 -> 256 -> 256 -> 256 -> 12 (GELU): input = strain
+// increment (6), log strain-rate feature (1), latent state (6); output = stress (6),
+// next latent state (6).
+//
+// Matrix cores: v_mfma_f32_16x16x32_bf16.  Operand layout per 64-lane wave:
+//   A (16 x 32): lane l holds row l % 16, k = 8 (l / 16) .. 8 (l / 16) + 7
+//   B (32 x 16): lane l holds col l % 16, k = 8 (l / 16) .. + 7
+//   D (16 x 16): lane l holds col l % 16, rows 4 (l / 16) .. + 3
+// (checked against a float64 reference by tests/test_gpu_mlp.py).
+#include <math.h>
+#include <string.h>
+
+#include "common.hpp"
+
+namespace cia {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int GM = 128, GN = 128, GK = 32, GLD = GK + 8;  // LDS row stride 80 B
+
+enum { EPI_LINEAR = 0, EPI_GELU = 1, EPI_DGELU = 2 };
+
+struct GemmArgs {
+  int64_t M, N, K;
+  const __bf16* A;
+  int64_t sAm, sAk;  // A(m, k) = A[m sAm + k sAk]
+  const __bf16* B;
+  int64_t sBk, sBn;  // B(k, n) = B[k sBk + n sBn]
+  int64_t ones_col;  // column n of B that reads 1 (bias gradient), -1 none
+  const float* bias; // [N] added in the epilogue (split 0 only), or null
+  const float* zin;  // EPI_DGELU: pre-activation z(m, n) = zin[m ldzin + n]
+  int64_t ldzin;
+  float* Zf;         // EPI_GELU: pre-activation out (fp32), or null
+  int64_t ldzf;
+  float* Cf;         // fp32 out, or null
+  int64_t ldcf;
+  __bf16* Cb;        // bf16 out, or null
+  int64_t ldcb;
+  int64_t k_split;   // K range per blockIdx.z (multiple of GK)
+  int64_t split_stride;  // Cf offset per split
+};
+
+__device__ __forceinline__ float gelu(float v) {
+  return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[GM * GLD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[GN * GLD];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t m0 = (int64_t)blockIdx.x * GM, n0 = (int64_t)blockIdx.y * GN;
+  const int64_t kb = (int64_t)blockIdx.z * g.k_split;
+  const int64_t ke = (kb + g.k_split < g.K) ? kb + g.k_split : g.K;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  const __bf16 zero = (__bf16)0.0f, one = (__bf16)1.0f;
+  for (int64_t k0 = kb; k0 < ke; k0 += GK) {
+    if (g.sAk == 1) {
+      for (int i = tid; i < GM * GK; i += 256) {
+        const int r = i / GK, c = i % GK;
+        const int64_t m = m0 + r, k = k0 + c;
+        As[r * GLD + c] = (m < g.M && k < ke) ? g.A[m * g.sAm + k] : zero;
+      }
+    } else {
+      for (int i = tid; i < GM * GK; i += 256) {
+        const int r = i % GM, c = i / GM;
+        const int64_t m = m0 + r, k = k0 + c;
+        As[r * GLD + c] = (m < g.M && k < ke) ? g.A[m * g.sAm + k * g.sAk] : zero;
+      }
+    }
+    if (g.sBk == 1) {
+      for (int i = tid; i < GN * GK; i += 256) {
+        const int r = i / GK, c = i % GK;
+        const int64_t n = n0 + r, k = k0 + c;
+        __bf16 v = zero;
+        if (n < g.N && k < ke) v = (n == g.ones_col) ? one : g.B[k + n * g.sBn];
+        Bs[r * GLD + c] = v;
+      }
+    } else {
+      for (int i = tid; i < GN * GK; i += 256) {
+        const int r = i % GN, c = i / GN;
+        const int64_t n = n0 + r, k = k0 + c;
+        __bf16 v = zero;
+        if (n < g.N && k < ke) v = (n == g.ones_col) ? one : g.B[k * g.sBk + n * g.sBn];
+        Bs[r * GLD + c] = v;
+      }
+    }
+    __syncthreads();
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
+    bf16x8 af[4], bfr[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8*)&As[(wm + 16 * i + lr) * GLD + lk];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8*)&Bs[(wn + 16 * j + lr) * GLD + lk];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  float* Cf = g.Cf ? g.Cf + (int64_t)blockIdx.z * g.split_stride : nullptr;
+  const bool add_bias = g.bias && blockIdx.z == 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t n = n0 + wn + 16 * j + (lane & 15);
+      if (n >= g.N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        if (m >= g.M) continue;
+        float v = acc[i][j][r];
+        if (add_bias) v += g.bias[n];
+        if (EPI == EPI_GELU) {
+          if (g.Zf) g.Zf[m * g.ldzf + n] = v;
+          v = gelu(v);
+        } else if (EPI == EPI_DGELU) {
+          v *= gelu_grad(g.zin[m * g.ldzin + n]);
+        }
+        if (Cf) Cf[m * g.ldcf + n] = v;
+        if (g.Cb) g.Cb[m * g.ldcb + n] = (__bf16)v;
+      }
+    }
+}
+
+constexpr int64_t kMaxWgradSplits = 64;
+
+// Sum the split partials [S][N][Kx] into dW [N][K] (column K -> db), fixed order.
+__global__ void k_wgrad_reduce(int S, int64_t N, int64_t K, int64_t Kx, const float* __restrict__ part,
+                               float* __restrict__ dW, float* __restrict__ db, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = N * Kx;
+  if (i >= per) return;
+  float a = 0.f;
+  for (int s = 0; s < S; ++s) a += part[s * per + i];
+  const int64_t n = i / Kx, k = i - n * Kx;
+  float* dst = (k < K) ? &dW[n * K + k] : &db[n];
+  *dst = accumulate ? *dst + a : a;
+}
+
+// Adam (Kingma & Ba 2015) with bias correction; refreshes the bf16 copy.
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ grad,
+                       float* __restrict__ m1, float* __restrict__ m2, __bf16* __restrict__ pb,
+                       float lr, float b1, float b2, float eps, float c1, float c2, float wd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gr = grad[i] + wd * p[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gr;
+  const float b = b2 * m2[i] + (1.f - b2) * gr * gr;
+  m1[i] = a;
+  m2[i] = b;
+  const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  p[i] = v;
+  if (pb) pb[i] = (__bf16)v;
+}
+
+__global__ void k_f32_to_bf16(int64_t n, const float* __restrict__ x, __bf16* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (__bf16)x[i];
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: 13 -> 256 -> 256 -> 256 -> 256 -> 12 for a tile of FM points
+// per workgroup.  Activations (bf16) alternate between two LDS buffers; each of
+// the 4 waves computes a 64-column slice of a hidden layer with B fragments read
+// from the weights in global memory (shared by every workgroup, so L2-resident)
+// as one 16-byte load per lane and MFMA subtile.
+constexpr int FM = 128;          // points per workgroup
+constexpr int FH = 256;          // hidden width
+constexpr int FIN = 13, FOUT = 12;
+constexpr int FLD = FH + 8;      // LDS row stride (bf16), 528 B
+
+__global__ void __launch_bounds__(256) k_mlp_fused(int64_t M, const float* __restrict__ X,
+                                                   const __bf16* __restrict__ Wb,
+                                                   const float* __restrict__ bias,
+                                                   float* __restrict__ Y) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [2][FM][FLD]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const int64_t m0 = (int64_t)blockIdx.x * FM;
+  __bf16* a0 = act;
+  __bf16* a1 = act + FM * FLD;
+  // inputs -> bf16, columns 13..31 zero
+  for (int i = tid; i < FM * 32; i += 256) {
+    const int r = i >> 5, c = i & 31;
+    const int64_t m = m0 + r;
+    a0[r * FLD + c] = (__bf16)((m < M && c < FIN) ? X[m * FIN + c] : 0.0f);
+  }
+  __syncthreads();
+  const __bf16* W1 = Wb;
+  const __bf16* Wh = Wb + FH * FIN;               // 3 x [256][256]
+  const __bf16* W5 = Wh + 3 * FH * FH;            // [12][256]
+  const int wn = wave * 64;
+  // ---- layer 1 (K = 13, one k-step of 32) and hidden layers 2..4
+  for (int layer = 0; layer < 4; ++layer) {
+    const __bf16* src = (layer & 1) ? a1 : a0;
+    __bf16* dst = (layer & 1) ? a0 : a1;
+    f32x4 acc[FM / 16][4];
+#pragma unroll
+    for (int i = 0; i < FM / 16; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int ksteps = layer == 0 ? 1 : FH / 32;
+    for (int ks = 0; ks < ksteps; ++ks) {
+      bf16x8 bfr[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int n = wn + 16 * j + lr;
+        if (layer == 0) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const int k = lk + t;
+            bfr[j][t] = k < FIN ? W1[n * FIN + k] : (__bf16)0.0f;
+          }
+        } else {
+          bfr[j] = *(const bf16x8*)&Wh[(int64_t)(layer - 1) * FH * FH + n * FH + 32 * ks + lk];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < FM / 16; ++i) {
+        const bf16x8 af = *(const bf16x8*)&src[(16 * i + lr) * FLD + 32 * ks + lk];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[j], acc[i][j], 0, 0, 0);
+      }
+    }
+    const float* b = bias + layer * FH;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = wn + 16 * j + lr;
+      const float bn = b[n];
+#pragma unroll
+      for (int i = 0; i < FM / 16; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * i + 4 * (lane >> 4) + r;
+          dst[row * FLD + n] = (__bf16)gelu(acc[i][j][r] + bn);
+        }
+    }
+    __syncthreads();
+  }
+  // ---- output layer: wave w takes rows 32 w .. 32 w + 31 (two 16-row subtiles)
+  {
+    const __bf16* src = a0;  // after 4 layers the result is back in a0
+    f32x4 acc[2] = {(f32x4){0.f, 0.f, 0.f, 0.f}, (f32x4){0.f, 0.f, 0.f, 0.f}};
+    for (int ks = 0; ks < FH / 32; ++ks) {
+      bf16x8 bfr;
+      if (lr < FOUT)
+        bfr = *(const bf16x8*)&W5[lr * FH + 32 * ks + lk];
+      else
+#pragma unroll
+        for (int t = 0; t < 8; ++t) bfr[t] = (__bf16)0.0f;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const bf16x8 af = *(const bf16x8*)&src[(32 * wave + 16 * i + lr) * FLD + 32 * ks + lk];
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc[i], 0, 0, 0);
+      }
+    }
+    if (lr < FOUT) {
+      const float bn = bias[4 * FH + lr];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t m = m0 + 32 * wave + 16 * i + 4 * (lane >> 4) + r;
+          if (m < M) Y[m * FOUT + lr] = acc[i][r] + bn;
+        }
+    }
+  }
+}
+
+}  // namespace cia
+
+extern "C" {
+
+using cia::GemmArgs;
+
+static cia_status gemm_launch(hipStream_t st, int epi, GemmArgs g, int splits) {
+  using namespace cia;
+  const unsigned gx = (unsigned)((g.M + GM - 1) / GM), gy = (unsigned)((g.N + GN - 1) / GN);
+  dim3 grid(gx, gy, (unsigned)splits);
+  if (epi == EPI_GELU)
+    hipLaunchKernelGGL(k_gemm_bf16<EPI_GELU>, grid, dim3(256), 0, st, g);
+  else if (epi == EPI_DGELU)
+    hipLaunchKernelGGL(k_gemm_bf16<EPI_DGELU>, grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_gemm_bf16<EPI_LINEAR>, grid, dim3(256), 0, st, g);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_mlp_linear_forward(void* stream, int64_t M, int64_t N, int64_t K, const void* X,
+                                  const void* W, const float* bias, int32_t activation,
+                                  float* Z, float* Y, void* Y_bf16) {
+  using namespace cia;
+  CIA_ARG(X && W && M >= 0 && N > 0 && K > 0, "invalid arguments to cia_mlp_linear_forward");
+  CIA_ARG(Y || Y_bf16, "cia_mlp_linear_forward needs an output");
+  if (M == 0) return CIA_OK;
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = (const __bf16*)X; g.sAm = K; g.sAk = 1;
+  g.B = (const __bf16*)W; g.sBk = 1; g.sBn = K;  // B(k, n) = W[n][k]
+  g.ones_col = -1;
+  g.bias = bias;
+  g.Zf = activation ? Z : nullptr; g.ldzf = N;
+  g.Cf = Y; g.ldcf = N;
+  g.Cb = (__bf16*)Y_bf16; g.ldcb = N;
+  g.k_split = ((K + GK - 1) / GK) * GK;
+  return gemm_launch((hipStream_t)stream, activation ? EPI_GELU : EPI_LINEAR, g, 1);
+}
+
+cia_status cia_mlp_linear_backward_data(void* stream, int64_t M, int64_t N, int64_t K,
+                                        const void* dZ, const void* W, const float* Zprev,
+                                        float* dX, void* dX_bf16) {
+  using namespace cia;
+  CIA_ARG(dZ && W && M >= 0 && N > 0 && K > 0, "invalid arguments to cia_mlp_linear_backward_data");
+  CIA_ARG(dX || dX_bf16, "cia_mlp_linear_backward_data needs an output");
+  if (M == 0) return CIA_OK;
+  GemmArgs g{};
+  // dX[M][K] = dZ[M][N] W[N][K]: GEMM with reduction over N
+  g.M = M; g.N = K; g.K = N;
+  g.A = (const __bf16*)dZ; g.sAm = N; g.sAk = 1;
+  g.B = (const __bf16*)W; g.sBk = K; g.sBn = 1;  // B(n', k') = W[n'][k']
+  g.ones_col = -1;
+  g.zin = Zprev; g.ldzin = K;
+  g.Cf = dX; g.ldcf = K;
+  g.Cb = (__bf16*)dX_bf16; g.ldcb = K;
+  g.k_split = ((N + GK - 1) / GK) * GK;
+  return gemm_launch((hipStream_t)stream, Zprev ? EPI_DGELU : EPI_LINEAR, g, 1);
+}
+
+cia_status cia_mlp_linear_backward_weight(void* stream, int64_t M, int64_t N, int64_t K,
+                                          const void* dZ, const void* X, float* dW, float* db,
+                                          float* workspace, int64_t workspace_floats,
+                                          int32_t accumulate) {
+  using namespace cia;
+  CIA_ARG(dZ && X && dW && M > 0 && N > 0 && K > 0 && workspace,
+          "invalid arguments to cia_mlp_linear_backward_weight");
+  // dW[N][K] (and db as an extra column of ones in X) = dZ^T X: a GEMM whose
+  // reduction runs over the M rows, split along M into fp32 partials.
+  const int64_t Kx = K + (db ? 1 : 0);
+  const int64_t per = N * Kx;
+  const int64_t tiles = ((N + GM - 1) / GM) * ((Kx + GN - 1) / GN);
+  int64_t splits = std::min<int64_t>(kMaxWgradSplits, std::max<int64_t>(1, 512 / tiles));
+  splits = std::min<int64_t>(splits, (M + GK - 1) / GK);
+  splits = std::min<int64_t>(splits, workspace_floats / per - 1);
+  CIA_ARG(splits >= 1, "workspace too small for cia_mlp_linear_backward_weight");
+  int64_t ks = (M + splits - 1) / splits;
+  ks = ((ks + GK - 1) / GK) * GK;
+  splits = (M + ks - 1) / ks;
+  GemmArgs g{};
+  g.M = N; g.N = Kx; g.K = M;
+  g.A = (const __bf16*)dZ; g.sAm = 1; g.sAk = N;  // A(n, m) = dZ[m][n]
+  g.B = (const __bf16*)X; g.sBk = K; g.sBn = 1;   // B(m, k) = X[m][k]
+  g.ones_col = db ? K : -1;
+  g.Cf = workspace; g.ldcf = Kx;
+  g.k_split = ks;
+  g.split_stride = per;
+  hipStream_t st = (hipStream_t)stream;
+  CIA_TRY(gemm_launch(st, EPI_LINEAR, g, (int)splits));
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(per)), dim3(kBlock), 0, st, (int)splits, N, K,
+                     Kx, (const float*)workspace, dW, db, accumulate);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+int64_t cia_mlp_backward_weight_workspace(int64_t N, int64_t K) {
+  return (cia::kMaxWgradSplits + 1) * N * (K + 1);
+}
+
+cia_status cia_mlp_adam_step(void* stream, int64_t n, float* params, const float* grad, float* m1,
+                             float* m2, void* params_bf16, float lr, float beta1, float beta2,
+                             float eps, int32_t step, float weight_decay) {
+  using namespace cia;
+  CIA_ARG(params && grad && m1 && m2 && n > 0 && step >= 1, "invalid arguments to cia_mlp_adam_step");
+  const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, params,
+                     grad, m1, m2, (__bf16*)params_bf16, lr, beta1, beta2, eps, c1, c2,
+                     weight_decay);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_f32_to_bf16(void* stream, int64_t n, const float* x, void* y) {
+  using namespace cia;
+  CIA_ARG(x && y && n >= 0, "invalid arguments to cia_f32_to_bf16");
+  if (n == 0) return CIA_OK;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, n, x,
+                     (__bf16*)y);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const void* weights,
+                                 const float* biases, float* Y) {
+  using namespace cia;
+  CIA_ARG(X && weights && biases && Y && M >= 0, "invalid arguments to cia_mlp_forward_fused");
+  if (M == 0) return CIA_OK;
+  const size_t lds = 2 * (size_t)FM * FLD * sizeof(__bf16);
+  static bool attr_set = false;
+  if (!attr_set) {
+    CIA_HIP(hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)((M + FM - 1) / FM)), dim3(256), lds,
+                     (hipStream_t)stream, M, X, (const __bf16*)weights, biases, Y);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+}  // extern "C"

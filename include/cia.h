@@ -460,6 +460,47 @@ cia_status cia_box_newmark_solve(cia_box* box, void* stream, const double* tange
                                  double absolute_tolerance, int32_t* iterations,
                                  double* residual_norm);
 
+/* ------------------------------------------------------------------------ */
+/* Recurrent MLP surrogate (the material-point network role of
+ * EnergyDissipationRVERNO, src/concrete_impact/nn/models/rve_rno.py:162-476, and
+ * its BPTT training, nn/training/rve_rno.py:113-415) on BF16 matrix cores with
+ * FP32 accumulation.  Matrices are row-major; bf16 buffers are passed as void*.
+ *   cia_mlp_linear_forward:  Y = act(X W^T + b), X [M][K] bf16, W [N][K] bf16,
+ *     b [N] fp32 or NULL, activation 0 = identity, 1 = GELU (erf form); Z [M][N]
+ *     fp32 receives the pre-activation when activation = 1 (or NULL); Y [M][N]
+ *     fp32 and/or Y_bf16 [M][N].
+ *   cia_mlp_linear_backward_data: dX = dZ W, dZ [M][N] bf16, W [N][K] bf16; with
+ *     Zprev [M][K] fp32 the result is multiplied by GELU'(Zprev) (the gradient
+ *     with respect to the previous layer's pre-activation).
+ *   cia_mlp_linear_backward_weight: dW [N][K] = dZ^T X and db [N] = column sums
+ *     of dZ (db may be NULL), reduced over the M rows through fp32 split partials
+ *     in a workspace of cia_mlp_backward_weight_workspace(N, K) floats; with
+ *     accumulate = 1 the results are added to dW, db.
+ *   cia_mlp_adam_step: Adam with bias correction (step >= 1) and decoupled weight
+ *     decay folded into the gradient; refreshes the bf16 copy (may be NULL).
+ *   cia_mlp_forward_fused: the whole 13 -> 256^4 -> 12 network for M points in one
+ *     kernel (activations stay in LDS); X [M][13] fp32, Y [M][12] fp32; weights as
+ *     one bf16 blob W1 [256][13] | W2..W4 [256][256] | W5 [12][256] and biases as
+ *     one fp32 blob b1..b4 [256] | b5 [12].
+ */
+cia_status cia_mlp_linear_forward(void* stream, int64_t M, int64_t N, int64_t K, const void* X,
+                                  const void* W, const float* bias, int32_t activation, float* Z,
+                                  float* Y, void* Y_bf16);
+cia_status cia_mlp_linear_backward_data(void* stream, int64_t M, int64_t N, int64_t K,
+                                        const void* dZ, const void* W, const float* Zprev,
+                                        float* dX, void* dX_bf16);
+cia_status cia_mlp_linear_backward_weight(void* stream, int64_t M, int64_t N, int64_t K,
+                                          const void* dZ, const void* X, float* dW, float* db,
+                                          float* workspace, int64_t workspace_floats,
+                                          int32_t accumulate);
+int64_t cia_mlp_backward_weight_workspace(int64_t N, int64_t K);
+cia_status cia_mlp_adam_step(void* stream, int64_t n, float* params, const float* grad,
+                             float* m1, float* m2, void* params_bf16, float lr, float beta1,
+                             float beta2, float eps, int32_t step, float weight_decay);
+cia_status cia_f32_to_bf16(void* stream, int64_t n, const float* x, void* y);
+cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const void* weights,
+                                 const float* biases, float* Y);
+
 /* Kernel timing with HIP events on the launch stream (no reference
  * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element
  * pass, 2 PCG node gather, 3 PCG vector update, 4 preconditioner (spectral: FFT,
