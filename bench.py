@@ -131,6 +131,106 @@ def cpu_baseline(n: int, seed: int, newton_per_step: float, backtracks_per_step:
     }
 
 
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_POINT = 2 * (13 * 256 + 3 * 256 * 256 + 256 * 12)
+
+
+def surrogate_section(args) -> dict:
+    """Surrogate vs plain FE²: train the recurrent surrogate on FE² responses of a
+    two-phase RVE (edge ``--surrogate-rve``), report its stress error on held-out
+    paths, the FE² rate that produced the data (RVE solves/s, concurrent streams), the
+    fused-inference rate at 2^20 points with its matrix-core roofline, and the BPTT
+    iteration time at 4096 paths x 32 steps."""
+    import torch
+
+    from concrete_impact_ai_amd import surrogate as S
+    from concrete_impact_ai_amd import surrogate_data as SD
+
+    n, T = args.surrogate_rve, args.surrogate_steps
+    p_train, p_test = args.surrogate_paths, max(8, args.surrogate_paths // 6)
+    model = SD.two_phase_rve(n, seed=args.seed)
+    times, strains = SD.path_batch(p_train + p_test, T, seed=1000 + args.seed)
+    print(f"surrogate: FE² data {p_train + p_test} paths x {T} steps on {n}^3 RVEs",
+          file=sys.stderr, flush=True)
+    d_eps, rate, stress, fe2_rate = SD.rve_responses(model, times, strains, workers=4)
+    tr = slice(0, p_train)
+    te = slice(p_train, p_train + p_test)
+    scales = S.SurrogateScales.fit(d_eps[tr], rate[tr], stress[tr])
+    params = S.SurrogateParams(seed=args.seed)
+    print(f"surrogate: training {args.surrogate_iters} BPTT iterations", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = S.train(params, scales, d_eps[tr], rate[tr], stress[tr], iterations=args.surrogate_iters,
+                   batch_paths=min(96, p_train), window=T, lr=2e-3, seed=args.seed,
+                   log_every=max(1, args.surrogate_iters // 10))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    S.envelope(params, scales, d_eps[tr], rate[tr])
+    dev = torch.device("cuda")
+    pred, _ = S.rollout(params, scales, torch.as_tensor(d_eps[te], dtype=torch.float32, device=dev),
+                        torch.as_tensor(rate[te], dtype=torch.float32, device=dev), fused=True)
+    errs = SD.stress_errors(pred.double().cpu().numpy(), stress[te])
+    rms = SD.stress_rms_errors(pred.double().cpu().numpy(), stress[te])
+    train_errs = SD.stress_errors(
+        S.rollout(params, scales, torch.as_tensor(d_eps[tr][:32], dtype=torch.float32, device=dev),
+                  torch.as_tensor(rate[tr][:32], dtype=torch.float32, device=dev),
+                  fused=True)[0].double().cpu().numpy(), stress[tr][:32])
+    # fused inference at 2^20 points
+    M = 1 << 20
+    X = torch.randn((M, 13), dtype=torch.float32, device=dev) * 0.5
+    for _ in range(3):
+        S.forward_fused(X, params)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        S.forward_fused(X, params)
+    e1.record()
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / reps
+    tflops = MLP_FLOP_PER_POINT * M / (inf_ms * 1e-3) / 1e12
+    # one BPTT iteration at 4096 paths x 32 steps (synthetic inputs)
+    Bp, W = 4096, 32
+    de = np.random.default_rng(0).standard_normal((Bp, W, 6)) * scales.eps_scale * 0.3
+    rt = np.full((Bp, W), scales.rate_ref)
+    sg = np.random.default_rng(1).standard_normal((Bp, W, 6)) * scales.stress_scale * 0.3
+    tmp = S.SurrogateParams(seed=1)
+    S.train(tmp, scales, de, rt, sg, iterations=2, batch_paths=Bp, window=W, lr=1e-4)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    S.train(tmp, scales, de, rt, sg, iterations=5, batch_paths=Bp, window=W, lr=1e-4)
+    torch.cuda.synchronize()
+    bptt_ms = (time.perf_counter() - t1) / 5 * 1e3
+    pts_per_s = M / (inf_ms * 1e-3)
+    return {
+        "rve_edge": n,
+        "fe2_rve_solves_per_s": fe2_rate,
+        "fe2_data": f"{p_train + p_test} paths x {T} steps, 4 concurrent streams",
+        "train_paths": p_train,
+        "test_paths": p_test,
+        "train_iterations": args.surrogate_iters,
+        "train_seconds": train_s,
+        "loss_history": hist,
+        "test_stress_error_median": float(np.median(errs)),
+        "test_stress_error_p95": float(np.percentile(errs, 95)),
+        "test_stress_error_max": float(np.max(errs)),
+        "train_stress_error_median": float(np.median(train_errs)),
+        "test_stress_rms_error_median": float(np.median(rms)),
+        "test_stress_rms_error_p95": float(np.percentile(rms, 95)),
+        "error_definition": "per held-out path: max error = max_t |sigma_sur - sigma_FE2|_inf / "
+                            "max_t |sigma_FE2|_inf; rms error = sqrt(mean (sigma_sur - sigma_FE2)^2) "
+                            "/ max_t |sigma_FE2|_inf",
+        "inference_points_per_s": pts_per_s,
+        "inference_ms_2^20": inf_ms,
+        "inference_roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS},
+        "bptt_ms_4096x32": bptt_ms,
+        "speedup_vs_fe2_per_point_update": pts_per_s / fe2_rate,
+        "meets_2pct_rms_median": bool(np.median(rms) <= 0.02),
+        "meets_2pct_max_p95": bool(np.percentile(errs, 95) <= 0.02),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +245,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (HBM-counter calibration kernel)")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate section")
+    ap.add_argument("--surrogate-only", action="store_true")
+    ap.add_argument("--surrogate-rve", type=int, default=12)
+    ap.add_argument("--surrogate-steps", type=int, default=48)
+    ap.add_argument("--surrogate-paths", type=int, default=384)
+    ap.add_argument("--surrogate-iters", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -163,6 +269,9 @@ def main() -> int:
     lib = _lib.lib()
     if args.copy_calibration:
         copy_calibration(torch)
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": surrogate_section(args)}))
+        return 0
     n = args.size
     model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol, not args.fixed_forcing)
     times, strains = random_strain_path(200, 8, 3.0e-3, (10.0, 1000.0), seed=args.seed)
@@ -287,6 +396,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n, args.seed, newton / K, backtracks / K)
+    if not args.no_surrogate:
+        line["surrogate"] = surrogate_section(args)
     print(json.dumps(line))
     return 0
 

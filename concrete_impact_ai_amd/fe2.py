@@ -212,14 +212,16 @@ class RVEHomogenizedMaterial:
         nxt_w = torch.empty_like(committed.w)
         results: list = [None] * n
 
+        steps = np.broadcast_to(np.asarray(time_step, dtype=np.float64), (n,))
+
         def work(wid: int, stream_ptr):
             cache = self._caches[wid]
             for i in range(wid, n, self._workers):
                 try:
                     results[i] = solve_increment_raw(
                         cache, stream_ptr, committed.packed[i].data_ptr(), committed.macro[i],
-                        committed.w[i].data_ptr(), strains[i], time_step, settings, need_tangent,
-                        nxt_packed[i].data_ptr(), nxt_w[i].data_ptr())
+                        committed.w[i].data_ptr(), strains[i], float(steps[i]), settings,
+                        need_tangent, nxt_packed[i].data_ptr(), nxt_w[i].data_ptr())
                 except Exception as exc:  # first failure of this worker, by point order
                     return i, exc
             return None
@@ -303,6 +305,14 @@ class RVEHomogenizedMaterial:
                          if requirements.dissipation else None),
             diagnostics=self._diagnostics(results, nxt),
         )
+
+    def solve_batch(self, strains: np.ndarray, time_steps, settings, state: RVEBatchState,
+                    need_tangent: bool = False):
+        """Per-point time steps (the material protocol has one step for all points):
+        returns the per-point solver results and the next batch state."""
+        strains = np.ascontiguousarray(np.asarray(strains, dtype=np.float64))
+        return self._solve_all(strains, np.asarray(time_steps, dtype=np.float64), settings,
+                               bool(need_tangent), self._device_batch(state, strains.shape[0]))
 
     def device_kernel(self, point_phase_ids=None) -> "_RVEKernel":
         return _RVEKernel(self)

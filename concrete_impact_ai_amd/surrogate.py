@@ -272,14 +272,17 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
           batch_paths: int, window: int, lr: float = 1e-3, seed: int = 0, lr_final: float = 1e-5,
           log_every: int = 0):
     """BPTT on random windows of the training paths (``train_rve_rno``,
-    ``nn/training/rve_rno.py:200-415``: windows of the path batch, MSE of the
-    normalised stress).  Windows start at step 0 so the latent starts from zero as
-    in deployment.  Cosine learning-rate decay from ``lr`` to ``lr_final``."""
+    ``nn/training/rve_rno.py:200-415``).  The loss is the path-normalised stress MSE
+    of ``_batch_loss`` (``:540-638``): each path's squared errors are divided by its
+    peak squared stress, so low-amplitude paths weigh as much as high ones.  Windows
+    start at step 0 so the latent starts from zero as in deployment.  Cosine
+    learning-rate decay from ``lr`` to ``lr_final``."""
     dev = device()
     de = torch.as_tensor(d_eps, dtype=torch.float32, device=dev)
     rt = torch.as_tensor(rate, dtype=torch.float32, device=dev)
     sg = torch.as_tensor(stress, dtype=torch.float32, device=dev) / scales.stress_scale
     P, T, _ = de.shape
+    path_w = 1.0 / torch.clamp(sg.abs().amax(dim=(1, 2)), min=1e-3) ** 2   # [P]
     rng = np.random.default_rng(seed)
     w_leaf = params.w.detach().requires_grad_(True)
     b_leaf = params.b.detach().requires_grad_(True)
@@ -290,14 +293,14 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
         L = min(window, T)
         pred, _ = rollout(params, SurrogateScales(scales.eps_scale, scales.rate_ref, scales.rate_scale, 1.0),
                           de[idx, :L], rt[idx, :L], False, w_leaf, b_leaf)
-        loss = torch.mean((pred - sg[idx, :L]) ** 2)
+        loss = torch.mean(((pred - sg[idx, :L]) ** 2).mean(dim=(1, 2)) * path_w[idx])
         w_leaf.grad = None
         b_leaf.grad = None
         loss.backward()
         cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / max(iterations - 1, 1)))
         params.adam(w_leaf.grad, b_leaf.grad, cur)
         if log_every and (it % log_every == 0 or it == iterations - 1):
-            history.append((it, float(loss)))
+            history.append((it, float(loss.detach())))
     return history
 
 

@@ -1,0 +1,96 @@
+"""Surrogate training data from FE² RVE responses, and the surrogate-vs-FE² measurement.
+
+The reference produces its recurrent-surrogate training data by driving RVEs along
+generated macro-strain paths (``src/concrete_impact/experiments/rve_response_generation.py``,
+paths from ``experiments/rve_path_generation.py:34-234``).  Here a batch of paths is
+driven through the FE² adapter at once: at every step each path is one macro point
+whose RVE is solved on the device (concurrent streams), so a whole path batch
+advances per adapter call.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from concrete_impact_ai_amd import fe2
+from concrete_impact_ai_amd import materials as M
+from concrete_impact_ai_amd import rve as R
+from concrete_impact_ai_amd import solvers as S
+from concrete_impact_ai_amd.microstructure import random_strain_path, spherical_aggregates
+
+
+def two_phase_rve(n: int, seed: int, precision: str = "fp64") -> R.StructuredHex8RVE:
+    """Unit-cube two-phase RVE with the concrete parameters of the 48^3 benchmark
+    (elastic aggregates E 50 GPa at 40 % in a J2 Perzyna mortar E 25 GPa, sigma_y 20 MPa,
+    H 0.5 GPa, eta 1e-3 s, m 1), spheres of radius U[n/24, n/8] elements."""
+    eph = spherical_aggregates((n, n, n), 0.40, (max(1.0, n / 24.0), max(1.5, n / 8.0)), seed=seed)
+    mortar = M.J2ViscoplasticMaterial("mortar", 2400.0, 25.0e9, 0.2, 20.0e6, 0.5e9, 1.0e-3, 20.0e6, 1.0)
+    agg = M.LinearElasticMaterial("aggregate", 2400.0, 50.0e9, 0.2)
+    nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (n, n, n))
+    newton = S.NonlinearNewtonSettings(25, 1.0e-10 * 20.0e6, 1.0e-10, 1.0e-12, 1.0e-10,
+                                       S.ArmijoSettings(True, 1.0e-4, 0.5, 20))
+    lin = S.LinearSolverSettings(method="spectral", relative_tolerance=1e-10, check_interval=8,
+                                 adaptive_forcing=True)
+    return R.StructuredHex8RVE(
+        nodes, elements, (n, n, n),
+        M.PhasedMaterialModel("rve", (M.RVEPhase("mortar", mortar), M.RVEPhase("aggregate", agg)),
+                              np.repeat(eph, 8), 2400.0),
+        R.build_structured_hex8_periodic_constraint(nodes, (n, n, n)), newton,
+        element_phase_ids=eph, phase_names=("mortar", "aggregate"), linear_settings=lin,
+        precision=precision)
+
+
+def path_batch(n_paths: int, n_steps: int, seed: int, max_strain: float = 3.0e-3,
+               rate_range=(10.0, 1000.0), n_segments: int = 6):
+    """Seeded random piecewise-linear 6-component paths: times (P, T+1), strains (P, T+1, 6)."""
+    ts, es = [], []
+    for p in range(n_paths):
+        t, e = random_strain_path(n_steps, n_segments, max_strain, rate_range, seed=seed + p)
+        ts.append(t)
+        es.append(e)
+    return np.asarray(ts), np.asarray(es)
+
+
+def rve_responses(model, times, strains, workers: int = 4):
+    """Drive every path through its own RVE (one adapter point per path, each with
+    its own time step).  Returns d_eps (P, T, 6), rate (P, T), stress
+    (P, T, 6) and the measured RVE solves per second."""
+    P, T1, _ = strains.shape
+    T = T1 - 1
+    ad = fe2.RVEHomogenizedMaterial(model, "data", fe2.RVEExecutionSettings(
+        "streams" if workers > 1 else "serial", workers))
+    settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    stress = np.zeros((P, T, 6))
+    d_eps = np.diff(strains, axis=1)
+    dts = np.diff(times, axis=1)
+    rate = np.linalg.norm(d_eps, axis=2) / dts
+    try:
+        state = ad.initialize_state(P)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(T):
+            results, state = ad.solve_batch(strains[:, k + 1], dts[:, k], settings, state)
+            stress[:, k] = np.asarray([r.macro_stress[:] for r in results])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+    finally:
+        ad.close()
+    return d_eps, rate, stress, P * T / elapsed
+
+
+def stress_errors(pred, ref):
+    """Per path: max_t |sigma_pred - sigma_ref| / max_t |sigma_ref| (component-wise max norm)."""
+    num = np.max(np.abs(pred - ref), axis=(1, 2))
+    den = np.maximum(np.max(np.abs(ref), axis=(1, 2)), 1e-30)
+    return num / den
+
+
+def stress_rms_errors(pred, ref):
+    """Per path: sqrt(mean_t,c (sigma_pred - sigma_ref)^2) / max_t |sigma_ref|_inf (the
+    path-normalised RMS error that the training loss minimises)."""
+    num = np.sqrt(np.mean((pred - ref) ** 2, axis=(1, 2)))
+    den = np.maximum(np.max(np.abs(ref), axis=(1, 2)), 1e-30)
+    return num / den
