@@ -1,0 +1,176 @@
+// Small-size 3-D DFTs of the spectral preconditioner (periodic RVE grids of edge
+// n <= 64), in LDS, unnormalised, forward kernel exp(-2 pi i j k / n).
+//
+// Each 1-D transform uses one level of the Cooley-Tukey factorisation n = n1 n2
+// with direct sub-DFTs: with j = n2 j1 + j2 and k = k1 + n1 k2,
+//     X[k1 + n1 k2] = sum_j2 W_n2^(j2 k2) [ W_n^(j2 k1) sum_j1 x[n2 j1 + j2] W_n1^(j1 k1) ],
+// i.e. (n1 + n2) complex multiply-adds per output instead of n.  A prime n uses
+// n1 = 1 (a direct DFT).  One kernel per axis: x-lines are contiguous (several
+// lines per workgroup); y and z lines are read as tiles of 16 consecutive x
+// positions so every load row is 256 contiguous bytes.  All return at once when
+// the solver's converged flag is set.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+namespace cia {
+
+struct DftAxis {
+  int n, n1, n2, pad;
+  const double2* w;  // W_n1[n1] | W_n2[n2] | W_n[n], forward sign
+};
+
+inline void dft_factor(int n, int& n1, int& n2) {
+  n1 = 1;
+  for (int d = 1; d * d <= n; ++d)
+    if (n % d == 0) n1 = d;
+  n2 = n / n1;
+}
+
+// Host twiddle table of one axis (appended to `out`); returns its offset.
+inline size_t dft_twiddles(int n, std::vector<double2>& out) {
+  int n1, n2;
+  dft_factor(n, n1, n2);
+  const size_t off = out.size();
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int len : {n1, n2, n})
+    for (int m = 0; m < len; ++m) {
+      const double a = two_pi * (double)m / (double)len;
+      out.push_back(make_double2(std::cos(a), -std::sin(a)));
+    }
+  return off;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b, bool conj_b) {
+  const double bi = conj_b ? -b.y : b.y;
+  return make_double2(a.x * b.x - a.y * bi, a.x * bi + a.y * b.x);
+}
+
+// q = floor(t / n) for t < 2^16, n <= 64 without an integer divide: the float
+// quotient's fractional part stays >= 1/(2n) away from an integer, far above its
+// rounding error.
+__device__ __forceinline__ int fdiv(int t, float inv_n) { return (int)(((float)t + 0.5f) * inv_n); }
+
+// One stage over n_lines lines of length n: element e of line l at l*ls + e*es.
+// STEP_A: out[(j2 n1 + k1)] = W_n^(j2 k1) sum_j1 in[n2 j1 + j2] W_n1^(j1 k1)
+// STEP_B: out[(k1 + n1 k2)] = sum_j2 in[j2 n1 + k1] W_n2^(j2 k2)
+// (j1 k1 mod n1 and j2 k2 mod n2 advance incrementally; j2 k1 < n needs no mod.)
+// w: twiddles of the axis in LDS.
+template <bool STEP_A, bool INV>
+__device__ __forceinline__ void dft_stage(const double2* __restrict__ in, double2* __restrict__ out,
+                                          int n_lines, int ls, int es, const DftAxis& ax,
+                                          const double2* __restrict__ w) {
+  const int n = ax.n, n1 = ax.n1, n2 = ax.n2;
+  const double2* w1 = w;
+  const double2* w2 = w + n1;
+  const double2* wn = w + n1 + n2;
+  const float inv_n = 1.0f / (float)n, inv_n1 = 1.0f / (float)n1;
+  const int total = n_lines * n;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int line = fdiv(t, inv_n), r = t - line * n;
+    const double2* src = in + line * ls;
+    double2 acc = make_double2(0.0, 0.0);
+    if (STEP_A) {
+      const int j2 = fdiv(r, inv_n1), k1 = r - j2 * n1;
+      int m = 0;
+      for (int j1 = 0; j1 < n1; ++j1) {
+        const double2 p = cmul(src[(n2 * j1 + j2) * es], w1[m], INV);
+        acc.x += p.x;
+        acc.y += p.y;
+        m += k1;
+        if (m >= n1) m -= n1;
+      }
+      acc = cmul(acc, wn[j2 * k1], INV);
+    } else {
+      const int k2 = fdiv(r, inv_n1), k1 = r - k2 * n1;
+      int m = 0;
+      for (int j2 = 0; j2 < n2; ++j2) {
+        const double2 p = cmul(src[(j2 * n1 + k1) * es], w2[m], INV);
+        acc.x += p.x;
+        acc.y += p.y;
+        m += k2;
+        if (m >= n2) m -= n2;
+      }
+    }
+    out[line * ls + r * es] = acc;
+  }
+}
+
+__device__ __forceinline__ void load_twiddles(const DftAxis& ax, double2* w) {
+  const int nt = ax.n1 + ax.n2 + ax.n;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) w[t] = ax.w[t];
+}
+
+constexpr int kDftTile = 16;
+constexpr int kDftMaxTw = 64 + 16 + 64;
+
+// x pass: each workgroup transforms `lines` consecutive x-lines (contiguous in
+// memory) of the [2][N] array; dynamic LDS 2 * lines * (nx + 1) double2.
+template <bool INV>
+__global__ void __launch_bounds__(256)
+    k_dft_x(int nx, int64_t total_lines, int lines, DftAxis ax, double2* __restrict__ data,
+            const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 w[kDftMaxTw];
+  const int64_t l0 = (int64_t)blockIdx.x * lines;
+  const int nl = (int)min<int64_t>(lines, total_lines - l0);
+  const int ld = nx + 1;  // padded LDS row: consecutive lines start 4 banks apart
+  double2* A = dft_sm;
+  double2* T = dft_sm + lines * ld;
+  double2* g = data + l0 * nx;
+  const float inv_nx = 1.0f / (float)nx;
+  load_twiddles(ax, w);
+  for (int t = threadIdx.x; t < nl * nx; t += blockDim.x) {
+    const int l = fdiv(t, inv_nx);
+    A[l * ld + (t - l * nx)] = g[t];
+  }
+  __syncthreads();
+  dft_stage<true, INV>(A, T, nl, ld, 1, ax, w);
+  __syncthreads();
+  dft_stage<false, INV>(T, A, nl, ld, 1, ax, w);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nl * nx; t += blockDim.x) {
+    const int l = fdiv(t, inv_nx);
+    g[t] = A[l * ld + (t - l * nx)];
+  }
+}
+
+// Strided pass (y: stride nx, z: stride nx ny) over 16 consecutive x positions:
+// blockIdx.x = x tile, blockIdx.y = the other grid index (z for the y pass, y for
+// the z pass), blockIdx.z = component.  Dynamic LDS 2 * 16 (n + 1) double2.
+template <bool INV, bool ZPASS>
+__global__ void __launch_bounds__(256)
+    k_dft_strided(int nx, int ny, int nz, int64_t N, DftAxis ax, double2* __restrict__ data,
+                  const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 w[kDftMaxTw];
+  const int n = ax.n, ld = n + 1;  // one padded LDS row per line
+  double2* A = dft_sm;
+  double2* T = dft_sm + kDftTile * ld;
+  const int i0 = blockIdx.x * kDftTile, o = blockIdx.y;
+  const int64_t plane = (int64_t)nx * ny;
+  const int64_t stride = ZPASS ? plane : nx;
+  double2* g = data + (int64_t)blockIdx.z * N + i0 + (ZPASS ? (int64_t)nx * o : plane * o);
+  const int width = min(kDftTile, nx - i0);
+  load_twiddles(ax, w);
+  for (int t = threadIdx.x; t < kDftTile * n; t += blockDim.x) {
+    const int e = t / kDftTile, ii = t - e * kDftTile;
+    A[ii * ld + e] = ii < width ? g[ii + stride * e] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  dft_stage<true, INV>(A, T, kDftTile, ld, 1, ax, w);
+  __syncthreads();
+  dft_stage<false, INV>(T, A, kDftTile, ld, 1, ax, w);
+  __syncthreads();
+  for (int t = threadIdx.x; t < kDftTile * n; t += blockDim.x) {
+    const int e = t / kDftTile, ii = t - e * kDftTile;
+    if (ii < width) g[ii + stride * e] = A[ii * ld + e];
+  }
+}
+
+}  // namespace cia

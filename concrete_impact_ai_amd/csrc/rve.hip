@@ -27,7 +27,6 @@
 // block partials.
 #include <string.h>
 
-#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -35,6 +34,8 @@
 #include <vector>
 
 #include "common.hpp"
+#include "dft.hpp"
+#include <hip/hip_complex.h>
 #include "hex8.hpp"
 
 namespace cia {
@@ -47,7 +48,7 @@ struct Scalars {
   int32_t done;    // 0 running, 1 converged, 2 breakdown
   int32_t iters;
   int32_t first;   // next spectral apply is the initial one (beta = 0)
-  int32_t pad;
+  int32_t it_cur;  // index of the spectral PCG iteration in progress (set by the element pass)
 };
 
 struct Red {
@@ -57,6 +58,46 @@ struct Red {
 };
 
 // Last-block finalisation of NV block partial sums into out[0..NV).
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int P,
+                                                double out[NV], double* lds) {
+  __shared__ double bc[NV];
+#pragma unroll
+  for (int s = 0; s < NV; ++s) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < P; b += blockDim.x) a += part[s * P + b];
+    a = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) bc[s] = a;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < NV; ++s) out[s] = bc[s];
+}
+
+template <int NV>
+__device__ __forceinline__ void write_partials(const double (&v)[NV], double* __restrict__ part,
+                                               double* lds) {
+#pragma unroll
+  for (int s = 0; s < NV; ++s) {
+    const double t = block_sum<kBlock>(v[s], lds);
+    if (threadIdx.x == 0) part[s * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// One block: out[s] = sum of the NV partial rows written by write_partials.
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_finalize(const double* __restrict__ part, int P,
+                                                 double* __restrict__ out) {
+  __shared__ double lds[kBlock / 64];
+#pragma unroll
+  for (int s = 0; s < NV; ++s) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < P; b += blockDim.x) a += part[s * P + b];
+    a = block_sum<kBlock>(a, lds);
+    if (threadIdx.x == 0) out[s] = a;
+  }
+}
+
 template <int NV>
 __device__ __forceinline__ void finalize_sums(const double (&v)[NV], double* partials,
                                               unsigned* counter, double* out, int nblocks,
@@ -250,7 +291,7 @@ __global__ void k_diagnose(Grid G, const PhaseTable* __restrict__ tab,
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_gather(Grid G, const T* __restrict__ F, double* __restrict__ r, double* __restrict__ class0,
-             double* partials, unsigned* counter, double* norm2_out) {
+             double* partials) {
   __shared__ double lds[kBlock / 64];
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double acc = 0.0;
@@ -278,14 +319,13 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   const double v[1] = {acc};
-  bool last;
-  finalize_sums<1>(v, partials, counter, norm2_out, (int)gridDim.x, lds, &last);
+  write_partials<1>(v, partials, lds);
 }
 
-// ||x||^2, ||y||^2 (y may be null) -> out[0..1]
+// ||x||^2, ||y||^2 (y may be null) -> block partials [2][gridDim]
 __global__ void __launch_bounds__(kBlock)
     k_norms(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
-            double* partials, unsigned* counter, double* out) {
+            double* partials) {
   __shared__ double lds[kBlock / 64];
   double a = 0.0, b = 0.0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -298,8 +338,7 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   const double v[2] = {a, b};
-  bool last;
-  finalize_sums<2>(v, partials, counter, out, (int)gridDim.x, lds, &last);
+  write_partials<2>(v, partials, lds);
 }
 
 // out = alpha * x
@@ -592,49 +631,6 @@ __global__ void __launch_bounds__(kBlock)
 // shift is applied by the consumers (z(c) - z(0)), and r.z is evaluated by
 // Parseval on the half spectrum.
 
-// r = b, x = 0, p_old = 0; r.r and sum(r) -> scalars.
-__global__ void __launch_bounds__(kBlock)
-    k_spec_init(int64_t C, const double* __restrict__ b, double* __restrict__ x,
-                double* __restrict__ r, double* __restrict__ p_old,
-                hipDoubleComplex* __restrict__ uab, double rtol, double atol, Scalars* sc,
-                double* partials, unsigned* counter) {
-  __shared__ double lds[kBlock / 64];
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double rr = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  if (c < C) {
-    double rv[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      rv[m] = b[3 * c + m];
-      x[3 * c + m] = 0.0;
-      r[3 * c + m] = rv[m];
-      p_old[3 * c + m] = 0.0;
-      rr += rv[m] * rv[m];
-    }
-    s0 = rv[0];
-    s1 = rv[1];
-    s2 = rv[2];
-    uab[c] = make_hipDoubleComplex(rv[0], rv[1]);
-    uab[C + c] = make_hipDoubleComplex(rv[2], 0.0);
-  }
-  const double v[4] = {rr, s0, s1, s2};
-  double out[4];
-  bool last;
-  finalize_sums<4>(v, partials, counter, out, (int)gridDim.x, lds, &last);
-  if (last && threadIdx.x == 0) {
-    sc->rr = out[0];
-    sc->sumr[0] = out[1];
-    sc->sumr[1] = out[2];
-    sc->sumr[2] = out[3];
-    const double tol = fmax(rtol * sqrt(out[0]), atol);
-    sc->tol2 = tol * tol;
-    sc->iters = 0;
-    sc->beta = 0.0;
-    sc->first = 1;
-    sc->done = (out[0] <= sc->tol2) ? 1 : 0;
-  }
-}
-
 // Packed spectra: ua = FFT(r_x + i r_y), ub = FFT(r_z) over the full grid of N
 // modes.  For each pair (k, -k) one thread splits the component spectra
 // (Rx = (ua(k) + conj ua(-k)) / 2, Ry = (ua(k) - conj ua(-k)) / 2i, Rz = ub(k)),
@@ -669,96 +665,6 @@ __device__ __forceinline__ void spec_solve(const hipFloatComplex* __restrict__ q
   }
 }
 
-__global__ void __launch_bounds__(kBlock)
-    k_spec_apply(int64_t N, int nx, int ny, int nz, double inv_n,
-                 const hipFloatComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ ua,
-                 hipDoubleComplex* __restrict__ ub, Scalars* sc, double* partials,
-                 unsigned* counter) {
-  __shared__ double lds[kBlock / 64];
-  if (sc->done) return;
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double rz = 0.0;
-  if (k < N) {
-    const int64_t mk = neg_mode(k, nx, ny, nz);
-    if (k <= mk) {
-      const hipDoubleComplex a = ua[k], am = ua[mk], b = ub[k], bm = ub[mk];
-      const double s0 = sc->sumr[0], s1 = sc->sumr[1], s2 = sc->sumr[2];
-      // component spectra at k and -k (minus the pinned-class reaction sum r)
-      double ar[3] = {0.5 * (a.x + am.x) - s0, 0.5 * (a.y + am.y) - s1, b.x - s2};
-      double ai[3] = {0.5 * (a.y - am.y), -0.5 * (a.x - am.x), b.y};
-      const hipFloatComplex* q = kinv + 6 * k;
-      double zr[3], zi[3];
-      spec_solve(q, false, ar, ai, zr, zi);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        zr[m] *= inv_n;
-        zi[m] *= inv_n;
-      }
-      const double wk = (k == mk) ? 1.0 : 2.0;
-      rz += wk * (ar[0] * zr[0] + ai[0] * zi[0] + ar[1] * zr[1] + ai[1] * zi[1] +
-                  ar[2] * zr[2] + ai[2] * zi[2]);
-      // Z(-k) = conj Z(k) for real z
-      ua[k] = make_hipDoubleComplex(zr[0] - zi[1], zi[0] + zr[1]);
-      ub[k] = make_hipDoubleComplex(zr[2], zi[2]);
-      if (mk != k) {
-        ua[mk] = make_hipDoubleComplex(zr[0] + zi[1], -zi[0] + zr[1]);
-        ub[mk] = make_hipDoubleComplex(zr[2], -zi[2]);
-      }
-    }
-  }
-  const double v[1] = {rz};
-  double out[1];
-  bool last;
-  finalize_sums<1>(v, partials, counter, out, (int)gridDim.x, lds, &last);
-  if (last && threadIdx.x == 0) {
-    if (sc->first) {
-      sc->beta = 0.0;
-      sc->first = 0;
-    } else {
-      sc->beta = out[0] / sc->rz;
-    }
-    sc->rz = out[0];
-    if (!(out[0] > 0.0) || !isfinite(out[0])) sc->done = 2;
-  }
-}
-
-// x += alpha p, r -= alpha y; r.r and sum(r).
-__global__ void __launch_bounds__(kBlock)
-    k_spec_update(int64_t C, const double* __restrict__ p, const double* __restrict__ y,
-                  double* __restrict__ x, double* __restrict__ r, hipDoubleComplex* __restrict__ uab,
-                  Scalars* sc, double* partials, unsigned* counter) {
-  __shared__ double lds[kBlock / 64];
-  if (sc->done) return;
-  const double alpha = sc->alpha;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double rr = 0.0, s[3] = {0.0, 0.0, 0.0};
-  if (c < C) {
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      x[3 * c + m] += alpha * p[3 * c + m];
-      const double rv = r[3 * c + m] - alpha * y[3 * c + m];
-      r[3 * c + m] = rv;
-      rr += rv * rv;
-      s[m] = rv;
-    }
-    uab[c] = make_hipDoubleComplex(s[0], s[1]);
-    uab[C + c] = make_hipDoubleComplex(s[2], 0.0);
-  }
-  const double v[4] = {rr, s[0], s[1], s[2]};
-  double out[4];
-  bool last;
-  finalize_sums<4>(v, partials, counter, out, (int)gridDim.x, lds, &last);
-  if (last && threadIdx.x == 0) {
-    sc->rr = out[0];
-    sc->sumr[0] = out[1];
-    sc->sumr[1] = out[2];
-    sc->sumr[2] = out[3];
-    sc->iters += 1;
-    if (out[0] <= sc->tol2) sc->done = 1;
-    else if (!isfinite(out[0])) sc->done = 2;
-  }
-}
-
 
 // -------------------------------------------------------------------------
 // Spectral PCG with deferred reductions.  A kernel that produces a dot product
@@ -767,31 +673,6 @@ __global__ void __launch_bounds__(kBlock)
 // scalars and no kernel needs a grid-wide fence (the kernel boundary makes the
 // partials visible across XCDs).  Block 0 records the scalars for the host.
 
-template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int P,
-                                                double out[NV], double* lds) {
-  __shared__ double bc[NV];
-#pragma unroll
-  for (int s = 0; s < NV; ++s) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < P; b += blockDim.x) a += part[s * P + b];
-    a = block_sum<kBlock>(a, lds);
-    if (threadIdx.x == 0) bc[s] = a;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < NV; ++s) out[s] = bc[s];
-}
-
-template <int NV>
-__device__ __forceinline__ void write_partials(const double (&v)[NV], double* __restrict__ part,
-                                               double* lds) {
-#pragma unroll
-  for (int s = 0; s < NV; ++s) {
-    const double t = block_sum<kBlock>(v[s], lds);
-    if (threadIdx.x == 0) part[s * gridDim.x + blockIdx.x] = t;
-  }
-}
 
 // r = b, x = 0, p_old = 0, packed r; partials (r.r, sum r).
 __global__ void __launch_bounds__(kBlock)
@@ -824,7 +705,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Convergence test on the partials of the last update, then the symbol solve
-// of the packed spectra (see k_spec_apply above) and r.z partials.
+// of the packed spectra (split per mode pair as described above spec_solve) and
+// r.z partials.
 __global__ void __launch_bounds__(kBlock)
     k_sp_apply(int64_t N, int nx, int ny, int nz, double inv_n,
                const hipFloatComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ ua,
@@ -917,7 +799,7 @@ __global__ void __launch_bounds__(kBlock)
     k_sp_update(int64_t C, const double* __restrict__ p, const double* __restrict__ y,
                 double* __restrict__ x, double* __restrict__ r, hipDoubleComplex* __restrict__ uab,
                 const double* __restrict__ part_pq, int P, double* __restrict__ part_upd,
-                Scalars* __restrict__ sc, int it) {
+                Scalars* __restrict__ sc) {
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
   double pq[1];
@@ -926,6 +808,7 @@ __global__ void __launch_bounds__(kBlock)
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
     return;
   }
+  const int it = sc->it_cur;
   const double alpha = sc->rzr[it & 1] / pq[0];
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
@@ -954,7 +837,7 @@ __global__ void __launch_bounds__(kBlock)
     k_sp_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
               const T* __restrict__ tan, const hipDoubleComplex* __restrict__ zc,
               const double* __restrict__ p_old, double* __restrict__ p_new, T* __restrict__ F,
-              const double* __restrict__ part_rz, int P, Scalars* __restrict__ sc, int it) {
+              const double* __restrict__ part_rz, int P, Scalars* __restrict__ sc) {
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
   double rz[1];
@@ -963,8 +846,10 @@ __global__ void __launch_bounds__(kBlock)
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
     return;
   }
+  const int it = sc->iters;  // completed iterations: written only by k_sp_update
   const double beta = it == 0 ? 0.0 : rz[0] / sc->rzr[(it + 1) & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->it_cur = it;
     sc->rzr[it & 1] = rz[0];
     sc->rz = rz[0];
     sc->beta = beta;
@@ -1128,36 +1013,41 @@ enum {
   R_N = R_PHASE + 9 * CIA_MAX_PHASES
 };
 
-template <class T>
+// NPH >= number of phases: the per-phase sums use compile-time register indices.
+template <class T, int NPH>
 __global__ void __launch_bounds__(kBlock)
     k_response(Grid G, const uint8_t* __restrict__ ephase, double dt, const T* __restrict__ sig,
                const T* __restrict__ rate, const T* __restrict__ psi, const T* __restrict__ diss,
                const T* __restrict__ diag, const T* __restrict__ F, double* __restrict__ partials,
                double* __restrict__ mm_partials) {
+  constexpr int RN = R_PHASE + 9 * NPH;
   __shared__ double lds[kBlock / 64];
+  __shared__ double red[kBlock / 64][RN];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t np = G.E * 8;
-  double v[R_N];
+  double v[RN];
 #pragma unroll
-  for (int s = 0; s < R_N; ++s) v[s] = 0.0;
+  for (int s = 0; s < RN; ++s) v[s] = 0.0;
   double mn = INFINITY, mx = -INFINITY;
   if (e < G.E) {
     const int ph = ephase[e];
+    double pe[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // this element's phase sums
     for (int q = 0; q < 8; ++q) {
       const int64_t p = 8 * e + q;
       double mw = 0.0;
+#pragma unroll
       for (int m = 0; m < 6; ++m) {
         const double sg = (double)sig[m * np + p];
         v[R_SIG + m] += sg;
-        v[R_PHASE + 9 * ph + 1 + m] += sg;
+        pe[1 + m] += sg;
         mw += sg * (dt * (double)rate[m * np + p]);
       }
       v[R_MICRO_WORK] += mw;
       v[R_PSI] += (double)psi[p];
       v[R_DISS] += (double)diss[p];
-      v[R_PHASE + 9 * ph + 0] += 1.0;
-      v[R_PHASE + 9 * ph + 7] += (double)psi[p];
-      v[R_PHASE + 9 * ph + 8] += (double)diss[p];
+      pe[0] += 1.0;
+      pe[7] += (double)psi[p];
+      pe[8] += (double)diss[p];
       const double pw = (double)diag[CIA_D_INCREMENTAL_WORK_DENSITY * np + p];
       v[R_POS_WORK] += fmax(pw, 0.0);
       v[R_NEG_WORK] += fmin(pw, 0.0);
@@ -1168,6 +1058,13 @@ __global__ void __launch_bounds__(kBlock)
         v[R_MARGIN_N] += 1.0;
         mn = fmin(mn, mg);
         mx = fmax(mx, mg);
+      }
+    }
+#pragma unroll
+    for (int P = 0; P < NPH; ++P) {
+      if (ph == P) {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) v[R_PHASE + 9 * P + m] += pe[m];
       }
     }
     // virtual work stress: sum_a A(x_a)^T F_a with x_a the unwrapped node position
@@ -1185,9 +1082,19 @@ __global__ void __launch_bounds__(kBlock)
       v[R_VIRTUAL + 5] += 0.5 * y * fx + 0.5 * x * fy;
     }
   }
-  for (int s = 0; s < R_N; ++s) {
-    const double t = block_sum<kBlock>(v[s], lds);
-    if (threadIdx.x == 0) partials[s * gridDim.x + blockIdx.x] = t;
+  // one pass of wave sums, then a single LDS exchange
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int s = 0; s < RN; ++s) {
+    const double t = wave_sum(v[s]);
+    if (lane == 0) red[wid][s] = t;
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < RN; s += blockDim.x) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += red[w][s];
+    partials[s * gridDim.x + blockIdx.x] = t;
   }
   const double tmn = block_min<kBlock>(mn, lds);
   const double tmx = block_max<kBlock>(mx, lds);
@@ -1401,8 +1308,16 @@ struct cia_rve {
   bool have_fields = false;
   // spectral preconditioner
   int prec = CIA_PCG_BLOCK_JACOBI;
-  hipfftHandle fft = 0;
   bool fft_ready = false;
+  double2* tw = nullptr;             // DFT twiddles of the three axes
+  hipStream_t cap_stream = nullptr;  // private stream for graph capture
+  struct PairGraph {
+    int slot;
+    double* x;
+    hipGraphExec_t exec;
+  };
+  std::vector<PairGraph> graphs;     // captured PCG iteration pairs
+  cia::DftAxis ax{}, ay{}, az{};
   hipFloatComplex* kinv = nullptr;   // [N][6] inverse symbol
   hipDoubleComplex* uab = nullptr;   // [2][N] packed residual / preconditioned spectra
   double ref_bulk = 0.0, ref_shear = 0.0;
@@ -1424,6 +1339,31 @@ static cia_status dmalloc(void** p, size_t bytes) {
   return CIA_OK;
 }
 
+
+// Capture two spectral PCG iterations (even: p_old = pp[1], odd: p_old = pp[0])
+// on the library's private stream and cache the executable graph per (tangent
+// slot, solution vector).
+template <class F>
+static cia_status pcg_pair_graph(cia_rve* R, int slot, double* x, F&& iteration,
+                                 hipGraphExec_t* out) {
+  for (auto& e : R->graphs)
+    if (e.slot == slot && e.x == x) {
+      *out = e.exec;
+      return CIA_OK;
+    }
+  hipGraph_t g = nullptr;
+  CIA_HIP(hipStreamBeginCapture(R->cap_stream, hipStreamCaptureModeThreadLocal));
+  iteration(R->cap_stream, 0, -1);
+  iteration(R->cap_stream, 1, -1);
+  CIA_HIP(hipStreamEndCapture(R->cap_stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  CIA_HIP(e);
+  R->graphs.push_back({slot, x, exec});
+  *out = exec;
+  return CIA_OK;
+}
 
 template <class T>
 struct Ops {
@@ -1450,9 +1390,11 @@ struct Ops {
 
   static cia_status gather(cia_rve* R, hipStream_t st, double* r, double* class0, int slot_norm) {
     prof::begin(prof::GATHER, st);
-    hipLaunchKernelGGL(k_gather<T>, dim3(grid_for(R->C)), dim3(kBlock), 0, st, R->G,
-                       (const T*)R->F, r, class0, R->partials, R->counter,
-                       &R->sc->norm2[slot_norm]);
+    const unsigned g = grid_for(R->C);
+    hipLaunchKernelGGL(k_gather<T>, dim3(g), dim3(kBlock), 0, st, R->G, (const T*)R->F, r, class0,
+                       R->partials);
+    hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kBlock), 0, st, (const double*)R->partials,
+                       (int)g, &R->sc->norm2[slot_norm]);
     prof::end(st);
     CIA_HIP(hipGetLastError());
     return CIA_OK;
@@ -1468,15 +1410,32 @@ struct Ops {
     double* part_pq = R->partials;
     double* part_upd = R->partials + P;
     double* part_rz = R->partials + 5 * P;
-    if (hipfftSetStream(R->fft, st) != HIPFFT_SUCCESS) {
-      set_error_msg("hipfftSetStream failed");
-      return CIA_ERR_HIP;
-    }
-    auto fft = [&](int dir) -> cia_status {
-      if (hipfftExecZ2Z(R->fft, (hipfftDoubleComplex*)R->uab, (hipfftDoubleComplex*)R->uab, dir) !=
-          HIPFFT_SUCCESS) {
-        set_error_msg("hipfftExecZ2Z failed");
-        return CIA_ERR_HIP;
+    // 3-D DFT of the packed spectra [2][C] in place, one kernel per axis.
+    const int lines_x = std::max(1, 256 / R->nx);
+    const int64_t nlines_x = 2 * (int64_t)R->ny * R->nz;
+    const size_t lds_x = 2 * (size_t)lines_x * (R->nx + 1) * sizeof(double2);
+    const size_t lds_y = 2 * (size_t)kDftTile * (R->ny + 1) * sizeof(double2);
+    const size_t lds_z = 2 * (size_t)kDftTile * (R->nz + 1) * sizeof(double2);
+    const unsigned tiles = (unsigned)((R->nx + kDftTile - 1) / kDftTile);
+    const dim3 g_x((unsigned)((nlines_x + lines_x - 1) / lines_x));
+    const dim3 g_y(tiles, (unsigned)R->nz, 2), g_z(tiles, (unsigned)R->ny, 2);
+    const int32_t* dflag = &R->sc->done;
+    auto fft = [&](bool inverse) -> cia_status {
+      double2* d = (double2*)R->uab;
+      if (!inverse) {
+        hipLaunchKernelGGL(k_dft_x<false>, g_x, dim3(256), lds_x, st, R->nx, nlines_x, lines_x,
+                           R->ax, d, dflag);
+        hipLaunchKernelGGL((k_dft_strided<false, false>), g_y, dim3(256), lds_y, st, R->nx, R->ny,
+                           R->nz, C, R->ay, d, dflag);
+        hipLaunchKernelGGL((k_dft_strided<false, true>), g_z, dim3(256), lds_z, st, R->nx, R->ny,
+                           R->nz, C, R->az, d, dflag);
+      } else {
+        hipLaunchKernelGGL((k_dft_strided<true, true>), g_z, dim3(256), lds_z, st, R->nx, R->ny,
+                           R->nz, C, R->az, d, dflag);
+        hipLaunchKernelGGL((k_dft_strided<true, false>), g_y, dim3(256), lds_y, st, R->nx, R->ny,
+                           R->nz, C, R->ay, d, dflag);
+        hipLaunchKernelGGL(k_dft_x<true>, g_x, dim3(256), lds_x, st, R->nx, nlines_x, lines_x,
+                           R->ax, d, dflag);
       }
       return CIA_OK;
     };
@@ -1489,44 +1448,59 @@ struct Ops {
     prof::begin(prof::PRECOND, st);
     hipLaunchKernelGGL(k_sp_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1], R->uab,
                        part_upd, R->sc);
-    CIA_TRY(fft(HIPFFT_FORWARD));
+    CIA_TRY(fft(false));
     apply(1);
-    CIA_TRY(fft(HIPFFT_BACKWARD));
+    CIA_TRY(fft(true));
     prof::end(st);
     CIA_HIP(hipGetLastError());
-    int done = 0, it = 0, pi = 0;
-    // Launches after convergence exit at once but the library FFTs still run, so
-    // the first batch is sized by the iteration count of the same Newton index in
-    // the previous increment; later batches are short.
+    int done = 0, it = 0;
+    // One PCG iteration; the iteration index lives on the device (k_sp_elem reads
+    // the completed count, k_sp_update advances it), so a launch sequence does not
+    // depend on it and two iterations (p_old / p_new swapped) form a graph.
+    auto iteration = [&](hipStream_t s, int pi, int tag) {
+      double* p_old = R->pp[1 - pi];
+      double* p_new = R->pp[pi];
+      prof::begin(prof::PCG_ELEM, s, tag);
+      hipLaunchKernelGGL(k_sp_elem<T>, dim3(ge), dim3(kBlock), 0, s, R->G, R->d_tab, R->d_ephase,
+                         (const T*)R->tan[slot], (const hipDoubleComplex*)R->uab, p_old, p_new,
+                         (T*)R->F, (const double*)part_rz, P, R->sc);
+      prof::end(s);
+      prof::begin(prof::PCG_NODE, s, tag);
+      hipLaunchKernelGGL(k_sp_node<T>, dim3(gn), dim3(kBlock), 0, s, R->G, (const T*)R->F, p_new,
+                         R->py, part_pq, R->sc);
+      prof::end(s);
+      prof::begin(prof::PCG_UPDATE, s, tag);
+      hipLaunchKernelGGL(k_sp_update, dim3(gn), dim3(kBlock), 0, s, C, p_new, R->py, x, R->pr,
+                         R->uab, (const double*)part_pq, P, part_upd, R->sc);
+      prof::end(s);
+      prof::begin(prof::PRECOND, s);
+      hipStream_t keep = st;
+      st = s;
+      fft(false);
+      apply(0);
+      fft(true);
+      st = keep;
+      prof::end(s);
+    };
+    hipGraphExec_t pair = nullptr;
+    if (!prof::g.on && R->cap_stream) CIA_TRY(pcg_pair_graph(R, slot, x, iteration, &pair));
+    // Launches after convergence exit at once, so the first batch is sized by the
+    // iteration count of the same Newton index in the previous increment; later
+    // batches are short.  Graph batches are whole pairs of iterations.
     const int interval = std::max(1, R->lin.check_interval);
     const bool hinted = hint >= 0 && hint < cia_rve::kHints && R->its_hint[hint] > 0;
     int chunk = hinted ? R->its_hint[hint] : interval;
     Scalars hs;
     while (it < R->lin.max_iterations) {
-      const int n = std::min(chunk, R->lin.max_iterations - it);
+      int n = std::min(chunk, R->lin.max_iterations - it);
       chunk = hinted ? std::max(1, interval / 4) : interval;
-      for (int s = 0; s < n; ++s, ++it) {
-        double* p_old = R->pp[1 - pi];
-        double* p_new = R->pp[pi];
-        prof::begin(prof::PCG_ELEM, st, it);
-        hipLaunchKernelGGL(k_sp_elem<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_tab, R->d_ephase,
-                           (const T*)R->tan[slot], (const hipDoubleComplex*)R->uab, p_old, p_new,
-                           (T*)R->F, (const double*)part_rz, P, R->sc, it);
-        prof::end(st);
-        prof::begin(prof::PCG_NODE, st, it);
-        hipLaunchKernelGGL(k_sp_node<T>, dim3(gn), dim3(kBlock), 0, st, R->G, (const T*)R->F, p_new,
-                           R->py, part_pq, R->sc);
-        prof::end(st);
-        prof::begin(prof::PCG_UPDATE, st, it);
-        hipLaunchKernelGGL(k_sp_update, dim3(gn), dim3(kBlock), 0, st, C, p_new, R->py, x, R->pr,
-                           R->uab, (const double*)part_pq, P, part_upd, R->sc, it);
-        prof::end(st);
-        prof::begin(prof::PRECOND, st);
-        CIA_TRY(fft(HIPFFT_FORWARD));
-        apply(0);
-        CIA_TRY(fft(HIPFFT_BACKWARD));
-        prof::end(st);
-        pi = 1 - pi;
+      if (pair) {
+        // whole pairs; an odd final iteration below the limit runs on its own
+        int pairs = std::min((n + 1) / 2, (R->lin.max_iterations - it) / 2);
+        for (int s = 0; s < pairs; ++s, it += 2) CIA_HIP(hipGraphLaunch(pair, st));
+        if (pairs == 0) iteration(st, (it++) & 1, it);
+      } else {
+        for (int s = 0; s < n; ++s, ++it) iteration(st, it & 1, it);
       }
       CIA_HIP(hipGetLastError());
       CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -1617,8 +1591,11 @@ static cia_status read_norms(cia_rve* R, hipStream_t st, double out[4]) {
 }
 
 static cia_status norm2_of(cia_rve* R, hipStream_t st, int64_t n, const double* x, double* out) {
-  hipLaunchKernelGGL(k_norms, dim3(std::min<int64_t>(grid_for(n), 1024)), dim3(kBlock), 0, st, n, x,
-                     (const double*)nullptr, R->partials, R->counter, &R->sc->norm2[2]);
+  const unsigned g = (unsigned)std::min<int64_t>(grid_for(n), 1024);
+  hipLaunchKernelGGL(k_norms, dim3(g), dim3(kBlock), 0, st, n, x, (const double*)nullptr,
+                     R->partials);
+  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kBlock), 0, st, (const double*)R->partials,
+                     (int)g, &R->sc->norm2[2]);
   CIA_HIP(hipGetLastError());
   double v[4];
   CIA_TRY(read_norms(R, st, v));
@@ -1857,9 +1834,19 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
     const unsigned ge = grid_for(R->E);
     double* part = R->red;
     double* mm = R->red + (int64_t)R_N * ge;
-    hipLaunchKernelGGL(k_response<T>, dim3(ge), dim3(kBlock), 0, st, R->G, R->d_ephase, dt,
-                       (const T*)R->sig[cur], (const T*)R->rate_out, (const T*)R->psi_out,
-                       (const T*)R->diss_out, (const T*)R->diag_out, (const T*)R->F, part, mm);
+#define CIA_RESPONSE(NPH)                                                                    \
+  hipLaunchKernelGGL((k_response<T, NPH>), dim3(ge), dim3(kBlock), 0, st, R->G, R->d_ephase, dt, \
+                     (const T*)R->sig[cur], (const T*)R->rate_out, (const T*)R->psi_out,         \
+                     (const T*)R->diss_out, (const T*)R->diag_out, (const T*)R->F, part, mm)
+    if (R->n_phases <= 1)
+      CIA_RESPONSE(1);
+    else if (R->n_phases <= 2)
+      CIA_RESPONSE(2);
+    else if (R->n_phases <= 4)
+      CIA_RESPONSE(4);
+    else
+      CIA_RESPONSE(8);
+#undef CIA_RESPONSE
     CIA_HIP(hipGetLastError());
     const unsigned gfull = grid_for(R->N);
     double* np_part = mm + 2 * ge;
@@ -2109,13 +2096,33 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
       set_error_msg("hipMemcpy of the spectral symbol failed");
       return CIA_ERR_HIP;
     }
-    int dims[3] = {R->nz, R->ny, R->nx};
-    if (hipfftPlanMany(&R->fft, 3, dims, dims, 1, (int)R->C, dims, 1, (int)R->C, HIPFFT_Z2Z, 2) !=
-        HIPFFT_SUCCESS) {
+    if (R->nx > 64 || R->ny > 64 || R->nz > 64) {
       cia_rve_destroy(R);
-      set_error_msg("hipfftPlanMany failed");
+      set_error_msg("the spectral preconditioner supports RVE edges up to 64 elements");
+      return CIA_ERR_INVALID_ARGUMENT;
+    }
+    std::vector<double2> tw;
+    const size_t ox = dft_twiddles(R->nx, tw), oy = dft_twiddles(R->ny, tw),
+                 oz = dft_twiddles(R->nz, tw);
+    ALLOC(R->tw, tw.size() * sizeof(double2));
+    if (hipMemcpy(R->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      cia_rve_destroy(R);
+      set_error_msg("hipMemcpy of the DFT twiddles failed");
       return CIA_ERR_HIP;
     }
+    auto axis = [&](int n, size_t off) {
+      DftAxis a{};
+      a.n = n;
+      dft_factor(n, a.n1, a.n2);
+      a.w = R->tw + off;
+      return a;
+    };
+    R->ax = axis(R->nx, ox);
+    R->ay = axis(R->ny, oy);
+    R->az = axis(R->nz, oz);
+    if (hipStreamCreateWithFlags(&R->cap_stream, hipStreamNonBlocking) != hipSuccess)
+      R->cap_stream = nullptr;  // graphs off: per-kernel launches
     R->fft_ready = true;
   }
 #undef ALLOC
@@ -2140,7 +2147,9 @@ cia_status cia_rve_destroy(cia_rve* R) {
     if (p) (void)hipFree(p);
   if (R->kinv) (void)hipFree(R->kinv);
   if (R->uab) (void)hipFree(R->uab);
-  if (R->fft) (void)hipfftDestroy(R->fft);
+  if (R->tw) (void)hipFree(R->tw);
+  for (auto& g : R->graphs) (void)hipGraphExecDestroy(g.exec);
+  if (R->cap_stream) (void)hipStreamDestroy(R->cap_stream);
   delete R;
   return CIA_OK;
 }
