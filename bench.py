@@ -231,6 +231,68 @@ def surrogate_section(args) -> dict:
     }
 
 
+def macro_explicit_section(args) -> dict:
+    """Explicit velocity-Verlet macro slab, plain J2 (the 128 x 128 x 64 Hex8 slab, h =
+    10 mm, rho 2400, E 30 GPa, nu 0.2, sigma_y 30 MPa, H 1 GPa, tau 1e-3 s, m 1; 50 MPa
+    half-sine over 0.2 ms on a centred 20 x 20-element top patch, bottom fixed, CFL 0.5).
+    Steps/s from the difference of two runs (warm-up steps + timed steps), so set-up and
+    the initial assembly cancel."""
+    import torch
+
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd.solvers import DirichletDofSet, TimeIntegrationSettings
+
+    nx, ny, nz = args.macro_divisions
+    h = 0.01
+    lengths = (nx * h, ny * h, nz * h)
+    j2 = M.J2ViscoplasticMaterial("slab", 2400.0, 30e9, 0.2, 30e6, 1e9, 1e-3, 30e6, 1.0)
+    bundle = D.build_structured_box_bundle(lengths, (nx, ny, nz), j2)
+    c0 = (nx - 20) // 2, (ny - 20) // 2
+    load = D.build_half_sine_pressure_load(bundle, 50e6, 2e-4, patch=((c0[0], c0[0] + 20),
+                                                                     (c0[1], c0[1] + 20)))
+    bottom = bundle.mesh_info.boundary_groups["bottom"].nodes
+    dofs = (3 * bottom[:, None] + np.arange(3)).ravel()
+    cp = float(j2.maximum_wave_speed)
+    dt = 0.5 * h / cp
+    n = bundle.mesh_info.dof_map.size
+    settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+
+    def run(steps):
+        ts = TimeIntegrationSettings("explicit", dt, steps, cfl_safety_factor=0.5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = D.solve_material_explicit(bundle, ts, np.zeros(n), np.zeros(n),
+                                        DirichletDofSet(dofs, np.zeros(dofs.size)), load,
+                                        "three_dimensional", settings, record_fields=False)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sol
+
+    w_steps, t_steps = 20, args.macro_steps
+    run(w_steps)
+    ta, _ = run(w_steps)
+    tb, sol = run(w_steps + t_steps)
+    per_step = (tb - ta) / t_steps
+    n_el = nx * ny * nz
+    n_gp = 8 * n_el
+    alg = n_gp * 128 + n * 64  # state 8 values read + written per point; u, v, a, m, f_ext per dof
+    return {
+        "workload": f"explicit velocity Verlet, {nx}x{ny}x{nz} Hex8 slab, plain J2, FP64, "
+                    f"dt {dt:.4e} s (CFL 0.5)",
+        "gauss_points": n_gp,
+        "dofs": n,
+        "steps_timed": t_steps,
+        "macro_steps_per_s": 1.0 / per_step,
+        "ms_per_step": per_step * 1e3,
+        "gp_updates_per_s": n_gp / per_step,
+        "roofline": {"bound": "hbm", "achieved": alg / per_step / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg / per_step / 1e9 / HBM_PEAK_GBS,
+                     "alg_bytes_per_step": alg},
+        "final_dissipated_energy": float(sol.dissipated_energy[-1]),
+        "final_active_fraction": float(sol.viscoplastic_active_volume_fraction[-1]),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +308,10 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (HBM-counter calibration kernel)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate section")
+    ap.add_argument("--no-macro", action="store_true", help="skip the macro explicit slab section")
+    ap.add_argument("--macro-only", action="store_true")
+    ap.add_argument("--macro-divisions", type=int, nargs=3, default=(128, 128, 64))
+    ap.add_argument("--macro-steps", type=int, default=200)
     ap.add_argument("--surrogate-only", action="store_true")
     ap.add_argument("--surrogate-rve", type=int, default=12)
     ap.add_argument("--surrogate-steps", type=int, default=48)
@@ -271,6 +337,9 @@ def main() -> int:
         copy_calibration(torch)
     if args.surrogate_only:
         print(json.dumps({"surrogate": surrogate_section(args)}))
+        return 0
+    if args.macro_only:
+        print(json.dumps({"macro_explicit": macro_explicit_section(args)}))
         return 0
     n = args.size
     model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol, not args.fixed_forcing)
@@ -404,6 +473,9 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n, args.seed, newton / K, backtracks / K)
+    if not args.no_macro:
+        print("macro explicit slab ...", file=sys.stderr, flush=True)
+        line["macro_explicit"] = macro_explicit_section(args)
     if not args.no_surrogate:
         line["surrogate"] = surrogate_section(args)
     print(json.dumps(line))

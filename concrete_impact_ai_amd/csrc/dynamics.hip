@@ -67,6 +67,16 @@ __device__ __forceinline__ void box_corners(int64_t e64, int nx, int ny, int64_t
   for (int a = 0; a < 8; ++a) nd[a] = base + ((a & 1) ? sx : 0) + ((a & 2) ? sy : 0) + ((a & 4) ? sz : 0);
 }
 
+// Node of corner a (a = ax + 2 ay + 4 az) of box element e.
+__device__ __forceinline__ int64_t box_corner(int64_t e64, int a, int nx, int ny) {
+  const unsigned e = (unsigned)e64;
+  const unsigned r = e / (unsigned)nx;
+  const int i = (int)(e - r * (unsigned)nx);
+  const int k = (int)(r / (unsigned)ny);
+  const int j = (int)(r - (unsigned)k * (unsigned)ny);
+  return (i + (a & 1)) + (int64_t)(nx + 1) * ((j + ((a >> 1) & 1)) + (int64_t)(ny + 1) * (k + ((a >> 2) & 1)));
+}
+
 template <int BLOCK>
 __device__ __forceinline__ void block_partials(const double* v, int nsum, const double* mins,
                                                const double* maxs, double* part, int P,
@@ -144,26 +154,42 @@ __global__ void __launch_bounds__(kBlock)
   const int q = (int)(p & 7);
   double vals[6] = {0, 0, 0, 0, 0, 0};
   double mn = INFINITY, mx = -INFINITY;
+  // Lane q forms the predictor at corner q only; the element's 8 lanes exchange
+  // their corners through LDS.
+  __shared__ double cu[kBlock][6];
+  {
+    double own[6] = {0, 0, 0, 0, 0, 0};
+    if (e < G.E) {
+      const int64_t nq = box_corner(e, q, G.nx, G.ny);
+      const double h2 = 0.5 * dt * dt;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const int64_t d = 3 * nq + m;
+        const double u0 = u[d];
+        double u1 = u0;
+        if (PREDICT) u1 = fixed[d] ? 0.0 : u0 + dt * v[d] + h2 * a[d];
+        own[m] = u1;
+        own[3 + m] = u1 - u0;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) cu[threadIdx.x][m] = own[m];
+  }
+  __syncthreads();
   if (e < G.E) {
     const int64_t np = G.E * 8;
-    int64_t nd[8];
-    box_corners(e, G.nx, G.ny, nd);
     T bx[8], by[8], bz[8];
     gp_factors<T>(q, G.g, bx, by, bz);
     T eps[6], rate[6];
     {
       double un[8][3], du[8][3];
-      const double h2 = 0.5 * dt * dt;
+      const int base = threadIdx.x & ~7;
 #pragma unroll
       for (int c = 0; c < 8; ++c)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          const int64_t d = 3 * nd[c] + m;
-          const double u0 = u[d];
-          double u1 = u0;
-          if (PREDICT) u1 = fixed[d] ? 0.0 : u0 + dt * v[d] + h2 * a[d];
-          un[c][m] = u1;
-          du[c][m] = u1 - u0;
+          un[c][m] = cu[base + c][m];
+          du[c][m] = cu[base + c][3 + m];
         }
       strain_from(un, bx, by, bz, eps);
       strain_from(du, bx, by, bz, rate);

@@ -195,6 +195,19 @@ __device__ __forceinline__ void periodic_corners(int64_t e64, int nx, int ny, in
 }
 
 // Element that holds periodic node (i, j, k) as its corner a.
+// Periodic class of corner a (a = ax + 2 ay + 4 az) of element e.
+__device__ __forceinline__ int64_t periodic_corner(int64_t e64, int a, int nx, int ny, int nz) {
+  const unsigned e = (unsigned)e64;
+  const unsigned r = e / (unsigned)nx;
+  int i = (int)(e - r * (unsigned)nx);
+  int k = (int)(r / (unsigned)ny);
+  int j = (int)(r - (unsigned)k * (unsigned)ny);
+  if (a & 1) i = i + 1 == nx ? 0 : i + 1;
+  if (a & 2) j = j + 1 == ny ? 0 : j + 1;
+  if (a & 4) k = k + 1 == nz ? 0 : k + 1;
+  return i + (int64_t)nx * (j + (int64_t)ny * k);
+}
+
 __device__ __forceinline__ int64_t periodic_corner_element(int i, int j, int k, int a, int nx,
                                                            int ny, int nz) {
   int ii = i - (a & 1), jj = j - ((a >> 1) & 1), kk = k - ((a >> 2) & 1);

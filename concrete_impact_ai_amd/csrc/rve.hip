@@ -857,29 +857,42 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t e = p >> 3;
   const int q = (int)(p & 7);
-  if (e >= G.E) return;
+  const bool valid = e < G.E;
   const int64_t np = G.E * 8;
   const int64_t C = G.E;
-  const double z0[3] = {zc[0].x, zc[0].y, zc[C].x};
-  int64_t c[8];
-  int i, j, k;
-  periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+  // Lane q of an element forms p = (z - z0) + beta p_old at corner q only; the 8
+  // lanes then read each other's corners from LDS (one load per corner instead of
+  // eight redundant ones).
+  __shared__ double cpv[kBlock][3];
+  {
+    double own[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+      const double z0[3] = {zc[0].x, zc[0].y, zc[C].x};
+      const int64_t cq = periodic_corner(e, q, G.nx, G.ny, G.nz);
+      const hipDoubleComplex t01 = zc[cq], t2 = zc[C + cq];
+      own[0] = (t01.x - z0[0]) + beta * p_old[3 * cq + 0];
+      own[1] = (t01.y - z0[1]) + beta * p_old[3 * cq + 1];
+      own[2] = (t2.x - z0[2]) + beta * p_old[3 * cq + 2];
+      if (q == 0) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) p_new[3 * e + m] = own[m];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) cpv[threadIdx.x][m] = own[m];
+  }
+  __syncthreads();
+  if (!valid) return;  // after the block's only barrier; whole 8-lane groups leave
   T eps[6];
   T bx[8], by[8], bz[8];
   gp_factors<T>(q, G.g, bx, by, bz);
   {
     double pe[8][3];
+    const int base = threadIdx.x & ~7;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const hipDoubleComplex t01 = zc[c[a]], t2 = zc[C + c[a]];
-      pe[a][0] = (t01.x - z0[0]) + beta * p_old[3 * c[a] + 0];
-      pe[a][1] = (t01.y - z0[1]) + beta * p_old[3 * c[a] + 1];
-      pe[a][2] = (t2.x - z0[2]) + beta * p_old[3 * c[a] + 2];
-    }
-    if (q == 0) {
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-      for (int m = 0; m < 3; ++m) p_new[3 * e + m] = pe[0][m];
-    }
+      for (int m = 0; m < 3; ++m) pe[a][m] = cpv[base + a][m];
     strain_from(pe, bx, by, bz, eps);
   }
   Tangent<T> tc;
@@ -1294,6 +1307,8 @@ struct cia_rve {
   double* u_full = nullptr;
   double* f_full = nullptr;
   double* partials = nullptr;
+  double* norm_part = nullptr;  // [2][1024] partials of the vector norms (kept apart from the
+                                // PCG partials, which stay live across PCG batches)
   unsigned* counter = nullptr;
   cia::Scalars* sc = nullptr;
   double* class0 = nullptr;
@@ -1339,6 +1354,8 @@ static cia_status dmalloc(void** p, size_t bytes) {
   return CIA_OK;
 }
 
+
+static cia_status norm2_async(cia_rve* R, hipStream_t st, int64_t n, const double* x, int slot);
 
 // Capture two spectral PCG iterations (even: p_old = pp[1], odd: p_old = pp[0])
 // on the library's private stream and cache the executable graph per (tangent
@@ -1402,7 +1419,8 @@ struct Ops {
 
   // Solve K x = b with the spectral reference-medium preconditioner.
   static cia_status pcg_spectral(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
-                                 int* iters_out, int* flag_out, double rtol, int hint) {
+                                 int* iters_out, int* flag_out, double rtol, int hint,
+                                 double* x_norm2) {
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
     const int P = (int)gn;
@@ -1503,6 +1521,7 @@ struct Ops {
         for (int s = 0; s < n; ++s, ++it) iteration(st, it & 1, it);
       }
       CIA_HIP(hipGetLastError());
+      CIA_TRY(norm2_async(R, st, 3 * C, x, 3));  // ||x||^2 read with the convergence flag
       CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
       CIA_HIP(hipStreamSynchronize(st));
       done = hs.done;
@@ -1515,6 +1534,7 @@ struct Ops {
     }
     *iters_out = hs.iters;
     *flag_out = hs.done;
+    if (x_norm2) *x_norm2 = hs.norm2[3];
     R->pcg_total += hs.iters;
     if (hint >= 0 && hint < cia_rve::kHints && hs.done == 1) R->its_hint[hint] = hs.iters;
     return CIA_OK;
@@ -1523,10 +1543,11 @@ struct Ops {
   // Solve K x = b (K from tan[slot]); returns iterations.  `hint` indexes the
   // per-Newton-iteration history of PCG counts (-1: none).
   static cia_status pcg(cia_rve* R, hipStream_t st, int slot, const double* b, double* x,
-                        int* iters_out, int* flag_out, double rtol = -1.0, int hint = -1) {
+                        int* iters_out, int* flag_out, double rtol = -1.0, int hint = -1,
+                        double* x_norm2 = nullptr) {
     if (rtol <= 0.0) rtol = R->lin.relative_tolerance;
     if (R->prec == CIA_PCG_SPECTRAL && R->fft_ready)
-      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out, rtol, hint);
+      return pcg_spectral(R, st, slot, b, x, iters_out, flag_out, rtol, hint, x_norm2);
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
     prof::begin(prof::PRECOND, st);
@@ -1577,6 +1598,13 @@ struct Ops {
     *iters_out = hs.iters;
     *flag_out = hs.done;
     R->pcg_total += hs.iters;
+    if (x_norm2) {
+      CIA_TRY(norm2_async(R, st, 3 * C, x, 3));
+      Scalars h2;
+      CIA_HIP(hipMemcpyAsync(&h2, R->sc, sizeof(h2), hipMemcpyDeviceToHost, st));
+      CIA_HIP(hipStreamSynchronize(st));
+      *x_norm2 = h2.norm2[3];
+    }
     return CIA_OK;
   }
 };
@@ -1590,11 +1618,34 @@ static cia_status read_norms(cia_rve* R, hipStream_t st, double out[4]) {
   return CIA_OK;
 }
 
+// ||x||^2 -> sc->norm2[slot] without a host synchronisation.
+static cia_status norm2_async(cia_rve* R, hipStream_t st, int64_t n, const double* x, int slot) {
+  const unsigned g = (unsigned)std::min<int64_t>(grid_for(n), 1024);
+  hipLaunchKernelGGL(k_norms, dim3(g), dim3(kBlock), 0, st, n, x, (const double*)nullptr,
+                     R->norm_part);
+  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kBlock), 0, st, (const double*)R->norm_part,
+                     (int)g, &R->sc->norm2[slot]);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+// One synchronisation: the scalar block (norms) and the first failing point.
+static cia_status read_norms_fail(cia_rve* R, hipStream_t st, double out[4],
+                                  unsigned long long* ff) {
+  Scalars hs;
+  CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipMemcpyAsync(ff, R->first_fail, sizeof(*ff), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  prof::flush();
+  for (int i = 0; i < 4; ++i) out[i] = hs.norm2[i];
+  return CIA_OK;
+}
+
 static cia_status norm2_of(cia_rve* R, hipStream_t st, int64_t n, const double* x, double* out) {
   const unsigned g = (unsigned)std::min<int64_t>(grid_for(n), 1024);
   hipLaunchKernelGGL(k_norms, dim3(g), dim3(kBlock), 0, st, n, x, (const double*)nullptr,
-                     R->partials);
-  hipLaunchKernelGGL(k_finalize<2>, dim3(1), dim3(kBlock), 0, st, (const double*)R->partials,
+                     R->norm_part);
+  hipLaunchKernelGGL(k_finalize<1>, dim3(1), dim3(kBlock), 0, st, (const double*)R->norm_part,
                      (int)g, &R->sc->norm2[2]);
   CIA_HIP(hipGetLastError());
   double v[4];
@@ -1711,15 +1762,16 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
     return CIA_ERR_POINT_FAILURE;
   };
 
-  // assembly at the committed fluctuation (iteration 0)
+  // assembly at the committed fluctuation (iteration 0); ||w||^2 -> norm2[2]
   CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, R->st_trial[cur], S, dt));
   res->gp_updates += R->np;
   CIA_TRY(Ops<T>::gather(R, st, R->r, nullptr, 0));
-  CIA_TRY(check_fail(R, st, &ff));
-  if (ff != ~0ull) return point_failure(R->w);
+  CIA_TRY(norm2_async(R, st, C3, R->w, 2));
   double nv[4];
-  CIA_TRY(read_norms(R, st, nv));
+  CIA_TRY(read_norms_fail(R, st, nv, &ff));
+  if (ff != ~0ull) return point_failure(R->w);
   double rn = sqrt(nv[0]);
+  double wn2 = nv[2];
   double r0 = -1.0;
   double last_inc = INFINITY;
   int backtracks = 0;
@@ -1728,8 +1780,6 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
   for (it = 0; it <= ns.max_iterations; ++it) {
     if (r0 < 0.0) r0 = rn;
     rtol = std::max(ns.residual_absolute_tolerance, ns.residual_relative_tolerance * r0);
-    double wn2 = 0.0;
-    CIA_TRY(norm2_of(R, st, C3, R->w, &wn2));
     const double itol = std::max(ns.increment_absolute_tolerance,
                                  ns.increment_relative_tolerance * std::max(sqrt(wn2), 1.0));
     if (rn <= rtol && (it == 0 || last_inc <= itol)) break;
@@ -1754,9 +1804,8 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
       lin_rtol = std::max(lin_rtol, std::max(eta, 0.5 * rtol / std::max(rn, 1e-300)));
       lin_rtol = std::min(lin_rtol, 0.5);
     }
-    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag, lin_rtol, it));
     double dn2 = 0.0;
-    CIA_TRY(norm2_of(R, st, C3, R->dw, &dn2));
+    CIA_TRY(Ops<T>::pcg(R, st, cur, R->r_cand, R->dw, &pit, &pflag, lin_rtol, it, &dn2));
     const double inc_n = sqrt(dn2);
     if (!std::isfinite(inc_n)) {
       res->reason = CIA_RVE_NON_FINITE_INCREMENT;
@@ -1776,19 +1825,20 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
         alpha = std::pow(ns.armijo_reduction_factor, (double)bt);
         hipLaunchKernelGGL(k_axpy, dim3(grid_for(C3)), dim3(kBlock), 0, st, C3, R->w, alpha, R->dw,
                            R->w_cand);
+        CIA_TRY(norm2_async(R, st, C3, R->w_cand, 2));
         CIA_TRY(Ops<T>::assemble(R, st, false, cand, R->w_cand, R->wc_buf, M, state_c,
                                  R->st_trial[cand], S, dt));
         res->gp_updates += R->np;
         CIA_TRY(Ops<T>::gather(R, st, R->r_cand, nullptr, 1));
-        CIA_TRY(check_fail(R, st, &ff));
+        CIA_TRY(read_norms_fail(R, st, nv, &ff));
         if (ff != ~0ull) return point_failure(R->w_cand);
-        CIA_TRY(read_norms(R, st, nv));
         const double cn = sqrt(nv[1]);
         if (0.5 * cn * cn <= 0.5 * rn * rn - ns.armijo_sufficient_decrease * alpha * rn * rn) {
           std::swap(R->w, R->w_cand);
           std::swap(R->r, R->r_cand);
           cur = cand;
           rn = cn;
+          wn2 = nv[2];
           last_inc = alpha * inc_n;
           backtracks += bt;
           accepted = true;
@@ -1808,14 +1858,15 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
       hipLaunchKernelGGL(k_axpy, dim3(grid_for(C3)), dim3(kBlock), 0, st, C3, R->w, 1.0, R->dw,
                          R->w_cand);
       std::swap(R->w, R->w_cand);
+      CIA_TRY(norm2_async(R, st, C3, R->w, 2));
       CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, R->st_trial[cur], S,
                                dt));
       res->gp_updates += R->np;
       CIA_TRY(Ops<T>::gather(R, st, R->r, nullptr, 0));
-      CIA_TRY(check_fail(R, st, &ff));
+      CIA_TRY(read_norms_fail(R, st, nv, &ff));
       if (ff != ~0ull) return point_failure(R->w);
-      CIA_TRY(read_norms(R, st, nv));
       rn = sqrt(nv[0]);
+      wn2 = nv[2];
       last_inc = inc_n;
     }
   }
@@ -2067,6 +2118,7 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
   ALLOC(R->f_full, 3 * R->N * 8);
   const int64_t maxb = std::max<int64_t>(grid_for(R->E), grid_for(R->N)) + 64;
   ALLOC(R->partials, 8 * maxb * 8);
+  ALLOC(R->norm_part, 2 * 1024 * sizeof(double));
   ALLOC(R->counter, 64);
   ALLOC(R->sc, sizeof(Scalars));
   ALLOC(R->class0, 3 * 8);
@@ -2148,6 +2200,7 @@ cia_status cia_rve_destroy(cia_rve* R) {
   if (R->kinv) (void)hipFree(R->kinv);
   if (R->uab) (void)hipFree(R->uab);
   if (R->tw) (void)hipFree(R->tw);
+  if (R->norm_part) (void)hipFree(R->norm_part);
   for (auto& g : R->graphs) (void)hipGraphExecDestroy(g.exec);
   if (R->cap_stream) (void)hipStreamDestroy(R->cap_stream);
   delete R;
