@@ -178,24 +178,10 @@ __global__ void __launch_bounds__(kBlock)
   __syncthreads();
   if (e < G.E) {
     const int64_t np = G.E * 8;
-    T bx[8], by[8], bz[8];
-    gp_factors<T>(q, G.g, bx, by, bz);
     T eps[6], rate[6];
-    {
-      double un[8][3], du[8][3];
-      const int base = threadIdx.x & ~7;
+    strain_pair_from_rows<T, 6>(&cu[threadIdx.x & ~7], 3, q, G.g, eps, rate);
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          un[c][m] = cu[base + c][m];
-          du[c][m] = cu[base + c][3 + m];
-        }
-      strain_from(un, bx, by, bz, eps);
-      strain_from(du, bx, by, bz, rate);
-#pragma unroll
-      for (int m = 0; m < 6; ++m) rate[m] /= (T)dt;
-    }
+    for (int m = 0; m < 6; ++m) rate[m] /= (T)dt;
     const PhaseConst P = tab->p[ephase[e]];
     const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
     T epsp[6];
@@ -240,7 +226,7 @@ __global__ void __launch_bounds__(kBlock)
       mx = mg;
     }
     T f[8][3];
-    force_from(o.sig, (T)G.g.jw, bx, by, bz, f);
+    force_at_gp(o.sig, (T)G.g.jw, q, G.g, f);
     T fin[3];
     element_reduce8(q, f, fin);
 #pragma unroll
@@ -259,7 +245,7 @@ __global__ void __launch_bounds__(kBlock)
                const double* __restrict__ part_e, int Pe, double* __restrict__ hist_row,
                double jw) {
   __shared__ double lds[kBlock / 64];
-  if (blockIdx.x == 0) fold_elem(part_e, Pe, jw, hist_row, lds);
+  if (hist_row && blockIdx.x == 0) fold_elem(part_e, Pe, jw, hist_row, lds);
   const int64_t n = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   double vals[2] = {0.0, 0.0};
   if (n < G.N) {
@@ -304,9 +290,51 @@ __global__ void __launch_bounds__(kBlock)
   block_partials<kBlock>(vals, 2, nullptr, nullptr, part_n, gridDim.x, lds);
 }
 
-__global__ void k_fold_node_only(const double* __restrict__ part_n, int P, double* hist_row) {
+// History folds with one block per partial row (the element pass has one partial per
+// block of 256 Gauss points, tens of thousands on a large slab; folding them in one
+// block of the next kernel would serialise that kernel behind it).  Same summation
+// order as fold_elem / fold_node.
+constexpr int kFoldBlock = 1024;
+
+__global__ void __launch_bounds__(kFoldBlock)
+    k_fold_elem_rows(const double* __restrict__ pe, int P, double jw, double* __restrict__ hist_row) {
+  __shared__ double lds[kFoldBlock / 64];
+  const int s = blockIdx.x;
+  const double init = (s == 6) ? INFINITY : (s == 7 ? -INFINITY : 0.0);
+  // four independent accumulators per thread keep several loads in flight
+  double acc[4] = {init, init, init, init};
+  const double* row = pe + (int64_t)s * P;
+  int b = threadIdx.x;
+  for (; b + 3 * kFoldBlock < P; b += 4 * kFoldBlock) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double v = row[b + u * kFoldBlock];
+      acc[u] = (s == 6) ? fmin(acc[u], v) : (s == 7 ? fmax(acc[u], v) : acc[u] + v);
+    }
+  }
+  for (; b < P; b += kFoldBlock) {
+    const double v = row[b];
+    acc[0] = (s == 6) ? fmin(acc[0], v) : (s == 7 ? fmax(acc[0], v) : acc[0] + v);
+  }
+  double a;
+  if (s == 6) a = block_min<kFoldBlock>(fmin(fmin(acc[0], acc[1]), fmin(acc[2], acc[3])), lds);
+  else if (s == 7) a = block_max<kFoldBlock>(fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3])), lds);
+  else a = block_sum<kFoldBlock>((acc[0] + acc[1]) + (acc[2] + acc[3]), lds);
+  if (threadIdx.x == 0) {
+    const int col[PE_N] = {H_FREE, H_DISS_RATE, H_POS_WORK, H_NEG_WORK, H_UNLOAD, H_ACTIVE,
+                           H_MARGIN_MIN, H_MARGIN_MAX};
+    hist_row[col[s]] = s < 6 ? a * jw : a;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_fold_node_rows(const double* __restrict__ pn, int P, double* __restrict__ hist_row) {
   __shared__ double lds[kBlock / 64];
-  fold_node(part_n, P, hist_row, lds);
+  const int s = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < P; b += blockDim.x) a += pn[s * P + b];
+  a = block_sum<kBlock>(a, lds);
+  if (threadIdx.x == 0) hist_row[s == 0 ? H_KINETIC : H_EXT_DU] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -961,25 +989,34 @@ cia_status cia_box_explicit_steps(cia_box* B, void* stream, int32_t initialize, 
   };
   // history row of step s is base + s; row 0 is the initial state when initialising
   const int64_t base = initialize ? 1 : 0;
+  auto fold = [&](double* row) {
+    hipLaunchKernelGGL(k_fold_elem_rows, dim3(PE_N), dim3(kFoldBlock), 0, st,
+                       (const double*)B->part_e, (int)ge, jw, row);
+  };
+  auto fold_n = [&](double* row) {
+    hipLaunchKernelGGL(k_fold_node_rows, dim3(PN_N), dim3(kBlock), 0, st, (const double*)B->part_n,
+                       (int)gn, row);
+  };
   if (initialize) {
     elem(false, nullptr, n_steps == 0);
+    fold(B->hist);
     hipLaunchKernelGGL((k_dyn_node<true>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
                        mass, fixed, f_unit, load_scale[0], u, v, a,
                        n_steps == 0 ? internal_force : nullptr, B->part_n, B->part_e, (int)ge,
-                       B->hist, jw);
+                       (double*)nullptr, jw);
+    fold_n(B->hist);
   }
   for (int32_t s = 0; s < n_steps; ++s) {
     const bool last = s + 1 == n_steps;
-    double* prev_row = (initialize || s > 0) ? B->hist + (base + s - 1) * H_N : nullptr;
-    elem(true, prev_row, last);
+    double* row = B->hist + (base + s) * H_N;
+    elem(true, nullptr, last);
+    fold(row);
     hipLaunchKernelGGL((k_dyn_node<false>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
                        mass, fixed, f_unit, load_scale[s + 1], u, v, a,
                        last ? internal_force : nullptr, B->part_n, B->part_e, (int)ge,
-                       B->hist + (base + s) * H_N, jw);
+                       (double*)nullptr, jw);
+    fold_n(row);
   }
-  if (initialize || n_steps > 0)
-    hipLaunchKernelGGL(k_fold_node_only, dim3(1), dim3(kBlock), 0, st, B->part_n, (int)gn,
-                       B->hist + (base + n_steps - 1) * H_N);
   CIA_HIP(hipGetLastError());
   const int64_t nrows = base + n_steps;
   if (nrows > 0)
@@ -1030,16 +1067,18 @@ cia_status cia_box_generic_correct(cia_box* B, void* stream, int32_t initialize,
   const unsigned ge = grid_for(8 * B->G.E), gn = grid_for(B->G.N);
   hipLaunchKernelGGL(k_box_force, dim3(ge), dim3(kBlock), 0, st, B->G, stress, psi, diss, work,
                      active, margin, B->F, B->part_e);
+  hipLaunchKernelGGL(k_fold_elem_rows, dim3(PE_N), dim3(kFoldBlock), 0, st,
+                     (const double*)B->part_e, (int)ge, B->G.g.jw, history_row);
   if (initialize)
     hipLaunchKernelGGL((k_dyn_node<true>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
                        mass, fixed, f_unit, load_scale, u, v, a, internal_force, B->part_n,
-                       B->part_e, (int)ge, history_row, B->G.g.jw);
+                       B->part_e, (int)ge, (double*)nullptr, B->G.g.jw);
   else
     hipLaunchKernelGGL((k_dyn_node<false>), dim3(gn), dim3(kBlock), 0, st, B->G, time_step, B->F,
                        mass, fixed, f_unit, load_scale, u, v, a, internal_force, B->part_n,
-                       B->part_e, (int)ge, history_row, B->G.g.jw);
-  hipLaunchKernelGGL(k_fold_node_only, dim3(1), dim3(kBlock), 0, st, B->part_n, (int)gn,
-                     history_row);
+                       B->part_e, (int)ge, (double*)nullptr, B->G.g.jw);
+  hipLaunchKernelGGL(k_fold_node_rows, dim3(PN_N), dim3(kBlock), 0, st, (const double*)B->part_n,
+                     (int)gn, history_row);
   CIA_HIP(hipGetLastError());
   return CIA_OK;
 }

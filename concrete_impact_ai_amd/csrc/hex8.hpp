@@ -103,6 +103,86 @@ __device__ __forceinline__ void gp_factors(int q, const BoxGeom& g, T bx[8], T b
   }
 }
 
+// Gradient factors of corner a at the Gauss point with per-direction weights
+// w?[0|1] (scaled by 1/h via ih?).
+template <class T>
+__device__ __forceinline__ void corner_factors(int a, const T wx[2], const T wy[2], const T wz[2],
+                                               T ix, T iy, T iz, T& bx, T& by, T& bz) {
+  const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+  bx = (ax ? ix : -ix) * wy[ay] * wz[az];
+  by = (ay ? iy : -iy) * wx[ax] * wz[az];
+  bz = (az ? iz : -iz) * wx[ax] * wy[ay];
+}
+
+template <class T>
+__device__ __forceinline__ void gp_weights(int q, T wx[2], T wy[2], T wz[2]) {
+  const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+  wx[0] = (T)(qx == 0 ? kWp : kWm);
+  wx[1] = (T)(qx == 1 ? kWp : kWm);
+  wy[0] = (T)(qy == 0 ? kWp : kWm);
+  wy[1] = (T)(qy == 1 ? kWp : kWm);
+  wz[0] = (T)(qz == 0 ? kWp : kWm);
+  wz[1] = (T)(qz == 1 ? kWp : kWm);
+}
+
+// Engineering strains of two nodal fields at once, read corner by corner from
+// rows[a][0..2] (field 1) and rows[a][off..off+2] (field 2), so the 48 corner values
+// are never held in registers together.
+template <class T, int STRIDE>
+__device__ __forceinline__ void strain_pair_from_rows(const double (*rows)[STRIDE], int off, int q,
+                                                      const BoxGeom& geo, T e1[6], T e2[6]) {
+  T wx[2], wy[2], wz[2];
+  gp_weights<T>(q, wx, wy, wz);
+  const T ix = (T)geo.ihx, iy = (T)geo.ihy, iz = (T)geo.ihz;
+  T g1[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, g2[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    T bx, by, bz;
+    corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v1 = (T)rows[a][c], v2 = (T)rows[a][off + c];
+      g1[c][0] += bx * v1;
+      g1[c][1] += by * v1;
+      g1[c][2] += bz * v1;
+      g2[c][0] += bx * v2;
+      g2[c][1] += by * v2;
+      g2[c][2] += bz * v2;
+    }
+  }
+  e1[0] = g1[0][0];
+  e1[1] = g1[1][1];
+  e1[2] = g1[2][2];
+  e1[3] = g1[1][2] + g1[2][1];
+  e1[4] = g1[0][2] + g1[2][0];
+  e1[5] = g1[0][1] + g1[1][0];
+  e2[0] = g2[0][0];
+  e2[1] = g2[1][1];
+  e2[2] = g2[2][2];
+  e2[3] = g2[1][2] + g2[2][1];
+  e2[4] = g2[0][2] + g2[2][0];
+  e2[5] = g2[0][1] + g2[1][0];
+}
+
+// f[a][c] = w detJ (B^T sigma)[a][c] at Gauss point q, factors formed per corner.
+template <class T>
+__device__ __forceinline__ void force_at_gp(const T s[6], T jw, int q, const BoxGeom& geo,
+                                            T f[8][3]) {
+  T wx[2], wy[2], wz[2];
+  gp_weights<T>(q, wx, wy, wz);
+  const T ix = (T)geo.ihx, iy = (T)geo.ihy, iz = (T)geo.ihz;
+  const T s0 = jw * s[0], s1 = jw * s[1], s2 = jw * s[2], s3 = jw * s[3], s4 = jw * s[4],
+          s5 = jw * s[5];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    T bx, by, bz;
+    corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
+    f[a][0] = bx * s0 + by * s5 + bz * s4;
+    f[a][1] = by * s1 + bz * s3 + bx * s5;
+    f[a][2] = bz * s2 + by * s3 + bx * s4;
+  }
+}
+
 // Engineering strain at the point whose factors are given.
 template <class T, class U>
 __device__ __forceinline__ void strain_from(const U u[8][3], const T bx[8], const T by[8],

@@ -156,26 +156,37 @@ __global__ void __launch_bounds__(kBlock)
   const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   const int64_t e = p >> 3;
   const int q = (int)(p & 7);
-  if (e >= G.E) return;  // whole 8-lane groups leave together
   const int64_t np = G.E * 8;
-  int64_t c[8];
-  int i, j, k;
-  periodic_corners(e, G.nx, G.ny, G.nz, c, i, j, k);
+  // Lane q loads the fluctuation at corner q only; the element's 8 lanes share
+  // their corners through LDS.
+  __shared__ double cw[kBlock][EXTRA ? 6 : 3];
+  if (e < G.E) {
+    const int64_t cq = periodic_corner(e, q, G.nx, G.ny, G.nz);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double wv = w[3 * cq + m];
+      cw[threadIdx.x][m] = wv;
+      if (EXTRA) cw[threadIdx.x][3 + m] = wv - wc[3 * cq + m];
+    }
+  }
+  __syncthreads();
+  if (e >= G.E) return;  // after the block's only barrier; whole 8-lane groups leave
   T bx[8], by[8], bz[8];
   gp_factors<T>(q, G.g, bx, by, bz);
   T eps[6], rate[6];
   {
     double we[8][3];
+    const int base = threadIdx.x & ~7;
 #pragma unroll
     for (int a = 0; a < 8; ++a)
 #pragma unroll
-      for (int m = 0; m < 3; ++m) we[a][m] = w[3 * c[a] + m];
+      for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][m];
     strain_from(we, bx, by, bz, eps);
     if (EXTRA) {
 #pragma unroll
       for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int m = 0; m < 3; ++m) we[a][m] -= wc[3 * c[a] + m];
+        for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][3 + m];
       strain_from(we, bx, by, bz, rate);
     }
   }

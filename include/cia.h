@@ -219,7 +219,8 @@ typedef struct {
 
 /* PCG preconditioners.  CIA_PCG_SPECTRAL is the exact inverse of the Hex8
  * stiffness of a homogeneous reference medium on the periodic grid, applied
- * with 3-D FFTs (hipFFT); CIA_PCG_BLOCK_JACOBI inverts the 3x3 nodal blocks. */
+ * with the library's own 3-D DFT kernels; CIA_PCG_BLOCK_JACOBI inverts the 3x3 nodal
+ * blocks. */
 enum { CIA_PCG_SPECTRAL = 0, CIA_PCG_BLOCK_JACOBI = 1 };
 
 enum { CIA_PREC_FP64 = 0, CIA_PREC_FP32_REFINED = 1 };
