@@ -267,6 +267,17 @@ _SIGNATURES = {
          C.POINTER(UpdateSettings), C.c_int32, _vp, _vp, C.POINTER(RVEResult)],
     ),
     "cia_rve_copy_field": (C.c_int32, [_vp, _vp, C.c_int32, _vp]),
+    "cia_rve_batch_create": (C.c_int32, [C.POINTER(RVEDesc), C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "cia_rve_batch_destroy": (C.c_int32, [_vp]),
+    "cia_rve_batch_layout": (C.c_int32, [_vp, C.POINTER(C.c_int64)]),
+    "cia_rve_batch_solve": (
+        C.c_int32,
+        [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+         C.POINTER(UpdateSettings), _vp, _vp, C.POINTER(RVEResult)],
+    ),
+    "cia_rve_batch_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                         C.c_int32]),
+    "cia_rve_batch_set_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.c_int32]),
     "cia_profile_enable": (C.c_int32, [C.c_int32]),
     "cia_profile_read": (C.c_int32, [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
 }
@@ -329,6 +340,9 @@ _SIGNATURES.update({
     "cia_f32_to_bf16": (C.c_int32, [_vp, C.c_int64, _vp, _vp]),
     "cia_mlp_forward_fused": (C.c_int32, [_vp, C.c_int64, _vp, _vp, _vp, _vp]),
 })
+
+BATCH_KERNELS = ["assemble", "gather", "pcg_elem", "pcg_node", "pcg_update", "symbol_solve",
+                 "dft_x", "dft_y", "dft_z"]
 
 PROFILE_SLOTS = ["assemble", "pcg_elem", "pcg_node", "pcg_update", "precond", "gather"]
 

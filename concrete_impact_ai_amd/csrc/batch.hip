@@ -1,0 +1,1697 @@
+// Batched periodic-RVE microequilibrium: many independent structured RVEs of
+// different sizes solved together on one MI355X.
+//
+// Reference: the per-point RVE solves behind RVEHomogenizedMaterial.update
+// (src/fem/rve/material.py:104-191, worker loop :267-291) and the path driver
+// iterate_prescribed_rve_path (src/fem/rve/path.py:47-100), each RVE increment
+// being solve_rve_microequilibrium (src/fem/rve/solver.py:62-227) with the
+// Armijo search of src/fem/solvers/line_search.py:40-67.  The reference runs
+// the RVEs one after another (or in a process pool); here they share every
+// kernel launch.
+//
+// Layout.  The RVEs are concatenated in one arena: RVE r owns elements (=
+// periodic classes) [e0, e0 + E) and points [8 e0, 8 (e0 + E)).  Per-point rows
+// are component-major over the whole arena ([c][NPT]), class vectors are
+// class-major ([3 CT]), the packed spectra of RVE r occupy [2 e0, 2 e0 + 2E).
+//
+// Tiles.  Every kernel works on tiles of one family (Gauss-point lanes, classes
+// or modes, DFT x-lines, DFT y/z line groups); a tile never straddles two RVEs,
+// so a tile's block reduction is a partial of exactly one RVE, stored at the
+// tile's index.  A kernel's blocks walk the family's tiles in rounds of
+// gridDim.x tiles; within a round each XCD takes one contiguous range (the
+// neighbour rows of a stencil stay in that XCD's L2).  The grid of each kernel
+// is chosen by the autotuner (one tile per block up to persistent grids of a
+// few blocks per CU).
+//
+// Control.  Each RVE carries a Newton state machine on the device (NEWTON ->
+// PCG -> LINE_SEARCH -> NEWTON ... -> DONE | FAILED).  Small control kernels
+// (one wavefront per RVE) reduce that RVE's tile partials in a fixed order and
+// advance its state with the reference's tolerance formulas and acceptance
+// tests; tiles of RVEs in another state exit at once.  All RVEs take their
+// Newton iterations in lockstep, so the host only counts states between phases.
+// PCG iterations (spectral preconditioner) are captured as a HIP graph of two
+// iterations for the whole batch and replayed until every RVE has converged.
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+#include "dft.hpp"
+#include "hex8.hpp"
+#include "symbol.hpp"
+
+namespace cia {
+namespace bt {
+
+enum { F_GP = 0, F_CL = 1, F_DX = 2, F_DY = 3, F_DZ = 4, NF = 5 };
+enum { S_NEWTON = 0, S_PCG = 1, S_LS = 2, S_DONE = 3, S_FAILED = 4 };
+
+// Rows of the tile-partial array.
+enum {
+  P_RR = 0,    // gather: ||r||^2
+  P_WW = 1,    // gather: ||w||^2
+  P_NORM = 2,  // ||dw||^2
+  P_PQ = 3,    // p.y
+  P_UPD = 4,   // update: r.r, sum r (3)
+  P_RZ = 8,    // r.z
+  P_RESP = 9,  // response sums (B_NRESP rows)
+  B_NRESP = 13,
+  P_ROWS = P_RESP + B_NRESP
+};
+
+// Response sums per RVE (final pass): stress (6), free energy, dissipation,
+// micro work, positive / negative incremental work, unloading and active point
+// counts.
+enum { BR_SIG = 0, BR_PSI = 6, BR_DISS = 7, BR_MICRO = 8, BR_POS = 9, BR_NEG = 10, BR_UNLOAD = 11,
+       BR_ACTIVE = 12 };
+
+struct Desc {
+  int nx, ny, nz, lines_x;
+  int64_t e0, E;
+  BoxGeom g;
+  DftAxis ax, ay, az;
+  const hipFloatComplex* kinv;
+  int t0[NF];
+  int nt[NF];
+  double inv_n;
+  double volume;
+};
+
+struct Req {
+  double e[6];   // requested macro strain
+  double ec[6];  // committed macro strain
+  double dt;
+  double pad;
+};
+
+struct Ctl {
+  int32_t state, reason, it, backtracks;
+  int32_t bt, cur, pcg_its, p_done;
+  int32_t p_iters, fail_it, fail_slot, pad0;
+  unsigned long long first_fail;  // lowest failing local point id of the last assembly
+  int64_t gp_updates;
+  double r0, rn, wn2, last_inc;
+  double rtol, itol, inc_n, alpha;
+  double lin_rtol, cn;
+  double p_tol2, p_rz, p_rz_old, p_alpha, p_beta, p_rr;
+  double p_sumr[3];
+  double fail_rn, fail_rtol, fail_inc, fail_itol;
+};
+
+// Everything a batched kernel needs, passed by value.
+struct Arena {
+  const Desc* desc;
+  const uint16_t* t2r[NF];
+  int T[NF];
+  int R;
+  int Tmax;
+  int force;  // autotuner: every RVE is active, whatever its state
+  int pad;
+  double lin_atol;  // absolute tolerance of the PCG residual
+  Ctl* ctl;
+  const Req* req;
+  const PhaseTable* tabs;
+  const uint8_t* ephase;  // [ET]
+  int64_t NPT, ET;        // total points, elements (= classes)
+  void* st[2];
+  void* sig[2];
+  void* tan[2];
+  void* F;  // [24][ET]
+  double* w[2];
+  double* r[2];
+  double* dw;
+  double* pr;
+  double* pp[2];
+  double* py;
+  double2* uab;
+  double* part;  // [P_ROWS][Tmax]
+};
+
+struct NewtonP {
+  cia_newton_settings ns;
+  double lin_rtol, lin_atol, forcing_max;
+  int adaptive;
+  int pad;
+};
+
+// Blocks walk the tiles in rounds of gridDim.x; xcd_block() hands each XCD one
+// contiguous range of every round.
+#define CIA_TILES(TOT)                                                           \
+  for (int base_ = 0; base_ < (TOT); base_ += (int)gridDim.x)                    \
+    for (int t = base_ + (int)xcd_block(), once_ = 0; once_ == 0 && t < (TOT); once_ = 1)
+
+__device__ __forceinline__ bool pcg_active(const Arena& A, const Ctl& c) {
+  return A.force || (c.state == S_PCG && c.p_done == 0);
+}
+
+// ---------------------------------------------------------------------------
+// Assembly: strain -> constitutive update -> element force rows, one lane per
+// Gauss point (as k_assemble of the single-RVE solver).  `want` selects the
+// RVEs: S_NEWTON (first assembly, slot cur), S_LS (line-search candidate,
+// slot 1 - cur) or S_DONE (final pass with energies, diagnostics, next state
+// and the response sums).
+template <class T, bool EXTRA>
+__global__ void __launch_bounds__(kBlock)
+    k_b_assemble(Arena A, Settings S, int want, const T* __restrict__ state_c,
+                 T* __restrict__ next_state, const double* __restrict__ wc) {
+  __shared__ double cw[kBlock][EXTRA ? 6 : 3];
+  __shared__ double red[kBlock / 64][B_NRESP];
+  const int64_t NPT = A.NPT, ET = A.ET;
+  CIA_TILES(A.T[F_GP]) {
+    const int r = A.t2r[F_GP][t];
+    const Desc& D = A.desc[r];
+    Ctl& c = A.ctl[r];
+    if (!(A.force || c.state == want)) continue;
+    const int slot = want == S_LS ? 1 - c.cur : c.cur;
+    const double* w = A.w[slot];
+    const Req& M = A.req[r];
+    const double dt = M.dt;
+    const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;  // local point
+    const int64_t e = p >> 3;
+    const int q = (int)(p & 7);
+    const bool valid = e < D.E;
+    if (valid) {
+      const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double wv = w[3 * cq + m];
+        cw[threadIdx.x][m] = wv;
+        if (EXTRA) cw[threadIdx.x][3 + m] = wv - wc[3 * cq + m];
+      }
+    }
+    __syncthreads();
+    double acc[B_NRESP];
+#pragma unroll
+    for (int s = 0; s < B_NRESP; ++s) acc[s] = 0.0;
+    if (valid) {
+      const int64_t gp = 8 * D.e0 + p;
+      const int64_t ge = D.e0 + e;
+      T bx[8], by[8], bz[8];
+      gp_factors<T>(q, D.g, bx, by, bz);
+      T eps[6], rate[6];
+      {
+        double we[8][3];
+        const int base = threadIdx.x & ~7;
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][m];
+        strain_from(we, bx, by, bz, eps);
+        if (EXTRA) {
+#pragma unroll
+          for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][3 + m];
+          strain_from(we, bx, by, bz, rate);
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
+      if (EXTRA) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
+      }
+      const PhaseConst P = A.tabs[r].p[A.ephase[ge]];
+      const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
+      T epsp[6];
+      T qb = (T)0, ga = (T)0;
+      if (j2) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * NPT + gp];
+        qb = state_c[6 * NPT + gp];
+        ga = state_c[7 * NPT + gp];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
+      }
+      PointOut<T> o;
+      PointExtra<T> ex;
+      const int st = material_point<T, EXTRA>(P, S, dt, eps, EXTRA ? rate : nullptr, epsp, qb, ga,
+                                              o, EXTRA ? &ex : nullptr, nullptr);
+      if (st != CIA_PT_OK && !A.force) atomicMin(&c.first_fail, (unsigned long long)p);
+      T* sig = (T*)A.sig[slot];
+      T* tan = (T*)A.tan[slot];
+      T* st_t = EXTRA ? next_state : (T*)A.st[slot];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) sig[m * NPT + gp] = o.sig[m];
+      if (j2) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) st_t[m * NPT + gp] = o.epsp[m];
+        st_t[6 * NPT + gp] = o.qbar;
+        st_t[7 * NPT + gp] = o.gam;
+      } else if (EXTRA) {
+        // elastic points carry no state: the caller's rows are zero-filled
+#pragma unroll
+        for (int m = 0; m < 8; ++m) st_t[m * NPT + gp] = (T)0;
+      }
+      tan[0 * NPT + gp] = o.tan.d;
+      tan[1 * NPT + gp] = o.tan.r;
+      if (o.tan.r != (T)0) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) tan[(2 + m) * NPT + gp] = o.tan.n[m];
+      }
+      if (EXTRA) {
+        double mw = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          acc[BR_SIG + m] = (double)o.sig[m];
+          mw += (double)o.sig[m] * (dt * (double)rate[m]);
+        }
+        acc[BR_PSI] = (double)ex.psi;
+        acc[BR_DISS] = (double)ex.diss;
+        acc[BR_MICRO] = mw;
+        const double pw = (double)ex.d[CIA_D_INCREMENTAL_WORK_DENSITY];
+        acc[BR_POS] = fmax(pw, 0.0);
+        acc[BR_NEG] = fmin(pw, 0.0);
+        acc[BR_UNLOAD] = pw < 0.0 ? 1.0 : 0.0;
+        acc[BR_ACTIVE] = (double)ex.d[CIA_D_VISCOPLASTIC_ACTIVE] > 0.5 ? 1.0 : 0.0;
+      }
+      T f[8][3];
+      force_from(o.sig, (T)D.g.jw, bx, by, bz, f);
+      T fin[3];
+      element_reduce8(q, f, fin);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * q + m) * ET + ge] = fin[m];
+    }
+    if (EXTRA) {
+      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+      for (int s = 0; s < B_NRESP; ++s) {
+        const double v = wave_sum(acc[s]);
+        if (lane == 0) red[wid][s] = v;
+      }
+      __syncthreads();
+      if (threadIdx.x < B_NRESP) {
+        double v = 0.0;
+#pragma unroll
+        for (int u = 0; u < kBlock / 64; ++u) v += red[u][threadIdx.x];
+        A.part[(P_RESP + threadIdx.x) * A.Tmax + t] = v;
+      }
+    }
+    __syncthreads();  // cw / red are reused by the next tile
+  }
+}
+
+// Failure record of local point p of RVE r (re-evaluated with the record).
+template <class T>
+__global__ void k_b_diagnose(Arena A, Settings S, int r, int slot, const T* __restrict__ state_c,
+                             const double* __restrict__ wc, int64_t p, cia_point_failure* rec) {
+  const Desc& D = A.desc[r];
+  const Req& M = A.req[r];
+  const int64_t e = p / 8;
+  const int q = (int)(p % 8);
+  const int64_t NPT = A.NPT;
+  const int64_t gp = 8 * D.e0 + p;
+  const double* w = A.w[slot];
+  T we[8][3], dwe[8][3];
+  for (int a = 0; a < 8; ++a) {
+    const int64_t c = D.e0 + periodic_corner(e, a, D.nx, D.ny, D.nz);
+    for (int m = 0; m < 3; ++m) {
+      we[a][m] = (T)w[3 * c + m];
+      dwe[a][m] = (T)(w[3 * c + m] - wc[3 * c + m]);
+    }
+  }
+  T bx[8], by[8], bz[8];
+  gp_factors<T>(q, D.g, bx, by, bz);
+  T eps[6], rate[6];
+  strain_from(we, bx, by, bz, eps);
+  strain_from(dwe, bx, by, bz, rate);
+  for (int m = 0; m < 6; ++m) {
+    eps[m] += (T)M.e[m];
+    rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)M.dt;
+  }
+  const PhaseConst P = A.tabs[r].p[A.ephase[D.e0 + e]];
+  T epsp[6] = {0, 0, 0, 0, 0, 0};
+  T qb = 0, ga = 0;
+  if (P.kind == CIA_MAT_J2_VISCOPLASTIC) {
+    for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * NPT + gp];
+    qb = state_c[6 * NPT + gp];
+    ga = state_c[7 * NPT + gp];
+  }
+  cia_point_failure out;
+  memset(&out, 0, sizeof(out));
+  PointOut<T> o;
+  PointExtra<T> ex;
+  out.reason = material_point<T, true>(P, S, M.dt, eps, rate, epsp, qb, ga, o, &ex, &out);
+  out.point_id = p;
+  *rec = out;
+}
+
+// Periodic node gather of the element force rows into r[slot] (class 0
+// zeroed); partials ||r||^2 and ||w[slot]||^2.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_b_gather(Arena A, int want) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t ET = A.ET;
+  const T* F = (const T*)A.F;
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (!(A.force || c.state == want)) continue;
+    const int slot = want == S_LS ? 1 - c.cur : c.cur;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double rr = 0.0, ww = 0.0;
+    if (cl < D.E) {
+      const unsigned cu = (unsigned)cl, rw = cu / (unsigned)D.nx;
+      const int i = (int)(cu - rw * (unsigned)D.nx);
+      const int k = (int)(rw / (unsigned)D.ny);
+      const int j = (int)(rw - (unsigned)k * (unsigned)D.ny);
+      double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int64_t e = D.e0 + periodic_corner_element(i, j, k, a, D.nx, D.ny, D.nz);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s[m] += (double)F[(3 * a + m) * ET + e];
+      }
+      if (cl == 0) s[0] = s[1] = s[2] = 0.0;
+      const int64_t gc = D.e0 + cl;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        A.r[slot][3 * gc + m] = s[m];
+        rr += s[m] * s[m];
+        const double wv = A.w[slot][3 * gc + m];
+        ww += wv * wv;
+      }
+    }
+    const double a = block_sum<kBlock>(rr, lds);
+    const double b = block_sum<kBlock>(ww, lds);
+    if (threadIdx.x == 0) {
+      A.part[P_RR * A.Tmax + t] = a;
+      A.part[P_WW * A.Tmax + t] = b;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Class-vector kernels.
+
+// w[0] = committed fluctuation (every RVE).
+__global__ void __launch_bounds__(kBlock) k_b_begin(Arena A, const double* __restrict__ wc) {
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    if (cl < D.E) {
+      const int64_t gc = D.e0 + cl;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) A.w[0][3 * gc + m] = wc[3 * gc + m];
+    }
+  }
+}
+
+// Line-search candidate w[1 - cur] = w[cur] + alpha dw.
+__global__ void __launch_bounds__(kBlock) k_b_axpy(Arena A) {
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (c.state != S_LS) continue;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    if (cl < D.E) {
+      const int64_t gc = D.e0 + cl;
+      const double al = c.alpha;
+      const double* src = A.w[c.cur];
+      double* dst = A.w[1 - c.cur];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) dst[3 * gc + m] = src[3 * gc + m] + al * A.dw[3 * gc + m];
+    }
+  }
+}
+
+// ||dw||^2 partials of the RVEs that just solved their Newton system.
+__global__ void __launch_bounds__(kBlock) k_b_norm_dw(Arena A) {
+  __shared__ double lds[kBlock / 64];
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    if (A.ctl[r].state != S_PCG) continue;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double s = 0.0;
+    if (cl < D.E) {
+      const int64_t gc = D.e0 + cl;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double v = A.dw[3 * gc + m];
+        s += v * v;
+      }
+    }
+    const double a = block_sum<kBlock>(s, lds);
+    if (threadIdx.x == 0) A.part[P_NORM * A.Tmax + t] = a;
+  }
+}
+
+// next_w = w[cur] of the converged RVEs.
+__global__ void __launch_bounds__(kBlock) k_b_copy_out(Arena A, double* __restrict__ next_w) {
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (c.state != S_DONE) continue;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    if (cl < D.E) {
+      const int64_t gc = D.e0 + cl;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) next_w[3 * gc + m] = A.w[c.cur][3 * gc + m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral PCG, batched (see the single-RVE solver for the scheme): the scalar
+// steps between the vector kernels run in one wavefront per RVE.
+
+// x = 0, r = -r_newton, p_old = 0, packed r; partials (r.r, sum r).
+__global__ void __launch_bounds__(kBlock) k_b_sp_init(Arena A) {
+  __shared__ double lds[kBlock / 64];
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (!pcg_active(A, c)) continue;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (cl < D.E) {
+      const int64_t gc = D.e0 + cl;
+      const double* rn = A.r[c.cur];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double rv = -rn[3 * gc + m];
+        A.dw[3 * gc + m] = 0.0;
+        A.pr[3 * gc + m] = rv;
+        A.pp[1][3 * gc + m] = 0.0;
+        v[0] += rv * rv;
+        v[1 + m] = rv;
+      }
+      A.uab[2 * D.e0 + cl] = make_double2(v[1], v[2]);
+      A.uab[2 * D.e0 + D.E + cl] = make_double2(v[3], 0.0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double a = block_sum<kBlock>(v[s], lds);
+      if (threadIdx.x == 0) A.part[(P_UPD + s) * A.Tmax + t] = a;
+    }
+  }
+}
+
+// Sum of one partial row over the tiles [t0, t0 + nt) of an RVE, by one
+// wavefront in a fixed order (every lane returns the total).
+__device__ __forceinline__ double rve_sum(const Arena& A, int row, int t0, int nt) {
+  const int lane = threadIdx.x & 63;
+  double a = 0.0;
+  for (int u = lane; u < nt; u += 64) a += A.part[row * A.Tmax + t0 + u];
+  return wave_sum(a);
+}
+
+#define CIA_RVE_WAVE()                                   \
+  const int r = (int)(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)); \
+  if (r >= A.R) return;                                  \
+  const int lane = threadIdx.x & 63;                     \
+  Ctl& c = A.ctl[r];                                     \
+  const Desc& D = A.desc[r];                             \
+  (void)lane;                                            \
+  (void)D
+
+// After an update (FIRST: after the initialisation): r.r, sum r, convergence.
+template <bool FIRST>
+__global__ void __launch_bounds__(kBlock) k_b_scal_upd(Arena A) {
+  CIA_RVE_WAVE();
+  if (!pcg_active(A, c)) return;
+  double u[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) u[s] = rve_sum(A, P_UPD + s, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0 || A.force) return;
+  if (FIRST) {
+    const double tol = fmax(c.lin_rtol * sqrt(u[0]), A.lin_atol);
+    c.p_tol2 = tol * tol;
+  } else {
+    c.p_iters += 1;
+  }
+  c.p_rr = u[0];
+  c.p_sumr[0] = u[1];
+  c.p_sumr[1] = u[2];
+  c.p_sumr[2] = u[3];
+  if (u[0] <= c.p_tol2 || !isfinite(u[0])) c.p_done = isfinite(u[0]) ? 1 : 2;
+}
+
+// r.z: beta of the next element pass.
+__global__ void __launch_bounds__(kBlock) k_b_scal_rz(Arena A) {
+  CIA_RVE_WAVE();
+  if (!pcg_active(A, c)) return;
+  const double rz = rve_sum(A, P_RZ, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0 || A.force) return;
+  if (!(rz > 0.0) || !isfinite(rz)) {
+    c.p_done = 2;
+    return;
+  }
+  c.p_rz_old = c.p_rz;
+  c.p_rz = rz;
+  c.p_beta = c.p_iters == 0 ? 0.0 : rz / c.p_rz_old;
+}
+
+// p.y: alpha of the update.
+__global__ void __launch_bounds__(kBlock) k_b_scal_pq(Arena A) {
+  CIA_RVE_WAVE();
+  if (!pcg_active(A, c)) return;
+  const double pq = rve_sum(A, P_PQ, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0 || A.force) return;
+  if (!(pq > 0.0) || !isfinite(pq)) {
+    c.p_done = 2;
+    return;
+  }
+  c.p_alpha = c.p_rz / pq;
+}
+
+// Symbol solve of the packed spectra per mode pair (k, -k); r.z partials.
+__global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
+  __shared__ double lds[kBlock / 64];
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (!pcg_active(A, c)) continue;
+    const int64_t k = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double2* ua = A.uab + 2 * D.e0;
+    double2* ub = ua + D.E;
+    const double s0 = c.p_sumr[0], s1 = c.p_sumr[1], s2 = c.p_sumr[2];
+    double rz = 0.0;
+    if (k < D.E) {
+      const int64_t mk = neg_mode(k, D.nx, D.ny, D.nz);
+      if (k <= mk) {
+        const double2 a = ua[k], am = ua[mk], b = ub[k];
+        double ar[3] = {0.5 * (a.x + am.x) - s0, 0.5 * (a.y + am.y) - s1, b.x - s2};
+        double ai[3] = {0.5 * (a.y - am.y), -0.5 * (a.x - am.x), b.y};
+        double zr[3], zi[3];
+        spec_solve(D.kinv + 6 * k, false, ar, ai, zr, zi);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          zr[m] *= D.inv_n;
+          zi[m] *= D.inv_n;
+        }
+        const double wk = (k == mk) ? 1.0 : 2.0;
+        rz += wk * (ar[0] * zr[0] + ai[0] * zi[0] + ar[1] * zr[1] + ai[1] * zi[1] +
+                    ar[2] * zr[2] + ai[2] * zi[2]);
+        ua[k] = make_double2(zr[0] - zi[1], zi[0] + zr[1]);
+        ub[k] = make_double2(zr[2], zi[2]);
+        if (mk != k) {
+          ua[mk] = make_double2(zr[0] + zi[1], -zi[0] + zr[1]);
+          ub[mk] = make_double2(zr[2], -zi[2]);
+        }
+      }
+    }
+    const double a = block_sum<kBlock>(rz, lds);
+    if (threadIdx.x == 0) A.part[P_RZ * A.Tmax + t] = a;
+  }
+}
+
+// Element pass: p = (z - z0) + beta p_old at the element corners, forces of
+// K p.  Lane q forms corner q; the 8 lanes of an element exchange corners in LDS.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
+  __shared__ double cpv[kBlock][3];
+  const int64_t NPT = A.NPT, ET = A.ET;
+  const double* p_old = A.pp[1 - pi];
+  double* p_new = A.pp[pi];
+  CIA_TILES(A.T[F_GP]) {
+    const int r = A.t2r[F_GP][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (!pcg_active(A, c)) continue;
+    const double beta = c.p_beta;
+    const T* tan = (const T*)A.tan[c.cur];
+    const double2* zc = A.uab + 2 * D.e0;
+    const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;
+    const int64_t e = p >> 3;
+    const int q = (int)(p & 7);
+    const bool valid = e < D.E;
+    {
+      double own[3] = {0.0, 0.0, 0.0};
+      if (valid) {
+        const double z0[3] = {zc[0].x, zc[0].y, zc[D.E].x};
+        const int64_t cq = periodic_corner(e, q, D.nx, D.ny, D.nz);
+        const double2 t01 = zc[cq], t2 = zc[D.E + cq];
+        const int64_t gq = D.e0 + cq;
+        own[0] = (t01.x - z0[0]) + beta * p_old[3 * gq + 0];
+        own[1] = (t01.y - z0[1]) + beta * p_old[3 * gq + 1];
+        own[2] = (t2.x - z0[2]) + beta * p_old[3 * gq + 2];
+        if (q == 0) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) p_new[3 * (D.e0 + e) + m] = own[m];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) cpv[threadIdx.x][m] = own[m];
+    }
+    __syncthreads();
+    if (valid) {
+      const int64_t gp = 8 * D.e0 + p;
+      T eps[6];
+      T bx[8], by[8], bz[8];
+      gp_factors<T>(q, D.g, bx, by, bz);
+      {
+        double pe[8][3];
+        const int base = threadIdx.x & ~7;
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int m = 0; m < 3; ++m) pe[a][m] = cpv[base + a][m];
+        strain_from(pe, bx, by, bz, eps);
+      }
+      Tangent<T> tc;
+      tc.K = (T)A.tabs[r].p[A.ephase[D.e0 + e]].K;
+      tc.d = tan[gp];
+      tc.r = tan[NPT + gp];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
+      if (tc.r != (T)0) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * NPT + gp];
+      }
+      T s[6];
+      tangent_apply(tc, eps, s);
+      T f[8][3];
+      force_from(s, (T)D.g.jw, bx, by, bz, f);
+      T fin[3];
+      element_reduce8(q, f, fin);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * q + m) * ET + D.e0 + e] = fin[m];
+    }
+    __syncthreads();  // cpv is reused by the next tile
+  }
+}
+
+// y = gather(F) (class 0 zeroed); p.y partials.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_b_node(Arena A, int pi) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t ET = A.ET;
+  const T* F = (const T*)A.F;
+  const double* p = A.pp[pi];
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    if (!pcg_active(A, A.ctl[r])) continue;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double pq = 0.0;
+    if (cl < D.E) {
+      const unsigned cu = (unsigned)cl, rw = cu / (unsigned)D.nx;
+      const int i = (int)(cu - rw * (unsigned)D.nx);
+      const int k = (int)(rw / (unsigned)D.ny);
+      const int j = (int)(rw - (unsigned)k * (unsigned)D.ny);
+      double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int64_t e = D.e0 + periodic_corner_element(i, j, k, a, D.nx, D.ny, D.nz);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s[m] += (double)F[(3 * a + m) * ET + e];
+      }
+      if (cl == 0) s[0] = s[1] = s[2] = 0.0;
+      const int64_t gc = D.e0 + cl;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        A.py[3 * gc + m] = s[m];
+        pq += p[3 * gc + m] * s[m];
+      }
+    }
+    const double a = block_sum<kBlock>(pq, lds);
+    if (threadIdx.x == 0) A.part[P_PQ * A.Tmax + t] = a;
+  }
+}
+
+// x += alpha p, r -= alpha y, packed r; partials (r.r, sum r).
+__global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
+  __shared__ double lds[kBlock / 64];
+  const double* p = A.pp[pi];
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (!pcg_active(A, c)) continue;
+    const double alpha = c.p_alpha;
+    const int64_t cl = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (cl < D.E) {
+      const int64_t gc = D.e0 + cl;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        A.dw[3 * gc + m] += alpha * p[3 * gc + m];
+        const double rv = A.pr[3 * gc + m] - alpha * A.py[3 * gc + m];
+        A.pr[3 * gc + m] = rv;
+        v[0] += rv * rv;
+        v[1 + m] = rv;
+      }
+      A.uab[2 * D.e0 + cl] = make_double2(v[1], v[2]);
+      A.uab[2 * D.e0 + D.E + cl] = make_double2(v[3], 0.0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double a = block_sum<kBlock>(v[s], lds);
+      if (threadIdx.x == 0) A.part[(P_UPD + s) * A.Tmax + t] = a;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 3-D DFT passes of the packed spectra (one tile = a group of x-lines,
+// or 16 consecutive x positions of y- or z-lines, of one RVE).
+
+template <bool INV>
+__global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 wt[kDftMaxTw];
+  CIA_TILES(A.T[F_DX]) {
+    const int r = A.t2r[F_DX][t];
+    const Desc& D = A.desc[r];
+    if (!pcg_active(A, A.ctl[r])) continue;
+    const int nx = D.nx, lines = D.lines_x;
+    const int64_t total_lines = 2 * (int64_t)D.ny * D.nz;
+    const int64_t l0 = (int64_t)(t - D.t0[F_DX]) * lines;
+    const int nl = (int)min<int64_t>(lines, total_lines - l0);
+    const int ld = nx + 1;
+    double2* X = dft_sm;
+    double2* Y = dft_sm + lines * ld;
+    double2* g = A.uab + 2 * D.e0 + l0 * nx;
+    const float inv_nx = 1.0f / (float)nx;
+    load_twiddles(D.ax, wt);
+    for (int u = threadIdx.x; u < nl * nx; u += blockDim.x) {
+      const int l = fdiv(u, inv_nx);
+      X[l * ld + (u - l * nx)] = g[u];
+    }
+    __syncthreads();
+    dft_stage<true, INV>(X, Y, nl, ld, 1, D.ax, wt);
+    __syncthreads();
+    dft_stage<false, INV>(Y, X, nl, ld, 1, D.ax, wt);
+    __syncthreads();
+    for (int u = threadIdx.x; u < nl * nx; u += blockDim.x) {
+      const int l = fdiv(u, inv_nx);
+      g[u] = X[l * ld + (u - l * nx)];
+    }
+    __syncthreads();
+  }
+}
+
+template <bool INV, bool ZPASS>
+__global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 wt[kDftMaxTw];
+  constexpr int FAM = ZPASS ? F_DZ : F_DY;
+  CIA_TILES(A.T[FAM]) {
+    const int r = A.t2r[FAM][t];
+    const Desc& D = A.desc[r];
+    if (!pcg_active(A, A.ctl[r])) continue;
+    const DftAxis& ax = ZPASS ? D.az : D.ay;
+    const int n = ax.n, ld = n + 1;
+    const int tiles_x = (D.nx + kDftTile - 1) / kDftTile;
+    const int lt = t - D.t0[FAM];
+    const int xt = lt % tiles_x, rest = lt / tiles_x;
+    const int no = ZPASS ? D.ny : D.nz;
+    const int o = rest % no, comp = rest / no;
+    double2* X = dft_sm;
+    double2* Y = dft_sm + kDftTile * ld;
+    const int i0 = xt * kDftTile;
+    const int64_t plane = (int64_t)D.nx * D.ny;
+    const int64_t stride = ZPASS ? plane : D.nx;
+    double2* g = A.uab + 2 * D.e0 + (int64_t)comp * D.E + i0 + (ZPASS ? (int64_t)D.nx * o : plane * o);
+    const int width = min(kDftTile, D.nx - i0);
+    load_twiddles(ax, wt);
+    for (int u = threadIdx.x; u < kDftTile * n; u += blockDim.x) {
+      const int e = u / kDftTile, ii = u - e * kDftTile;
+      X[ii * ld + e] = ii < width ? g[ii + stride * e] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    dft_stage<true, INV>(X, Y, kDftTile, ld, 1, ax, wt);
+    __syncthreads();
+    dft_stage<false, INV>(Y, X, kDftTile, ld, 1, ax, wt);
+    __syncthreads();
+    for (int u = threadIdx.x; u < kDftTile * n; u += blockDim.x) {
+      const int e = u / kDftTile, ii = u - e * kDftTile;
+      if (ii < width) g[ii + stride * e] = X[ii * ld + e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Newton control (one wavefront per RVE).
+
+__device__ void newton_test(Ctl& c, const NewtonP& N) {
+  const cia_newton_settings& ns = N.ns;
+  c.rtol = fmax(ns.residual_absolute_tolerance, ns.residual_relative_tolerance * c.r0);
+  c.itol = fmax(ns.increment_absolute_tolerance,
+                ns.increment_relative_tolerance * fmax(sqrt(c.wn2), 1.0));
+  if (c.rn <= c.rtol && (c.it == 0 || c.last_inc <= c.itol)) {
+    c.state = S_DONE;
+    return;
+  }
+  if (c.it == ns.max_iterations) {
+    c.state = S_FAILED;
+    c.reason = CIA_RVE_MAXIMUM_ITERATIONS_EXCEEDED;
+    c.fail_it = c.it;
+    c.fail_rn = c.rn;
+    c.fail_rtol = c.rtol;
+    c.fail_inc = c.last_inc;
+    c.fail_itol = c.itol;
+    return;
+  }
+  double lin = N.lin_rtol;
+  if (N.adaptive) {
+    // inexact Newton: solve only as accurately as the current residual warrants
+    const double eta = fmin(N.forcing_max > 0 ? N.forcing_max : 0.1, c.rn / c.r0);
+    lin = fmax(lin, fmax(eta, 0.5 * c.rtol / fmax(c.rn, 1e-300)));
+    lin = fmin(lin, 0.5);
+  }
+  c.lin_rtol = lin;
+  c.state = S_PCG;
+  c.p_done = 0;
+  c.p_iters = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_ctl_init(Arena A) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= A.R) return;
+  Ctl c;
+  memset(&c, 0, sizeof(c));
+  c.state = S_NEWTON;
+  c.p_done = 1;
+  c.first_fail = ~0ull;
+  c.r0 = -1.0;
+  c.last_inc = INFINITY;
+  c.alpha = 1.0;
+  c.fail_slot = -1;
+  A.ctl[r] = c;
+}
+
+// First assembly: residual and fluctuation norms, then the Newton test.
+__global__ void __launch_bounds__(kBlock) k_b_ctl_first(Arena A, NewtonP N) {
+  CIA_RVE_WAVE();
+  if (c.state != S_NEWTON) return;
+  const double rn2 = rve_sum(A, P_RR, D.t0[F_CL], D.nt[F_CL]);
+  const double wn2 = rve_sum(A, P_WW, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0) return;
+  c.gp_updates = 8 * D.E;
+  if (c.first_fail != ~0ull) {
+    c.state = S_FAILED;
+    c.reason = CIA_RVE_MATERIAL_POINT_FAILURE;
+    c.fail_slot = c.cur;
+    return;
+  }
+  c.rn = sqrt(rn2);
+  c.wn2 = wn2;
+  c.r0 = c.rn;
+  newton_test(c, N);
+}
+
+// After the Newton system solve: increment norm, convergence, line search.
+__global__ void __launch_bounds__(kBlock) k_b_ctl_pcg(Arena A, NewtonP N) {
+  CIA_RVE_WAVE();
+  if (c.state != S_PCG) return;
+  const double dn2 = rve_sum(A, P_NORM, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0) return;
+  c.pcg_its += c.p_iters;
+  const double inc = sqrt(dn2);
+  c.inc_n = inc;
+  if (!isfinite(inc)) {
+    c.state = S_FAILED;
+    c.reason = CIA_RVE_NON_FINITE_INCREMENT;
+    c.fail_it = c.it;
+    c.fail_rn = c.rn;
+    c.fail_rtol = c.rtol;
+    c.fail_inc = inc;
+    c.fail_itol = c.itol;
+    return;
+  }
+  if (c.rn <= c.rtol && inc <= c.itol) {
+    c.state = S_DONE;
+    return;
+  }
+  c.state = S_LS;
+  c.bt = 0;
+  c.alpha = 1.0;
+}
+
+// Line-search candidate evaluated: Armijo test, accept (then the next Newton
+// test) or backtrack.
+__global__ void __launch_bounds__(kBlock) k_b_ctl_ls(Arena A, NewtonP N) {
+  CIA_RVE_WAVE();
+  if (c.state != S_LS) return;
+  const double cn2 = rve_sum(A, P_RR, D.t0[F_CL], D.nt[F_CL]);
+  const double wn2 = rve_sum(A, P_WW, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0) return;
+  const cia_newton_settings& ns = N.ns;
+  c.gp_updates += 8 * D.E;
+  if (c.first_fail != ~0ull) {
+    c.state = S_FAILED;
+    c.reason = CIA_RVE_MATERIAL_POINT_FAILURE;
+    c.fail_slot = 1 - c.cur;
+    return;
+  }
+  const double cn = sqrt(cn2);
+  const double rn = c.rn;
+  const bool accept = !ns.armijo_enabled ||
+                      0.5 * cn * cn <= 0.5 * rn * rn - ns.armijo_sufficient_decrease * c.alpha * rn * rn;
+  if (accept) {
+    c.cur = 1 - c.cur;
+    c.rn = cn;
+    c.wn2 = wn2;
+    c.last_inc = c.alpha * c.inc_n;
+    c.backtracks += c.bt;
+    c.it += 1;
+    newton_test(c, N);
+    return;
+  }
+  c.bt += 1;
+  if (c.bt > ns.armijo_max_backtracks) {
+    c.state = S_FAILED;
+    c.reason = CIA_RVE_ARMIJO_EXHAUSTED;
+    c.fail_it = c.it;
+    c.fail_rn = c.rn;
+    c.fail_rtol = c.rtol;
+    c.fail_inc = c.inc_n;
+    c.fail_itol = c.itol;
+    return;
+  }
+  c.alpha = pow(ns.armijo_reduction_factor, (double)c.bt);
+}
+
+// Response of the converged RVEs: out[r][B_NRESP] (raw sums over points).
+__global__ void __launch_bounds__(kBlock) k_b_resp(Arena A, double* __restrict__ out) {
+  CIA_RVE_WAVE();
+  if (c.state != S_DONE) return;
+  for (int s = 0; s < B_NRESP; ++s) {
+    const double v = rve_sum(A, P_RESP + s, D.t0[F_GP], D.nt[F_GP]);
+    if (lane == 0) out[r * B_NRESP + s] = v;
+  }
+}
+
+// Counts of the RVE states into cnt[0..4], of RVEs with a running PCG into
+// cnt[5], and the largest PCG iteration count of the RVEs in PCG into cnt[6].
+__global__ void __launch_bounds__(kBlock) k_b_count(Arena A, int* __restrict__ cnt) {
+  __shared__ int c[7];
+  if (threadIdx.x < 7) c[threadIdx.x] = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < A.R; r += blockDim.x) {
+    const Ctl& x = A.ctl[r];
+    atomicAdd(&c[x.state], 1);
+    if (x.state == S_PCG && x.p_done == 0) atomicAdd(&c[5], 1);
+    if (x.state == S_PCG) atomicMax(&c[6], x.p_iters);
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) cnt[threadIdx.x] = c[threadIdx.x];
+}
+
+}  // namespace bt
+}  // namespace cia
+
+// ===========================================================================
+// Host object
+// ===========================================================================
+
+using namespace cia;
+using namespace cia::bt;
+
+// Kernels whose grid the autotuner chooses.
+enum { K_ASM = 0, K_GATHER, K_ELEM, K_NODE, K_UPD, K_APPLY, K_DX, K_DY, K_DZ, K_N };
+
+struct cia_rve_batch {
+  int R = 0;
+  int precision = CIA_PREC_FP64;
+  size_t tsize = 8;
+  cia_newton_settings newton{};
+  cia_linear_settings lin{};
+  double hm_scale = 1.0;
+  int64_t NPT = 0, ET = 0;
+  std::vector<Desc> h_desc;
+  std::vector<int64_t> e0;
+  int T[NF] = {0};
+  int Tmax = 0;
+  size_t lds_x = 0, lds_y = 0, lds_z = 0;
+  Arena A{};
+  // device allocations owned by the batch
+  std::vector<void*> allocs;
+  Ctl* ctl = nullptr;
+  Req* req = nullptr;
+  double* resp = nullptr;
+  int* cnt = nullptr;
+  cia_point_failure* rec = nullptr;
+  // pinned host staging
+  Req* h_req = nullptr;
+  int* h_cnt = nullptr;
+  // autotuned grids
+  int grid[K_N] = {0};
+  float tuned_us[K_N] = {0};
+  bool tuned = false;
+  bool autotune = true;
+  int cus = 256;
+  // graph of two PCG iterations
+  hipStream_t cap = nullptr;
+  hipGraphExec_t pair = nullptr;
+  static constexpr int kHints = 32;
+  int its_hint[kHints] = {0};
+};
+
+namespace {
+
+cia_status balloc(cia_rve_batch* B, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc", e, __FILE__, __LINE__);
+    return CIA_ERR_OUT_OF_MEMORY;
+  }
+  B->allocs.push_back(*p);
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMemset", e, __FILE__, __LINE__);
+    return CIA_ERR_HIP;
+  }
+  return CIA_OK;
+}
+
+unsigned kgrid(const cia_rve_batch* B, int k, int fam) {
+  const int g = B->grid[k];
+  const int T = B->T[fam];
+  return (unsigned)std::max(1, (g > 0 && g < T) ? g : T);
+}
+
+template <class T>
+struct BOps {
+  static void assemble(cia_rve_batch* B, hipStream_t st, const Settings& S, int want,
+                       const void* state_c, void* next_state, const double* wc, bool extra) {
+    const unsigned g = kgrid(B, K_ASM, F_GP);
+    if (extra)
+      hipLaunchKernelGGL((k_b_assemble<T, true>), dim3(g), dim3(kBlock), 0, st, B->A, S, want,
+                         (const T*)state_c, (T*)next_state, wc);
+    else
+      hipLaunchKernelGGL((k_b_assemble<T, false>), dim3(g), dim3(kBlock), 0, st, B->A, S, want,
+                         (const T*)state_c, (T*)nullptr, wc);
+  }
+  static void gather(cia_rve_batch* B, hipStream_t st, int want) {
+    hipLaunchKernelGGL(k_b_gather<T>, dim3(kgrid(B, K_GATHER, F_CL)), dim3(kBlock), 0, st, B->A,
+                       want);
+  }
+  static void dft(cia_rve_batch* B, hipStream_t st, bool inverse) {
+    const unsigned gx = kgrid(B, K_DX, F_DX), gy = kgrid(B, K_DY, F_DY), gz = kgrid(B, K_DZ, F_DZ);
+    if (!inverse) {
+      hipLaunchKernelGGL(k_b_dft_x<false>, dim3(gx), dim3(256), B->lds_x, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<false, false>), dim3(gy), dim3(256), B->lds_y, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<false, true>), dim3(gz), dim3(256), B->lds_z, st, B->A);
+    } else {
+      hipLaunchKernelGGL((k_b_dft_strided<true, true>), dim3(gz), dim3(256), B->lds_z, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<true, false>), dim3(gy), dim3(256), B->lds_y, st, B->A);
+      hipLaunchKernelGGL(k_b_dft_x<true>, dim3(gx), dim3(256), B->lds_x, st, B->A);
+    }
+  }
+  static unsigned waves(cia_rve_batch* B) { return (unsigned)((B->R + 3) / 4); }
+  // One PCG iteration (pi selects p_new = pp[pi], p_old = pp[1 - pi]).
+  static void iteration(cia_rve_batch* B, hipStream_t st, int pi) {
+    const unsigned gw = waves(B);
+    hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_GP)), dim3(kBlock), 0, st, B->A, pi);
+    hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, st, B->A, pi);
+    hipLaunchKernelGGL(k_b_scal_pq, dim3(gw), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_update, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A, pi);
+    hipLaunchKernelGGL(k_b_scal_upd<false>, dim3(gw), dim3(kBlock), 0, st, B->A);
+    dft(B, st, false);
+    hipLaunchKernelGGL(k_b_apply, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_scal_rz, dim3(gw), dim3(kBlock), 0, st, B->A);
+    dft(B, st, true);
+  }
+  static void pcg_start(cia_rve_batch* B, hipStream_t st) {
+    const unsigned gw = waves(B);
+    hipLaunchKernelGGL(k_b_sp_init, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_scal_upd<true>, dim3(gw), dim3(kBlock), 0, st, B->A);
+    dft(B, st, false);
+    hipLaunchKernelGGL(k_b_apply, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_scal_rz, dim3(gw), dim3(kBlock), 0, st, B->A);
+    dft(B, st, true);
+  }
+};
+
+cia_status read_counts(cia_rve_batch* B, hipStream_t st, int out[7]) {
+  hipLaunchKernelGGL(k_b_count, dim3(1), dim3(kBlock), 0, st, B->A, B->cnt);
+  CIA_HIP(hipGetLastError());
+  CIA_HIP(hipMemcpyAsync(B->h_cnt, B->cnt, 7 * sizeof(int), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < 7; ++i) out[i] = B->h_cnt[i];
+  return CIA_OK;
+}
+
+// Time `launch` with grid candidates; keep the fastest.
+template <class L>
+cia_status tune_one(cia_rve_batch* B, hipStream_t st, int k, int fam, L&& launch) {
+  const int T = B->T[fam];
+  std::vector<int> cand;
+  cand.push_back(T);  // one tile per block
+  for (int m : {1, 2, 4, 8, 16}) {
+    const int g = m * B->cus;
+    if (g < T) cand.push_back(g);
+  }
+  hipEvent_t e0, e1;
+  CIA_HIP(hipEventCreate(&e0));
+  CIA_HIP(hipEventCreate(&e1));
+  float best = INFINITY;
+  int best_g = T;
+  for (int g : cand) {
+    B->grid[k] = g;
+    launch(st);  // warm-up
+    CIA_HIP(hipEventRecord(e0, st));
+    const int reps = 3;
+    for (int i = 0; i < reps; ++i) launch(st);
+    CIA_HIP(hipEventRecord(e1, st));
+    CIA_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CIA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const float us = 1000.f * ms / reps;
+    if (us < best) {
+      best = us;
+      best_g = g;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  CIA_HIP(hipGetLastError());
+  B->grid[k] = best_g;
+  B->tuned_us[k] = best;
+  return CIA_OK;
+}
+
+// Autotune the grid of every batched kernel on the live operator: called after
+// the first assembly (slot 0 holds valid tangents) and before the first PCG.
+// Only kernels whose outputs are rewritten before they are read are timed on
+// the live data; the assembly and gather recompute their own results.
+template <class T>
+cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const void* state_c,
+                    const double* wc) {
+  B->A.force = 1;
+  cia_status s = CIA_OK;
+  auto run = [&](cia_status x) {
+    if (s == CIA_OK) s = x;
+  };
+  run(tune_one(B, st, K_ASM, F_GP, [&](hipStream_t q) {
+    BOps<T>::assemble(B, q, S, S_NEWTON, state_c, nullptr, wc, false);
+  }));
+  run(tune_one(B, st, K_GATHER, F_CL, [&](hipStream_t q) { BOps<T>::gather(B, q, S_NEWTON); }));
+  run(tune_one(B, st, K_ELEM, F_GP, [&](hipStream_t q) {
+    hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_GP)), dim3(kBlock), 0, q, B->A, 0);
+  }));
+  run(tune_one(B, st, K_NODE, F_CL, [&](hipStream_t q) {
+    hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, q, B->A, 0);
+  }));
+  run(tune_one(B, st, K_UPD, F_CL, [&](hipStream_t q) {
+    hipLaunchKernelGGL(k_b_update, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, q, B->A, 0);
+  }));
+  run(tune_one(B, st, K_APPLY, F_CL, [&](hipStream_t q) {
+    hipLaunchKernelGGL(k_b_apply, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, q, B->A);
+  }));
+  run(tune_one(B, st, K_DX, F_DX, [&](hipStream_t q) {
+    hipLaunchKernelGGL(k_b_dft_x<false>, dim3(kgrid(B, K_DX, F_DX)), dim3(256), B->lds_x, q, B->A);
+  }));
+  run(tune_one(B, st, K_DY, F_DY, [&](hipStream_t q) {
+    hipLaunchKernelGGL((k_b_dft_strided<false, false>), dim3(kgrid(B, K_DY, F_DY)), dim3(256),
+                       B->lds_y, q, B->A);
+  }));
+  run(tune_one(B, st, K_DZ, F_DZ, [&](hipStream_t q) {
+    hipLaunchKernelGGL((k_b_dft_strided<false, true>), dim3(kgrid(B, K_DZ, F_DZ)), dim3(256),
+                       B->lds_z, q, B->A);
+  }));
+  B->A.force = 0;
+  if (s != CIA_OK) return s;
+  // The timed assembly and gather recomputed slot 0, r[0] and the gather
+  // partials bit for bit; the other timed kernels only touched buffers that the
+  // PCG initialisation rewrites (z spectra, p, y, dw, the PCG residual, F).
+  B->tuned = true;
+  return CIA_OK;
+}
+
+template <class T>
+cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, const double* wc,
+                       const double* cmacro, const double* macro, const double* dt,
+                       const Settings& S, void* next_state, double* next_w,
+                       cia_rve_result* res) {
+  const int R = B->R;
+  for (int r = 0; r < R; ++r) {
+    Req q{};
+    for (int m = 0; m < 6; ++m) {
+      q.e[m] = macro[6 * r + m];
+      q.ec[m] = cmacro[6 * r + m];
+    }
+    q.dt = dt[r];
+    B->h_req[r] = q;
+  }
+  CIA_HIP(hipMemcpyAsync(B->req, B->h_req, R * sizeof(Req), hipMemcpyHostToDevice, st));
+  const unsigned gcl = kgrid(B, K_UPD, F_CL);
+  hipLaunchKernelGGL(k_b_begin, dim3(gcl), dim3(kBlock), 0, st, B->A, wc);
+  hipLaunchKernelGGL(k_b_ctl_init, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, st, B->A);
+  BOps<T>::assemble(B, st, S, S_NEWTON, state_c, nullptr, wc, false);
+  BOps<T>::gather(B, st, S_NEWTON);
+  CIA_HIP(hipGetLastError());
+  if (!B->tuned) {
+    if (B->autotune) {
+      CIA_TRY(autotune<T>(B, st, S, state_c, wc));
+    } else {
+      B->tuned = true;
+    }
+  }
+  if (!B->pair && B->cap) {
+    hipGraph_t g = nullptr;
+    CIA_HIP(hipStreamBeginCapture(B->cap, hipStreamCaptureModeThreadLocal));
+    BOps<T>::iteration(B, B->cap, 0);
+    BOps<T>::iteration(B, B->cap, 1);
+    CIA_HIP(hipStreamEndCapture(B->cap, &g));
+    const hipError_t e = hipGraphInstantiate(&B->pair, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CIA_HIP(e);
+  }
+  NewtonP N{};
+  N.ns = B->newton;
+  N.lin_rtol = B->lin.relative_tolerance;
+  N.lin_atol = B->lin.absolute_tolerance;
+  N.forcing_max = B->lin.forcing_max;
+  N.adaptive = B->lin.adaptive_forcing;
+  const unsigned gw = (unsigned)((R + 3) / 4);
+  hipLaunchKernelGGL(k_b_ctl_first, dim3(gw), dim3(kBlock), 0, st, B->A, N);
+  CIA_HIP(hipGetLastError());
+  int cnt[7];
+  const int interval = std::max(1, B->lin.check_interval);
+  for (int round = 0;; ++round) {
+    CIA_TRY(read_counts(B, st, cnt));
+    if (cnt[S_PCG] == 0) break;
+    // Newton systems of every RVE in this round, solved together
+    BOps<T>::pcg_start(B, st);
+    int it = 0;
+    const bool hinted = round < cia_rve_batch::kHints && B->its_hint[round] > 0;
+    int chunk = hinted ? B->its_hint[round] : interval;
+    while (it < B->lin.max_iterations) {
+      const int n = std::min(chunk, B->lin.max_iterations - it);
+      chunk = hinted ? std::max(2, interval / 4) : interval;
+      if (B->pair) {
+        // whole pairs; an odd final iteration below the limit runs on its own
+        const int pairs = std::min((n + 1) / 2, (B->lin.max_iterations - it) / 2);
+        for (int s = 0; s < pairs; ++s, it += 2) CIA_HIP(hipGraphLaunch(B->pair, st));
+        if (pairs == 0) {
+          BOps<T>::iteration(B, st, it & 1);
+          ++it;
+        }
+      } else {
+        for (int s = 0; s < n; ++s, ++it) BOps<T>::iteration(B, st, it & 1);
+      }
+      CIA_HIP(hipGetLastError());
+      CIA_TRY(read_counts(B, st, cnt));
+      if (cnt[5] == 0) break;
+    }
+    if (round < cia_rve_batch::kHints) B->its_hint[round] = std::max(1, cnt[6]);
+    hipLaunchKernelGGL(k_b_norm_dw, dim3(gcl), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_ctl_pcg, dim3(gw), dim3(kBlock), 0, st, B->A, N);
+    CIA_HIP(hipGetLastError());
+    for (;;) {
+      CIA_TRY(read_counts(B, st, cnt));
+      if (cnt[S_LS] == 0) break;
+      hipLaunchKernelGGL(k_b_axpy, dim3(gcl), dim3(kBlock), 0, st, B->A);
+      BOps<T>::assemble(B, st, S, S_LS, state_c, nullptr, wc, false);
+      BOps<T>::gather(B, st, S_LS);
+      hipLaunchKernelGGL(k_b_ctl_ls, dim3(gw), dim3(kBlock), 0, st, B->A, N);
+      CIA_HIP(hipGetLastError());
+    }
+  }
+  // converged RVEs: final pass (energies, next state, response sums), next w
+  BOps<T>::assemble(B, st, S, S_DONE, state_c, next_state, wc, true);
+  hipLaunchKernelGGL(k_b_resp, dim3(gw), dim3(kBlock), 0, st, B->A, B->resp);
+  hipLaunchKernelGGL(k_b_copy_out, dim3(gcl), dim3(kBlock), 0, st, B->A, next_w);
+  CIA_HIP(hipGetLastError());
+  std::vector<Ctl> hc(R);
+  std::vector<double> hr((size_t)R * B_NRESP);
+  CIA_HIP(hipMemcpyAsync(hc.data(), B->ctl, R * sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipMemcpyAsync(hr.data(), B->resp, hr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  bool all_ok = true;
+  for (int r = 0; r < R; ++r) {
+    cia_rve_result& o = res[r];
+    memset(&o, 0, sizeof(o));
+    o.point_failure.point_id = -1;
+    const Ctl& c = hc[r];
+    const Desc& D = B->h_desc[r];
+    o.newton_iterations = c.it;
+    o.armijo_backtracks = c.backtracks;
+    o.linear_iterations = c.pcg_its;
+    o.gp_updates = c.gp_updates;
+    o.micro_residual_norm = c.rn;
+    o.micro_residual_tolerance = c.rtol;
+    o.volume = D.volume;
+    if (c.state != S_DONE) {
+      all_ok = false;
+      o.reason = c.state == S_FAILED ? c.reason : CIA_RVE_MAXIMUM_ITERATIONS_EXCEEDED;
+      o.failure_iteration = c.fail_it;
+      o.failure_residual_norm = c.fail_rn;
+      o.failure_residual_tolerance = c.fail_rtol;
+      o.failure_increment_norm = c.fail_inc;
+      o.failure_increment_tolerance = c.fail_itol;
+      if (c.reason == CIA_RVE_MATERIAL_POINT_FAILURE && c.fail_slot >= 0) {
+        hipLaunchKernelGGL(k_b_diagnose<T>, dim3(1), dim3(1), 0, st, B->A, S, r, c.fail_slot,
+                           (const T*)state_c, wc, (int64_t)c.first_fail, B->rec);
+        CIA_HIP(hipGetLastError());
+        CIA_HIP(hipMemcpyAsync(&o.point_failure, B->rec, sizeof(cia_point_failure),
+                               hipMemcpyDeviceToHost, st));
+        CIA_HIP(hipStreamSynchronize(st));
+      }
+      continue;
+    }
+    o.reason = CIA_RVE_OK;
+    const double* s = &hr[(size_t)r * B_NRESP];
+    const double jw = D.g.jw, V = D.volume;
+    for (int m = 0; m < 6; ++m) o.macro_stress[m] = s[BR_SIG + m] * jw / V;
+    o.free_energy_density = s[BR_PSI] * jw / V;
+    o.dissipation_density = s[BR_DISS] * jw / V;
+    const double micro = s[BR_MICRO] * jw / V;
+    double macro_w = 0.0;
+    for (int m = 0; m < 6; ++m) macro_w += o.macro_stress[m] * (macro[6 * r + m] - cmacro[6 * r + m]);
+    o.macro_work_increment = macro_w;
+    o.micro_work_increment = micro;
+    o.hill_mandel_residual = macro_w - micro;
+    o.hill_mandel_error = fabs(macro_w - micro) / (fabs(macro_w) + fabs(micro) + B->hm_scale * dt[r]);
+    o.positive_work_density = s[BR_POS] * jw / V;
+    o.negative_work_density = s[BR_NEG] * jw / V;
+    o.unloading_volume_fraction = s[BR_UNLOAD] * jw / V;
+    o.viscoplastic_active_volume_fraction = s[BR_ACTIVE] * jw / V;
+    o.minimum_yield_activation_margin = NAN;
+    o.maximum_yield_activation_margin = NAN;
+    o.virtual_work_stress_error = NAN;
+    o.periodic_fluctuation_error = NAN;
+    o.reaction_antiperiodicity_error = NAN;
+    o.periodic_class_reaction_error = NAN;
+  }
+  return all_ok ? CIA_OK : CIA_ERR_RVE_FAILURE;
+}
+
+}  // namespace
+
+extern "C" {
+
+cia_status cia_rve_batch_destroy(cia_rve_batch* B) {
+  if (!B) return CIA_OK;
+  for (void* p : B->allocs) (void)hipFree(p);
+  if (B->h_req) (void)hipHostFree(B->h_req);
+  if (B->h_cnt) (void)hipHostFree(B->h_cnt);
+  if (B->pair) (void)hipGraphExecDestroy(B->pair);
+  if (B->cap) (void)hipStreamDestroy(B->cap);
+  delete B;
+  return CIA_OK;
+}
+
+cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t autotune,
+                                cia_rve_batch** out) {
+  CIA_ARG(descs && out && n >= 1 && n <= 65535, "invalid batch descriptor array");
+  const cia_rve_desc& d0 = descs[0];
+  for (int r = 0; r < n; ++r) {
+    const cia_rve_desc& d = descs[r];
+    CIA_ARG(d.nx >= 2 && d.ny >= 2 && d.nz >= 2 && d.nx <= 64 && d.ny <= 64 && d.nz <= 64,
+            "batched RVE edges must lie in [2, 64] elements");
+    CIA_ARG(d.n_phases >= 1 && d.n_phases <= CIA_MAX_PHASES && d.phases, "invalid phases");
+    CIA_ARG(d.lengths[0] > 0 && d.lengths[1] > 0 && d.lengths[2] > 0, "lengths must be positive");
+    CIA_ARG(d.precision == d0.precision && d.hill_mandel_power_scale == d0.hill_mandel_power_scale &&
+                memcmp(&d.newton, &d0.newton, sizeof(d.newton)) == 0 &&
+                memcmp(&d.linear, &d0.linear, sizeof(d.linear)) == 0,
+            "all RVEs of a batch share precision, Newton and linear-solver settings");
+  }
+  CIA_ARG(d0.precision == CIA_PREC_FP64 || d0.precision == CIA_PREC_FP32_REFINED, "bad precision");
+  CIA_ARG(d0.linear.preconditioner == CIA_PCG_SPECTRAL,
+          "the batched solver uses the spectral preconditioner");
+  auto* B = new cia_rve_batch();
+  B->R = n;
+  B->precision = d0.precision;
+  B->tsize = d0.precision == CIA_PREC_FP64 ? sizeof(double) : sizeof(float);
+  B->newton = d0.newton;
+  B->lin = d0.linear;
+  if (B->lin.max_iterations <= 0) B->lin.max_iterations = 20000;
+  if (B->lin.check_interval <= 0) B->lin.check_interval = 16;
+  B->hm_scale = d0.hill_mandel_power_scale;
+  B->autotune = autotune != 0;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      B->cus = prop.multiProcessorCount;
+  }
+  // sizes and tiles
+  B->h_desc.resize(n);
+  int64_t e0 = 0;
+  int tcount[NF] = {0};
+  size_t lx = 0, ly = 0, lz = 0;
+  for (int r = 0; r < n; ++r) {
+    const cia_rve_desc& d = descs[r];
+    Desc& D = B->h_desc[r];
+    D.nx = d.nx;
+    D.ny = d.ny;
+    D.nz = d.nz;
+    D.lines_x = std::max(1, 256 / d.nx);
+    D.e0 = e0;
+    D.E = (int64_t)d.nx * d.ny * d.nz;
+    const double hx = d.lengths[0] / d.nx, hy = d.lengths[1] / d.ny, hz = d.lengths[2] / d.nz;
+    D.g.ihx = 1.0 / hx;
+    D.g.ihy = 1.0 / hy;
+    D.g.ihz = 1.0 / hz;
+    D.g.jw = hx * hy * hz / 8.0;
+    D.volume = D.g.jw * 8.0 * (double)D.E;
+    D.inv_n = 1.0 / (double)D.E;
+    const int nt[NF] = {
+        (int)((8 * D.E + kBlock - 1) / kBlock), (int)((D.E + kBlock - 1) / kBlock),
+        (int)((2 * (int64_t)d.ny * d.nz + D.lines_x - 1) / D.lines_x),
+        ((d.nx + kDftTile - 1) / kDftTile) * d.nz * 2, ((d.nx + kDftTile - 1) / kDftTile) * d.ny * 2};
+    for (int f = 0; f < NF; ++f) {
+      D.t0[f] = tcount[f];
+      D.nt[f] = nt[f];
+      tcount[f] += nt[f];
+    }
+    lx = std::max(lx, 2 * (size_t)D.lines_x * (d.nx + 1) * sizeof(double2));
+    ly = std::max(ly, 2 * (size_t)kDftTile * (d.ny + 1) * sizeof(double2));
+    lz = std::max(lz, 2 * (size_t)kDftTile * (d.nz + 1) * sizeof(double2));
+    e0 += D.E;
+  }
+  B->ET = e0;
+  B->NPT = 8 * e0;
+  B->lds_x = lx;
+  B->lds_y = ly;
+  B->lds_z = lz;
+  for (int f = 0; f < NF; ++f) {
+    B->T[f] = tcount[f];
+    B->Tmax = std::max(B->Tmax, tcount[f]);
+  }
+  const size_t ts = B->tsize;
+  const int64_t NPT = B->NPT, ET = B->ET;
+  cia_status s = CIA_OK;
+#define BALLOC(ptr, bytes)                         \
+  do {                                             \
+    s = balloc(B, (void**)&(ptr), (bytes));        \
+    if (s != CIA_OK) {                             \
+      cia_rve_batch_destroy(B);                    \
+      return s;                                    \
+    }                                              \
+  } while (0)
+  Arena& A = B->A;
+  A.R = n;
+  A.lin_atol = B->lin.absolute_tolerance;
+  A.Tmax = B->Tmax;
+  A.NPT = NPT;
+  A.ET = ET;
+  for (int f = 0; f < NF; ++f) A.T[f] = B->T[f];
+  Desc* d_desc = nullptr;
+  BALLOC(d_desc, n * sizeof(Desc));
+  for (int f = 0; f < NF; ++f) {
+    uint16_t* t2r = nullptr;
+    BALLOC(t2r, std::max(1, B->T[f]) * sizeof(uint16_t));
+    std::vector<uint16_t> h(B->T[f]);
+    for (int r = 0; r < n; ++r)
+      for (int u = 0; u < B->h_desc[r].nt[f]; ++u) h[B->h_desc[r].t0[f] + u] = (uint16_t)r;
+    if (hipMemcpy(t2r, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess) {
+      cia_rve_batch_destroy(B);
+      set_error_msg("hipMemcpy of the batch tile table failed");
+      return CIA_ERR_HIP;
+    }
+    A.t2r[f] = t2r;
+  }
+  PhaseTable* tabs = nullptr;
+  BALLOC(tabs, n * sizeof(PhaseTable));
+  uint8_t* eph = nullptr;
+  BALLOC(eph, ET);
+  {
+    std::vector<PhaseTable> ht(n);
+    std::vector<uint8_t> he(ET, 0);
+    for (int r = 0; r < n; ++r) {
+      const cia_rve_desc& d = descs[r];
+      ht[r] = PhaseTable{};
+      ht[r].n = d.n_phases;
+      for (int i = 0; i < d.n_phases; ++i) ht[r].p[i] = make_phase(d.phases[i]);
+      if (d.element_phase) {
+        for (int64_t e = 0; e < B->h_desc[r].E; ++e) {
+          const int32_t v = d.element_phase[e];
+          if (v < 0 || v >= d.n_phases) {
+            cia_rve_batch_destroy(B);
+            set_error_msg("element_phase out of range");
+            return CIA_ERR_INVALID_ARGUMENT;
+          }
+          he[B->h_desc[r].e0 + e] = (uint8_t)v;
+        }
+      }
+    }
+    if (hipMemcpy(tabs, ht.data(), n * sizeof(PhaseTable), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(eph, he.data(), ET, hipMemcpyHostToDevice) != hipSuccess) {
+      cia_rve_batch_destroy(B);
+      set_error_msg("hipMemcpy of the batch phase tables failed");
+      return CIA_ERR_HIP;
+    }
+  }
+  A.tabs = tabs;
+  A.ephase = eph;
+  // Spectral symbols and DFT twiddles, shared by RVEs of equal geometry and medium.
+  {
+    std::map<std::tuple<int, int, int, double, double, double, double, double>, size_t> sym_of;
+    std::vector<hipFloatComplex> sym_all;
+    std::map<int, size_t> tw_of;
+    std::vector<double2> tw_all;
+    std::vector<size_t> sym_off(n);
+    std::vector<size_t> tw_off(3 * n);
+    for (int r = 0; r < n; ++r) {
+      const cia_rve_desc& d = descs[r];
+      double kmin = INFINITY, kmax = 0.0, gmin = INFINITY, gmax = 0.0;
+      for (int i = 0; i < d.n_phases; ++i) {
+        const PhaseConst p = make_phase(d.phases[i]);
+        kmin = std::min(kmin, p.K);
+        kmax = std::max(kmax, p.K);
+        gmin = std::min(gmin, p.G);
+        gmax = std::max(gmax, p.G);
+      }
+      const double K0 = d.linear.reference_bulk > 0 ? d.linear.reference_bulk : std::sqrt(kmin * kmax);
+      const double G0 = d.linear.reference_shear > 0 ? d.linear.reference_shear : std::sqrt(gmin * gmax);
+      const auto key = std::make_tuple(d.nx, d.ny, d.nz, d.lengths[0], d.lengths[1], d.lengths[2], K0, G0);
+      auto it = sym_of.find(key);
+      if (it == sym_of.end()) {
+        std::vector<hipFloatComplex> sym;
+        spectral_symbol(d.nx, d.ny, d.nz, B->h_desc[r].g, K0, G0, sym);
+        it = sym_of.emplace(key, sym_all.size()).first;
+        sym_all.insert(sym_all.end(), sym.begin(), sym.end());
+      }
+      sym_off[r] = it->second;
+      const int edges[3] = {d.nx, d.ny, d.nz};
+      for (int a = 0; a < 3; ++a) {
+        auto jt = tw_of.find(edges[a]);
+        if (jt == tw_of.end()) jt = tw_of.emplace(edges[a], dft_twiddles(edges[a], tw_all)).first;
+        tw_off[3 * r + a] = jt->second;
+      }
+    }
+    hipFloatComplex* kinv = nullptr;
+    double2* tw = nullptr;
+    BALLOC(kinv, sym_all.size() * sizeof(hipFloatComplex));
+    BALLOC(tw, tw_all.size() * sizeof(double2));
+    if (hipMemcpy(kinv, sym_all.data(), sym_all.size() * sizeof(hipFloatComplex),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(tw, tw_all.data(), tw_all.size() * sizeof(double2), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      cia_rve_batch_destroy(B);
+      set_error_msg("hipMemcpy of the batch spectral tables failed");
+      return CIA_ERR_HIP;
+    }
+    for (int r = 0; r < n; ++r) {
+      Desc& D = B->h_desc[r];
+      D.kinv = kinv + sym_off[r];
+      auto axis = [&](int nn, size_t off) {
+        DftAxis a{};
+        a.n = nn;
+        dft_factor(nn, a.n1, a.n2);
+        a.w = tw + off;
+        return a;
+      };
+      D.ax = axis(D.nx, tw_off[3 * r + 0]);
+      D.ay = axis(D.ny, tw_off[3 * r + 1]);
+      D.az = axis(D.nz, tw_off[3 * r + 2]);
+    }
+  }
+  if (hipMemcpy(d_desc, B->h_desc.data(), n * sizeof(Desc), hipMemcpyHostToDevice) != hipSuccess) {
+    cia_rve_batch_destroy(B);
+    set_error_msg("hipMemcpy of the batch descriptors failed");
+    return CIA_ERR_HIP;
+  }
+  A.desc = d_desc;
+  for (int k = 0; k < 2; ++k) {
+    BALLOC(A.st[k], 8 * NPT * ts);
+    BALLOC(A.sig[k], 6 * NPT * ts);
+    BALLOC(A.tan[k], 8 * NPT * ts);
+    BALLOC(A.w[k], 3 * ET * 8);
+    BALLOC(A.r[k], 3 * ET * 8);
+    BALLOC(A.pp[k], 3 * ET * 8);
+  }
+  BALLOC(A.F, 24 * ET * ts);
+  BALLOC(A.dw, 3 * ET * 8);
+  BALLOC(A.pr, 3 * ET * 8);
+  BALLOC(A.py, 3 * ET * 8);
+  BALLOC(A.uab, 2 * ET * sizeof(double2));
+  BALLOC(A.part, (size_t)P_ROWS * B->Tmax * 8);
+  BALLOC(B->ctl, n * sizeof(Ctl));
+  BALLOC(B->req, n * sizeof(Req));
+  BALLOC(B->resp, (size_t)n * B_NRESP * 8);
+  BALLOC(B->cnt, 64);
+  BALLOC(B->rec, sizeof(cia_point_failure));
+#undef BALLOC
+  A.ctl = B->ctl;
+  A.req = B->req;
+  if (hipHostMalloc((void**)&B->h_req, n * sizeof(Req)) != hipSuccess ||
+      hipHostMalloc((void**)&B->h_cnt, 64) != hipSuccess) {
+    cia_rve_batch_destroy(B);
+    set_error_msg("hipHostMalloc of the batch staging buffers failed");
+    return CIA_ERR_OUT_OF_MEMORY;
+  }
+  if (hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking) != hipSuccess) B->cap = nullptr;
+  *out = B;
+  return CIA_OK;
+}
+
+cia_status cia_rve_batch_layout(const cia_rve_batch* B, int64_t* element_offsets) {
+  CIA_ARG(B && element_offsets, "NULL argument");
+  for (int r = 0; r < B->R; ++r) element_offsets[r] = B->h_desc[r].e0;
+  element_offsets[B->R] = B->ET;
+  return CIA_OK;
+}
+
+cia_status cia_rve_batch_solve(cia_rve_batch* B, void* stream, const void* committed_state,
+                               const double* committed_w, const double* committed_macro,
+                               const double* macro_strain, const double* time_step,
+                               const cia_update_settings* settings, void* next_state,
+                               double* next_w, cia_rve_result* results) {
+  CIA_ARG(B && committed_state && committed_w && committed_macro && macro_strain && time_step &&
+              settings && next_state && next_w && results,
+          "NULL argument to cia_rve_batch_solve");
+  CIA_ARG((const void*)next_state != committed_state, "next_state aliases committed_state");
+  CIA_ARG((const void*)next_w != (const void*)committed_w, "next_w aliases committed_w");
+  for (int r = 0; r < B->R; ++r) CIA_ARG(time_step[r] > 0.0, "RVE time step must be positive.");
+  const Settings S = make_settings(*settings);
+  hipStream_t st = (hipStream_t)stream;
+  if (B->precision == CIA_PREC_FP64)
+    return solve_batch<double>(B, st, committed_state, committed_w, committed_macro, macro_strain,
+                               time_step, S, next_state, next_w, results);
+  return solve_batch<float>(B, st, committed_state, committed_w, committed_macro, macro_strain,
+                            time_step, S, next_state, next_w, results);
+}
+
+cia_status cia_rve_batch_tuning(const cia_rve_batch* B, int32_t* grids, double* microseconds,
+                                int32_t n) {
+  CIA_ARG(B && grids && n >= 0, "NULL argument");
+  for (int k = 0; k < n && k < K_N; ++k) {
+    grids[k] = B->grid[k];
+    if (microseconds) microseconds[k] = B->tuned_us[k];
+  }
+  return CIA_OK;
+}
+
+cia_status cia_rve_batch_set_tuning(cia_rve_batch* B, const int32_t* grids, int32_t n) {
+  CIA_ARG(B && grids && n == K_N, "grids must hold one entry per tuned kernel");
+  CIA_ARG(!B->pair, "the tuning of a batch is fixed once its first solve captured the PCG graph");
+  for (int k = 0; k < K_N; ++k) {
+    CIA_ARG(grids[k] >= 0, "grid sizes must be non-negative (0: one tile per block)");
+    B->grid[k] = grids[k];
+  }
+  B->tuned = true;
+  return CIA_OK;
+}
+
+}  // extern "C"

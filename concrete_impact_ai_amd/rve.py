@@ -300,6 +300,43 @@ def _box_lengths(model: StructuredHex8RVE):
     return lengths
 
 
+
+def rve_descriptor(model: StructuredHex8RVE, phases, divisions, lengths, element_phase_ids):
+    """C descriptor of one structured periodic RVE (``cia_rve_desc``); returns it
+    with the ctypes arrays it points to, which must outlive the create call."""
+    nx, ny, nz = divisions
+    ns = model.newton_settings
+    ls = model.linear_settings
+    mats = (_lib.Material * len(phases))(*[cia_material(m) for _, m in phases])
+    eph = np.ascontiguousarray(np.asarray(element_phase_ids).astype(np.int32))
+    desc = _lib.RVEDesc()
+    desc.nx, desc.ny, desc.nz = nx, ny, nz
+    desc.n_phases = len(phases)
+    desc.lengths = (C.c_double * 3)(*[float(v) for v in lengths])
+    desc.phases = mats
+    desc.element_phase = eph.ctypes.data_as(C.POINTER(C.c_int32))
+    desc.newton = _lib.NewtonSettings(
+        int(ns.max_iterations), int(bool(ns.armijo.enabled)),
+        float(ns.residual_absolute_tolerance), float(ns.residual_relative_tolerance),
+        float(ns.increment_absolute_tolerance), float(ns.increment_relative_tolerance),
+        float(ns.armijo.sufficient_decrease), float(ns.armijo.reduction_factor),
+        int(ns.armijo.max_backtracks), 0,
+    )
+    if ls.method not in ("spectral", "block_jacobi"):
+        raise ValueError(f"unknown PCG preconditioner {ls.method!r}")
+    desc.linear = _lib.LinearSettings(
+        int(ls.max_iterations), int(ls.check_interval), float(ls.relative_tolerance),
+        float(ls.absolute_tolerance),
+        _lib.CIA_PCG_SPECTRAL if ls.method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI,
+        int(bool(ls.adaptive_forcing)), float(ls.reference_bulk), float(ls.reference_shear),
+        float(ls.forcing_max))
+    desc.hill_mandel_power_scale = float(model.hill_mandel_power_scale)
+    if model.precision not in ("fp64", "fp32"):
+        raise ValueError(f"unknown RVE precision {model.precision!r}")
+    desc.precision = _lib.CIA_PREC_FP64 if model.precision == "fp64" else _lib.CIA_PREC_FP32_REFINED
+    return desc, (mats, eph)
+
+
 class RVEOperatorCache:
     """Device operator of one RVE (``build_rve_operator_cache``, solver.py:367-388)."""
 
@@ -321,34 +358,7 @@ class RVEOperatorCache:
         self.volume = float(np.sum(self.quadrature_weights))
         self.precision = model.precision
         self.dtype = torch.float64 if model.precision == "fp64" else torch.float32
-        ns = model.newton_settings
-        ls = model.linear_settings
-        mats = (_lib.Material * len(self.phases))(*[cia_material(m) for _, m in self.phases])
-        self._mats = mats
-        eph = np.ascontiguousarray(ep.astype(np.int32))
-        desc = _lib.RVEDesc()
-        desc.nx, desc.ny, desc.nz = nx, ny, nz
-        desc.n_phases = len(self.phases)
-        desc.lengths = (C.c_double * 3)(*[float(v) for v in self.lengths])
-        desc.phases = mats
-        desc.element_phase = eph.ctypes.data_as(C.POINTER(C.c_int32))
-        desc.newton = _lib.NewtonSettings(
-            int(ns.max_iterations), int(bool(ns.armijo.enabled)),
-            float(ns.residual_absolute_tolerance), float(ns.residual_relative_tolerance),
-            float(ns.increment_absolute_tolerance), float(ns.increment_relative_tolerance),
-            float(ns.armijo.sufficient_decrease), float(ns.armijo.reduction_factor),
-            int(ns.armijo.max_backtracks), 0,
-        )
-        if ls.method not in ("spectral", "block_jacobi"):
-            raise ValueError(f"unknown PCG preconditioner {ls.method!r}")
-        desc.linear = _lib.LinearSettings(
-            int(ls.max_iterations), int(ls.check_interval), float(ls.relative_tolerance),
-            float(ls.absolute_tolerance),
-            _lib.CIA_PCG_SPECTRAL if ls.method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI,
-            int(bool(ls.adaptive_forcing)), float(ls.reference_bulk), float(ls.reference_shear),
-            float(ls.forcing_max))
-        desc.hill_mandel_power_scale = float(model.hill_mandel_power_scale)
-        desc.precision = _lib.CIA_PREC_FP64 if model.precision == "fp64" else _lib.CIA_PREC_FP32_REFINED
+        desc, self._keep = rve_descriptor(model, self.phases, self.divisions, self.lengths, ep)
         device()
         handle = C.c_void_p()
         _lib.check(_lib.lib().cia_rve_create(C.byref(desc), C.byref(handle)), "cia_rve_create")

@@ -345,6 +345,55 @@ cia_status cia_rve_copy_field(cia_rve* rve, void* stream, int32_t field, double*
 
 
 /* ------------------------------------------------------------------------ */
+/* Batched periodic RVEs: many independent structured RVEs of any sizes (edges
+ * 2..64 elements) solved together, each increment being
+ * solve_rve_microequilibrium (src/fem/rve/solver.py:62-227).  Replaces the
+ * serial / process-pool loop over RVE solves of RVEHomogenizedMaterial.update
+ * (src/fem/rve/material.py:104-191, workers :267-291) and of several
+ * iterate_prescribed_rve_path drivers (src/fem/rve/path.py:47-100).
+ *
+ * All descriptors share precision, Newton and linear-solver settings
+ * (spectral preconditioner) and the Hill-Mandel power scale; phases,
+ * microstructure, lengths and divisions are per RVE.  With autotune = 1 the
+ * first solve times each batched kernel over a set of grid sizes (one tile per
+ * workgroup up to persistent grids of 1-16 workgroups per CU) and keeps the
+ * fastest.
+ *
+ * Arena layout: RVE r owns elements (= periodic classes) [off[r], off[r+1])
+ * from cia_rve_batch_layout (off has n + 1 entries) and points
+ * [8 off[r], 8 off[r+1]).  State rows are [8][8 off[n]] (component-major over
+ * the whole arena, element type by precision), fluctuations FP64 [3 off[n]]
+ * (class 0 of every RVE holds zeros).
+ *
+ * cia_rve_batch_solve: committed_macro, macro_strain host [n][6], time_step
+ * host [n]; results host [n].  Returns CIA_OK when every RVE converged, else
+ * CIA_ERR_RVE_FAILURE with each RVE's reason in its result (next_state and
+ * next_w rows of failed RVEs are not written).  The homogenised stress,
+ * energies, work terms, Hill-Mandel residual, unloading and active fractions
+ * and the iteration counts are filled; field diagnostics that need the full
+ * node set (periodic, reaction and virtual-work errors, activation margins)
+ * read NaN, and micro fields are not retained.
+ */
+typedef struct cia_rve_batch cia_rve_batch;
+cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t autotune,
+                                cia_rve_batch** out);
+cia_status cia_rve_batch_destroy(cia_rve_batch* batch);
+cia_status cia_rve_batch_layout(const cia_rve_batch* batch, int64_t* element_offsets);
+cia_status cia_rve_batch_solve(cia_rve_batch* batch, void* stream, const void* committed_state,
+                               const double* committed_w, const double* committed_macro,
+                               const double* macro_strain, const double* time_step,
+                               const cia_update_settings* settings, void* next_state,
+                               double* next_w, cia_rve_result* results);
+/* Autotuned grid per batched kernel (0 = one tile per workgroup) and its time
+ * in microseconds; order: assemble, gather, PCG element pass, PCG node pass,
+ * PCG update, symbol solve, DFT x, y, z passes (CIA_BATCH_N_KERNELS entries).
+ * No reference counterpart (the autotuner of the batched path). */
+enum { CIA_BATCH_N_KERNELS = 9 };
+cia_status cia_rve_batch_tuning(const cia_rve_batch* batch, int32_t* grids, double* microseconds,
+                                int32_t n);
+cia_status cia_rve_batch_set_tuning(cia_rve_batch* batch, const int32_t* grids, int32_t n);
+
+/* ------------------------------------------------------------------------ */
 /* Explicit material dynamics on a structured Hex8 box
  * (solve_material_explicit, src/fem/dynamics/material.py:89-297). */
 
