@@ -231,6 +231,136 @@ def surrogate_section(args) -> dict:
     }
 
 
+def batched_models(n_rves: int, seed: int, precision: str, newton_rtol: float):
+    """The batched FE² workload: RVEs with edges drawn uniformly from 16-32
+    elements, each with its own seeded spherical-aggregate microstructure (40 %
+    aggregates, radii U[2, 6] elements) and its own seeded random 6-component
+    strain path (100 steps, |eps| <= 3e-3, rates 10-1000 1/s)."""
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd.microstructure import random_strain_path, spherical_aggregates
+    from concrete_impact_ai_amd.solvers import (ArmijoSettings, LinearSolverSettings,
+                                                NonlinearNewtonSettings)
+
+    rng = np.random.default_rng(seed)
+    edges = rng.integers(16, 33, size=n_rves)
+    mortar = M.J2ViscoplasticMaterial("mortar", 2400.0, 25.0e9, 0.2, 20.0e6, 0.5e9, 1.0e-3, 20.0e6, 1.0)
+    agg = M.LinearElasticMaterial("aggregate", 2400.0, 50.0e9, 0.2)
+    newton = NonlinearNewtonSettings(25, newton_rtol * 20.0e6, newton_rtol, 1.0e-12, newton_rtol,
+                                     ArmijoSettings(True, 1.0e-4, 0.5, 20))
+    lin = LinearSolverSettings(method="spectral", max_iterations=5000, relative_tolerance=1e-10,
+                               check_interval=8, adaptive_forcing=True)
+    models, times, strains = [], [], []
+    for i, n in enumerate(int(v) for v in edges):
+        eph = spherical_aggregates((n, n, n), 0.40, (2.0, 6.0), seed=seed + 1 + i)
+        nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (n, n, n))
+        models.append(R.StructuredHex8RVE(
+            nodes, elements, (n, n, n),
+            M.PhasedMaterialModel("rve", (M.RVEPhase("mortar", mortar), M.RVEPhase("aggregate", agg)),
+                                  np.repeat(eph, 8), 2400.0),
+            R.build_structured_hex8_periodic_constraint(nodes, (n, n, n)), newton,
+            element_phase_ids=eph, phase_names=("mortar", "aggregate"), linear_settings=lin,
+            precision=precision))
+        t, e = random_strain_path(100, 6, 3.0e-3, (10.0, 1000.0), seed=seed + 1000 + i)
+        times.append(t)
+        strains.append(e)
+    return models, np.asarray(times), np.asarray(strains), edges
+
+
+def batched_section(args) -> dict:
+    """Batched FE²: 256 uneven RVEs advanced together (FP32 element work, FP64
+    residuals / dots / multiplier solve, Newton relative tolerance 1e-6) next to the
+    same batch in FP64 (Newton 1e-10) and the plain FP64 single-RVE solver on a
+    sample of the RVEs.  Rates are RVE increments (solves) per second over the
+    timed increments; the FP32 error is the homogenised-stress difference to the
+    FP64 batch on the same increments."""
+    import dataclasses
+
+    import torch
+
+    from concrete_impact_ai_amd import batch as B
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd.materials import MaterialResponseRequirements
+
+    nr, W, K = args.batch_rves, args.batch_warmup, args.batch_steps
+    models32, times, strains, edges = batched_models(nr, 3000 + args.seed, "fp32", 1e-6)
+    newton64 = dataclasses.replace(models32[0].newton_settings,
+                                   residual_absolute_tolerance=1e-10 * 20.0e6,
+                                   residual_relative_tolerance=1e-10,
+                                   increment_relative_tolerance=1e-10)
+    models64 = [dataclasses.replace(m, precision="fp64", newton_settings=newton64) for m in models32]
+    settings = M.MaterialUpdateSettings(50, 1.0e-10, 1.0e-12, 1.0e-10)
+    n_el = int(np.sum(edges.astype(np.int64) ** 3))
+    out = {"rves": nr, "edges_min": int(edges.min()), "edges_max": int(edges.max()),
+           "elements": n_el, "gauss_points": 8 * n_el, "steps_warmup": W, "steps_timed": K}
+    stresses = {}
+    for tag, models in (("fp32", models32), ("fp64", models64)):
+        print(f"batched {tag}: building {nr} RVEs", file=sys.stderr, flush=True)
+        batch = B.RVEBatch(models)
+        st = batch.initial_state()
+        for k in range(1, W + 1):
+            st = B.solve_rve_batch(batch, strains[:, k], times[:, k] - times[:, k - 1], settings, st).state
+            print(f"batched {tag}: warm-up increment {k} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gp = newton = lin = 0
+        sig = []
+        for k in range(W + 1, W + K + 1):
+            r = B.solve_rve_batch(batch, strains[:, k], times[:, k] - times[:, k - 1], settings, st)
+            st = r.state
+            sig.append(r.macro_stress)
+            gp += int(np.sum(r.diagnostics["gauss_point_updates"]))
+            newton += int(np.sum(r.diagnostics["newton_iterations"]))
+            lin += int(np.max(r.diagnostics["linear_iterations"]))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        print(f"batched {tag}: {K} increments in {el:.3f} s", file=sys.stderr, flush=True)
+        stresses[tag] = np.asarray(sig)
+        out[tag] = {"rve_solves_per_s": nr * K / el, "ms_per_batch_increment": 1e3 * el / K,
+                    "gp_updates_per_s": gp / el, "newton_iterations_per_rve_step": newton / (nr * K),
+                    "max_pcg_iterations_per_step": lin / K,
+                    "active_fraction_last": float(np.mean(
+                        r.diagnostics["viscoplastic_active_volume_fraction"]))}
+        if tag == "fp32":
+            out["autotuned_grids"] = batch.tuning()
+        del batch, st
+        torch.cuda.empty_cache()
+    den = np.maximum(np.max(np.abs(stresses["fp64"]), axis=2), 1e-300)
+    err = np.max(np.abs(stresses["fp32"] - stresses["fp64"]), axis=2) / den
+    out["fp32_stress_error_max"] = float(np.max(err))
+    out["fp32_stress_error_median"] = float(np.median(err))
+    # plain FP64 single-RVE solver (one RVE at a time) on a sample of the batch
+    sample = list(range(0, nr, max(1, nr // args.batch_sample)))[: args.batch_sample]
+    t_single = 0.0
+    n_single = 0
+    req = MaterialResponseRequirements(tangent=False)
+    for i in sample:
+        cache = R.build_rve_operator_cache(models64[i])
+        st = R.initialize_rve_state(models64[i], cache)
+        for k in range(1, W + K + 1):
+            if k == W + 1:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            st = R.solve_rve_microequilibrium(
+                models64[i], R.RVERequest(strains[i, k], float(times[i, k] - times[i, k - 1]), settings,
+                                          req), st, cache, retain_micro_fields=False).state
+        torch.cuda.synchronize()
+        t_single += time.perf_counter() - t0
+        n_single += K
+        cache.close()
+    # the sample's mean time per solve, scaled to the batch's mean RVE size
+    sample_el = float(np.mean(edges[sample].astype(np.float64) ** 3))
+    per_solve = t_single / n_single * (n_el / nr) / sample_el
+    out["fp64_single_rve"] = {"rve_solves_per_s": 1.0 / per_solve, "sample_rves": len(sample),
+                              "note": "one RVE at a time (the reference's serial loop) on the GPU; "
+                                      "time per solve scaled by the batch's mean RVE size"}
+    out["speedup_fp32_batched_vs_fp64_single"] = out["fp32"]["rve_solves_per_s"] * per_solve
+    out["speedup_fp32_batched_vs_fp64_batched"] = (out["fp32"]["rve_solves_per_s"] /
+                                                   out["fp64"]["rve_solves_per_s"])
+    return out
+
+
 def macro_explicit_section(args) -> dict:
     """Explicit velocity-Verlet macro slab, plain J2 (the 128 x 128 x 64 Hex8 slab, h =
     10 mm, rho 2400, E 30 GPa, nu 0.2, sigma_y 30 MPa, H 1 GPa, tau 1e-3 s, m 1; 50 MPa
@@ -313,6 +443,12 @@ def main() -> int:
     ap.add_argument("--macro-divisions", type=int, nargs=3, default=(128, 128, 64))
     ap.add_argument("--macro-steps", type=int, default=200)
     ap.add_argument("--surrogate-only", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched FE² section")
+    ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--batch-rves", type=int, default=256)
+    ap.add_argument("--batch-warmup", type=int, default=2)
+    ap.add_argument("--batch-steps", type=int, default=8)
+    ap.add_argument("--batch-sample", type=int, default=8)
     ap.add_argument("--surrogate-rve", type=int, default=12)
     ap.add_argument("--surrogate-steps", type=int, default=48)
     ap.add_argument("--surrogate-paths", type=int, default=384)
@@ -340,6 +476,9 @@ def main() -> int:
         return 0
     if args.macro_only:
         print(json.dumps({"macro_explicit": macro_explicit_section(args)}))
+        return 0
+    if args.batched_only:
+        print(json.dumps({"batched": batched_section(args)}))
         return 0
     n = args.size
     model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol, not args.fixed_forcing)
@@ -476,6 +615,9 @@ def main() -> int:
     if not args.no_macro:
         print("macro explicit slab ...", file=sys.stderr, flush=True)
         line["macro_explicit"] = macro_explicit_section(args)
+    if not args.no_batched:
+        print("batched FE² ...", file=sys.stderr, flush=True)
+        line["batched"] = batched_section(args)
     if not args.no_surrogate:
         line["surrogate"] = surrogate_section(args)
     print(json.dumps(line))
