@@ -5,10 +5,10 @@
 // with direct sub-DFTs: with j = n2 j1 + j2 and k = k1 + n1 k2,
 //     X[k1 + n1 k2] = sum_j2 W_n2^(j2 k2) [ W_n^(j2 k1) sum_j1 x[n2 j1 + j2] W_n1^(j1 k1) ],
 // i.e. (n1 + n2) complex multiply-adds per output instead of n.  A prime n uses
-// n1 = 1 (a direct DFT).  One kernel per axis: x-lines are contiguous (several
-// lines per workgroup); y and z lines are read as tiles of 16 consecutive x
-// positions so every load row is 256 contiguous bytes.  All return at once when
-// the solver's converged flag is set.
+// n1 = 1 (a direct DFT).  k_dft_strided transforms y or z lines read as tiles
+// of 16 consecutive x positions, so every load row is 256 contiguous bytes; it
+// returns at once when the solver's converged flag is set.  The x-lines are
+// transformed inside the fused passes of spectral.hpp and the batched kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -106,38 +106,6 @@ __device__ __forceinline__ void load_twiddles(const DftAxis& ax, double2* w) {
 
 constexpr int kDftTile = 16;
 constexpr int kDftMaxTw = 64 + 16 + 64;
-
-// x pass: each workgroup transforms `lines` consecutive x-lines (contiguous in
-// memory) of the [2][N] array; dynamic LDS 2 * lines * (nx + 1) double2.
-template <bool INV>
-__global__ void __launch_bounds__(256)
-    k_dft_x(int nx, int64_t total_lines, int lines, DftAxis ax, double2* __restrict__ data,
-            const int32_t* __restrict__ done) {
-  if (done && *done) return;
-  extern __shared__ double2 dft_sm[];
-  __shared__ double2 w[kDftMaxTw];
-  const int64_t l0 = (int64_t)blockIdx.x * lines;
-  const int nl = (int)min<int64_t>(lines, total_lines - l0);
-  const int ld = nx + 1;  // padded LDS row: consecutive lines start 4 banks apart
-  double2* A = dft_sm;
-  double2* T = dft_sm + lines * ld;
-  double2* g = data + l0 * nx;
-  const float inv_nx = 1.0f / (float)nx;
-  load_twiddles(ax, w);
-  for (int t = threadIdx.x; t < nl * nx; t += blockDim.x) {
-    const int l = fdiv(t, inv_nx);
-    A[l * ld + (t - l * nx)] = g[t];
-  }
-  __syncthreads();
-  dft_stage<true, INV>(A, T, nl, ld, 1, ax, w);
-  __syncthreads();
-  dft_stage<false, INV>(T, A, nl, ld, 1, ax, w);
-  __syncthreads();
-  for (int t = threadIdx.x; t < nl * nx; t += blockDim.x) {
-    const int l = fdiv(t, inv_nx);
-    g[t] = A[l * ld + (t - l * nx)];
-  }
-}
 
 // Strided pass (y: stride nx, z: stride nx ny) over 16 consecutive x positions:
 // blockIdx.x = x tile, blockIdx.y = the other grid index (z for the y pass, y for

@@ -37,6 +37,7 @@
 #include "dft.hpp"
 #include <hip/hip_complex.h>
 #include "hex8.hpp"
+#include "spectral.hpp"
 #include "symbol.hpp"
 
 namespace cia {
@@ -657,45 +658,63 @@ __global__ void __launch_bounds__(kBlock)
 // partials visible across XCDs).  Block 0 records the scalars for the host.
 
 
-// r = b, x = 0, p_old = 0, packed r; partials (r.r, sum r).
+// Spectral PCG kernels (the passes of spectral.hpp, in their x-line form: a
+// single RVE needs many small workgroups to fill the chip).
+//   k_sp_xfwd<INIT>  one block per group of x-lines: alpha from the p.y partials,
+//                    the vector update (INIT: x = 0, r = b, p_old = 0) and the
+//                    forward x-DFT of r; partials (r.r, sum r)
+//   k_dft_strided    y passes (dft.hpp), three components
+//   k_sp_zcol        convergence test on the update partials, then z-DFT, symbol
+//                    solve, inverse z-DFT per z-column tile; r.z partials
+//   k_sp_xinv        inverse x-DFT per group of x-lines: z (real, interleaved)
+template <bool INIT>
 __global__ void __launch_bounds__(kBlock)
-    k_sp_init(int64_t C, const double* __restrict__ b, double* __restrict__ x,
-              double* __restrict__ r, double* __restrict__ p_old,
-              hipDoubleComplex* __restrict__ uab, double* __restrict__ part_upd,
+    k_sp_xfwd(int nx, int64_t n_lines, DftAxis ax, const double* __restrict__ b,
+              const double* __restrict__ p, const double* __restrict__ y, double* __restrict__ x,
+              double* __restrict__ r, double* __restrict__ p_old, double2* __restrict__ S, int64_t C,
+              const double* __restrict__ part_pq, int P, double* __restrict__ part_upd,
               Scalars* __restrict__ sc) {
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 wx[kDftMaxTw];
   __shared__ double lds[kBlock / 64];
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (c < C) {
-    double rv[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      rv[m] = b[3 * c + m];
-      x[3 * c + m] = 0.0;
-      r[3 * c + m] = rv[m];
-      p_old[3 * c + m] = 0.0;
-      v[0] += rv[m] * rv[m];
-      v[1 + m] = rv[m];
+  double alpha = 0.0;
+  int it = 0;
+  if (!INIT) {
+    if (sc->done) return;
+    double pq[1];
+    reduce_partials<1>(part_pq, P, pq, lds);
+    if (!(pq[0] > 0.0) || !isfinite(pq[0])) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
+      return;
     }
-    uab[c] = make_hipDoubleComplex(rv[0], rv[1]);
-    uab[C + c] = make_hipDoubleComplex(rv[2], 0.0);
+    it = sc->it_cur;
+    alpha = sc->rzr[it & 1] / pq[0];
   }
-  write_partials<4>(v, part_upd, lds);
+  load_twiddles(ax, wx);
+  const int L = xline_group(nx);
+  const int64_t l0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)min<int64_t>(L, n_lines - l0);
+  double acc[4];
+  xlines_update_fwd<INIT>(nx, l0, nl, alpha, 1.0, b, p, y, x, r, p_old, S, C, ax, dft_sm, wx, acc);
+  write_partials<4>(acc, part_upd, lds);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->done = 0;
-    sc->iters = 0;
+    if (INIT) {
+      sc->done = 0;
+      sc->iters = 0;
+    } else {
+      sc->iters = it + 1;
+      sc->alpha = alpha;
+    }
   }
 }
 
-// Convergence test on the partials of the last update, then the symbol solve
-// of the packed spectra (split per mode pair as described above spec_solve) and
-// r.z partials.
 __global__ void __launch_bounds__(kBlock)
-    k_sp_apply(int64_t N, int nx, int ny, int nz, double inv_n,
-               const hipFloatComplex* __restrict__ kinv, hipDoubleComplex* __restrict__ ua,
-               hipDoubleComplex* __restrict__ ub, const double* __restrict__ part_upd, int P,
-               double* __restrict__ part_rz, Scalars* __restrict__ sc, int first, double rtol,
-               double atol) {
+    k_sp_zcol(int nx, int ny, int nz, int tile_w, double inv_n, DftAxis az,
+              const hipFloatComplex* __restrict__ kinv, double2* __restrict__ S, int64_t C,
+              const double* __restrict__ part_upd, int P, double* __restrict__ part_rz,
+              Scalars* __restrict__ sc, int first, double rtol, double atol) {
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 wz[kDftMaxTw];
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
   double u[4];
@@ -715,34 +734,25 @@ __global__ void __launch_bounds__(kBlock)
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = isfinite(u[0]) ? 1 : 2;
     return;
   }
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double rz = 0.0;
-  if (k < N) {
-    const int64_t mk = neg_mode(k, nx, ny, nz);
-    if (k <= mk) {
-      const hipDoubleComplex a = ua[k], am = ua[mk], b = ub[k];
-      double ar[3] = {0.5 * (a.x + am.x) - u[1], 0.5 * (a.y + am.y) - u[2], b.x - u[3]};
-      double ai[3] = {0.5 * (a.y - am.y), -0.5 * (a.x - am.x), b.y};
-      double zr[3], zi[3];
-      spec_solve(kinv + 6 * k, false, ar, ai, zr, zi);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        zr[m] *= inv_n;
-        zi[m] *= inv_n;
-      }
-      const double wk = (k == mk) ? 1.0 : 2.0;
-      rz += wk * (ar[0] * zr[0] + ai[0] * zi[0] + ar[1] * zr[1] + ai[1] * zi[1] +
-                  ar[2] * zr[2] + ai[2] * zi[2]);
-      ua[k] = make_hipDoubleComplex(zr[0] - zi[1], zi[0] + zr[1]);
-      ub[k] = make_hipDoubleComplex(zr[2], zi[2]);
-      if (mk != k) {
-        ua[mk] = make_hipDoubleComplex(zr[0] + zi[1], -zi[0] + zr[1]);
-        ub[mk] = make_hipDoubleComplex(zr[2], -zi[2]);
-      }
-    }
-  }
-  const double v[1] = {rz};
+  load_twiddles(az, wz);
+  const int tiles_x = (nx + tile_w - 1) / tile_w;
+  const int xt = (int)blockIdx.x % tiles_x, ky = (int)blockIdx.x / tiles_x;
+  const double sr[3] = {u[1], u[2], u[3]};
+  const double v[1] = {zcol_solve(nx, ny, nz, tile_w, xt, ky, S, C, kinv, sr, inv_n, az, dft_sm, wz)};
   write_partials<1>(v, part_rz, lds);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_sp_xinv(int nx, int64_t n_lines, DftAxis ax, const double2* __restrict__ S, int64_t C,
+              double* __restrict__ zout, const Scalars* __restrict__ sc) {
+  extern __shared__ double2 dft_sm[];
+  __shared__ double2 wx[kDftMaxTw];
+  if (sc->done) return;
+  load_twiddles(ax, wx);
+  const int L = xline_group(nx);
+  const int64_t l0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)min<int64_t>(L, n_lines - l0);
+  xlines_inv(nx, l0, nl, S, C, zout, ax, dft_sm, wx);
 }
 
 // y = gather(F) (class 0 zeroed); p.y partials.
@@ -777,48 +787,11 @@ __global__ void __launch_bounds__(kBlock)
   write_partials<1>(v, part_pq, lds);
 }
 
-// alpha = r.z / p.y; x += alpha p, r -= alpha y, packed r; partials (r.r, sum r).
-__global__ void __launch_bounds__(kBlock)
-    k_sp_update(int64_t C, const double* __restrict__ p, const double* __restrict__ y,
-                double* __restrict__ x, double* __restrict__ r, hipDoubleComplex* __restrict__ uab,
-                const double* __restrict__ part_pq, int P, double* __restrict__ part_upd,
-                Scalars* __restrict__ sc) {
-  __shared__ double lds[kBlock / 64];
-  if (sc->done) return;
-  double pq[1];
-  reduce_partials<1>(part_pq, P, pq, lds);
-  if (!(pq[0] > 0.0) || !isfinite(pq[0])) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
-    return;
-  }
-  const int it = sc->it_cur;
-  const double alpha = sc->rzr[it & 1] / pq[0];
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (c < C) {
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      x[3 * c + m] += alpha * p[3 * c + m];
-      const double rv = r[3 * c + m] - alpha * y[3 * c + m];
-      r[3 * c + m] = rv;
-      v[0] += rv * rv;
-      v[1 + m] = rv;
-    }
-    uab[c] = make_hipDoubleComplex(v[1], v[2]);
-    uab[C + c] = make_hipDoubleComplex(v[3], 0.0);
-  }
-  write_partials<4>(v, part_upd, lds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->iters = it + 1;
-    sc->alpha = alpha;
-  }
-}
-
 // Element pass: beta from the r.z partials, p = (z - z0) + beta p_old, forces.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_sp_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
-              const T* __restrict__ tan, const hipDoubleComplex* __restrict__ zc,
+              const T* __restrict__ tan, const double* __restrict__ zr,
               const double* __restrict__ p_old, double* __restrict__ p_new, T* __restrict__ F,
               const double* __restrict__ part_rz, int P, Scalars* __restrict__ sc) {
   __shared__ double lds[kBlock / 64];
@@ -842,7 +815,6 @@ __global__ void __launch_bounds__(kBlock)
   const int q = (int)(p & 7);
   const bool valid = e < G.E;
   const int64_t np = G.E * 8;
-  const int64_t C = G.E;
   // Lane q of an element forms p = (z - z0) + beta p_old at corner q only; the 8
   // lanes then read each other's corners from LDS (one load per corner instead of
   // eight redundant ones).
@@ -850,12 +822,9 @@ __global__ void __launch_bounds__(kBlock)
   {
     double own[3] = {0.0, 0.0, 0.0};
     if (valid) {
-      const double z0[3] = {zc[0].x, zc[0].y, zc[C].x};
       const int64_t cq = periodic_corner(e, q, G.nx, G.ny, G.nz);
-      const hipDoubleComplex t01 = zc[cq], t2 = zc[C + cq];
-      own[0] = (t01.x - z0[0]) + beta * p_old[3 * cq + 0];
-      own[1] = (t01.y - z0[1]) + beta * p_old[3 * cq + 1];
-      own[2] = (t2.x - z0[2]) + beta * p_old[3 * cq + 2];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) own[m] = (zr[3 * cq + m] - zr[m]) + beta * p_old[3 * cq + m];
       if (q == 0) {
 #pragma unroll
         for (int m = 0; m < 3; ++m) p_new[3 * e + m] = own[m];
@@ -1317,7 +1286,7 @@ struct cia_rve {
   std::vector<PairGraph> graphs;     // captured PCG iteration pairs
   cia::DftAxis ax{}, ay{}, az{};
   hipFloatComplex* kinv = nullptr;   // [N][6] inverse symbol
-  hipDoubleComplex* uab = nullptr;   // [2][N] packed residual / preconditioned spectra
+  hipDoubleComplex* uab = nullptr;   // [3][N] residual / preconditioned spectra per component
   double ref_bulk = 0.0, ref_shear = 0.0;
 };
 
@@ -1406,81 +1375,65 @@ struct Ops {
                                  double* x_norm2) {
     const int64_t C = R->C;
     const unsigned gn = grid_for(C), ge = grid_for(R->np);
-    const int P = (int)gn;
+    const int nx = R->nx, ny = R->ny, nz = R->nz;
+    const int64_t n_lines = (int64_t)ny * nz;
+    const unsigned g_xl = (unsigned)((n_lines + xline_group(nx) - 1) / xline_group(nx));
+    // z-column tiles: narrower tiles until the pass has enough workgroups
+    int tile_w = kZTile;
+    while (tile_w > 2 && (int64_t)((nx + tile_w - 1) / tile_w) * ny < 512) tile_w /= 2;
+    const unsigned g_zc = (unsigned)(((nx + tile_w - 1) / tile_w) * ny);
     const double inv_n = 1.0 / (double)C;
+    // block partials: p.y (node pass), (r.r, sum r) (x-line pass), r.z (z-column pass)
     double* part_pq = R->partials;
-    double* part_upd = R->partials + P;
-    double* part_rz = R->partials + 5 * P;
-    // 3-D DFT of the packed spectra [2][C] in place, one kernel per axis.
-    const int lines_x = std::max(1, 256 / R->nx);
-    const int64_t nlines_x = 2 * (int64_t)R->ny * R->nz;
-    const size_t lds_x = 2 * (size_t)lines_x * (R->nx + 1) * sizeof(double2);
-    const size_t lds_y = 2 * (size_t)kDftTile * (R->ny + 1) * sizeof(double2);
-    const size_t lds_z = 2 * (size_t)kDftTile * (R->nz + 1) * sizeof(double2);
-    const unsigned tiles = (unsigned)((R->nx + kDftTile - 1) / kDftTile);
-    const dim3 g_x((unsigned)((nlines_x + lines_x - 1) / lines_x));
-    const dim3 g_y(tiles, (unsigned)R->nz, 2), g_z(tiles, (unsigned)R->ny, 2);
+    double* part_upd = R->partials + gn;
+    double* part_rz = part_upd + 4 * g_xl;
+    const size_t lds_xl = xline_lds_bytes(nx), lds_zc = zcol_lds_bytes(nz, tile_w);
+    const size_t lds_y = 2 * (size_t)kDftTile * (ny + 1) * sizeof(double2);
+    const dim3 g_y((unsigned)((nx + kDftTile - 1) / kDftTile), (unsigned)nz, 3);
+    double2* S = (double2*)R->uab;
     const int32_t* dflag = &R->sc->done;
-    auto fft = [&](bool inverse) -> cia_status {
-      double2* d = (double2*)R->uab;
-      if (!inverse) {
-        hipLaunchKernelGGL(k_dft_x<false>, g_x, dim3(256), lds_x, st, R->nx, nlines_x, lines_x,
-                           R->ax, d, dflag);
-        hipLaunchKernelGGL((k_dft_strided<false, false>), g_y, dim3(256), lds_y, st, R->nx, R->ny,
-                           R->nz, C, R->ay, d, dflag);
-        hipLaunchKernelGGL((k_dft_strided<false, true>), g_z, dim3(256), lds_z, st, R->nx, R->ny,
-                           R->nz, C, R->az, d, dflag);
-      } else {
-        hipLaunchKernelGGL((k_dft_strided<true, true>), g_z, dim3(256), lds_z, st, R->nx, R->ny,
-                           R->nz, C, R->az, d, dflag);
-        hipLaunchKernelGGL((k_dft_strided<true, false>), g_y, dim3(256), lds_y, st, R->nx, R->ny,
-                           R->nz, C, R->ay, d, dflag);
-        hipLaunchKernelGGL(k_dft_x<true>, g_x, dim3(256), lds_x, st, R->nx, nlines_x, lines_x,
-                           R->ax, d, dflag);
-      }
-      return CIA_OK;
-    };
-    auto apply = [&](int first) {
-      hipLaunchKernelGGL(k_sp_apply, dim3(gn), dim3(kBlock), 0, st, C, R->nx, R->ny, R->nz, inv_n,
-                         (const hipFloatComplex*)R->kinv, R->uab, R->uab + C,
-                         (const double*)part_upd, P, part_rz, R->sc, first, rtol,
-                         R->lin.absolute_tolerance);
+    auto precond = [&](hipStream_t s, int first) {
+      hipLaunchKernelGGL((k_dft_strided<false, false>), g_y, dim3(256), lds_y, s, nx, ny, nz, C,
+                         R->ay, S, dflag);
+      hipLaunchKernelGGL(k_sp_zcol, dim3(g_zc), dim3(kBlock), lds_zc, s, nx, ny, nz, tile_w, inv_n,
+                         R->az, (const hipFloatComplex*)R->kinv, S, C, (const double*)part_upd,
+                         (int)g_xl, part_rz, R->sc, first, rtol, R->lin.absolute_tolerance);
+      hipLaunchKernelGGL((k_dft_strided<true, false>), g_y, dim3(256), lds_y, s, nx, ny, nz, C,
+                         R->ay, S, dflag);
+      hipLaunchKernelGGL(k_sp_xinv, dim3(g_xl), dim3(kBlock), lds_xl, s, nx, n_lines, R->ax,
+                         (const double2*)S, C, R->pz, (const Scalars*)R->sc);
     };
     prof::begin(prof::PRECOND, st);
-    hipLaunchKernelGGL(k_sp_init, dim3(gn), dim3(kBlock), 0, st, C, b, x, R->pr, R->pp[1], R->uab,
-                       part_upd, R->sc);
-    CIA_TRY(fft(false));
-    apply(1);
-    CIA_TRY(fft(true));
+    hipLaunchKernelGGL(k_sp_xfwd<true>, dim3(g_xl), dim3(kBlock), lds_xl, st, nx, n_lines, R->ax, b,
+                       (const double*)nullptr, (const double*)nullptr, x, R->pr, R->pp[1], S, C,
+                       (const double*)part_pq, (int)gn, part_upd, R->sc);
+    precond(st, 1);
     prof::end(st);
     CIA_HIP(hipGetLastError());
     int done = 0, it = 0;
     // One PCG iteration; the iteration index lives on the device (k_sp_elem reads
-    // the completed count, k_sp_update advances it), so a launch sequence does not
-    // depend on it and two iterations (p_old / p_new swapped) form a graph.
+    // the completed count, k_sp_xfwd advances it), so a launch sequence does
+    // not depend on it and two iterations (p_old / p_new swapped) form a graph.
     auto iteration = [&](hipStream_t s, int pi, int tag) {
       double* p_old = R->pp[1 - pi];
       double* p_new = R->pp[pi];
       prof::begin(prof::PCG_ELEM, s, tag);
       hipLaunchKernelGGL(k_sp_elem<T>, dim3(ge), dim3(kBlock), 0, s, R->G, R->d_tab, R->d_ephase,
-                         (const T*)R->tan[slot], (const hipDoubleComplex*)R->uab, p_old, p_new,
-                         (T*)R->F, (const double*)part_rz, P, R->sc);
+                         (const T*)R->tan[slot], (const double*)R->pz, p_old, p_new,
+                         (T*)R->F, (const double*)part_rz, (int)g_zc, R->sc);
       prof::end(s);
       prof::begin(prof::PCG_NODE, s, tag);
       hipLaunchKernelGGL(k_sp_node<T>, dim3(gn), dim3(kBlock), 0, s, R->G, (const T*)R->F, p_new,
                          R->py, part_pq, R->sc);
       prof::end(s);
       prof::begin(prof::PCG_UPDATE, s, tag);
-      hipLaunchKernelGGL(k_sp_update, dim3(gn), dim3(kBlock), 0, s, C, p_new, R->py, x, R->pr,
-                         R->uab, (const double*)part_pq, P, part_upd, R->sc);
+      hipLaunchKernelGGL(k_sp_xfwd<false>, dim3(g_xl), dim3(kBlock), lds_xl, s, nx, n_lines, R->ax,
+                         (const double*)nullptr, (const double*)p_new, (const double*)R->py, x,
+                         R->pr, (double*)nullptr, S, C, (const double*)part_pq, (int)gn, part_upd,
+                         R->sc);
       prof::end(s);
-      prof::begin(prof::PRECOND, s);
-      hipStream_t keep = st;
-      st = s;
-      fft(false);
-      apply(0);
-      fft(true);
-      st = keep;
+      prof::begin(prof::PRECOND, s, tag);
+      precond(s, 0);
       prof::end(s);
     };
     hipGraphExec_t pair = nullptr;
@@ -1499,7 +1452,10 @@ struct Ops {
         // whole pairs; an odd final iteration below the limit runs on its own
         int pairs = std::min((n + 1) / 2, (R->lin.max_iterations - it) / 2);
         for (int s = 0; s < pairs; ++s, it += 2) CIA_HIP(hipGraphLaunch(pair, st));
-        if (pairs == 0) iteration(st, (it++) & 1, it);
+        if (pairs == 0) {
+          iteration(st, it & 1, it);
+          ++it;
+        }
       } else {
         for (int s = 0; s < n; ++s, ++it) iteration(st, it & 1, it);
       }
@@ -2030,7 +1986,10 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
   ALLOC(R->u_full, 3 * R->N * 8);
   ALLOC(R->f_full, 3 * R->N * 8);
   const int64_t maxb = std::max<int64_t>(grid_for(R->E), grid_for(R->N)) + 64;
-  ALLOC(R->partials, 8 * maxb * 8);
+  // spectral PCG partials: p.y per class block, 4 per x-line group, r.z per z-column tile
+  const int64_t sp_parts = grid_for(R->E) + 4 * ((int64_t)d->ny * d->nz / xline_group(d->nx) + 1) +
+                           (int64_t)d->nx * d->ny + 64;
+  ALLOC(R->partials, std::max(8 * maxb, sp_parts) * 8);
   ALLOC(R->norm_part, 2 * 1024 * sizeof(double));
   ALLOC(R->counter, 64);
   ALLOC(R->sc, sizeof(Scalars));
@@ -2052,7 +2011,7 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
     R->ref_bulk = d->linear.reference_bulk > 0 ? d->linear.reference_bulk : std::sqrt(kmin * kmax);
     R->ref_shear = d->linear.reference_shear > 0 ? d->linear.reference_shear : std::sqrt(gmin * gmax);
     ALLOC(R->kinv, 6 * R->C * sizeof(hipFloatComplex));
-    ALLOC(R->uab, 2 * R->C * sizeof(hipDoubleComplex));
+    ALLOC(R->uab, 3 * R->C * sizeof(hipDoubleComplex));
     std::vector<hipFloatComplex> sym;
     spectral_symbol(R->nx, R->ny, R->nz, R->G.g, R->ref_bulk, R->ref_shear, sym);
     if (hipMemcpy(R->kinv, sym.data(), sym.size() * sizeof(hipFloatComplex),

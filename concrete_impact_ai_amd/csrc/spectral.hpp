@@ -1,0 +1,208 @@
+// Spectral reference-medium preconditioner of the single-RVE PCG, in fused
+// passes.
+//
+// The preconditioner is z = P K0^+ (r - (sum r) delta_0): the exact inverse of
+// the Hex8 stiffness K0 of a homogeneous isotropic medium on the periodic grid
+// (block-circulant, so the 3-D DFT turns it into 3x3 Hermitian blocks per
+// mode), applied to the residual with the pinned-class reaction removed; the
+// shift P (z - z(class 0)) is applied by the consumer.  Each residual component
+// is transformed as its own complex field so that the z-pass needs no mode
+// pairing and fuses with the symbol solve:
+//
+//   xlines_update_fwd  a group of x-lines: the PCG vector update (x += alpha p,
+//                      r -= alpha y; or the initialisation) of the lines'
+//                      classes, then the x-DFT of each component into S[m]
+//   (y pass)           k_dft_strided of dft.hpp over the three components
+//   zcol_solve         tile_w consecutive kx at one ky, all z: forward z-DFT,
+//                      the 3x3 symbol solve per mode, r.z by Parseval, inverse
+//                      z-DFT, in place
+//   (inverse y pass)
+//   xlines_inv         a group of x-lines: inverse x-DFT, real part = z, written
+//                      interleaved per class
+//
+// Five launches per application (with the update) instead of eight: the single
+// RVE's PCG iteration is bound by launch latency, not by bytes.  The 1-D
+// transforms are dft_stage (one Cooley-Tukey level, twiddles in LDS); edges up to
+// 64 elements.
+#pragma once
+
+#include <hip/hip_complex.h>
+#include <hip/hip_runtime.h>
+
+#include "dft.hpp"
+#include "symbol.hpp"
+
+namespace cia {
+
+constexpr int kZTile = 16;  // largest z-column tile (kx columns)
+
+// Dynamic LDS of the z-column pass.
+inline size_t zcol_lds_bytes(int nz, int tile_w = kZTile) {
+  return 4 * (size_t)tile_w * (nz + 1) * sizeof(double2);
+}
+// x-line groups: lines per workgroup and dynamic LDS of the line passes.
+__host__ __device__ inline int xline_group(int nx) { return nx >= 256 ? 1 : 256 / nx; }
+inline size_t xline_lds_bytes(int nx) { return 4 * (size_t)xline_group(nx) * (nx + 1) * sizeof(double2); }
+
+// z-column tile (kx in [tile_w xt, tile_w (xt + 1)), ky) of an RVE's spectra: forward
+// z-DFT, Z = Khat^-1 (R - sum r) / N per mode, inverse z-DFT, in place.  Returns
+// this thread's part of r.z = sum_k Re(conj(R - sum r) Z).  The caller loads the
+// z twiddles into wz first.
+__device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w, int xt, int ky,
+                                             double2* __restrict__ S, int64_t C,
+                                             const hipFloatComplex* __restrict__ kinv,
+                                             const double sr[3], double inv_n, const DftAxis& az,
+                                             double2* sm, const double2* wz) {
+  const int ldz = nz + 1;
+  const int TW = tile_w;
+  const int i0 = xt * TW;
+  const int width = min(TW, nx - i0);
+  const int64_t plane = (int64_t)nx * ny;
+  const int64_t col0 = i0 + (int64_t)nx * ky;
+  double2* A = sm;                   // [3][TW][nz + 1]
+  double2* T = sm + 3 * TW * ldz;    // [TW][nz + 1]
+  const int per = TW * nz;
+  for (int u = threadIdx.x; u < 3 * per; u += blockDim.x) {
+    const int m = u / per, rem = u - m * per;
+    const int z = rem / TW, ii = rem - z * TW;
+    A[(m * TW + ii) * ldz + z] =
+        ii < width ? S[(int64_t)m * C + col0 + ii + plane * z] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    dft_stage<true, false>(A + m * TW * ldz, T, width, ldz, 1, az, wz);
+    __syncthreads();
+    dft_stage<false, false>(T, A + m * TW * ldz, width, ldz, 1, az, wz);
+    __syncthreads();
+  }
+  double rz = 0.0;
+  for (int u = threadIdx.x; u < per; u += blockDim.x) {
+    const int kz = u / TW, ii = u - kz * TW;
+    if (ii >= width) continue;
+    const int64_t k = col0 + ii + plane * kz;
+    double ar[3], ai[3], zr[3], zi[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const double2 v = A[(m * TW + ii) * ldz + kz];
+      ar[m] = v.x - sr[m];
+      ai[m] = v.y;
+    }
+    spec_solve(kinv + 6 * k, false, ar, ai, zr, zi);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      zr[m] *= inv_n;
+      zi[m] *= inv_n;
+      rz += ar[m] * zr[m] + ai[m] * zi[m];
+      A[(m * TW + ii) * ldz + kz] = make_double2(zr[m], zi[m]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    dft_stage<true, true>(A + m * TW * ldz, T, width, ldz, 1, az, wz);
+    __syncthreads();
+    dft_stage<false, true>(T, A + m * TW * ldz, width, ldz, 1, az, wz);
+    __syncthreads();
+  }
+  for (int u = threadIdx.x; u < 3 * per; u += blockDim.x) {
+    const int m = u / per, rem = u - m * per;
+    const int z = rem / TW, ii = rem - z * TW;
+    if (ii < width) S[(int64_t)m * C + col0 + ii + plane * z] = A[(m * TW + ii) * ldz + z];
+  }
+  return rz;
+}
+
+// x-line group [l0, l0 + nl) of an RVE (line l = y + n_y z holds classes
+// l n_x .. l n_x + n_x - 1; nl n_x <= blockDim.x, one class per thread; class
+// vectors interleaved [3 c + m], spectra S = [3][C] complex).  INIT: r = b_scale b,
+// x = 0, p_old = 0; otherwise x += alpha p, r -= alpha y.  Then the x-DFT of each
+// residual component into S[m] (y and z untransformed); acc receives this
+// thread's (r.r, sum r_x, sum r_y, sum r_z).  wx loaded by the caller.
+template <bool INIT>
+__device__ __forceinline__ void xlines_update_fwd(int nx, int64_t l0, int nl, double alpha,
+                                                  double b_scale, const double* __restrict__ b,
+                                                  const double* __restrict__ p,
+                                                  const double* __restrict__ y,
+                                                  double* __restrict__ x, double* __restrict__ r,
+                                                  double* __restrict__ p_old,
+                                                  double2* __restrict__ S, int64_t C,
+                                                  const DftAxis& ax, double2* sm,
+                                                  const double2* wx, double acc[4]) {
+  const int ld = nx + 1, L = xline_group(nx);
+  const int ncl = nl * nx;
+  const int64_t c0 = l0 * nx;
+  const float inv_nx = 1.0f / (float)nx;
+  double2* A = sm;               // [3][L][nx + 1]
+  double2* T = sm + 3 * L * ld;  // [L][nx + 1]
+#pragma unroll
+  for (int s = 0; s < 4; ++s) acc[s] = 0.0;
+  const int t = threadIdx.x;
+  if (t < ncl) {
+    const int64_t c = c0 + t;
+    const int l = fdiv(t, inv_nx), i = t - l * nx;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double v;
+      if (INIT) {
+        v = b_scale * b[3 * c + m];
+        x[3 * c + m] = 0.0;
+        p_old[3 * c + m] = 0.0;
+      } else {
+        x[3 * c + m] += alpha * p[3 * c + m];
+        v = r[3 * c + m] - alpha * y[3 * c + m];
+      }
+      r[3 * c + m] = v;
+      acc[0] += v * v;
+      acc[1 + m] += v;
+      A[(m * L + l) * ld + i] = make_double2(v, 0.0);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    dft_stage<true, false>(A + m * L * ld, T, nl, ld, 1, ax, wx);
+    __syncthreads();
+    dft_stage<false, false>(T, A + m * L * ld, nl, ld, 1, ax, wx);
+    __syncthreads();
+  }
+  for (int u = threadIdx.x; u < 3 * ncl; u += blockDim.x) {
+    const int m = u / ncl, tt = u - m * ncl;
+    const int l = fdiv(tt, inv_nx);
+    S[(int64_t)m * C + c0 + tt] = A[(m * L + l) * ld + (tt - l * nx)];
+  }
+}
+
+// x-line group: inverse x-DFT of each component; z[3 c + m] = real part.
+__device__ __forceinline__ void xlines_inv(int nx, int64_t l0, int nl,
+                                           const double2* __restrict__ S, int64_t C,
+                                           double* __restrict__ zout, const DftAxis& ax,
+                                           double2* sm, const double2* wx) {
+  const int ld = nx + 1, L = xline_group(nx);
+  const int ncl = nl * nx;
+  const int64_t c0 = l0 * nx;
+  const float inv_nx = 1.0f / (float)nx;
+  double2* A = sm;
+  double2* T = sm + 3 * L * ld;
+  for (int u = threadIdx.x; u < 3 * ncl; u += blockDim.x) {
+    const int m = u / ncl, tt = u - m * ncl;
+    const int l = fdiv(tt, inv_nx);
+    A[(m * L + l) * ld + (tt - l * nx)] = S[(int64_t)m * C + c0 + tt];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    dft_stage<true, true>(A + m * L * ld, T, nl, ld, 1, ax, wx);
+    __syncthreads();
+    dft_stage<false, true>(T, A + m * L * ld, nl, ld, 1, ax, wx);
+    __syncthreads();
+  }
+  const int t = threadIdx.x;
+  if (t < ncl) {
+    const int l = fdiv(t, inv_nx), i = t - l * nx;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) zout[3 * (c0 + t) + m] = A[(m * L + l) * ld + i].x;
+  }
+}
+
+}  // namespace cia
