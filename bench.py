@@ -32,7 +32,7 @@ METRIC = "Gauss-pt updates/s & RVE solves/s; surrogate vs plain FE² speedup at 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-KERNEL_NAMES = {"pcg_elem": "k_sp_elem", "pcg_node": "k_sp_node", "pcg_update": "k_sp_update",
+KERNEL_NAMES = {"pcg_elem": "k_sp_elem", "pcg_node": "k_sp_node", "pcg_update": "k_sp_xfwd",
                 "assemble": "k_assemble"}
 
 # HBM bytes per launch of a kernel from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
@@ -40,7 +40,7 @@ KERNEL_NAMES = {"pcg_elem": "k_sp_elem", "pcg_node": "k_sp_node", "pcg_update": 
 # copy kernel of known size moves to what the same counter reads for it.  Keyed by
 # (slot, RVE edge, precision); absent keys report null.  Source summary:
 # profiles/k_sp_elem_rve48_fp64_hbm_traffic.md
-MEASURED_TRAFFIC: dict = {("pcg_elem", 48, "fp64"): 72_623_945}
+MEASURED_TRAFFIC: dict = {("pcg_elem", 48, "fp64"): 71_622_581}
 
 
 def copy_calibration(torch) -> None:
@@ -540,13 +540,15 @@ def main() -> int:
     #  pcg_elem: compact tangent (d, r per point; flow direction n[6] on plastic
     #            points), p_old + z read and p_new written once per class, the
     #            24 element force rows written.
+    #  pcg_update: p, y, x, r read and x, r written once per class, then the
+    #            x-transformed spectra of the three residual components written.
     #  assemble: committed state read + trial state written (J2 points), stress and
     #            compact tangent written, w read once per class, force rows written.
     mortar_gp = float(np.mean(eph == 0)) * n_gp
     alg = {
         "pcg_elem": n_gp * bpe * (2 + 6 * plastic_frac) + n_dof * 8 * 3 + n_el * 24 * bpe,
         "pcg_node": n_el * 24 * bpe + n_dof * 8 * 2,
-        "pcg_update": n_dof * 8 * 6,
+        "pcg_update": n_dof * 8 * 6 + n_el * 48,
         "assemble": mortar_gp * bpe * 16 + n_gp * bpe * (6 + 2 + 6 * plastic_frac) + n_dof * 8
                     + n_el * 24 * bpe,
     }
