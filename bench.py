@@ -147,12 +147,12 @@ def surrogate_section(args) -> dict:
     from concrete_impact_ai_amd import surrogate_data as SD
 
     n, T = args.surrogate_rve, args.surrogate_steps
-    p_train, p_test = args.surrogate_paths, max(8, args.surrogate_paths // 6)
+    p_train, p_test = args.surrogate_paths, max(8, args.surrogate_paths // 5)
     model = SD.two_phase_rve(n, seed=args.seed)
     times, strains = SD.path_batch(p_train + p_test, T, seed=1000 + args.seed)
     print(f"surrogate: FE² data {p_train + p_test} paths x {T} steps on {n}^3 RVEs",
           file=sys.stderr, flush=True)
-    d_eps, rate, stress, fe2_rate = SD.rve_responses(model, times, strains, workers=4)
+    d_eps, rate, stress, fe2_rate = SD.rve_responses_batched(model, times, strains)
     tr = slice(0, p_train)
     te = slice(p_train, p_train + p_test)
     scales = S.SurrogateScales.fit(d_eps[tr], rate[tr], stress[tr])
@@ -162,7 +162,7 @@ def surrogate_section(args) -> dict:
     t0 = time.perf_counter()
     hist = S.train(params, scales, d_eps[tr], rate[tr], stress[tr], iterations=args.surrogate_iters,
                    batch_paths=min(96, p_train), window=T, lr=2e-3, seed=args.seed,
-                   log_every=max(1, args.surrogate_iters // 10))
+                   log_every=max(1, args.surrogate_iters // 10), progress=sys.stderr)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     S.envelope(params, scales, d_eps[tr], rate[tr])
@@ -205,7 +205,7 @@ def surrogate_section(args) -> dict:
     return {
         "rve_edge": n,
         "fe2_rve_solves_per_s": fe2_rate,
-        "fe2_data": f"{p_train + p_test} paths x {T} steps, 4 concurrent streams",
+        "fe2_data": f"{p_train + p_test} paths x {T} steps, one batched FE² solve per step",
         "train_paths": p_train,
         "test_paths": p_test,
         "train_iterations": args.surrogate_iters,
@@ -450,9 +450,9 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--batch-sample", type=int, default=8)
     ap.add_argument("--surrogate-rve", type=int, default=12)
-    ap.add_argument("--surrogate-steps", type=int, default=48)
-    ap.add_argument("--surrogate-paths", type=int, default=384)
-    ap.add_argument("--surrogate-iters", type=int, default=1500)
+    ap.add_argument("--surrogate-steps", type=int, default=64)
+    ap.add_argument("--surrogate-paths", type=int, default=1280)
+    ap.add_argument("--surrogate-iters", type=int, default=4000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "

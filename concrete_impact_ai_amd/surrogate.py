@@ -7,7 +7,13 @@ through time on RVE responses (``nn/training/rve_rno.py:113-415``).  This networ
 the build's own (13 -> 256 -> 256 -> 256 -> 256 -> 12, GELU):
 
     x_t = [d_eps_t / eps_scale (6), log10(rate_t / rate_ref) / rate_scale (1), z_t (6)]
-    o_t = MLP(x_t);   sigma_t = stress_scale * o_t[0:6];   z_{t+1} = z_t + o_t[6:12]
+    o_t = MLP(x_t)
+    sigma_t = sigma_{t-1} + c stress_scale o_t[0:6],   z_{t+1} = z_t + o_t[6:12]
+
+with c = STRESS_STEP and sigma_{-1} = z_0 = 0: the network predicts stress increments, so
+the six internal state variables carry only the inelastic state (the elastic part of
+an increment needs none); predicting the stress itself from the latent needs the
+latent to integrate the total strain and measured 1.6x the held-out error.
 
 Every matrix product runs in the HIP kernels of ``libcia_hip.so`` on BF16 matrix
 cores with FP32 accumulation: layer-wise GEMMs with fused bias/GELU epilogues for
@@ -43,6 +49,7 @@ from concrete_impact_ai_amd.materials import (
 
 SIZES = (13, 256, 256, 256, 256, 12)
 N_LAYERS = len(SIZES) - 1
+STRESS_STEP = 0.1  # scale of the network's stress-increment outputs
 
 
 def _p(t):
@@ -255,6 +262,7 @@ def rollout(params: SurrogateParams, scales: SurrogateScales, d_eps: torch.Tenso
     """d_eps [B][T][6] (physical), rate [B][T] -> stress [B][T][6] (physical), latents."""
     B, T, _ = d_eps.shape
     z = torch.zeros((B, 6), dtype=torch.float32, device=d_eps.device)
+    s = torch.zeros((B, 6), dtype=torch.float32, device=d_eps.device)
     feats_e = (d_eps / scales.eps_scale).float()
     feats_r = scales.rate_feature(rate).float()
     out = []
@@ -262,7 +270,8 @@ def rollout(params: SurrogateParams, scales: SurrogateScales, d_eps: torch.Tenso
     for t in range(T):
         x = torch.cat([feats_e[:, t], feats_r[:, t, None], z], dim=1)
         o = forward_fused(x, params) if fused else mlp(x, params, w_leaf, b_leaf)
-        out.append(o[:, :6])
+        s = s + STRESS_STEP * o[:, :6]
+        out.append(s)
         z = z + o[:, 6:]
         lat.append(z)
     return torch.stack(out, dim=1) * scales.stress_scale, torch.stack(lat, dim=1)
@@ -270,7 +279,7 @@ def rollout(params: SurrogateParams, scales: SurrogateScales, d_eps: torch.Tenso
 
 def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress, iterations: int,
           batch_paths: int, window: int, lr: float = 1e-3, seed: int = 0, lr_final: float = 1e-5,
-          log_every: int = 0):
+          log_every: int = 0, progress=None):
     """BPTT on random windows of the training paths (``train_rve_rno``,
     ``nn/training/rve_rno.py:200-415``).  The loss is the path-normalised stress MSE
     of ``_batch_loss`` (``:540-638``): each path's squared errors are divided by its
@@ -301,6 +310,9 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
         params.adam(w_leaf.grad, b_leaf.grad, cur)
         if log_every and (it % log_every == 0 or it == iterations - 1):
             history.append((it, float(loss.detach())))
+            if progress is not None:
+                print(f"surrogate training: iteration {it} loss {history[-1][1]:.3e}", file=progress,
+                      flush=True)
     return history
 
 
@@ -342,6 +354,7 @@ class SurrogateMaterial:
 
     def initialize_state(self, n_points: int) -> MaterialState:
         return MaterialState({"surrogate_strain": np.zeros((n_points, 6)),
+                              "surrogate_stress": np.zeros((n_points, 6)),
                               "surrogate_latent": np.zeros((n_points, 6)),
                               "surrogate_fallback": np.zeros(n_points)})
 
@@ -364,8 +377,9 @@ class SurrogateMaterial:
 
 
 class _SurrogateState:
-    def __init__(self, strain, latent, fallback_mask, rve_state=None, rve_index=None):
+    def __init__(self, strain, stress, latent, fallback_mask, rve_state=None, rve_index=None):
         self.strain = strain            # [n][6] fp64 committed strain
+        self.stress = stress            # [n][6] fp64 committed stress
         self.latent = latent            # [n][6] fp32
         self.fallback = fallback_mask   # [n] bool
         self.rve_state = rve_state      # adapter batch state of the fallback points
@@ -379,6 +393,7 @@ class _SurrogateKernel:
     def initial_state(self, n):
         dev = device()
         return _SurrogateState(torch.zeros((n, 6), dtype=torch.float64, device=dev),
+                               torch.zeros((n, 6), dtype=torch.float64, device=dev),
                                torch.zeros((n, 6), dtype=torch.float32, device=dev),
                                torch.zeros(n, dtype=torch.bool, device=dev))
 
@@ -386,11 +401,13 @@ class _SurrogateKernel:
         dev = device()
         v = state.variables
         return _SurrogateState(torch.as_tensor(v["surrogate_strain"], device=dev),
+                               torch.as_tensor(v["surrogate_stress"], dtype=torch.float64, device=dev),
                                torch.as_tensor(v["surrogate_latent"], dtype=torch.float32, device=dev),
                                torch.as_tensor(np.asarray(v["surrogate_fallback"]) > 0.5, device=dev))
 
     def material_state(self, st):
         return MaterialState({"surrogate_strain": st.strain.cpu().numpy(),
+                              "surrogate_stress": st.stress.cpu().numpy(),
                               "surrogate_latent": st.latent.double().cpu().numpy(),
                               "surrogate_fallback": st.fallback.double().cpu().numpy()})
 
@@ -408,7 +425,7 @@ class _SurrogateKernel:
         feat_r = sc.rate_feature(rmag).float()
         x = torch.cat([(d_eps / sc.eps_scale).float(), feat_r[:, None], st.latent], dim=1)
         o = forward_fused(x, m.params)
-        sig = o[:, :6].double() * sc.stress_scale
+        sig = st.stress + STRESS_STEP * sc.stress_scale * o[:, :6].double()
         lat = st.latent + o[:, 6:]
         trip = ((d_eps.abs().max(dim=1).values / sc.eps_scale > m.gate_margin * sc.d_eps_max)
                 | (feat_r < sc.rate_lo - 0.5) | (feat_r > sc.rate_hi + 0.5)
@@ -427,7 +444,7 @@ class _SurrogateKernel:
                                    "FE² fallback is configured.")
             sig, rve_state, rve_index = self._fallback(eps, st, fb, sig, dt, settings)
         return DeviceResponse(stress=sig.t().contiguous(),
-                              state=_SurrogateState(eps.clone(), lat, fb, rve_state, rve_index))
+                              state=_SurrogateState(eps.clone(), sig, lat, fb, rve_state, rve_index))
 
     def _fallback(self, eps, st, fb, sig, dt, settings):
         """FE² for the points on the fallback: earlier trips continue from their

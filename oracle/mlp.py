@@ -9,7 +9,10 @@ build's own network restated in NumPy float64: per step
     x_t = [d_eps_t (6), rate_t (1), z_t (6)]          (13 features)
     h   = GELU(W4 GELU(W3 GELU(W2 GELU(W1 x + b1) + b2) + b3) + b4)
     o   = W5 h + b5                                   (12 outputs)
-    sigma_t = o[0:6],   z_{t+1} = z_t + o[6:12],   z_0 = 0
+    sigma_t = sigma_{t-1} + c o[0:6],   z_{t+1} = z_t + o[6:12],   sigma_{-1} = z_0 = 0
+
+(c = STRESS_STEP: the network predicts stress increments, so the latent only has
+to carry the inelastic state, not the total strain)
 
 with GELU(v) = v Phi(v) (erf form) and loss = mean over paths, steps and the six
 components of (sigma_t - target_t)^2.  ``loss_and_grad`` is a hand-written
@@ -26,6 +29,7 @@ import numpy as np
 from scipy.special import erf
 
 SIZES = (13, 256, 256, 256, 256, 12)
+STRESS_STEP = 0.1
 
 
 def init_params(seed: int = 0, sizes=SIZES):
@@ -64,6 +68,7 @@ def rollout(params, d_eps, rate):
     """d_eps (B, T, 6), rate (B, T) -> stresses (B, T, 6), latents (B, T + 1, 6), caches."""
     B, T, _ = d_eps.shape
     z = np.zeros((B, 6))
+    s = np.zeros((B, 6))
     sig = np.zeros((B, T, 6))
     lat = [z]
     caches = []
@@ -71,7 +76,8 @@ def rollout(params, d_eps, rate):
         x = np.concatenate([d_eps[:, t], rate[:, t, None], z], axis=1)
         o, cache = mlp_forward(params, x)
         caches.append(cache)
-        sig[:, t] = o[:, :6]
+        s = s + STRESS_STEP * o[:, :6]
+        sig[:, t] = s
         z = z + o[:, 6:]
         lat.append(z)
     return sig, np.stack(lat, axis=1), caches
@@ -86,9 +92,11 @@ def loss_and_grad(params, d_eps, rate, target):
     loss = float(np.sum(diff * diff) / n)
     grads = [(np.zeros_like(w), np.zeros_like(b)) for w, b in params]
     gz = np.zeros((B, 6))  # dL/dz_{t+1} flowing back
+    gs = np.zeros((B, 6))  # dL/dsigma_t (sigma_{t+1} = sigma_t + c o[0:6] carries it back)
     for t in range(T - 1, -1, -1):
         go = np.zeros((B, 12))
-        go[:, :6] = 2.0 * diff[:, t] / n
+        gs = gs + 2.0 * diff[:, t] / n
+        go[:, :6] = STRESS_STEP * gs
         go[:, 6:] = gz                    # z_{t+1} = z_t + o[6:12]
         cache = caches[t]
         g = go

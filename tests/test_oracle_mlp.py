@@ -53,6 +53,7 @@ def test_bptt_gradient_matches_torch_autograd():
     tp = [(torch.tensor(w, requires_grad=True), torch.tensor(b, requires_grad=True)) for w, b in params]
     de, rt, tg = (torch.tensor(a) for a in (d_eps, rate, target))
     z = torch.zeros(4, 6, dtype=torch.float64)
+    s = torch.zeros(4, 6, dtype=torch.float64)
     tl = 0.0
     for t in range(de.shape[1]):
         a = torch.cat([de[:, t], rt[:, t, None], z], dim=1)
@@ -60,7 +61,8 @@ def test_bptt_gradient_matches_torch_autograd():
             a = a @ w.T + b
             if li < len(tp) - 1:
                 a = torch.nn.functional.gelu(a)
-        tl = tl + ((a[:, :6] - tg[:, t]) ** 2).sum()
+        s = s + 0.1 * a[:, :6]     # stress increments, step 0.1
+        tl = tl + ((s - tg[:, t]) ** 2).sum()
         z = z + a[:, 6:]
     tl = tl / (4 * 6 * 6)
     tl.backward()
@@ -109,3 +111,22 @@ def test_planted_mistake_in_latent_recurrence_is_caught():
     _, bad = broken(params, d_eps, rate, target)
     assert orig is OM.loss_and_grad
     assert np.max(np.abs(bad[0][0] - good[0][0])) > 1e-6 * np.max(np.abs(good[0][0]))
+
+
+def test_planted_mistake_in_stress_recurrence_is_caught():
+    """Treating each step's stress as the network output alone (no accumulation of the
+    increments) is a different model: its loss and gradient differ from the oracle's."""
+    params = OM.init_params(5, SMALL)
+    d_eps, rate, target = _data()
+    loss, good = OM.loss_and_grad(params, d_eps, rate, target)
+    sig, _, _ = OM.rollout(params, d_eps, rate)
+    B, T, _ = d_eps.shape
+    direct = np.zeros_like(sig)
+    z = np.zeros((B, 6))
+    for t in range(T):
+        o, _ = OM.mlp_forward(params, np.concatenate([d_eps[:, t], rate[:, t, None], z], axis=1))
+        direct[:, t] = OM.STRESS_STEP * o[:, :6]
+        z = z + o[:, 6:]
+    np.testing.assert_allclose(direct[:, 0], sig[:, 0], rtol=0, atol=1e-15)  # equal at t = 0
+    assert np.max(np.abs(direct[:, 1:] - sig[:, 1:])) > 1e-6 * np.max(np.abs(sig))
+    assert abs(float(np.mean((direct - target) ** 2)) - loss) > 1e-9 * loss
