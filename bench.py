@@ -55,7 +55,8 @@ def copy_calibration(torch) -> None:
           file=sys.stderr, flush=True)
 
 
-def build_model(n: int, seed: int, precision: str, pcg_rtol: float, forcing: bool = True):
+def build_model(n: int, seed: int, precision: str, pcg_rtol: float, forcing: bool = True,
+                newton_rtol: float = 1.0e-10):
     from concrete_impact_ai_amd import materials as M
     from concrete_impact_ai_amd import rve as R
     from concrete_impact_ai_amd.microstructure import spherical_aggregates
@@ -66,9 +67,10 @@ def build_model(n: int, seed: int, precision: str, pcg_rtol: float, forcing: boo
     mortar = M.J2ViscoplasticMaterial("mortar", 2400.0, 25.0e9, 0.2, 20.0e6, 0.5e9, 1.0e-3, 20.0e6, 1.0)
     agg = M.LinearElasticMaterial("aggregate", 2400.0, 50.0e9, 0.2)
     nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), (n, n, n))
-    # Newton tolerance 1e-10 (relative); absolute tolerances force-scaled for SI units:
-    # 1e-10 * sigma_y * L^2 for the residual, 1e-12 m for the increment.
-    newton = NonlinearNewtonSettings(25, 1.0e-10 * 20.0e6, 1.0e-10, 1.0e-12, 1.0e-10,
+    # Newton tolerance 1e-10 (relative; 1e-6 for FP32 element work, its residual floor);
+    # absolute tolerances force-scaled for SI units: rtol * sigma_y * L^2 for the
+    # residual, 1e-12 m for the increment.
+    newton = NonlinearNewtonSettings(25, newton_rtol * 20.0e6, newton_rtol, 1.0e-12, newton_rtol,
                                      ArmijoSettings(True, 1.0e-4, 0.5, 20))
     lin = LinearSolverSettings(method="spectral", max_iterations=5000, relative_tolerance=pcg_rtol,
                                check_interval=8, adaptive_forcing=forcing)
@@ -303,6 +305,7 @@ def batched_section(args) -> dict:
             st = B.solve_rve_batch(batch, strains[:, k], times[:, k] - times[:, k - 1], settings, st).state
             print(f"batched {tag}: warm-up increment {k} done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
+        cnt0 = batch.stats()
         t0 = time.perf_counter()
         gp = newton = lin = 0
         sig = []
@@ -322,6 +325,8 @@ def batched_section(args) -> dict:
                     "max_pcg_iterations_per_step": lin / K,
                     "active_fraction_last": float(np.mean(
                         r.diagnostics["viscoplastic_active_volume_fraction"]))}
+        cnt1 = batch.stats()
+        out[tag]["counters_timed"] = {k: cnt1[k] - cnt0[k] for k in cnt1}
         if tag == "fp32":
             out["autotuned_grids"] = batch.tuning()
         del batch, st
@@ -435,6 +440,7 @@ def main() -> int:
     ap.add_argument("--fixed-forcing", action="store_true",
                     help="solve every Newton step to --pcg-rtol instead of the inexact-Newton forcing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 side-by-side run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (HBM-counter calibration kernel)")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate section")
@@ -503,10 +509,12 @@ def main() -> int:
     start_state = state  # committed states are never modified by a solve
     newton = backtracks = lin_its = 0
     gp_updates = 0
+    sig64 = []
     t0 = time.perf_counter()
     for k in range(args.warmup + 1, total + 1):
         resp = step(k, state)
         state = resp.state
+        sig64.append(resp.macro_stress)
         d = resp.diagnostics
         newton += d["newton_iterations"]
         backtracks += d["armijo_backtracks"]
@@ -611,6 +619,43 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if args.precision == "fp64" and not args.no_fp32:
+        # the same increments with FP32 element work and FP64 residuals, side by side
+        print("fp32 + refinement side by side ...", file=sys.stderr, flush=True)
+        m32, _ = build_model(n, args.seed, "fp32", args.pcg_rtol, not args.fixed_forcing,
+                             newton_rtol=1.0e-6)
+        c32 = R.build_rve_operator_cache(m32)
+        st32 = R.initialize_rve_state(m32, c32)
+
+        def step32(k, st):
+            return R.solve_rve_microequilibrium(
+                m32, R.RVERequest(strains[k], float(times[k] - times[k - 1]), settings, req_no_tan),
+                st, c32, retain_micro_fields=False)
+
+        for k in range(1, args.warmup + 1):
+            st32 = step32(k, st32).state
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sig32, n32, l32 = [], 0, 0
+        for k in range(args.warmup + 1, total + 1):
+            r32 = step32(k, st32)
+            st32 = r32.state
+            sig32.append(r32.macro_stress)
+            n32 += r32.diagnostics["newton_iterations"]
+            l32 += r32.diagnostics["linear_iterations"]
+        torch.cuda.synchronize()
+        el32 = time.perf_counter() - t1
+        a32, a64 = np.asarray(sig32), np.asarray(sig64)
+        err = np.max(np.abs(a32 - a64), axis=1) / np.max(np.abs(a64), axis=1)
+        line["fp32_refined"] = {
+            "rve_solves_per_s": K / el32, "ms_per_step": 1e3 * el32 / K,
+            "newton_tolerance": 1e-6, "newton_iterations_per_step": n32 / K,
+            "pcg_iterations_per_step": l32 / K,
+            "stress_error_vs_fp64_max": float(np.max(err)),
+            "stress_error_vs_fp64_median": float(np.median(err)),
+            "error_definition": "per increment |sigma_fp32 - sigma_fp64|_inf / |sigma_fp64|_inf",
+        }
+        c32.close()
     if not args.no_cpu_baseline:
         print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n, args.seed, newton / K, backtracks / K)

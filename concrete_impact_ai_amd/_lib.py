@@ -278,6 +278,7 @@ _SIGNATURES = {
     "cia_rve_batch_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                          C.c_int32]),
     "cia_rve_batch_set_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.c_int32]),
+    "cia_rve_batch_stats": (C.c_int32, [_vp, C.POINTER(C.c_int64), C.c_int32]),
     "cia_profile_enable": (C.c_int32, [C.c_int32]),
     "cia_profile_read": (C.c_int32, [C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
 }

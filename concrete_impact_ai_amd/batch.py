@@ -119,6 +119,14 @@ class RVEBatch:
         return {k: {"grid": int(grids[i]), "us": float(us[i])}
                 for i, k in enumerate(_lib.BATCH_KERNELS)}
 
+    def stats(self) -> dict:
+        """Counters since creation: solves, Newton rounds, PCG iterations launched for
+        the whole batch, PCG iterations summed over the RVEs, host synchronisations."""
+        c = (C.c_int64 * 5)()
+        _lib.check(_lib.lib().cia_rve_batch_stats(self.handle, c, 5), "cia_rve_batch_stats")
+        return dict(zip(("solves", "newton_rounds", "pcg_iterations_launched",
+                         "pcg_iterations_rve_sum", "host_syncs"), (int(v) for v in c)))
+
     def set_tuning(self, grids: dict) -> None:
         """Fix the kernel grids (before the first solve) instead of autotuning."""
         n = len(_lib.BATCH_KERNELS)

@@ -1051,6 +1051,9 @@ struct cia_rve_batch {
   hipGraphExec_t pair = nullptr;
   static constexpr int kHints = 32;
   int its_hint[kHints] = {0};
+  // counters since creation: solves, Newton rounds, PCG iterations launched (whole
+  // batch), sum over RVEs of their PCG iterations, host synchronisations
+  int64_t stats[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -1130,6 +1133,7 @@ struct BOps {
 };
 
 cia_status read_counts(cia_rve_batch* B, hipStream_t st, int out[7]) {
+  B->stats[4] += 1;
   hipLaunchKernelGGL(k_b_count, dim3(1), dim3(kBlock), 0, st, B->A, B->cnt);
   CIA_HIP(hipGetLastError());
   CIA_HIP(hipMemcpyAsync(B->h_cnt, B->cnt, 7 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1278,6 +1282,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   for (int round = 0;; ++round) {
     CIA_TRY(read_counts(B, st, cnt));
     if (cnt[S_PCG] == 0) break;
+    B->stats[1] += 1;
     // Newton systems of every RVE in this round, solved together
     BOps<T>::pcg_start(B, st);
     int it = 0;
@@ -1302,6 +1307,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
       if (cnt[5] == 0) break;
     }
     if (round < cia_rve_batch::kHints) B->its_hint[round] = std::max(1, cnt[6]);
+    B->stats[2] += it;
     hipLaunchKernelGGL(k_b_norm_dw, dim3(gcl), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_ctl_pcg, dim3(gw), dim3(kBlock), 0, st, B->A, N);
     CIA_HIP(hipGetLastError());
@@ -1326,6 +1332,8 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   CIA_HIP(hipMemcpyAsync(hr.data(), B->resp, hr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   CIA_HIP(hipStreamSynchronize(st));
   bool all_ok = true;
+  B->stats[0] += 1;
+  for (int r = 0; r < R; ++r) B->stats[3] += hc[r].pcg_its;
   for (int r = 0; r < R; ++r) {
     cia_rve_result& o = res[r];
     memset(&o, 0, sizeof(o));
@@ -1680,6 +1688,12 @@ cia_status cia_rve_batch_tuning(const cia_rve_batch* B, int32_t* grids, double* 
     grids[k] = B->grid[k];
     if (microseconds) microseconds[k] = B->tuned_us[k];
   }
+  return CIA_OK;
+}
+
+cia_status cia_rve_batch_stats(const cia_rve_batch* B, int64_t* counters, int32_t n) {
+  CIA_ARG(B && counters && n >= 0, "NULL argument");
+  for (int i = 0; i < n && i < 5; ++i) counters[i] = B->stats[i];
   return CIA_OK;
 }
 

@@ -392,6 +392,10 @@ enum { CIA_BATCH_N_KERNELS = 9 };
 cia_status cia_rve_batch_tuning(const cia_rve_batch* batch, int32_t* grids, double* microseconds,
                                 int32_t n);
 cia_status cia_rve_batch_set_tuning(cia_rve_batch* batch, const int32_t* grids, int32_t n);
+/* Counters since creation (no reference counterpart; measurement): solves, Newton
+ * rounds, PCG iterations launched for the whole batch, PCG iterations summed over
+ * the RVEs, host synchronisations. */
+cia_status cia_rve_batch_stats(const cia_rve_batch* batch, int64_t* counters, int32_t n);
 
 /* ------------------------------------------------------------------------ */
 /* Explicit material dynamics on a structured Hex8 box
