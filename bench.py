@@ -271,9 +271,10 @@ def batched_models(n_rves: int, seed: int, precision: str, newton_rtol: float):
 
 def batched_section(args) -> dict:
     """Batched FE²: 256 uneven RVEs advanced together (FP32 element work, FP64
-    residuals / dots / multiplier solve, Newton relative tolerance 1e-6) next to the
-    same batch in FP64 (Newton 1e-10) and the plain FP64 single-RVE solver on a
-    sample of the RVEs.  Rates are RVE increments (solves) per second over the
+    residuals / dots / multiplier solve, Newton relative tolerance 1e-6; once from
+    the committed fluctuation and once from the extrapolated one) next to the same
+    batch in FP64 (Newton 1e-10) and the plain FP64 single-RVE solver on a sample
+    of the RVEs.  Rates are RVE increments (solves) per second over the
     timed increments; the FP32 error is the homogenised-stress difference to the
     FP64 batch on the same increments."""
     import dataclasses
@@ -297,12 +298,20 @@ def batched_section(args) -> dict:
     out = {"rves": nr, "edges_min": int(edges.min()), "edges_max": int(edges.max()),
            "elements": n_el, "gauss_points": 8 * n_el, "steps_warmup": W, "steps_timed": K}
     stresses = {}
-    for tag, models in (("fp32", models32), ("fp64", models64)):
+    for tag, models, use_pred in (("fp32", models32, False), ("fp32_predictor", models32, True),
+                                  ("fp64", models64, False)):
         print(f"batched {tag}: building {nr} RVEs", file=sys.stderr, flush=True)
         batch = B.RVEBatch(models)
         st = batch.initial_state()
+        pred = B.FluctuationPredictor(batch) if use_pred else None
+
+        def solve(k, st):
+            w0 = pred.initial_w(st, strains[:, k]) if pred is not None else None
+            return B.solve_rve_batch(batch, strains[:, k], times[:, k] - times[:, k - 1], settings,
+                                     st, initial_w=w0)
+
         for k in range(1, W + 1):
-            st = B.solve_rve_batch(batch, strains[:, k], times[:, k] - times[:, k - 1], settings, st).state
+            st = solve(k, st).state
             print(f"batched {tag}: warm-up increment {k} done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         cnt0 = batch.stats()
@@ -310,7 +319,7 @@ def batched_section(args) -> dict:
         gp = newton = lin = 0
         sig = []
         for k in range(W + 1, W + K + 1):
-            r = B.solve_rve_batch(batch, strains[:, k], times[:, k] - times[:, k - 1], settings, st)
+            r = solve(k, st)
             st = r.state
             sig.append(r.macro_stress)
             gp += int(np.sum(r.diagnostics["gauss_point_updates"]))
@@ -332,9 +341,10 @@ def batched_section(args) -> dict:
         del batch, st
         torch.cuda.empty_cache()
     den = np.maximum(np.max(np.abs(stresses["fp64"]), axis=2), 1e-300)
-    err = np.max(np.abs(stresses["fp32"] - stresses["fp64"]), axis=2) / den
-    out["fp32_stress_error_max"] = float(np.max(err))
-    out["fp32_stress_error_median"] = float(np.median(err))
+    for tag in ("fp32", "fp32_predictor"):
+        err = np.max(np.abs(stresses[tag] - stresses["fp64"]), axis=2) / den
+        out[f"{tag}_stress_error_max"] = float(np.max(err))
+        out[f"{tag}_stress_error_median"] = float(np.median(err))
     # plain FP64 single-RVE solver (one RVE at a time) on a sample of the batch
     sample = list(range(0, nr, max(1, nr // args.batch_sample)))[: args.batch_sample]
     t_single = 0.0
@@ -361,6 +371,8 @@ def batched_section(args) -> dict:
                               "note": "one RVE at a time (the reference's serial loop) on the GPU; "
                                       "time per solve scaled by the batch's mean RVE size"}
     out["speedup_fp32_batched_vs_fp64_single"] = out["fp32"]["rve_solves_per_s"] * per_solve
+    out["speedup_fp32_predictor_batched_vs_fp64_single"] = (out["fp32_predictor"]["rve_solves_per_s"]
+                                                            * per_solve)
     out["speedup_fp32_batched_vs_fp64_batched"] = (out["fp32"]["rve_solves_per_s"] /
                                                    out["fp64"]["rve_solves_per_s"])
     return out

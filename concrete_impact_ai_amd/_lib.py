@@ -272,8 +272,8 @@ _SIGNATURES = {
     "cia_rve_batch_layout": (C.c_int32, [_vp, C.POINTER(C.c_int64)]),
     "cia_rve_batch_solve": (
         C.c_int32,
-        [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
-         C.POINTER(UpdateSettings), _vp, _vp, C.POINTER(RVEResult)],
+        [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+         C.POINTER(C.c_double), C.POINTER(UpdateSettings), _vp, _vp, C.POINTER(RVEResult)],
     ),
     "cia_rve_batch_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                          C.c_int32]),

@@ -197,10 +197,14 @@ class RVEBatchResponse:
 
 def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
                     update_settings: MaterialUpdateSettings, committed: RVEBatchState,
-                    raise_on_failure: bool = True) -> RVEBatchResponse:
+                    raise_on_failure: bool = True,
+                    initial_w: torch.Tensor | None = None) -> RVEBatchResponse:
     """One strain-driven increment of every RVE of the batch.
 
-    ``macro_strains`` (n, 6), ``time_steps`` (n,).  With ``raise_on_failure`` the
+    ``macro_strains`` (n, 6), ``time_steps`` (n,).  ``initial_w`` (device, shaped
+    like ``committed.w``) starts Newton from a predicted fluctuation instead of the
+    committed one (the reference's start); the converged state meets the same
+    tolerances.  With ``raise_on_failure`` the
     lowest-index failed RVE raises the reference's exception
     (``RVEEquilibriumError`` or ``MaterialPointConvergenceError``) with
     ``rve_index`` in its diagnostics; otherwise failures are reported in
@@ -217,6 +221,11 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
         raise ValueError("committed state does not belong to this batch")
     packed = committed.packed.contiguous()
     w = committed.w.contiguous()
+    w0 = None
+    if initial_w is not None:
+        if initial_w.shape != w.shape or initial_w.dtype != torch.float64:
+            raise ValueError("initial_w must be a float64 tensor shaped like the fluctuations")
+        w0 = initial_w.contiguous()
     next_packed = torch.empty_like(packed)
     next_w = torch.empty_like(w)
     res = (_lib.RVEResult * n)()
@@ -225,7 +234,7 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
     dptr = C.POINTER(C.c_double)
     status = _lib.lib().cia_rve_batch_solve(
         batch.handle, stream_handle(), C.c_void_p(packed.data_ptr()), C.c_void_p(w.data_ptr()),
-        cmacro.ctypes.data_as(dptr), macro.ctypes.data_as(dptr), dts.ctypes.data_as(dptr),
+        C.c_void_p(None if w0 is None else w0.data_ptr()), cmacro.ctypes.data_as(dptr), macro.ctypes.data_as(dptr), dts.ctypes.data_as(dptr),
         C.byref(settings), C.c_void_p(next_packed.data_ptr()), C.c_void_p(next_w.data_ptr()), res)
     _lib.check(status, "cia_rve_batch_solve")
     reasons = tuple(_lib.RVE_REASONS[r.reason] for r in res)
@@ -294,12 +303,43 @@ class AcceptedRVEBatchStep:
     response: RVEBatchResponse
 
 
+class FluctuationPredictor:
+    """Newton start for the next increment of every RVE: the committed fluctuation
+    extrapolated along the last increment, w_k + s (w_k - w_{k-1}) with
+    s = clamp((d_eps_k . d_eps_{k-1}) / |d_eps_{k-1}|^2, 0, 2) per RVE (the
+    fluctuation responds to the macro strain to first order, so on a straight
+    path segment this is its linear response)."""
+
+    def __init__(self, batch: RVEBatch):
+        counts = np.diff(batch.element_offsets) * 3
+        self._rep = torch.as_tensor(counts, device=device())
+        self._w_prev = None
+        self._de_prev = None
+
+    def initial_w(self, committed: RVEBatchState, macro_next) -> torch.Tensor | None:
+        de = np.asarray(macro_next, dtype=np.float64) - committed.macro
+        w0 = None
+        if self._w_prev is not None:
+            den = np.einsum("ij,ij->i", self._de_prev, self._de_prev)
+            s = np.where(den > 0.0, np.einsum("ij,ij->i", de, self._de_prev) / np.maximum(den, 1e-300),
+                         0.0)
+            s = np.clip(s, 0.0, 2.0)
+            scale = torch.repeat_interleave(torch.as_tensor(s, device=self._rep.device), self._rep)
+            w0 = committed.w + scale * (committed.w - self._w_prev)
+        self._w_prev = committed.w
+        self._de_prev = de
+        return w0
+
+
 def iterate_prescribed_rve_paths(batch: RVEBatch, times, macro_strains,
                                  update_settings: MaterialUpdateSettings,
-                                 state: RVEBatchState | None = None) -> Iterator[AcceptedRVEBatchStep]:
+                                 state: RVEBatchState | None = None,
+                                 predictor: bool = False) -> Iterator[AcceptedRVEBatchStep]:
     """``iterate_prescribed_rve_path`` (``src/fem/rve/path.py:47-100``) for every
     RVE of the batch at once: ``times`` (n, m + 1) and ``macro_strains``
-    (n, m + 1, 6), one path per RVE, all with m increments."""
+    (n, m + 1, 6), one path per RVE, all with m increments.  ``predictor`` starts
+    each increment's Newton from :class:`FluctuationPredictor` instead of the
+    committed fluctuation."""
     times = np.asarray(times, dtype=np.float64)
     strains = np.asarray(macro_strains, dtype=np.float64)
     n = batch.n_rves
@@ -309,7 +349,10 @@ def iterate_prescribed_rve_paths(batch: RVEBatch, times, macro_strains,
     if times.shape[1] < 2 or np.any(dts <= 0.0):
         raise ValueError("Prescribed RVE path requires at least two increasing times.")
     st = state if state is not None else batch.initial_state()
+    pred = FluctuationPredictor(batch) if predictor else None
     for k in range(1, times.shape[1]):
-        resp = solve_rve_batch(batch, strains[:, k], dts[:, k - 1], update_settings, st)
+        w0 = pred.initial_w(st, strains[:, k]) if pred is not None else None
+        resp = solve_rve_batch(batch, strains[:, k], dts[:, k - 1], update_settings, st,
+                               initial_w=w0)
         st = resp.state
         yield AcceptedRVEBatchStep(k, times[:, k].copy(), dts[:, k - 1].copy(), resp)

@@ -390,7 +390,8 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(Arena A, int want) {
 // ---------------------------------------------------------------------------
 // Class-vector kernels.
 
-// w[0] = committed fluctuation (every RVE).
+// w[0] = the Newton start: the committed fluctuation, or the caller's initial
+// guess w0 (every RVE).
 __global__ void __launch_bounds__(kBlock) k_b_begin(Arena A, const double* __restrict__ wc) {
   CIA_TILES(A.T[F_CL]) {
     const int r = A.t2r[F_CL][t];
@@ -1231,8 +1232,8 @@ cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const v
 
 template <class T>
 cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, const double* wc,
-                       const double* cmacro, const double* macro, const double* dt,
-                       const Settings& S, void* next_state, double* next_w,
+                       const double* w0, const double* cmacro, const double* macro,
+                       const double* dt, const Settings& S, void* next_state, double* next_w,
                        cia_rve_result* res) {
   const int R = B->R;
   for (int r = 0; r < R; ++r) {
@@ -1246,7 +1247,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   }
   CIA_HIP(hipMemcpyAsync(B->req, B->h_req, R * sizeof(Req), hipMemcpyHostToDevice, st));
   const unsigned gcl = kgrid(B, K_UPD, F_CL);
-  hipLaunchKernelGGL(k_b_begin, dim3(gcl), dim3(kBlock), 0, st, B->A, wc);
+  hipLaunchKernelGGL(k_b_begin, dim3(gcl), dim3(kBlock), 0, st, B->A, w0 ? w0 : wc);
   hipLaunchKernelGGL(k_b_ctl_init, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, st, B->A);
   BOps<T>::assemble(B, st, S, S_NEWTON, state_c, nullptr, wc, false);
   BOps<T>::gather(B, st, S_NEWTON);
@@ -1662,23 +1663,25 @@ cia_status cia_rve_batch_layout(const cia_rve_batch* B, int64_t* element_offsets
 }
 
 cia_status cia_rve_batch_solve(cia_rve_batch* B, void* stream, const void* committed_state,
-                               const double* committed_w, const double* committed_macro,
-                               const double* macro_strain, const double* time_step,
-                               const cia_update_settings* settings, void* next_state,
-                               double* next_w, cia_rve_result* results) {
+                               const double* committed_w, const double* initial_w,
+                               const double* committed_macro, const double* macro_strain,
+                               const double* time_step, const cia_update_settings* settings,
+                               void* next_state, double* next_w, cia_rve_result* results) {
   CIA_ARG(B && committed_state && committed_w && committed_macro && macro_strain && time_step &&
               settings && next_state && next_w && results,
           "NULL argument to cia_rve_batch_solve");
   CIA_ARG((const void*)next_state != committed_state, "next_state aliases committed_state");
   CIA_ARG((const void*)next_w != (const void*)committed_w, "next_w aliases committed_w");
+  CIA_ARG(initial_w == nullptr || (const void*)initial_w != (const void*)next_w,
+          "next_w aliases initial_w");
   for (int r = 0; r < B->R; ++r) CIA_ARG(time_step[r] > 0.0, "RVE time step must be positive.");
   const Settings S = make_settings(*settings);
   hipStream_t st = (hipStream_t)stream;
   if (B->precision == CIA_PREC_FP64)
-    return solve_batch<double>(B, st, committed_state, committed_w, committed_macro, macro_strain,
-                               time_step, S, next_state, next_w, results);
-  return solve_batch<float>(B, st, committed_state, committed_w, committed_macro, macro_strain,
-                            time_step, S, next_state, next_w, results);
+    return solve_batch<double>(B, st, committed_state, committed_w, initial_w, committed_macro,
+                               macro_strain, time_step, S, next_state, next_w, results);
+  return solve_batch<float>(B, st, committed_state, committed_w, initial_w, committed_macro,
+                            macro_strain, time_step, S, next_state, next_w, results);
 }
 
 cia_status cia_rve_batch_tuning(const cia_rve_batch* B, int32_t* grids, double* microseconds,

@@ -220,10 +220,35 @@ __device__ __forceinline__ void force_from(const T s[6], T jw, const T bx[8], co
   }
 }
 
+// Value of the lane whose index differs in bit MASK (1, 2 or 4) inside an
+// aligned group of 8 lanes, by DPP moves on the vector ALU (quad permutes for
+// bits 0 and 1, row shifts by 4 for bit 2) instead of LDS-crossbar shuffles.  q
+// is the lane index mod 8.  All lanes of the group must be active.
+template <int MASK>
+__device__ __forceinline__ int lane_xor(int v, int q) {
+  if (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // [1,0,3,2]
+  if (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // [2,3,0,1]
+  const int from_up = __builtin_amdgcn_mov_dpp(v, 0x104, 0xF, 0xF, true);  // lane + 4
+  const int from_dn = __builtin_amdgcn_mov_dpp(v, 0x114, 0xF, 0xF, true);  // lane - 4
+  return (q & 4) ? from_dn : from_up;
+}
+
+template <int MASK>
+__device__ __forceinline__ float lane_xor(float v, int q) {
+  return __int_as_float(lane_xor<MASK>(__float_as_int(v), q));
+}
+
+template <int MASK>
+__device__ __forceinline__ double lane_xor(double v, int q) {
+  const int lo = lane_xor<MASK>(__double2loint(v), q);
+  const int hi = lane_xor<MASK>(__double2hiint(v), q);
+  return __hiloint2double(hi, lo);
+}
+
 // Sum f over the 8 lanes of an element (lanes q = 0..7 of an aligned 8-lane
-// group, one per Gauss point) by recursive halving: 12 + 6 + 3 shuffles, after
-// which lane q holds the element force at its corner q.  Fixed order, so the
-// result is bitwise reproducible.
+// group, one per Gauss point) by recursive halving: 12 + 6 + 3 lane exchanges,
+// after which lane q holds the element force at its corner q.  Fixed order, so
+// the result is bitwise reproducible.
 template <class T>
 __device__ __forceinline__ void element_reduce8(int q, const T f[8][3], T out[3]) {
   T h1[4][3];
@@ -233,7 +258,7 @@ __device__ __forceinline__ void element_reduce8(int q, const T f[8][3], T out[3]
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       const T send = b4 ? f[j][m] : f[4 + j][m];
-      const T recv = __shfl_xor(send, 4, 8);
+      const T recv = lane_xor<4>(send, q);
       h1[j][m] = (b4 ? f[4 + j][m] : f[j][m]) + recv;
     }
   T h2[2][3];
@@ -243,14 +268,14 @@ __device__ __forceinline__ void element_reduce8(int q, const T f[8][3], T out[3]
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       const T send = b2 ? h1[j][m] : h1[2 + j][m];
-      const T recv = __shfl_xor(send, 2, 8);
+      const T recv = lane_xor<2>(send, q);
       h2[j][m] = (b2 ? h1[2 + j][m] : h1[j][m]) + recv;
     }
   const bool b1 = (q & 1) != 0;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const T send = b1 ? h2[0][m] : h2[1][m];
-    const T recv = __shfl_xor(send, 1, 8);
+    const T recv = lane_xor<1>(send, q);
     out[m] = (b1 ? h2[1][m] : h2[0][m]) + recv;
   }
 }

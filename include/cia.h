@@ -366,7 +366,9 @@ cia_status cia_rve_copy_field(cia_rve* rve, void* stream, int32_t field, double*
  * (class 0 of every RVE holds zeros).
  *
  * cia_rve_batch_solve: committed_macro, macro_strain host [n][6], time_step
- * host [n]; results host [n].  Returns CIA_OK when every RVE converged, else
+ * host [n]; results host [n].  initial_w (device, like committed_w, or NULL) is
+ * the Newton start; NULL starts at the committed fluctuation as the reference
+ * does.  Strain rates are always taken relative to committed_w.  Returns CIA_OK when every RVE converged, else
  * CIA_ERR_RVE_FAILURE with each RVE's reason in its result (next_state and
  * next_w rows of failed RVEs are not written).  The homogenised stress,
  * energies, work terms, Hill-Mandel residual, unloading and active fractions
@@ -380,10 +382,10 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
 cia_status cia_rve_batch_destroy(cia_rve_batch* batch);
 cia_status cia_rve_batch_layout(const cia_rve_batch* batch, int64_t* element_offsets);
 cia_status cia_rve_batch_solve(cia_rve_batch* batch, void* stream, const void* committed_state,
-                               const double* committed_w, const double* committed_macro,
-                               const double* macro_strain, const double* time_step,
-                               const cia_update_settings* settings, void* next_state,
-                               double* next_w, cia_rve_result* results);
+                               const double* committed_w, const double* initial_w,
+                               const double* committed_macro, const double* macro_strain,
+                               const double* time_step, const cia_update_settings* settings,
+                               void* next_state, double* next_w, cia_rve_result* results);
 /* Autotuned grid per batched kernel (0 = one tile per workgroup) and its time
  * in microseconds; order: assemble, gather, PCG element pass, PCG node pass,
  * PCG update, symbol solve, DFT x, y, z passes (CIA_BATCH_N_KERNELS entries).

@@ -199,3 +199,25 @@ def test_batch_autotuned_grids_change_no_bit(gpu):
             assert all(v["us"] > 0 for v in tuning.values())
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_batch_predictor_reaches_the_same_states(gpu):
+    """Starting Newton from the extrapolated fluctuation converges to the states of
+    the reference start within the Newton tolerance, with no more Newton iterations."""
+    from concrete_impact_ai_amd import batch as B
+
+    models, _ = _models(SIZES[:3], newton_rtol=1e-12)
+    steps = 6
+    times, strains = _paths(len(models), steps, seed=21)
+    outs = []
+    for pred in (False, True):
+        batch = B.RVEBatch(models)
+        sig, its = [], 0
+        for step in B.iterate_prescribed_rve_paths(batch, times, strains, _settings(), predictor=pred):
+            sig.append(step.response.macro_stress)
+            its += int(np.sum(step.response.diagnostics["newton_iterations"]))
+        outs.append((np.asarray(sig), its, step.response.state.w.cpu().numpy()))
+    (s0, n0, w0), (s1, n1, w1) = outs
+    assert np.max(np.abs(s1 - s0)) <= 1e-9 * np.max(np.abs(s0))
+    assert np.max(np.abs(w1 - w0)) <= 1e-7 * np.max(np.abs(w0))
+    assert n1 <= n0
