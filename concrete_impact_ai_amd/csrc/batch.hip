@@ -127,7 +127,7 @@ struct Arena {
   double* pr;
   double* pp[2];
   double* py;
-  double2* uab;
+  void* uab;     // packed spectra [2 ET] (double2 for FP64 batches, float2 for FP32)
   double* part;  // [P_ROWS][Tmax]
 };
 
@@ -466,7 +466,28 @@ __global__ void __launch_bounds__(kBlock) k_b_copy_out(Arena A, double* __restri
 // Spectral PCG, batched (see the single-RVE solver for the scheme): the scalar
 // steps between the vector kernels run in one wavefront per RVE.
 
+// Complex type of the preconditioner spectra: FP32 batches keep them in FP32
+// (the preconditioner only has to be a fixed SPD operator; the PCG vectors, dot
+// products and Newton residuals stay FP64).
+template <class T>
+struct SpecOf {
+  using type = double2;
+};
+template <>
+struct SpecOf<float> {
+  using type = float2;
+};
+
+template <class SC>
+__device__ __forceinline__ SC mk2(double a, double b) {
+  SC v;
+  v.x = a;
+  v.y = b;
+  return v;
+}
+
 // x = 0, r = -r_newton, p_old = 0, packed r; partials (r.r, sum r).
+template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_sp_init(Arena A) {
   __shared__ double lds[kBlock / 64];
   CIA_TILES(A.T[F_CL]) {
@@ -488,8 +509,9 @@ __global__ void __launch_bounds__(kBlock) k_b_sp_init(Arena A) {
         v[0] += rv * rv;
         v[1 + m] = rv;
       }
-      A.uab[2 * D.e0 + cl] = make_double2(v[1], v[2]);
-      A.uab[2 * D.e0 + D.E + cl] = make_double2(v[3], 0.0);
+      SC* uab = (SC*)A.uab;
+      uab[2 * D.e0 + cl] = mk2<SC>(v[1], v[2]);
+      uab[2 * D.e0 + D.E + cl] = mk2<SC>(v[3], 0.0);
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -568,6 +590,7 @@ __global__ void __launch_bounds__(kBlock) k_b_scal_pq(Arena A) {
 }
 
 // Symbol solve of the packed spectra per mode pair (k, -k); r.z partials.
+template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
   __shared__ double lds[kBlock / 64];
   CIA_TILES(A.T[F_CL]) {
@@ -576,14 +599,14 @@ __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
     const Ctl& c = A.ctl[r];
     if (!pcg_active(A, c)) continue;
     const int64_t k = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
-    double2* ua = A.uab + 2 * D.e0;
-    double2* ub = ua + D.E;
+    SC* ua = (SC*)A.uab + 2 * D.e0;
+    SC* ub = ua + D.E;
     const double s0 = c.p_sumr[0], s1 = c.p_sumr[1], s2 = c.p_sumr[2];
     double rz = 0.0;
     if (k < D.E) {
       const int64_t mk = neg_mode(k, D.nx, D.ny, D.nz);
       if (k <= mk) {
-        const double2 a = ua[k], am = ua[mk], b = ub[k];
+        const SC a = ua[k], am = ua[mk], b = ub[k];
         double ar[3] = {0.5 * (a.x + am.x) - s0, 0.5 * (a.y + am.y) - s1, b.x - s2};
         double ai[3] = {0.5 * (a.y - am.y), -0.5 * (a.x - am.x), b.y};
         double zr[3], zi[3];
@@ -596,11 +619,11 @@ __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
         const double wk = (k == mk) ? 1.0 : 2.0;
         rz += wk * (ar[0] * zr[0] + ai[0] * zi[0] + ar[1] * zr[1] + ai[1] * zi[1] +
                     ar[2] * zr[2] + ai[2] * zi[2]);
-        ua[k] = make_double2(zr[0] - zi[1], zi[0] + zr[1]);
-        ub[k] = make_double2(zr[2], zi[2]);
+        ua[k] = mk2<SC>(zr[0] - zi[1], zi[0] + zr[1]);
+        ub[k] = mk2<SC>(zr[2], zi[2]);
         if (mk != k) {
-          ua[mk] = make_double2(zr[0] + zi[1], -zi[0] + zr[1]);
-          ub[mk] = make_double2(zr[2], -zi[2]);
+          ua[mk] = mk2<SC>(zr[0] + zi[1], -zi[0] + zr[1]);
+          ub[mk] = mk2<SC>(zr[2], -zi[2]);
         }
       }
     }
@@ -613,6 +636,7 @@ __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
 // K p.  Lane q forms corner q; the 8 lanes of an element exchange corners in LDS.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
+  using SC = typename SpecOf<T>::type;
   __shared__ double cpv[kBlock][3];
   const int64_t NPT = A.NPT, ET = A.ET;
   const double* p_old = A.pp[1 - pi];
@@ -624,7 +648,7 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
     if (!pcg_active(A, c)) continue;
     const double beta = c.p_beta;
     const T* tan = (const T*)A.tan[c.cur];
-    const double2* zc = A.uab + 2 * D.e0;
+    const SC* zc = (const SC*)A.uab + 2 * D.e0;
     const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;
     const int64_t e = p >> 3;
     const int q = (int)(p & 7);
@@ -634,7 +658,7 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
       if (valid) {
         const double z0[3] = {zc[0].x, zc[0].y, zc[D.E].x};
         const int64_t cq = periodic_corner(e, q, D.nx, D.ny, D.nz);
-        const double2 t01 = zc[cq], t2 = zc[D.E + cq];
+        const SC t01 = zc[cq], t2 = zc[D.E + cq];
         const int64_t gq = D.e0 + cq;
         own[0] = (t01.x - z0[0]) + beta * p_old[3 * gq + 0];
         own[1] = (t01.y - z0[1]) + beta * p_old[3 * gq + 1];
@@ -724,6 +748,7 @@ __global__ void __launch_bounds__(kBlock) k_b_node(Arena A, int pi) {
 }
 
 // x += alpha p, r -= alpha y, packed r; partials (r.r, sum r).
+template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
   __shared__ double lds[kBlock / 64];
   const double* p = A.pp[pi];
@@ -745,8 +770,9 @@ __global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
         v[0] += rv * rv;
         v[1 + m] = rv;
       }
-      A.uab[2 * D.e0 + cl] = make_double2(v[1], v[2]);
-      A.uab[2 * D.e0 + D.E + cl] = make_double2(v[3], 0.0);
+      SC* uab = (SC*)A.uab;
+      uab[2 * D.e0 + cl] = mk2<SC>(v[1], v[2]);
+      uab[2 * D.e0 + D.E + cl] = mk2<SC>(v[3], 0.0);
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -760,10 +786,11 @@ __global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
 // Batched 3-D DFT passes of the packed spectra (one tile = a group of x-lines,
 // or 16 consecutive x positions of y- or z-lines, of one RVE).
 
-template <bool INV>
+template <bool INV, class SC>
 __global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
-  extern __shared__ double2 dft_sm[];
-  __shared__ double2 wt[kDftMaxTw];
+  extern __shared__ __align__(16) unsigned char dft_raw[];
+  SC* dft_sm = reinterpret_cast<SC*>(dft_raw);
+  __shared__ SC wt[kDftMaxTw];
   CIA_TILES(A.T[F_DX]) {
     const int r = A.t2r[F_DX][t];
     const Desc& D = A.desc[r];
@@ -773,9 +800,9 @@ __global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
     const int64_t l0 = (int64_t)(t - D.t0[F_DX]) * lines;
     const int nl = (int)min<int64_t>(lines, total_lines - l0);
     const int ld = nx + 1;
-    double2* X = dft_sm;
-    double2* Y = dft_sm + lines * ld;
-    double2* g = A.uab + 2 * D.e0 + l0 * nx;
+    SC* X = dft_sm;
+    SC* Y = dft_sm + lines * ld;
+    SC* g = (SC*)A.uab + 2 * D.e0 + l0 * nx;
     const float inv_nx = 1.0f / (float)nx;
     load_twiddles(D.ax, wt);
     for (int u = threadIdx.x; u < nl * nx; u += blockDim.x) {
@@ -795,10 +822,11 @@ __global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
   }
 }
 
-template <bool INV, bool ZPASS>
+template <bool INV, bool ZPASS, class SC>
 __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
-  extern __shared__ double2 dft_sm[];
-  __shared__ double2 wt[kDftMaxTw];
+  extern __shared__ __align__(16) unsigned char dft_raw[];
+  SC* dft_sm = reinterpret_cast<SC*>(dft_raw);
+  __shared__ SC wt[kDftMaxTw];
   constexpr int FAM = ZPASS ? F_DZ : F_DY;
   CIA_TILES(A.T[FAM]) {
     const int r = A.t2r[FAM][t];
@@ -811,17 +839,17 @@ __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
     const int xt = lt % tiles_x, rest = lt / tiles_x;
     const int no = ZPASS ? D.ny : D.nz;
     const int o = rest % no, comp = rest / no;
-    double2* X = dft_sm;
-    double2* Y = dft_sm + kDftTile * ld;
+    SC* X = dft_sm;
+    SC* Y = dft_sm + kDftTile * ld;
     const int i0 = xt * kDftTile;
     const int64_t plane = (int64_t)D.nx * D.ny;
     const int64_t stride = ZPASS ? plane : D.nx;
-    double2* g = A.uab + 2 * D.e0 + (int64_t)comp * D.E + i0 + (ZPASS ? (int64_t)D.nx * o : plane * o);
+    SC* g = (SC*)A.uab + 2 * D.e0 + (int64_t)comp * D.E + i0 + (ZPASS ? (int64_t)D.nx * o : plane * o);
     const int width = min(kDftTile, D.nx - i0);
     load_twiddles(ax, wt);
     for (int u = threadIdx.x; u < kDftTile * n; u += blockDim.x) {
       const int e = u / kDftTile, ii = u - e * kDftTile;
-      X[ii * ld + e] = ii < width ? g[ii + stride * e] : make_double2(0.0, 0.0);
+      X[ii * ld + e] = ii < width ? g[ii + stride * e] : mk2<SC>(0.0, 0.0);
     }
     __syncthreads();
     dft_stage<true, INV>(X, Y, kDftTile, ld, 1, ax, wt);
@@ -1082,6 +1110,7 @@ unsigned kgrid(const cia_rve_batch* B, int k, int fam) {
 
 template <class T>
 struct BOps {
+  using SC = typename SpecOf<T>::type;
   static void assemble(cia_rve_batch* B, hipStream_t st, const Settings& S, int want,
                        const void* state_c, void* next_state, const double* wc, bool extra) {
     const unsigned g = kgrid(B, K_ASM, F_GP);
@@ -1099,13 +1128,13 @@ struct BOps {
   static void dft(cia_rve_batch* B, hipStream_t st, bool inverse) {
     const unsigned gx = kgrid(B, K_DX, F_DX), gy = kgrid(B, K_DY, F_DY), gz = kgrid(B, K_DZ, F_DZ);
     if (!inverse) {
-      hipLaunchKernelGGL(k_b_dft_x<false>, dim3(gx), dim3(256), B->lds_x, st, B->A);
-      hipLaunchKernelGGL((k_b_dft_strided<false, false>), dim3(gy), dim3(256), B->lds_y, st, B->A);
-      hipLaunchKernelGGL((k_b_dft_strided<false, true>), dim3(gz), dim3(256), B->lds_z, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_x<false, SC>), dim3(gx), dim3(256), B->lds_x, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<false, false, SC>), dim3(gy), dim3(256), B->lds_y, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<false, true, SC>), dim3(gz), dim3(256), B->lds_z, st, B->A);
     } else {
-      hipLaunchKernelGGL((k_b_dft_strided<true, true>), dim3(gz), dim3(256), B->lds_z, st, B->A);
-      hipLaunchKernelGGL((k_b_dft_strided<true, false>), dim3(gy), dim3(256), B->lds_y, st, B->A);
-      hipLaunchKernelGGL(k_b_dft_x<true>, dim3(gx), dim3(256), B->lds_x, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<true, true, SC>), dim3(gz), dim3(256), B->lds_z, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_strided<true, false, SC>), dim3(gy), dim3(256), B->lds_y, st, B->A);
+      hipLaunchKernelGGL((k_b_dft_x<true, SC>), dim3(gx), dim3(256), B->lds_x, st, B->A);
     }
   }
   static unsigned waves(cia_rve_batch* B) { return (unsigned)((B->R + 3) / 4); }
@@ -1115,19 +1144,19 @@ struct BOps {
     hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_GP)), dim3(kBlock), 0, st, B->A, pi);
     hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, st, B->A, pi);
     hipLaunchKernelGGL(k_b_scal_pq, dim3(gw), dim3(kBlock), 0, st, B->A);
-    hipLaunchKernelGGL(k_b_update, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A, pi);
+    hipLaunchKernelGGL(k_b_update<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A, pi);
     hipLaunchKernelGGL(k_b_scal_upd<false>, dim3(gw), dim3(kBlock), 0, st, B->A);
     dft(B, st, false);
-    hipLaunchKernelGGL(k_b_apply, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_apply<SC>, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_rz, dim3(gw), dim3(kBlock), 0, st, B->A);
     dft(B, st, true);
   }
   static void pcg_start(cia_rve_batch* B, hipStream_t st) {
     const unsigned gw = waves(B);
-    hipLaunchKernelGGL(k_b_sp_init, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_sp_init<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_upd<true>, dim3(gw), dim3(kBlock), 0, st, B->A);
     dft(B, st, false);
-    hipLaunchKernelGGL(k_b_apply, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_apply<SC>, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_rz, dim3(gw), dim3(kBlock), 0, st, B->A);
     dft(B, st, true);
   }
@@ -1189,6 +1218,7 @@ cia_status tune_one(cia_rve_batch* B, hipStream_t st, int k, int fam, L&& launch
 template <class T>
 cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const void* state_c,
                     const double* wc) {
+  using SC = typename SpecOf<T>::type;
   B->A.force = 1;
   cia_status s = CIA_OK;
   auto run = [&](cia_status x) {
@@ -1205,20 +1235,20 @@ cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const v
     hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, q, B->A, 0);
   }));
   run(tune_one(B, st, K_UPD, F_CL, [&](hipStream_t q) {
-    hipLaunchKernelGGL(k_b_update, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, q, B->A, 0);
+    hipLaunchKernelGGL(k_b_update<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, q, B->A, 0);
   }));
   run(tune_one(B, st, K_APPLY, F_CL, [&](hipStream_t q) {
-    hipLaunchKernelGGL(k_b_apply, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, q, B->A);
+    hipLaunchKernelGGL(k_b_apply<SC>, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, q, B->A);
   }));
   run(tune_one(B, st, K_DX, F_DX, [&](hipStream_t q) {
-    hipLaunchKernelGGL(k_b_dft_x<false>, dim3(kgrid(B, K_DX, F_DX)), dim3(256), B->lds_x, q, B->A);
+    hipLaunchKernelGGL((k_b_dft_x<false, SC>), dim3(kgrid(B, K_DX, F_DX)), dim3(256), B->lds_x, q, B->A);
   }));
   run(tune_one(B, st, K_DY, F_DY, [&](hipStream_t q) {
-    hipLaunchKernelGGL((k_b_dft_strided<false, false>), dim3(kgrid(B, K_DY, F_DY)), dim3(256),
+    hipLaunchKernelGGL((k_b_dft_strided<false, false, SC>), dim3(kgrid(B, K_DY, F_DY)), dim3(256),
                        B->lds_y, q, B->A);
   }));
   run(tune_one(B, st, K_DZ, F_DZ, [&](hipStream_t q) {
-    hipLaunchKernelGGL((k_b_dft_strided<false, true>), dim3(kgrid(B, K_DZ, F_DZ)), dim3(256),
+    hipLaunchKernelGGL((k_b_dft_strided<false, true, SC>), dim3(kgrid(B, K_DZ, F_DZ)), dim3(256),
                        B->lds_z, q, B->A);
   }));
   B->A.force = 0;
@@ -1473,9 +1503,11 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
       D.nt[f] = nt[f];
       tcount[f] += nt[f];
     }
-    lx = std::max(lx, 2 * (size_t)D.lines_x * (d.nx + 1) * sizeof(double2));
-    ly = std::max(ly, 2 * (size_t)kDftTile * (d.ny + 1) * sizeof(double2));
-    lz = std::max(lz, 2 * (size_t)kDftTile * (d.nz + 1) * sizeof(double2));
+    // spectra in FP64 for FP64 batches, in FP32 for FP32 batches
+    const size_t csz = d0.precision == CIA_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+    lx = std::max(lx, 2 * (size_t)D.lines_x * (d.nx + 1) * csz);
+    ly = std::max(ly, 2 * (size_t)kDftTile * (d.ny + 1) * csz);
+    lz = std::max(lz, 2 * (size_t)kDftTile * (d.nz + 1) * csz);
     e0 += D.E;
   }
   B->ET = e0;
@@ -1591,11 +1623,18 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
     }
     hipFloatComplex* kinv = nullptr;
     double2* tw = nullptr;
+    float2* twf = nullptr;
+    std::vector<float2> twf_all(tw_all.size());
+    for (size_t i = 0; i < tw_all.size(); ++i)
+      twf_all[i] = make_float2((float)tw_all[i].x, (float)tw_all[i].y);
     BALLOC(kinv, sym_all.size() * sizeof(hipFloatComplex));
     BALLOC(tw, tw_all.size() * sizeof(double2));
+    BALLOC(twf, twf_all.size() * sizeof(float2));
     if (hipMemcpy(kinv, sym_all.data(), sym_all.size() * sizeof(hipFloatComplex),
                   hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(tw, tw_all.data(), tw_all.size() * sizeof(double2), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(twf, twf_all.data(), twf_all.size() * sizeof(float2), hipMemcpyHostToDevice) !=
             hipSuccess) {
       cia_rve_batch_destroy(B);
       set_error_msg("hipMemcpy of the batch spectral tables failed");
@@ -1609,6 +1648,7 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
         a.n = nn;
         dft_factor(nn, a.n1, a.n2);
         a.w = tw + off;
+        a.wf = twf + off;
         return a;
       };
       D.ax = axis(D.nx, tw_off[3 * r + 0]);
@@ -1634,7 +1674,7 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
   BALLOC(A.dw, 3 * ET * 8);
   BALLOC(A.pr, 3 * ET * 8);
   BALLOC(A.py, 3 * ET * 8);
-  BALLOC(A.uab, 2 * ET * sizeof(double2));
+  BALLOC(A.uab, 2 * ET * (d0.precision == CIA_PREC_FP64 ? sizeof(double2) : sizeof(float2)));
   BALLOC(A.part, (size_t)P_ROWS * B->Tmax * 8);
   BALLOC(B->ctl, n * sizeof(Ctl));
   BALLOC(B->req, n * sizeof(Req));

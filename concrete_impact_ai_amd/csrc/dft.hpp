@@ -20,7 +20,8 @@ namespace cia {
 
 struct DftAxis {
   int n, n1, n2, pad;
-  const double2* w;  // W_n1[n1] | W_n2[n2] | W_n[n], forward sign
+  const double2* w;   // W_n1[n1] | W_n2[n2] | W_n[n], forward sign
+  const float2* wf;   // the same in FP32 (FP32 spectra), or null
 };
 
 inline void dft_factor(int n, int& n1, int& n2) {
@@ -49,6 +50,11 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b, bool conj_b) {
   return make_double2(a.x * b.x - a.y * bi, a.x * bi + a.y * b.x);
 }
 
+__device__ __forceinline__ float2 cmul(float2 a, float2 b, bool conj_b) {
+  const float bi = conj_b ? -b.y : b.y;
+  return make_float2(a.x * b.x - a.y * bi, a.x * bi + a.y * b.x);
+}
+
 // q = floor(t / n) for t < 2^16, n <= 64 without an integer divide: the float
 // quotient's fractional part stays >= 1/(2n) away from an integer, far above its
 // rounding error.
@@ -59,25 +65,27 @@ __device__ __forceinline__ int fdiv(int t, float inv_n) { return (int)(((float)t
 // STEP_B: out[(k1 + n1 k2)] = sum_j2 in[j2 n1 + k1] W_n2^(j2 k2)
 // (j1 k1 mod n1 and j2 k2 mod n2 advance incrementally; j2 k1 < n needs no mod.)
 // w: twiddles of the axis in LDS.
-template <bool STEP_A, bool INV>
-__device__ __forceinline__ void dft_stage(const double2* __restrict__ in, double2* __restrict__ out,
+template <bool STEP_A, bool INV, class C2>
+__device__ __forceinline__ void dft_stage(const C2* __restrict__ in, C2* __restrict__ out,
                                           int n_lines, int ls, int es, const DftAxis& ax,
-                                          const double2* __restrict__ w) {
+                                          const C2* __restrict__ w) {
   const int n = ax.n, n1 = ax.n1, n2 = ax.n2;
-  const double2* w1 = w;
-  const double2* w2 = w + n1;
-  const double2* wn = w + n1 + n2;
+  const C2* w1 = w;
+  const C2* w2 = w + n1;
+  const C2* wn = w + n1 + n2;
   const float inv_n = 1.0f / (float)n, inv_n1 = 1.0f / (float)n1;
   const int total = n_lines * n;
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
     const int line = fdiv(t, inv_n), r = t - line * n;
-    const double2* src = in + line * ls;
-    double2 acc = make_double2(0.0, 0.0);
+    const C2* src = in + line * ls;
+    C2 acc;
+    acc.x = 0;
+    acc.y = 0;
     if (STEP_A) {
       const int j2 = fdiv(r, inv_n1), k1 = r - j2 * n1;
       int m = 0;
       for (int j1 = 0; j1 < n1; ++j1) {
-        const double2 p = cmul(src[(n2 * j1 + j2) * es], w1[m], INV);
+        const C2 p = cmul(src[(n2 * j1 + j2) * es], w1[m], INV);
         acc.x += p.x;
         acc.y += p.y;
         m += k1;
@@ -88,7 +96,7 @@ __device__ __forceinline__ void dft_stage(const double2* __restrict__ in, double
       const int k2 = fdiv(r, inv_n1), k1 = r - k2 * n1;
       int m = 0;
       for (int j2 = 0; j2 < n2; ++j2) {
-        const double2 p = cmul(src[(j2 * n1 + k1) * es], w2[m], INV);
+        const C2 p = cmul(src[(j2 * n1 + k1) * es], w2[m], INV);
         acc.x += p.x;
         acc.y += p.y;
         m += k2;
@@ -102,6 +110,11 @@ __device__ __forceinline__ void dft_stage(const double2* __restrict__ in, double
 __device__ __forceinline__ void load_twiddles(const DftAxis& ax, double2* w) {
   const int nt = ax.n1 + ax.n2 + ax.n;
   for (int t = threadIdx.x; t < nt; t += blockDim.x) w[t] = ax.w[t];
+}
+
+__device__ __forceinline__ void load_twiddles(const DftAxis& ax, float2* w) {
+  const int nt = ax.n1 + ax.n2 + ax.n;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) w[t] = ax.wf[t];
 }
 
 constexpr int kDftTile = 16;
