@@ -219,28 +219,13 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
         raise ValueError("RVE time step must be positive.")
     if committed.packed.dtype != batch.dtype or committed.packed.shape != (8, batch.n_points):
         raise ValueError("committed state does not belong to this batch")
-    packed = committed.packed.contiguous()
-    w = committed.w.contiguous()
-    w0 = None
-    if initial_w is not None:
-        if initial_w.shape != w.shape or initial_w.dtype != torch.float64:
-            raise ValueError("initial_w must be a float64 tensor shaped like the fluctuations")
-        w0 = initial_w.contiguous()
-    next_packed = torch.empty_like(packed)
-    next_w = torch.empty_like(w)
-    res = (_lib.RVEResult * n)()
-    cmacro = np.ascontiguousarray(committed.macro, dtype=np.float64)
-    settings = cia_settings(update_settings)
-    dptr = C.POINTER(C.c_double)
-    status = _lib.lib().cia_rve_batch_solve(
-        batch.handle, stream_handle(), C.c_void_p(packed.data_ptr()), C.c_void_p(w.data_ptr()),
-        C.c_void_p(None if w0 is None else w0.data_ptr()), cmacro.ctypes.data_as(dptr), macro.ctypes.data_as(dptr), dts.ctypes.data_as(dptr),
-        C.byref(settings), C.c_void_p(next_packed.data_ptr()), C.c_void_p(next_w.data_ptr()), res)
-    _lib.check(status, "cia_rve_batch_solve")
+    status, res, next_packed, next_w = solve_batch_raw(
+        batch, committed.packed, committed.w, committed.macro, macro, dts, update_settings,
+        initial_w)
     reasons = tuple(_lib.RVE_REASONS[r.reason] for r in res)
     if status == _lib.CIA_ERR_RVE_FAILURE and raise_on_failure:
         i = next(k for k, r in enumerate(reasons) if r != "ok")
-        _raise_failure(batch, i, res[i], macro[i], float(dts[i]), update_settings)
+        raise_failure(batch, i, res[i], macro[i], float(dts[i]), update_settings)
     diag = {
         "micro_residual_norm": [r.micro_residual_norm for r in res],
         "micro_residual_tolerance": [r.micro_residual_tolerance for r in res],
@@ -269,7 +254,37 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
     )
 
 
-def _raise_failure(batch: RVEBatch, i: int, res, macro, dt, update_settings):
+def solve_batch_raw(batch: RVEBatch, packed: torch.Tensor, w: torch.Tensor, committed_macro,
+                    macro, time_steps, update_settings, initial_w: torch.Tensor | None = None):
+    """One increment on raw device arrays (state rows [8][n_points], fluctuations
+    [3 n_elements]); returns the status, the per-RVE ``cia_rve_result`` array and
+    the next state rows and fluctuations."""
+    packed = packed.contiguous()
+    w = w.contiguous()
+    w0 = None
+    if initial_w is not None:
+        if initial_w.shape != w.shape or initial_w.dtype != torch.float64:
+            raise ValueError("initial_w must be a float64 tensor shaped like the fluctuations")
+        w0 = initial_w.contiguous()
+    n = batch.n_rves
+    macro = np.ascontiguousarray(macro, dtype=np.float64)
+    dts = np.ascontiguousarray(np.broadcast_to(np.asarray(time_steps, dtype=np.float64), (n,)))
+    next_packed = torch.empty_like(packed)
+    next_w = torch.empty_like(w)
+    res = (_lib.RVEResult * n)()
+    cmacro = np.ascontiguousarray(committed_macro, dtype=np.float64)
+    settings = cia_settings(update_settings)
+    dptr = C.POINTER(C.c_double)
+    status = _lib.lib().cia_rve_batch_solve(
+        batch.handle, stream_handle(), C.c_void_p(packed.data_ptr()), C.c_void_p(w.data_ptr()),
+        C.c_void_p(None if w0 is None else w0.data_ptr()), cmacro.ctypes.data_as(dptr),
+        macro.ctypes.data_as(dptr), dts.ctypes.data_as(dptr), C.byref(settings),
+        C.c_void_p(next_packed.data_ptr()), C.c_void_p(next_w.data_ptr()), res)
+    _lib.check(status, "cia_rve_batch_solve")
+    return status, res, next_packed, next_w
+
+
+def raise_failure(batch: RVEBatch, i: int, res, macro, dt, update_settings):
     if res.reason == 5:  # material point failure
         fail = res.point_failure
         p = int(fail.point_id)

@@ -60,7 +60,10 @@ enum {
   P_RZ = 8,    // r.z
   P_RESP = 9,  // response sums (B_NRESP rows)
   B_NRESP = 13,
-  P_ROWS = P_RESP + B_NRESP
+  P_PHASE = P_RESP + B_NRESP,  // per-phase sums (B_NPHASE_ROWS rows)
+  B_PHASES = 4,                // phases with per-phase sums
+  B_NPHASE_ROWS = 9 * B_PHASES,
+  P_ROWS = P_PHASE + B_NPHASE_ROWS
 };
 
 // Response sums per RVE (final pass): stress (6), free energy, dissipation,
@@ -68,6 +71,7 @@ enum {
 // counts.
 enum { BR_SIG = 0, BR_PSI = 6, BR_DISS = 7, BR_MICRO = 8, BR_POS = 9, BR_NEG = 10, BR_UNLOAD = 11,
        BR_ACTIVE = 12 };
+// Per-phase sums, 9 per phase: points, stress (6), free energy, dissipation.
 
 struct Desc {
   int nx, ny, nz, lines_x;
@@ -121,6 +125,7 @@ struct Arena {
   void* sig[2];
   void* tan[2];
   void* F;  // [24][ET]
+  void* pd;  // [2][NPT] free energy and dissipation of the final pass
   double* w[2];
   double* r[2];
   double* dw;
@@ -263,6 +268,8 @@ __global__ void __launch_bounds__(kBlock)
         }
         acc[BR_PSI] = (double)ex.psi;
         acc[BR_DISS] = (double)ex.diss;
+        ((T*)A.pd)[gp] = ex.psi;
+        ((T*)A.pd)[NPT + gp] = ex.diss;
         acc[BR_MICRO] = mw;
         const double pw = (double)ex.d[CIA_D_INCREMENTAL_WORK_DENSITY];
         acc[BR_POS] = fmax(pw, 0.0);
@@ -1006,13 +1013,70 @@ __global__ void __launch_bounds__(kBlock) k_b_ctl_ls(Arena A, NewtonP N) {
   c.alpha = pow(ns.armijo_reduction_factor, (double)c.bt);
 }
 
-// Response of the converged RVEs: out[r][B_NRESP] (raw sums over points).
+// Per-phase sums of the converged RVEs (element per lane; an element has one
+// phase): block partials of B_NPHASE_ROWS rows per class tile.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_b_phase_sums(Arena A) {
+  __shared__ double red[kBlock / 64][B_NPHASE_ROWS];
+  const int64_t NPT = A.NPT;
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (c.state != S_DONE) continue;
+    const int64_t e = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double acc[B_NPHASE_ROWS];
+#pragma unroll
+    for (int s = 0; s < B_NPHASE_ROWS; ++s) acc[s] = 0.0;
+    if (e < D.E) {
+      const T* sig = (const T*)A.sig[c.cur];
+      const T* pd = (const T*)A.pd;
+      double v[9] = {8.0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = 0; q < 8; ++q) {
+        const int64_t gp = 8 * (D.e0 + e) + q;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v[1 + m] += (double)sig[m * NPT + gp];
+        v[7] += (double)pd[gp];
+        v[8] += (double)pd[NPT + gp];
+      }
+      const int ph = A.ephase[D.e0 + e];
+#pragma unroll
+      for (int P = 0; P < B_PHASES; ++P) {
+        if (ph == P) {
+#pragma unroll
+          for (int m = 0; m < 9; ++m) acc[9 * P + m] = v[m];
+        }
+      }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int s = 0; s < B_NPHASE_ROWS; ++s) {
+      const double w = wave_sum(acc[s]);
+      if (lane == 0) red[wid][s] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < B_NPHASE_ROWS) {
+      double w = 0.0;
+#pragma unroll
+      for (int u = 0; u < kBlock / 64; ++u) w += red[u][threadIdx.x];
+      A.part[(P_PHASE + threadIdx.x) * A.Tmax + t] = w;
+    }
+    __syncthreads();
+  }
+}
+
+// Response of the converged RVEs: out[r][B_NRESP + B_NPHASE_ROWS] (raw sums).
 __global__ void __launch_bounds__(kBlock) k_b_resp(Arena A, double* __restrict__ out) {
   CIA_RVE_WAVE();
   if (c.state != S_DONE) return;
+  constexpr int NR = B_NRESP + B_NPHASE_ROWS;
   for (int s = 0; s < B_NRESP; ++s) {
     const double v = rve_sum(A, P_RESP + s, D.t0[F_GP], D.nt[F_GP]);
-    if (lane == 0) out[r * B_NRESP + s] = v;
+    if (lane == 0) out[r * NR + s] = v;
+  }
+  for (int s = 0; s < B_NPHASE_ROWS; ++s) {
+    const double v = rve_sum(A, P_PHASE + s, D.t0[F_CL], D.nt[F_CL]);
+    if (lane == 0) out[r * NR + B_NRESP + s] = v;
   }
 }
 
@@ -1354,11 +1418,13 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   }
   // converged RVEs: final pass (energies, next state, response sums), next w
   BOps<T>::assemble(B, st, S, S_DONE, state_c, next_state, wc, true);
+  hipLaunchKernelGGL(k_b_phase_sums<T>, dim3(gcl), dim3(kBlock), 0, st, B->A);
   hipLaunchKernelGGL(k_b_resp, dim3(gw), dim3(kBlock), 0, st, B->A, B->resp);
   hipLaunchKernelGGL(k_b_copy_out, dim3(gcl), dim3(kBlock), 0, st, B->A, next_w);
   CIA_HIP(hipGetLastError());
   std::vector<Ctl> hc(R);
-  std::vector<double> hr((size_t)R * B_NRESP);
+  constexpr int NR = B_NRESP + B_NPHASE_ROWS;
+  std::vector<double> hr((size_t)R * NR);
   CIA_HIP(hipMemcpyAsync(hc.data(), B->ctl, R * sizeof(Ctl), hipMemcpyDeviceToHost, st));
   CIA_HIP(hipMemcpyAsync(hr.data(), B->resp, hr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   CIA_HIP(hipStreamSynchronize(st));
@@ -1397,7 +1463,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
       continue;
     }
     o.reason = CIA_RVE_OK;
-    const double* s = &hr[(size_t)r * B_NRESP];
+    const double* s = &hr[(size_t)r * NR];
     const double jw = D.g.jw, V = D.volume;
     for (int m = 0; m < 6; ++m) o.macro_stress[m] = s[BR_SIG + m] * jw / V;
     o.free_energy_density = s[BR_PSI] * jw / V;
@@ -1419,6 +1485,15 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
     o.periodic_fluctuation_error = NAN;
     o.reaction_antiperiodicity_error = NAN;
     o.periodic_class_reaction_error = NAN;
+    for (int p = 0; p < 8; ++p) {
+      const double* ps = s + B_NRESP + 9 * p;
+      const double pv = p < B_PHASES ? ps[0] * jw : NAN;
+      o.phase_volume[p] = p < B_PHASES ? pv : NAN;
+      for (int m = 0; m < 6; ++m)
+        o.phase_average_stress[p][m] = p < B_PHASES && pv > 0 ? ps[1 + m] * jw / pv : NAN;
+      o.phase_free_energy_density[p] = p < B_PHASES && pv > 0 ? ps[7] * jw / pv : NAN;
+      o.phase_dissipation_density[p] = p < B_PHASES && pv > 0 ? ps[8] * jw / pv : NAN;
+    }
   }
   return all_ok ? CIA_OK : CIA_ERR_RVE_FAILURE;
 }
@@ -1671,6 +1746,7 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
     BALLOC(A.pp[k], 3 * ET * 8);
   }
   BALLOC(A.F, 24 * ET * ts);
+  BALLOC(A.pd, 2 * NPT * ts);
   BALLOC(A.dw, 3 * ET * 8);
   BALLOC(A.pr, 3 * ET * 8);
   BALLOC(A.py, 3 * ET * 8);
@@ -1678,7 +1754,7 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
   BALLOC(A.part, (size_t)P_ROWS * B->Tmax * 8);
   BALLOC(B->ctl, n * sizeof(Ctl));
   BALLOC(B->req, n * sizeof(Req));
-  BALLOC(B->resp, (size_t)n * B_NRESP * 8);
+  BALLOC(B->resp, (size_t)n * (B_NRESP + B_NPHASE_ROWS) * 8);
   BALLOC(B->cnt, 64);
   BALLOC(B->rec, sizeof(cia_point_failure));
 #undef BALLOC

@@ -51,16 +51,19 @@ VOIGT_COMPONENTS = ("xx", "yy", "zz", "yz", "xz", "xy")
 
 @dataclass(frozen=True)
 class RVEExecutionSettings:
-    """Serial, process-pool or device-stream execution of the per-point RVE solves
-    (``src/fem/rve/data.py:116-138``; ``streams`` is this build's backend and
-    ``process_pool`` is executed by it)."""
+    """Serial, process-pool, device-stream or batched execution of the per-point RVE
+    solves (``src/fem/rve/data.py:116-138``; ``streams`` and ``batched`` are this
+    build's backends and ``process_pool`` is executed by ``streams``).  ``batched``
+    advances every macro point's RVE in one batched device solve
+    (``concrete_impact_ai_amd.batch``); updates that request the effective tangent
+    (which the batched response does not carry) run as per-point solves."""
 
     backend: str = "serial"
     workers: int = 1
     threads_per_worker: int = 1
 
     def __post_init__(self) -> None:
-        if self.backend not in ("serial", "process_pool", "streams"):
+        if self.backend not in ("serial", "process_pool", "streams", "batched"):
             raise ValueError(f"Unknown RVE execution backend {self.backend!r}.")
         if self.workers <= 0:
             raise ValueError("RVE execution workers must be positive.")
@@ -150,7 +153,9 @@ class RVEHomogenizedMaterial:
         self.name = name
         self.execution_settings = execution_settings or RVEExecutionSettings()
         es = self.execution_settings
-        self._workers = 1 if es.backend == "serial" else es.workers
+        self._batched = es.backend == "batched"
+        self._batches = {}  # number of macro points -> batched device operator
+        self._workers = 1 if es.backend in ("serial", "batched") else es.workers
         self.operator_cache = RVEOperatorCache(model)
         self._caches = [self.operator_cache] + [RVEOperatorCache(model)
                                                 for _ in range(self._workers - 1)]
@@ -205,9 +210,32 @@ class RVEHomogenizedMaterial:
                              torch.as_tensor(w, device=dev),
                              np.asarray(state.variables["rve_macro_strain"]), c)
 
+    def _solve_batched(self, strains: np.ndarray, time_step, settings, committed: RVEBatchState):
+        """All macro points' RVEs in one batched solve: the per-point state layout
+        [n][8][n_micro] is permuted to the batch arena's [8][n n_micro] and back."""
+        from concrete_impact_ai_amd import batch as B
+
+        n = strains.shape[0]
+        c = self.operator_cache
+        bt = self._batches.get(n)
+        if bt is None:
+            bt = self._batches[n] = B.RVEBatch([self.model] * n)
+        packed = committed.packed.permute(1, 0, 2).reshape(8, n * c.n_points)
+        steps = np.broadcast_to(np.asarray(time_step, dtype=np.float64), (n,))
+        status, res, nxt_p, nxt_w = B.solve_batch_raw(bt, packed, committed.w.reshape(-1),
+                                                      committed.macro, strains, steps, settings)
+        if status == _lib.CIA_ERR_RVE_FAILURE:
+            i = next(k for k, r in enumerate(res) if r.reason != 0)
+            B.raise_failure(bt, i, res[i], strains[i], float(steps[i]), settings)
+        nxt_packed = nxt_p.reshape(8, n, c.n_points).permute(1, 0, 2).contiguous()
+        return list(res), RVEBatchState(nxt_packed, nxt_w.reshape(n, 3 * c.n_classes),
+                                        strains.copy(), c)
+
     def _solve_all(self, strains: np.ndarray, time_step: float, settings, need_tangent: bool,
                    committed: RVEBatchState):
         n = strains.shape[0]
+        if self._batched and not need_tangent and n > 0:
+            return self._solve_batched(strains, time_step, settings, committed)
         nxt_packed = torch.empty_like(committed.packed)
         nxt_w = torch.empty_like(committed.w)
         results: list = [None] * n
@@ -318,6 +346,9 @@ class RVEHomogenizedMaterial:
         return _RVEKernel(self)
 
     def close(self) -> None:
+        for bt in self._batches.values():
+            bt.close()
+        self._batches = {}
         if self._executor is not None:
             self._executor.shutdown(wait=True, cancel_futures=False)
             self._executor = None

@@ -371,10 +371,11 @@ cia_status cia_rve_copy_field(cia_rve* rve, void* stream, int32_t field, double*
  * does.  Strain rates are always taken relative to committed_w.  Returns CIA_OK when every RVE converged, else
  * CIA_ERR_RVE_FAILURE with each RVE's reason in its result (next_state and
  * next_w rows of failed RVEs are not written).  The homogenised stress,
- * energies, work terms, Hill-Mandel residual, unloading and active fractions
- * and the iteration counts are filled; field diagnostics that need the full
- * node set (periodic, reaction and virtual-work errors, activation margins)
- * read NaN, and micro fields are not retained.
+ * energies, work terms, Hill-Mandel residual, unloading and active fractions,
+ * the iteration counts and the phase diagnostics of phases 0-3 are filled;
+ * field diagnostics that need the full node set (periodic, reaction and
+ * virtual-work errors, activation margins) and phases 4-7 read NaN, and micro
+ * fields are not retained.
  */
 typedef struct cia_rve_batch cia_rve_batch;
 cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t autotune,

@@ -8,6 +8,7 @@ from concrete_impact_ai_amd.fe2 import RVEExecutionSettings
 def test_execution_settings_validation():
     RVEExecutionSettings()
     RVEExecutionSettings("streams", 4)
+    RVEExecutionSettings("batched", 1)
     with pytest.raises(ValueError, match="workers must be positive"):
         RVEExecutionSettings("streams", 0)
     with pytest.raises(ValueError, match="threads_per_worker"):
