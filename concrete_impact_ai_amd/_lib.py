@@ -297,6 +297,13 @@ class BoxDesc(C.Structure):
     ]
 
 
+class SurrogateGate(C.Structure):
+    _fields_ = [(name, C.c_double) for name in (
+        "eps_scale", "rate_ref", "rate_scale", "inelastic_step", "d_eps_max", "rate_lo",
+        "rate_hi", "latent_max", "gate_margin", "rate_margin", "commit_strain",
+        "commit_time")] + [("stiffness", C.c_double * 36)]
+
+
 N_HISTORY = 10
 HISTORY_COLUMNS = ["kinetic", "free_energy", "dissipation_rate", "ext_du", "positive_work",
                    "negative_work", "unloading_volume", "active_volume", "margin_min",
@@ -340,6 +347,9 @@ _SIGNATURES.update({
                     C.c_float, C.c_int32, C.c_float]),
     "cia_f32_to_bf16": (C.c_int32, [_vp, C.c_int64, _vp, _vp]),
     "cia_mlp_forward_fused": (C.c_int32, [_vp, C.c_int64, _vp, _vp, _vp, _vp]),
+    "cia_surrogate_update": (
+        C.c_int32, [_vp, C.c_int64, _vp, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                    _vp, _vp, _vp, _vp, _vp, _vp]),
 })
 
 BATCH_KERNELS = ["assemble", "gather", "pcg_elem", "pcg_node", "pcg_update", "symbol_solve",

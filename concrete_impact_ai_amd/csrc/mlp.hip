@@ -189,28 +189,14 @@ constexpr int FH = 256;          // hidden width
 constexpr int FIN = 13, FOUT = 12;
 constexpr int FLD = FH + 8;      // LDS row stride (bf16), 528 B
 
-__global__ void __launch_bounds__(256) k_mlp_fused(int64_t M, const float* __restrict__ X,
-                                                   const __bf16* __restrict__ Wb,
-                                                   const float* __restrict__ bias,
-                                                   float* __restrict__ Y) {
-  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [2][FM][FLD]
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+// Hidden layers 1..4 on one FM-point tile: a0 holds the 13 input features per row
+// (columns 13..31 zero); the last hidden activations end in a0, a1 is free after.
+__device__ __forceinline__ void fused_hidden(__bf16* a0, __bf16* a1, const __bf16* __restrict__ Wb,
+                                             const float* __restrict__ bias, int wave, int lane) {
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  const int64_t m0 = (int64_t)blockIdx.x * FM;
-  __bf16* a0 = act;
-  __bf16* a1 = act + FM * FLD;
-  // inputs -> bf16, columns 13..31 zero
-  for (int i = tid; i < FM * 32; i += 256) {
-    const int r = i >> 5, c = i & 31;
-    const int64_t m = m0 + r;
-    a0[r * FLD + c] = (__bf16)((m < M && c < FIN) ? X[m * FIN + c] : 0.0f);
-  }
-  __syncthreads();
   const __bf16* W1 = Wb;
   const __bf16* Wh = Wb + FH * FIN;               // 3 x [256][256]
-  const __bf16* W5 = Wh + 3 * FH * FH;            // [12][256]
   const int wn = wave * 64;
-  // ---- layer 1 (K = 13, one k-step of 32) and hidden layers 2..4
   for (int layer = 0; layer < 4; ++layer) {
     const __bf16* src = (layer & 1) ? a1 : a0;
     __bf16* dst = (layer & 1) ? a0 : a1;
@@ -258,33 +244,208 @@ __global__ void __launch_bounds__(256) k_mlp_fused(int64_t M, const float* __res
     }
     __syncthreads();
   }
-  // ---- output layer: wave w takes rows 32 w .. 32 w + 31 (two 16-row subtiles)
-  {
-    const __bf16* src = a0;  // after 4 layers the result is back in a0
-    f32x4 acc[2] = {(f32x4){0.f, 0.f, 0.f, 0.f}, (f32x4){0.f, 0.f, 0.f, 0.f}};
-    for (int ks = 0; ks < FH / 32; ++ks) {
-      bf16x8 bfr;
-      if (lr < FOUT)
-        bfr = *(const bf16x8*)&W5[lr * FH + 32 * ks + lk];
-      else
+}
+
+// Output layer on the tile left in a0 by fused_hidden: wave w takes rows
+// 32 w .. 32 w + 31 (two 16-row subtiles); out(row, column, value) receives the
+// FOUT outputs of every row.
+template <class Out>
+__device__ __forceinline__ void fused_output(const __bf16* a0, const __bf16* __restrict__ Wb,
+                                             const float* __restrict__ bias, int wave, int lane,
+                                             Out out) {
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const __bf16* W5 = Wb + FH * FIN + 3 * FH * FH;  // [12][256]
+  f32x4 acc[2] = {(f32x4){0.f, 0.f, 0.f, 0.f}, (f32x4){0.f, 0.f, 0.f, 0.f}};
+  for (int ks = 0; ks < FH / 32; ++ks) {
+    bf16x8 bfr;
+    if (lr < FOUT)
+      bfr = *(const bf16x8*)&W5[lr * FH + 32 * ks + lk];
+    else
 #pragma unroll
-        for (int t = 0; t < 8; ++t) bfr[t] = (__bf16)0.0f;
+      for (int t = 0; t < 8; ++t) bfr[t] = (__bf16)0.0f;
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const bf16x8 af = *(const bf16x8*)&src[(32 * wave + 16 * i + lr) * FLD + 32 * ks + lk];
-        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc[i], 0, 0, 0);
+    for (int i = 0; i < 2; ++i) {
+      const bf16x8 af = *(const bf16x8*)&a0[(32 * wave + 16 * i + lr) * FLD + 32 * ks + lk];
+      acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc[i], 0, 0, 0);
+    }
+  }
+  if (lr < FOUT) {
+    const float bn = bias[4 * FH + lr];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out(32 * wave + 16 * i + 4 * (lane >> 4) + r, lr, acc[i][r] + bn);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_fused(int64_t M, const float* __restrict__ X,
+                                                   const __bf16* __restrict__ Wb,
+                                                   const float* __restrict__ bias,
+                                                   float* __restrict__ Y) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [2][FM][FLD]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t m0 = (int64_t)blockIdx.x * FM;
+  __bf16* a0 = act;
+  __bf16* a1 = act + FM * FLD;
+  // inputs -> bf16, columns 13..31 zero
+  for (int i = tid; i < FM * 32; i += 256) {
+    const int r = i >> 5, c = i & 31;
+    const int64_t m = m0 + r;
+    a0[r * FLD + c] = (__bf16)((m < M && c < FIN) ? X[m * FIN + c] : 0.0f);
+  }
+  __syncthreads();
+  fused_hidden(a0, a1, Wb, bias, wave, lane);
+  fused_output(a0, Wb, bias, wave, lane, [&](int row, int col, float v) {
+    const int64_t m = m0 + row;
+    if (m < M) Y[m * FOUT + col] = v;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Surrogate material-point update (the material protocol of the reference's
+// RVERNOMaterialAdapter.update, src/concrete_impact/nn/deployment.py:97-165, for
+// this build's network).  The network predicts inelastic strain increments; the
+// stress is the homogenised elastic stiffness C times the elastic strain.  Each
+// point keeps an anchor: the strain, inelastic strain, latent and elapsed time of
+// its last committed network step.  Every call evaluates the network once from the
+// anchor,
+//   d = eps - eps_a,  tau = age + dt,  rate = |d|_2 / tau,
+//   x = [d / eps_scale, clamp_lo(log10(rate / rate_ref) / rate_scale), z_a],
+//   ein' = ein_a + inelastic_step eps_scale o[0:6],  sigma = C (eps - ein'),
+//   z' = z_a + o[6:12],
+// and commits (anchor <- eps, ein', z', age 0) only when |d|_inf reaches
+// commit_strain (or tau reaches commit_time > 0); otherwise the anchor is kept and
+// its age advanced.  The network therefore always sees increments of the size it
+// was trained on, whatever the caller's time step.  Rates below the training
+// range are clamped to its lower end (the quasi-static limit of the viscoplastic
+// response); a point trips the gate when |d|_inf / eps_scale exceeds gate_margin
+// d_eps_max, the rate feature exceeds rate_hi + rate_margin, |z'|_inf exceeds
+// gate_margin latent_max, or a result is not finite.
+
+struct SurGate {
+  double eps_scale, rate_ref, rate_scale, inelastic_step;
+  double d_eps_max, rate_lo, rate_hi, latent_max, gate_margin, rate_margin;
+  double commit_strain, commit_time;
+  double C[36];  // row-major, engineering Voigt
+};
+
+struct SurPoint {
+  double d[6];
+  double dinf, tau, feat_raw, feat;
+};
+
+__device__ __forceinline__ SurPoint sur_inputs(int64_t n, int64_t m, const double* __restrict__ strain,
+                                               double dt, const SurGate& g,
+                                               const double* __restrict__ a_strain,
+                                               const float* __restrict__ a_age) {
+  SurPoint p;
+  double n2 = 0.0;
+  p.dinf = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    p.d[c] = strain[c * n + m] - a_strain[c * n + m];
+    n2 += p.d[c] * p.d[c];
+    p.dinf = fmax(p.dinf, fabs(p.d[c]));
+  }
+  p.tau = (double)a_age[m] + dt;
+  const double rate = sqrt(n2) / p.tau;
+  p.feat_raw = log10(fmax(rate, 1e-30) / g.rate_ref) / g.rate_scale;
+  p.feat = fmax(p.feat_raw, g.rate_lo);
+  return p;
+}
+
+__global__ void __launch_bounds__(256)
+    k_sur_update(int64_t n, const double* __restrict__ strain, double dt,
+                 const __bf16* __restrict__ Wb, const float* __restrict__ bias, SurGate g,
+                 const double* __restrict__ a_strain, const double* __restrict__ a_inel,
+                 const float* __restrict__ a_lat, const float* __restrict__ a_age,
+                 const uint8_t* __restrict__ fb_in, double* __restrict__ stress,
+                 double* __restrict__ n_strain, double* __restrict__ n_inel,
+                 float* __restrict__ n_lat, float* __restrict__ n_age, uint8_t* __restrict__ fb_out,
+                 unsigned int* __restrict__ trips) {
+  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [2][FM][FLD]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t m0 = (int64_t)blockIdx.x * FM;
+  __bf16* a0 = act;
+  __bf16* a1 = act + FM * FLD;
+  // features: one thread per point of the tile
+  if (tid < FM) {
+    const int64_t m = m0 + tid;
+    __bf16* row = a0 + tid * FLD;
+    if (m < n) {
+      const SurPoint p = sur_inputs(n, m, strain, dt, g, a_strain, a_age);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) row[c] = (__bf16)(float)(p.d[c] / g.eps_scale);
+      row[6] = (__bf16)(float)p.feat;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) row[7 + c] = (__bf16)a_lat[c * n + m];
+    } else {
+#pragma unroll
+      for (int c = 0; c < FIN; ++c) row[c] = (__bf16)0.0f;
+    }
+#pragma unroll
+    for (int c = FIN; c < 32; ++c) row[c] = (__bf16)0.0f;
+  }
+  __syncthreads();
+  fused_hidden(a0, a1, Wb, bias, wave, lane);
+  float* o_s = (float*)a1;  // [FM][FOUT + 1], a1 is free after the hidden layers
+  fused_output(a0, Wb, bias, wave, lane,
+               [&](int row, int col, float v) { o_s[row * (FOUT + 1) + col] = v; });
+  __syncthreads();
+  if (tid < FM) {  // waves 0 and 1, whole
+    const int64_t m = m0 + tid;
+    bool new_trip = false;
+    if (m < n) {
+      const SurPoint p = sur_inputs(n, m, strain, dt, g, a_strain, a_age);
+      const float* o = o_s + tid * (FOUT + 1);
+      double ein[6], eel[6], sig[6];
+      float z[6];
+      bool finite = true;
+      double zinf = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        ein[c] = a_inel[c * n + m] + g.inelastic_step * g.eps_scale * (double)o[c];
+        eel[c] = strain[c * n + m] - ein[c];
+        z[c] = a_lat[c * n + m] + o[6 + c];
       }
-    }
-    if (lr < FOUT) {
-      const float bn = bias[4 * FH + lr];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int r = 0; r < 6; ++r) {
+        double v = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t m = m0 + 32 * wave + 16 * i + 4 * (lane >> 4) + r;
-          if (m < M) Y[m * FOUT + lr] = acc[i][r] + bn;
+        for (int c = 0; c < 6; ++c) v += g.C[6 * r + c] * eel[c];
+        sig[r] = v;
+        finite = finite && isfinite(v) && isfinite(z[r]);
+        zinf = fmax(zinf, fabs((double)z[r]));
+      }
+      const bool trip = p.dinf / g.eps_scale > g.gate_margin * g.d_eps_max ||
+                        p.feat_raw > g.rate_hi + g.rate_margin ||
+                        zinf > g.gate_margin * g.latent_max || !finite;
+      const bool commit = p.dinf >= g.commit_strain || (g.commit_time > 0.0 && p.tau >= g.commit_time);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) stress[c * n + m] = sig[c];
+      if (commit) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          n_strain[c * n + m] = strain[c * n + m];
+          n_inel[c * n + m] = ein[c];
+          n_lat[c * n + m] = z[c];
         }
+        n_age[m] = 0.0f;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          n_strain[c * n + m] = a_strain[c * n + m];
+          n_inel[c * n + m] = a_inel[c * n + m];
+          n_lat[c * n + m] = a_lat[c * n + m];
+        }
+        n_age[m] = (float)p.tau;
+      }
+      const uint8_t was = fb_in[m];
+      fb_out[m] = (uint8_t)(was | (trip ? 1 : 0));
+      new_trip = trip && !was;
     }
+    const unsigned long long mask = __ballot(new_trip);
+    if (lane == 0 && mask) atomicAdd(trips, (unsigned int)__popcll(mask));
   }
 }
 
@@ -424,6 +585,55 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
   }
   hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)((M + FM - 1) / FM)), dim3(256), lds,
                      (hipStream_t)stream, M, X, (const __bf16*)weights, biases, Y);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, double time_step,
+                                const void* weights, const float* biases,
+                                const cia_surrogate_gate* gate, const double* anchor_strain,
+                                const double* anchor_inelastic, const float* anchor_latent,
+                                const float* anchor_age, const uint8_t* fallback, double* stress,
+                                double* next_anchor_strain, double* next_anchor_inelastic,
+                                float* next_latent, float* next_age, uint8_t* next_fallback,
+                                uint32_t* trip_counter) {
+  using namespace cia;
+  CIA_ARG(strain && weights && biases && gate && anchor_strain && anchor_inelastic &&
+              anchor_latent && anchor_age && fallback && stress && next_anchor_strain &&
+              next_anchor_inelastic && next_latent && next_age && next_fallback && trip_counter &&
+              n >= 0,
+          "invalid arguments to cia_surrogate_update");
+  CIA_ARG(time_step > 0.0 && gate->eps_scale > 0.0 && gate->rate_ref > 0.0 &&
+              gate->rate_scale > 0.0 && gate->commit_strain >= 0.0,
+          "cia_surrogate_update: time step and scales must be positive");
+  CIA_ARG(n < ((int64_t)1 << 31), "cia_surrogate_update: at most 2^31 - 1 points");
+  if (n == 0) return CIA_OK;
+  const size_t lds = 2 * (size_t)FM * FLD * sizeof(__bf16);
+  static bool attr_set = false;
+  if (!attr_set) {
+    CIA_HIP(hipFuncSetAttribute((const void*)k_sur_update,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  SurGate g;
+  g.eps_scale = gate->eps_scale;
+  g.rate_ref = gate->rate_ref;
+  g.rate_scale = gate->rate_scale;
+  g.inelastic_step = gate->inelastic_step;
+  g.d_eps_max = gate->d_eps_max;
+  g.rate_lo = gate->rate_lo;
+  g.rate_hi = gate->rate_hi;
+  g.latent_max = gate->latent_max;
+  g.gate_margin = gate->gate_margin;
+  g.rate_margin = gate->rate_margin;
+  g.commit_strain = gate->commit_strain;
+  g.commit_time = gate->commit_time;
+  for (int i = 0; i < 36; ++i) g.C[i] = gate->stiffness[i];
+  hipLaunchKernelGGL(k_sur_update, dim3((unsigned)((n + FM - 1) / FM)), dim3(256), lds,
+                     (hipStream_t)stream, n, strain, time_step, (const __bf16*)weights, biases, g,
+                     anchor_strain, anchor_inelastic, anchor_latent, anchor_age, fallback, stress,
+                     next_anchor_strain, next_anchor_inelastic, next_latent, next_age,
+                     next_fallback, (unsigned int*)trip_counter);
   CIA_HIP(hipGetLastError());
   return CIA_OK;
 }

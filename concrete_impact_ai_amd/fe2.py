@@ -219,6 +219,8 @@ class RVEHomogenizedMaterial:
         c = self.operator_cache
         bt = self._batches.get(n)
         if bt is None:
+            if len(self._batches) >= 4:  # keep the operators of the last few point counts
+                self._batches.pop(next(iter(self._batches))).close()
             bt = self._batches[n] = B.RVEBatch([self.model] * n)
         packed = committed.packed.permute(1, 0, 2).reshape(8, n * c.n_points)
         steps = np.broadcast_to(np.asarray(time_step, dtype=np.float64), (n,))

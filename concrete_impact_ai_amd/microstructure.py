@@ -79,3 +79,75 @@ def uniaxial_ramp(n_steps=100, final_strain=2.0e-3, rate=100.0):
     strains[:, 0] = final_strain * np.arange(n_steps + 1) / n_steps
     dt = final_strain / n_steps / rate
     return dt * np.arange(n_steps + 1), strains
+
+
+# Production path design of the reference (``rve_path_generation.py:55-195``):
+# canonical engineering-Voigt tensor directions, loading families given as scalar
+# control knots along the direction, cubic smoothstep in time on every segment, and a
+# duration chosen so the peak strain rate equals the target rate.
+TENSOR_WEIGHTS = np.asarray([1.0, 1.0, 1.0, 0.5, 0.5, 0.5])
+TENSOR_DIRECTIONS = {
+    "deviatoric_uniaxial": (1.0, -0.5, -0.5, 0.0, 0.0, 0.0),
+    "deviatoric_biaxial": (0.5, 0.5, -1.0, 0.0, 0.0, 0.0),
+    "shear_dominant": (0.1, -0.1, 0.0, 1.0, 0.25, 0.0),
+    "positive_triaxial": (1.0, 0.35, 0.35, 0.15, 0.0, 0.0),
+    "negative_triaxial": (-1.0, -0.35, -0.35, 0.15, 0.0, 0.0),
+}
+FAMILY_CONTROLS = {
+    "monotonic": (0.0, 1.0),
+    "load_unload_reload": (0.0, 1.0, 0.25, 1.2),
+    "reverse": (0.0, 1.0, -1.0, 0.5),
+    "hold": (0.0, 1.0, 1.0),
+}
+
+
+def tensor_norm(v):
+    """Engineering-tensor norm (shear components weighted 1/2)."""
+    v = np.asarray(v, dtype=np.float64)
+    return np.sqrt(np.sum(v * v * TENSOR_WEIGHTS, axis=-1))
+
+
+def production_strain_path(family, direction, amplitude, target_rate, n_steps, seed=0,
+                           perturbation=0.0, turning_angle=None):
+    """One path of the reference's production design: times (n+1), strains (n+1, 6).
+
+    ``direction`` is a name of ``TENSOR_DIRECTIONS`` or a 6-vector; it is normalised
+    and perturbed by ``perturbation`` times a seeded unit direction orthogonal to it
+    (in the engineering-tensor inner product).  ``family`` is one of
+    ``FAMILY_CONTROLS`` (scalar knots along the direction) or ``nonproportional``
+    (knots 0, d, d + 0.75 r with r the direction turned by ``turning_angle`` towards
+    a seeded orthogonal one).  The knots are scaled to unit peak norm, interpolated
+    by a cubic smoothstep on each segment in normalised time, and the duration is
+    ``amplitude * (peak normalised rate) / target_rate`` with the peak rate of
+    the smoothstep, 1.5 * segments * max segment norm."""
+    rng = np.random.default_rng(seed)
+    d = np.asarray(TENSOR_DIRECTIONS[direction] if isinstance(direction, str) else direction,
+                   dtype=np.float64)
+    d = d / tensor_norm(d)
+
+    def orthogonal():
+        o = rng.normal(size=6)
+        o -= np.sum(o * d * TENSOR_WEIGHTS) * d
+        return o / tensor_norm(o)
+
+    if perturbation:
+        d = d + perturbation * orthogonal()
+        d = d / tensor_norm(d)
+    if family == "nonproportional":
+        if turning_angle is None:
+            raise ValueError("a nonproportional path needs a turning angle")
+        r = np.cos(turning_angle) * d + np.sin(turning_angle) * orthogonal()
+        knots = np.stack([np.zeros(6), d, d + 0.75 * r])
+    else:
+        knots = np.asarray(FAMILY_CONTROLS[family])[:, None] * d[None, :]
+    knots = knots / float(np.max(tensor_norm(knots)))
+    segments = knots.shape[0] - 1
+    peak_rate = 1.5 * segments * float(np.max(tensor_norm(np.diff(knots, axis=0))))
+    duration = amplitude * peak_rate / target_rate
+    times = np.linspace(0.0, duration, int(n_steps) + 1)
+    u = times / duration * segments
+    seg = np.minimum(u.astype(np.int64), segments - 1)
+    loc = u - seg
+    s = loc * loc * (3.0 - 2.0 * loc)
+    strains = amplitude * (knots[seg] + s[:, None] * (knots[seg + 1] - knots[seg]))
+    return times, strains

@@ -8,30 +8,40 @@ the build's own (13 -> 256 -> 256 -> 256 -> 256 -> 12, GELU):
 
     x_t = [d_eps_t / eps_scale (6), log10(rate_t / rate_ref) / rate_scale (1), z_t (6)]
     o_t = MLP(x_t)
-    sigma_t = sigma_{t-1} + c stress_scale o_t[0:6],   z_{t+1} = z_t + o_t[6:12]
+    ein_t = ein_{t-1} + c eps_scale o_t[0:6],   z_{t+1} = z_t + o_t[6:12]
+    sigma_t = C (eps_t - ein_t)
 
-with c = STRESS_STEP and sigma_{-1} = z_0 = 0: the network predicts stress increments, so
-the six internal state variables carry only the inelastic state (the elastic part of
-an increment needs none); predicting the stress itself from the latent needs the
-latent to integrate the total strain and measured 1.6x the held-out error.
+with c = INELASTIC_STEP, ein_{-1} = z_0 = 0 and C the homogenised elastic stiffness of
+the RVE (its effective tangent at a small strain): the network predicts inelastic
+strain increments, so an elastic increment, loading or unloading, needs no network
+output, and the six internal state variables carry only the inelastic state.  On
+macro impact strain paths this structure measured 0.7x the stress error of
+predicting stress increments directly (same data and training budget); on held-out
+training paths stress increments in turn measured 0.6x the error of predicting the
+stress from the latent.
 
 Every matrix product runs in the HIP kernels of ``libcia_hip.so`` on BF16 matrix
 cores with FP32 accumulation: layer-wise GEMMs with fused bias/GELU epilogues for
 training (forward and backward through time, GELU' fused into the backward-data
 GEMM, split-K weight gradients, Adam), and one fused all-layer kernel for inference.
 
-The gate: a point whose inputs leave the envelope seen in training (strain
-increment, rate feature) or whose latent state leaves the training range is handed
-to the FE² adapter (``RVEHomogenizedMaterial``) for the rest of the run.  Its RVE
-starts from the virgin state and is driven to the point's current strain in
-``fallback_substeps`` proportional increments (the history before the trip is not
-replayed).
+As a material (``SurrogateMaterial``) every point keeps an anchor, the state of its
+last committed network step: each call evaluates the network once, in one fused
+kernel, from the anchor with the whole increment since it, and commits only when
+that increment reaches the size of a training step, so the network sees training-
+sized increments whatever the macro time step.  The gate: a point whose inputs leave
+the envelope seen in training (strain increment, rate above the trained range) or
+whose latent state leaves the training range is handed to the FE² adapter
+(``RVEHomogenizedMaterial``) for the rest of the run.  Its RVE starts from the
+virgin state and is driven to the point's current strain in ``fallback_substeps``
+proportional increments (the history before the trip is not replayed).
 """
 
 from __future__ import annotations
 
 import ctypes as C
 import math
+import time
 from dataclasses import dataclass
 
 import numpy as np
@@ -49,7 +59,7 @@ from concrete_impact_ai_amd.materials import (
 
 SIZES = (13, 256, 256, 256, 256, 12)
 N_LAYERS = len(SIZES) - 1
-STRESS_STEP = 0.1  # scale of the network's stress-increment outputs
+INELASTIC_STEP = 0.3  # scale of the network's inelastic-strain-increment outputs
 
 
 def _p(t):
@@ -241,40 +251,63 @@ class SurrogateScales:
     rate_lo: float = -1.0         # rate feature range seen in training
     rate_hi: float = 1.0
     latent_max: float = 1.0       # max |z| seen in training rollouts
+    d_eps_commit: float = 0.0     # median |d_eps|_inf of a training step (physical strain)
+    stiffness: tuple = ()         # homogenised elastic stiffness, 36 values row-major (Pa)
 
     @classmethod
-    def fit(cls, d_eps, rate, stress):
+    def fit(cls, d_eps, rate, stress, stiffness, rate_floor: float = 1.0):
+        """Scales from training data and the RVE's homogenised elastic stiffness
+        (6 x 6, engineering Voigt); rates below ``rate_floor`` (1/s, the
+        quasi-static end: increments at the ends of smooth path segments and holds)
+        count as ``rate_floor``, which becomes the lower clamp of the rate feature."""
         eps_scale = float(np.max(np.abs(d_eps))) or 1.0
-        lr = np.log10(np.maximum(rate, 1e-30))
+        lr = np.log10(np.maximum(rate, rate_floor))
         ref = float(np.median(lr))
         rs = float(np.max(np.abs(lr - ref))) or 1.0
         return cls(eps_scale, 10.0 ** ref, rs, float(np.max(np.abs(stress))) or 1.0, 1.0,
-                   float(np.min((lr - ref) / rs)), float(np.max((lr - ref) / rs)), 1.0)
+                   float(np.min((lr - ref) / rs)), float(np.max((lr - ref) / rs)), 1.0,
+                   float(np.median(np.max(np.abs(d_eps), axis=-1))),
+                   tuple(float(v) for v in np.asarray(stiffness, dtype=np.float64).ravel()))
+
+    def stiffness_matrix(self) -> np.ndarray:
+        if len(self.stiffness) != 36:
+            raise ValueError("the surrogate scales carry no 6 x 6 homogenised stiffness")
+        return np.asarray(self.stiffness, dtype=np.float64).reshape(6, 6)
 
     def rate_feature(self, rate):
+        """log10(rate / rate_ref) / rate_scale, clamped below at rate_lo (as the
+        material kernel clamps it)."""
         if isinstance(rate, torch.Tensor):
-            return torch.log10(torch.clamp(rate, min=1e-30) / self.rate_ref) / self.rate_scale
-        return np.log10(np.maximum(rate, 1e-30) / self.rate_ref) / self.rate_scale
+            f = torch.log10(torch.clamp(rate, min=1e-30) / self.rate_ref) / self.rate_scale
+            return torch.clamp(f, min=self.rate_lo)
+        return np.maximum(np.log10(np.maximum(rate, 1e-30) / self.rate_ref) / self.rate_scale, self.rate_lo)
 
 
 def rollout(params: SurrogateParams, scales: SurrogateScales, d_eps: torch.Tensor,
-            rate: torch.Tensor, fused: bool = False, w_leaf=None, b_leaf=None):
-    """d_eps [B][T][6] (physical), rate [B][T] -> stress [B][T][6] (physical), latents."""
+            rate: torch.Tensor, fused: bool = False, w_leaf=None, b_leaf=None,
+            normalized: bool = False):
+    """d_eps [B][T][6] (physical, from zero strain), rate [B][T] -> stress [B][T][6]
+    (physical, or divided by stress_scale when ``normalized``), latents."""
     B, T, _ = d_eps.shape
-    z = torch.zeros((B, 6), dtype=torch.float32, device=d_eps.device)
-    s = torch.zeros((B, 6), dtype=torch.float32, device=d_eps.device)
+    dev = d_eps.device
+    z = torch.zeros((B, 6), dtype=torch.float32, device=dev)
+    ein = torch.zeros((B, 6), dtype=torch.float32, device=dev)
     feats_e = (d_eps / scales.eps_scale).float()
+    eps = torch.cumsum(feats_e, dim=1)                     # strain / eps_scale
     feats_r = scales.rate_feature(rate).float()
+    Cs = torch.as_tensor(scales.stiffness_matrix() * (scales.eps_scale / scales.stress_scale),
+                         dtype=torch.float32, device=dev)
     out = []
     lat = [z]
     for t in range(T):
         x = torch.cat([feats_e[:, t], feats_r[:, t, None], z], dim=1)
         o = forward_fused(x, params) if fused else mlp(x, params, w_leaf, b_leaf)
-        s = s + STRESS_STEP * o[:, :6]
-        out.append(s)
+        ein = ein + INELASTIC_STEP * o[:, :6]
+        out.append((eps[:, t] - ein) @ Cs.T)
         z = z + o[:, 6:]
         lat.append(z)
-    return torch.stack(out, dim=1) * scales.stress_scale, torch.stack(lat, dim=1)
+    sig = torch.stack(out, dim=1)
+    return (sig if normalized else sig * scales.stress_scale), torch.stack(lat, dim=1)
 
 
 def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress, iterations: int,
@@ -300,8 +333,8 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
         idx = torch.as_tensor(rng.choice(P, size=min(batch_paths, P), replace=P < batch_paths),
                               device=dev)
         L = min(window, T)
-        pred, _ = rollout(params, SurrogateScales(scales.eps_scale, scales.rate_ref, scales.rate_scale, 1.0),
-                          de[idx, :L], rt[idx, :L], False, w_leaf, b_leaf)
+        pred, _ = rollout(params, scales, de[idx, :L], rt[idx, :L], False, w_leaf, b_leaf,
+                          normalized=True)
         loss = torch.mean(((pred - sg[idx, :L]) ** 2).mean(dim=(1, 2)) * path_w[idx])
         w_leaf.grad = None
         b_leaf.grad = None
@@ -327,6 +360,7 @@ def envelope(params, scales, d_eps, rate):
     scales.rate_lo = float(np.min(rf))
     scales.rate_hi = float(np.max(rf))
     scales.latent_max = float(lat.abs().max())
+    scales.d_eps_commit = float(np.median(np.max(np.abs(d_eps), axis=-1)))
     return scales
 
 
@@ -335,13 +369,30 @@ def envelope(params, scales, d_eps, rate):
 
 
 class SurrogateMaterial:
-    """Material-point protocol around the trained surrogate.  State per point: the
-    committed strain (6), the latent (6) and a fallback flag; points on the FE²
-    fallback carry their RVE states in the adapter's batch."""
+    """Material-point protocol around the trained surrogate (the role of
+    ``RVERNOMaterialAdapter``, ``nn/deployment.py:68-165``), one
+    ``cia_surrogate_update`` kernel per call.
+
+    Every point keeps an anchor (strain, inelastic strain, latent, age): each call
+    evaluates the network once from the anchor with the whole strain increment since
+    it and the mean rate over that time, and the anchor moves only when the increment
+    reaches ``commit_fraction`` times the median training increment (or its age
+    reaches ``commit_time`` > 0).  The network so always sees increments of the size
+    it was trained on, whatever the macro time step; between commits the stress is
+    C (strain - inelastic strain) with the anchor's inelastic strain advanced by one
+    network step.  Rates below the training range are
+    clamped to its lower end (the quasi-static limit of the viscoplastic response);
+    rates above it by more than ``rate_margin`` (in units of the rate feature), an
+    increment beyond ``gate_margin`` times the training envelope, or a latent beyond
+    ``gate_margin`` times the range seen in training trip the gate.
+
+    ``probe_points``: point ids whose strain, stress and time step are recorded at
+    every call (``probe_history()``), for checking the run against the FE² oracle."""
 
     def __init__(self, name, params: SurrogateParams, scales: SurrogateScales, density: float,
                  maximum_wave_speed: float, fallback=None, fallback_substeps: int = 4,
-                 gate_margin: float = 1.25):
+                 gate_margin: float = 1.25, rate_margin: float = 0.5, commit_fraction: float = 0.5,
+                 commit_time: float = 0.0, probe_points=None):
         self.name = name
         self.params = params
         self.scales = scales
@@ -350,12 +401,38 @@ class SurrogateMaterial:
         self.fallback = fallback
         self.fallback_substeps = int(fallback_substeps)
         self.gate_margin = float(gate_margin)
-        self.stats = {"surrogate_points": 0, "fallback_points": 0, "gate_trips": 0}
+        self.rate_margin = float(rate_margin)
+        self.commit_fraction = float(commit_fraction)
+        self.commit_time = float(commit_time)
+        self.probe_points = None if probe_points is None else np.asarray(probe_points, dtype=np.int64)
+        self._probe = {"strain": [], "stress": [], "dt": []}
+        self.stats = {"surrogate_points": 0, "fallback_points": 0, "gate_trips": 0, "calls": 0}
+        self.call_times = []  # host clock after each call's device work (the trip count read)
+
+    def gate(self) -> _lib.SurrogateGate:
+        sc = self.scales
+        g = _lib.SurrogateGate(sc.eps_scale, sc.rate_ref, sc.rate_scale, INELASTIC_STEP, sc.d_eps_max,
+                               sc.rate_lo, sc.rate_hi, sc.latent_max, self.gate_margin, self.rate_margin,
+                               self.commit_fraction * sc.d_eps_commit, self.commit_time)
+        g.stiffness[:] = [float(v) for v in sc.stiffness_matrix().ravel()]
+        return g
+
+    def probe_history(self):
+        """(strain (T, k, 6), stress (T, k, 6), dt (T,)) of the probe points, one row per call."""
+        if not self._probe["dt"]:
+            return np.zeros((0, 0, 6)), np.zeros((0, 0, 6)), np.zeros(0)
+        return (torch.stack(self._probe["strain"]).transpose(1, 2).cpu().numpy(),
+                torch.stack(self._probe["stress"]).transpose(1, 2).cpu().numpy(),
+                np.asarray(self._probe["dt"]))
+
+    def clear_probes(self):
+        self._probe = {"strain": [], "stress": [], "dt": []}
 
     def initialize_state(self, n_points: int) -> MaterialState:
         return MaterialState({"surrogate_strain": np.zeros((n_points, 6)),
-                              "surrogate_stress": np.zeros((n_points, 6)),
+                              "surrogate_inelastic_strain": np.zeros((n_points, 6)),
                               "surrogate_latent": np.zeros((n_points, 6)),
+                              "surrogate_age": np.zeros(n_points),
                               "surrogate_fallback": np.zeros(n_points)})
 
     def device_kernel(self, point_phase_ids=None):
@@ -366,114 +443,150 @@ class SurrogateMaterial:
             raise ValueError("The surrogate material provides no consistent tangent.")
         dev = device()
         k = _SurrogateKernel(self)
-        strain = torch.as_tensor(np.ascontiguousarray(np.asarray(request.strains).T), device=dev)
-        rate = (None if request.strain_rates is None else
-                torch.as_tensor(np.ascontiguousarray(np.asarray(request.strain_rates).T), device=dev))
+        strain = torch.as_tensor(np.ascontiguousarray(np.asarray(request.strains, dtype=np.float64).T),
+                                 device=dev)
         st = k.from_host(state)
-        r = k.update(strain, rate, st, request.time_step, request.update_settings, False)
+        r = k.update(strain, None, st, request.time_step, request.update_settings, False)
         return MaterialPointResponse(stresses=r.stress.cpu().numpy().T.copy(),
                                      state=k.material_state(r.state),
                                      free_energy=None, dissipation=None, diagnostics={})
 
 
 class _SurrogateState:
-    def __init__(self, strain, stress, latent, fallback_mask, rve_state=None, rve_index=None):
-        self.strain = strain            # [n][6] fp64 committed strain
-        self.stress = stress            # [n][6] fp64 committed stress
-        self.latent = latent            # [n][6] fp32
-        self.fallback = fallback_mask   # [n] bool
-        self.rve_state = rve_state      # adapter batch state of the fallback points
-        self.rve_index = rve_index      # np.ndarray of point ids in rve_state order
+    """Anchors in the kernel's layout: strain / inelastic strain [6][n] fp64, latent
+    [6][n] fp32, age [n] fp32, fallback [n] uint8; n_fallback is the host's count of
+    fallback points; rve_state / rve_index hold the FE² states of the fallback points."""
+
+    def __init__(self, strain, inelastic, latent, age, fallback, n_fallback, rve_state=None,
+                 rve_index=None):
+        self.strain = strain
+        self.inelastic = inelastic
+        self.latent = latent
+        self.age = age
+        self.fallback = fallback
+        self.n_fallback = int(n_fallback)
+        self.rve_state = rve_state
+        self.rve_index = rve_index
 
 
 class _SurrogateKernel:
     def __init__(self, material: SurrogateMaterial):
         self.m = material
+        self.trips = torch.zeros(1, dtype=torch.int32, device=device())  # device counter
+        self._trips_seen = 0
 
     def initial_state(self, n):
         dev = device()
-        return _SurrogateState(torch.zeros((n, 6), dtype=torch.float64, device=dev),
-                               torch.zeros((n, 6), dtype=torch.float64, device=dev),
-                               torch.zeros((n, 6), dtype=torch.float32, device=dev),
-                               torch.zeros(n, dtype=torch.bool, device=dev))
+        return _SurrogateState(torch.zeros((6, n), dtype=torch.float64, device=dev),
+                               torch.zeros((6, n), dtype=torch.float64, device=dev),
+                               torch.zeros((6, n), dtype=torch.float32, device=dev),
+                               torch.zeros(n, dtype=torch.float32, device=dev),
+                               torch.zeros(n, dtype=torch.uint8, device=dev), 0)
 
     def from_host(self, state):
         dev = device()
         v = state.variables
-        return _SurrogateState(torch.as_tensor(v["surrogate_strain"], device=dev),
-                               torch.as_tensor(v["surrogate_stress"], dtype=torch.float64, device=dev),
-                               torch.as_tensor(v["surrogate_latent"], dtype=torch.float32, device=dev),
-                               torch.as_tensor(np.asarray(v["surrogate_fallback"]) > 0.5, device=dev))
+        n = np.asarray(v["surrogate_strain"]).shape[0]
+        fb = np.asarray(v["surrogate_fallback"]) > 0.5
+
+        def rows(key, dtype):
+            return torch.as_tensor(np.ascontiguousarray(np.asarray(v[key], dtype=np.float64).T),
+                                   dtype=dtype, device=dev).contiguous()
+
+        age = np.asarray(v["surrogate_age"]) if "surrogate_age" in v else np.zeros(n)
+        return _SurrogateState(rows("surrogate_strain", torch.float64),
+                               rows("surrogate_inelastic_strain", torch.float64),
+                               rows("surrogate_latent", torch.float32),
+                               torch.as_tensor(age, dtype=torch.float32, device=dev),
+                               torch.as_tensor(fb.astype(np.uint8), device=dev), int(fb.sum()),
+                               v.get("_rve_state"), v.get("_rve_index"))
 
     def material_state(self, st):
-        return MaterialState({"surrogate_strain": st.strain.cpu().numpy(),
-                              "surrogate_stress": st.stress.cpu().numpy(),
-                              "surrogate_latent": st.latent.double().cpu().numpy(),
-                              "surrogate_fallback": st.fallback.double().cpu().numpy()})
+        out = {"surrogate_strain": st.strain.t().cpu().numpy(),
+               "surrogate_inelastic_strain": st.inelastic.t().cpu().numpy(),
+               "surrogate_latent": st.latent.t().double().cpu().numpy(),
+               "surrogate_age": st.age.double().cpu().numpy(),
+               "surrogate_fallback": st.fallback.double().cpu().numpy()}
+        if st.rve_state is not None:
+            out["_rve_state"] = st.rve_state
+            out["_rve_index"] = st.rve_index
+        return MaterialState(out)
 
     def update(self, strain, rate, st: _SurrogateState, dt, settings, tangent=False) -> DeviceResponse:
+        """strain [6][n] fp64 on the device; ``rate`` is not used (the rate feature is
+        the mean rate since the anchor)."""
         if tangent:
             raise ValueError("The surrogate material provides no consistent tangent.")
         m = self.m
-        sc = m.scales
-        eps = strain.t().contiguous().double()             # [n][6]
-        d_eps = eps - st.strain
-        if rate is not None:
-            rmag = torch.linalg.norm(rate.t().double(), dim=1)
-        else:
-            rmag = torch.linalg.norm(d_eps, dim=1) / dt
-        feat_r = sc.rate_feature(rmag).float()
-        x = torch.cat([(d_eps / sc.eps_scale).float(), feat_r[:, None], st.latent], dim=1)
-        o = forward_fused(x, m.params)
-        sig = st.stress + STRESS_STEP * sc.stress_scale * o[:, :6].double()
-        lat = st.latent + o[:, 6:]
-        trip = ((d_eps.abs().max(dim=1).values / sc.eps_scale > m.gate_margin * sc.d_eps_max)
-                | (feat_r < sc.rate_lo - 0.5) | (feat_r > sc.rate_hi + 0.5)
-                | (lat.abs().max(dim=1).values > m.gate_margin * sc.latent_max)
-                | ~torch.isfinite(lat).all(dim=1))
-        fb = st.fallback | trip
-        rve_state, rve_index = st.rve_state, st.rve_index
-        n_fb = int(fb.sum())
-        new_trips = int((trip & ~st.fallback).sum())
+        n = strain.shape[1]
+        dev = strain.device
+        strain = strain.contiguous()
+        if strain.dtype != torch.float64 or strain.shape[0] != 6:
+            raise ValueError("surrogate strains must be [6][n] float64")
+        stress = torch.empty((6, n), dtype=torch.float64, device=dev)
+        nxt = _SurrogateState(torch.empty_like(st.strain), torch.empty_like(st.inelastic),
+                              torch.empty_like(st.latent), torch.empty_like(st.age),
+                              torch.empty_like(st.fallback), st.n_fallback, st.rve_state, st.rve_index)
+        gate = m.gate()
+        _check(_lib.lib().cia_surrogate_update(
+            stream_handle(), n, _p(strain), float(dt), _p(m.params.w_bf16), _p(m.params.b), C.byref(gate),
+            _p(st.strain), _p(st.inelastic), _p(st.latent), _p(st.age), _p(st.fallback), _p(stress),
+            _p(nxt.strain), _p(nxt.inelastic), _p(nxt.latent), _p(nxt.age), _p(nxt.fallback),
+            _p(self.trips)), "cia_surrogate_update")
+        total = int(self.trips.item())
+        m.call_times.append(time.perf_counter())
+        new_trips = total - self._trips_seen
+        self._trips_seen = total
+        nxt.n_fallback = st.n_fallback + new_trips
+        m.stats["calls"] += 1
         m.stats["gate_trips"] += new_trips
-        m.stats["fallback_points"] += n_fb
-        m.stats["surrogate_points"] += int(fb.numel() - n_fb)
-        if n_fb:
+        m.stats["fallback_points"] += nxt.n_fallback
+        m.stats["surrogate_points"] += n - nxt.n_fallback
+        if nxt.n_fallback:
             if m.fallback is None:
-                raise RuntimeError(f"{n_fb} surrogate points left the training envelope and no "
-                                   "FE² fallback is configured.")
-            sig, rve_state, rve_index = self._fallback(eps, st, fb, sig, dt, settings)
-        return DeviceResponse(stress=sig.t().contiguous(),
-                              state=_SurrogateState(eps.clone(), sig, lat, fb, rve_state, rve_index))
+                raise RuntimeError(f"{nxt.n_fallback} surrogate points left the training envelope and "
+                                   "no FE² fallback is configured.")
+            self._fallback(strain, nxt, stress, dt, settings)
+        if m.probe_points is not None:
+            ids = torch.as_tensor(m.probe_points, device=dev)
+            m._probe["strain"].append(strain.index_select(1, ids))
+            m._probe["stress"].append(stress.index_select(1, ids))
+            m._probe["dt"].append(float(dt))
+        return DeviceResponse(stress=stress, state=nxt)
 
-    def _fallback(self, eps, st, fb, sig, dt, settings):
-        """FE² for the points on the fallback: earlier trips continue from their
-        committed RVE states; new trips start a virgin RVE driven to the current
-        strain in proportional sub-increments."""
+    def _fallback(self, strain, nxt, stress, dt, settings):
+        """FE² for the points on the fallback (their stresses replace the network's in
+        ``stress``): earlier trips continue from their committed RVE states; new trips
+        start a virgin RVE driven to the current strain in proportional sub-increments."""
         from concrete_impact_ai_amd.fe2 import RVEBatchState
         from concrete_impact_ai_amd.materials import MaterialPointRequest
 
         ad = self.m.fallback
-        ids = torch.nonzero(fb).flatten().cpu().numpy()
-        old = st.rve_index if st.rve_index is not None else np.zeros(0, dtype=np.int64)
+        ids = torch.nonzero(nxt.fallback).flatten().cpu().numpy()
+        old = nxt.rve_index if nxt.rve_index is not None else np.zeros(0, dtype=np.int64)
         new = np.setdiff1d(ids, old)
-        eps_h = eps.cpu().numpy()
-        sig = sig.clone()
+        eps_h = strain.t()[torch.as_tensor(ids, device=strain.device)].cpu().numpy()
+        pos = {int(p): i for i, p in enumerate(ids)}
         states, order = [], []
+
+        def put(points, values):
+            stress[:, torch.as_tensor(points, device=stress.device)] = torch.as_tensor(
+                np.ascontiguousarray(values.T), device=stress.device)
+
         if old.size:
-            r = ad.update(MaterialPointRequest(eps_h[old], None, dt, "three_dimensional", settings),
-                          st.rve_state)
-            sig[torch.as_tensor(old, device=sig.device)] = torch.as_tensor(r.stresses, device=sig.device)
+            e = eps_h[[pos[int(p)] for p in old]]
+            r = ad.update(MaterialPointRequest(e, None, dt, "three_dimensional", settings), nxt.rve_state)
+            put(old, r.stresses)
             states.append(r.state)
             order.append(old)
         if new.size:
+            e = eps_h[[pos[int(p)] for p in new]]
             ns = ad.initialize_state(new.size)
             S = max(1, self.m.fallback_substeps)
             for k in range(1, S + 1):
-                r = ad.update(MaterialPointRequest(eps_h[new] * (k / S), None, dt / S,
-                                                   "three_dimensional", settings), ns)
+                r = ad.update(MaterialPointRequest(e * (k / S), None, dt / S, "three_dimensional", settings), ns)
                 ns = r.state
-            sig[torch.as_tensor(new, device=sig.device)] = torch.as_tensor(r.stresses, device=sig.device)
+            put(new, r.stresses)
             states.append(ns)
             order.append(new)
         if len(states) == 1:
@@ -482,4 +595,5 @@ class _SurrogateKernel:
             batch = RVEBatchState(torch.cat([s_.packed for s_ in states]).contiguous(),
                                   torch.cat([s_.w for s_ in states]).contiguous(),
                                   np.concatenate([s_.macro for s_ in states]), ad.operator_cache)
-        return sig, batch, np.concatenate(order).astype(np.int64)
+        nxt.rve_state = batch
+        nxt.rve_index = np.concatenate(order).astype(np.int64)

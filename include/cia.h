@@ -558,6 +558,42 @@ cia_status cia_f32_to_bf16(void* stream, int64_t n, const float* x, void* y);
 cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const void* weights,
                                  const float* biases, float* Y);
 
+/* Surrogate material-point update: the material protocol of
+ * RVERNOMaterialAdapter.update (src/concrete_impact/nn/deployment.py:97-165) for this
+ * build's network, one kernel per call (features, the fused network, stress and
+ * state update, gate).  The network predicts inelastic strain increments; the stress
+ * is the homogenised elastic stiffness times the elastic strain.  Every point keeps
+ * an anchor (strain, inelastic strain, latent, age = time since the anchor); the
+ * network is evaluated once from the anchor,
+ *   d = strain - anchor_strain, tau = age + time_step, rate = |d|_2 / tau,
+ *   x = [d / eps_scale, max(log10(rate / rate_ref) / rate_scale, rate_lo), latent],
+ *   ein' = anchor_inelastic + inelastic_step eps_scale o[0:6],
+ *   stress = stiffness (strain - ein'),  z' = latent + o[6:12],
+ * and the anchor moves to (strain, ein', z', 0) only when |d|_inf >= commit_strain
+ * (or tau >= commit_time when commit_time > 0); otherwise it is kept with age tau.
+ * A point trips when |d|_inf / eps_scale > gate_margin d_eps_max, the unclamped rate
+ * feature > rate_hi + rate_margin, |z'|_inf > gate_margin latent_max, or a result is
+ * not finite: next_fallback = fallback | trip, and trip_counter (device, never
+ * reset here) is incremented by the number of points that trip with fallback = 0.
+ * Layouts: strain, stress, anchor strain / inelastic strain [6][n] fp64, latents
+ * [6][n] fp32, age [n] fp32, fallback [n] uint8; stiffness 6 x 6 row-major
+ * (engineering Voigt, Pa); weights / biases as for cia_mlp_forward_fused.
+ * Stresses of fallback points are the network's; the caller replaces them. */
+typedef struct cia_surrogate_gate {
+  double eps_scale, rate_ref, rate_scale, inelastic_step;
+  double d_eps_max, rate_lo, rate_hi, latent_max, gate_margin, rate_margin;
+  double commit_strain, commit_time;
+  double stiffness[36];
+} cia_surrogate_gate;
+cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, double time_step,
+                                const void* weights, const float* biases,
+                                const cia_surrogate_gate* gate, const double* anchor_strain,
+                                const double* anchor_inelastic, const float* anchor_latent,
+                                const float* anchor_age, const uint8_t* fallback, double* stress,
+                                double* next_anchor_strain, double* next_anchor_inelastic,
+                                float* next_latent, float* next_age, uint8_t* next_fallback,
+                                uint32_t* trip_counter);
+
 /* Kernel timing with HIP events on the launch stream (no reference
  * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element
  * pass, 2 PCG node gather, 3 PCG vector update, 4 preconditioner (spectral: FFT,

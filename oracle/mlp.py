@@ -4,15 +4,18 @@ see ``oracle/__init__.py``).
 The reference's surrogate (``EnergyDissipationRVERNO``,
 ``src/concrete_impact/nn/models/rve_rno.py:162-476``; rollout
 ``nn/training/rve_rno.py:113-198``) is a different network, so this oracle is the
-build's own network restated in NumPy float64: per step
+build's own network restated in NumPy float64: per step, in scaled units (strain
+in units of the feature scale, stress in units of the stress scale),
 
     x_t = [d_eps_t (6), rate_t (1), z_t (6)]          (13 features)
     h   = GELU(W4 GELU(W3 GELU(W2 GELU(W1 x + b1) + b2) + b3) + b4)
     o   = W5 h + b5                                   (12 outputs)
-    sigma_t = sigma_{t-1} + c o[0:6],   z_{t+1} = z_t + o[6:12],   sigma_{-1} = z_0 = 0
+    eps_t = eps_{t-1} + d_eps_t,   ein_t = ein_{t-1} + c o[0:6]
+    sigma_t = C (eps_t - ein_t),   z_{t+1} = z_t + o[6:12],   eps_{-1} = ein_{-1} = z_0 = 0
 
-(c = STRESS_STEP: the network predicts stress increments, so the latent only has
-to carry the inelastic state, not the total strain)
+(c = INELASTIC_STEP; C = the homogenised elastic stiffness of the RVE: the network
+predicts inelastic strain increments, so an elastic increment needs no network
+output at all and the latent carries only the inelastic state)
 
 with GELU(v) = v Phi(v) (erf form) and loss = mean over paths, steps and the six
 components of (sigma_t - target_t)^2.  ``loss_and_grad`` is a hand-written
@@ -20,7 +23,12 @@ backpropagation through time.
 
 Pinning (tests/test_oracle_mlp.py): gradients against central finite differences and
 against torch-CPU float64 autograd of the same rollout; batched rollout equals the
-per-path rollout (the pattern of ``test_rve_rno_training.py:285-315``).
+per-path rollout (the pattern of ``test_rve_rno_training.py:285-315``); a zero output
+layer gives the elastic response C eps exactly.  ``material_update`` (the anchored
+material-point policy) is pinned to ``rollout``: with commits at every call it
+reproduces the rollout, a path split into sub-steps reproduces the rollout at the
+commit steps, low rates equal the clamped rate, and planted mistakes in the anchor
+bookkeeping break those identities.
 """
 
 from __future__ import annotations
@@ -29,7 +37,7 @@ import numpy as np
 from scipy.special import erf
 
 SIZES = (13, 256, 256, 256, 256, 12)
-STRESS_STEP = 0.1
+INELASTIC_STEP = 0.3
 
 
 def init_params(seed: int = 0, sizes=SIZES):
@@ -64,11 +72,13 @@ def mlp_forward(params, x):
     return a, cache
 
 
-def rollout(params, d_eps, rate):
-    """d_eps (B, T, 6), rate (B, T) -> stresses (B, T, 6), latents (B, T + 1, 6), caches."""
+def rollout(params, d_eps, rate, C):
+    """d_eps (B, T, 6), rate (B, T), stiffness C (6, 6), all scaled -> stresses
+    (B, T, 6), latents (B, T + 1, 6), caches."""
     B, T, _ = d_eps.shape
     z = np.zeros((B, 6))
-    s = np.zeros((B, 6))
+    eps = np.zeros((B, 6))
+    ein = np.zeros((B, 6))
     sig = np.zeros((B, T, 6))
     lat = [z]
     caches = []
@@ -76,28 +86,29 @@ def rollout(params, d_eps, rate):
         x = np.concatenate([d_eps[:, t], rate[:, t, None], z], axis=1)
         o, cache = mlp_forward(params, x)
         caches.append(cache)
-        s = s + STRESS_STEP * o[:, :6]
-        sig[:, t] = s
+        eps = eps + d_eps[:, t]
+        ein = ein + INELASTIC_STEP * o[:, :6]
+        sig[:, t] = (eps - ein) @ C.T
         z = z + o[:, 6:]
         lat.append(z)
     return sig, np.stack(lat, axis=1), caches
 
 
-def loss_and_grad(params, d_eps, rate, target):
+def loss_and_grad(params, d_eps, rate, target, C):
     """Mean squared stress error of the rollout and its gradient by BPTT."""
     B, T, _ = d_eps.shape
-    sig, _, caches = rollout(params, d_eps, rate)
+    sig, _, caches = rollout(params, d_eps, rate, C)
     n = B * T * 6
     diff = sig - target
     loss = float(np.sum(diff * diff) / n)
     grads = [(np.zeros_like(w), np.zeros_like(b)) for w, b in params]
-    gz = np.zeros((B, 6))  # dL/dz_{t+1} flowing back
-    gs = np.zeros((B, 6))  # dL/dsigma_t (sigma_{t+1} = sigma_t + c o[0:6] carries it back)
+    gz = np.zeros((B, 6))   # dL/dz_{t+1} flowing back
+    ge = np.zeros((B, 6))   # dL/dein_t (ein_{t+1} = ein_t + c o[0:6] carries it back)
     for t in range(T - 1, -1, -1):
         go = np.zeros((B, 12))
-        gs = gs + 2.0 * diff[:, t] / n
-        go[:, :6] = STRESS_STEP * gs
-        go[:, 6:] = gz                    # z_{t+1} = z_t + o[6:12]
+        ge = ge - (2.0 * diff[:, t] / n) @ C    # sigma_t = C (eps_t - ein_t)
+        go[:, :6] = INELASTIC_STEP * ge
+        go[:, 6:] = gz                          # z_{t+1} = z_t + o[6:12]
         cache = caches[t]
         g = go
         for li in range(len(params) - 1, -1, -1):
@@ -108,5 +119,67 @@ def loss_and_grad(params, d_eps, rate, target):
             gw += g.T @ a
             gb += g.sum(axis=0)
             g = g @ params[li][0]
-        gz = gz + g[:, 7:13]              # dL/dz_t = dL/dz_{t+1} + dL/dx_t[7:13]
+        gz = gz + g[:, 7:13]                    # dL/dz_t = dL/dz_{t+1} + dL/dx_t[7:13]
     return loss, grads
+
+
+# ---------------------------------------------------------------------------
+# Material-point update with anchors (the policy of the deployed surrogate)
+
+GATE_FIELDS = ("eps_scale", "rate_ref", "rate_scale", "inelastic_step", "d_eps_max", "rate_lo",
+               "rate_hi", "latent_max", "gate_margin", "rate_margin", "commit_strain", "commit_time")
+
+
+def initial_anchor(n):
+    return {"strain": np.zeros((n, 6)), "inelastic": np.zeros((n, 6)), "latent": np.zeros((n, 6)),
+            "age": np.zeros(n), "fallback": np.zeros(n, dtype=bool)}
+
+
+def material_update(params, gate, strain, dt, anchor):
+    """One call of the surrogate material at n points (float64, physical units).
+
+    Every point keeps an anchor (strain, inelastic strain, latent, age = time since
+    the anchor) and the network is evaluated once from it:
+
+        d = strain - anchor_strain,  tau = age + dt,  rate = |d|_2 / tau
+        x = [d / eps_scale, max(log10(rate / rate_ref) / rate_scale, rate_lo), latent]
+        ein' = anchor_inelastic + inelastic_step eps_scale o[0:6],  z' = latent + o[6:12]
+        stress = C (strain - ein')          (C = gate["stiffness"], 6 x 6)
+
+    The anchor moves to (strain, ein', z', 0) when |d|_inf >= commit_strain or
+    (commit_time > 0 and) tau >= commit_time, and otherwise stays with age tau.  A
+    point trips when |d|_inf / eps_scale > gate_margin d_eps_max, the unclamped rate
+    feature > rate_hi + rate_margin, |z'|_inf > gate_margin latent_max, or a result is
+    not finite.  Returns (stress, next anchor, trip mask, number of new trips).
+
+    The reference's deployed network is evaluated at every call from the committed
+    latent (``RVERNOMaterialAdapter.update``, ``nn/deployment.py:97-165``); the anchors
+    are this build's policy so that the network always sees increments of the size it
+    was trained on, whatever the macro time step."""
+    g = gate
+    C = np.asarray(g["stiffness"], dtype=np.float64).reshape(6, 6)
+    d = strain - anchor["strain"]
+    tau = anchor["age"] + dt
+    rate = np.linalg.norm(d, axis=1) / tau
+    feat_raw = np.log10(np.maximum(rate, 1e-30) / g["rate_ref"]) / g["rate_scale"]
+    feat = np.maximum(feat_raw, g["rate_lo"])
+    x = np.concatenate([d / g["eps_scale"], feat[:, None], anchor["latent"]], axis=1)
+    o, _ = mlp_forward(params, x)
+    ein = anchor["inelastic"] + g["inelastic_step"] * g["eps_scale"] * o[:, :6]
+    stress = (strain - ein) @ C.T
+    z = anchor["latent"] + o[:, 6:]
+    dinf = np.max(np.abs(d), axis=1)
+    finite = np.all(np.isfinite(stress), axis=1) & np.all(np.isfinite(z), axis=1)
+    trip = ((dinf / g["eps_scale"] > g["gate_margin"] * g["d_eps_max"])
+            | (feat_raw > g["rate_hi"] + g["rate_margin"])
+            | (np.max(np.abs(z), axis=1) > g["gate_margin"] * g["latent_max"]) | ~finite)
+    commit = dinf >= g["commit_strain"]
+    if g["commit_time"] > 0.0:
+        commit |= tau >= g["commit_time"]
+    c = commit[:, None]
+    nxt = {"strain": np.where(c, strain, anchor["strain"]),
+           "inelastic": np.where(c, ein, anchor["inelastic"]),
+           "latent": np.where(c, z, anchor["latent"]),
+           "age": np.where(commit, 0.0, tau),
+           "fallback": anchor["fallback"] | trip}
+    return stress, nxt, trip, int(np.sum(trip & ~anchor["fallback"]))

@@ -42,8 +42,9 @@ def test_struct_layouts_match_header():
     src = r"""
 #include <stdio.h>
 #include "cia.h"
-int main(){printf("%zu %zu %zu %zu %zu\n", sizeof(cia_rve_result), sizeof(cia_rve_desc),
- sizeof(cia_point_failure), sizeof(cia_material), sizeof(cia_update_settings));}
+int main(){printf("%zu %zu %zu %zu %zu %zu\n", sizeof(cia_rve_result), sizeof(cia_rve_desc),
+ sizeof(cia_point_failure), sizeof(cia_material), sizeof(cia_update_settings),
+ sizeof(cia_surrogate_gate));}
 """
     with tempfile.TemporaryDirectory() as d:
         c = Path(d) / "p.c"
@@ -54,4 +55,4 @@ int main(){printf("%zu %zu %zu %zu %zu\n", sizeof(cia_rve_result), sizeof(cia_rv
                                                 check=True).stdout.split()]
     assert sizes == [ctypes.sizeof(_lib.RVEResult), ctypes.sizeof(_lib.RVEDesc),
                      ctypes.sizeof(_lib.PointFailure), ctypes.sizeof(_lib.Material),
-                     ctypes.sizeof(_lib.UpdateSettings)]
+                     ctypes.sizeof(_lib.UpdateSettings), ctypes.sizeof(_lib.SurrogateGate)]

@@ -150,7 +150,9 @@ def test_bptt_gradient_matches_oracle(gpu):
     d_eps = rng.standard_normal((B, T, 6)) * 0.5
     rate = rng.standard_normal((B, T)) * 0.3
     target = rng.standard_normal((B, T, 6)) * 0.2
-    loss_ref, g_ref = OM.loss_and_grad(layers, d_eps, rate, target)
+    a = np.random.default_rng(11).standard_normal((6, 6))
+    Cs = a @ a.T / 6.0 + np.eye(6)     # scaled stiffness, symmetric positive definite
+    loss_ref, g_ref = OM.loss_and_grad(layers, d_eps, rate, target, Cs)
     params = S.SurrogateParams(layers)
     scales = S.SurrogateScales(1.0, 1.0, 1.0, 1.0)
     w_leaf = params.w.detach().requires_grad_(True)
@@ -160,13 +162,16 @@ def test_bptt_gradient_matches_oracle(gpu):
     de = _dev(d_eps, torch.float32)
     rt = _dev(rate, torch.float32)
     z = torch.zeros((B, 6), dtype=torch.float32, device="cuda")
-    sg = torch.zeros((B, 6), dtype=torch.float32, device="cuda")
+    ein = torch.zeros((B, 6), dtype=torch.float32, device="cuda")
+    eps = torch.cumsum(de, dim=1)
+    Ct = _dev(Cs, torch.float32)
     loss = 0.0
     tg = _dev(target, torch.float32)
     for t in range(T):
         x = torch.cat([de[:, t], rt[:, t, None], z], dim=1)
         o = S.mlp(x, params, w_leaf, b_leaf)
-        sg = sg + S.STRESS_STEP * o[:, :6]
+        ein = ein + S.INELASTIC_STEP * o[:, :6]
+        sg = (eps[:, t] - ein) @ Ct.T
         loss = loss + ((sg - tg[:, t]) ** 2).sum()
         z = z + o[:, 6:]
     loss = loss / (B * T * 6)
@@ -184,7 +189,8 @@ def test_bptt_gradient_matches_oracle(gpu):
 
 
 def test_training_reduces_rollout_loss(gpu):
-    """Fit a viscoelastic-like recurrence sigma_t = 0.8 sigma_{t-1} + d_eps_t."""
+    """Fit a viscoelastic-like recurrence sigma_t = 0.8 sigma_{t-1} + 1e10 d_eps_t
+    (elastic stiffness 1e10 I: the inelastic strain takes up the relaxation)."""
     from concrete_impact_ai_amd import surrogate as S
 
     rng = np.random.default_rng(2)
@@ -196,7 +202,7 @@ def test_training_reduces_rollout_loss(gpu):
     for t in range(T):
         s = 0.8 * s + 1e10 * d_eps[:, t]
         sig[:, t] = s
-    scales = S.SurrogateScales.fit(d_eps, rate, sig)
+    scales = S.SurrogateScales.fit(d_eps, rate, sig, 1e10 * np.eye(6))
     params = S.SurrogateParams(seed=3)
     hist = S.train(params, scales, d_eps, rate, sig, iterations=300, batch_paths=128, window=T,
                    lr=2e-3, log_every=50)
@@ -220,7 +226,7 @@ def test_gate_sends_out_of_envelope_points_to_fe2(gpu):
                                 Sv.NonlinearNewtonSettings(20, 1e-3, 1e-10, 1e-14, 1e-10))
     ad = fe2.RVEHomogenizedMaterial(model, "fallback")
     scales = S.SurrogateScales(1e-4, 100.0, 1.0, 1e8, d_eps_max=1.0, rate_lo=-1.0, rate_hi=1.0,
-                               latent_max=1e6)
+                               latent_max=1e6, stiffness=tuple(3e10 * np.eye(6).ravel()))
     sm = S.SurrogateMaterial("sur", S.SurrogateParams(seed=0), scales, 2400.0, 4000.0, fallback=ad,
                              fallback_substeps=4)
     try:
