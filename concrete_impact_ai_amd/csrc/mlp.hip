@@ -48,12 +48,22 @@ struct GemmArgs {
   int64_t split_stride;  // Cf offset per split
 };
 
-__device__ __forceinline__ float gelu(float v) {
-  return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f));
+// GELU in its tanh form (Hendrycks & Gimpel, "Gaussian Error Linear Units"):
+// 0.5 x (1 + tanh(k (x + 0.044715 x^3))), k = sqrt(2 / pi), evaluated as
+// x sigmoid(2 k (x + 0.044715 x^3)) with the hardware exp2 and reciprocal: about
+// eight VALU instructions, where the epilogue of every hidden layer evaluates 32
+// of them per lane and MFMA subtile.
+constexpr float GELU_2K = 1.5957691216057308f;  // 2 sqrt(2 / pi)
+constexpr float GELU_C = 0.044715f;
+__device__ __forceinline__ float gelu(float x) {
+  const float v = GELU_2K * x * (1.0f + GELU_C * x * x);
+  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-v));
 }
-__device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * 0.39894228040143268f * expf(-0.5f * z * z);
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float v = GELU_2K * x * (1.0f + GELU_C * x * x);
+  const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-v));
+  const float dv = GELU_2K * (1.0f + 3.0f * GELU_C * x * x);
+  return sg + x * sg * (1.0f - sg) * dv;
 }
 
 template <int EPI>
@@ -180,122 +190,138 @@ __global__ void k_f32_to_bf16(int64_t n, const float* __restrict__ x, __bf16* __
 
 // ---------------------------------------------------------------------------
 // Fused inference: 13 -> 256 -> 256 -> 256 -> 256 -> 12 for a tile of FM points
-// per workgroup.  Activations (bf16) alternate between two LDS buffers; each of
-// the 4 waves computes a 64-column slice of a hidden layer with B fragments read
-// from the weights in global memory (shared by every workgroup, so L2-resident)
-// as one 16-byte load per lane and MFMA subtile.
+// per workgroup, the activations (bf16, row-major [point][feature]) in one LDS
+// buffer of 66 KB, so two workgroups share a CU and one wave's GELU epilogue
+// overlaps the other's matrix-core work.  Each of the 4 waves computes a
+// 64-feature slice of a layer as the transposed product out^T = W act^T: the A
+// operand is 16 weight rows (one 16-byte load per lane from the L2-resident
+// weights, prefetched one k-step ahead), the B operand 16 points' activations
+// (one 16-byte LDS load per lane), and the accumulator holds 4 consecutive
+// features of one point per lane, written back with one 8-byte LDS store.
 constexpr int FM = 128;          // points per workgroup
 constexpr int FH = 256;          // hidden width
 constexpr int FIN = 13, FOUT = 12;
-constexpr int FLD = FH + 8;      // LDS row stride (bf16), 528 B
+constexpr int FLD = FH + 8;      // LDS row stride (bf16), 528 B: conflict-free 16-byte reads
+constexpr size_t FUSED_ACT_BYTES = (size_t)FM * FLD * sizeof(__bf16);
 
-// Hidden layers 1..4 on one FM-point tile: a0 holds the 13 input features per row
-// (columns 13..31 zero); the last hidden activations end in a0, a1 is free after.
-__device__ __forceinline__ void fused_hidden(__bf16* a0, __bf16* a1, const __bf16* __restrict__ Wb,
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// A operand of a hidden layer: rows n of W (layer 0: W1 [256][13], k >= 13 read as zero).
+__device__ __forceinline__ bf16x8 weight_frag(const __bf16* __restrict__ Wb, int layer, int n,
+                                              int k0) {
+  bf16x8 f;
+  if (layer == 0) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) f[t] = (k0 + t) < FIN ? Wb[n * FIN + k0 + t] : (__bf16)0.0f;
+  } else {
+    f = *(const bf16x8*)&Wb[FH * FIN + (int64_t)(layer - 1) * FH * FH + n * FH + k0];
+  }
+  return f;
+}
+
+// Hidden layers 1..4 on one FM-point tile: act holds the 13 input features per row
+// (columns 13..31 zero) on entry and the last hidden activations on exit.
+__device__ __forceinline__ void fused_hidden(__bf16* act, const __bf16* __restrict__ Wb,
                                              const float* __restrict__ bias, int wave, int lane) {
   const int lr = lane & 15, lk = (lane >> 4) * 8;
-  const __bf16* W1 = Wb;
-  const __bf16* Wh = Wb + FH * FIN;               // 3 x [256][256]
   const int wn = wave * 64;
   for (int layer = 0; layer < 4; ++layer) {
-    const __bf16* src = (layer & 1) ? a1 : a0;
-    __bf16* dst = (layer & 1) ? a0 : a1;
-    f32x4 acc[FM / 16][4];
+    f32x4 acc[4][FM / 16];  // [feature subtile j][point subtile i]
 #pragma unroll
-    for (int i = 0; i < FM / 16; ++i)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+      for (int i = 0; i < FM / 16; ++i) acc[j][i] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const int ksteps = layer == 0 ? 1 : FH / 32;
+    bf16x8 wf[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wf[j] = weight_frag(Wb, layer, wn + 16 * j + lr, lk);
     for (int ks = 0; ks < ksteps; ++ks) {
-      bf16x8 bfr[4];
+      bf16x8 wn_next[4];
+      if (ks + 1 < ksteps) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int n = wn + 16 * j + lr;
-        if (layer == 0) {
-#pragma unroll
-          for (int t = 0; t < 8; ++t) {
-            const int k = lk + t;
-            bfr[j][t] = k < FIN ? W1[n * FIN + k] : (__bf16)0.0f;
-          }
-        } else {
-          bfr[j] = *(const bf16x8*)&Wh[(int64_t)(layer - 1) * FH * FH + n * FH + 32 * ks + lk];
-        }
+        for (int j = 0; j < 4; ++j) wn_next[j] = weight_frag(Wb, layer, wn + 16 * j + lr, 32 * (ks + 1) + lk);
       }
 #pragma unroll
       for (int i = 0; i < FM / 16; ++i) {
-        const bf16x8 af = *(const bf16x8*)&src[(16 * i + lr) * FLD + 32 * ks + lk];
+        const bf16x8 af = *(const bf16x8*)&act[(16 * i + lr) * FLD + 32 * ks + lk];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[j], acc[i][j], 0, 0, 0);
+          acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[j], af, acc[j][i], 0, 0, 0);
+      }
+      if (ks + 1 < ksteps) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) wf[j] = wn_next[j];
       }
     }
+    __syncthreads();  // every wave has read this layer's input
     const float* b = bias + layer * FH;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int n = wn + 16 * j + lr;
-      const float bn = b[n];
+      const int n0 = wn + 16 * j + 4 * (lane >> 4);
+      const float b0 = b[n0], b1 = b[n0 + 1], b2 = b[n0 + 2], b3 = b[n0 + 3];
 #pragma unroll
-      for (int i = 0; i < FM / 16; ++i)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = 16 * i + 4 * (lane >> 4) + r;
-          dst[row * FLD + n] = (__bf16)gelu(acc[i][j][r] + bn);
-        }
+      for (int i = 0; i < FM / 16; ++i) {
+        bf16x4 o;
+        o[0] = (__bf16)gelu(acc[j][i][0] + b0);
+        o[1] = (__bf16)gelu(acc[j][i][1] + b1);
+        o[2] = (__bf16)gelu(acc[j][i][2] + b2);
+        o[3] = (__bf16)gelu(acc[j][i][3] + b3);
+        *(bf16x4*)&act[(16 * i + lr) * FLD + n0] = o;
+      }
     }
     __syncthreads();
   }
 }
 
-// Output layer on the tile left in a0 by fused_hidden: wave w takes rows
-// 32 w .. 32 w + 31 (two 16-row subtiles); out(row, column, value) receives the
-// FOUT outputs of every row.
+// Output layer on the activations left by fused_hidden: wave w takes points
+// 32 w .. 32 w + 31 (two 16-point subtiles); out(point, column, value) receives the
+// FOUT outputs of every point.
 template <class Out>
-__device__ __forceinline__ void fused_output(const __bf16* a0, const __bf16* __restrict__ Wb,
+__device__ __forceinline__ void fused_output(const __bf16* act, const __bf16* __restrict__ Wb,
                                              const float* __restrict__ bias, int wave, int lane,
                                              Out out) {
   const int lr = lane & 15, lk = (lane >> 4) * 8;
   const __bf16* W5 = Wb + FH * FIN + 3 * FH * FH;  // [12][256]
   f32x4 acc[2] = {(f32x4){0.f, 0.f, 0.f, 0.f}, (f32x4){0.f, 0.f, 0.f, 0.f}};
   for (int ks = 0; ks < FH / 32; ++ks) {
-    bf16x8 bfr;
+    bf16x8 wf;
     if (lr < FOUT)
-      bfr = *(const bf16x8*)&W5[lr * FH + 32 * ks + lk];
+      wf = *(const bf16x8*)&W5[lr * FH + 32 * ks + lk];
     else
 #pragma unroll
-      for (int t = 0; t < 8; ++t) bfr[t] = (__bf16)0.0f;
+      for (int t = 0; t < 8; ++t) wf[t] = (__bf16)0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const bf16x8 af = *(const bf16x8*)&a0[(32 * wave + 16 * i + lr) * FLD + 32 * ks + lk];
-      acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, acc[i], 0, 0, 0);
+      const bf16x8 af = *(const bf16x8*)&act[(32 * wave + 16 * i + lr) * FLD + 32 * ks + lk];
+      acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, af, acc[i], 0, 0, 0);
     }
   }
-  if (lr < FOUT) {
-    const float bn = bias[4 * FH + lr];
+  const int n0 = 4 * (lane >> 4);  // outputs n0 .. n0 + 3 of point lr
+  if (n0 < FOUT) {
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) out(32 * wave + 16 * i + 4 * (lane >> 4) + r, lr, acc[i][r] + bn);
+      for (int r = 0; r < 4; ++r)
+        out(32 * wave + 16 * i + lr, n0 + r, acc[i][r] + bias[4 * FH + n0 + r]);
   }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_fused(int64_t M, const float* __restrict__ X,
+__global__ void __launch_bounds__(256, 2) k_mlp_fused(int64_t M, const float* __restrict__ X,
                                                    const __bf16* __restrict__ Wb,
                                                    const float* __restrict__ bias,
                                                    float* __restrict__ Y) {
-  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [2][FM][FLD]
+  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [FM][FLD]
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t m0 = (int64_t)blockIdx.x * FM;
-  __bf16* a0 = act;
-  __bf16* a1 = act + FM * FLD;
   // inputs -> bf16, columns 13..31 zero
   for (int i = tid; i < FM * 32; i += 256) {
     const int r = i >> 5, c = i & 31;
     const int64_t m = m0 + r;
-    a0[r * FLD + c] = (__bf16)((m < M && c < FIN) ? X[m * FIN + c] : 0.0f);
+    act[r * FLD + c] = (__bf16)((m < M && c < FIN) ? X[m * FIN + c] : 0.0f);
   }
   __syncthreads();
-  fused_hidden(a0, a1, Wb, bias, wave, lane);
-  fused_output(a0, Wb, bias, wave, lane, [&](int row, int col, float v) {
+  fused_hidden(act, Wb, bias, wave, lane);
+  fused_output(act, Wb, bias, wave, lane, [&](int row, int col, float v) {
     const int64_t m = m0 + row;
     if (m < M) Y[m * FOUT + col] = v;
   });
@@ -354,7 +380,7 @@ __device__ __forceinline__ SurPoint sur_inputs(int64_t n, int64_t m, const doubl
   return p;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
     k_sur_update(int64_t n, const double* __restrict__ strain, double dt,
                  const __bf16* __restrict__ Wb, const float* __restrict__ bias, SurGate g,
                  const double* __restrict__ a_strain, const double* __restrict__ a_inel,
@@ -363,15 +389,14 @@ __global__ void __launch_bounds__(256)
                  double* __restrict__ n_strain, double* __restrict__ n_inel,
                  float* __restrict__ n_lat, float* __restrict__ n_age, uint8_t* __restrict__ fb_out,
                  unsigned int* __restrict__ trips) {
-  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [2][FM][FLD]
+  extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [FM][FLD], then o_s
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t m0 = (int64_t)blockIdx.x * FM;
-  __bf16* a0 = act;
-  __bf16* a1 = act + FM * FLD;
+  float* o_s = (float*)((char*)act + FUSED_ACT_BYTES);  // [FM][FOUT + 1] network outputs
   // features: one thread per point of the tile
   if (tid < FM) {
     const int64_t m = m0 + tid;
-    __bf16* row = a0 + tid * FLD;
+    __bf16* row = act + tid * FLD;
     if (m < n) {
       const SurPoint p = sur_inputs(n, m, strain, dt, g, a_strain, a_age);
 #pragma unroll
@@ -387,9 +412,8 @@ __global__ void __launch_bounds__(256)
     for (int c = FIN; c < 32; ++c) row[c] = (__bf16)0.0f;
   }
   __syncthreads();
-  fused_hidden(a0, a1, Wb, bias, wave, lane);
-  float* o_s = (float*)a1;  // [FM][FOUT + 1], a1 is free after the hidden layers
-  fused_output(a0, Wb, bias, wave, lane,
+  fused_hidden(act, Wb, bias, wave, lane);
+  fused_output(act, Wb, bias, wave, lane,
                [&](int row, int col, float v) { o_s[row * (FOUT + 1) + col] = v; });
   __syncthreads();
   if (tid < FM) {  // waves 0 and 1, whole
@@ -576,7 +600,7 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
   using namespace cia;
   CIA_ARG(X && weights && biases && Y && M >= 0, "invalid arguments to cia_mlp_forward_fused");
   if (M == 0) return CIA_OK;
-  const size_t lds = 2 * (size_t)FM * FLD * sizeof(__bf16);
+  const size_t lds = FUSED_ACT_BYTES;
   static bool attr_set = false;
   if (!attr_set) {
     CIA_HIP(hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -608,7 +632,7 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
           "cia_surrogate_update: time step and scales must be positive");
   CIA_ARG(n < ((int64_t)1 << 31), "cia_surrogate_update: at most 2^31 - 1 points");
   if (n == 0) return CIA_OK;
-  const size_t lds = 2 * (size_t)FM * FLD * sizeof(__bf16);
+  const size_t lds = FUSED_ACT_BYTES + (size_t)FM * (FOUT + 1) * sizeof(float);
   static bool attr_set = false;
   if (!attr_set) {
     CIA_HIP(hipFuncSetAttribute((const void*)k_sur_update,

@@ -523,9 +523,9 @@ cia_status cia_box_newmark_solve(cia_box* box, void* stream, const double* tange
  * its BPTT training, nn/training/rve_rno.py:113-415) on BF16 matrix cores with
  * FP32 accumulation.  Matrices are row-major; bf16 buffers are passed as void*.
  *   cia_mlp_linear_forward:  Y = act(X W^T + b), X [M][K] bf16, W [N][K] bf16,
- *     b [N] fp32 or NULL, activation 0 = identity, 1 = GELU (erf form); Z [M][N]
- *     fp32 receives the pre-activation when activation = 1 (or NULL); Y [M][N]
- *     fp32 and/or Y_bf16 [M][N].
+ *     b [N] fp32 or NULL, activation 0 = identity, 1 = GELU in its tanh form,
+ *     0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))); Z [M][N] fp32 receives the
+ *     pre-activation when activation = 1 (or NULL); Y [M][N] fp32 and/or Y_bf16 [M][N].
  *   cia_mlp_linear_backward_data: dX = dZ W, dZ [M][N] bf16, W [N][K] bf16; with
  *     Zprev [M][K] fp32 the result is multiplied by GELU'(Zprev) (the gradient
  *     with respect to the previous layer's pre-activation).

@@ -17,7 +17,7 @@ in units of the feature scale, stress in units of the stress scale),
 predicts inelastic strain increments, so an elastic increment needs no network
 output at all and the latent carries only the inelastic state)
 
-with GELU(v) = v Phi(v) (erf form) and loss = mean over paths, steps and the six
+with GELU in its tanh form, 0.5 v (1 + tanh(sqrt(2/pi) (v + 0.044715 v^3))), and loss = mean over paths, steps and the six
 components of (sigma_t - target_t)^2.  ``loss_and_grad`` is a hand-written
 backpropagation through time.
 
@@ -34,7 +34,6 @@ bookkeeping break those identities.
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
 
 SIZES = (13, 256, 256, 256, 256, 12)
 INELASTIC_STEP = 0.3
@@ -53,12 +52,18 @@ def init_params(seed: int = 0, sizes=SIZES):
     return params
 
 
+GELU_K = np.sqrt(2.0 / np.pi)
+GELU_C = 0.044715
+
+
 def gelu(v):
-    return 0.5 * v * (1.0 + erf(v / np.sqrt(2.0)))
+    """GELU, tanh form (Hendrycks & Gimpel): 0.5 v (1 + tanh(k (v + c v^3)))."""
+    return 0.5 * v * (1.0 + np.tanh(GELU_K * (v + GELU_C * v ** 3)))
 
 
 def gelu_grad(v):
-    return 0.5 * (1.0 + erf(v / np.sqrt(2.0))) + v * np.exp(-0.5 * v * v) / np.sqrt(2.0 * np.pi)
+    t = np.tanh(GELU_K * (v + GELU_C * v ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * v * (1.0 - t * t) * GELU_K * (1.0 + 3.0 * GELU_C * v * v)
 
 
 def mlp_forward(params, x):
