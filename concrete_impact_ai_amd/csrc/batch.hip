@@ -156,15 +156,13 @@ __device__ __forceinline__ bool pcg_active(const Arena& A, const Ctl& c) {
 // ---------------------------------------------------------------------------
 // Assembly: strain -> constitutive update -> element force rows, one lane per
 // Gauss point (as k_assemble of the single-RVE solver).  `want` selects the
-// RVEs: S_NEWTON (first assembly, slot cur), S_LS (line-search candidate,
-// slot 1 - cur) or S_DONE (final pass with energies, diagnostics, next state
-// and the response sums).
-template <class T, bool EXTRA>
+// RVEs: S_NEWTON (first assembly, slot cur) or S_LS (line-search candidate,
+// slot 1 - cur).
+template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_b_assemble(Arena A, Settings S, int want, const T* __restrict__ state_c,
-                 T* __restrict__ next_state, const double* __restrict__ wc) {
-  __shared__ double cw[kBlock][EXTRA ? 6 : 3];
-  __shared__ double red[kBlock / 64][B_NRESP];
+                 const double* __restrict__ wc) {
+  __shared__ T cw[kBlock][3];  // corner values, rounded to T as strain_from rounds them
   const int64_t NPT = A.NPT, ET = A.ET;
   CIA_TILES(A.T[F_GP]) {
     const int r = A.t2r[F_GP][t];
@@ -182,45 +180,27 @@ __global__ void __launch_bounds__(kBlock)
     if (valid) {
       const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const double wv = w[3 * cq + m];
-        cw[threadIdx.x][m] = wv;
-        if (EXTRA) cw[threadIdx.x][3 + m] = wv - wc[3 * cq + m];
-      }
+      for (int m = 0; m < 3; ++m) cw[threadIdx.x][m] = (T)w[3 * cq + m];
     }
     __syncthreads();
-    double acc[B_NRESP];
-#pragma unroll
-    for (int s = 0; s < B_NRESP; ++s) acc[s] = 0.0;
     if (valid) {
       const int64_t gp = 8 * D.e0 + p;
       const int64_t ge = D.e0 + e;
       T bx[8], by[8], bz[8];
       gp_factors<T>(q, D.g, bx, by, bz);
-      T eps[6], rate[6];
+      T eps[6];
       {
-        double we[8][3];
+        T we[8][3];
         const int base = threadIdx.x & ~7;
 #pragma unroll
         for (int a = 0; a < 8; ++a)
 #pragma unroll
           for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][m];
         strain_from(we, bx, by, bz, eps);
-        if (EXTRA) {
-#pragma unroll
-          for (int a = 0; a < 8; ++a)
-#pragma unroll
-            for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][3 + m];
-          strain_from(we, bx, by, bz, rate);
-        }
       }
 #pragma unroll
       for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
-      if (EXTRA) {
-#pragma unroll
-        for (int m = 0; m < 6; ++m) rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
-      }
-      const PhaseConst P = A.tabs[r].p[A.ephase[ge]];
+      const PhaseConst& P = A.tabs[r].p[A.ephase[ge]];  // read where used: fewer live registers
       const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
       T epsp[6];
       T qb = (T)0, ga = (T)0;
@@ -234,13 +214,12 @@ __global__ void __launch_bounds__(kBlock)
         for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
       }
       PointOut<T> o;
-      PointExtra<T> ex;
-      const int st = material_point<T, EXTRA>(P, S, dt, eps, EXTRA ? rate : nullptr, epsp, qb, ga,
-                                              o, EXTRA ? &ex : nullptr, nullptr);
+      const int st = material_point<T, false>(P, S, dt, eps, nullptr, epsp, qb, ga, o, nullptr,
+                                              nullptr);
       if (st != CIA_PT_OK && !A.force) atomicMin(&c.first_fail, (unsigned long long)p);
       T* sig = (T*)A.sig[slot];
       T* tan = (T*)A.tan[slot];
-      T* st_t = EXTRA ? next_state : (T*)A.st[slot];
+      T* st_t = (T*)A.st[slot];
 #pragma unroll
       for (int m = 0; m < 6; ++m) sig[m * NPT + gp] = o.sig[m];
       if (j2) {
@@ -248,34 +227,12 @@ __global__ void __launch_bounds__(kBlock)
         for (int m = 0; m < 6; ++m) st_t[m * NPT + gp] = o.epsp[m];
         st_t[6 * NPT + gp] = o.qbar;
         st_t[7 * NPT + gp] = o.gam;
-      } else if (EXTRA) {
-        // elastic points carry no state: the caller's rows are zero-filled
-#pragma unroll
-        for (int m = 0; m < 8; ++m) st_t[m * NPT + gp] = (T)0;
       }
       tan[0 * NPT + gp] = o.tan.d;
       tan[1 * NPT + gp] = o.tan.r;
       if (o.tan.r != (T)0) {
 #pragma unroll
         for (int m = 0; m < 6; ++m) tan[(2 + m) * NPT + gp] = o.tan.n[m];
-      }
-      if (EXTRA) {
-        double mw = 0.0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) {
-          acc[BR_SIG + m] = (double)o.sig[m];
-          mw += (double)o.sig[m] * (dt * (double)rate[m]);
-        }
-        acc[BR_PSI] = (double)ex.psi;
-        acc[BR_DISS] = (double)ex.diss;
-        ((T*)A.pd)[gp] = ex.psi;
-        ((T*)A.pd)[NPT + gp] = ex.diss;
-        acc[BR_MICRO] = mw;
-        const double pw = (double)ex.d[CIA_D_INCREMENTAL_WORK_DENSITY];
-        acc[BR_POS] = fmax(pw, 0.0);
-        acc[BR_NEG] = fmin(pw, 0.0);
-        acc[BR_UNLOAD] = pw < 0.0 ? 1.0 : 0.0;
-        acc[BR_ACTIVE] = (double)ex.d[CIA_D_VISCOPLASTIC_ACTIVE] > 0.5 ? 1.0 : 0.0;
       }
       T f[8][3];
       force_from(o.sig, (T)D.g.jw, bx, by, bz, f);
@@ -284,20 +241,178 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * q + m) * ET + ge] = fin[m];
     }
-    if (EXTRA) {
-      const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();  // cw is reused by the next tile
+  }
+}
+
+// Final pass of the converged RVEs, without a second constitutive update: the
+// assembly at the converged fluctuation (slot cur) already holds the stresses,
+// the compact tangent and the next state, so this pass only forms the strain
+// and strain rate, the energies and the increment diagnostics of the reference's
+// material point (plasticity.py:1312-1315, 588, 662-672) from the stored values
+// -- the plastic strain increment and multiplier as differences of the next and
+// the committed state, the viscoplastic flag from the tangent's rank-one
+// coefficient (nonzero exactly on the plastic branch) -- copies the next state
+// out (elastic points: zero rows) and reduces the response sums.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+    k_b_final(Arena A, const T* __restrict__ state_c, T* __restrict__ next_state,
+              const double* __restrict__ wc) {
+  __shared__ T cw[kBlock][6];  // corner values, already rounded to T as strain_from rounds them
+  __shared__ double red[kBlock / 64][B_NRESP];
+  const int64_t NPT = A.NPT;
+  CIA_TILES(A.T[F_GP]) {
+    const int r = A.t2r[F_GP][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (c.state != S_DONE) continue;
+    const int slot = c.cur;
+    const double* w = A.w[slot];
+    const Req& M = A.req[r];
+    const double dt = M.dt;
+    const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;
+    const int64_t e = p >> 3;
+    const int q = (int)(p & 7);
+    const bool valid = e < D.E;
+    if (valid) {
+      const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
 #pragma unroll
-      for (int s = 0; s < B_NRESP; ++s) {
-        const double v = wave_sum(acc[s]);
-        if (lane == 0) red[wid][s] = v;
+      for (int m = 0; m < 3; ++m) {
+        const double wv = w[3 * cq + m];
+        cw[threadIdx.x][m] = (T)wv;
+        cw[threadIdx.x][3 + m] = (T)(wv - wc[3 * cq + m]);
       }
-      __syncthreads();
-      if (threadIdx.x < B_NRESP) {
-        double v = 0.0;
+    }
+    __syncthreads();
+    double acc[B_NRESP];
 #pragma unroll
-        for (int u = 0; u < kBlock / 64; ++u) v += red[u][threadIdx.x];
-        A.part[(P_RESP + threadIdx.x) * A.Tmax + t] = v;
+    for (int s = 0; s < B_NRESP; ++s) acc[s] = 0.0;
+    if (valid) {
+      const int64_t gp = 8 * D.e0 + p;
+      T eps[6], rate[6];
+      {
+        // strain_from of the corner values, streamed corner by corner (the same
+        // products and summation order, fewer live registers)
+        T wx[2], wy[2], wz[2];
+        gp_weights<T>(q, wx, wy, wz);
+        const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
+        T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, g1[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        const int base = threadIdx.x & ~7;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          T bx, by, bz;
+          corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            const T u = cw[base + a][m], du = cw[base + a][3 + m];
+            g0[m][0] += bx * u;
+            g0[m][1] += by * u;
+            g0[m][2] += bz * u;
+            g1[m][0] += bx * du;
+            g1[m][1] += by * du;
+            g1[m][2] += bz * du;
+          }
+        }
+        eps[0] = g0[0][0];
+        eps[1] = g0[1][1];
+        eps[2] = g0[2][2];
+        eps[3] = g0[1][2] + g0[2][1];
+        eps[4] = g0[0][2] + g0[2][0];
+        eps[5] = g0[0][1] + g0[1][0];
+        rate[0] = g1[0][0];
+        rate[1] = g1[1][1];
+        rate[2] = g1[2][2];
+        rate[3] = g1[1][2] + g1[2][1];
+        rate[4] = g1[0][2] + g1[2][0];
+        rate[5] = g1[0][1] + g1[1][0];
       }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        eps[m] += (T)M.e[m];
+        rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
+      }
+      const PhaseConst& P = A.tabs[r].p[A.ephase[D.e0 + e]];
+      const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
+      const T* sig = (const T*)A.sig[slot];
+      const T* stn = (const T*)A.st[slot];
+      T s[6], ep[6], ep0[6], qb = (T)0, qb0 = (T)0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s[m] = sig[m * NPT + gp];
+      if (j2) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          ep[m] = stn[m * NPT + gp];
+          ep0[m] = state_c[m * NPT + gp];
+        }
+        qb = stn[6 * NPT + gp];
+        qb0 = state_c[6 * NPT + gp];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = stn[m * NPT + gp];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) ep[m] = ep0[m] = (T)0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = (T)0;
+      }
+      const bool active = j2 && ((const T*)A.tan[slot])[1 * NPT + gp] != (T)0;
+      // free energy: elastic strain energy + hardening storage
+      T ee[6], cee[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) ee[m] = eps[m] - ep[m];
+      elastic_stress(P, ee, cee);
+      T psi = (T)0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) psi += cee[m] * ee[m];
+      psi = (T)0.5 * psi + (T)0.5 * (T)P.H * qb * qb;
+      // dissipation: plastic power minus hardening storage rate
+      T diss = (T)0;
+      if (active) {
+        T pwork = (T)0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pwork += s[m] * (ep[m] - ep0[m]);
+        diss = pwork / (T)dt - (T)P.H * qb * (qb - qb0) / (T)dt;
+      }
+      // incremental work density: trapezoid between the elastic trial at the
+      // previous strain (committed plastic strain) and the converged stress
+      T de[6], pe[6], ps[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        de[m] = (T)dt * rate[m];
+        pe[m] = eps[m] - de[m] - ep0[m];
+      }
+      elastic_stress(P, pe, ps);
+      T wsum = (T)0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) wsum += (ps[m] + s[m]) * de[m];
+      const double pw = (double)((T)0.5 * wsum);
+      ((T*)A.pd)[gp] = psi;
+      ((T*)A.pd)[NPT + gp] = diss;
+      double mw = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        acc[BR_SIG + m] = (double)s[m];
+        mw += (double)s[m] * (dt * (double)rate[m]);
+      }
+      acc[BR_PSI] = (double)psi;
+      acc[BR_DISS] = (double)diss;
+      acc[BR_MICRO] = mw;
+      acc[BR_POS] = fmax(pw, 0.0);
+      acc[BR_NEG] = fmin(pw, 0.0);
+      acc[BR_UNLOAD] = pw < 0.0 ? 1.0 : 0.0;
+      acc[BR_ACTIVE] = active ? 1.0 : 0.0;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int s2 = 0; s2 < B_NRESP; ++s2) {
+      const double v = wave_sum(acc[s2]);
+      if (lane == 0) red[wid][s2] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < B_NRESP) {
+      double v = 0.0;
+#pragma unroll
+      for (int u = 0; u < kBlock / 64; ++u) v += red[u][threadIdx.x];
+      A.part[(P_RESP + threadIdx.x) * A.Tmax + t] = v;
     }
     __syncthreads();  // cw / red are reused by the next tile
   }
@@ -1176,14 +1291,9 @@ template <class T>
 struct BOps {
   using SC = typename SpecOf<T>::type;
   static void assemble(cia_rve_batch* B, hipStream_t st, const Settings& S, int want,
-                       const void* state_c, void* next_state, const double* wc, bool extra) {
-    const unsigned g = kgrid(B, K_ASM, F_GP);
-    if (extra)
-      hipLaunchKernelGGL((k_b_assemble<T, true>), dim3(g), dim3(kBlock), 0, st, B->A, S, want,
-                         (const T*)state_c, (T*)next_state, wc);
-    else
-      hipLaunchKernelGGL((k_b_assemble<T, false>), dim3(g), dim3(kBlock), 0, st, B->A, S, want,
-                         (const T*)state_c, (T*)nullptr, wc);
+                       const void* state_c, const double* wc) {
+    hipLaunchKernelGGL(k_b_assemble<T>, dim3(kgrid(B, K_ASM, F_GP)), dim3(kBlock), 0, st, B->A, S,
+                       want, (const T*)state_c, wc);
   }
   static void gather(cia_rve_batch* B, hipStream_t st, int want) {
     hipLaunchKernelGGL(k_b_gather<T>, dim3(kgrid(B, K_GATHER, F_CL)), dim3(kBlock), 0, st, B->A,
@@ -1289,7 +1399,7 @@ cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const v
     if (s == CIA_OK) s = x;
   };
   run(tune_one(B, st, K_ASM, F_GP, [&](hipStream_t q) {
-    BOps<T>::assemble(B, q, S, S_NEWTON, state_c, nullptr, wc, false);
+    BOps<T>::assemble(B, q, S, S_NEWTON, state_c, wc);
   }));
   run(tune_one(B, st, K_GATHER, F_CL, [&](hipStream_t q) { BOps<T>::gather(B, q, S_NEWTON); }));
   run(tune_one(B, st, K_ELEM, F_GP, [&](hipStream_t q) {
@@ -1343,7 +1453,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   const unsigned gcl = kgrid(B, K_UPD, F_CL);
   hipLaunchKernelGGL(k_b_begin, dim3(gcl), dim3(kBlock), 0, st, B->A, w0 ? w0 : wc);
   hipLaunchKernelGGL(k_b_ctl_init, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, st, B->A);
-  BOps<T>::assemble(B, st, S, S_NEWTON, state_c, nullptr, wc, false);
+  BOps<T>::assemble(B, st, S, S_NEWTON, state_c, wc);
   BOps<T>::gather(B, st, S_NEWTON);
   CIA_HIP(hipGetLastError());
   if (!B->tuned) {
@@ -1410,14 +1520,15 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
       CIA_TRY(read_counts(B, st, cnt));
       if (cnt[S_LS] == 0) break;
       hipLaunchKernelGGL(k_b_axpy, dim3(gcl), dim3(kBlock), 0, st, B->A);
-      BOps<T>::assemble(B, st, S, S_LS, state_c, nullptr, wc, false);
+      BOps<T>::assemble(B, st, S, S_LS, state_c, wc);
       BOps<T>::gather(B, st, S_LS);
       hipLaunchKernelGGL(k_b_ctl_ls, dim3(gw), dim3(kBlock), 0, st, B->A, N);
       CIA_HIP(hipGetLastError());
     }
   }
   // converged RVEs: final pass (energies, next state, response sums), next w
-  BOps<T>::assemble(B, st, S, S_DONE, state_c, next_state, wc, true);
+  hipLaunchKernelGGL(k_b_final<T>, dim3(kgrid(B, K_ASM, F_GP)), dim3(kBlock), 0, st, B->A,
+                     (const T*)state_c, (T*)next_state, wc);
   hipLaunchKernelGGL(k_b_phase_sums<T>, dim3(gcl), dim3(kBlock), 0, st, B->A);
   hipLaunchKernelGGL(k_b_resp, dim3(gw), dim3(kBlock), 0, st, B->A, B->resp);
   hipLaunchKernelGGL(k_b_copy_out, dim3(gcl), dim3(kBlock), 0, st, B->A, next_w);
