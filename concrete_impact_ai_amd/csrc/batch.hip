@@ -134,6 +134,9 @@ struct Arena {
   double* py;
   void* uab;     // packed spectra [2 ET] (double2 for FP64 batches, float2 for FP32)
   double* part;  // [P_ROWS][Tmax]
+  int* act[NF];  // active-tile lists of the PCG kernels, [T[f]] each
+  int* n_act;    // [NF] list lengths
+  int* act_off;  // [NF][R] offset of every RVE's tiles in the lists
 };
 
 struct NewtonP {
@@ -151,6 +154,75 @@ struct NewtonP {
 
 __device__ __forceinline__ bool pcg_active(const Arena& A, const Ctl& c) {
   return A.force || (c.state == S_PCG && c.p_done == 0);
+}
+
+// The PCG kernels walk only the listed tiles: those of the RVEs whose PCG was
+// running at the last compaction (every tile while the autotuner forces all RVEs
+// active), in rounds of gridDim.x as CIA_TILES walks all tiles.  RVEs only stop
+// during a PCG solve, so the list is a superset of the active tiles and a listed
+// tile still checks its RVE; results do not depend on which block takes a tile.
+#define CIA_ACTIVE_TILES(FAM)                                                             \
+  for (int base_ = 0, n_ = A.n_act[FAM]; base_ < n_; base_ += (int)gridDim.x)            \
+    for (int i_ = base_ + (int)xcd_block(), once_ = 0; once_ == 0 && i_ < n_; once_ = 1) \
+      for (int t = A.act[FAM][i_], one_ = 0; one_ == 0; one_ = 1)
+
+// Active-tile lists, step 1 (one block): per family, the offset of every RVE's
+// tiles (exclusive scan over the RVEs with a running PCG, in RVE order) and the
+// list lengths.
+__global__ void __launch_bounds__(1024) k_b_compact_scan(Arena A) {
+  __shared__ int wsum[16][NF];
+  __shared__ int carry[NF];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid < NF) carry[tid] = 0;
+  __syncthreads();
+  for (int r0 = 0; r0 < A.R; r0 += 1024) {
+    const int r = r0 + tid;
+    const bool on = r < A.R && pcg_active(A, A.ctl[r]);
+    int v[NF], x[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) x[f] = v[f] = on ? A.desc[r].nt[f] : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const int y = __shfl_up(x[f], d, 64);
+        if (lane >= d) x[f] += y;
+      }
+    }
+    if (lane == 63) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) wsum[wid][f] = x[f];
+    }
+    __syncthreads();
+    if (r < A.R) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        int pre = carry[f];
+        for (int u = 0; u < wid; ++u) pre += wsum[u][f];
+        A.act_off[f * A.R + r] = pre + x[f] - v[f];
+      }
+    }
+    __syncthreads();
+    if (tid < NF) {
+      int tot = 0;
+      for (int u = 0; u < 16; ++u) tot += wsum[u][tid];
+      carry[tid] += tot;
+    }
+    __syncthreads();
+  }
+  if (tid < NF) A.n_act[tid] = carry[tid];
+}
+
+// Active-tile lists, step 2 (one block per RVE): the tile ids of the RVE.
+__global__ void __launch_bounds__(kBlock) k_b_compact_fill(Arena A) {
+  const int r = blockIdx.x;
+  if (!pcg_active(A, A.ctl[r])) return;
+  const Desc& D = A.desc[r];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    int* out = A.act[f] + A.act_off[f * A.R + r];
+    for (int j = threadIdx.x; j < D.nt[f]; j += blockDim.x) out[j] = D.t0[f] + j;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -715,7 +787,7 @@ __global__ void __launch_bounds__(kBlock) k_b_scal_pq(Arena A) {
 template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
   __shared__ double lds[kBlock / 64];
-  CIA_TILES(A.T[F_CL]) {
+  CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
@@ -763,7 +835,7 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
   const int64_t NPT = A.NPT, ET = A.ET;
   const double* p_old = A.pp[1 - pi];
   double* p_new = A.pp[pi];
-  CIA_TILES(A.T[F_GP]) {
+  CIA_ACTIVE_TILES(F_GP) {
     const int r = A.t2r[F_GP][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
@@ -838,7 +910,7 @@ __global__ void __launch_bounds__(kBlock) k_b_node(Arena A, int pi) {
   const int64_t ET = A.ET;
   const T* F = (const T*)A.F;
   const double* p = A.pp[pi];
-  CIA_TILES(A.T[F_CL]) {
+  CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     if (!pcg_active(A, A.ctl[r])) continue;
@@ -874,7 +946,7 @@ template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
   __shared__ double lds[kBlock / 64];
   const double* p = A.pp[pi];
-  CIA_TILES(A.T[F_CL]) {
+  CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
@@ -913,7 +985,7 @@ __global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
   extern __shared__ __align__(16) unsigned char dft_raw[];
   SC* dft_sm = reinterpret_cast<SC*>(dft_raw);
   __shared__ SC wt[kDftMaxTw];
-  CIA_TILES(A.T[F_DX]) {
+  CIA_ACTIVE_TILES(F_DX) {
     const int r = A.t2r[F_DX][t];
     const Desc& D = A.desc[r];
     if (!pcg_active(A, A.ctl[r])) continue;
@@ -950,7 +1022,7 @@ __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
   SC* dft_sm = reinterpret_cast<SC*>(dft_raw);
   __shared__ SC wt[kDftMaxTw];
   constexpr int FAM = ZPASS ? F_DZ : F_DY;
-  CIA_TILES(A.T[FAM]) {
+  CIA_ACTIVE_TILES(FAM) {
     const int r = A.t2r[FAM][t];
     const Desc& D = A.desc[r];
     if (!pcg_active(A, A.ctl[r])) continue;
@@ -1312,6 +1384,11 @@ struct BOps {
     }
   }
   static unsigned waves(cia_rve_batch* B) { return (unsigned)((B->R + 3) / 4); }
+  // Rebuild the active-tile lists of the PCG kernels.
+  static void compact(cia_rve_batch* B, hipStream_t st) {
+    hipLaunchKernelGGL(k_b_compact_scan, dim3(1), dim3(1024), 0, st, B->A);
+    hipLaunchKernelGGL(k_b_compact_fill, dim3((unsigned)B->R), dim3(kBlock), 0, st, B->A);
+  }
   // One PCG iteration (pi selects p_new = pp[pi], p_old = pp[1 - pi]).
   static void iteration(cia_rve_batch* B, hipStream_t st, int pi) {
     const unsigned gw = waves(B);
@@ -1320,6 +1397,7 @@ struct BOps {
     hipLaunchKernelGGL(k_b_scal_pq, dim3(gw), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_update<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A, pi);
     hipLaunchKernelGGL(k_b_scal_upd<false>, dim3(gw), dim3(kBlock), 0, st, B->A);
+    compact(B, st);
     dft(B, st, false);
     hipLaunchKernelGGL(k_b_apply<SC>, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_rz, dim3(gw), dim3(kBlock), 0, st, B->A);
@@ -1329,6 +1407,7 @@ struct BOps {
     const unsigned gw = waves(B);
     hipLaunchKernelGGL(k_b_sp_init<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_upd<true>, dim3(gw), dim3(kBlock), 0, st, B->A);
+    compact(B, st);
     dft(B, st, false);
     hipLaunchKernelGGL(k_b_apply<SC>, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_rz, dim3(gw), dim3(kBlock), 0, st, B->A);
@@ -1348,13 +1427,17 @@ cia_status read_counts(cia_rve_batch* B, hipStream_t st, int out[7]) {
 
 // Time `launch` with grid candidates; keep the fastest.
 template <class L>
-cia_status tune_one(cia_rve_batch* B, hipStream_t st, int k, int fam, L&& launch) {
+cia_status tune_one(cia_rve_batch* B, hipStream_t st, int k, int fam, L&& launch,
+                    bool persistent = false) {
+  // PCG kernels (persistent = true) keep to persistent grids over the active-tile
+  // list: most of their launches come when part of the RVEs have converged, and a
+  // one-tile-per-block grid would dispatch every tile of the batch every time.
   const int T = B->T[fam];
   std::vector<int> cand;
-  cand.push_back(T);  // one tile per block
-  for (int m : {1, 2, 4, 8, 16}) {
+  if (!persistent || T <= 32 * B->cus) cand.push_back(T);  // one tile per block
+  for (int m : {1, 2, 4, 8, 16, 32}) {
     const int g = m * B->cus;
-    if (g < T) cand.push_back(g);
+    if (g < T && (persistent || m <= 16)) cand.push_back(g);
   }
   hipEvent_t e0, e1;
   CIA_HIP(hipEventCreate(&e0));
@@ -1394,6 +1477,7 @@ cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const v
                     const double* wc) {
   using SC = typename SpecOf<T>::type;
   B->A.force = 1;
+  BOps<T>::compact(B, st);  // every tile listed while all RVEs are forced active
   cia_status s = CIA_OK;
   auto run = [&](cia_status x) {
     if (s == CIA_OK) s = x;
@@ -1404,27 +1488,27 @@ cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const v
   run(tune_one(B, st, K_GATHER, F_CL, [&](hipStream_t q) { BOps<T>::gather(B, q, S_NEWTON); }));
   run(tune_one(B, st, K_ELEM, F_GP, [&](hipStream_t q) {
     hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_GP)), dim3(kBlock), 0, q, B->A, 0);
-  }));
+  }, true));
   run(tune_one(B, st, K_NODE, F_CL, [&](hipStream_t q) {
     hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, q, B->A, 0);
-  }));
+  }, true));
   run(tune_one(B, st, K_UPD, F_CL, [&](hipStream_t q) {
     hipLaunchKernelGGL(k_b_update<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, q, B->A, 0);
-  }));
+  }, true));
   run(tune_one(B, st, K_APPLY, F_CL, [&](hipStream_t q) {
     hipLaunchKernelGGL(k_b_apply<SC>, dim3(kgrid(B, K_APPLY, F_CL)), dim3(kBlock), 0, q, B->A);
-  }));
+  }, true));
   run(tune_one(B, st, K_DX, F_DX, [&](hipStream_t q) {
     hipLaunchKernelGGL((k_b_dft_x<false, SC>), dim3(kgrid(B, K_DX, F_DX)), dim3(256), B->lds_x, q, B->A);
-  }));
+  }, true));
   run(tune_one(B, st, K_DY, F_DY, [&](hipStream_t q) {
     hipLaunchKernelGGL((k_b_dft_strided<false, false, SC>), dim3(kgrid(B, K_DY, F_DY)), dim3(256),
                        B->lds_y, q, B->A);
-  }));
+  }, true));
   run(tune_one(B, st, K_DZ, F_DZ, [&](hipStream_t q) {
     hipLaunchKernelGGL((k_b_dft_strided<false, true, SC>), dim3(kgrid(B, K_DZ, F_DZ)), dim3(256),
                        B->lds_z, q, B->A);
-  }));
+  }, true));
   B->A.force = 0;
   if (s != CIA_OK) return s;
   // The timed assembly and gather recomputed slot 0, r[0] and the gather
@@ -1723,6 +1807,9 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
   A.NPT = NPT;
   A.ET = ET;
   for (int f = 0; f < NF; ++f) A.T[f] = B->T[f];
+  for (int f = 0; f < NF; ++f) BALLOC(A.act[f], std::max(1, B->T[f]) * sizeof(int));
+  BALLOC(A.n_act, NF * sizeof(int));
+  BALLOC(A.act_off, (size_t)NF * n * sizeof(int));
   Desc* d_desc = nullptr;
   BALLOC(d_desc, n * sizeof(Desc));
   for (int f = 0; f < NF; ++f) {
