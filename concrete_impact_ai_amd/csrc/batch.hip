@@ -229,8 +229,8 @@ __global__ void __launch_bounds__(kBlock) k_b_compact_fill(Arena A) {
 // Assembly: strain -> constitutive update -> element force rows, one lane per
 // Gauss point (as k_assemble of the single-RVE solver).  `want` selects the
 // RVEs: S_NEWTON (first assembly, slot cur) or S_LS (line-search candidate,
-// slot 1 - cur).
-template <class T>
+// slot 1 - cur).  M1: every J2 phase of the batch has rate exponent 1.
+template <class T, bool M1>
 __global__ void __launch_bounds__(kBlock)
     k_b_assemble(Arena A, Settings S, int want, const T* __restrict__ state_c,
                  const double* __restrict__ wc) {
@@ -258,17 +258,34 @@ __global__ void __launch_bounds__(kBlock)
     if (valid) {
       const int64_t gp = 8 * D.e0 + p;
       const int64_t ge = D.e0 + e;
-      T bx[8], by[8], bz[8];
-      gp_factors<T>(q, D.g, bx, by, bz);
       T eps[6];
       {
-        T we[8][3];
+        // strain_from of the corner values, streamed corner by corner (the same
+        // products and summation order; the gradient factors are formed again for
+        // the forces instead of staying live across the constitutive update)
+        T wx[2], wy[2], wz[2];
+        gp_weights<T>(q, wx, wy, wz);
+        const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
+        T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         const int base = threadIdx.x & ~7;
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int a = 0; a < 8; ++a) {
+          T bx, by, bz;
+          corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
 #pragma unroll
-          for (int m = 0; m < 3; ++m) we[a][m] = cw[base + a][m];
-        strain_from(we, bx, by, bz, eps);
+          for (int m = 0; m < 3; ++m) {
+            const T u = cw[base + a][m];
+            g0[m][0] += bx * u;
+            g0[m][1] += by * u;
+            g0[m][2] += bz * u;
+          }
+        }
+        eps[0] = g0[0][0];
+        eps[1] = g0[1][1];
+        eps[2] = g0[2][2];
+        eps[3] = g0[1][2] + g0[2][1];
+        eps[4] = g0[0][2] + g0[2][0];
+        eps[5] = g0[0][1] + g0[1][0];
       }
 #pragma unroll
       for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
@@ -286,8 +303,8 @@ __global__ void __launch_bounds__(kBlock)
         for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
       }
       PointOut<T> o;
-      const int st = material_point<T, false>(P, S, dt, eps, nullptr, epsp, qb, ga, o, nullptr,
-                                              nullptr);
+      const int st = material_point<T, false, M1>(P, S, dt, eps, nullptr, epsp, qb, ga, o, nullptr,
+                                                  nullptr);
       if (st != CIA_PT_OK && !A.force) atomicMin(&c.first_fail, (unsigned long long)p);
       T* sig = (T*)A.sig[slot];
       T* tan = (T*)A.tan[slot];
@@ -306,6 +323,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int m = 0; m < 6; ++m) tan[(2 + m) * NPT + gp] = o.tan.n[m];
       }
+      T bx[8], by[8], bz[8];
+      gp_factors<T>(q, D.g, bx, by, bz);
       T f[8][3];
       force_from(o.sig, (T)D.g.jw, bx, by, bz, f);
       T fin[3];
@@ -1304,6 +1323,7 @@ struct cia_rve_batch {
   cia_linear_settings lin{};
   double hm_scale = 1.0;
   int64_t NPT = 0, ET = 0;
+  bool all_m1 = true;  // every J2 phase has rate exponent 1 (closed-form multiplier)
   std::vector<Desc> h_desc;
   std::vector<int64_t> e0;
   int T[NF] = {0};
@@ -1364,8 +1384,12 @@ struct BOps {
   using SC = typename SpecOf<T>::type;
   static void assemble(cia_rve_batch* B, hipStream_t st, const Settings& S, int want,
                        const void* state_c, const double* wc) {
-    hipLaunchKernelGGL(k_b_assemble<T>, dim3(kgrid(B, K_ASM, F_GP)), dim3(kBlock), 0, st, B->A, S,
-                       want, (const T*)state_c, wc);
+    if (B->all_m1)
+      hipLaunchKernelGGL((k_b_assemble<T, true>), dim3(kgrid(B, K_ASM, F_GP)), dim3(kBlock), 0, st,
+                         B->A, S, want, (const T*)state_c, wc);
+    else
+      hipLaunchKernelGGL((k_b_assemble<T, false>), dim3(kgrid(B, K_ASM, F_GP)), dim3(kBlock), 0, st,
+                         B->A, S, want, (const T*)state_c, wc);
   }
   static void gather(cia_rve_batch* B, hipStream_t st, int want) {
     hipLaunchKernelGGL(k_b_gather<T>, dim3(kgrid(B, K_GATHER, F_CL)), dim3(kBlock), 0, st, B->A,
@@ -1836,7 +1860,10 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
       const cia_rve_desc& d = descs[r];
       ht[r] = PhaseTable{};
       ht[r].n = d.n_phases;
-      for (int i = 0; i < d.n_phases; ++i) ht[r].p[i] = make_phase(d.phases[i]);
+      for (int i = 0; i < d.n_phases; ++i) {
+        ht[r].p[i] = make_phase(d.phases[i]);
+        if (ht[r].p[i].kind == CIA_MAT_J2_VISCOPLASTIC && ht[r].p[i].m != 1.0) B->all_m1 = false;
+      }
       if (d.element_phase) {
         for (int64_t e = 0; e < B->h_desc[r].E; ++e) {
           const int32_t v = d.element_phase[e];

@@ -123,7 +123,10 @@ __host__ __device__ __forceinline__ T tangent_entry(const Tangent<T>& c, int i, 
   return v;
 }
 
-// Yield-consistency scalar solve, plasticity.py:711-854, always FP64.
+// Yield-consistency scalar solve, plasticity.py:711-854, always FP64.  M1: the
+// caller guarantees rate exponent m = 1 (closed form only; the Newton iteration
+// for other exponents is not compiled, which keeps the point update lean).
+template <bool M1 = false>
 __host__ __device__ inline int solve_multiplier(const PhaseConst& P, const Settings& S, double dt,
                                        double f, double& dg, int& iters, double& res_out,
                                        cia_point_failure* rec) {
@@ -131,7 +134,7 @@ __host__ __device__ inline int solve_multiplier(const PhaseConst& P, const Setti
   const double ub = f / hs;
   const double tol = fmax(S.res_abs, S.res_rel * ub);
   const double ts = P.tau * P.sref;
-  if (P.m == 1.0) {
+  if (M1 || P.m == 1.0) {
     const double den = ts / dt + hs;
     const double mult = f / den;
     const double corr = f - hs * mult;
@@ -202,7 +205,7 @@ __host__ __device__ inline int solve_multiplier(const PhaseConst& P, const Setti
 
 // Full J2 viscoplastic point update.  Returns a CIA_PT_* status.
 // rate == nullptr disables the increment diagnostics (strain_rates None).
-template <class T, bool EXTRA>
+template <class T, bool EXTRA, bool M1 = false>
 __host__ __device__ inline int j2vp_point(const PhaseConst& P, const Settings& S, double dt, const T eps[6],
                                  const T* rate, const T epsp0[6], T qbar0, T gam0,
                                  PointOut<T>& o, PointExtra<T>* ex, cia_point_failure* rec) {
@@ -256,7 +259,7 @@ __host__ __device__ inline int j2vp_point(const PhaseConst& P, const Settings& S
     T n[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) n[i] = (T)1.5 * dev[i] / q;
-    const int st = solve_multiplier(P, S, dt, (double)f, dg, iters, lres, rec);
+    const int st = solve_multiplier<M1>(P, S, dt, (double)f, dg, iters, lres, rec);
     if (st != CIA_PT_OK) return st;
     const T dgT = (T)dg;
     const T g2dg = (T)(2.0 * P.G) * dgT;
@@ -269,7 +272,7 @@ __host__ __device__ inline int j2vp_point(const PhaseConst& P, const Settings& S
     }
     o.qbar = qbar0 + dgT;
     o.gam = gam0 + dgT;
-    if (P.m == 1.0) {
+    if (M1 || P.m == 1.0) {
       hv = P.tau * P.sref / dt + P.hs;
     } else {
       T fdev[6];
@@ -395,13 +398,14 @@ __host__ __device__ inline int elastic_point(const PhaseConst& P, double dt, con
   return CIA_PT_OK;
 }
 
-template <class T, bool EXTRA>
+// M1: every J2 phase the caller passes has rate exponent m = 1.
+template <class T, bool EXTRA, bool M1 = false>
 __host__ __device__ __forceinline__ int material_point(const PhaseConst& P, const Settings& S, double dt,
                                               const T eps[6], const T* rate, const T epsp0[6],
                                               T qbar0, T gam0, PointOut<T>& o, PointExtra<T>* ex,
                                               cia_point_failure* rec) {
   if (P.kind == CIA_MAT_J2_VISCOPLASTIC)
-    return j2vp_point<T, EXTRA>(P, S, dt, eps, rate, epsp0, qbar0, gam0, o, ex, rec);
+    return j2vp_point<T, EXTRA, M1>(P, S, dt, eps, rate, epsp0, qbar0, gam0, o, ex, rec);
   return elastic_point<T, EXTRA>(P, dt, eps, rate, epsp0, qbar0, gam0, o, ex);
 }
 
