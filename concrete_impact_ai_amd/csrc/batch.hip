@@ -156,9 +156,10 @@ __device__ __forceinline__ bool pcg_active(const Arena& A, const Ctl& c) {
   return A.force || (c.state == S_PCG && c.p_done == 0);
 }
 
-// The PCG kernels walk only the listed tiles: those of the RVEs whose PCG was
-// running at the last compaction (every tile while the autotuner forces all RVEs
-// active), in rounds of gridDim.x as CIA_TILES walks all tiles.  RVEs only stop
+// The PCG, assembly, gather and line-search kernels walk only the listed tiles:
+// those of the RVEs whose PCG was running (or that were in the kernel's Newton
+// state) at the last compaction, every tile while the autotuner forces all RVEs
+// active, in rounds of gridDim.x as CIA_TILES walks all tiles.  RVEs only stop
 // during a PCG solve, so the list is a superset of the active tiles and a listed
 // tile still checks its RVE; results do not depend on which block takes a tile.
 #define CIA_ACTIVE_TILES(FAM)                                                             \
@@ -166,10 +167,15 @@ __device__ __forceinline__ bool pcg_active(const Arena& A, const Ctl& c) {
     for (int i_ = base_ + (int)xcd_block(), once_ = 0; once_ == 0 && i_ < n_; once_ = 1) \
       for (int t = A.act[FAM][i_], one_ = 0; one_ == 0; one_ = 1)
 
+// RVEs whose tiles a list holds: want < 0, those with a running PCG; otherwise
+// those in Newton state `want` (the assembly, gather and line-search kernels).
+__device__ __forceinline__ bool listed(const Arena& A, const Ctl& c, int want) {
+  return want < 0 ? pcg_active(A, c) : (A.force || c.state == want);
+}
+
 // Active-tile lists, step 1 (one block): per family, the offset of every RVE's
-// tiles (exclusive scan over the RVEs with a running PCG, in RVE order) and the
-// list lengths.
-__global__ void __launch_bounds__(1024) k_b_compact_scan(Arena A) {
+// tiles (exclusive scan over the listed RVEs, in RVE order) and the list lengths.
+__global__ void __launch_bounds__(1024) k_b_compact_scan(Arena A, int want) {
   __shared__ int wsum[16][NF];
   __shared__ int carry[NF];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -177,7 +183,7 @@ __global__ void __launch_bounds__(1024) k_b_compact_scan(Arena A) {
   __syncthreads();
   for (int r0 = 0; r0 < A.R; r0 += 1024) {
     const int r = r0 + tid;
-    const bool on = r < A.R && pcg_active(A, A.ctl[r]);
+    const bool on = r < A.R && listed(A, A.ctl[r], want);
     int v[NF], x[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) x[f] = v[f] = on ? A.desc[r].nt[f] : 0;
@@ -214,9 +220,9 @@ __global__ void __launch_bounds__(1024) k_b_compact_scan(Arena A) {
 }
 
 // Active-tile lists, step 2 (one block per RVE): the tile ids of the RVE.
-__global__ void __launch_bounds__(kBlock) k_b_compact_fill(Arena A) {
+__global__ void __launch_bounds__(kBlock) k_b_compact_fill(Arena A, int want) {
   const int r = blockIdx.x;
-  if (!pcg_active(A, A.ctl[r])) return;
+  if (!listed(A, A.ctl[r], want)) return;
   const Desc& D = A.desc[r];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -236,7 +242,7 @@ __global__ void __launch_bounds__(kBlock)
                  const double* __restrict__ wc) {
   __shared__ T cw[kBlock][3];  // corner values, rounded to T as strain_from rounds them
   const int64_t NPT = A.NPT, ET = A.ET;
-  CIA_TILES(A.T[F_GP]) {
+  CIA_ACTIVE_TILES(F_GP) {
     const int r = A.t2r[F_GP][t];
     const Desc& D = A.desc[r];
     Ctl& c = A.ctl[r];
@@ -561,7 +567,7 @@ __global__ void __launch_bounds__(kBlock) k_b_gather(Arena A, int want) {
   __shared__ double lds[kBlock / 64];
   const int64_t ET = A.ET;
   const T* F = (const T*)A.F;
-  CIA_TILES(A.T[F_CL]) {
+  CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
@@ -620,7 +626,7 @@ __global__ void __launch_bounds__(kBlock) k_b_begin(Arena A, const double* __res
 
 // Line-search candidate w[1 - cur] = w[cur] + alpha dw.
 __global__ void __launch_bounds__(kBlock) k_b_axpy(Arena A) {
-  CIA_TILES(A.T[F_CL]) {
+  CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
@@ -850,7 +856,7 @@ __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
   using SC = typename SpecOf<T>::type;
-  __shared__ double cpv[kBlock][3];
+  __shared__ T cpv[kBlock][3];  // corner values of p, rounded to T as strain_from rounds them
   const int64_t NPT = A.NPT, ET = A.ET;
   const double* p_old = A.pp[1 - pi];
   double* p_new = A.pp[pi];
@@ -882,22 +888,37 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
         }
       }
 #pragma unroll
-      for (int m = 0; m < 3; ++m) cpv[threadIdx.x][m] = own[m];
+      for (int m = 0; m < 3; ++m) cpv[threadIdx.x][m] = (T)own[m];
     }
     __syncthreads();
     if (valid) {
       const int64_t gp = 8 * D.e0 + p;
       T eps[6];
-      T bx[8], by[8], bz[8];
-      gp_factors<T>(q, D.g, bx, by, bz);
       {
-        double pe[8][3];
+        // strain_from streamed corner by corner (same products and order)
+        T wx[2], wy[2], wz[2];
+        gp_weights<T>(q, wx, wy, wz);
+        const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
+        T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         const int base = threadIdx.x & ~7;
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+        for (int a = 0; a < 8; ++a) {
+          T bx, by, bz;
+          corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
 #pragma unroll
-          for (int m = 0; m < 3; ++m) pe[a][m] = cpv[base + a][m];
-        strain_from(pe, bx, by, bz, eps);
+          for (int m = 0; m < 3; ++m) {
+            const T u = cpv[base + a][m];
+            g0[m][0] += bx * u;
+            g0[m][1] += by * u;
+            g0[m][2] += bz * u;
+          }
+        }
+        eps[0] = g0[0][0];
+        eps[1] = g0[1][1];
+        eps[2] = g0[2][2];
+        eps[3] = g0[1][2] + g0[2][1];
+        eps[4] = g0[0][2] + g0[2][0];
+        eps[5] = g0[0][1] + g0[1][0];
       }
       Tangent<T> tc;
       tc.K = (T)A.tabs[r].p[A.ephase[D.e0 + e]].K;
@@ -911,6 +932,8 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
       }
       T s[6];
       tangent_apply(tc, eps, s);
+      T bx[8], by[8], bz[8];
+      gp_factors<T>(q, D.g, bx, by, bz);
       T f[8][3];
       force_from(s, (T)D.g.jw, bx, by, bz, f);
       T fin[3];
@@ -1408,10 +1431,11 @@ struct BOps {
     }
   }
   static unsigned waves(cia_rve_batch* B) { return (unsigned)((B->R + 3) / 4); }
-  // Rebuild the active-tile lists of the PCG kernels.
-  static void compact(cia_rve_batch* B, hipStream_t st) {
-    hipLaunchKernelGGL(k_b_compact_scan, dim3(1), dim3(1024), 0, st, B->A);
-    hipLaunchKernelGGL(k_b_compact_fill, dim3((unsigned)B->R), dim3(kBlock), 0, st, B->A);
+  // Rebuild the active-tile lists: want < 0 for the PCG kernels, else the
+  // Newton state of the next assembly / gather / line-search launches.
+  static void compact(cia_rve_batch* B, hipStream_t st, int want = -1) {
+    hipLaunchKernelGGL(k_b_compact_scan, dim3(1), dim3(1024), 0, st, B->A, want);
+    hipLaunchKernelGGL(k_b_compact_fill, dim3((unsigned)B->R), dim3(kBlock), 0, st, B->A, want);
   }
   // One PCG iteration (pi selects p_new = pp[pi], p_old = pp[1 - pi]).
   static void iteration(cia_rve_batch* B, hipStream_t st, int pi) {
@@ -1561,6 +1585,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   const unsigned gcl = kgrid(B, K_UPD, F_CL);
   hipLaunchKernelGGL(k_b_begin, dim3(gcl), dim3(kBlock), 0, st, B->A, w0 ? w0 : wc);
   hipLaunchKernelGGL(k_b_ctl_init, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, st, B->A);
+  BOps<T>::compact(B, st, S_NEWTON);
   BOps<T>::assemble(B, st, S, S_NEWTON, state_c, wc);
   BOps<T>::gather(B, st, S_NEWTON);
   CIA_HIP(hipGetLastError());
@@ -1627,6 +1652,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
     for (;;) {
       CIA_TRY(read_counts(B, st, cnt));
       if (cnt[S_LS] == 0) break;
+      BOps<T>::compact(B, st, S_LS);
       hipLaunchKernelGGL(k_b_axpy, dim3(gcl), dim3(kBlock), 0, st, B->A);
       BOps<T>::assemble(B, st, S, S_LS, state_c, wc);
       BOps<T>::gather(B, st, S_LS);
