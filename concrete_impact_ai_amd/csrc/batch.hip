@@ -1077,24 +1077,28 @@ __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
     const int o = rest % no, comp = rest / no;
     SC* X = dft_sm;
     SC* Y = dft_sm + kDftTile * ld;
-    const int i0 = xt * kDftTile;
+    // the tiles_x x-tiles of a row share the nx positions evenly (at most 16
+    // each), so an edge that is not a multiple of 16 leaves no near-empty tile
+    const int tw = (D.nx + tiles_x - 1) / tiles_x;
+    const int i0 = xt * tw;
     const int64_t plane = (int64_t)D.nx * D.ny;
     const int64_t stride = ZPASS ? plane : D.nx;
     SC* g = (SC*)A.uab + 2 * D.e0 + (int64_t)comp * D.E + i0 + (ZPASS ? (int64_t)D.nx * o : plane * o);
-    const int width = min(kDftTile, D.nx - i0);
+    const int width = min(tw, D.nx - i0);
+    const float inv_w = 1.0f / (float)width;
     load_twiddles(ax, wt);
-    for (int u = threadIdx.x; u < kDftTile * n; u += blockDim.x) {
-      const int e = u / kDftTile, ii = u - e * kDftTile;
-      X[ii * ld + e] = ii < width ? g[ii + stride * e] : mk2<SC>(0.0, 0.0);
+    for (int u = threadIdx.x; u < width * n; u += blockDim.x) {
+      const int e = fdiv(u, inv_w), ii = u - e * width;
+      X[ii * ld + e] = g[ii + stride * e];
     }
     __syncthreads();
-    dft_stage<true, INV>(X, Y, kDftTile, ld, 1, ax, wt);
+    dft_stage<true, INV>(X, Y, width, ld, 1, ax, wt);
     __syncthreads();
-    dft_stage<false, INV>(Y, X, kDftTile, ld, 1, ax, wt);
+    dft_stage<false, INV>(Y, X, width, ld, 1, ax, wt);
     __syncthreads();
-    for (int u = threadIdx.x; u < kDftTile * n; u += blockDim.x) {
-      const int e = u / kDftTile, ii = u - e * kDftTile;
-      if (ii < width) g[ii + stride * e] = X[ii * ld + e];
+    for (int u = threadIdx.x; u < width * n; u += blockDim.x) {
+      const int e = fdiv(u, inv_w), ii = u - e * width;
+      g[ii + stride * e] = X[ii * ld + e];
     }
     __syncthreads();
   }
