@@ -222,36 +222,30 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
     status, res, next_packed, next_w = solve_batch_raw(
         batch, committed.packed, committed.w, committed.macro, macro, dts, update_settings,
         initial_w)
-    reasons = tuple(_lib.RVE_REASONS[r.reason] for r in res)
+    a = np.frombuffer(res, dtype=_RESULT_DTYPE)  # the cia_rve_result array as columns
+    reasons = tuple(_REASONS[a["reason"]])
     if status == _lib.CIA_ERR_RVE_FAILURE and raise_on_failure:
         i = next(k for k, r in enumerate(reasons) if r != "ok")
         raise_failure(batch, i, res[i], macro[i], float(dts[i]), update_settings)
-    diag = {
-        "micro_residual_norm": [r.micro_residual_norm for r in res],
-        "micro_residual_tolerance": [r.micro_residual_tolerance for r in res],
-        "newton_iterations": [r.newton_iterations for r in res],
-        "armijo_backtracks": [r.armijo_backtracks for r in res],
-        "volume": [r.volume for r in res],
-        "macro_work_increment": [r.macro_work_increment for r in res],
-        "micro_work_increment": [r.micro_work_increment for r in res],
-        "hill_mandel_residual": [r.hill_mandel_residual for r in res],
-        "hill_mandel_error": [r.hill_mandel_error for r in res],
-        "positive_work_density": [r.positive_work_density for r in res],
-        "negative_work_density": [r.negative_work_density for r in res],
-        "unloading_volume_fraction": [r.unloading_volume_fraction for r in res],
-        "viscoplastic_active_volume_fraction": [r.viscoplastic_active_volume_fraction for r in res],
-        "linear_iterations": [r.linear_iterations for r in res],
-        "gauss_point_updates": [r.gp_updates for r in res],
-    }
-    diag = {k: np.asarray(v) for k, v in diag.items()}
+    diag = {key: a[field].copy() for key, field in _DIAG_FIELDS}
     return RVEBatchResponse(
-        macro_stress=np.asarray([r.macro_stress[:] for r in res], dtype=np.float64),
-        free_energy_density=np.asarray([r.free_energy_density for r in res]),
-        dissipation_density=np.asarray([r.dissipation_density for r in res]),
+        macro_stress=a["macro_stress"].astype(np.float64),
+        free_energy_density=a["free_energy_density"].copy(),
+        dissipation_density=a["dissipation_density"].copy(),
         diagnostics=diag,
         reasons=reasons,
         state=RVEBatchState(batch, next_packed, next_w, macro.copy()),
     )
+
+
+_RESULT_DTYPE = np.dtype(_lib.RVEResult)
+_REASONS = np.asarray(_lib.RVE_REASONS, dtype=object)
+_DIAG_FIELDS = tuple((k, k) for k in (
+    "micro_residual_norm", "micro_residual_tolerance", "newton_iterations", "armijo_backtracks",
+    "volume", "macro_work_increment", "micro_work_increment", "hill_mandel_residual",
+    "hill_mandel_error", "positive_work_density", "negative_work_density",
+    "unloading_volume_fraction", "viscoplastic_active_volume_fraction",
+    "linear_iterations")) + (("gauss_point_updates", "gp_updates"),)
 
 
 def solve_batch_raw(batch: RVEBatch, packed: torch.Tensor, w: torch.Tensor, committed_macro,
@@ -327,7 +321,8 @@ class FluctuationPredictor:
 
     def __init__(self, batch: RVEBatch):
         counts = np.diff(batch.element_offsets) * 3
-        self._rep = torch.as_tensor(counts, device=device())
+        # RVE index of every fluctuation entry (expanded once: no host sync later)
+        self._owner = torch.as_tensor(np.repeat(np.arange(len(counts)), counts), device=device())
         self._w_prev = None
         self._de_prev = None
 
@@ -339,7 +334,7 @@ class FluctuationPredictor:
             s = np.where(den > 0.0, np.einsum("ij,ij->i", de, self._de_prev) / np.maximum(den, 1e-300),
                          0.0)
             s = np.clip(s, 0.0, 2.0)
-            scale = torch.repeat_interleave(torch.as_tensor(s, device=self._rep.device), self._rep)
+            scale = torch.as_tensor(s, device=self._owner.device)[self._owner]
             w0 = committed.w + scale * (committed.w - self._w_prev)
         self._w_prev = committed.w
         self._de_prev = de

@@ -92,10 +92,31 @@ __device__ __forceinline__ unsigned xcd_block() {
 
 // ---- deterministic block reductions --------------------------------------
 
+// DPP move of a double (both halves) with control CTRL, full row and bank masks.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront: DPP moves on the vector ALU inside each
+// row of 16 lanes (swap neighbours, swap pairs, mirror the halves of 8, mirror the
+// row) leave every lane with its row sum; the four row sums are read from lanes
+// 0, 16, 32, 48.  No LDS traffic, a fixed summation order; every lane returns the
+// total.  All 64 lanes must be active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 __device__ __forceinline__ double wave_max(double v) {
