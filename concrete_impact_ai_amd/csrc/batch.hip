@@ -137,6 +137,9 @@ struct Arena {
   int* act[NF];  // active-tile lists of the PCG kernels, [T[f]] each
   int* n_act;    // [NF] list lengths
   int* act_off;  // [NF][R] offset of every RVE's tiles in the lists
+  const int* grp;  // leader tiles of the final pass's Gauss-point tile groups
+  int n_grp;
+  int pad2;
 };
 
 struct NewtonP {
@@ -350,153 +353,167 @@ __global__ void __launch_bounds__(kBlock)
 // -- the plastic strain increment and multiplier as differences of the next and
 // the committed state, the viscoplastic flag from the tangent's rank-one
 // coefficient (nonzero exactly on the plastic branch) -- copies the next state
-// out (elastic points: zero rows) and reduces the response sums.
+// out (elastic points: zero rows) and reduces the response sums.  A block takes
+// groups of kFinalGroup consecutive Gauss-point tiles of one RVE (the leader
+// tiles listed in A.grp): each lane accumulates its points over the group and
+// the group's sums go to the leader tile's partials (zeros to the others), one
+// reduction per group instead of per tile, independent of the grid.
+constexpr int kFinalGroup = 16;
+
 template <class T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 2)
     k_b_final(Arena A, const T* __restrict__ state_c, T* __restrict__ next_state,
               const double* __restrict__ wc) {
   __shared__ T cw[kBlock][6];  // corner values, already rounded to T as strain_from rounds them
   __shared__ double red[kBlock / 64][B_NRESP];
   const int64_t NPT = A.NPT;
-  CIA_TILES(A.T[F_GP]) {
-    const int r = A.t2r[F_GP][t];
+  for (int base_ = 0; base_ < A.n_grp; base_ += (int)gridDim.x) {
+    const int gi = base_ + (int)xcd_block();
+    if (gi >= A.n_grp) break;
+    const int tl = A.grp[gi];
+    const int r = A.t2r[F_GP][tl];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
     if (c.state != S_DONE) continue;
+    const int tend = min(tl + kFinalGroup, D.t0[F_GP] + D.nt[F_GP]);
     const int slot = c.cur;
     const double* w = A.w[slot];
     const Req& M = A.req[r];
     const double dt = M.dt;
-    const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;
-    const int64_t e = p >> 3;
-    const int q = (int)(p & 7);
-    const bool valid = e < D.E;
-    if (valid) {
-      const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const double wv = w[3 * cq + m];
-        cw[threadIdx.x][m] = (T)wv;
-        cw[threadIdx.x][3 + m] = (T)(wv - wc[3 * cq + m]);
-      }
-    }
-    __syncthreads();
+    const T* sig = (const T*)A.sig[slot];
+    const T* stn = (const T*)A.st[slot];
+    const T* tanr = (const T*)A.tan[slot] + NPT;
     double acc[B_NRESP];
 #pragma unroll
-    for (int s = 0; s < B_NRESP; ++s) acc[s] = 0.0;
-    if (valid) {
-      const int64_t gp = 8 * D.e0 + p;
-      T eps[6], rate[6];
-      {
-        // strain_from of the corner values, streamed corner by corner (the same
-        // products and summation order, fewer live registers)
-        T wx[2], wy[2], wz[2];
-        gp_weights<T>(q, wx, wy, wz);
-        const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
-        T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, g1[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        const int base = threadIdx.x & ~7;
+    for (int s2 = 0; s2 < B_NRESP; ++s2) acc[s2] = 0.0;
+    for (int t = tl; t < tend; ++t) {
+      const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;
+      const int64_t e = p >> 3;
+      const int q = (int)(p & 7);
+      const bool valid = e < D.E;
+      if (valid) {
+        const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          T bx, by, bz;
-          corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            const T u = cw[base + a][m], du = cw[base + a][3 + m];
-            g0[m][0] += bx * u;
-            g0[m][1] += by * u;
-            g0[m][2] += bz * u;
-            g1[m][0] += bx * du;
-            g1[m][1] += by * du;
-            g1[m][2] += bz * du;
-          }
+        for (int m = 0; m < 3; ++m) {
+          const double wv = w[3 * cq + m];
+          cw[threadIdx.x][m] = (T)wv;
+          cw[threadIdx.x][3 + m] = (T)(wv - wc[3 * cq + m]);
         }
-        eps[0] = g0[0][0];
-        eps[1] = g0[1][1];
-        eps[2] = g0[2][2];
-        eps[3] = g0[1][2] + g0[2][1];
-        eps[4] = g0[0][2] + g0[2][0];
-        eps[5] = g0[0][1] + g0[1][0];
-        rate[0] = g1[0][0];
-        rate[1] = g1[1][1];
-        rate[2] = g1[2][2];
-        rate[3] = g1[1][2] + g1[2][1];
-        rate[4] = g1[0][2] + g1[2][0];
-        rate[5] = g1[0][1] + g1[1][0];
       }
+      __syncthreads();
+      if (valid) {
+        const int64_t gp = 8 * D.e0 + p;
+        T eps[6], rate[6];
+        {
+          // strain_from of the corner values, streamed corner by corner (the same
+          // products and summation order, fewer live registers)
+          T wx[2], wy[2], wz[2];
+          gp_weights<T>(q, wx, wy, wz);
+          const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
+          T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, g1[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+          const int base = threadIdx.x & ~7;
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        eps[m] += (T)M.e[m];
-        rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
-      }
-      const PhaseConst& P = A.tabs[r].p[A.ephase[D.e0 + e]];
-      const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
-      const T* sig = (const T*)A.sig[slot];
-      const T* stn = (const T*)A.st[slot];
-      T s[6], ep[6], ep0[6], qb = (T)0, qb0 = (T)0;
+          for (int a = 0; a < 8; ++a) {
+            T bx, by, bz;
+            corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
 #pragma unroll
-      for (int m = 0; m < 6; ++m) s[m] = sig[m * NPT + gp];
-      if (j2) {
+            for (int m = 0; m < 3; ++m) {
+              const T u = cw[base + a][m], du = cw[base + a][3 + m];
+              g0[m][0] += bx * u;
+              g0[m][1] += by * u;
+              g0[m][2] += bz * u;
+              g1[m][0] += bx * du;
+              g1[m][1] += by * du;
+              g1[m][2] += bz * du;
+            }
+          }
+          eps[0] = g0[0][0];
+          eps[1] = g0[1][1];
+          eps[2] = g0[2][2];
+          eps[3] = g0[1][2] + g0[2][1];
+          eps[4] = g0[0][2] + g0[2][0];
+          eps[5] = g0[0][1] + g0[1][0];
+          rate[0] = g1[0][0];
+          rate[1] = g1[1][1];
+          rate[2] = g1[2][2];
+          rate[3] = g1[1][2] + g1[2][1];
+          rate[4] = g1[0][2] + g1[2][0];
+          rate[5] = g1[0][1] + g1[1][0];
+        }
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
-          ep[m] = stn[m * NPT + gp];
-          ep0[m] = state_c[m * NPT + gp];
+          eps[m] += (T)M.e[m];
+          rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
         }
-        qb = stn[6 * NPT + gp];
-        qb0 = state_c[6 * NPT + gp];
+        const PhaseConst& P = A.tabs[r].p[A.ephase[D.e0 + e]];
+        const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
+        T s[6], ep[6], ep0[6], qb = (T)0, qb0 = (T)0;
 #pragma unroll
-        for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = stn[m * NPT + gp];
-      } else {
+        for (int m = 0; m < 6; ++m) s[m] = sig[m * NPT + gp];
+        if (j2) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) ep[m] = ep0[m] = (T)0;
+          for (int m = 0; m < 6; ++m) {
+            ep[m] = stn[m * NPT + gp];
+            ep0[m] = state_c[m * NPT + gp];
+          }
+          qb = stn[6 * NPT + gp];
+          qb0 = state_c[6 * NPT + gp];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = (T)0;
+          for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = stn[m * NPT + gp];
+        } else {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) ep[m] = ep0[m] = (T)0;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = (T)0;
+        }
+        const bool active = j2 && tanr[gp] != (T)0;
+        // free energy: elastic strain energy + hardening storage
+        T ee[6], cee[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) ee[m] = eps[m] - ep[m];
+        elastic_stress(P, ee, cee);
+        T psi = (T)0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) psi += cee[m] * ee[m];
+        psi = (T)0.5 * psi + (T)0.5 * (T)P.H * qb * qb;
+        // dissipation: plastic power minus hardening storage rate
+        T diss = (T)0;
+        if (active) {
+          T pwork = (T)0;
+#pragma unroll
+          for (int m = 0; m < 6; ++m) pwork += s[m] * (ep[m] - ep0[m]);
+          diss = pwork / (T)dt - (T)P.H * qb * (qb - qb0) / (T)dt;
+        }
+        // incremental work density: trapezoid between the elastic trial at the
+        // previous strain (committed plastic strain) and the converged stress
+        T de[6], pe[6], ps[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          de[m] = (T)dt * rate[m];
+          pe[m] = eps[m] - de[m] - ep0[m];
+        }
+        elastic_stress(P, pe, ps);
+        T wsum = (T)0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) wsum += (ps[m] + s[m]) * de[m];
+        const double pw = (double)((T)0.5 * wsum);
+        ((T*)A.pd)[gp] = psi;
+        ((T*)A.pd)[NPT + gp] = diss;
+        double mw = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          acc[BR_SIG + m] += (double)s[m];
+          mw += (double)s[m] * (dt * (double)rate[m]);
+        }
+        acc[BR_PSI] += (double)psi;
+        acc[BR_DISS] += (double)diss;
+        acc[BR_MICRO] += mw;
+        acc[BR_POS] += fmax(pw, 0.0);
+        acc[BR_NEG] += fmin(pw, 0.0);
+        acc[BR_UNLOAD] += pw < 0.0 ? 1.0 : 0.0;
+        acc[BR_ACTIVE] += active ? 1.0 : 0.0;
       }
-      const bool active = j2 && ((const T*)A.tan[slot])[1 * NPT + gp] != (T)0;
-      // free energy: elastic strain energy + hardening storage
-      T ee[6], cee[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) ee[m] = eps[m] - ep[m];
-      elastic_stress(P, ee, cee);
-      T psi = (T)0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) psi += cee[m] * ee[m];
-      psi = (T)0.5 * psi + (T)0.5 * (T)P.H * qb * qb;
-      // dissipation: plastic power minus hardening storage rate
-      T diss = (T)0;
-      if (active) {
-        T pwork = (T)0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) pwork += s[m] * (ep[m] - ep0[m]);
-        diss = pwork / (T)dt - (T)P.H * qb * (qb - qb0) / (T)dt;
-      }
-      // incremental work density: trapezoid between the elastic trial at the
-      // previous strain (committed plastic strain) and the converged stress
-      T de[6], pe[6], ps[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        de[m] = (T)dt * rate[m];
-        pe[m] = eps[m] - de[m] - ep0[m];
-      }
-      elastic_stress(P, pe, ps);
-      T wsum = (T)0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) wsum += (ps[m] + s[m]) * de[m];
-      const double pw = (double)((T)0.5 * wsum);
-      ((T*)A.pd)[gp] = psi;
-      ((T*)A.pd)[NPT + gp] = diss;
-      double mw = 0.0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        acc[BR_SIG + m] = (double)s[m];
-        mw += (double)s[m] * (dt * (double)rate[m]);
-      }
-      acc[BR_PSI] = (double)psi;
-      acc[BR_DISS] = (double)diss;
-      acc[BR_MICRO] = mw;
-      acc[BR_POS] = fmax(pw, 0.0);
-      acc[BR_NEG] = fmin(pw, 0.0);
-      acc[BR_UNLOAD] = pw < 0.0 ? 1.0 : 0.0;
-      acc[BR_ACTIVE] = active ? 1.0 : 0.0;
+      __syncthreads();  // cw is reused by the next tile
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -509,9 +526,11 @@ __global__ void __launch_bounds__(kBlock)
       double v = 0.0;
 #pragma unroll
       for (int u = 0; u < kBlock / 64; ++u) v += red[u][threadIdx.x];
-      A.part[(P_RESP + threadIdx.x) * A.Tmax + t] = v;
+      double* row = A.part + (P_RESP + threadIdx.x) * A.Tmax;
+      row[tl] = v;
+      for (int t = tl + 1; t < tend; ++t) row[t] = 0.0;
     }
-    __syncthreads();  // cw / red are reused by the next tile
+    __syncthreads();  // red is reused by the next group
   }
 }
 
@@ -1862,6 +1881,21 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
   A.ET = ET;
   for (int f = 0; f < NF; ++f) A.T[f] = B->T[f];
   for (int f = 0; f < NF; ++f) BALLOC(A.act[f], std::max(1, B->T[f]) * sizeof(int));
+  {
+    std::vector<int> hg;
+    for (int r = 0; r < n; ++r)
+      for (int u = 0; u < B->h_desc[r].nt[F_GP]; u += kFinalGroup) hg.push_back(B->h_desc[r].t0[F_GP] + u);
+    int* grp = nullptr;
+    BALLOC(grp, std::max<size_t>(1, hg.size()) * sizeof(int));
+    if (!hg.empty() &&
+        hipMemcpy(grp, hg.data(), hg.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      cia_rve_batch_destroy(B);
+      set_error_msg("hipMemcpy of the batch tile groups failed");
+      return CIA_ERR_HIP;
+    }
+    A.grp = grp;
+    A.n_grp = (int)hg.size();
+  }
   BALLOC(A.n_act, NF * sizeof(int));
   BALLOC(A.act_off, (size_t)NF * n * sizeof(int));
   Desc* d_desc = nullptr;
