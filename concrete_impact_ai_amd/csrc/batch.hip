@@ -222,6 +222,45 @@ __global__ void __launch_bounds__(1024) k_b_compact_scan(Arena A, int want) {
   if (tid < NF) A.n_act[tid] = carry[tid];
 }
 
+// Active-tile lists in one launch for batches of at most 1024 RVEs (one block per
+// RVE): every block sums the tile counts of the listed RVEs before it itself and
+// writes its tile ids; the last block writes the list lengths.  Same lists as
+// k_b_compact_scan + k_b_compact_fill.
+__global__ void __launch_bounds__(1024) k_b_compact_small(Arena A, int want) {
+  __shared__ int wsum[16][NF];
+  __shared__ int off[NF];
+  const int r = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int v[NF];
+  const bool mine = listed(A, A.ctl[r], want);
+  const bool on = tid < r && listed(A, A.ctl[tid], want);  // RVE tid precedes r
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    int x = on ? A.desc[tid].nt[f] : 0;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    v[f] = x;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) wsum[wid][f] = v[f];
+  }
+  __syncthreads();
+  if (tid < NF) {
+    int o = 0;
+    for (int u = 0; u < 16; ++u) o += wsum[u][tid];
+    off[tid] = o;
+    if (r == A.R - 1) A.n_act[tid] = o + (mine ? A.desc[r].nt[tid] : 0);
+  }
+  __syncthreads();
+  if (!mine) return;
+  const Desc& D = A.desc[r];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    int* out = A.act[f] + off[f];
+    for (int j = tid; j < D.nt[f]; j += blockDim.x) out[j] = D.t0[f] + j;
+  }
+}
+
 // Active-tile lists, step 2 (one block per RVE): the tile ids of the RVE.
 __global__ void __launch_bounds__(kBlock) k_b_compact_fill(Arena A, int want) {
   const int r = blockIdx.x;
@@ -728,7 +767,7 @@ __device__ __forceinline__ SC mk2(double a, double b) {
 template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_sp_init(Arena A) {
   __shared__ double lds[kBlock / 64];
-  CIA_TILES(A.T[F_CL]) {
+  CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
@@ -1457,6 +1496,10 @@ struct BOps {
   // Rebuild the active-tile lists: want < 0 for the PCG kernels, else the
   // Newton state of the next assembly / gather / line-search launches.
   static void compact(cia_rve_batch* B, hipStream_t st, int want = -1) {
+    if (B->R <= 1024) {
+      hipLaunchKernelGGL(k_b_compact_small, dim3((unsigned)B->R), dim3(1024), 0, st, B->A, want);
+      return;
+    }
     hipLaunchKernelGGL(k_b_compact_scan, dim3(1), dim3(1024), 0, st, B->A, want);
     hipLaunchKernelGGL(k_b_compact_fill, dim3((unsigned)B->R), dim3(kBlock), 0, st, B->A, want);
   }
@@ -1476,6 +1519,7 @@ struct BOps {
   }
   static void pcg_start(cia_rve_batch* B, hipStream_t st) {
     const unsigned gw = waves(B);
+    compact(B, st);
     hipLaunchKernelGGL(k_b_sp_init<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_scal_upd<true>, dim3(gw), dim3(kBlock), 0, st, B->A);
     compact(B, st);
@@ -1640,8 +1684,8 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   CIA_HIP(hipGetLastError());
   int cnt[7];
   const int interval = std::max(1, B->lin.check_interval);
+  CIA_TRY(read_counts(B, st, cnt));
   for (int round = 0;; ++round) {
-    CIA_TRY(read_counts(B, st, cnt));
     if (cnt[S_PCG] == 0) break;
     B->stats[1] += 1;
     // Newton systems of every RVE in this round, solved together
@@ -1672,16 +1716,17 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
     hipLaunchKernelGGL(k_b_norm_dw, dim3(gcl), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_ctl_pcg, dim3(gw), dim3(kBlock), 0, st, B->A, N);
     CIA_HIP(hipGetLastError());
-    for (;;) {
-      CIA_TRY(read_counts(B, st, cnt));
-      if (cnt[S_LS] == 0) break;
+    // Line search: one step at once (empty lists when no RVE entered it), then
+    // until no RVE is left in it; the last count also opens the next round.
+    do {
       BOps<T>::compact(B, st, S_LS);
       hipLaunchKernelGGL(k_b_axpy, dim3(gcl), dim3(kBlock), 0, st, B->A);
       BOps<T>::assemble(B, st, S, S_LS, state_c, wc);
       BOps<T>::gather(B, st, S_LS);
       hipLaunchKernelGGL(k_b_ctl_ls, dim3(gw), dim3(kBlock), 0, st, B->A, N);
       CIA_HIP(hipGetLastError());
-    }
+      CIA_TRY(read_counts(B, st, cnt));
+    } while (cnt[S_LS] > 0);
   }
   // converged RVEs: final pass (energies, next state, response sums), next w
   hipLaunchKernelGGL(k_b_final<T>, dim3(kgrid(B, K_ASM, F_GP)), dim3(kBlock), 0, st, B->A,
