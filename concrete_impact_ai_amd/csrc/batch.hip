@@ -81,6 +81,8 @@ struct Desc {
   const hipFloatComplex* kinv;
   int t0[NF];
   int nt[NF];
+  int j2mask;  // bit p set: phase p is J2 viscoplastic (carries state rows)
+  int pad;
   double inv_n;
   double volume;
 };
@@ -297,15 +299,30 @@ __global__ void __launch_bounds__(kBlock)
     const int64_t e = p >> 3;
     const int q = (int)(p & 7);
     const bool valid = e < D.E;
+    const int64_t gp = 8 * D.e0 + p;
+    const int64_t ge = D.e0 + e;
+    // the committed state is loaded before the corner exchange, so its latency
+    // overlaps the corner gather's
+    T epsp[6];
+    T qb = (T)0, ga = (T)0;
+    bool j2 = false;
     if (valid) {
       const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
 #pragma unroll
       for (int m = 0; m < 3; ++m) cw[threadIdx.x][m] = (T)w[3 * cq + m];
+      j2 = (D.j2mask >> A.ephase[ge]) & 1;
+      if (j2) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * NPT + gp];
+        qb = state_c[6 * NPT + gp];
+        ga = state_c[7 * NPT + gp];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
+      }
     }
     __syncthreads();
     if (valid) {
-      const int64_t gp = 8 * D.e0 + p;
-      const int64_t ge = D.e0 + e;
       T eps[6];
       {
         // strain_from of the corner values, streamed corner by corner (the same
@@ -338,18 +355,6 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
       const PhaseConst& P = A.tabs[r].p[A.ephase[ge]];  // read where used: fewer live registers
-      const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
-      T epsp[6];
-      T qb = (T)0, ga = (T)0;
-      if (j2) {
-#pragma unroll
-        for (int m = 0; m < 6; ++m) epsp[m] = state_c[m * NPT + gp];
-        qb = state_c[6 * NPT + gp];
-        ga = state_c[7 * NPT + gp];
-      } else {
-#pragma unroll
-        for (int m = 0; m < 6; ++m) epsp[m] = (T)0;
-      }
       PointOut<T> o;
       const int st = material_point<T, false, M1>(P, S, dt, eps, nullptr, epsp, qb, ga, o, nullptr,
                                                   nullptr);
@@ -430,6 +435,10 @@ __global__ void __launch_bounds__(kBlock, 2)
       const int64_t e = p >> 3;
       const int q = (int)(p & 7);
       const bool valid = e < D.E;
+      const int64_t gp = 8 * D.e0 + p;
+      // the point's stored values are loaded before the corner exchange
+      T s[6], ep[6], ep0[6], qb = (T)0, qb0 = (T)0, ga = (T)0;
+      bool j2 = false, active = false;
       if (valid) {
         const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
 #pragma unroll
@@ -438,10 +447,26 @@ __global__ void __launch_bounds__(kBlock, 2)
           cw[threadIdx.x][m] = (T)wv;
           cw[threadIdx.x][3 + m] = (T)(wv - wc[3 * cq + m]);
         }
+#pragma unroll
+        for (int m = 0; m < 6; ++m) s[m] = sig[m * NPT + gp];
+        j2 = (D.j2mask >> A.ephase[D.e0 + e]) & 1;
+        if (j2) {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) {
+            ep[m] = stn[m * NPT + gp];
+            ep0[m] = state_c[m * NPT + gp];
+          }
+          qb = stn[6 * NPT + gp];
+          ga = stn[7 * NPT + gp];
+          qb0 = state_c[6 * NPT + gp];
+          active = tanr[gp] != (T)0;
+        } else {
+#pragma unroll
+          for (int m = 0; m < 6; ++m) ep[m] = ep0[m] = (T)0;
+        }
       }
       __syncthreads();
       if (valid) {
-        const int64_t gp = 8 * D.e0 + p;
         T eps[6], rate[6];
         {
           // strain_from of the corner values, streamed corner by corner (the same
@@ -485,27 +510,15 @@ __global__ void __launch_bounds__(kBlock, 2)
           rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)dt;
         }
         const PhaseConst& P = A.tabs[r].p[A.ephase[D.e0 + e]];
-        const bool j2 = P.kind == CIA_MAT_J2_VISCOPLASTIC;
-        T s[6], ep[6], ep0[6], qb = (T)0, qb0 = (T)0;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) s[m] = sig[m * NPT + gp];
         if (j2) {
 #pragma unroll
-          for (int m = 0; m < 6; ++m) {
-            ep[m] = stn[m * NPT + gp];
-            ep0[m] = state_c[m * NPT + gp];
-          }
-          qb = stn[6 * NPT + gp];
-          qb0 = state_c[6 * NPT + gp];
-#pragma unroll
-          for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = stn[m * NPT + gp];
+          for (int m = 0; m < 6; ++m) next_state[m * NPT + gp] = ep[m];
+          next_state[6 * NPT + gp] = qb;
+          next_state[7 * NPT + gp] = ga;
         } else {
-#pragma unroll
-          for (int m = 0; m < 6; ++m) ep[m] = ep0[m] = (T)0;
 #pragma unroll
           for (int m = 0; m < 8; ++m) next_state[m * NPT + gp] = (T)0;
         }
-        const bool active = j2 && tanr[gp] != (T)0;
         // free energy: elastic strain energy + hardening storage
         T ee[6], cee[6];
 #pragma unroll
@@ -930,6 +943,22 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
     const int64_t e = p >> 3;
     const int q = (int)(p & 7);
     const bool valid = e < D.E;
+    const int64_t gp = 8 * D.e0 + p;
+    // the point's compact tangent is loaded before the corner exchange
+    Tangent<T> tc;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
+    tc.K = (T)0;
+    tc.d = tc.r = (T)0;
+    if (valid) {
+      tc.K = (T)A.tabs[r].p[A.ephase[D.e0 + e]].K;
+      tc.d = tan[gp];
+      tc.r = tan[NPT + gp];
+      if (tc.r != (T)0) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * NPT + gp];
+      }
+    }
     {
       double own[3] = {0.0, 0.0, 0.0};
       if (valid) {
@@ -950,7 +979,6 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
     }
     __syncthreads();
     if (valid) {
-      const int64_t gp = 8 * D.e0 + p;
       T eps[6];
       {
         // strain_from streamed corner by corner (same products and order)
@@ -977,16 +1005,6 @@ __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
         eps[3] = g0[1][2] + g0[2][1];
         eps[4] = g0[0][2] + g0[2][0];
         eps[5] = g0[0][1] + g0[1][0];
-      }
-      Tangent<T> tc;
-      tc.K = (T)A.tabs[r].p[A.ephase[D.e0 + e]].K;
-      tc.d = tan[gp];
-      tc.r = tan[NPT + gp];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
-      if (tc.r != (T)0) {
-#pragma unroll
-        for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * NPT + gp];
       }
       T s[6];
       tangent_apply(tc, eps, s);
@@ -1882,6 +1900,9 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
     D.g.jw = hx * hy * hz / 8.0;
     D.volume = D.g.jw * 8.0 * (double)D.E;
     D.inv_n = 1.0 / (double)D.E;
+    D.j2mask = 0;
+    for (int i = 0; i < d.n_phases; ++i)
+      if (d.phases[i].kind == CIA_MAT_J2_VISCOPLASTIC) D.j2mask |= 1 << i;
     const int nt[NF] = {
         (int)((8 * D.E + kBlock - 1) / kBlock), (int)((D.E + kBlock - 1) / kBlock),
         (int)((2 * (int64_t)d.ny * d.nz + D.lines_x - 1) / D.lines_x),
