@@ -139,9 +139,9 @@ struct Arena {
   int* act[NF];  // active-tile lists of the PCG kernels, [T[f]] each
   int* n_act;    // [NF] list lengths
   int* act_off;  // [NF][R] offset of every RVE's tiles in the lists
-  const int* grp;  // leader tiles of the final pass's Gauss-point tile groups
-  int n_grp;
-  int pad2;
+  const int* grp;     // leader tiles of the final pass's Gauss-point tile groups
+  const int* grp_cl;  // leader tiles of the per-phase sums' class tile groups
+  int n_grp, n_grp_cl;
 };
 
 struct NewtonP {
@@ -400,8 +400,8 @@ __global__ void __launch_bounds__(kBlock)
 // out (elastic points: zero rows) and reduces the response sums.  A block takes
 // groups of kFinalGroup consecutive Gauss-point tiles of one RVE (the leader
 // tiles listed in A.grp): each lane accumulates its points over the group and
-// the group's sums go to the leader tile's partials (zeros to the others), one
-// reduction per group instead of per tile, independent of the grid.
+// the group's sums go to the leader tile's partials (k_b_resp reads those only),
+// one reduction per group instead of per tile, independent of the grid.
 constexpr int kFinalGroup = 16;
 
 template <class T>
@@ -578,9 +578,7 @@ __global__ void __launch_bounds__(kBlock, 2)
       double v = 0.0;
 #pragma unroll
       for (int u = 0; u < kBlock / 64; ++u) v += red[u][threadIdx.x];
-      double* row = A.part + (P_RESP + threadIdx.x) * A.Tmax;
-      row[tl] = v;
-      for (int t = tl + 1; t < tend; ++t) row[t] = 0.0;
+      A.part[(P_RESP + threadIdx.x) * A.Tmax + tl] = v;
     }
     __syncthreads();  // red is reused by the next group
   }
@@ -1323,37 +1321,45 @@ __global__ void __launch_bounds__(kBlock) k_b_ctl_ls(Arena A, NewtonP N) {
 }
 
 // Per-phase sums of the converged RVEs (element per lane; an element has one
-// phase): block partials of B_NPHASE_ROWS rows per class tile.
+// phase), over groups of kFinalGroup class tiles of one RVE (leader tiles in
+// A.grp_cl): each lane accumulates its elements over the group, and the group's
+// B_NPHASE_ROWS sums go to the leader tile's partials (k_b_resp reads those only).
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_b_phase_sums(Arena A) {
   __shared__ double red[kBlock / 64][B_NPHASE_ROWS];
   const int64_t NPT = A.NPT;
-  CIA_TILES(A.T[F_CL]) {
-    const int r = A.t2r[F_CL][t];
+  for (int base_ = 0; base_ < A.n_grp_cl; base_ += (int)gridDim.x) {
+    const int gi = base_ + (int)xcd_block();
+    if (gi >= A.n_grp_cl) break;
+    const int tl = A.grp_cl[gi];
+    const int r = A.t2r[F_CL][tl];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
     if (c.state != S_DONE) continue;
-    const int64_t e = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    const int tend = min(tl + kFinalGroup, D.t0[F_CL] + D.nt[F_CL]);
+    const T* sig = (const T*)A.sig[c.cur];
+    const T* pd = (const T*)A.pd;
     double acc[B_NPHASE_ROWS];
 #pragma unroll
     for (int s = 0; s < B_NPHASE_ROWS; ++s) acc[s] = 0.0;
-    if (e < D.E) {
-      const T* sig = (const T*)A.sig[c.cur];
-      const T* pd = (const T*)A.pd;
-      double v[9] = {8.0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int q = 0; q < 8; ++q) {
-        const int64_t gp = 8 * (D.e0 + e) + q;
+    for (int t = tl; t < tend; ++t) {
+      const int64_t e = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+      if (e < D.E) {
+        double v[9] = {8.0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 8; ++q) {
+          const int64_t gp = 8 * (D.e0 + e) + q;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) v[1 + m] += (double)sig[m * NPT + gp];
-        v[7] += (double)pd[gp];
-        v[8] += (double)pd[NPT + gp];
-      }
-      const int ph = A.ephase[D.e0 + e];
+          for (int m = 0; m < 6; ++m) v[1 + m] += (double)sig[m * NPT + gp];
+          v[7] += (double)pd[gp];
+          v[8] += (double)pd[NPT + gp];
+        }
+        const int ph = A.ephase[D.e0 + e];
 #pragma unroll
-      for (int P = 0; P < B_PHASES; ++P) {
-        if (ph == P) {
+        for (int P = 0; P < B_PHASES; ++P) {
+          if (ph == P) {
 #pragma unroll
-          for (int m = 0; m < 9; ++m) acc[9 * P + m] = v[m];
+            for (int m = 0; m < 9; ++m) acc[9 * P + m] += v[m];
+          }
         }
       }
     }
@@ -1368,7 +1374,7 @@ __global__ void __launch_bounds__(kBlock) k_b_phase_sums(Arena A) {
       double w = 0.0;
 #pragma unroll
       for (int u = 0; u < kBlock / 64; ++u) w += red[u][threadIdx.x];
-      A.part[(P_PHASE + threadIdx.x) * A.Tmax + t] = w;
+      A.part[(P_PHASE + threadIdx.x) * A.Tmax + tl] = w;
     }
     __syncthreads();
   }
@@ -1380,11 +1386,19 @@ __global__ void __launch_bounds__(kBlock) k_b_resp(Arena A, double* __restrict__
   if (c.state != S_DONE) return;
   constexpr int NR = B_NRESP + B_NPHASE_ROWS;
   for (int s = 0; s < B_NRESP; ++s) {
-    const double v = rve_sum(A, P_RESP + s, D.t0[F_GP], D.nt[F_GP]);
+    // the final pass writes one partial per group of kFinalGroup tiles (at the
+    // group's leader tile): sum those only
+    double a = 0.0;
+    for (int u = lane * kFinalGroup; u < D.nt[F_GP]; u += 64 * kFinalGroup)
+      a += A.part[(P_RESP + s) * A.Tmax + D.t0[F_GP] + u];
+    const double v = wave_sum(a);
     if (lane == 0) out[r * NR + s] = v;
   }
   for (int s = 0; s < B_NPHASE_ROWS; ++s) {
-    const double v = rve_sum(A, P_PHASE + s, D.t0[F_CL], D.nt[F_CL]);
+    double a = 0.0;  // one partial per group of kFinalGroup class tiles
+    for (int u = lane * kFinalGroup; u < D.nt[F_CL]; u += 64 * kFinalGroup)
+      a += A.part[(P_PHASE + s) * A.Tmax + D.t0[F_CL] + u];
+    const double v = wave_sum(a);
     if (lane == 0) out[r * NR + B_NRESP + s] = v;
   }
 }
@@ -1947,10 +1961,10 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
   A.ET = ET;
   for (int f = 0; f < NF; ++f) A.T[f] = B->T[f];
   for (int f = 0; f < NF; ++f) BALLOC(A.act[f], std::max(1, B->T[f]) * sizeof(int));
-  {
+  for (int fam : {F_GP, F_CL}) {
     std::vector<int> hg;
     for (int r = 0; r < n; ++r)
-      for (int u = 0; u < B->h_desc[r].nt[F_GP]; u += kFinalGroup) hg.push_back(B->h_desc[r].t0[F_GP] + u);
+      for (int u = 0; u < B->h_desc[r].nt[fam]; u += kFinalGroup) hg.push_back(B->h_desc[r].t0[fam] + u);
     int* grp = nullptr;
     BALLOC(grp, std::max<size_t>(1, hg.size()) * sizeof(int));
     if (!hg.empty() &&
@@ -1959,8 +1973,13 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
       set_error_msg("hipMemcpy of the batch tile groups failed");
       return CIA_ERR_HIP;
     }
-    A.grp = grp;
-    A.n_grp = (int)hg.size();
+    if (fam == F_GP) {
+      A.grp = grp;
+      A.n_grp = (int)hg.size();
+    } else {
+      A.grp_cl = grp;
+      A.n_grp_cl = (int)hg.size();
+    }
   }
   BALLOC(A.n_act, NF * sizeof(int));
   BALLOC(A.act_off, (size_t)NF * n * sizeof(int));
