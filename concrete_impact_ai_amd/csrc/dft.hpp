@@ -84,6 +84,7 @@ __device__ __forceinline__ void dft_stage(const C2* __restrict__ in, C2* __restr
     if (STEP_A) {
       const int j2 = fdiv(r, inv_n1), k1 = r - j2 * n1;
       int m = 0;
+#pragma unroll 4
       for (int j1 = 0; j1 < n1; ++j1) {
         const C2 p = cmul(src[(n2 * j1 + j2) * es], w1[m], INV);
         acc.x += p.x;
@@ -95,6 +96,7 @@ __device__ __forceinline__ void dft_stage(const C2* __restrict__ in, C2* __restr
     } else {
       const int k2 = fdiv(r, inv_n1), k1 = r - k2 * n1;
       int m = 0;
+#pragma unroll 4
       for (int j2 = 0; j2 < n2; ++j2) {
         const C2 p = cmul(src[(j2 * n1 + k1) * es], w2[m], INV);
         acc.x += p.x;
