@@ -777,7 +777,7 @@ __device__ __forceinline__ SC mk2(double a, double b) {
 // x = 0, r = -r_newton, p_old = 0, packed r; partials (r.r, sum r).
 template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_sp_init(Arena A) {
-  __shared__ double lds[kBlock / 64];
+  __shared__ double red[kBlock / 64][4];
   CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
@@ -801,11 +801,21 @@ __global__ void __launch_bounds__(kBlock) k_b_sp_init(Arena A) {
       uab[2 * D.e0 + cl] = mk2<SC>(v[1], v[2]);
       uab[2 * D.e0 + D.E + cl] = mk2<SC>(v[3], 0.0);
     }
+    // the four block sums in one LDS round
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const double a = block_sum<kBlock>(v[s], lds);
-      if (threadIdx.x == 0) A.part[(P_UPD + s) * A.Tmax + t] = a;
+      const double a = wave_sum(v[s]);
+      if (lane == 0) red[wid][s] = a;
     }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double a = 0.0;
+#pragma unroll
+      for (int u = 0; u < kBlock / 64; ++u) a += red[u][threadIdx.x];
+      A.part[(P_UPD + threadIdx.x) * A.Tmax + t] = a;
+    }
+    __syncthreads();
   }
 }
 
@@ -1060,7 +1070,7 @@ __global__ void __launch_bounds__(kBlock) k_b_node(Arena A, int pi) {
 // x += alpha p, r -= alpha y, packed r; partials (r.r, sum r).
 template <class SC>
 __global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
-  __shared__ double lds[kBlock / 64];
+  __shared__ double red[kBlock / 64][4];
   const double* p = A.pp[pi];
   CIA_ACTIVE_TILES(F_CL) {
     const int r = A.t2r[F_CL][t];
@@ -1084,11 +1094,21 @@ __global__ void __launch_bounds__(kBlock) k_b_update(Arena A, int pi) {
       uab[2 * D.e0 + cl] = mk2<SC>(v[1], v[2]);
       uab[2 * D.e0 + D.E + cl] = mk2<SC>(v[3], 0.0);
     }
+    // the four block sums in one LDS round
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const double a = block_sum<kBlock>(v[s], lds);
-      if (threadIdx.x == 0) A.part[(P_UPD + s) * A.Tmax + t] = a;
+      const double a = wave_sum(v[s]);
+      if (lane == 0) red[wid][s] = a;
     }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double a = 0.0;
+#pragma unroll
+      for (int u = 0; u < kBlock / 64; ++u) a += red[u][threadIdx.x];
+      A.part[(P_UPD + threadIdx.x) * A.Tmax + t] = a;
+    }
+    __syncthreads();
   }
 }
 
@@ -1132,6 +1152,11 @@ __global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
   }
 }
 
+// x positions per tile of the strided passes: up to 32, so an RVE edge of up to
+// 32 is one tile per (line row, component) and every synchronisation of a tile
+// covers twice the lines of a 16-wide tile
+constexpr int kBatchDftTile = 32;
+
 template <bool INV, bool ZPASS, class SC>
 __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
   extern __shared__ __align__(16) unsigned char dft_raw[];
@@ -1144,15 +1169,15 @@ __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
     if (!pcg_active(A, A.ctl[r])) continue;
     const DftAxis& ax = ZPASS ? D.az : D.ay;
     const int n = ax.n, ld = n + 1;
-    const int tiles_x = (D.nx + kDftTile - 1) / kDftTile;
+    const int tiles_x = (D.nx + kBatchDftTile - 1) / kBatchDftTile;
     const int lt = t - D.t0[FAM];
     const int xt = lt % tiles_x, rest = lt / tiles_x;
     const int no = ZPASS ? D.ny : D.nz;
     const int o = rest % no, comp = rest / no;
     SC* X = dft_sm;
-    SC* Y = dft_sm + kDftTile * ld;
-    // the tiles_x x-tiles of a row share the nx positions evenly (at most 16
-    // each), so an edge that is not a multiple of 16 leaves no near-empty tile
+    SC* Y = dft_sm + kBatchDftTile * ld;
+    // the tiles_x x-tiles of a row share the nx positions evenly (at most
+    // kBatchDftTile each), so no tile is left near-empty
     const int tw = (D.nx + tiles_x - 1) / tiles_x;
     const int i0 = xt * tw;
     const int64_t plane = (int64_t)D.nx * D.ny;
@@ -1474,6 +1499,16 @@ struct cia_rve_batch {
 };
 
 namespace {
+
+template <class SC>
+hipError_t set_dft_lds(int bytes) {
+  const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+  hipError_t e = hipFuncSetAttribute((const void*)k_b_dft_strided<false, false, SC>, a, bytes);
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_b_dft_strided<false, true, SC>, a, bytes);
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_b_dft_strided<true, false, SC>, a, bytes);
+  if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_b_dft_strided<true, true, SC>, a, bytes);
+  return e;
+}
 
 cia_status balloc(cia_rve_batch* B, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -1920,7 +1955,8 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
     const int nt[NF] = {
         (int)((8 * D.E + kBlock - 1) / kBlock), (int)((D.E + kBlock - 1) / kBlock),
         (int)((2 * (int64_t)d.ny * d.nz + D.lines_x - 1) / D.lines_x),
-        ((d.nx + kDftTile - 1) / kDftTile) * d.nz * 2, ((d.nx + kDftTile - 1) / kDftTile) * d.ny * 2};
+        ((d.nx + kBatchDftTile - 1) / kBatchDftTile) * d.nz * 2,
+        ((d.nx + kBatchDftTile - 1) / kBatchDftTile) * d.ny * 2};
     for (int f = 0; f < NF; ++f) {
       D.t0[f] = tcount[f];
       D.nt[f] = nt[f];
@@ -1929,8 +1965,8 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
     // spectra in FP64 for FP64 batches, in FP32 for FP32 batches
     const size_t csz = d0.precision == CIA_PREC_FP64 ? sizeof(double2) : sizeof(float2);
     lx = std::max(lx, 2 * (size_t)D.lines_x * (d.nx + 1) * csz);
-    ly = std::max(ly, 2 * (size_t)kDftTile * (d.ny + 1) * csz);
-    lz = std::max(lz, 2 * (size_t)kDftTile * (d.nz + 1) * csz);
+    ly = std::max(ly, 2 * (size_t)kBatchDftTile * (d.ny + 1) * csz);
+    lz = std::max(lz, 2 * (size_t)kBatchDftTile * (d.nz + 1) * csz);
     e0 += D.E;
   }
   B->ET = e0;
@@ -1938,6 +1974,23 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
   B->lds_x = lx;
   B->lds_y = ly;
   B->lds_z = lz;
+  {
+    // strided passes of edges above 32 in FP64 take more than 64 KB of LDS
+    const int big = (int)std::max(ly, lz);
+    hipError_t e = hipSuccess;
+    if (big > 65536) {
+      if (d0.precision == CIA_PREC_FP64) {
+        e = set_dft_lds<double2>(big);
+      } else {
+        e = set_dft_lds<float2>(big);
+      }
+    }
+    if (e != hipSuccess) {
+      set_error("hipFuncSetAttribute", e, __FILE__, __LINE__);
+      delete B;
+      return CIA_ERR_HIP;
+    }
+  }
   for (int f = 0; f < NF; ++f) {
     B->T[f] = tcount[f];
     B->Tmax = std::max(B->Tmax, tcount[f]);
