@@ -1939,7 +1939,7 @@ cia_status cia_rve_batch_create(const cia_rve_desc* descs, int32_t n, int32_t au
     D.nx = d.nx;
     D.ny = d.ny;
     D.nz = d.nz;
-    D.lines_x = std::max(1, 512 / d.nx);  // x-lines per tile of the x pass
+    D.lines_x = std::max(1, 1024 / d.nx);  // x-lines per tile of the x pass
     D.e0 = e0;
     D.E = (int64_t)d.nx * d.ny * d.nz;
     const double hx = d.lengths[0] / d.nx, hy = d.lengths[1] / d.ny, hz = d.lengths[2] / d.nz;
