@@ -931,101 +931,105 @@ __global__ void __launch_bounds__(kBlock) k_b_apply(Arena A) {
 }
 
 // Element pass: p = (z - z0) + beta p_old at the element corners, forces of
-// K p.  Lane q forms corner q; the 8 lanes of an element exchange corners in LDS.
+// K p.  One lane per element: the lane gathers its 8 corners, forms the strain
+// of all 8 Gauss points by sum factorisation (box_gradients), applies the
+// compact tangents and spreads the traction sums back to the corners
+// (box_forces).  No corner exchange and no cross-lane force reduction: about a
+// quarter of the vector instructions of one lane per Gauss point with an
+// 8-lane exchange and reduction, whose issue rate (not HBM) bounded the pass.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_b_elem(Arena A, int pi) {
   using SC = typename SpecOf<T>::type;
-  __shared__ T cpv[kBlock][3];  // corner values of p, rounded to T as strain_from rounds them
   const int64_t NPT = A.NPT, ET = A.ET;
   const double* p_old = A.pp[1 - pi];
   double* p_new = A.pp[pi];
-  CIA_ACTIVE_TILES(F_GP) {
-    const int r = A.t2r[F_GP][t];
+  CIA_ACTIVE_TILES(F_CL) {
+    const int r = A.t2r[F_CL][t];
     const Desc& D = A.desc[r];
     const Ctl& c = A.ctl[r];
     if (!pcg_active(A, c)) continue;
+    const int64_t e = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    if (e >= D.E) continue;
     const double beta = c.p_beta;
     const T* tan = (const T*)A.tan[c.cur];
-    const SC* zc = (const SC*)A.uab + 2 * D.e0;
-    const int64_t p = (int64_t)(t - D.t0[F_GP]) * kBlock + threadIdx.x;
-    const int64_t e = p >> 3;
-    const int q = (int)(p & 7);
-    const bool valid = e < D.E;
-    const int64_t gp = 8 * D.e0 + p;
-    // the point's compact tangent is loaded before the corner exchange
+    const SC* ua = (const SC*)A.uab + 2 * D.e0;
+    const SC* ub = ua + D.E;
+    const double z0[3] = {ua[0].x, ua[0].y, ub[0].x};
+    const int nx = D.nx, ny = D.ny, nz = D.nz;
+    const unsigned eu = (unsigned)e, rw = eu / (unsigned)nx;
+    const int i = (int)(eu - rw * (unsigned)nx);
+    const int k = (int)(rw / (unsigned)ny);
+    const int j = (int)(rw - (unsigned)k * (unsigned)ny);
+    const int ip = i + 1 == nx ? 0 : i + 1;
+    const int jp = j + 1 == ny ? 0 : j + 1;
+    const int kp = k + 1 == nz ? 0 : k + 1;
+    // corner values of p = (z - z0) + beta p_old, rounded to T (the strains are formed in T)
+    T u[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int ii = (a & 1) ? ip : i, jj = (a & 2) ? jp : j, kk = (a & 4) ? kp : k;
+      const int64_t cq = ii + (int64_t)nx * (jj + (int64_t)ny * kk);
+      const SC t01 = ua[cq], t2 = ub[cq];
+      const int64_t gq = D.e0 + cq;
+      const double o0 = (t01.x - z0[0]) + beta * p_old[3 * gq + 0];
+      const double o1 = (t01.y - z0[1]) + beta * p_old[3 * gq + 1];
+      const double o2 = (t2.x - z0[2]) + beta * p_old[3 * gq + 2];
+      if (a == 0) {
+        p_new[3 * gq + 0] = o0;
+        p_new[3 * gq + 1] = o1;
+        p_new[3 * gq + 2] = o2;
+      }
+      u[a][0] = (T)o0;
+      u[a][1] = (T)o1;
+      u[a][2] = (T)o2;
+    }
+    T gX[2][2][3], gY[2][2][3], gZ[2][2][3];
+    box_gradients<T>(u, D.g, gX, gY, gZ);
     Tangent<T> tc;
+    tc.K = (T)A.tabs[r].p[A.ephase[D.e0 + e]].K;
+    const int64_t g8 = 8 * (D.e0 + e);
+    T dd[8], rr[8];
+    load8(tan + g8, dd);
+    load8(tan + NPT + g8, rr);
+    bool plastic = false;
 #pragma unroll
-    for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
-    tc.K = (T)0;
-    tc.d = tc.r = (T)0;
-    if (valid) {
-      tc.K = (T)A.tabs[r].p[A.ephase[D.e0 + e]].K;
-      tc.d = tan[gp];
-      tc.r = tan[NPT + gp];
-      if (tc.r != (T)0) {
+    for (int q = 0; q < 8; ++q) plastic |= rr[q] != (T)0;
+    T SX[2][2][3], SY[2][2][3], SZ[2][2][3];  // traction sums (box_traction_add)
 #pragma unroll
-        for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * NPT + gp];
-      }
-    }
-    {
-      double own[3] = {0.0, 0.0, 0.0};
-      if (valid) {
-        const double z0[3] = {zc[0].x, zc[0].y, zc[D.E].x};
-        const int64_t cq = periodic_corner(e, q, D.nx, D.ny, D.nz);
-        const SC t01 = zc[cq], t2 = zc[D.E + cq];
-        const int64_t gq = D.e0 + cq;
-        own[0] = (t01.x - z0[0]) + beta * p_old[3 * gq + 0];
-        own[1] = (t01.y - z0[1]) + beta * p_old[3 * gq + 1];
-        own[2] = (t2.x - z0[2]) + beta * p_old[3 * gq + 2];
-        if (q == 0) {
+    for (int qx = 0; qx < 2; ++qx) {
+      T nn[6][4];  // the compact tangents' directions of the 4 points with this qx
 #pragma unroll
-          for (int m = 0; m < 3; ++m) p_new[3 * (D.e0 + e) + m] = own[m];
+      for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) nn[m][h] = (T)0;
+      if (plastic) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          const T* src = tan + (2 + m) * NPT + g8 + 4 * qx;
+#pragma unroll
+          for (int h = 0; h < 4; ++h) nn[m][h] = src[h];
         }
       }
 #pragma unroll
-      for (int m = 0; m < 3; ++m) cpv[threadIdx.x][m] = (T)own[m];
-    }
-    __syncthreads();
-    if (valid) {
-      T eps[6];
-      {
-        // strain_from streamed corner by corner (same products and order)
-        T wx[2], wy[2], wz[2];
-        gp_weights<T>(q, wx, wy, wz);
-        const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
-        T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        const int base = threadIdx.x & ~7;
+      for (int h = 0; h < 4; ++h) {
+        const int qy = h >> 1, qz = h & 1, q = 4 * qx + h;
+        T eps[6];
+        box_strain(gX, gY, gZ, qx, qy, qz, eps);
+        tc.d = dd[q];
+        tc.r = rr[q];
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          T bx, by, bz;
-          corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            const T u = cpv[base + a][m];
-            g0[m][0] += bx * u;
-            g0[m][1] += by * u;
-            g0[m][2] += bz * u;
-          }
-        }
-        eps[0] = g0[0][0];
-        eps[1] = g0[1][1];
-        eps[2] = g0[2][2];
-        eps[3] = g0[1][2] + g0[2][1];
-        eps[4] = g0[0][2] + g0[2][0];
-        eps[5] = g0[0][1] + g0[1][0];
+        for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != (T)0 ? nn[m][h] : (T)0;
+        T sg[6];
+        tangent_apply(tc, eps, sg);
+        box_traction_add(sg, qx, qy, qz, SX, SY, SZ);
       }
-      T s[6];
-      tangent_apply(tc, eps, s);
-      T bx[8], by[8], bz[8];
-      gp_factors<T>(q, D.g, bx, by, bz);
-      T f[8][3];
-      force_from(s, (T)D.g.jw, bx, by, bz, f);
-      T fin[3];
-      element_reduce8(q, f, fin);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * q + m) * ET + D.e0 + e] = fin[m];
     }
-    __syncthreads();  // cpv is reused by the next tile
+    T f[8][3];
+    box_forces<T>(SX, SY, SZ, D.g, f);
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * a + m) * ET + D.e0 + e] = f[a][m];
   }
 }
 
@@ -1571,9 +1575,12 @@ struct BOps {
     hipLaunchKernelGGL(k_b_compact_fill, dim3((unsigned)B->R), dim3(kBlock), 0, st, B->A, want);
   }
   // One PCG iteration (pi selects p_new = pp[pi], p_old = pp[1 - pi]).
+  static void elem(cia_rve_batch* B, hipStream_t st, int pi) {
+    hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_CL)), dim3(kBlock), 0, st, B->A, pi);
+  }
   static void iteration(cia_rve_batch* B, hipStream_t st, int pi) {
     const unsigned gw = waves(B);
-    hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_GP)), dim3(kBlock), 0, st, B->A, pi);
+    elem(B, st, pi);
     hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, st, B->A, pi);
     hipLaunchKernelGGL(k_b_scal_pq, dim3(gw), dim3(kBlock), 0, st, B->A);
     hipLaunchKernelGGL(k_b_update<SC>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A, pi);
@@ -1668,9 +1675,8 @@ cia_status autotune(cia_rve_batch* B, hipStream_t st, const Settings& S, const v
     BOps<T>::assemble(B, q, S, S_NEWTON, state_c, wc);
   }));
   run(tune_one(B, st, K_GATHER, F_CL, [&](hipStream_t q) { BOps<T>::gather(B, q, S_NEWTON); }));
-  run(tune_one(B, st, K_ELEM, F_GP, [&](hipStream_t q) {
-    hipLaunchKernelGGL(k_b_elem<T>, dim3(kgrid(B, K_ELEM, F_GP)), dim3(kBlock), 0, q, B->A, 0);
-  }, true));
+  run(tune_one(B, st, K_ELEM, F_CL, [&](hipStream_t q) { BOps<T>::elem(B, q, 0); },
+               true));
   run(tune_one(B, st, K_NODE, F_CL, [&](hipStream_t q) {
     hipLaunchKernelGGL(k_b_node<T>, dim3(kgrid(B, K_NODE, F_CL)), dim3(kBlock), 0, q, B->A, 0);
   }, true));

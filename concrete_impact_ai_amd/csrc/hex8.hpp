@@ -280,6 +280,133 @@ __device__ __forceinline__ void element_reduce8(int q, const T f[8][3], T out[3]
   }
 }
 
+// Eight consecutive values (one element's Gauss points) in 16-byte loads.
+__device__ __forceinline__ void load8(const float* p, float v[8]) {
+  const float4 a = *(const float4*)p, b = *(const float4*)(p + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void load8(const double* p, double v[8]) {
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const double2 a = *(const double2*)(p + 2 * h);
+    v[2 * h] = a.x;
+    v[2 * h + 1] = a.y;
+  }
+}
+
+// Sum-factorised kinematics of one box element at all 8 Gauss points (one lane
+// per element).  On a box element du/dx does not depend on the point's x
+// coordinate: the x-gradients at the 4 (qy, qz) points are the 4 edge
+// differences along x interpolated in y and then z, and likewise for y and z
+// -- 180 operations for the 72 gradient entries instead of 8 x 72 products.
+// gX[qy][qz][c] = du_c/dx, gY[qx][qz][c] = du_c/dy, gZ[qx][qy][c] = du_c/dz.
+template <class T>
+__device__ __forceinline__ void box_gradients(const T u[8][3], const BoxGeom& g, T gX[2][2][3],
+                                              T gY[2][2][3], T gZ[2][2][3]) {
+  const T wp = (T)kWp, wm = (T)kWm;
+  const T ix = (T)g.ihx, iy = (T)g.ihy, iz = (T)g.ihz;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    T dx[2][2], dy[2][2], dz[2][2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        dx[b][d] = u[1 + 2 * b + 4 * d][m] - u[2 * b + 4 * d][m];  // edge along x at (ay, az)
+        dy[b][d] = u[b + 2 + 4 * d][m] - u[b + 4 * d][m];          // along y at (ax, az)
+        dz[b][d] = u[b + 2 * d + 4][m] - u[b + 2 * d][m];          // along z at (ax, ay)
+      }
+#pragma unroll
+    for (int q1 = 0; q1 < 2; ++q1) {
+      const T w10 = q1 == 0 ? wp : wm, w11 = q1 == 1 ? wp : wm;
+      T tx[2], ty[2], tz[2];
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        tx[d] = w10 * dx[0][d] + w11 * dx[1][d];  // x edges at y point q1, z corner d
+        ty[d] = w10 * dy[0][d] + w11 * dy[1][d];  // y edges at x point q1, z corner d
+        tz[d] = w10 * dz[0][d] + w11 * dz[1][d];  // z edges at x point q1, y corner d
+      }
+#pragma unroll
+      for (int q2 = 0; q2 < 2; ++q2) {
+        const T w20 = q2 == 0 ? wp : wm, w21 = q2 == 1 ? wp : wm;
+        gX[q1][q2][m] = ix * (w20 * tx[0] + w21 * tx[1]);
+        gY[q1][q2][m] = iy * (w20 * ty[0] + w21 * ty[1]);
+        gZ[q1][q2][m] = iz * (w20 * tz[0] + w21 * tz[1]);
+      }
+    }
+  }
+}
+
+// Engineering strain at Gauss point (qx, qy, qz) from box_gradients.
+template <class T>
+__device__ __forceinline__ void box_strain(const T gX[2][2][3], const T gY[2][2][3],
+                                           const T gZ[2][2][3], int qx, int qy, int qz, T e[6]) {
+  e[0] = gX[qy][qz][0];
+  e[1] = gY[qx][qz][1];
+  e[2] = gZ[qx][qy][2];
+  e[3] = gY[qx][qz][2] + gZ[qx][qy][1];
+  e[4] = gX[qy][qz][2] + gZ[qx][qy][0];
+  e[5] = gX[qy][qz][1] + gY[qx][qz][0];
+}
+
+// Adds the traction rows of the stress s at Gauss point (qx, qy, qz) to the sums
+// over the coordinate each gradient factor ignores: SX[qy][qz] over qx of
+// (s0, s5, s4), SY[qx][qz] over qy of (s5, s1, s3), SZ[qx][qy] over qz of
+// (s4, s3, s2).  FIRST: the point with qx = 0 (SX), qy = 0 (SY), qz = 0 (SZ)
+// starts the respective sum.
+template <class T>
+__device__ __forceinline__ void box_traction_add(const T s[6], int qx, int qy, int qz,
+                                                 T SX[2][2][3], T SY[2][2][3], T SZ[2][2][3]) {
+  const T rx[3] = {s[0], s[5], s[4]}, ry[3] = {s[5], s[1], s[3]}, rz[3] = {s[4], s[3], s[2]};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    SX[qy][qz][m] = qx == 0 ? rx[m] : SX[qy][qz][m] + rx[m];
+    SY[qx][qz][m] = qy == 0 ? ry[m] : SY[qx][qz][m] + ry[m];
+    SZ[qx][qy][m] = qz == 0 ? rz[m] : SZ[qx][qy][m] + rz[m];
+  }
+}
+
+// Element forces f[a][c] = sum_q w detJ (B_q^T s_q)[a][c] from the traction sums
+// of box_traction_add: the transposed interpolation back to the corners.
+template <class T>
+__device__ __forceinline__ void box_forces(const T SX[2][2][3], const T SY[2][2][3],
+                                           const T SZ[2][2][3], const BoxGeom& g, T f[8][3]) {
+  const T wp = (T)kWp, wm = (T)kWm;
+  const T jw = (T)g.jw;
+  const T cx = jw * (T)g.ihx, cy = jw * (T)g.ihy, cz = jw * (T)g.ihz;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    T HX[2][2], HY[2][2], HZ[2][2];  // HX[ay][az], HY[ax][az], HZ[ax][ay]
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const T w0 = d == 0 ? wp : wm, w1 = d == 1 ? wp : wm;  // W(point 0|1, corner d)
+      T jx[2], jy[2], jz[2];
+#pragma unroll
+      for (int q1 = 0; q1 < 2; ++q1) {
+        jx[q1] = w0 * SX[q1][0][m] + w1 * SX[q1][1][m];
+        jy[q1] = w0 * SY[q1][0][m] + w1 * SY[q1][1][m];
+        jz[q1] = w0 * SZ[q1][0][m] + w1 * SZ[q1][1][m];
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const T v0 = b == 0 ? wp : wm, v1 = b == 1 ? wp : wm;
+        HX[b][d] = cx * (v0 * jx[0] + v1 * jx[1]);
+        HY[b][d] = cy * (v0 * jy[0] + v1 * jy[1]);
+        HZ[b][d] = cz * (v0 * jz[0] + v1 * jz[1]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+      const T fx = ax ? HX[ay][az] : -HX[ay][az];
+      const T fy = ay ? HY[ax][az] : -HY[ax][az];
+      const T fz = az ? HZ[ax][ay] : -HZ[ax][ay];
+      f[a][m] = fx + fy + fz;
+    }
+  }
+}
+
 // Element (i, j, k) of a periodic nx x ny x nz grid: node classes of its 8 corners.
 // (32-bit index arithmetic: an RVE holds fewer than 2^31 elements.)
 __device__ __forceinline__ void periodic_corners(int64_t e64, int nx, int ny, int nz, int64_t c[8],
