@@ -788,6 +788,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Element pass: beta from the r.z partials, p = (z - z0) + beta p_old, forces.
+// One lane per element: the lane forms p at its 8 corners, the strains of its 8
+// Gauss points by sum factorisation (box_gradients), applies the compact
+// tangents and spreads the traction sums back to the corners (box_forces) -- no
+// corner exchange and no cross-lane force reduction.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
     k_sp_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
@@ -810,61 +814,69 @@ __global__ void __launch_bounds__(kBlock)
     sc->rz = rz[0];
     sc->beta = beta;
   }
-  const int64_t p = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  const int64_t e = p >> 3;
-  const int q = (int)(p & 7);
-  const bool valid = e < G.E;
+  const int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (e >= G.E) return;
   const int64_t np = G.E * 8;
-  // Lane q of an element forms p = (z - z0) + beta p_old at corner q only; the 8
-  // lanes then read each other's corners from LDS (one load per corner instead of
-  // eight redundant ones).
-  __shared__ double cpv[kBlock][3];
-  {
-    double own[3] = {0.0, 0.0, 0.0};
-    if (valid) {
-      const int64_t cq = periodic_corner(e, q, G.nx, G.ny, G.nz);
+  int64_t cq[8];
+  int i, j, k;
+  periodic_corners(e, G.nx, G.ny, G.nz, cq, i, j, k);
+  T u[8][3];  // corner values of p, rounded to T (the strains are formed in T)
 #pragma unroll
-      for (int m = 0; m < 3; ++m) own[m] = (zr[3 * cq + m] - zr[m]) + beta * p_old[3 * cq + m];
-      if (q == 0) {
+  for (int a = 0; a < 8; ++a) {
+    double o[3];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) p_new[3 * e + m] = own[m];
-      }
+    for (int m = 0; m < 3; ++m) o[m] = (zr[3 * cq[a] + m] - zr[m]) + beta * p_old[3 * cq[a] + m];
+    if (a == 0) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) p_new[3 * e + m] = o[m];
     }
 #pragma unroll
-    for (int m = 0; m < 3; ++m) cpv[threadIdx.x][m] = own[m];
+    for (int m = 0; m < 3; ++m) u[a][m] = (T)o[m];
   }
-  __syncthreads();
-  if (!valid) return;  // after the block's only barrier; whole 8-lane groups leave
-  T eps[6];
-  T bx[8], by[8], bz[8];
-  gp_factors<T>(q, G.g, bx, by, bz);
-  {
-    double pe[8][3];
-    const int base = threadIdx.x & ~7;
-#pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-      for (int m = 0; m < 3; ++m) pe[a][m] = cpv[base + a][m];
-    strain_from(pe, bx, by, bz, eps);
-  }
+  T gX[2][2][3], gY[2][2][3], gZ[2][2][3];
+  box_gradients<T>(u, G.g, gX, gY, gZ);
   Tangent<T> tc;
   tc.K = (T)tab->p[ephase[e]].K;
-  tc.d = tan[p];
-  tc.r = tan[np + p];
+  T dd[8], rr[8];
+  load8(tan + 8 * e, dd);
+  load8(tan + np + 8 * e, rr);
+  bool plastic = false;
 #pragma unroll
-  for (int m = 0; m < 6; ++m) tc.n[m] = (T)0;
-  if (tc.r != (T)0) {
+  for (int q = 0; q < 8; ++q) plastic |= rr[q] != (T)0;
+  T SX[2][2][3], SY[2][2][3], SZ[2][2][3];  // traction sums (box_traction_add)
 #pragma unroll
-    for (int m = 0; m < 6; ++m) tc.n[m] = tan[(2 + m) * np + p];
+  for (int qx = 0; qx < 2; ++qx) {
+    T nn[6][4];  // the compact tangents' directions of the 4 points with this qx
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) nn[m][h] = (T)0;
+    if (plastic) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) nn[m][h] = tan[(2 + m) * np + 8 * e + 4 * qx + h];
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int qy = h >> 1, qz = h & 1;
+      T eps[6];
+      box_strain(gX, gY, gZ, qx, qy, qz, eps);
+      tc.d = dd[4 * qx + h];
+      tc.r = rr[4 * qx + h];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != (T)0 ? nn[m][h] : (T)0;
+      T sg[6];
+      tangent_apply(tc, eps, sg);
+      box_traction_add(sg, qx, qy, qz, SX, SY, SZ);
+    }
   }
-  T s[6];
-  tangent_apply(tc, eps, s);
   T f[8][3];
-  force_from(s, (T)G.g.jw, bx, by, bz, f);
-  T fin[3];
-  element_reduce8(q, f, fin);
+  box_forces<T>(SX, SY, SZ, G.g, f);
 #pragma unroll
-  for (int m = 0; m < 3; ++m) F[(3 * q + m) * G.E + e] = fin[m];
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) F[(3 * a + m) * G.E + e] = f[a][m];
 }
 
 // -------------------------------------------------------------------------
@@ -1374,7 +1386,7 @@ struct Ops {
                                  int* iters_out, int* flag_out, double rtol, int hint,
                                  double* x_norm2) {
     const int64_t C = R->C;
-    const unsigned gn = grid_for(C), ge = grid_for(R->np);
+    const unsigned gn = grid_for(C), ge = grid_for(R->np), gel = grid_for(R->E);
     const int nx = R->nx, ny = R->ny, nz = R->nz;
     const int64_t n_lines = (int64_t)ny * nz;
     const unsigned g_xl = (unsigned)((n_lines + xline_group(nx) - 1) / xline_group(nx));
@@ -1418,7 +1430,7 @@ struct Ops {
       double* p_old = R->pp[1 - pi];
       double* p_new = R->pp[pi];
       prof::begin(prof::PCG_ELEM, s, tag);
-      hipLaunchKernelGGL(k_sp_elem<T>, dim3(ge), dim3(kBlock), 0, s, R->G, R->d_tab, R->d_ephase,
+      hipLaunchKernelGGL(k_sp_elem<T>, dim3(gel), dim3(kBlock), 0, s, R->G, R->d_tab, R->d_ephase,
                          (const T*)R->tan[slot], (const double*)R->pz, p_old, p_new,
                          (T*)R->F, (const double*)part_rz, (int)g_zc, R->sc);
       prof::end(s);
