@@ -40,7 +40,7 @@ KERNEL_NAMES = {"pcg_elem": "k_sp_elem", "pcg_node": "k_sp_node", "pcg_update": 
 # copy kernel of known size moves to what the same counter reads for it.  Keyed by
 # (slot, RVE edge, precision); absent keys report null.  Source summary:
 # profiles/k_sp_elem_rve48_fp64_hbm_traffic.md
-MEASURED_TRAFFIC: dict = {("pcg_elem", 48, "fp64"): 71_622_581}
+MEASURED_TRAFFIC: dict = {("pcg_elem", 48, "fp64"): 77_042_165}
 
 
 def copy_calibration(torch) -> None:
