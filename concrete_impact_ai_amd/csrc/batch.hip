@@ -284,7 +284,8 @@ template <class T, bool M1>
 __global__ void __launch_bounds__(kBlock)
     k_b_assemble(Arena A, Settings S, int want, const T* __restrict__ state_c,
                  const double* __restrict__ wc) {
-  __shared__ T cw[kBlock][3];  // corner values, rounded to T as strain_from rounds them
+  __shared__ double cw[kBlock][3];  // corner fluctuations (differenced in FP64, gp_strain_sf)
+  __shared__ T cs[kBlock][8];       // the points' stresses, for the corner forces (rows padded)
   const int64_t NPT = A.NPT, ET = A.ET;
   CIA_ACTIVE_TILES(F_GP) {
     const int r = A.t2r[F_GP][t];
@@ -309,7 +310,7 @@ __global__ void __launch_bounds__(kBlock)
     if (valid) {
       const int64_t cq = D.e0 + periodic_corner(e, q, D.nx, D.ny, D.nz);
 #pragma unroll
-      for (int m = 0; m < 3; ++m) cw[threadIdx.x][m] = (T)w[3 * cq + m];
+      for (int m = 0; m < 3; ++m) cw[threadIdx.x][m] = w[3 * cq + m];
       j2 = (D.j2mask >> A.ephase[ge]) & 1;
       if (j2) {
 #pragma unroll
@@ -324,34 +325,7 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
     if (valid) {
       T eps[6];
-      {
-        // strain_from of the corner values, streamed corner by corner (the same
-        // products and summation order; the gradient factors are formed again for
-        // the forces instead of staying live across the constitutive update)
-        T wx[2], wy[2], wz[2];
-        gp_weights<T>(q, wx, wy, wz);
-        const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
-        T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        const int base = threadIdx.x & ~7;
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          T bx, by, bz;
-          corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            const T u = cw[base + a][m];
-            g0[m][0] += bx * u;
-            g0[m][1] += by * u;
-            g0[m][2] += bz * u;
-          }
-        }
-        eps[0] = g0[0][0];
-        eps[1] = g0[1][1];
-        eps[2] = g0[2][2];
-        eps[3] = g0[1][2] + g0[2][1];
-        eps[4] = g0[0][2] + g0[2][0];
-        eps[5] = g0[0][1] + g0[1][0];
-      }
+      gp_strain_sf<T, double, 3>(&cw[threadIdx.x & ~7], 0, q, D.g, eps);
 #pragma unroll
       for (int m = 0; m < 6; ++m) eps[m] += (T)M.e[m];
       const PhaseConst& P = A.tabs[r].p[A.ephase[ge]];  // read where used: fewer live registers
@@ -376,16 +350,18 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int m = 0; m < 6; ++m) tan[(2 + m) * NPT + gp] = o.tan.n[m];
       }
-      T bx[8], by[8], bz[8];
-      gp_factors<T>(q, D.g, bx, by, bz);
-      T f[8][3];
-      force_from(o.sig, (T)D.g.jw, bx, by, bz, f);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) cs[threadIdx.x][m] = o.sig[m];
+    }
+    __syncthreads();
+    if (valid) {
+      // lane q: the element force at corner q from the 8 points' stresses
       T fin[3];
-      element_reduce8(q, f, fin);
+      corner_force<T>(&cs[threadIdx.x & ~7], q, D.g, fin);
 #pragma unroll
       for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * q + m) * ET + ge] = fin[m];
     }
-    __syncthreads();  // cw is reused by the next tile
+    __syncthreads();  // cw and cs are reused by the next tile
   }
 }
 
@@ -408,7 +384,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock, 2)
     k_b_final(Arena A, const T* __restrict__ state_c, T* __restrict__ next_state,
               const double* __restrict__ wc) {
-  __shared__ T cw[kBlock][6];  // corner values, already rounded to T as strain_from rounds them
+  __shared__ double cw[kBlock][6];  // corner fluctuation and its increment (as the assembly)
   __shared__ double red[kBlock / 64][B_NRESP];
   const int64_t NPT = A.NPT;
   for (int base_ = 0; base_ < A.n_grp; base_ += (int)gridDim.x) {
@@ -444,8 +420,8 @@ __global__ void __launch_bounds__(kBlock, 2)
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
           const double wv = w[3 * cq + m];
-          cw[threadIdx.x][m] = (T)wv;
-          cw[threadIdx.x][3 + m] = (T)(wv - wc[3 * cq + m]);
+          cw[threadIdx.x][m] = wv;
+          cw[threadIdx.x][3 + m] = wv - wc[3 * cq + m];
         }
 #pragma unroll
         for (int m = 0; m < 6; ++m) s[m] = sig[m * NPT + gp];
@@ -468,42 +444,10 @@ __global__ void __launch_bounds__(kBlock, 2)
       __syncthreads();
       if (valid) {
         T eps[6], rate[6];
-        {
-          // strain_from of the corner values, streamed corner by corner (the same
-          // products and summation order, fewer live registers)
-          T wx[2], wy[2], wz[2];
-          gp_weights<T>(q, wx, wy, wz);
-          const T ix = (T)D.g.ihx, iy = (T)D.g.ihy, iz = (T)D.g.ihz;
-          T g0[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, g1[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-          const int base = threadIdx.x & ~7;
-#pragma unroll
-          for (int a = 0; a < 8; ++a) {
-            T bx, by, bz;
-            corner_factors<T>(a, wx, wy, wz, ix, iy, iz, bx, by, bz);
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-              const T u = cw[base + a][m], du = cw[base + a][3 + m];
-              g0[m][0] += bx * u;
-              g0[m][1] += by * u;
-              g0[m][2] += bz * u;
-              g1[m][0] += bx * du;
-              g1[m][1] += by * du;
-              g1[m][2] += bz * du;
-            }
-          }
-          eps[0] = g0[0][0];
-          eps[1] = g0[1][1];
-          eps[2] = g0[2][2];
-          eps[3] = g0[1][2] + g0[2][1];
-          eps[4] = g0[0][2] + g0[2][0];
-          eps[5] = g0[0][1] + g0[1][0];
-          rate[0] = g1[0][0];
-          rate[1] = g1[1][1];
-          rate[2] = g1[2][2];
-          rate[3] = g1[1][2] + g1[2][1];
-          rate[4] = g1[0][2] + g1[2][0];
-          rate[5] = g1[0][1] + g1[1][0];
-        }
+        // the assembly's strain (same corner values, same operations) and the
+        // strain increment of the step
+        gp_strain_sf<T, double, 6>(&cw[threadIdx.x & ~7], 0, q, D.g, eps);
+        gp_strain_sf<T, double, 6>(&cw[threadIdx.x & ~7], 3, q, D.g, rate);
 #pragma unroll
         for (int m = 0; m < 6; ++m) {
           eps[m] += (T)M.e[m];
@@ -595,19 +539,17 @@ __global__ void k_b_diagnose(Arena A, Settings S, int r, int slot, const T* __re
   const int64_t NPT = A.NPT;
   const int64_t gp = 8 * D.e0 + p;
   const double* w = A.w[slot];
-  T we[8][3], dwe[8][3];
+  double we[8][3], dwe[8][3];
   for (int a = 0; a < 8; ++a) {
     const int64_t c = D.e0 + periodic_corner(e, a, D.nx, D.ny, D.nz);
     for (int m = 0; m < 3; ++m) {
-      we[a][m] = (T)w[3 * c + m];
-      dwe[a][m] = (T)(w[3 * c + m] - wc[3 * c + m]);
+      we[a][m] = w[3 * c + m];
+      dwe[a][m] = w[3 * c + m] - wc[3 * c + m];
     }
   }
-  T bx[8], by[8], bz[8];
-  gp_factors<T>(q, D.g, bx, by, bz);
-  T eps[6], rate[6];
-  strain_from(we, bx, by, bz, eps);
-  strain_from(dwe, bx, by, bz, rate);
+  T eps[6], rate[6];  // as the assembly and the final pass form them
+  gp_strain_sf<T, double, 3>(we, 0, q, D.g, eps);
+  gp_strain_sf<T, double, 3>(dwe, 0, q, D.g, rate);
   for (int m = 0; m < 6; ++m) {
     eps[m] += (T)M.e[m];
     rate[m] = (rate[m] + (T)(M.e[m] - M.ec[m])) / (T)M.dt;

@@ -407,6 +407,94 @@ __device__ __forceinline__ void box_forces(const T SX[2][2][3], const T SY[2][2]
   }
 }
 
+// Engineering strain at Gauss point q of a box element from its corner values
+// c8[a][off + 0..2] (a = ax + 2 ay + 4 az), factorised: du/dx at q is the 4 edge
+// differences along x weighted by W(qy, ay) W(qz, az) (likewise y and z), 84
+// operations instead of 8 corners x 9 products plus their factors.  The edge
+// differences are formed in the corner values' precision U and rounded to T
+// once: with FP64 corners an FP32 strain keeps its relative accuracy when the
+// fluctuation varies little across an element (no cancellation in T).
+template <class T, class U, int W>
+__device__ __forceinline__ void gp_strain_sf(const U (*c8)[W], int off, int q, const BoxGeom& g,
+                                             T e[6]) {
+  const int qx = q >> 2, qy = (q >> 1) & 1, qz = q & 1;
+  const T wp = (T)kWp, wm = (T)kWm;
+  const T wx[2] = {qx == 0 ? wp : wm, qx == 1 ? wp : wm};  // W(qx, ax = 0 | 1)
+  const T wy[2] = {qy == 0 ? wp : wm, qy == 1 ? wp : wm};
+  const T wz[2] = {qz == 0 ? wp : wm, qz == 1 ? wp : wm};
+  T gx[3] = {(T)0, (T)0, (T)0}, gy[3] = {(T)0, (T)0, (T)0}, gz[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const T wyz = wy[u] * wz[v], wxz = wx[u] * wz[v], wxy = wx[u] * wy[v];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        gx[m] += wyz * (T)(c8[1 + 2 * u + 4 * v][off + m] - c8[2 * u + 4 * v][off + m]);
+        gy[m] += wxz * (T)(c8[u + 2 + 4 * v][off + m] - c8[u + 4 * v][off + m]);
+        gz[m] += wxy * (T)(c8[u + 2 * v + 4][off + m] - c8[u + 2 * v][off + m]);
+      }
+    }
+  const T ix = (T)g.ihx, iy = (T)g.ihy, iz = (T)g.ihz;
+  e[0] = ix * gx[0];
+  e[1] = iy * gy[1];
+  e[2] = iz * gz[2];
+  e[3] = iz * gz[1] + iy * gy[2];
+  e[4] = iz * gz[0] + ix * gx[2];
+  e[5] = iy * gy[0] + ix * gx[1];
+}
+
+// Force at corner a of a box element, w detJ sum_q B_qa^T s_q, from the stresses
+// of its 8 Gauss points (s8[q][0..5], q = 4 qx + 2 qy + qz; rows padded to 8).
+// Factorised: the x-gradient factor of corner a does not depend on qx, so the x
+// rows (s0, s5, s4) of the two points that differ only in qx are added first and
+// weighted once (likewise the y rows (s5, s1, s3) over qy and the z rows
+// (s4, s3, s2) over qz): ~95 operations instead of the 8-point force rows and an
+// 8-lane reduction.
+template <class T>
+__device__ __forceinline__ void corner_force(const T (*s8)[8], int a, const BoxGeom& g, T f[3]) {
+  const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+  const T wp = (T)kWp, wm = (T)kWm;
+  const T wx[2] = {ax == 0 ? wp : wm, ax == 1 ? wp : wm};  // W(qx = 0 | 1, ax)
+  const T wy[2] = {ay == 0 ? wp : wm, ay == 1 ? wp : wm};
+  const T wz[2] = {az == 0 ? wp : wm, az == 1 ? wp : wm};
+  T hx[3] = {(T)0, (T)0, (T)0}, hy[3] = {(T)0, (T)0, (T)0}, hz[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      {  // x rows at (qy, qz) = (u, v), summed over qx
+        const T* p0 = s8[2 * u + v];
+        const T* p1 = s8[4 + 2 * u + v];
+        const T w = wy[u] * wz[v];
+        hx[0] += w * (p0[0] + p1[0]);
+        hx[1] += w * (p0[5] + p1[5]);
+        hx[2] += w * (p0[4] + p1[4]);
+      }
+      {  // y rows at (qx, qz) = (u, v), summed over qy
+        const T* p0 = s8[4 * u + v];
+        const T* p1 = s8[4 * u + 2 + v];
+        const T w = wx[u] * wz[v];
+        hy[0] += w * (p0[5] + p1[5]);
+        hy[1] += w * (p0[1] + p1[1]);
+        hy[2] += w * (p0[3] + p1[3]);
+      }
+      {  // z rows at (qx, qy) = (u, v), summed over qz
+        const T* p0 = s8[4 * u + 2 * v];
+        const T* p1 = s8[4 * u + 2 * v + 1];
+        const T w = wx[u] * wy[v];
+        hz[0] += w * (p0[4] + p1[4]);
+        hz[1] += w * (p0[3] + p1[3]);
+        hz[2] += w * (p0[2] + p1[2]);
+      }
+    }
+  const T jw = (T)g.jw;
+  const T cx = (ax ? jw : -jw) * (T)g.ihx, cy = (ay ? jw : -jw) * (T)g.ihy,
+          cz = (az ? jw : -jw) * (T)g.ihz;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) f[m] = cx * hx[m] + cy * hy[m] + cz * hz[m];
+}
+
 // Element (i, j, k) of a periodic nx x ny x nz grid: node classes of its 8 corners.
 // (32-bit index arithmetic: an RVE holds fewer than 2^31 elements.)
 __device__ __forceinline__ void periodic_corners(int64_t e64, int nx, int ny, int nz, int64_t c[8],
