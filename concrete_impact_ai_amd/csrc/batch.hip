@@ -15,7 +15,8 @@
 // class-major ([3 CT]), the packed spectra of RVE r occupy [2 e0, 2 e0 + 2E).
 //
 // Tiles.  Every kernel works on tiles of one family (Gauss-point lanes, classes
-// or modes, DFT x-lines, DFT y/z line groups); a tile never straddles two RVEs,
+// -- one per element, the lanes of the element pass -- or modes, DFT x-lines, DFT
+// y/z line groups); a tile never straddles two RVEs,
 // so a tile's block reduction is a partial of exactly one RVE, stored at the
 // tile's index.  A kernel's blocks walk the family's tiles in rounds of
 // gridDim.x tiles; within a round each XCD takes one contiguous range (the
