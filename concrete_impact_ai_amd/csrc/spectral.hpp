@@ -38,11 +38,11 @@ constexpr int kZTile = 16;  // largest z-column tile (kx columns)
 
 // Dynamic LDS of the z-column pass.
 inline size_t zcol_lds_bytes(int nz, int tile_w = kZTile) {
-  return 4 * (size_t)tile_w * (nz + 1) * sizeof(double2);
+  return 6 * (size_t)tile_w * (nz + 1) * sizeof(double2);
 }
 // x-line groups: lines per workgroup and dynamic LDS of the line passes.
 __host__ __device__ inline int xline_group(int nx) { return nx >= 256 ? 1 : 256 / nx; }
-inline size_t xline_lds_bytes(int nx) { return 4 * (size_t)xline_group(nx) * (nx + 1) * sizeof(double2); }
+inline size_t xline_lds_bytes(int nx) { return 6 * (size_t)xline_group(nx) * (nx + 1) * sizeof(double2); }
 
 // z-column tile (kx in [tile_w xt, tile_w (xt + 1)), ky) of an RVE's spectra: forward
 // z-DFT, Z = Khat^-1 (R - sum r) / N per mode, inverse z-DFT, in place.  Returns
@@ -60,7 +60,7 @@ __device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w,
   const int64_t plane = (int64_t)nx * ny;
   const int64_t col0 = i0 + (int64_t)nx * ky;
   double2* A = sm;                   // [3][TW][nz + 1]
-  double2* T = sm + 3 * TW * ldz;    // [TW][nz + 1]
+  double2* T = sm + 3 * TW * ldz;    // [3][TW][nz + 1]
   const int per = TW * nz;
   for (int u = threadIdx.x; u < 3 * per; u += blockDim.x) {
     const int m = u / per, rem = u - m * per;
@@ -69,13 +69,11 @@ __device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w,
         ii < width ? S[(int64_t)m * C + col0 + ii + plane * z] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    dft_stage<true, false>(A + m * TW * ldz, T, width, ldz, 1, az, wz);
-    __syncthreads();
-    dft_stage<false, false>(T, A + m * TW * ldz, width, ldz, 1, az, wz);
-    __syncthreads();
-  }
+  // the three components' lines in one pass per stage (lines past `width` hold zeros)
+  dft_stage<true, false>(A, T, 3 * TW, ldz, 1, az, wz);
+  __syncthreads();
+  dft_stage<false, false>(T, A, 3 * TW, ldz, 1, az, wz);
+  __syncthreads();
   double rz = 0.0;
   for (int u = threadIdx.x; u < per; u += blockDim.x) {
     const int kz = u / TW, ii = u - kz * TW;
@@ -98,13 +96,10 @@ __device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w,
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    dft_stage<true, true>(A + m * TW * ldz, T, width, ldz, 1, az, wz);
-    __syncthreads();
-    dft_stage<false, true>(T, A + m * TW * ldz, width, ldz, 1, az, wz);
-    __syncthreads();
-  }
+  dft_stage<true, true>(A, T, 3 * TW, ldz, 1, az, wz);
+  __syncthreads();
+  dft_stage<false, true>(T, A, 3 * TW, ldz, 1, az, wz);
+  __syncthreads();
   for (int u = threadIdx.x; u < 3 * per; u += blockDim.x) {
     const int m = u / per, rem = u - m * per;
     const int z = rem / TW, ii = rem - z * TW;
@@ -134,7 +129,7 @@ __device__ __forceinline__ void xlines_update_fwd(int nx, int64_t l0, int nl, do
   const int64_t c0 = l0 * nx;
   const float inv_nx = 1.0f / (float)nx;
   double2* A = sm;               // [3][L][nx + 1]
-  double2* T = sm + 3 * L * ld;  // [L][nx + 1]
+  double2* T = sm + 3 * L * ld;  // [3][L][nx + 1]
 #pragma unroll
   for (int s = 0; s < 4; ++s) acc[s] = 0.0;
   const int t = threadIdx.x;
@@ -159,13 +154,11 @@ __device__ __forceinline__ void xlines_update_fwd(int nx, int64_t l0, int nl, do
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    dft_stage<true, false>(A + m * L * ld, T, nl, ld, 1, ax, wx);
-    __syncthreads();
-    dft_stage<false, false>(T, A + m * L * ld, nl, ld, 1, ax, wx);
-    __syncthreads();
-  }
+  // the three components' lines in one pass per stage (lines past nl are not stored)
+  dft_stage<true, false>(A, T, 3 * L, ld, 1, ax, wx);
+  __syncthreads();
+  dft_stage<false, false>(T, A, 3 * L, ld, 1, ax, wx);
+  __syncthreads();
   for (int u = threadIdx.x; u < 3 * ncl; u += blockDim.x) {
     const int m = u / ncl, tt = u - m * ncl;
     const int l = fdiv(tt, inv_nx);
@@ -190,13 +183,10 @@ __device__ __forceinline__ void xlines_inv(int nx, int64_t l0, int nl,
     A[(m * L + l) * ld + (tt - l * nx)] = S[(int64_t)m * C + c0 + tt];
   }
   __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    dft_stage<true, true>(A + m * L * ld, T, nl, ld, 1, ax, wx);
-    __syncthreads();
-    dft_stage<false, true>(T, A + m * L * ld, nl, ld, 1, ax, wx);
-    __syncthreads();
-  }
+  dft_stage<true, true>(A, T, 3 * L, ld, 1, ax, wx);
+  __syncthreads();
+  dft_stage<false, true>(T, A, 3 * L, ld, 1, ax, wx);
+  __syncthreads();
   const int t = threadIdx.x;
   if (t < ncl) {
     const int l = fdiv(t, inv_nx), i = t - l * nx;
