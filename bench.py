@@ -505,8 +505,12 @@ def batched_section(args) -> dict:
         err = np.max(np.abs(stresses[tag] - stresses["fp64"]), axis=2) / den
         out[f"{tag}_stress_error_max"] = float(np.max(err))
         out[f"{tag}_stress_error_median"] = float(np.median(err))
-    # plain FP64 single-RVE solver (one RVE at a time) on a sample of the batch
-    sample = list(range(0, nr, max(1, nr // args.batch_sample)))[: args.batch_sample]
+    # plain FP64 single-RVE solver, one RVE at a time: every RVE of the batch (the
+    # default), or a sample of --batch-sample RVEs scaled to the batch's mean size
+    if args.batch_sample <= 0 or args.batch_sample >= nr:
+        sample = list(range(nr))
+    else:
+        sample = list(range(0, nr, max(1, nr // args.batch_sample)))[: args.batch_sample]
     t_single = 0.0
     n_single = 0
     req = MaterialResponseRequirements(tangent=False)
@@ -524,12 +528,18 @@ def batched_section(args) -> dict:
         t_single += time.perf_counter() - t0
         n_single += K
         cache.close()
-    # the sample's mean time per solve, scaled to the batch's mean RVE size
-    sample_el = float(np.mean(edges[sample].astype(np.float64) ** 3))
-    per_solve = t_single / n_single * (n_el / nr) / sample_el
+    if len(sample) == nr:
+        per_solve = t_single / n_single
+        note = ("every RVE of the batch, one at a time (the reference's serial loop) on the GPU, "
+                "over the same timed increments")
+    else:
+        # the sample's mean time per solve, scaled to the batch's mean RVE size
+        sample_el = float(np.mean(edges[sample].astype(np.float64) ** 3))
+        per_solve = t_single / n_single * (n_el / nr) / sample_el
+        note = ("a sample of the batch, one RVE at a time (the reference's serial loop) on the GPU; "
+                "time per solve scaled by the batch's mean RVE size")
     out["fp64_single_rve"] = {"rve_solves_per_s": 1.0 / per_solve, "sample_rves": len(sample),
-                              "note": "one RVE at a time (the reference's serial loop) on the GPU; "
-                                      "time per solve scaled by the batch's mean RVE size"}
+                              "note": note}
     out["speedup_fp32_batched_vs_fp64_single"] = out["fp32"]["rve_solves_per_s"] * per_solve
     out["speedup_fp32_predictor_batched_vs_fp64_single"] = (out["fp32_predictor"]["rve_solves_per_s"]
                                                             * per_solve)
@@ -626,7 +636,8 @@ def main() -> int:
     ap.add_argument("--batch-rves", type=int, default=256)
     ap.add_argument("--batch-warmup", type=int, default=2)
     ap.add_argument("--batch-steps", type=int, default=8)
-    ap.add_argument("--batch-sample", type=int, default=8)
+    ap.add_argument("--batch-sample", type=int, default=0,
+                    help="RVEs of the single-RVE comparison (0: all)")
     ap.add_argument("--surrogate-rve", type=int, default=12)
     ap.add_argument("--surrogate-steps", type=int, default=64)
     ap.add_argument("--surrogate-paths", type=int, default=1280)
