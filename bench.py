@@ -155,7 +155,7 @@ def surrogate_section(args) -> dict:
                                        production_fraction=args.surrogate_production_fraction)
     print(f"surrogate: FE² data {p_train + p_test} paths x {T} steps on {n}^3 RVEs",
           file=sys.stderr, flush=True)
-    d_eps, rate, stress, fe2_rate = SD.rve_responses_batched(model, times, strains)
+    d_eps, rate, stress, fe2_rate = SD.rve_responses_batched(model, times, strains, progress=sys.stderr)
     tr = slice(0, p_train)
     te = slice(p_train, p_train + p_test)
     scales = S.SurrogateScales.fit(d_eps[tr], rate[tr], stress[tr], SD.homogenised_stiffness(model))

@@ -131,10 +131,11 @@ def rve_responses(model, times, strains, workers: int = 4):
     return d_eps, rate, stress, P * T / elapsed
 
 
-def rve_responses_batched(model, times, strains, precision: str = "fp64"):
+def rve_responses_batched(model, times, strains, precision: str = "fp64", progress=None):
     """As :func:`rve_responses`, with every path on its own replica of the RVE in one
     batched solve per step (``cia_rve_batch_solve``): returns d_eps (P, T, 6),
-    rate (P, T), stress (P, T, 6) and the RVE solves per second."""
+    rate (P, T), stress (P, T, 6) and the RVE solves per second.  ``progress``: a
+    stream that receives a line every 20 steps."""
     import dataclasses
 
     from concrete_impact_ai_amd import batch as B
@@ -156,6 +157,8 @@ def rve_responses_batched(model, times, strains, precision: str = "fp64"):
             r = B.solve_rve_batch(bt, strains[:, k + 1], dts[:, k], settings, st)
             stress[:, k] = r.macro_stress
             st = r.state
+            if progress is not None and (k + 1) % 20 == 0:
+                print(f"FE² data: step {k + 1} of {T}", file=progress, flush=True)
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
     finally:
