@@ -163,9 +163,18 @@ def surrogate_section(args) -> dict:
     print(f"surrogate: training {args.surrogate_iters} BPTT iterations", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = S.train(params, scales, d_eps[tr], rate[tr], stress[tr], iterations=args.surrogate_iters,
-                   batch_paths=min(96, p_train), window=T, lr=2e-3, seed=args.seed,
-                   log_every=max(1, args.surrogate_iters // 10), progress=sys.stderr)
+    # paths longer than 64 steps: train on their first 64 steps first, then on the
+    # whole path at a tenth of the learning rate (the recurrence of an untrained
+    # network diverges over ~100 commits, and full-length BPTT at the first stage's
+    # rate diverges too)
+    hist = []
+    stages = [(64, args.surrogate_iters // 2, 2e-3),
+              (T, args.surrogate_iters - args.surrogate_iters // 2, 2e-4)] \
+        if T > 64 else [(T, args.surrogate_iters, 2e-3)]
+    for window, iters, lr in stages:
+        hist += S.train(params, scales, d_eps[tr], rate[tr], stress[tr], iterations=iters,
+                        batch_paths=min(96, p_train), window=window, lr=lr, seed=args.seed,
+                        log_every=max(1, iters // 10), progress=sys.stderr)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     S.envelope(params, scales, d_eps[tr], rate[tr])
