@@ -23,6 +23,9 @@ the reference ``file:line`` it follows:
 * ``oracle.dynamics``  -- explicit velocity-Verlet and Newmark updates
                           (``src/fem/dynamics/material.py``, ``nonlinear.py``)
 * ``oracle.mlp``       -- FP64 reference of the material-point surrogate MLP
+* ``oracle.laminate``  -- closed-form check of the layered two-phase RVE (per-phase
+                          recurrence on the point update) that pins ``oracle.rve``
+                          on heterogeneous plastic paths
 
 Pinning status of every function is recorded in each module header.
 """
