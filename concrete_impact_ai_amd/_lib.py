@@ -342,6 +342,8 @@ _SIGNATURES.update({
         C.c_int32, [_vp, C.c_int64, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, _vp, C.c_int64,
                     C.c_int32]),
     "cia_mlp_backward_weight_workspace": (C.c_int64, [C.c_int64, C.c_int64]),
+    "cia_mlp_linear_backward_weight_noalloc": (
+        C.c_int32, [_vp, C.c_int64, C.c_int64, C.c_int64, _vp, _vp, _vp, _vp, C.c_int32]),
     "cia_mlp_adam_step": (
         C.c_int32, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, C.c_float, C.c_float, C.c_float,
                     C.c_float, C.c_int32, C.c_float]),

@@ -17,6 +17,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "common.hpp"
 
 namespace cia {
@@ -153,6 +155,13 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 }
 
 constexpr int64_t kMaxWgradSplits = 64;
+
+// Static scratch of cia_mlp_linear_backward_weight_noalloc: the split partials of
+// a 256 x 257 weight gradient at the full split count, i.e. exactly the
+// workspace cia_mlp_backward_weight_workspace(256, 256) asks for (17.1 MB per GPU).
+constexpr int64_t kNoAllocScratchFloats = (kMaxWgradSplits + 1) * 256 * 257;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
 
 // Sum the split partials [S][N][Kx] into dW [N][K] (column K -> db), fixed order.
 __global__ void k_wgrad_reduce(int S, int64_t N, int64_t K, int64_t Kx, const float* __restrict__ part,
@@ -533,15 +542,14 @@ cia_status cia_mlp_linear_backward_data(void* stream, int64_t M, int64_t N, int6
   return gemm_launch((hipStream_t)stream, Zprev ? EPI_DGELU : EPI_LINEAR, g, 1);
 }
 
-cia_status cia_mlp_linear_backward_weight(void* stream, int64_t M, int64_t N, int64_t K,
-                                          const void* dZ, const void* X, float* dW, float* db,
-                                          float* workspace, int64_t workspace_floats,
-                                          int32_t accumulate) {
+// dW[N][K] (and db as an extra column of ones in X) = dZ^T X: a GEMM whose
+// reduction runs over the M rows, split along M into fp32 partials held in
+// `workspace` (every partial the reduction reads is written by the GEMM first,
+// so the workspace needs no clearing).
+static cia_status wgrad_launch(hipStream_t st, int64_t M, int64_t N, int64_t K, const void* dZ,
+                               const void* X, float* dW, float* db, float* workspace,
+                               int64_t workspace_floats, int32_t accumulate) {
   using namespace cia;
-  CIA_ARG(dZ && X && dW && M > 0 && N > 0 && K > 0 && workspace,
-          "invalid arguments to cia_mlp_linear_backward_weight");
-  // dW[N][K] (and db as an extra column of ones in X) = dZ^T X: a GEMM whose
-  // reduction runs over the M rows, split along M into fp32 partials.
   const int64_t Kx = K + (db ? 1 : 0);
   const int64_t per = N * Kx;
   const int64_t tiles = ((N + GM - 1) / GM) * ((Kx + GN - 1) / GN);
@@ -560,12 +568,67 @@ cia_status cia_mlp_linear_backward_weight(void* stream, int64_t M, int64_t N, in
   g.Cf = workspace; g.ldcf = Kx;
   g.k_split = ks;
   g.split_stride = per;
-  hipStream_t st = (hipStream_t)stream;
   CIA_TRY(gemm_launch(st, EPI_LINEAR, g, (int)splits));
   hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(per)), dim3(kBlock), 0, st, (int)splits, N, K,
                      Kx, (const float*)workspace, dW, db, accumulate);
   CIA_HIP(hipGetLastError());
   return CIA_OK;
+}
+
+cia_status cia_mlp_linear_backward_weight(void* stream, int64_t M, int64_t N, int64_t K,
+                                          const void* dZ, const void* X, float* dW, float* db,
+                                          float* workspace, int64_t workspace_floats,
+                                          int32_t accumulate) {
+  CIA_ARG(dZ && X && dW && M > 0 && N > 0 && K > 0 && workspace,
+          "invalid arguments to cia_mlp_linear_backward_weight");
+  return wgrad_launch((hipStream_t)stream, M, N, K, dZ, X, dW, db, workspace, workspace_floats,
+                      accumulate);
+}
+
+// Allocation-free variant: the split partials live in the file-scope device
+// array `scratch` (one copy per GPU, held until the process ends).  Host code
+// never passes `scratch` itself to a kernel: its address on each device is
+// taken once with hipGetSymbolAddress, with that device current, and kept.
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[cia::kNoAllocMaxDevices] = {};
+
+static cia_status scratch_on_current_device(hipStream_t st, float** out) {
+  int dev = -1;
+  CIA_HIP(hipGetDevice(&dev));
+  CIA_ARG(dev >= 0 && dev < cia::kNoAllocMaxDevices,
+          "cia_mlp_linear_backward_weight_noalloc: device index beyond the supported range");
+  if (st) {
+    int sdev = -1;
+    CIA_HIP(hipStreamGetDevice(st, &sdev));
+    CIA_ARG(sdev == dev,
+            "cia_mlp_linear_backward_weight_noalloc: the stream's device must be the current device");
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    CIA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(cia::scratch)));
+    CIA_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (float*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CIA_OK;
+}
+
+cia_status cia_mlp_linear_backward_weight_noalloc(void* stream, int64_t M, int64_t N, int64_t K,
+                                                  const void* dZ, const void* X, float* dW,
+                                                  float* db, int32_t accumulate) {
+  using namespace cia;
+  CIA_ARG(dZ && X && dW && M > 0 && N > 0 && K > 0,
+          "invalid arguments to cia_mlp_linear_backward_weight_noalloc");
+  // at least one split partial plus the reserve the split rule keeps
+  CIA_ARG(N * (K + 1) <= kNoAllocScratchFloats / 2,
+          "cia_mlp_linear_backward_weight_noalloc: N * (K + 1) exceeds the static scratch "
+          "(N * (K + 1) <= 2138240 floats, e.g. N = K = 1024 is too large; use "
+          "cia_mlp_linear_backward_weight with a workspace)");
+  hipStream_t st = (hipStream_t)stream;
+  float* ws = nullptr;
+  CIA_TRY(scratch_on_current_device(st, &ws));
+  return wgrad_launch(st, M, N, K, dZ, X, dW, db, ws, kNoAllocScratchFloats, accumulate);
 }
 
 int64_t cia_mlp_backward_weight_workspace(int64_t N, int64_t K) {

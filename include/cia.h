@@ -551,6 +551,20 @@ cia_status cia_mlp_linear_backward_weight(void* stream, int64_t M, int64_t N, in
                                           float* workspace, int64_t workspace_floats,
                                           int32_t accumulate);
 int64_t cia_mlp_backward_weight_workspace(int64_t N, int64_t K);
+/* Allocation-free variant of cia_mlp_linear_backward_weight (no counterpart in
+ * the reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight training or serving loops, harnesses that forbid
+ * hipMalloc).  The split partials live in a static device array of the library
+ * (17.1 MB per GPU, held until the process ends); requires N * (K + 1) <=
+ * 2138240 floats and rejects larger shapes with CIA_ERR_INVALID_ARGUMENT.  The
+ * stream must belong to the current device.  It calls no hipMalloc / hipFree.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams or host threads) race and must be serialised by the
+ * caller.  Results equal cia_mlp_linear_backward_weight given a workspace of
+ * cia_mlp_backward_weight_workspace(N, K) floats, bit for bit, for N, K <= 256. */
+cia_status cia_mlp_linear_backward_weight_noalloc(void* stream, int64_t M, int64_t N, int64_t K,
+                                                  const void* dZ, const void* X, float* dW,
+                                                  float* db, int32_t accumulate);
 cia_status cia_mlp_adam_step(void* stream, int64_t n, float* params, const float* grad,
                              float* m1, float* m2, void* params_bf16, float lr, float beta1,
                              float beta2, float eps, int32_t step, float weight_decay);
