@@ -567,8 +567,9 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     )
 
 
-class RecoverableNonlinearStepError(RuntimeError):
-    """``src/fem/solvers/errors.py`` recoverable step failure (reason + diagnostics)."""
+class NonlinearStepConvergenceError(RuntimeError):
+    """A failed global Newton step with its diagnostics
+    (``src/fem/dynamics/material.py:75-82``); not eligible for a cut-back."""
 
     def __init__(self, reason: str, diagnostics: dict):
         super().__init__(f"Implicit Newmark step failed: {reason}.")
@@ -576,8 +577,9 @@ class RecoverableNonlinearStepError(RuntimeError):
         self.diagnostics = diagnostics
 
 
-class NonlinearStepConvergenceError(RecoverableNonlinearStepError):
-    pass
+class RecoverableNonlinearStepError(NonlinearStepConvergenceError):
+    """A convergence failure eligible for the configured time-step cut-back
+    (``src/fem/dynamics/material.py:85-86``)."""
 
 
 def cut_back_time_step(current, initial, settings):

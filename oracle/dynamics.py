@@ -21,6 +21,8 @@ dissipation >= 0 and the CFL rejection (``test_beam_response_benchmarks.py:70``)
 
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 
 from oracle import hex8, j2vp
@@ -157,24 +159,56 @@ def explicit(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, cfl=Non
 
 
 class NewmarkStepError(RuntimeError):
-    def __init__(self, reason, diagnostics):
+    """A failed Newton step (``NonlinearStepConvergenceError``, ``material.py:75-82``);
+    ``recoverable`` marks the reasons eligible for a cut-back
+    (``RecoverableNonlinearStepError``, ``material.py:85-86``)."""
+
+    def __init__(self, reason, diagnostics, recoverable=True):
         super().__init__(f"Implicit Newmark step failed: {reason}.")
         self.reason = reason
         self.diagnostics = diagnostics
+        self.recoverable = recoverable
+
+
+@dataclass(frozen=True)
+class Adaptive:
+    """``AdaptiveTimeStepSettings`` (``src/fem/solvers/data.py:51-74``)."""
+
+    enabled: bool = False
+    minimum_factor: float = 1.0 / 16.0
+    maximum_factor: float = 2.0
+    cutback_factor: float = 0.5
+    growth_factor: float = 2.0
+    easy_iteration_limit: int = 4
+    easy_step_count: int = 3
+
+
+def cut_back(current, initial, adaptive: Adaptive):
+    """``cut_back_time_step`` (``src/fem/solvers/time_step.py:12-21``)."""
+    return max(initial * adaptive.minimum_factor, current * adaptive.cutback_factor)
+
+
+def grow(current, initial, adaptive: Adaptive):
+    """``grow_time_step`` (``src/fem/solvers/time_step.py:24-31``)."""
+    return min(initial * adaptive.maximum_factor, current * adaptive.growth_factor)
 
 
 def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, beta=0.25,
-            gamma=0.5):
+            gamma=0.5, adaptive: Adaptive | None = None):
     """Implicit Newmark with full-step Newton and Armijo (``solve_material_implicit_newmark``,
     ``material.py:300-443``; ``_solve_implicit_step`` ``:446-662``; ``_evaluate_implicit_trial``
-    ``:665-701``), fixed step (adaptivity disabled), consistent mass, SuperLU solves.
+    ``:665-701``), consistent mass, SuperLU solves, and the adaptive cut-back / growth
+    of the step (``material.py:377-430``, ``time_step.py:12-31``) when
+    ``adaptive.enabled``.
 
-    ``newton`` is an ``oracle.rve.Newton``.  Returns histories like
-    ``MaterialDynamicSolution`` (consistent kinetic energy ``:859-864``)."""
+    ``newton`` is an ``oracle.rve.Newton``.  Returns histories of the accepted
+    states like ``MaterialDynamicSolution`` (consistent kinetic energy ``:859-864``),
+    with ``times``, ``time_steps``, ``rejected_counts`` and ``rejected_reasons``."""
     from scipy.sparse.linalg import spsolve
 
     from oracle.rve import ArmijoExhausted, armijo_search
 
+    adaptive = adaptive or Adaptive()
     n = box.n_dof
     M = hex8.consistent_mass(box.nodes, box.elements, box.density)
     fixed = np.zeros(n, dtype=bool)
@@ -192,7 +226,9 @@ def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, 
     hist = {"u": [u.copy()], "v": [v.copy()], "a": [a.copy()], "kinetic": [0.5 * v @ (M @ v)],
             "free": [np.sum(asm["free_energy"] * w)], "dissipated": [0.0], "external_work": [0.0],
             "stress": [asm["stress"].copy()], "internal_force": [asm["internal_force"].copy()],
-            "newton_iterations": [0], "armijo_backtracks": [0]}
+            "newton_iterations": [0], "armijo_backtracks": [0], "times": [0.0], "time_steps": [0.0],
+            "rejected_counts": [0]}
+    rejected_reasons = [()]
 
     def trial(disp, up, ustar, vstar, t_dt, ext, committed):
         acc = (disp - ustar) / (beta * t_dt**2)
@@ -201,10 +237,11 @@ def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, 
         res = M @ acc + r["internal_force"] - ext
         return acc, vel, r, res
 
-    for s in range(n_steps):
-        t0, t1 = s * dt, (s + 1) * dt
-        ustar = u + dt * v + dt**2 * (0.5 - beta) * a
-        vstar = v + dt * (1.0 - gamma) * a
+    def step(t0, u, v, a, state, h):
+        """``_solve_implicit_step`` (``material.py:446-662``)."""
+        t1 = t0 + h
+        ustar = u + h * v + h**2 * (0.5 - beta) * a
+        vstar = v + h * (1.0 - gamma) * a
         disp = ustar.copy()
         disp[fixed] = 0.0
         ext = scale(t1) * f_unit
@@ -212,7 +249,7 @@ def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, 
         last_inc = np.inf
         backtracks = 0
         for it in range(newton.max_iterations + 1):
-            acc, vel, r, res = trial(disp, u, ustar, vstar, dt, ext, state)
+            acc, vel, r, res = trial(disp, u, ustar, vstar, h, ext, state)
             rn = float(np.linalg.norm(res[free]))
             r0 = rn if r0 is None else r0
             rtol = max(newton.residual_absolute_tolerance, newton.residual_relative_tolerance * r0)
@@ -221,11 +258,13 @@ def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, 
             if rn <= rtol and (it == 0 or last_inc <= itol):
                 break
             if it == newton.max_iterations:
-                raise NewmarkStepError("maximum_iterations_exceeded", {"iteration": it})
-            keff = (r["tangent_matrix"] + M / (beta * dt**2))[free][:, free].tocsc()
+                raise NewmarkStepError("maximum_iterations_exceeded",
+                                       {"iteration": it, "time_step": h})
+            keff = (r["tangent_matrix"] + M / (beta * h**2))[free][:, free].tocsc()
             inc = spsolve(keff, -res[free])
             if not np.all(np.isfinite(inc)):
-                raise NewmarkStepError("non_finite_increment", {"iteration": it})
+                raise NewmarkStepError("non_finite_increment", {"iteration": it, "time_step": h},
+                                       recoverable=False)
             inc_n = float(np.linalg.norm(inc))
             if rn <= rtol and inc_n <= itol:
                 break
@@ -233,32 +272,71 @@ def newmark(box: Box, dt, n_steps, fixed_dofs, f_unit, scale, settings, newton, 
                 def evaluate(alpha, base=disp, d=inc):
                     cand = base.copy()
                     cand[free] += alpha * d
-                    return trial(cand, u, ustar, vstar, dt, ext, state)[3][free]
+                    return trial(cand, u, ustar, vstar, h, ext, state)[3][free]
                 try:
                     alpha, bt, _ = armijo_search(res[free], evaluate, newton.armijo)
                 except ArmijoExhausted as exc:
-                    raise NewmarkStepError("armijo_exhausted", exc.diagnostics) from exc
+                    raise NewmarkStepError("armijo_exhausted",
+                                           {"iteration": it, "time_step": h,
+                                            "armijo": exc.diagnostics}) from exc
                 disp[free] += alpha * inc
                 last_inc = float(np.linalg.norm(alpha * inc))
                 backtracks += bt
             else:
                 disp[free] += inc
                 last_inc = inc_n
-        hist["dissipated"].append(hist["dissipated"][-1] + dt * np.sum(r["dissipation"] * w))
-        hist["external_work"].append(hist["external_work"][-1]
-                                     + 0.5 * (scale(t0) * f_unit + ext) @ (disp - u))
-        u, v, a, state = disp, vel, acc, r["state"]
+        return {"t": t1, "u": disp, "v": vel, "a": acc, "r": r, "it": it, "bt": backtracks,
+                "diss": h * np.sum(r["dissipation"] * w),
+                "work": 0.5 * (scale(t0) * f_unit + ext) @ (disp - u)}
+
+    final_time = dt * n_steps
+    minimum = dt * adaptive.minimum_factor
+    cur_dt = dt
+    easy = 0
+    pending = []
+    t = 0.0
+    while t < final_time:
+        h = min(cur_dt, final_time - t)
+        try:
+            out = step(t, u, v, a, state, h)
+        except NewmarkStepError as err:
+            if not (adaptive.enabled and err.recoverable):
+                raise
+            pending.append(err.reason)
+            if h <= minimum:
+                err.diagnostics["minimum_time_step"] = minimum
+                err.diagnostics["rejected_reasons"] = tuple(pending)
+                raise
+            cur_dt = cut_back(h, dt, adaptive)
+            easy = 0
+            continue
+        t, u, v, a, state = out["t"], out["u"], out["v"], out["a"], out["r"]["state"]
+        hist["dissipated"].append(hist["dissipated"][-1] + out["diss"])
+        hist["external_work"].append(hist["external_work"][-1] + out["work"])
         hist["u"].append(u.copy())
         hist["v"].append(v.copy())
         hist["a"].append(a.copy())
         hist["kinetic"].append(0.5 * v @ (M @ v))
-        hist["free"].append(np.sum(r["free_energy"] * w))
-        hist["stress"].append(r["stress"].copy())
-        hist["internal_force"].append(r["internal_force"].copy())
-        hist["newton_iterations"].append(it)
-        hist["armijo_backtracks"].append(backtracks)
+        hist["free"].append(np.sum(out["r"]["free_energy"] * w))
+        hist["stress"].append(out["r"]["stress"].copy())
+        hist["internal_force"].append(out["r"]["internal_force"].copy())
+        hist["newton_iterations"].append(out["it"])
+        hist["armijo_backtracks"].append(out["bt"])
+        hist["times"].append(t)
+        hist["time_steps"].append(h)
+        hist["rejected_counts"].append(len(pending))
+        rejected_reasons.append(tuple(pending))
+        is_easy = (not pending and out["bt"] == 0 and out["it"] <= adaptive.easy_iteration_limit)
+        easy = easy + 1 if is_easy else 0
+        pending = []
+        if adaptive.enabled and easy >= adaptive.easy_step_count:
+            cur_dt = grow(h, dt, adaptive)
+            easy = 0
+        else:
+            cur_dt = h
     out = {k: np.asarray(v) for k, v in hist.items()}
     out["energy_residual"] = (out["kinetic"] + out["free"] + out["dissipated"] - out["external_work"]
                               - (out["kinetic"][0] + out["free"][0]))
     out["state"] = state
+    out["rejected_reasons"] = tuple(rejected_reasons)
     return out

@@ -138,3 +138,46 @@ def test_fe2_implicit_embedding_matches_direct(gpu):
     assert sc(a.stresses, b.stresses, 10.0) <= 1e-9
     assert sc(a.energy_residual, b.energy_residual, 0.01) <= 1e-9
     assert sc(a.equivalent_plastic_strain, b.equivalent_plastic_strain, 0.01) <= 1e-9
+
+
+def test_newmark_adaptive_cut_back_and_growth_matches_oracle(gpu):
+    """material.py:377-430 with adaptive stepping on and three Newton iterations
+    allowed: the same steps are rejected and cut back, grown after three easy
+    steps and clipped at the final time as in the oracle; accepted times, step
+    sizes, rejection counts and reasons, Newton counts and the histories agree."""
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import solvers as S
+
+    j2 = M.J2ViscoplasticMaterial("mortar", 1.0, E, NU, 10.0, 80.0, 0.02, 10.0, 2.0)
+    oj2 = OJ.J2VPParams(E, NU, 10.0, 80.0, 0.02, 10.0, 2.0, 1.0)
+    bundle = D.build_structured_box_bundle(LEN, DIV, j2)
+    obox = OD.Box(LEN, DIV, [("m", oj2)])
+    load = D.build_half_sine_pressure_load(bundle, 60.0, 0.02, patch=((1, 5), (0, 3)))
+    bottom = bundle.mesh_info.boundary_groups["bottom"].nodes
+    dofs = (3 * bottom[:, None] + np.arange(3)).ravel()
+    newton = S.NonlinearNewtonSettings(3, 1e-9, 1e-10, 1e-14, 1e-10, S.ArmijoSettings(True, 1e-4, 0.5, 20))
+    n = bundle.mesh_info.dof_map.size
+    events = []
+    sol = D.solve_material_implicit_newmark(
+        bundle, S.TimeIntegrationSettings("implicit_newmark", 1e-3, 30,
+                                          adaptive=S.AdaptiveTimeStepSettings(True)),
+        np.zeros(n), np.zeros(n), S.DirichletDofSet(dofs, np.zeros(dofs.size)), load,
+        "three_dimensional", M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10), newton,
+        events.append, linear_settings=S.LinearSolverSettings(relative_tolerance=1e-13, max_iterations=20000))
+    ref = OD.newmark(obox, 1e-3, 30, dofs, load.unit_force, OD.half_sine(60.0, 0.02),
+                     OJ.UpdateSettings(50, 1e-10, 1e-12, 1e-10),
+                     Newton(3, 1e-9, 1e-10, 1e-14, 1e-10, Armijo(True, 1e-4, 0.5, 20)),
+                     adaptive=OD.Adaptive(True))
+    assert np.any(ref["rejected_counts"] > 0) and np.any(ref["time_steps"] > 1e-3)
+    np.testing.assert_allclose(sol.times, ref["times"], rtol=1e-14, atol=1e-18)
+    np.testing.assert_allclose(sol.accepted_time_steps, ref["time_steps"], rtol=1e-12, atol=1e-18)
+    np.testing.assert_array_equal(sol.rejected_step_counts, ref["rejected_counts"])
+    assert tuple(sol.rejected_reasons) == ref["rejected_reasons"]
+    np.testing.assert_array_equal(sol.newton_iterations, ref["newton_iterations"])
+    assert _rel(sol.displacement, ref["u"]) < 1e-9
+    assert _rel(sol.velocity, ref["v"]) < 1e-9
+    assert _rel(sol.stresses, ref["stress"]) < 1e-9
+    assert _rel(sol.dissipated_energy, ref["dissipated"]) < 1e-9
+    assert len(events) == len(ref["times"]) - 1
+    assert [e.accepted_step for e in events] == list(range(1, len(ref["times"])))

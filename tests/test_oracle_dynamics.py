@@ -135,3 +135,67 @@ def test_newmark_viscoplastic_bound_and_dissipation():
     assert out["dissipated"][-1] > 0.0
     assert np.all(np.diff(out["dissipated"]) >= -1e-14)
     assert np.max(np.abs(out["energy_residual"])) < 0.01 * np.max(np.abs(out["external_work"]))
+
+
+def test_time_step_rules_match_the_reference_cases():
+    """``test_convergence_time_step_rules_respect_configured_bounds``
+    (tests/unit/fem/test_nonlinear_controls.py:61-68)."""
+    ad = OD.Adaptive(True)
+    assert OD.cut_back(1.0, 1.0, ad) == 0.5
+    assert OD.cut_back(0.0625, 1.0, ad) == 0.0625
+    assert OD.grow(1.0, 1.0, ad) == 2.0
+    assert OD.grow(2.0, 1.0, ad) == 2.0
+
+
+def _adaptive_case(max_iterations, adaptive, steps=30):
+    from oracle.rve import Armijo, Newton
+
+    mat = OJ.J2VPParams(3339.9160679991464, 0.36994096308414537, 10.0, 80.0, 0.02, 10.0, 2.0, 1.0)
+    box = _slab(mat)
+    fixed, funit = _slab_load(box)
+    newton = Newton(max_iterations, 1e-9, 1e-10, 1e-14, 1e-10, Armijo(True))
+    return OD.newmark(box, 1e-3, steps, fixed, funit, OD.half_sine(60.0, 0.02), OJ.UpdateSettings(),
+                      newton, adaptive=adaptive)
+
+
+def test_newmark_adaptive_cut_back_and_growth():
+    """material.py:377-430: with three Newton iterations allowed, steps that need
+    more are rejected and retried at half the step; after three easy accepted
+    steps the step doubles (at most twice the initial one); the last step is cut
+    to reach the final time.  The controller's decisions are replayed from the
+    recorded Newton counts and rejections, and the plastic run stays within the
+    reference's implicit energy bound (0.01 of the work)."""
+    ad = OD.Adaptive(True)
+    out = _adaptive_case(3, ad)
+    h = out["time_steps"][1:]
+    assert out["times"][-1] == pytest.approx(0.03, rel=1e-14)
+    assert np.any(out["rejected_counts"] > 0) and np.any(h > 1e-3) and np.any(h < 1e-3)
+    assert set(r for rr in out["rejected_reasons"] for r in rr) == {"maximum_iterations_exceeded"}
+    # replay the controller
+    cur, easy, t = 1e-3, 0, 0.0
+    for k in range(h.size):
+        trial = min(cur, 0.03 - t)
+        for _ in range(out["rejected_counts"][k + 1]):
+            trial = OD.cut_back(trial, 1e-3, ad)
+        trial = min(trial, 0.03 - t)
+        assert h[k] == pytest.approx(trial, rel=1e-12), k
+        t += h[k]
+        is_easy = (out["rejected_counts"][k + 1] == 0 and out["armijo_backtracks"][k + 1] == 0
+                   and out["newton_iterations"][k + 1] <= ad.easy_iteration_limit)
+        easy = easy + 1 if is_easy else 0
+        if easy >= ad.easy_step_count:
+            cur, easy = OD.grow(h[k], 1e-3, ad), 0
+        else:
+            cur = h[k]
+    assert np.max(np.abs(out["energy_residual"])) < 0.01 * np.max(np.abs(out["external_work"]))
+    # without the controller the same Newton limit fails the run
+    with pytest.raises(OD.NewmarkStepError) as err:
+        _adaptive_case(3, OD.Adaptive(False))
+    assert err.value.reason == "maximum_iterations_exceeded"
+
+
+def test_newmark_adaptive_without_failures_equals_fixed_steps():
+    fixed = _adaptive_case(20, OD.Adaptive(False), steps=12)
+    still = _adaptive_case(20, OD.Adaptive(True, easy_step_count=10**6), steps=12)
+    for key in ("u", "v", "a", "stress", "dissipated"):
+        assert np.array_equal(fixed[key], still[key]), key
