@@ -60,15 +60,88 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b, bool conj_b) {
 // rounding error.
 __device__ __forceinline__ int fdiv(int t, float inv_n) { return (int)(((float)t + 0.5f) * inv_n); }
 
+// The factorisation rule of dft_factor at compile time.
+constexpr int ct_n1(int n) {
+  int n1 = 1;
+  for (int d = 1; d * d <= n; ++d)
+    if (n % d == 0) n1 = d;
+  return n1;
+}
+
+// a * w where w = W_R^idx (W_R = exp(-2 pi i / R), conjugated when INV); idx is a
+// constant once the caller's loops are unrolled, so the trivial factors 1, -1
+// and -+i fold into moves and sign changes.
+template <class C2>
+__device__ __forceinline__ C2 tw_mul(C2 a, C2 w, int idx, int R, bool inv) {
+  if (idx == 0) return a;
+  if (2 * idx == R) return C2{-a.x, -a.y};
+  if (4 * idx == R) return inv ? C2{-a.y, a.x} : C2{a.y, -a.x};      // -i (forward)
+  if (4 * idx == 3 * R) return inv ? C2{a.y, -a.x} : C2{-a.y, a.x};  // +i (forward)
+  return cmul(a, w, inv);
+}
+
+// dft_stage for an edge N fixed at compile time: one thread per small DFT of a
+// line (STEP_A: the n1-point DFT of the inputs j2, j2 + n2, ...; STEP_B: the
+// n2-point DFT of the inputs k1, k1 + n1, ...), its twiddles W_R held in
+// registers, every index constant.  The same sums as dft_stage, in another order.
+template <int N, bool STEP_A, bool INV, class C2>
+__device__ __forceinline__ void dft_stage_ct(const C2* __restrict__ in, C2* __restrict__ out,
+                                             int n_lines, int ls, int es,
+                                             const C2* __restrict__ w) {
+  constexpr int n1 = ct_n1(N), n2 = N / n1;
+  constexpr int R = STEP_A ? n1 : n2;  // points of one small DFT
+  constexpr int M = STEP_A ? n2 : n1;  // small DFTs per line
+  const C2* wr = STEP_A ? w : w + n1;
+  const C2* wn = w + n1 + n2;
+  C2 tw[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) tw[r] = wr[r];
+  const int total = n_lines * M;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int line = t / M, q = t - line * M;
+    const C2* src = in + line * ls;
+    C2* dst = out + line * ls;
+    C2 a[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a[j] = STEP_A ? src[(n2 * j + q) * es] : src[(j * n1 + q) * es];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      C2 acc = a[0];
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        const int idx = (j * k) % R;
+        const C2 p = tw_mul(a[j], tw[idx], idx, R, INV);
+        acc.x += p.x;
+        acc.y += p.y;
+      }
+      if (STEP_A) {
+        if (k != 0) acc = cmul(acc, wn[q * k], INV);  // W_N^(j2 k1), j2 = q
+        dst[(q * n1 + k) * es] = acc;
+      } else {
+        dst[(q + n1 * k) * es] = acc;
+      }
+    }
+  }
+}
+
 // One stage over n_lines lines of length n: element e of line l at l*ls + e*es.
 // STEP_A: out[(j2 n1 + k1)] = W_n^(j2 k1) sum_j1 in[n2 j1 + j2] W_n1^(j1 k1)
 // STEP_B: out[(k1 + n1 k2)] = sum_j2 in[j2 n1 + k1] W_n2^(j2 k2)
 // (j1 k1 mod n1 and j2 k2 mod n2 advance incrementally; j2 k1 < n needs no mod.)
-// w: twiddles of the axis in LDS.
+// w: twiddles of the axis in LDS.  The edges of the benchmark workloads run the
+// compile-time specialisation above (dft_stage_ct).
 template <bool STEP_A, bool INV, class C2>
 __device__ __forceinline__ void dft_stage(const C2* __restrict__ in, C2* __restrict__ out,
                                           int n_lines, int ls, int es, const DftAxis& ax,
                                           const C2* __restrict__ w) {
+  switch (ax.n) {
+    case 12: dft_stage_ct<12, STEP_A, INV>(in, out, n_lines, ls, es, w); return;
+    case 16: dft_stage_ct<16, STEP_A, INV>(in, out, n_lines, ls, es, w); return;
+    case 24: dft_stage_ct<24, STEP_A, INV>(in, out, n_lines, ls, es, w); return;
+    case 32: dft_stage_ct<32, STEP_A, INV>(in, out, n_lines, ls, es, w); return;
+    case 48: dft_stage_ct<48, STEP_A, INV>(in, out, n_lines, ls, es, w); return;
+    default: break;
+  }
   const int n = ax.n, n1 = ax.n1, n2 = ax.n2;
   const C2* w1 = w;
   const C2* w2 = w + n1;

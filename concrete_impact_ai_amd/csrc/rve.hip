@@ -59,30 +59,59 @@ struct Red {
   int maxblocks;
 };
 
-// Last-block finalisation of NV block partial sums into out[0..NV).
+// Every block sums the NV partial rows of the previous kernel into out[0..NV) in
+// one LDS round: each thread sums its strided partials of all rows, each wave
+// reduces them (wave_sum), and after one barrier every thread adds the wave sums
+// in wave order.  The order is the same in every block (and the same as one
+// block_sum per row), so all blocks obtain bitwise-identical scalars.  A kernel
+// calls it at most once (its LDS array is not reused).
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int P,
-                                                double out[NV], double* lds) {
-  __shared__ double bc[NV];
+                                                double out[NV], double* /*lds*/) {
+  constexpr int NW = kBlock / 64;
+  __shared__ double rp[NV][NW];
+  double a[NV];
+#pragma unroll
+  for (int s = 0; s < NV; ++s) a[s] = 0.0;
+  for (int b = threadIdx.x; b < P; b += blockDim.x) {
+#pragma unroll
+    for (int s = 0; s < NV; ++s) a[s] += part[s * P + b];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int s = 0; s < NV; ++s) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < P; b += blockDim.x) a += part[s * P + b];
-    a = block_sum<kBlock>(a, lds);
-    if (threadIdx.x == 0) bc[s] = a;
+    const double w = wave_sum(a[s]);
+    if (lane == 0) rp[s][wid] = w;
   }
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < NV; ++s) out[s] = bc[s];
+  for (int s = 0; s < NV; ++s) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) r += rp[s][w];
+    out[s] = r;
+  }
 }
 
+// This block's NV partial sums, one LDS round (the same summation order as one
+// block_sum per value); thread s writes row s.  A kernel calls it at most once.
 template <int NV>
 __device__ __forceinline__ void write_partials(const double (&v)[NV], double* __restrict__ part,
-                                               double* lds) {
+                                               double* /*lds*/) {
+  constexpr int NW = kBlock / 64;
+  __shared__ double wp[NV][NW];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int s = 0; s < NV; ++s) {
-    const double t = block_sum<kBlock>(v[s], lds);
-    if (threadIdx.x == 0) part[s * gridDim.x + blockIdx.x] = t;
+    const double w = wave_sum(v[s]);
+    if (lane == 0) wp[s][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) r += wp[threadIdx.x][w];
+    part[threadIdx.x * gridDim.x + blockIdx.x] = r;
   }
 }
 
@@ -717,6 +746,12 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ double2 wz[kDftMaxTw];
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
+  // every global load of the tile is issued before the wait for the scalars
+  const int tiles_x = (nx + tile_w - 1) / tile_w;
+  const int bz = (int)xcd_block();  // neighbouring tiles share the XCD's L2
+  ZcolTile tile;
+  zcol_prefetch(nx, ny, nz, tile_w, bz % tiles_x, bz / tiles_x, S, C, kinv, tile);
+  load_twiddles(az, wz);
   double u[4];
   reduce_partials<4>(part_upd, P, u, lds);
   double tol2;
@@ -734,11 +769,8 @@ __global__ void __launch_bounds__(kBlock)
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = isfinite(u[0]) ? 1 : 2;
     return;
   }
-  load_twiddles(az, wz);
-  const int tiles_x = (nx + tile_w - 1) / tile_w;
-  const int xt = (int)blockIdx.x % tiles_x, ky = (int)blockIdx.x / tiles_x;
   const double sr[3] = {u[1], u[2], u[3]};
-  const double v[1] = {zcol_solve(nx, ny, nz, tile_w, xt, ky, S, C, kinv, sr, inv_n, az, dft_sm, wz)};
+  const double v[1] = {zcol_solve(tile, S, C, kinv, sr, inv_n, az, dft_sm, wz)};
   write_partials<1>(v, part_rz, lds);
 }
 

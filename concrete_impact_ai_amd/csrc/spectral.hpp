@@ -45,28 +45,80 @@ __host__ __device__ inline int xline_group(int nx) { return nx >= 256 ? 1 : 256 
 inline size_t xline_lds_bytes(int nx) { return 6 * (size_t)xline_group(nx) * (nx + 1) * sizeof(double2); }
 
 // z-column tile (kx in [tile_w xt, tile_w (xt + 1)), ky) of an RVE's spectra: forward
-// z-DFT, Z = Khat^-1 (R - sum r) / N per mode, inverse z-DFT, in place.  Returns
-// this thread's part of r.z = sum_k Re(conj(R - sum r) Z).  The caller loads the
-// z twiddles into wz first.
-__device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w, int xt, int ky,
-                                             double2* __restrict__ S, int64_t C,
+// z-DFT, Z = Khat^-1 (R - sum r) / N per mode, inverse z-DFT, in place.
+//
+// Split in two so that the kernel can issue every global load of the tile before
+// it waits for the PCG scalars: zcol_prefetch reads the first kZcolPf spectral
+// values of each thread and the symbols of its first kZcolPfModes modes into
+// registers; zcol_solve stores them into LDS (loading the rest of a large tile
+// directly), transforms, solves and writes back, and returns this thread's part
+// of r.z = sum_k Re(conj(R - sum r) Z).  The caller loads the z twiddles into wz
+// between the two.
+constexpr int kZcolPf = 4, kZcolPfModes = 2;
+
+struct ZcolTile {
+  int ldz, TW, width, per;
+  int64_t plane, col0;
+  double2 pre[kZcolPf];
+  hipFloatComplex q[kZcolPfModes][6];
+};
+
+__device__ __forceinline__ void zcol_prefetch(int nx, int ny, int nz, int tile_w, int xt, int ky,
+                                              const double2* __restrict__ S, int64_t C,
+                                              const hipFloatComplex* __restrict__ kinv,
+                                              ZcolTile& t) {
+  t.ldz = nz + 1;
+  t.TW = tile_w;
+  const int i0 = xt * tile_w;
+  t.width = min(tile_w, nx - i0);
+  t.plane = (int64_t)nx * ny;
+  t.col0 = i0 + (int64_t)nx * ky;
+  t.per = tile_w * nz;
+#pragma unroll
+  for (int j = 0; j < kZcolPf; ++j) {
+    const int u = threadIdx.x + j * blockDim.x;
+    t.pre[j] = make_double2(0.0, 0.0);
+    if (u < 3 * t.per) {
+      const int m = u / t.per, rem = u - m * t.per;
+      const int z = rem / t.TW, ii = rem - z * t.TW;
+      if (ii < t.width) t.pre[j] = S[(int64_t)m * C + t.col0 + ii + t.plane * z];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kZcolPfModes; ++j) {
+    const int u = threadIdx.x + j * blockDim.x;
+    if (u < t.per) {
+      const int kz = u / t.TW, ii = u - kz * t.TW;
+      if (ii < t.width) {
+        const int64_t k = t.col0 + ii + t.plane * kz;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t.q[j][c] = kinv[6 * k + c];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double zcol_solve(const ZcolTile& t, double2* __restrict__ S, int64_t C,
                                              const hipFloatComplex* __restrict__ kinv,
                                              const double sr[3], double inv_n, const DftAxis& az,
                                              double2* sm, const double2* wz) {
-  const int ldz = nz + 1;
-  const int TW = tile_w;
-  const int i0 = xt * TW;
-  const int width = min(TW, nx - i0);
-  const int64_t plane = (int64_t)nx * ny;
-  const int64_t col0 = i0 + (int64_t)nx * ky;
+  const int ldz = t.ldz, TW = t.TW, width = t.width, per = t.per;
   double2* A = sm;                   // [3][TW][nz + 1]
   double2* T = sm + 3 * TW * ldz;    // [3][TW][nz + 1]
-  const int per = TW * nz;
-  for (int u = threadIdx.x; u < 3 * per; u += blockDim.x) {
+#pragma unroll
+  for (int j = 0; j < kZcolPf; ++j) {
+    const int u = threadIdx.x + j * blockDim.x;
+    if (u < 3 * per) {
+      const int m = u / per, rem = u - m * per;
+      const int z = rem / TW, ii = rem - z * TW;
+      A[(m * TW + ii) * ldz + z] = t.pre[j];
+    }
+  }
+  for (int u = threadIdx.x + kZcolPf * blockDim.x; u < 3 * per; u += blockDim.x) {
     const int m = u / per, rem = u - m * per;
     const int z = rem / TW, ii = rem - z * TW;
     A[(m * TW + ii) * ldz + z] =
-        ii < width ? S[(int64_t)m * C + col0 + ii + plane * z] : make_double2(0.0, 0.0);
+        ii < width ? S[(int64_t)m * C + t.col0 + ii + t.plane * z] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   // the three components' lines in one pass per stage (lines past `width` hold zeros)
@@ -75,10 +127,10 @@ __device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w,
   dft_stage<false, false>(T, A, 3 * TW, ldz, 1, az, wz);
   __syncthreads();
   double rz = 0.0;
-  for (int u = threadIdx.x; u < per; u += blockDim.x) {
+  for (int u = threadIdx.x, j = 0; u < per; u += blockDim.x, ++j) {
     const int kz = u / TW, ii = u - kz * TW;
     if (ii >= width) continue;
-    const int64_t k = col0 + ii + plane * kz;
+    const int64_t k = t.col0 + ii + t.plane * kz;
     double ar[3], ai[3], zr[3], zi[3];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
@@ -86,7 +138,14 @@ __device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w,
       ar[m] = v.x - sr[m];
       ai[m] = v.y;
     }
-    spec_solve(kinv + 6 * k, false, ar, ai, zr, zi);
+    if (j < kZcolPfModes) {
+      hipFloatComplex q[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] = j == 0 ? t.q[0][c] : t.q[kZcolPfModes - 1][c];
+      spec_solve(q, false, ar, ai, zr, zi);
+    } else {
+      spec_solve(kinv + 6 * k, false, ar, ai, zr, zi);
+    }
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       zr[m] *= inv_n;
@@ -103,7 +162,7 @@ __device__ __forceinline__ double zcol_solve(int nx, int ny, int nz, int tile_w,
   for (int u = threadIdx.x; u < 3 * per; u += blockDim.x) {
     const int m = u / per, rem = u - m * per;
     const int z = rem / TW, ii = rem - z * TW;
-    if (ii < width) S[(int64_t)m * C + col0 + ii + plane * z] = A[(m * TW + ii) * ldz + z];
+    if (ii < width) S[(int64_t)m * C + t.col0 + ii + t.plane * z] = A[(m * TW + ii) * ldz + z];
   }
   return rz;
 }
