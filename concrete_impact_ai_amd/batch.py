@@ -40,6 +40,7 @@ from concrete_impact_ai_amd.rve import (
     StructuredHex8RVE,
     _box_lengths,
     _phases_of,
+    linear_method,
     rve_descriptor,
 )
 
@@ -72,7 +73,7 @@ class RVEBatch:
         if any(m.newton_settings != models[0].newton_settings or
                m.linear_settings != models[0].linear_settings for m in models):
             raise ValueError("All RVEs of a batch share Newton and linear-solver settings.")
-        if models[0].linear_settings.method != "spectral":
+        if linear_method(models[0].linear_settings) != "spectral":
             raise ValueError("The batched RVE solver uses the spectral preconditioner.")
         self.models = models
         self.precision = p0

@@ -133,6 +133,60 @@ class SeparableLoad:
         return self.scale(time) * self.unit_force
 
 
+class _DeviceLoad:
+    """Device view of a load function ``f(t) -> f_ext`` (the reference's
+    ``LoadFunction``, called at ``src/fem/dynamics/material.py:139,165,337``).
+
+    A :class:`SeparableLoad` keeps one device vector and hands its scale to the
+    kernels.  Any other callable is evaluated on the host at each time the solver
+    needs and copied to a device vector that the kernels take with scale 1 (the
+    last few times are kept, so a time evaluated twice is copied once)."""
+
+    def __init__(self, load_function, n_dof: int, dev):
+        self.fn = load_function
+        self.n_dof = int(n_dof)
+        self.dev = dev
+        self.separable = hasattr(load_function, "unit_force") and hasattr(load_function, "scale")
+        if self.separable:
+            self.unit = torch.as_tensor(np.asarray(load_function.unit_force, dtype=np.float64),
+                                        device=dev)
+            if self.unit.numel() != self.n_dof:
+                raise ValueError("The load's unit force does not match the number of dofs.")
+        self._cache: dict = {}
+
+    def scale(self, t: float) -> float:
+        return float(self.fn.scale(t)) if self.separable else 1.0
+
+    def host(self, t: float) -> np.ndarray:
+        if self.separable:
+            return float(self.fn.scale(t)) * np.asarray(self.fn.unit_force, dtype=np.float64)
+        f = np.asarray(self.fn(float(t)), dtype=np.float64)
+        if f.shape != (self.n_dof,):
+            raise ValueError(f"load_function({t}) returned shape {f.shape}, expected ({self.n_dof},).")
+        return f
+
+    def at(self, t: float):
+        """(device vector, scale) with f(t) = scale * vector."""
+        if self.separable:
+            return self.unit, float(self.fn.scale(t))
+        key = float(t)
+        vec = self._cache.get(key)
+        if vec is None:
+            if len(self._cache) >= 4:
+                self._cache.pop(next(iter(self._cache)))
+            vec = torch.as_tensor(self.host(t), device=self.dev)
+            self._cache[key] = vec
+        return vec, 1.0
+
+    def work(self, t0: float, t1: float, du) -> float:
+        """Trapezoidal external work 0.5 (f(t0) + f(t1)) . du (``material.py:221-229,497-499``)."""
+        v0, s0 = self.at(t0)
+        v1, s1 = self.at(t1)
+        if self.separable:
+            return 0.5 * (s0 + s1) * float(torch.dot(self.unit, du))
+        return 0.5 * (float(torch.dot(v0, du)) + float(torch.dot(v1, du)))
+
+
 def uniform_pressure_unit_force(bundle, group: str, outward_normal, cell_mask=None):
     """``assemble_uniform_boundary_pressure`` with pressure 1 (``load.py:17-58``):
     traction -n, area / 4 to each corner of every (selected) quad."""
@@ -293,9 +347,6 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
         raise ValueError("Material dynamics on the GPU requires three_dimensional kinematics.")
     if np.any(np.asarray(dirichlet.values) != 0.0):
         raise ValueError("Material dynamics currently requires homogeneous Dirichlet values.")
-    if not hasattr(load_function, "unit_force") or not hasattr(load_function, "scale"):
-        raise TypeError("GPU explicit dynamics needs a load of the form scale(t) * unit_force "
-                        "(SeparableLoad / build_half_sine_pressure_load).")
     check_explicit_stability(bundle, time_settings)
     point_material = isinstance(bundle.material, (J2ViscoplasticMaterial, LinearElasticMaterial,
                                                   PhasedMaterialModel))
@@ -309,10 +360,11 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
     dt = float(time_settings.time_step)
     n_steps = int(time_settings.num_steps)
     times = np.arange(n_steps + 1) * dt
-    scales = np.asarray([float(load_function.scale(t)) for t in times])
     dev = device()
     box = _BoxHandle(bundle, precision)
     n_dof = bundle.mesh_info.dof_map.size
+    load = _DeviceLoad(load_function, n_dof, dev)
+    scales = np.asarray([load.scale(t) for t in times])
     n_el = bundle.mesh_info.elements.shape[0]
     n_gp = 8 * n_el
     dtype = torch.float64 if precision == "fp64" else torch.float32
@@ -327,7 +379,6 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
     a = torch.zeros(n_dof, dtype=torch.float64, device=dev)
     mass = torch.as_tensor(lumped_mass(bundle), device=dev)
     fixed = torch.as_tensor(fixed_np, device=dev)
-    funit = torch.as_tensor(load_function.unit_force, device=dev)
     state = torch.zeros((8, n_gp), dtype=dtype, device=dev)
     stress = torch.zeros((6, n_gp), dtype=dtype, device=dev)
     fint = torch.zeros(n_dof, dtype=torch.float64, device=dev)
@@ -350,7 +401,12 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
 
     def run(initialize, k0, nst):
         hist = np.zeros(((1 if initialize else 0) + nst, _lib.N_HISTORY))
-        sc = np.ascontiguousarray(scales[k0:k0 + nst + 1])
+        if load.separable:
+            funit = load.unit
+            sc = np.ascontiguousarray(scales[k0:k0 + nst + 1])
+        else:  # one step per call: the device vector holds f(t) of the step's end
+            funit = load.at(times[k0 + nst])[0]
+            sc = np.ones(nst + 1)
         fail = _lib.PointFailure()
         status = lib.cia_box_explicit_steps(
             box.handle, stream_handle(), int(initialize), dt, int(nst),
@@ -370,9 +426,17 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
     run(True, 0, 0)
     record(0)
     k = 0
+    ext_inc = []  # external work per step of a general (non-separable) load
+    u_prev = torch.empty_like(u)
     while k < n_steps:
         nst = min(stride, n_steps - k)
-        run(False, k, nst)
+        if load.separable:
+            run(False, k, nst)
+        else:
+            for j in range(k, k + nst):
+                u_prev.copy_(u)
+                run(False, j, 1)
+                ext_inc.append(load.work(times[j], times[j + 1], u - u_prev))
         k += nst
         record(k)
         if progress_callback is not None:
@@ -384,7 +448,11 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
     H = np.vstack(hist_rows)
     col = {name: H[:, i] for i, name in enumerate(_lib.HISTORY_COLUMNS)}
     volume = float(np.prod(bundle.lengths))
-    ext_work = np.concatenate([[0.0], np.cumsum(0.5 * (scales[:-1] + scales[1:]) * col["ext_du"][1:])])
+    if load.separable:
+        ext_work = np.concatenate([[0.0], np.cumsum(0.5 * (scales[:-1] + scales[1:])
+                                                    * col["ext_du"][1:])])
+    else:
+        ext_work = np.concatenate([[0.0], np.cumsum(ext_inc)])
     dissipated = np.concatenate([[0.0], np.cumsum(dt * col["dissipation_rate"][1:])])
     kinetic, free = col["kinetic"], col["free_energy"]
     residual = kinetic + free + dissipated - ext_work - (kinetic[0] + free[0])
@@ -397,7 +465,7 @@ def solve_material_explicit(bundle, time_settings: TimeIntegrationSettings, init
     final_state = MaterialState(unpack_state(phases, ppid, packed, phased))
     f_ext_hist = None
     if record_fields:
-        f_ext_hist = np.asarray([scales[s] * load_function.unit_force for s in rec["steps"]])
+        f_ext_hist = np.asarray([load.host(times[s]) for s in rec["steps"]])
     return MaterialDynamicSolution(
         times=times,
         displacement=np.asarray(rec["u"]) if record_fields else None,
@@ -448,10 +516,11 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     dt = float(time_settings.time_step)
     n_steps = int(time_settings.num_steps)
     times = np.arange(n_steps + 1) * dt
-    scales = np.asarray([float(load_function.scale(t)) for t in times])
     dev = device()
     box = _BoxHandle(bundle, "fp64", geometry_only=True)
     n_dof = bundle.mesh_info.dof_map.size
+    load = _DeviceLoad(load_function, n_dof, dev)
+    scales = np.asarray([load.scale(t) for t in times])
     n_gp = 8 * bundle.mesh_info.elements.shape[0]
     ppid = (None if bundle.element_phase_ids is None
             else np.repeat(np.asarray(bundle.element_phase_ids), 8))
@@ -467,7 +536,6 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     a = torch.zeros(n_dof, dtype=torch.float64, device=dev)
     mass = torch.as_tensor(lumped_mass(bundle), device=dev)
     fixed = torch.as_tensor(fixed_np, device=dev)
-    funit = torch.as_tensor(load_function.unit_force, device=dev)
     strain = torch.empty((6, n_gp), dtype=torch.float64, device=dev)
     rate = torch.empty((6, n_gp), dtype=torch.float64, device=dev)
     fint = torch.zeros(n_dof, dtype=torch.float64, device=dev)
@@ -487,10 +555,11 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
             box.handle, st, 0 if initialize else 1, dt, ptr(u), ptr(v), ptr(a), ptr(fixed),
             ptr(strain), ptr(rate)), "cia_box_generic_strains")
         r = kernel.update(strain, rate, state, dt, update_settings, False)
+        fvec, fscale = load.at(times[k])
         _lib.check(lib.cia_box_generic_correct(
-            box.handle, st, 1 if initialize else 0, dt, float(scales[k]), ptr(r.stress),
+            box.handle, st, 1 if initialize else 0, dt, fscale, ptr(r.stress),
             ptr(r.free_energy), ptr(r.dissipation), ptr(r.work), ptr(r.active), ptr(r.margin),
-            ptr(u), ptr(v), ptr(a), ptr(mass), ptr(fixed), ptr(funit), ptr(fint),
+            ptr(u), ptr(v), ptr(a), ptr(mass), ptr(fixed), ptr(fvec), ptr(fint),
             P(hist[k].data_ptr())), "cia_box_generic_correct")
         return r
 
@@ -510,8 +579,14 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     state = r.state
     record(0, r)
     has_diag = r.work is not None and r.active is not None and r.margin is not None
+    ext_inc = []  # external work per step of a general (non-separable) load
+    u_prev = torch.empty_like(u)
     for k in range(1, n_steps + 1):
+        if not load.separable:
+            u_prev.copy_(u)
         r = step(k, False)
+        if not load.separable:
+            ext_inc.append(load.work(times[k - 1], times[k], u - u_prev))
         state = r.state
         record(k, r)
         if progress_callback is not None and (k % stride == 0 or k == n_steps):
@@ -523,7 +598,11 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     H = hist.cpu().numpy()
     col = {name: H[:, i] for i, name in enumerate(_lib.HISTORY_COLUMNS)}
     volume = float(np.prod(bundle.lengths))
-    ext_work = np.concatenate([[0.0], np.cumsum(0.5 * (scales[:-1] + scales[1:]) * col["ext_du"][1:])])
+    if load.separable:
+        ext_work = np.concatenate([[0.0], np.cumsum(0.5 * (scales[:-1] + scales[1:])
+                                                    * col["ext_du"][1:])])
+    else:
+        ext_work = np.concatenate([[0.0], np.cumsum(ext_inc)])
     dissipated = np.concatenate([[0.0], np.cumsum(dt * col["dissipation_rate"][1:])])
     kinetic, free = col["kinetic"], col["free_energy"]
     residual = kinetic + free + dissipated - ext_work - (kinetic[0] + free[0])
@@ -531,7 +610,7 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     q_hist = None
     if record_fields and all(q is not None for q in rec["q"]):
         q_hist = np.asarray(rec["q"])
-    f_ext_hist = (np.asarray([scales[s] * load_function.unit_force for s in rec["steps"]])
+    f_ext_hist = (np.asarray([load.host(times[s]) for s in rec["steps"]])
                   if record_fields else None)
     return MaterialDynamicSolution(
         times=times,
@@ -610,8 +689,6 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
         raise ValueError("Material dynamics on the GPU requires three_dimensional kinematics.")
     if np.any(np.asarray(dirichlet.values) != 0.0):
         raise ValueError("Material dynamics currently requires homogeneous Dirichlet values.")
-    if not hasattr(load_function, "unit_force") or not hasattr(load_function, "scale"):
-        raise TypeError("GPU Newmark needs a load of the form scale(t) * unit_force.")
     if newton_settings.max_iterations <= 0:
         raise ValueError("Implicit Newmark requires max_iterations > 0.")
     lin = linear_settings or LinearSolverSettings(relative_tolerance=1e-12, max_iterations=20000)
@@ -636,7 +713,7 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
     fixed_np[np.asarray(dirichlet.dofs, dtype=np.int64)] = 1
     fixed = torch.as_tensor(fixed_np, device=dev)
     free_mask = torch.as_tensor(fixed_np == 0, device=dev)
-    funit = torch.as_tensor(load_function.unit_force, device=dev)
+    load = _DeviceLoad(load_function, n_dof, dev)
     P = C.c_void_p
 
     def ptr(t):
@@ -657,7 +734,7 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
         _lib.check(status, "cia_box_newmark_solve")
         return True, int(its.value)
 
-    def evaluate(disp, u_prev, u_pred, v_pred, dt, s_ext, committed, need_tangent):
+    def evaluate(disp, u_prev, u_pred, v_pred, dt, ext, committed, need_tangent):
         _lib.check(lib.cia_box_strains_between(box.handle, st, float(dt), ptr(disp), ptr(u_prev),
                                                ptr(strain), ptr(rate)), "cia_box_strains_between")
         r = kernel.update(strain, rate, committed, dt, update_settings, need_tangent)
@@ -667,7 +744,7 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
         res = torch.empty_like(disp)
         norms = (C.c_double * 2)()
         _lib.check(lib.cia_box_newmark_residual(
-            box.handle, st, rho, beta, gamma, float(dt), ptr(r.stress), float(s_ext), ptr(funit),
+            box.handle, st, rho, beta, gamma, float(dt), ptr(r.stress), float(ext[1]), ptr(ext[0]),
             ptr(fixed), ptr(disp), ptr(u_pred), ptr(v_pred), ptr(acc), ptr(vel), ptr(fint),
             ptr(res), norms), "cia_box_newmark_residual")
         return acc, vel, r, fint, res, float(norms[0]), float(norms[1])
@@ -688,11 +765,12 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
     fint0 = torch.empty_like(u)
     res0 = torch.empty_like(u)
     norms = (C.c_double * 2)()
+    f0, s0 = load.at(0.0)
     _lib.check(lib.cia_box_newmark_residual(
-        box.handle, st, rho, beta, gamma, dt0, ptr(r0.stress), 0.0, ptr(funit), ptr(fixed),
+        box.handle, st, rho, beta, gamma, dt0, ptr(r0.stress), 0.0, ptr(f0), ptr(fixed),
         ptr(zeros), ptr(zeros), ptr(zeros), ptr(acc0), ptr(vel0), ptr(fint0), ptr(res0), norms),
         "cia_box_newmark_residual")
-    rhs = torch.where(free_mask, float(load_function.scale(0.0)) * funit - fint0, zeros)
+    rhs = torch.where(free_mask, s0 * f0 - fint0, zeros)
     a = torch.zeros_like(u)
     ok, _ = pcg(None, 1.0, rhs, a)
     if not ok:
@@ -727,7 +805,7 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
         cur = states[-1]
         dt = min(cur_dt, final_time - cur["t"])
         try:
-            interval = _newmark_step(cur, dt, beta, gamma, load_function, newton_settings,
+            interval = _newmark_step(cur, dt, beta, gamma, load, newton_settings,
                                      evaluate, pcg, free_mask, jw)
         except RecoverableNonlinearStepError as err:
             if not time_settings.adaptive.enabled:
@@ -767,14 +845,14 @@ def solve_material_implicit_newmark(bundle, time_settings: TimeIntegrationSettin
                               kernel, perf_counter() - start)
 
 
-def _newmark_step(cur, dt, beta, gamma, load_function, newton, evaluate, pcg, free_mask, jw):
+def _newmark_step(cur, dt, beta, gamma, load, newton, evaluate, pcg, free_mask, jw):
     """``_solve_implicit_step`` (``material.py:446-662``) on device vectors."""
     u0, v0, a0 = cur["u"], cur["v"], cur["a"]
     t1 = cur["t"] + dt
     u_pred = u0 + dt * v0 + dt * dt * (0.5 - beta) * a0
     v_pred = v0 + dt * (1.0 - gamma) * a0
     disp = torch.where(free_mask, u_pred, torch.zeros_like(u_pred))
-    s1 = float(load_function.scale(t1))
+    s1 = load.at(t1)
     r0n = None
     last_inc_n = np.inf
     last_rn = np.inf
@@ -824,10 +902,7 @@ def _newmark_step(cur, dt, beta, gamma, load_function, newton, evaluate, pcg, fr
             disp = disp + inc
             last_inc_n = inc_n
     diss_inc = dt * float(r.dissipation.sum()) * jw if r.dissipation is not None else 0.0
-    s0 = float(load_function.scale(cur["t"]))
-    funit_du = float(torch.dot(torch.as_tensor(load_function.unit_force, device=disp.device),
-                               disp - u0))
-    ext_inc = 0.5 * (s0 + s1) * funit_du
+    ext_inc = load.work(cur["t"], t1, disp - u0)
     nxt = {"t": t1, "u": disp, "v": vel, "a": acc, "state": r.state, "resp": r, "fint": fint}
     return {"state": nxt, "dissipation_increment": diss_inc, "external_work_increment": ext_inc,
             "newton_iterations": it, "armijo_backtracks": backtracks, "linear_iterations": lin_its}

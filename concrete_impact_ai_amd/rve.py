@@ -301,6 +301,18 @@ def _box_lengths(model: StructuredHex8RVE):
 
 
 
+def linear_method(ls) -> str:
+    """The PCG preconditioner of a ``LinearSolverSettings``.  The reference's direct
+    sparse solve (``method="direct"``, SciPy SuperLU, ``src/fem/solvers/linear.py:15-22``;
+    set e.g. by ``src/concrete_impact/benchmarks/dynamic_viscoplastic.py:325``) maps
+    onto the spectral PCG driven to the settings' relative tolerance (1e-12 by
+    default, far below the Newton tolerances), whatever the ``backend``."""
+    method = "spectral" if ls.method == "direct" else ls.method
+    if method not in ("spectral", "block_jacobi"):
+        raise ValueError(f"unknown linear solver method {ls.method!r}")
+    return method
+
+
 def rve_descriptor(model: StructuredHex8RVE, phases, divisions, lengths, element_phase_ids):
     """C descriptor of one structured periodic RVE (``cia_rve_desc``); returns it
     with the ctypes arrays it points to, which must outlive the create call."""
@@ -322,12 +334,11 @@ def rve_descriptor(model: StructuredHex8RVE, phases, divisions, lengths, element
         float(ns.armijo.sufficient_decrease), float(ns.armijo.reduction_factor),
         int(ns.armijo.max_backtracks), 0,
     )
-    if ls.method not in ("spectral", "block_jacobi"):
-        raise ValueError(f"unknown PCG preconditioner {ls.method!r}")
+    method = linear_method(ls)
     desc.linear = _lib.LinearSettings(
         int(ls.max_iterations), int(ls.check_interval), float(ls.relative_tolerance),
         float(ls.absolute_tolerance),
-        _lib.CIA_PCG_SPECTRAL if ls.method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI,
+        _lib.CIA_PCG_SPECTRAL if method == "spectral" else _lib.CIA_PCG_BLOCK_JACOBI,
         int(bool(ls.adaptive_forcing)), float(ls.reference_bulk), float(ls.reference_shear),
         float(ls.forcing_max))
     desc.hill_mandel_power_scale = float(model.hill_mandel_power_scale)

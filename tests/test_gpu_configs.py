@@ -1,0 +1,118 @@
+"""GPU runs of the reference's own YAML benchmark configs through this package,
+judged by each config's verification block with the reference's metric
+definitions (``src/concrete_impact/benchmarks/rve.py``, ``plasticity.py``,
+``dynamic_viscoplastic.py``), and the general ``load_function(t) -> f_ext`` of
+the dynamics solvers (``src/fem/dynamics/material.py:139,165,337``).
+
+The configs are the reference's files, copied to tests/golden/configs by
+tests/golden/copy_configs.py.
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden" / "configs"
+
+
+def test_single_phase_rve_config_passes(gpu, tmp_path):
+    from concrete_impact_ai_amd import benchmarks as B
+
+    res = B.run_config(GOLD / "single_phase_j2_viscoplastic.yaml", tmp_path)
+    assert res.passed, res.metrics
+    m = res.metrics
+    assert m["grid_4_stress_error"] <= 1e-9 and m["grid_4_tangent_error"] <= 1e-8
+    assert m["effective_tangent_direction_error"] <= 1e-7
+    assert (tmp_path / "metrics.json").exists()
+
+
+def test_material_point_config_passes(gpu, tmp_path):
+    from concrete_impact_ai_amd import benchmarks as B
+
+    res = B.run_config(GOLD / "plastic_j2_viscoplastic_material_point.yaml", tmp_path)
+    assert res.passed, res.metrics
+    assert res.metrics["shear_stress_numerical_relative_l2_error"] <= 1e-10
+    assert (tmp_path / "curves" / "response.csv").exists()
+
+
+def test_dynamic_pressure_pulse_config_passes(gpu, tmp_path):
+    from concrete_impact_ai_amd import benchmarks as B
+
+    res = B.run_config(GOLD / "dynamic_j2_viscoplastic_pressure_pulse_3d.yaml", tmp_path)
+    assert res.passed, res.metrics
+    assert res.metrics["implicit_total_newton_iterations"] > 0
+
+
+def _elastic_slab():
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+
+    mat = M.LinearElasticMaterial("m", 1.0, 3339.9160679991464, 0.36994096308414537)
+    bundle = D.build_structured_box_bundle((0.6, 0.3, 0.3), (6, 3, 3), mat)
+    bottom = bundle.mesh_info.boundary_groups["bottom"].nodes
+    dofs = (3 * bottom[:, None] + np.arange(3)).ravel()
+    return bundle, dofs
+
+
+def test_general_load_function_equals_separable_load(gpu):
+    """A plain callable returning scale(t) * unit gives the separable load's run
+    (fused explicit loop, generic explicit loop and Newmark)."""
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import solvers as S
+
+    bundle, dofs = _elastic_slab()
+    sep = D.build_half_sine_pressure_load(bundle, 5.0, 0.02, patch=((1, 5), (0, 3)))
+    gen = lambda t: sep.scale(t) * sep.unit_force  # noqa: E731
+    n = bundle.mesh_info.dof_map.size
+    z = np.zeros(n)
+    dir_ = S.DirichletDofSet(dofs, np.zeros(dofs.size))
+    upd = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    te = S.TimeIntegrationSettings("explicit_central_difference", 2e-4, 60)
+    for generic in (False, True):
+        a = D.solve_material_explicit(bundle, te, z, z, dir_, sep, "three_dimensional", upd, generic=generic)
+        b = D.solve_material_explicit(bundle, te, z, z, dir_, gen, "three_dimensional", upd, generic=generic)
+        scale = np.max(np.abs(a.displacement))
+        assert scale > 0
+        assert np.max(np.abs(a.displacement - b.displacement)) <= 1e-12 * scale
+        assert np.max(np.abs(a.external_work - b.external_work)) <= 1e-12 * np.max(np.abs(a.external_work))
+        np.testing.assert_allclose(b.external_force, a.external_force, rtol=1e-15, atol=0)
+    newton = S.NonlinearNewtonSettings(20, 1e-9, 1e-10, 1e-14, 1e-10, S.ArmijoSettings(True))
+    ti = S.TimeIntegrationSettings("implicit_newmark", 1e-3, 20)
+    lin = S.LinearSolverSettings(relative_tolerance=1e-13, max_iterations=20000)
+    a = D.solve_material_implicit_newmark(bundle, ti, z, z, dir_, sep, "three_dimensional", upd, newton,
+                                          linear_settings=lin)
+    b = D.solve_material_implicit_newmark(bundle, ti, z, z, dir_, gen, "three_dimensional", upd, newton,
+                                          linear_settings=lin)
+    assert np.max(np.abs(a.displacement - b.displacement)) <= 1e-11 * np.max(np.abs(a.displacement))
+    assert np.max(np.abs(a.external_work - b.external_work)) <= 1e-11 * np.max(np.abs(a.external_work))
+
+
+def test_non_separable_load_superposes_on_an_elastic_body(gpu):
+    """Linear elastic explicit dynamics from rest is linear in the load: a load
+    f1(t) + f2(t) with different patches and time profiles (not of the form
+    scale(t) * unit) gives the sum of the two separable runs."""
+    from concrete_impact_ai_amd import dynamics as D
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import solvers as S
+
+    bundle, dofs = _elastic_slab()
+    f1 = D.build_half_sine_pressure_load(bundle, 5.0, 0.02, patch=((0, 3), (0, 2)))
+    f2u = D.build_half_sine_pressure_load(bundle, 1.0, 1.0, patch=((3, 6), (1, 3))).unit_force
+    f2 = D.SeparableLoad(f2u, lambda t: 2.0 * np.sin(300.0 * t) ** 2)
+    both = lambda t: f1(t) + f2(t)  # noqa: E731
+    n = bundle.mesh_info.dof_map.size
+    z = np.zeros(n)
+    dir_ = S.DirichletDofSet(dofs, np.zeros(dofs.size))
+    upd = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    te = S.TimeIntegrationSettings("explicit_central_difference", 2e-4, 80)
+    r1, r2, r12 = (D.solve_material_explicit(bundle, te, z, z, dir_, f, "three_dimensional", upd)
+                   for f in (f1, f2, both))
+    scale = np.max(np.abs(r12.displacement))
+    assert np.max(np.abs(r12.displacement - r1.displacement - r2.displacement)) <= 1e-10 * scale
+    # the work of the sum is not the sum of the works, but the energy balance closes
+    w = np.max(np.abs(r12.external_work))
+    assert w > 0 and np.max(np.abs(r12.energy_residual)) <= 0.05 * w
