@@ -351,7 +351,7 @@ _SIGNATURES.update({
     "cia_mlp_forward_fused": (C.c_int32, [_vp, C.c_int64, _vp, _vp, _vp, _vp]),
     "cia_surrogate_update": (
         C.c_int32, [_vp, C.c_int64, _vp, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                    _vp, _vp, _vp, _vp, _vp, _vp]),
+                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 })
 
 BATCH_KERNELS = ["assemble", "gather", "pcg_elem", "pcg_node", "pcg_update", "symbol_solve",

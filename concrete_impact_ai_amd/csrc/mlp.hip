@@ -397,7 +397,9 @@ __global__ void __launch_bounds__(256, 2)
                  const uint8_t* __restrict__ fb_in, double* __restrict__ stress,
                  double* __restrict__ n_strain, double* __restrict__ n_inel,
                  float* __restrict__ n_lat, float* __restrict__ n_age, uint8_t* __restrict__ fb_out,
-                 unsigned int* __restrict__ trips) {
+                 unsigned int* __restrict__ trips, const double* __restrict__ cur_in,
+                 double* __restrict__ cur_out, double* __restrict__ psi,
+                 double* __restrict__ diss) {
   extern __shared__ __attribute__((aligned(16))) __bf16 act[];  // [FM][FLD], then o_s
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t m0 = (int64_t)blockIdx.x * FM;
@@ -456,6 +458,24 @@ __global__ void __launch_bounds__(256, 2)
       const bool commit = p.dinf >= g.commit_strain || (g.commit_time > 0.0 && p.tau >= g.commit_time);
 #pragma unroll
       for (int c = 0; c < 6; ++c) stress[c * n + m] = sig[c];
+      // energies: psi = 1/2 (strain - ein') . sigma; D = sigma . (ein' - ein_prev) / dt with
+      // ein_prev the previous call's inelastic strain (cur_in)
+      if (psi) {
+        double e = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) e += eel[c] * sig[c];
+        psi[m] = 0.5 * e;
+      }
+      if (diss) {
+        double e = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) e += sig[c] * (ein[c] - cur_in[c * n + m]);
+        diss[m] = e / dt;
+      }
+      if (cur_out) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cur_out[c * n + m] = ein[c];
+      }
       if (commit) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -683,7 +703,9 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
                                 const float* anchor_age, const uint8_t* fallback, double* stress,
                                 double* next_anchor_strain, double* next_anchor_inelastic,
                                 float* next_latent, float* next_age, uint8_t* next_fallback,
-                                uint32_t* trip_counter) {
+                                uint32_t* trip_counter, const double* inelastic,
+                                double* next_inelastic, double* free_energy,
+                                double* dissipation) {
   using namespace cia;
   CIA_ARG(strain && weights && biases && gate && anchor_strain && anchor_inelastic &&
               anchor_latent && anchor_age && fallback && stress && next_anchor_strain &&
@@ -694,6 +716,8 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
               gate->rate_scale > 0.0 && gate->commit_strain >= 0.0,
           "cia_surrogate_update: time step and scales must be positive");
   CIA_ARG(n < ((int64_t)1 << 31), "cia_surrogate_update: at most 2^31 - 1 points");
+  CIA_ARG(!dissipation || inelastic,
+          "cia_surrogate_update: the dissipation needs the previous call's inelastic strain");
   if (n == 0) return CIA_OK;
   const size_t lds = FUSED_ACT_BYTES + (size_t)FM * (FOUT + 1) * sizeof(float);
   static bool attr_set = false;
@@ -720,7 +744,8 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
                      (hipStream_t)stream, n, strain, time_step, (const __bf16*)weights, biases, g,
                      anchor_strain, anchor_inelastic, anchor_latent, anchor_age, fallback, stress,
                      next_anchor_strain, next_anchor_inelastic, next_latent, next_age,
-                     next_fallback, (unsigned int*)trip_counter);
+                     next_fallback, (unsigned int*)trip_counter, inelastic, next_inelastic,
+                     free_energy, dissipation);
   CIA_HIP(hipGetLastError());
   return CIA_OK;
 }

@@ -579,6 +579,10 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
     state = r.state
     record(0, r)
     has_diag = r.work is not None and r.active is not None and r.margin is not None
+    # a material that supplies no energies leaves those histories unavailable (NaN),
+    # not zero: the force pass reads absent energies as zeros
+    has_free = r.free_energy is not None
+    has_diss = r.dissipation is not None
     ext_inc = []  # external work per step of a general (non-separable) load
     u_prev = torch.empty_like(u)
     for k in range(1, n_steps + 1):
@@ -605,6 +609,10 @@ def _solve_explicit_generic(bundle, time_settings, initial_displacement, initial
         ext_work = np.concatenate([[0.0], np.cumsum(ext_inc)])
     dissipated = np.concatenate([[0.0], np.cumsum(dt * col["dissipation_rate"][1:])])
     kinetic, free = col["kinetic"], col["free_energy"]
+    if not has_free:
+        free = np.full(n_steps + 1, np.nan)
+    if not has_diss:
+        dissipated = np.full(n_steps + 1, np.nan)
     residual = kinetic + free + dissipated - ext_work - (kinetic[0] + free[0])
     zeros = np.zeros(n_steps + 1)
     q_hist = None

@@ -31,10 +31,10 @@ kernel, from the anchor with the whole increment since it, and commits only when
 that increment reaches the size of a training step, so the network sees training-
 sized increments whatever the macro time step.  The gate: a point whose inputs leave
 the envelope seen in training (strain increment, rate above the trained range) or
-whose latent state leaves the training range is handed to the FE² adapter
-(``RVEHomogenizedMaterial``) for the rest of the run.  Its RVE starts from the
-virgin state and is driven to the point's current strain in ``fallback_substeps``
-proportional increments (the history before the trip is not replayed).
+whose latent state leaves the training range, and a monitor point whose stress
+differs from its own FE² solution by more than the error bound (the error gate
+judged by FE²), is handed to the FE² adapter (``RVEHomogenizedMaterial``) for the
+rest of the run; see ``SurrogateMaterial`` for how its RVE state is formed.
 """
 
 from __future__ import annotations
@@ -380,11 +380,36 @@ class SurrogateMaterial:
     reaches ``commit_time`` > 0).  The network so always sees increments of the size
     it was trained on, whatever the macro time step; between commits the stress is
     C (strain - inelastic strain) with the anchor's inelastic strain advanced by one
-    network step.  Rates below the training range are
-    clamped to its lower end (the quasi-static limit of the viscoplastic response);
-    rates above it by more than ``rate_margin`` (in units of the rate feature), an
-    increment beyond ``gate_margin`` times the training envelope, or a latent beyond
-    ``gate_margin`` times the range seen in training trip the gate.
+    network step.  Rates below the training range are clamped to its lower end (the
+    quasi-static limit of the viscoplastic response).
+
+    Energies (the reference adapter's ``free_energy`` / ``dissipation``,
+    ``nn/models/rve_rno.py:285-308``, in this network's variables): stored energy
+    1/2 (strain - ein) . stress and inelastic power stress . d ein / dt, from the
+    kernel.  Tangent: d stress / d strain = C (I - d ein / d strain) through the
+    network's input Jacobian (layer-wise BF16 GEMMs, autograd), or the fallback
+    RVE's effective tangent.
+
+    Two gates send a point to the FE² adapter (``fallback``, a
+    ``RVEHomogenizedMaterial``) for the rest of the run:
+
+    * the input gate, in the kernel: an increment beyond ``gate_margin`` times the
+      training envelope, a rate above it by more than ``rate_margin`` (in units of the
+      rate feature), a latent beyond ``gate_margin`` times the range seen in
+      training, or a non-finite result;
+    * the error gate judged by FE²: ``monitor_points`` carry their own RVE states,
+      advanced every ``check_interval`` calls to the current strain over the time
+      since the last check (one FE² increment per check); a monitor whose surrogate
+      stress differs from its FE² stress by more than ``error_bound`` times the
+      stress scale (max norm) trips, and continues from that RVE state, i.e. from
+      the FE² solution along its whole (checked) history.
+
+    A point that trips the input gate without a monitor RVE replays its recorded
+    path in a virgin RVE: zero strain at time 0, its anchor strain at the anchor's
+    time (elapsed time minus the anchor's age), its current strain now, each segment
+    in ``fallback_substeps`` increments over the segment's real duration, so the
+    replayed rates are the rates the point saw (a point still on its first anchor is
+    replayed along the straight path over the elapsed time).
 
     ``probe_points``: point ids whose strain, stress and time step are recorded at
     every call (``probe_history()``), for checking the run against the FE² oracle."""
@@ -392,7 +417,8 @@ class SurrogateMaterial:
     def __init__(self, name, params: SurrogateParams, scales: SurrogateScales, density: float,
                  maximum_wave_speed: float, fallback=None, fallback_substeps: int = 4,
                  gate_margin: float = 1.25, rate_margin: float = 0.5, commit_fraction: float = 0.5,
-                 commit_time: float = 0.0, probe_points=None):
+                 commit_time: float = 0.0, probe_points=None, monitor_points=None,
+                 check_interval: int = 1, error_bound: float = 0.05):
         self.name = name
         self.params = params
         self.scales = scales
@@ -405,8 +431,16 @@ class SurrogateMaterial:
         self.commit_fraction = float(commit_fraction)
         self.commit_time = float(commit_time)
         self.probe_points = None if probe_points is None else np.asarray(probe_points, dtype=np.int64)
+        self.monitor_points = (None if monitor_points is None
+                               else np.unique(np.asarray(monitor_points, dtype=np.int64)))
+        if self.monitor_points is not None and fallback is None:
+            raise ValueError("monitor points need an FE² fallback material to judge them")
+        self.check_interval = max(1, int(check_interval))
+        self.error_bound = float(error_bound)
         self._probe = {"strain": [], "stress": [], "dt": []}
-        self.stats = {"surrogate_points": 0, "fallback_points": 0, "gate_trips": 0, "calls": 0}
+        self.stats = {"surrogate_points": 0, "fallback_points": 0, "gate_trips": 0, "calls": 0,
+                      "error_checks": 0, "error_trips": 0, "judged_error_max": 0.0,
+                      "judged_error_last": []}
         self.call_times = []  # host clock after each call's device work (the trip count read)
 
     def gate(self) -> _lib.SurrogateGate:
@@ -431,34 +465,77 @@ class SurrogateMaterial:
     def initialize_state(self, n_points: int) -> MaterialState:
         return MaterialState({"surrogate_strain": np.zeros((n_points, 6)),
                               "surrogate_inelastic_strain": np.zeros((n_points, 6)),
+                              "surrogate_current_inelastic_strain": np.zeros((n_points, 6)),
                               "surrogate_latent": np.zeros((n_points, 6)),
                               "surrogate_age": np.zeros(n_points),
+                              "surrogate_time": np.zeros(n_points),
                               "surrogate_fallback": np.zeros(n_points)})
 
     def device_kernel(self, point_phase_ids=None):
         return _SurrogateKernel(self)
 
     def update(self, request, state, requirements=DEFAULT_RESPONSE_REQUIREMENTS):
-        if requirements.tangent:
-            raise ValueError("The surrogate material provides no consistent tangent.")
         dev = device()
         k = _SurrogateKernel(self)
         strain = torch.as_tensor(np.ascontiguousarray(np.asarray(request.strains, dtype=np.float64).T),
                                  device=dev)
         st = k.from_host(state)
-        r = k.update(strain, None, st, request.time_step, request.update_settings, False)
-        return MaterialPointResponse(stresses=r.stress.cpu().numpy().T.copy(),
-                                     state=k.material_state(r.state),
-                                     free_energy=None, dissipation=None, diagnostics={})
+        r = k.update(strain, None, st, request.time_step, request.update_settings,
+                     bool(requirements.tangent))
+        tangents = None
+        if requirements.tangent:
+            tangents = r.tangent.t().reshape(-1, 6, 6).cpu().numpy().copy()
+        return MaterialPointResponse(
+            stresses=r.stress.cpu().numpy().T.copy(), state=k.material_state(r.state),
+            tangents=tangents,
+            free_energy=r.free_energy.cpu().numpy().copy() if requirements.free_energy else None,
+            dissipation=r.dissipation.cpu().numpy().copy() if requirements.dissipation else None,
+            diagnostics={})
+
+    # -- tangent -----------------------------------------------------------------
+
+    def network_tangent(self, strain: torch.Tensor, st: "_SurrogateState", dt: float) -> torch.Tensor:
+        """d stress / d strain [36][n] (row-major 6 x 6 per point) of the network
+        branch: stress = C (strain - ein'), ein' = anchor ein + c eps_scale o[0:6](x),
+        x = [d / eps_scale, feature(|d|_2 / tau), latent], d = strain - anchor strain;
+        the rate feature's derivative is d / (|d|_2^2 ln 10 rate_scale) where it is
+        not clamped.  The six Jacobian rows come from six backward passes through the
+        layer-wise kernels."""
+        sc = self.scales
+        n = strain.shape[1]
+        d = (strain - st.strain).t()                         # [n][6]
+        tau = st.age.double() + dt
+        nrm = torch.linalg.norm(d, dim=1)
+        rate = nrm / tau
+        feat_raw = torch.log10(torch.clamp(rate, min=1e-30) / sc.rate_ref) / sc.rate_scale
+        feat = torch.clamp(feat_raw, min=sc.rate_lo)
+        x = torch.cat([d / sc.eps_scale, feat[:, None], st.latent.t().double()], dim=1).float()
+        x = x.detach().requires_grad_(True)
+        o = mlp(x, self.params)
+        jac = torch.empty((n, 6, 13), dtype=torch.float64, device=strain.device)
+        for i in range(6):
+            (g,) = torch.autograd.grad(o[:, i].sum(), x, retain_graph=i < 5)
+            jac[:, i, :] = g.double()
+        dfeat = torch.where((feat_raw > sc.rate_lo) & (nrm > 0.0),
+                            1.0 / (torch.clamp(nrm, min=1e-300) ** 2 * math.log(10.0) * sc.rate_scale),
+                            torch.zeros_like(nrm))[:, None] * d           # [n][6]
+        dein = INELASTIC_STEP * (jac[:, :, :6] + sc.eps_scale * jac[:, :, 6:7] * dfeat[:, None, :])
+        Cm = torch.as_tensor(sc.stiffness_matrix(), device=strain.device)
+        eye = torch.eye(6, dtype=torch.float64, device=strain.device)
+        tan = torch.einsum("ij,njk->nik", Cm, eye[None] - dein)          # [n][6][6]
+        return tan.reshape(n, 36).t().contiguous()
 
 
 class _SurrogateState:
     """Anchors in the kernel's layout: strain / inelastic strain [6][n] fp64, latent
-    [6][n] fp32, age [n] fp32, fallback [n] uint8; n_fallback is the host's count of
-    fallback points; rve_state / rve_index hold the FE² states of the fallback points."""
+    [6][n] fp32, age [n] fp32, fallback [n] uint8; ``current`` [6][n] fp64 the last
+    call's inelastic strain (for the dissipation); ``time`` the elapsed physical
+    time (host); n_fallback the host's count of fallback points; rve_state /
+    rve_index the FE² states of the fallback points; mon_state / mon_time /
+    mon_calls the monitor RVEs, the time of their last check and the calls since."""
 
     def __init__(self, strain, inelastic, latent, age, fallback, n_fallback, rve_state=None,
-                 rve_index=None):
+                 rve_index=None, current=None, time=0.0, mon_state=None, mon_time=0.0, mon_calls=0):
         self.strain = strain
         self.inelastic = inelastic
         self.latent = latent
@@ -467,6 +544,32 @@ class _SurrogateState:
         self.n_fallback = int(n_fallback)
         self.rve_state = rve_state
         self.rve_index = rve_index
+        self.current = current if current is not None else torch.zeros_like(inelastic)
+        self.time = float(time)
+        self.mon_state = mon_state
+        self.mon_time = float(mon_time)
+        self.mon_calls = int(mon_calls)
+
+
+def _select(state, rows):
+    """Rows of an FE² batch state (``fe2.RVEBatchState``)."""
+    from concrete_impact_ai_amd.fe2 import RVEBatchState
+
+    idx = torch.as_tensor(np.asarray(rows, dtype=np.int64), device=state.packed.device)
+    return RVEBatchState(state.packed.index_select(0, idx).contiguous(),
+                         state.w.index_select(0, idx).contiguous(),
+                         np.asarray(state.macro)[np.asarray(rows, dtype=np.int64)], state._cache)
+
+
+def _concat(states):
+    from concrete_impact_ai_amd.fe2 import RVEBatchState
+
+    states = [s for s in states if s is not None and len(s) > 0]
+    if len(states) == 1:
+        return states[0]
+    return RVEBatchState(torch.cat([s.packed for s in states]).contiguous(),
+                         torch.cat([s.w for s in states]).contiguous(),
+                         np.concatenate([s.macro for s in states]), states[0]._cache)
 
 
 class _SurrogateKernel:
@@ -477,11 +580,16 @@ class _SurrogateKernel:
 
     def initial_state(self, n):
         dev = device()
-        return _SurrogateState(torch.zeros((6, n), dtype=torch.float64, device=dev),
-                               torch.zeros((6, n), dtype=torch.float64, device=dev),
-                               torch.zeros((6, n), dtype=torch.float32, device=dev),
-                               torch.zeros(n, dtype=torch.float32, device=dev),
-                               torch.zeros(n, dtype=torch.uint8, device=dev), 0)
+        st = _SurrogateState(torch.zeros((6, n), dtype=torch.float64, device=dev),
+                             torch.zeros((6, n), dtype=torch.float64, device=dev),
+                             torch.zeros((6, n), dtype=torch.float32, device=dev),
+                             torch.zeros(n, dtype=torch.float32, device=dev),
+                             torch.zeros(n, dtype=torch.uint8, device=dev), 0)
+        if self.m.monitor_points is not None:
+            if self.m.monitor_points.size and self.m.monitor_points[-1] >= n:
+                raise ValueError("monitor point ids beyond the number of points")
+            st.mon_state = self.m.fallback.initialize_state(self.m.monitor_points.size)
+        return st
 
     def from_host(self, state):
         dev = device()
@@ -494,29 +602,42 @@ class _SurrogateKernel:
                                    dtype=dtype, device=dev).contiguous()
 
         age = np.asarray(v["surrogate_age"]) if "surrogate_age" in v else np.zeros(n)
-        return _SurrogateState(rows("surrogate_strain", torch.float64),
-                               rows("surrogate_inelastic_strain", torch.float64),
-                               rows("surrogate_latent", torch.float32),
-                               torch.as_tensor(age, dtype=torch.float32, device=dev),
-                               torch.as_tensor(fb.astype(np.uint8), device=dev), int(fb.sum()),
-                               v.get("_rve_state"), v.get("_rve_index"))
+        cur = (rows("surrogate_current_inelastic_strain", torch.float64)
+               if "surrogate_current_inelastic_strain" in v else None)
+        t = float(np.asarray(v["surrogate_time"])[0]) if "surrogate_time" in v and n else 0.0
+        st = _SurrogateState(rows("surrogate_strain", torch.float64),
+                             rows("surrogate_inelastic_strain", torch.float64),
+                             rows("surrogate_latent", torch.float32),
+                             torch.as_tensor(age, dtype=torch.float32, device=dev),
+                             torch.as_tensor(fb.astype(np.uint8), device=dev), int(fb.sum()),
+                             v.get("_rve_state"), v.get("_rve_index"), cur, t,
+                             v.get("_monitor_state"), float(v.get("_monitor_time", 0.0)),
+                             int(v.get("_monitor_calls", 0)))
+        if st.mon_state is None and self.m.monitor_points is not None:
+            st.mon_state = self.m.fallback.initialize_state(self.m.monitor_points.size)
+        return st
 
     def material_state(self, st):
+        n = st.strain.shape[1]
         out = {"surrogate_strain": st.strain.t().cpu().numpy(),
                "surrogate_inelastic_strain": st.inelastic.t().cpu().numpy(),
+               "surrogate_current_inelastic_strain": st.current.t().cpu().numpy(),
                "surrogate_latent": st.latent.t().double().cpu().numpy(),
                "surrogate_age": st.age.double().cpu().numpy(),
+               "surrogate_time": np.full(n, st.time),
                "surrogate_fallback": st.fallback.double().cpu().numpy()}
         if st.rve_state is not None:
             out["_rve_state"] = st.rve_state
             out["_rve_index"] = st.rve_index
+        if st.mon_state is not None:
+            out["_monitor_state"] = st.mon_state
+            out["_monitor_time"] = st.mon_time
+            out["_monitor_calls"] = st.mon_calls
         return MaterialState(out)
 
     def update(self, strain, rate, st: _SurrogateState, dt, settings, tangent=False) -> DeviceResponse:
         """strain [6][n] fp64 on the device; ``rate`` is not used (the rate feature is
         the mean rate since the anchor)."""
-        if tangent:
-            raise ValueError("The surrogate material provides no consistent tangent.")
         m = self.m
         n = strain.shape[1]
         dev = strain.device
@@ -524,15 +645,20 @@ class _SurrogateKernel:
         if strain.dtype != torch.float64 or strain.shape[0] != 6:
             raise ValueError("surrogate strains must be [6][n] float64")
         stress = torch.empty((6, n), dtype=torch.float64, device=dev)
+        psi = torch.empty(n, dtype=torch.float64, device=dev)
+        diss = torch.empty(n, dtype=torch.float64, device=dev)
         nxt = _SurrogateState(torch.empty_like(st.strain), torch.empty_like(st.inelastic),
                               torch.empty_like(st.latent), torch.empty_like(st.age),
-                              torch.empty_like(st.fallback), st.n_fallback, st.rve_state, st.rve_index)
+                              torch.empty_like(st.fallback), st.n_fallback, st.rve_state, st.rve_index,
+                              torch.empty_like(st.current), st.time + float(dt), st.mon_state,
+                              st.mon_time, st.mon_calls + 1)
         gate = m.gate()
         _check(_lib.lib().cia_surrogate_update(
             stream_handle(), n, _p(strain), float(dt), _p(m.params.w_bf16), _p(m.params.b), C.byref(gate),
             _p(st.strain), _p(st.inelastic), _p(st.latent), _p(st.age), _p(st.fallback), _p(stress),
             _p(nxt.strain), _p(nxt.inelastic), _p(nxt.latent), _p(nxt.age), _p(nxt.fallback),
-            _p(self.trips)), "cia_surrogate_update")
+            _p(self.trips), _p(st.current), _p(nxt.current), _p(psi), _p(diss)), "cia_surrogate_update")
+        tan = m.network_tangent(strain, st, float(dt)) if tangent else None
         total = int(self.trips.item())
         m.call_times.append(time.perf_counter())
         new_trips = total - self._trips_seen
@@ -540,60 +666,147 @@ class _SurrogateKernel:
         nxt.n_fallback = st.n_fallback + new_trips
         m.stats["calls"] += 1
         m.stats["gate_trips"] += new_trips
-        m.stats["fallback_points"] += nxt.n_fallback
-        m.stats["surrogate_points"] += n - nxt.n_fallback
+        mon_now = None
+        if m.monitor_points is not None and len(m.monitor_points):
+            mon_now = self._judge(strain, stress, st, nxt, dt, settings, new_trips > 0)
         if nxt.n_fallback:
             if m.fallback is None:
                 raise RuntimeError(f"{nxt.n_fallback} surrogate points left the training envelope and "
                                    "no FE² fallback is configured.")
-            self._fallback(strain, nxt, stress, dt, settings)
+            self._fallback(strain, st, nxt, stress, psi, diss, tan, dt, settings, mon_now)
+        m.stats["fallback_points"] += nxt.n_fallback
+        m.stats["surrogate_points"] += n - nxt.n_fallback
         if m.probe_points is not None:
             ids = torch.as_tensor(m.probe_points, device=dev)
             m._probe["strain"].append(strain.index_select(1, ids))
             m._probe["stress"].append(stress.index_select(1, ids))
             m._probe["dt"].append(float(dt))
-        return DeviceResponse(stress=stress, state=nxt)
+        return DeviceResponse(stress=stress, state=nxt, tangent=tan, free_energy=psi, dissipation=diss)
 
-    def _fallback(self, strain, nxt, stress, dt, settings):
-        """FE² for the points on the fallback (their stresses replace the network's in
-        ``stress``): earlier trips continue from their committed RVE states; new trips
-        start a virgin RVE driven to the current strain in proportional sub-increments."""
+    # -- error gate judged by FE² ------------------------------------------------
+
+    def _judge(self, strain, stress, st, nxt, dt, settings, envelope_trips):
+        """Advance the monitor RVEs on check calls (and on calls where a monitor point
+        has newly tripped the input gate), compare, trip.  Returns the monitor rows
+        advanced to this call's strain (their FE² stresses) or None."""
+        m = self.m
+        mon = m.monitor_points
+        fb_prev = st.fallback.index_select(0, torch.as_tensor(mon, device=strain.device)).cpu().numpy() > 0
+        fb_now = nxt.fallback.index_select(0, torch.as_tensor(mon, device=strain.device)).cpu().numpy() > 0
+        live = ~fb_prev  # monitors still on the surrogate before this call
+        newly = live & fb_now
+        is_check = nxt.mon_calls >= m.check_interval
+        if not live.any() or (not is_check and not newly.any()):
+            return None
+        rows = np.flatnonzero(live) if is_check else np.flatnonzero(newly)
+        ids = mon[rows]
+        idx = torch.as_tensor(ids, device=strain.device)
+        eps = strain.index_select(1, idx).t().cpu().numpy()
+        span = nxt.time - st.mon_time  # one FE² increment since the monitors' last advance
+        res, adv = m.fallback.solve_batch(eps, np.full(rows.size, span), settings,
+                                          _select(st.mon_state, rows))
+        fe2 = np.asarray([r.macro_stress[:] for r in res]).reshape(-1, 6)
+        sur = stress.index_select(1, idx).t().cpu().numpy()
+        err = np.max(np.abs(sur - fe2), axis=1) / m.scales.stress_scale
+        # the advanced rows replace the monitors' states (points not advanced keep theirs)
+        packed = st.mon_state.packed.clone()
+        w = st.mon_state.w.clone()
+        macro = np.asarray(st.mon_state.macro).copy()
+        ridx = torch.as_tensor(rows, device=packed.device)
+        packed.index_copy_(0, ridx, adv.packed)
+        w.index_copy_(0, ridx, adv.w)
+        macro[rows] = eps
         from concrete_impact_ai_amd.fe2 import RVEBatchState
-        from concrete_impact_ai_amd.materials import MaterialPointRequest
 
-        ad = self.m.fallback
+        nxt.mon_state = RVEBatchState(packed, w, macro, st.mon_state._cache)
+        if is_check:
+            nxt.mon_time = nxt.time
+            nxt.mon_calls = 0
+            m.stats["error_checks"] += 1
+            m.stats["judged_error_last"] = err.tolist()
+            m.stats["judged_error_max"] = max(m.stats["judged_error_max"], float(np.max(err)))
+            judged = err > m.error_bound
+            trip_rows = rows[judged & ~newly[rows]]
+            if trip_rows.size:
+                nxt.fallback.index_fill_(0, torch.as_tensor(mon[trip_rows], device=strain.device), 1)
+                nxt.n_fallback += int(trip_rows.size)
+                m.stats["error_trips"] += int(trip_rows.size)
+        else:
+            nxt.mon_time = st.mon_time
+        return {"rows": rows, "ids": ids, "stress": fe2, "results": res, "state": adv, "span": span}
+
+    # -- FE² fallback ------------------------------------------------------------
+
+    def _fallback(self, strain, st, nxt, stress, psi, diss, tan, dt, settings, mon_now):
+        """FE² for the points on the fallback (their stresses, energies and tangents
+        replace the network's): earlier trips continue from their committed RVE
+        states; monitor points that trip continue from their monitor RVEs; other new
+        trips replay their anchor path in a virgin RVE over its real durations."""
+        m = self.m
+        ad = m.fallback
+        dev = strain.device
+        need_tangent = tan is not None
         ids = torch.nonzero(nxt.fallback).flatten().cpu().numpy()
         old = nxt.rve_index if nxt.rve_index is not None else np.zeros(0, dtype=np.int64)
         new = np.setdiff1d(ids, old)
-        eps_h = strain.t()[torch.as_tensor(ids, device=strain.device)].cpu().numpy()
-        pos = {int(p): i for i, p in enumerate(ids)}
+        mon_ids = m.monitor_points if m.monitor_points is not None else np.zeros(0, dtype=np.int64)
+        new_mon = np.intersect1d(new, mon_ids)
+        new_rep = np.setdiff1d(new, new_mon)
         states, order = [], []
 
-        def put(points, values):
-            stress[:, torch.as_tensor(points, device=stress.device)] = torch.as_tensor(
-                np.ascontiguousarray(values.T), device=stress.device)
+        def put(points, res_list):
+            pts = torch.as_tensor(points, device=dev)
+            vals = np.asarray([r.macro_stress[:] for r in res_list]).reshape(-1, 6)
+            stress[:, pts] = torch.as_tensor(np.ascontiguousarray(vals.T), device=dev)
+            psi[pts] = torch.as_tensor(np.asarray([r.free_energy_density for r in res_list]), device=dev)
+            diss[pts] = torch.as_tensor(np.asarray([r.dissipation_density for r in res_list]), device=dev)
+            if need_tangent:
+                t = np.asarray([r.effective_tangent[:] for r in res_list]).reshape(-1, 36)
+                tan[:, pts] = torch.as_tensor(np.ascontiguousarray(t.T), device=dev)
+
+        def eps_of(points):
+            return strain.index_select(1, torch.as_tensor(points, device=dev)).t().cpu().numpy()
 
         if old.size:
-            e = eps_h[[pos[int(p)] for p in old]]
-            r = ad.update(MaterialPointRequest(e, None, dt, "three_dimensional", settings), nxt.rve_state)
-            put(old, r.stresses)
-            states.append(r.state)
+            res, state = ad.solve_batch(eps_of(old), np.full(old.size, float(dt)), settings,
+                                        nxt.rve_state, need_tangent)
+            put(old, res)
+            states.append(state)
             order.append(old)
-        if new.size:
-            e = eps_h[[pos[int(p)] for p in new]]
-            ns = ad.initialize_state(new.size)
-            S = max(1, self.m.fallback_substeps)
-            for k in range(1, S + 1):
-                r = ad.update(MaterialPointRequest(e * (k / S), None, dt / S, "three_dimensional", settings), ns)
-                ns = r.state
-            put(new, r.stresses)
+        if new_mon.size:
+            # the monitor RVEs were advanced to this call by _judge (error or input trips)
+            pos = {int(p): k for k, p in enumerate(mon_now["ids"])}
+            rows = [pos[int(p)] for p in new_mon]
+            res = [mon_now["results"][k] for k in rows]
+            if need_tangent:  # tangents are not formed in the monitor solve: solve again
+                prev = _select(st.mon_state, [int(np.searchsorted(mon_ids, p)) for p in new_mon])
+                res, _ = ad.solve_batch(eps_of(new_mon), np.full(new_mon.size, mon_now["span"]),
+                                        settings, prev, True)
+            put(new_mon, res)
+            states.append(_select(mon_now["state"], rows))
+            order.append(new_mon)
+        if new_rep.size:
+            e_now = eps_of(new_rep)
+            pidx = torch.as_tensor(new_rep, device=dev)
+            e_a = st.strain.index_select(1, pidx).t().cpu().numpy()
+            age = st.age.index_select(0, pidx).double().cpu().numpy()
+            t_now = nxt.time
+            t_a = np.clip(st.time - age, 0.0, None)
+            first = t_a <= 0.0  # still on the first anchor: the straight path, split in two
+            t_a = np.where(first, 0.5 * t_now, t_a)
+            e_a = np.where(first[:, None], 0.5 * e_now, e_a)
+            S = max(1, m.fallback_substeps)
+            ns = ad.initialize_state(new_rep.size)
+            segments = ((np.zeros_like(e_a), e_a, np.zeros_like(t_a), t_a),
+                        (e_a, e_now, t_a, np.full_like(t_a, t_now)))
+            for seg, (e0, e1, t0, t1) in enumerate(segments):
+                h = (t1 - t0) / S
+                for k in range(1, S + 1):
+                    final = seg == 1 and k == S
+                    res, ns = ad.solve_batch(e0 + (e1 - e0) * (k / S), h, settings, ns,
+                                             need_tangent and final)
+            put(new_rep, res)
             states.append(ns)
-            order.append(new)
-        if len(states) == 1:
-            batch = states[0]
-        else:
-            batch = RVEBatchState(torch.cat([s_.packed for s_ in states]).contiguous(),
-                                  torch.cat([s_.w for s_ in states]).contiguous(),
-                                  np.concatenate([s_.macro for s_ in states]), ad.operator_cache)
-        nxt.rve_state = batch
+            order.append(new_rep)
+        nxt.rve_state = _concat(states)
         nxt.rve_index = np.concatenate(order).astype(np.int64)

@@ -592,7 +592,12 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
  * Layouts: strain, stress, anchor strain / inelastic strain [6][n] fp64, latents
  * [6][n] fp32, age [n] fp32, fallback [n] uint8; stiffness 6 x 6 row-major
  * (engineering Voigt, Pa); weights / biases as for cia_mlp_forward_fused.
- * Stresses of fallback points are the network's; the caller replaces them. */
+ * Stresses of fallback points are the network's; the caller replaces them.
+ * Optional energies (any of the last four pointers may be NULL): next_inelastic
+ * [6][n] receives this call's inelastic strain ein'; free_energy [n] the stored
+ * elastic energy 1/2 (strain - ein') . stress; dissipation [n] the inelastic power
+ * stress . (ein' - inelastic) / time_step, with inelastic [6][n] the previous
+ * call's ein' (zeros at the start); dissipation requires inelastic. */
 typedef struct cia_surrogate_gate {
   double eps_scale, rate_ref, rate_scale, inelastic_step;
   double d_eps_max, rate_lo, rate_hi, latent_max, gate_margin, rate_margin;
@@ -606,7 +611,8 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
                                 const float* anchor_age, const uint8_t* fallback, double* stress,
                                 double* next_anchor_strain, double* next_anchor_inelastic,
                                 float* next_latent, float* next_age, uint8_t* next_fallback,
-                                uint32_t* trip_counter);
+                                uint32_t* trip_counter, const double* inelastic,
+                                double* next_inelastic, double* free_energy, double* dissipation);
 
 /* Kernel timing with HIP events on the launch stream (no reference
  * counterpart; measurement plumbing).  Slots: 0 assemble, 1 PCG element

@@ -137,7 +137,19 @@ GATE_FIELDS = ("eps_scale", "rate_ref", "rate_scale", "inelastic_step", "d_eps_m
 
 def initial_anchor(n):
     return {"strain": np.zeros((n, 6)), "inelastic": np.zeros((n, 6)), "latent": np.zeros((n, 6)),
-            "age": np.zeros(n), "fallback": np.zeros(n, dtype=bool)}
+            "age": np.zeros(n), "fallback": np.zeros(n, dtype=bool), "current": np.zeros((n, 6))}
+
+
+def material_energies(strain, stress, current_prev, current, dt):
+    """Stored energy 1/2 (strain - ein') . stress and inelastic power
+    stress . (ein' - ein_prev) / dt of one surrogate call (ein' = ``current``, the
+    call's inelastic strain; ein_prev the previous call's).  The network has no
+    hardening energy, so the stored energy is the elastic one of the homogenised
+    stiffness (the free energy / dissipation pair the reference's adapter returns,
+    ``nn/models/rve_rno.py:285-308``, in this network's state variables)."""
+    psi = 0.5 * np.einsum("ni,ni->n", strain - current, stress)
+    diss = np.einsum("ni,ni->n", stress, current - current_prev) / dt
+    return psi, diss
 
 
 def material_update(params, gate, strain, dt, anchor):
@@ -186,5 +198,6 @@ def material_update(params, gate, strain, dt, anchor):
            "inelastic": np.where(c, ein, anchor["inelastic"]),
            "latent": np.where(c, z, anchor["latent"]),
            "age": np.where(commit, 0.0, tau),
-           "fallback": anchor["fallback"] | trip}
+           "fallback": anchor["fallback"] | trip,
+           "current": ein}
     return stress, nxt, trip, int(np.sum(trip & ~anchor["fallback"]))

@@ -239,10 +239,12 @@ def test_gate_sends_out_of_envelope_points_to_fe2(gpu):
         r = sm.update(req, st)
         fb = r.state.variables["surrogate_fallback"]
         assert fb[2] == 1.0 and fb.sum() == 1.0
-        # the fallback stress equals a virgin RVE driven to the same strain in 4 increments
+        # the point trips on its first anchor: the fallback stress equals a virgin RVE
+        # driven along the straight path over the elapsed time, in two segments of
+        # 4 increments (8 increments of dt / 8)
         ns = ad.initialize_state(1)
-        for k in range(1, 5):
-            rr = ad.update(M.MaterialPointRequest(eps[2:3] * k / 4, None, 1e-5 / 4, "three_dimensional",
+        for k in range(1, 9):
+            rr = ad.update(M.MaterialPointRequest(eps[2:3] * k / 8, None, 1e-5 / 8, "three_dimensional",
                                                   M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)), ns)
             ns = rr.state
         np.testing.assert_allclose(r.stresses[2], rr.stresses[0], rtol=1e-12)

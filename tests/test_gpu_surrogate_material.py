@@ -29,7 +29,8 @@ def _anchor_to_oracle(st):
     return {"strain": st.strain.t().cpu().numpy().copy(),
             "inelastic": st.inelastic.t().cpu().numpy().copy(),
             "latent": st.latent.t().double().cpu().numpy().copy(),
-            "age": st.age.double().cpu().numpy().copy(), "fallback": st.fallback.cpu().numpy() > 0}
+            "age": st.age.double().cpu().numpy().copy(), "fallback": st.fallback.cpu().numpy() > 0,
+            "current": st.current.t().cpu().numpy().copy()}
 
 
 def _material(seed, **kw):
@@ -63,14 +64,17 @@ def _raw_update(sm, st, strain, dt, trips):
     stress = torch.empty_like(st.strain)
     nxt = S._SurrogateState(torch.empty_like(st.strain), torch.empty_like(st.inelastic),
                             torch.empty_like(st.latent), torch.empty_like(st.age),
-                            torch.empty_like(st.fallback), 0)
+                            torch.empty_like(st.fallback), 0, current=torch.empty_like(st.current))
+    psi = torch.empty(strain.shape[1], dtype=torch.float64, device=strain.device)
+    diss = torch.empty_like(psi)
     gate = sm.gate()
     _lib.check(_lib.lib().cia_surrogate_update(
         stream_handle(), strain.shape[1], p(strain), float(dt), p(sm.params.w_bf16), p(sm.params.b),
         C.byref(gate), p(st.strain), p(st.inelastic), p(st.latent), p(st.age), p(st.fallback), p(stress),
-        p(nxt.strain), p(nxt.inelastic), p(nxt.latent), p(nxt.age), p(nxt.fallback), p(trips)),
+        p(nxt.strain), p(nxt.inelastic), p(nxt.latent), p(nxt.age), p(nxt.fallback), p(trips),
+        p(st.current), p(nxt.current), p(psi), p(diss)),
         "cia_surrogate_update")
-    return stress, nxt
+    return stress, nxt, psi, diss
 
 
 @pytest.mark.parametrize("n", [1, 127, 1000, 4133])
@@ -93,7 +97,7 @@ def test_surrogate_update_kernel_matches_oracle(gpu, n):
         eps = eps + d
         dt = 10.0 ** rng.uniform(-7, -6)
         strain = torch.as_tensor(np.ascontiguousarray(eps.T), device=st.strain.device)
-        r_stress, r_state = _raw_update(sm, st, strain, dt, trips)
+        r_stress, r_state, r_psi, r_diss = _raw_update(sm, st, strain, dt, trips)
         s_ref, a_ref, trip, n_new = OM.material_update(layers, g, eps, dt, a)
         total_trips += n_new
         nxt = _anchor_to_oracle(r_state)
@@ -112,7 +116,20 @@ def test_surrogate_update_kernel_matches_oracle(gpu, n):
         assert np.max(np.abs(nxt["inelastic"] - a_ref["inelastic"])) <= 2e-2 * contrib, call
         assert np.max(np.abs(nxt["latent"] - a_ref["latent"])) <= 2e-2 * max(
             1.0, np.max(np.abs(a_ref["latent"] - a["latent"]))), call
+        # energies: the identities on the kernel's own outputs (round-off), and the
+        # oracle's energies within the bf16 tolerance of the stress
+        got_cur = r_state.current.t().cpu().numpy()
+        prev_cur = st.current.t().cpu().numpy()
+        psi_id, diss_id = OM.material_energies(eps, got, prev_cur, got_cur, dt)
+        np.testing.assert_allclose(r_psi.cpu().numpy(), psi_id, rtol=1e-12, atol=1e-12 * np.max(np.abs(psi_id)))
+        np.testing.assert_allclose(r_diss.cpu().numpy(), diss_id, rtol=1e-10,
+                                   atol=1e-10 * np.max(np.abs(diss_id)))
+        np.testing.assert_allclose(got_cur, eps - np.linalg.solve(C6, got.T).T, rtol=0,
+                                   atol=1e-9 * np.max(np.abs(eps)))
+        psi_ref, _ = OM.material_energies(eps, s_ref, a["current"], a_ref["current"], dt)
+        assert np.max(np.abs(r_psi.cpu().numpy() - psi_ref)) <= 5e-2 * np.max(np.abs(psi_ref)), call
         # the oracle continues from the kernel's anchors (the same decisions, bf16 values)
+        nxt["current"] = got_cur
         st, a = r_state, nxt
     assert total_trips > 0 or n < 100
 
