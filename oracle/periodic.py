@@ -67,14 +67,22 @@ def affine_matrix(nodes):
 
 
 def structured_constraint(nodes, divisions) -> Constraint:
-    cls_raw = structured_class_ids(divisions)
+    """``build_structured_hex8_periodic_constraint`` (``src/fem/rve/periodic.py:20-47``)."""
+    return class_constraint(nodes, structured_class_ids(divisions), divisions, face_nodes(divisions))
+
+
+def class_constraint(nodes, class_ids, divisions, faces) -> Constraint:
+    """``build_periodic_constraint_from_classes`` (``src/fem/rve/periodic.py:50-104``):
+    classes renumbered by ``np.unique``, each class's representative its first node,
+    the reference class that of the first x- face node (its fluctuation is
+    eliminated), T the 0/1 map from the other classes' fluctuations to the nodes."""
+    cls_raw = np.asarray(class_ids, dtype=np.int64)
     uniq, cls = np.unique(cls_raw, return_inverse=True)
     cls = cls.astype(np.int64)
     count = uniq.size
     reps = np.full(count, -1, dtype=np.int64)
     order = np.arange(cls.size)[::-1]
     reps[cls[order]] = order  # first occurrence wins
-    faces = face_nodes(divisions)
     ref = int(cls[faces[0][0][0]])
     reduced = np.cumsum(np.arange(count) != ref) - 1  # class -> reduced id
     keep = cls != ref

@@ -99,10 +99,17 @@ class Model:
     newton: Newton
     element_phase_ids: np.ndarray | None = None
     hill_mandel_power_scale: float = 1.0
+    mesh: tuple | None = None  # (nodes, elements, class_ids, faces) of an unstructured periodic RVE
 
     def __post_init__(self):
-        self.nodes, self.elements = hex8.structured_box(self.lengths, self.divisions)
-        self.constraint = periodic.structured_constraint(self.nodes, self.divisions)
+        if self.mesh is None:
+            self.nodes, self.elements = hex8.structured_box(self.lengths, self.divisions)
+            self.constraint = periodic.structured_constraint(self.nodes, self.divisions)
+        else:
+            nodes, elements, class_ids, faces = self.mesh
+            self.nodes = np.asarray(nodes, dtype=np.float64)
+            self.elements = np.asarray(elements, dtype=np.int64)
+            self.constraint = periodic.class_constraint(self.nodes, class_ids, self.divisions, faces)
         self.b, self.jw = hex8.geometry_cache(self.nodes, self.elements)
         self.weights = self.jw.reshape(-1)
         self.volume = float(np.sum(self.weights))

@@ -5,7 +5,7 @@ definitions (``src/concrete_impact/benchmarks/rve.py``, ``plasticity.py``,
 the dynamics solvers (``src/fem/dynamics/material.py:139,165,337``).
 
 The configs are the reference's files, copied to tests/golden/configs by
-tests/golden/copy_configs.py.
+tests/golden/copy_reference_data.py.
 """
 
 from pathlib import Path
