@@ -12,9 +12,11 @@ LZF (filter 32000, the LZF format of liblzf) filters, fixed-point, floating-poin
 fixed-length string and variable-length string datatypes (the latter through
 the global heap), and attributes.
 
-The writer produces superblock-0 files: symbol-table groups, contiguous
-datasets, version-1 attributes (numeric, fixed-length ASCII strings); arrays are
-written uncompressed.
+The writer produces superblock-0 files with the encodings h5py itself writes
+(as found in the reference's sample shard): symbol-table groups (multi-level
+B-trees for large groups), contiguous datasets (uncompressed), the bool
+enumeration, fixed-length byte strings, and version-1 attributes whose ``str``
+values are variable-length UTF-8 strings in a global heap.
 """
 
 from __future__ import annotations
@@ -249,7 +251,6 @@ class File(Group):
                 name, value = self._attribute(data)
                 out["attrs"][name] = value
             elif mtype == 0x11:
-                btree, heap = self._off(0), self._off(self.so)
                 btree = int.from_bytes(data[:self.so], "little")
                 heap = int.from_bytes(data[self.so:2 * self.so], "little")
                 out["links"].update(self._symbol_table(btree, heap))
@@ -286,8 +287,10 @@ class File(Group):
                 return np.dtype(object), "str"
             base, _ = self._datatype(d[8:])
             return np.dtype(object), base
-        if cls == 8:  # enumeration: read the base type
+        if cls == 8:  # enumeration: h5py's FALSE / TRUE over int8 is a bool, others the base type
             base, _ = self._datatype(d[8:])
+            if (bits & 0xFFFF) == 2 and size == 1:
+                return np.dtype(bool), None
             return base, None
         raise ValueError(f"unsupported datatype class {cls}")
 
@@ -559,3 +562,267 @@ def read_tree(path) -> dict:
 
     f.visit(visit)
     return out
+
+
+# ---------------------------------------------------------------------------
+# writer
+
+_LEAF_K, _INTERNAL_K = 4, 16        # symbol-table node capacity 2 * 4, B-tree fan-out 2 * 16
+_SNOD_SIZE = 8 + 2 * _LEAF_K * 40
+_TREE_SIZE = 24 + (2 * _INTERNAL_K + 1) * 8 + 2 * _INTERNAL_K * 8
+
+
+def _pad8(b: bytes) -> bytes:
+    return b + bytes((-len(b)) % 8)
+
+
+class _Node:
+    def __init__(self, kind, data=None):
+        self.kind = kind
+        self.children: dict = {}
+        self.attrs: dict = {}
+        self.data = data
+
+
+class Writer:
+    """Build a file in memory (``group``, ``dataset``, ``attrs``) and ``save`` it.
+
+    Datatypes: float32/64, signed and unsigned integers, bool (the FALSE/TRUE
+    enumeration over int8 that h5py maps to numpy bool), fixed-length byte strings
+    (numpy ``S``); attribute values may also be ``str`` (variable-length UTF-8,
+    stored in the global heap, read back as ``str`` by h5py) or numbers."""
+
+    def __init__(self):
+        self.root = _Node("group")
+
+    @property
+    def attrs(self) -> dict:
+        return self.root.attrs
+
+    def _walk(self, path, create_last_as=None, data=None):
+        parts = [p for p in path.split("/") if p]
+        node = self.root
+        for i, part in enumerate(parts):
+            last = i == len(parts) - 1
+            if part not in node.children:
+                if last and create_last_as == "dataset":
+                    node.children[part] = _Node("dataset", data)
+                else:
+                    node.children[part] = _Node("group")
+            elif last and create_last_as == "dataset":
+                raise ValueError(f"{path} exists")
+            node = node.children[part]
+            if node.kind != "group" and not last:
+                raise ValueError(f"{path}: {part} is a dataset")
+        return node
+
+    def group(self, path) -> _Node:
+        node = self._walk(path)
+        if node.kind != "group":
+            raise ValueError(f"{path} is a dataset")
+        return node
+
+    def dataset(self, path, data) -> _Node:
+        arr = np.asarray(data)
+        if arr.dtype.kind == "U":
+            arr = np.char.encode(arr, "utf-8")
+        return self._walk(path, "dataset", np.ascontiguousarray(arr))
+
+    def __contains__(self, path):
+        node = self.root
+        for part in [p for p in path.split("/") if p]:
+            if node.kind != "group" or part not in node.children:
+                return False
+            node = node.children[part]
+        return True
+
+    def get(self, path) -> _Node:
+        node = self.root
+        for part in [p for p in path.split("/") if p]:
+            node = node.children[part]
+        return node
+
+    # -- encoding
+    @staticmethod
+    def _dtype_msg(dt: np.dtype) -> bytes:
+        dt = np.dtype(dt)
+        if dt.kind == "f":
+            prec = dt.itemsize * 8
+            exp_loc, exp_size, man_size, bias = {8: (52, 11, 52, 1023), 4: (23, 8, 23, 127)}[dt.itemsize]
+            return (bytes([0x11, 0x20, prec - 1, 0]) + struct.pack("<I", dt.itemsize)
+                    + struct.pack("<HHBBBBI", 0, prec, exp_loc, exp_size, 0, man_size, bias))
+        if dt.kind in "iu":
+            return (bytes([0x10, 0x08 if dt.kind == "i" else 0x00, 0, 0]) + struct.pack("<I", dt.itemsize)
+                    + struct.pack("<HH", 0, dt.itemsize * 8))
+        if dt.kind == "b":
+            base = bytes([0x10, 0x08, 0, 0]) + struct.pack("<I", 1) + struct.pack("<HH", 0, 8)
+            return (bytes([0x18, 2, 0, 0]) + struct.pack("<I", 1) + base + _pad8(b"FALSE\0")
+                    + _pad8(b"TRUE\0") + bytes([0, 1]))
+        if dt.kind == "S":
+            return bytes([0x13, 0x01, 0, 0]) + struct.pack("<I", max(dt.itemsize, 1))
+        raise ValueError(f"unsupported dtype {dt}")
+
+    @staticmethod
+    def _space_msg(shape, maxdims=True) -> bytes:
+        rank = len(shape)
+        out = bytes([1, rank, 1 if (maxdims and rank) else 0, 0]) + bytes(4)
+        out += b"".join(struct.pack("<Q", int(d)) for d in shape)
+        if maxdims and rank:
+            out += b"".join(struct.pack("<Q", int(d)) for d in shape)
+        return out
+
+    @staticmethod
+    def _msg(mtype, data, flags=0) -> bytes:
+        data = _pad8(data)
+        return struct.pack("<HHB3x", mtype, len(data), flags) + data
+
+    @staticmethod
+    def _header(msgs) -> bytes:
+        body = b"".join(msgs)
+        return struct.pack("<BBHII4x", 1, 0, len(msgs), 1, len(body)) + body
+
+    def _attr_msg(self, name, value, heap_ids) -> bytes:
+        nm = name.encode("utf-8") + b"\0"
+        if isinstance(value, str):
+            dt = bytes([0x19, 0x01, 0x01, 0x00]) + struct.pack("<I", 16) + \
+                bytes([0x10, 0, 0, 0]) + struct.pack("<I", 1) + struct.pack("<HH", 0, 8)
+            sp = self._space_msg((), False)
+            raw = value.encode("utf-8")
+            addr, idx = heap_ids[raw]
+            data = struct.pack("<IQI", len(raw), addr, idx)
+        else:
+            arr = np.asarray(value)
+            if arr.dtype.kind == "U":
+                arr = np.char.encode(arr, "utf-8")
+            if arr.dtype.kind in "f":
+                arr = arr.astype("<f8")
+            elif arr.dtype.kind in "iu" and arr.dtype.itemsize < 8:
+                arr = arr.astype("<i8" if arr.dtype.kind == "i" else "<u8")
+            dt = self._dtype_msg(arr.dtype)
+            sp = self._space_msg(arr.shape, False)
+            data = np.ascontiguousarray(arr).tobytes()
+        body = struct.pack("<BBHHH", 1, 0, len(nm), len(dt), len(sp)) + _pad8(nm) + _pad8(dt) + _pad8(sp) + data
+        return self._msg(0x0C, body)
+
+    # -- layout
+    def save(self, path) -> Path:
+        path = Path(path)
+        strings = []
+
+        def collect(node):
+            for v in node.attrs.values():
+                if isinstance(v, str):
+                    strings.append(v.encode("utf-8"))
+            for c in node.children.values():
+                collect(c)
+
+        collect(self.root)
+        out = bytearray(96)
+        heap_ids = {}
+        if strings:
+            uniq = list(dict.fromkeys(strings))
+            objs = b""
+            offs = {}
+            for i, raw in enumerate(uniq, start=1):
+                offs[raw] = i
+                objs += struct.pack("<HH4xQ", i, 0, len(raw)) + _pad8(raw)
+            size = max(4096, 16 + len(objs) + 16)
+            free = size - 16 - len(objs)
+            gaddr = len(out)
+            out += b"GCOL" + bytes([1, 0, 0, 0]) + struct.pack("<Q", size) + objs
+            out += struct.pack("<HH4xQ", 0, 0, free) + bytes(free - 16)
+            heap_ids = {raw: (gaddr, i) for raw, i in offs.items()}
+        root_addr, btree, lheap = self._write_group(out, self.root, heap_ids)
+        sb = bytearray(SIGNATURE + bytes([0, 0, 0, 0, 0, 8, 8, 0]))
+        sb += struct.pack("<HH", _LEAF_K, _INTERNAL_K) + struct.pack("<I", 0)
+        sb += struct.pack("<QQQQ", 0, UNDEF, len(out), UNDEF)
+        sb += struct.pack("<QQI4xQQ", 0, root_addr, 1, btree, lheap)
+        out[:96] = sb
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_bytes(bytes(out))
+        return path
+
+    @staticmethod
+    def _align(out):
+        out += bytes((-len(out)) % 8)
+
+    def _write_dataset(self, out, node, heap_ids) -> int:
+        arr = node.data
+        dt = arr.dtype
+        if dt.kind == "f" and dt.byteorder == ">":
+            arr = arr.astype(dt.newbyteorder("<"))
+        self._align(out)
+        nbytes = arr.nbytes
+        if nbytes:
+            addr = len(out)
+            out += arr.tobytes()
+        else:
+            addr = UNDEF
+        self._align(out)
+        msgs = [self._msg(0x01, self._space_msg(arr.shape)),
+                self._msg(0x03, self._dtype_msg(arr.dtype), flags=1),
+                self._msg(0x05, bytes([2, 2, 2, 1]), flags=1),
+                self._msg(0x08, bytes([3, 1]) + struct.pack("<QQ", addr, nbytes))]
+        msgs += [self._attr_msg(k, v, heap_ids) for k, v in node.attrs.items()]
+        haddr = len(out)
+        out += self._header(msgs)
+        return haddr
+
+    def _write_group(self, out, node, heap_ids):
+        names = sorted(node.children, key=lambda s: s.encode("utf-8"))
+        addrs = {}
+        for nm in names:
+            child = node.children[nm]
+            if child.kind == "group":
+                addrs[nm] = self._write_group(out, child, heap_ids)[0]
+            else:
+                addrs[nm] = self._write_dataset(out, child, heap_ids)
+        # local heap: "" at offset 0, then the names
+        seg = bytearray(8)
+        name_off = {}
+        for nm in names:
+            name_off[nm] = len(seg)
+            seg += _pad8(nm.encode("utf-8") + b"\0")
+        self._align(out)
+        seg_addr = len(out)
+        out += seg
+        heap_addr = len(out)
+        out += b"HEAP" + bytes(4) + struct.pack("<QQQ", len(seg), UNDEF, seg_addr)
+        # symbol-table nodes, 2 * leaf K entries each
+        leaves = []  # (address, first key offset, last name offset)
+        for s in range(0, max(len(names), 1), 2 * _LEAF_K) if names else []:
+            chunk = names[s:s + 2 * _LEAF_K]
+            addr = len(out)
+            snod = bytearray(b"SNOD" + bytes([1, 0]) + struct.pack("<H", len(chunk)))
+            for nm in chunk:
+                snod += struct.pack("<QQI4x16x", name_off[nm], addrs[nm], 0)
+            snod += bytes(_SNOD_SIZE - len(snod))
+            out += snod
+            leaves.append((addr, name_off[chunk[-1]]))
+        # B-tree levels over the symbol-table nodes
+        level = 0
+        nodes = leaves
+        while True:
+            parents = []
+            groups = [nodes[i:i + 2 * _INTERNAL_K] for i in range(0, max(len(nodes), 1), 2 * _INTERNAL_K)] \
+                if nodes else [[]]
+            for grp in groups:
+                addr = len(out)
+                t = bytearray(b"TREE" + bytes([0, level]) + struct.pack("<H", len(grp))
+                              + struct.pack("<QQ", UNDEF, UNDEF))
+                t += struct.pack("<Q", 0)
+                for child_addr, last in grp:
+                    t += struct.pack("<QQ", child_addr, last)
+                t += bytes(_TREE_SIZE - len(t))
+                out += t
+                parents.append((addr, grp[-1][1] if grp else 0))
+            if len(parents) == 1:
+                btree = parents[0][0]
+                break
+            nodes = parents
+            level += 1
+        msgs = [self._msg(0x11, struct.pack("<QQ", btree, heap_addr))]
+        msgs += [self._attr_msg(k, v, heap_ids) for k, v in node.attrs.items()]
+        haddr = len(out)
+        out += self._header(msgs)
+        return haddr, btree, heap_addr

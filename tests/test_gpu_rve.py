@@ -254,3 +254,22 @@ def mat_tangent(mat):
     C[np.arange(3), np.arange(3)] += 2 * G
     C[np.arange(3, 6), np.arange(3, 6)] = G
     return C
+
+
+def test_device_rve_path_writes_a_reference_schema_shard(gpu, tmp_path):
+    """solve_prescribed_rve_path on the device, written with write_rve_path_hdf5
+    (``src/fem/rve/io.py:32-50``), reads back with the accepted responses' values."""
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import rve_io as IO
+
+    model, _ = _two_phase((4, 4, 4), seed=3)
+    times = np.asarray([0.0, 1e-5, 2e-5, 3e-5])
+    d = np.asarray([1.0, -0.3, -0.2, 0.5, -0.4, 0.6])
+    strains = np.outer(np.arange(4) / 3.0, 2.5e-3 * d)
+    res = R.solve_prescribed_rve_path(model, times, strains, _settings())
+    out = IO.write_rve_path_hdf5(tmp_path / "p.h5", model, "ramp", res, {"model_id": "box4"})
+    assert IO.validate_rve_data_shard(out) == {"model_count": 1, "path_count": 1}
+    arr = IO.read_rve_path_arrays(out, "ramp")
+    np.testing.assert_array_equal(arr["macro_stress"], np.stack([r.macro_stress for r in res.responses]))
+    np.testing.assert_array_equal(arr["micro_strain"], np.stack([r.micro_strains for r in res.responses]))
+    np.testing.assert_array_equal(arr["macro_strain"], strains[1:])
