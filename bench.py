@@ -622,6 +622,91 @@ def macro_explicit_section(args) -> dict:
     }
 
 
+FP32_PEAK_TFLOPS = 157.3
+RNO_FLOP_PER_POINT = 708_864  # algorithmic: 3 energy-gradient chains + 1 evolution forward
+
+
+def rve_rno_section(args) -> dict:
+    """The reference's trained RVE-RNO (artifacts/rve_rno/c48_v2_liu_direct, committed
+    as tests/golden/rve_rno_c48_v2_liu_direct.npz) through cia_rno_update, FP32:
+
+    * frozen-test evaluation shape of the reference (training_summary.json: one padded
+      batch, 38 400 material slots = 200 paths x 192 steps, 24 064 valid steps,
+      159 156 valid steps/s on an RTX 5060 Ti): 200 synthetic load / unload paths
+      rolled out for 192 sequential steps, slots/s and valid steps/s at the same
+      62.67 % padding efficiency;
+    * batch throughput: one update of ``--rno-batch`` points (HIP events around K
+      updates), without and with the tangent; roofline on the FP32 matrix-core peak
+      with 708 864 algorithmic flop per point (the two energies' value-and-gradient
+      chains at 6 and twice at 12 inputs, the evolution forward; 8 x that with the
+      tangent's six directions and one padding column)."""
+    import torch
+
+    from concrete_impact_ai_amd import rve_rno as RR
+
+    model = RR.load_rve_rno_artifact(ROOT / "tests" / "golden" / "rve_rno_c48_v2_liu_direct.npz")
+    dev = model.device
+    meta_scale = torch.tensor([0.0107, 0.0049, 0.0059, 0.0082, 0.0024, 0.0014],
+                              dtype=torch.float32, device=dev)  # training strain rms
+    gen = torch.Generator(device=dev).manual_seed(5)
+
+    # frozen-test shape: 200 paths x 192 steps, sequential
+    paths, steps = 200, 192
+    amp = torch.randn((paths, 6), generator=gen, device=dev) * meta_scale * 2.0
+    t = torch.arange(1, steps + 1, device=dev, dtype=torch.float32)
+    shape = torch.where(t <= steps // 2, t / (steps // 2), (steps - t) / (steps // 2))
+    strain = shape[:, None, None] * amp[None]
+    dt = torch.full((paths,), 1.0e-4, device=dev)
+
+    def rollout():
+        z = model.initial_state_batch(paths)
+        for k in range(steps):
+            z = model.update_batch(strain[k], z, dt).latent_state
+        return z
+
+    rollout()
+    torch.cuda.synchronize()
+    reps = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        rollout()
+        torch.cuda.synchronize()
+        reps.append(time.perf_counter() - t0)
+    el = float(np.median(reps))
+    slots = paths * steps
+    frozen = {"paths": paths, "steps": steps, "seconds_median_of_5": el,
+              "slots_per_s": slots / el, "valid_steps_per_s_at_reference_padding": 0.6267 * slots / el,
+              "reference_valid_steps_per_s": 159156.0,
+              "reference_hardware": "NVIDIA GeForce RTX 5060 Ti, torch inductor, FP32"}
+
+    def batch_rate(n, tangent, reps=10):
+        s = torch.randn((n, 6), generator=gen, device=dev) * meta_scale * 2.0
+        z = torch.randn((n, 6), generator=gen, device=dev) * 0.5
+        h = torch.full((n,), 1.0e-4, device=dev)
+        model.update_batch(s, z, h, compute_tangent=tangent)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            model.update_batch(s, z, h, compute_tangent=tangent)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        flop = RNO_FLOP_PER_POINT * n * (8 if tangent else 1)
+        return {"points": n, "ms_per_update": ms, "point_updates_per_s": n / ms * 1e3,
+                "tflops": flop / ms / 1e9, "frac_fp32_mfma_peak": flop / ms / 1e9 / FP32_PEAK_TFLOPS}
+
+    n = args.rno_batch
+    plain = batch_rate(n, False)
+    tangent = batch_rate(max(n // 8, 64), True)
+    return {"workload": "reference RVE-RNO artifact c48_v2_liu_direct (6 -> 128^4 SiLU energies, "
+                        "direct-rate evolution, latent 6), FP32",
+            "frozen_test_shape": frozen, "batch": plain, "batch_with_tangent": tangent,
+            "roofline": {"bound": "mfma", "achieved": plain["tflops"], "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": plain["frac_fp32_mfma_peak"],
+                         "flop_per_point": RNO_FLOP_PER_POINT}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -645,6 +730,9 @@ def main() -> int:
     ap.add_argument("--surrogate-only", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched FE² section")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--no-rno", action="store_true", help="skip the reference RVE-RNO section")
+    ap.add_argument("--rno-only", action="store_true")
+    ap.add_argument("--rno-batch", type=int, default=1 << 20)
     ap.add_argument("--batch-rves", type=int, default=256)
     ap.add_argument("--batch-warmup", type=int, default=2)
     ap.add_argument("--batch-steps", type=int, default=8)
@@ -690,6 +778,9 @@ def main() -> int:
         return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_section(args)}))
+        return 0
+    if args.rno_only:
+        print(json.dumps({"rve_rno": rve_rno_section(args)}))
         return 0
     n = args.size
     model, eph = build_model(n, args.seed, args.precision, args.pcg_rtol, not args.fixed_forcing)
@@ -880,6 +971,9 @@ def main() -> int:
     if not args.no_batched:
         print("batched FE² ...", file=sys.stderr, flush=True)
         line["batched"] = batched_section(args)
+    if not args.no_rno:
+        print("reference RVE-RNO ...", file=sys.stderr, flush=True)
+        line["rve_rno"] = rve_rno_section(args)
     if not args.no_surrogate:
         line["surrogate"], ctx = surrogate_section(args)
         if not args.no_macro_surrogate:

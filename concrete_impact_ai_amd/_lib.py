@@ -34,6 +34,11 @@ CIA_OUT_DIAGNOSTICS = 4
 CIA_PREC_FP64 = 0
 CIA_PREC_FP32_REFINED = 1
 
+CIA_RNO_DIRECT_RATE = 0
+CIA_RNO_MOBILITY_GRADIENT = 1
+CIA_RNO_CS_SEMI_IMPLICIT = 0
+CIA_RNO_CURRENT_STATE = 1
+
 # Point failure reason strings, index = CIA_PT_* code (plasticity.py:543-854,918,1441-1465).
 POINT_REASONS = [
     "ok",
@@ -352,6 +357,12 @@ _SIGNATURES.update({
     "cia_surrogate_update": (
         C.c_int32, [_vp, C.c_int64, _vp, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cia_rno_packed_size": (C.c_int32, [C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
+    "cia_rno_anchors": (C.c_int32, [_vp, _vp, C.c_int32, C.c_int32, _vp]),
+    "cia_rno_update": (
+        C.c_int32, [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                    C.POINTER(C.c_double), _vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                    _vp, _vp, _vp, _vp]),
 })
 
 BATCH_KERNELS = ["assemble", "gather", "pcg_elem", "pcg_node", "pcg_update", "symbol_solve",

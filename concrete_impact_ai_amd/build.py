@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 INCLUDE = PKG.parent / "include"
 LIB = PKG / "libcia_hip.so"
-SOURCES = ["material.hip", "rve.hip", "dynamics.hip", "mlp.hip", "batch.hip"]
+SOURCES = ["material.hip", "rve.hip", "dynamics.hip", "mlp.hip", "batch.hip", "rno.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

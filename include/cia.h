@@ -621,6 +621,42 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
 cia_status cia_profile_enable(int32_t on);
 cia_status cia_profile_read(double* total_ms, int64_t* counts, int32_t n);
 
+/* Energy-based RVE recurrent neural operator (the reference's trained surrogate
+ * family): one explicit update of n material points,
+ * EnergyDissipationRVERNO.update_batch / _constitutive_tensors
+ * (src/concrete_impact/nn/models/rve_rno.py:249-418), FP32 like the reference's
+ * artifacts, with the algorithmic tangent of rve_rno.py:290-302 when tangent is not
+ * NULL.  Architecture: 4 SiLU hidden layers of width 128 (the artifact's), latent
+ * dimension m = 1..8, evolution CIA_RNO_DIRECT_RATE or CIA_RNO_MOBILITY_GRADIENT,
+ * integrator CIA_RNO_CS_SEMI_IMPLICIT ("cs_semi_implicit_forward_euler") or
+ * CIA_RNO_CURRENT_STATE ("current_state_stress_forward_euler").
+ * packed: device FP32 weights, cia_rno_packed_size floats, three networks in the
+ * order equilibrium_energy, nonequilibrium_energy, state_evolution, each laid out as
+ *   w1 [128][16] (input columns beyond the input width zero), b1 [128],
+ *   3 x (w [128][128], b [128]) for hidden layers 2..4, their 3 transposes
+ *   [128][128], w1 transposed [16][128], wout [R][128] and bout [R],
+ *   R = 16 ceil(n_out / 16), rows beyond n_out and the energies' bout zero
+ * (nn.Linear weights are [out][in]).  anchors: device [14] from cia_rno_anchors
+ * (prepare_energy_anchors, rve_rno.py:224-247).  Per point, row-major: strain [n][6]
+ * (engineering shear, Voigt xx yy zz yz xz xy), latent [n][m], time_step [n];
+ * outputs stress [n][6], latent_out [n][m] (may alias latent), state_rate and
+ * dimensionless_state_rate [n][m], dissipation, free_energy,
+ * thermodynamic_violation, nonequilibrium_energy_increment [n], tangent [n][6][6]
+ * (d stress_i / d strain_j at [i][j]) or NULL.  strain_input_scale: host [6]. */
+enum { CIA_RNO_DIRECT_RATE = 0, CIA_RNO_MOBILITY_GRADIENT = 1 };
+enum { CIA_RNO_CS_SEMI_IMPLICIT = 0, CIA_RNO_CURRENT_STATE = 1 };
+cia_status cia_rno_packed_size(int32_t latent_dimension, int32_t evolution, int64_t* floats);
+cia_status cia_rno_anchors(void* stream, const float* packed, int32_t latent_dimension,
+                           int32_t evolution, float* anchors);
+cia_status cia_rno_update(void* stream, const float* packed, int32_t latent_dimension,
+                          int32_t evolution, int32_t integrator, double reference_time_scale,
+                          const double* strain_input_scale, const float* anchors, int64_t n,
+                          const float* strain, const float* latent, const float* time_step,
+                          float* stress, float* latent_out, float* state_rate,
+                          float* dimensionless_state_rate, float* dissipation, float* free_energy,
+                          float* thermodynamic_violation, float* nonequilibrium_energy_increment,
+                          float* tangent);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from oracle import hex8, j2vp
+from oracle import hex8, j2vp, rve_rno
 
 
 def box_faces(divisions):
@@ -79,6 +79,9 @@ class Box:
 
     def init_state(self):
         st = {}
+        mat0 = self.phases[0][1]
+        if isinstance(mat0, rve_rno.RVERNO):
+            return {"latent_state": np.zeros((self.ppid.size, mat0.config.latent_dimension))}
         for pid, (name, mat) in enumerate(self.phases):
             if isinstance(mat, j2vp.J2VPParams):
                 cnt = int(np.sum(self.ppid == pid))
@@ -97,7 +100,13 @@ class Box:
             r = j2vp.phased_update(self.phases, self.ppid, eps, state, dt, settings, rate, True)
         else:
             mat = self.phases[0][1]
-            if isinstance(mat, j2vp.J2VPParams):
+            if isinstance(mat, rve_rno.RVERNO):
+                # the RVE-RNO deployment adapter (nn/deployment.py:97-165): one explicit
+                # update of every point from its committed latent state
+                r = mat.update_batch(eps, state["latent_state"], np.full(eps.shape[0], dt),
+                                     tangent=need_tangent)
+                r["state"] = {"latent_state": r["latent_state"]}
+            elif isinstance(mat, j2vp.J2VPParams):
                 r = j2vp.j2vp_update(mat, eps, state["plastic_strain"], state["equivalent_plastic_strain"],
                                      state["viscoplastic_multiplier"], dt, settings, rate, need_tangent)
                 r["state"] = {k: r[k] for k in ("plastic_strain", "equivalent_plastic_strain",
