@@ -17,6 +17,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 
 #include "common.hpp"
@@ -502,6 +503,648 @@ __global__ void __launch_bounds__(256, 2)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused BPTT training step (the batch step of train_rve_rno,
+// src/concrete_impact/nn/training/rve_rno.py:200-415, for this build's network and
+// its path-normalised stress loss).  One call computes the loss and the weight and
+// bias gradients of a batch of B paths over a window of L steps from step 0:
+//
+//   k_bptt_prep    training weight blob (bf16, rows padded to multiples of 32
+//                  elements): the forward layers and the transposed layers of the
+//                  backward pass, from the fp32 master weights
+//   k_bptt_fwd     one workgroup per tile of 16 paths runs the L-step recurrence
+//                  (network, inelastic strain, latent, stress, loss) and stores the
+//                  bf16 network inputs x_t (transposed) and q_t = Cs^T dL/dsigma_t
+//   k_bptt_bwd     the same tiles in reverse time: the hidden layers are recomputed
+//                  from x_t (GELU' kept in registers), dL/do_t is formed from the
+//                  adjoints of the inelastic strain and the latent, and the layers are
+//                  back-propagated; layer inputs and pre-activation gradients go to
+//                  HBM transposed ([feature][point-step], bf16)
+//   k_wgrad_nt     dW = sum over point-steps of dz a^T, split over point-steps into
+//                  fp32 partials, with the bias row sums
+//   k_bptt_reduce  partials -> gradient blobs, loss partials -> loss
+//
+// Point-step index m = t Bp + b (Bp = B rounded up to the tile).  Padding paths
+// have zero inputs and zero loss weight, so they contribute nothing.
+//
+// Recurrence (scaled units): x_t = [f_t (6), r_t, z_t (6)], o_t = MLP(x_t),
+// ein_t = ein_{t-1} + c o_t[0:6], sigma_t = Cs (eps_t - ein_t), z_{t+1} = z_t + o_t[6:12],
+// loss = sum_b w_b sum_{t,i} (sigma_t - target_t)_i^2 / (B L 6).
+// Adjoints in reverse time: G_t = G_{t+1} - q_t (dL/dein_t), do_t = [c G_t, gz_{t+1}],
+// gz_t = gz_{t+1} + dL/dx_t[7:13] (dL/dz_t).
+//
+// Every layer is the transposed product out^T = W act^T on the matrix cores: the A
+// operand is 16 weight rows (16-byte loads from the L2-resident blob, issued one
+// layer ahead), the B operand 16 points' activations from LDS; 4 waves (one per
+// SIMD, up to 512 registers each) share the 256 rows of a layer (64 each).  The
+// 16-row layers (output layer, input gradient) split their 8 k-steps over the waves
+// and sum the partials through LDS.
+
+constexpr int kTrainThreads = 256;
+constexpr int TLD = FH + 8;                 // LDS row stride of activations (bf16)
+constexpr int DLD = 40;                     // LDS row stride of dL/do (bf16)
+// training weight blob (bf16 element offsets)
+constexpr int64_t oF0 = 0;                  // W1 [256][32], columns >= 13 zero
+constexpr int64_t oF1 = oF0 + 256 * 32;     // W2 [256][256]
+constexpr int64_t oF2 = oF1 + 65536;        // W3
+constexpr int64_t oF3 = oF2 + 65536;        // W4
+constexpr int64_t oF4 = oF3 + 65536;        // W5 [16][256], rows >= 12 zero
+constexpr int64_t oT4 = oF4 + 16 * 256;     // W5^T [256][32], columns >= 12 zero
+constexpr int64_t oT3 = oT4 + 256 * 32;     // W4^T [256][256]
+constexpr int64_t oT2 = oT3 + 65536;        // W3^T
+constexpr int64_t oT1 = oT2 + 65536;        // W2^T
+constexpr int64_t oT0 = oT1 + 65536;        // W1^T [16][256], rows >= 13 zero
+constexpr int64_t kTrainBlob = oT0 + 16 * 256;
+// fp32 master blob: W1 [256][13] | W2..W4 [256][256] | W5 [12][256]
+constexpr int64_t mW1 = 0, mW2 = 256 * 13, mW3 = mW2 + 65536, mW4 = mW3 + 65536,
+                  mW5 = mW4 + 65536;
+
+// Training blob in fragment order: a matrix of R rows (multiple of 16) and K columns
+// (multiple of 32) is stored as 16-row blocks rb x 32-column k-steps ks x 64 lanes x 8
+// elements, element e of lane l being M[16 rb + l % 16][32 ks + 8 (l / 16) + e] -- the
+// A operand of v_mfma_f32_16x16x32_bf16 -- so that every fragment load of a wave is
+// one contiguous 1 KB read.
+__device__ __forceinline__ float master_value(const float* __restrict__ w, int seg, int r, int c) {
+  switch (seg) {
+    case 0: return c < FIN ? w[mW1 + r * FIN + c] : 0.f;            // W1 [256][32]
+    case 1: return w[mW2 + r * 256 + c];                             // W2
+    case 2: return w[mW3 + r * 256 + c];                             // W3
+    case 3: return w[mW4 + r * 256 + c];                             // W4
+    case 4: return r < FOUT ? w[mW5 + r * 256 + c] : 0.f;            // W5 [16][256]
+    case 5: return c < FOUT ? w[mW5 + c * 256 + r] : 0.f;            // W5^T [256][32]
+    case 6: return w[mW4 + c * 256 + r];                             // W4^T
+    case 7: return w[mW3 + c * 256 + r];                             // W3^T
+    case 8: return w[mW2 + c * 256 + r];                             // W2^T
+    default: return r < FIN ? w[mW1 + c * FIN + r] : 0.f;            // W1^T [16][256]
+  }
+}
+
+__global__ void k_bptt_prep(const float* __restrict__ w, __bf16* __restrict__ blob) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kTrainBlob) return;
+  const int64_t off[11] = {oF0, oF1, oF2, oF3, oF4, oT4, oT3, oT2, oT1, oT0, kTrainBlob};
+  const int cols[10] = {32, 256, 256, 256, 256, 32, 256, 256, 256, 256};
+  int seg = 0;
+  while (i >= off[seg + 1]) ++seg;
+  const int64_t j = i - off[seg];
+  const int e = (int)(j % 8), l = (int)((j / 8) % 64);
+  const int64_t f = j / 512;
+  const int KS = cols[seg] / 32;
+  const int ks = (int)(f % KS), rb = (int)(f / KS);
+  const int r = 16 * rb + (l & 15), c = 32 * ks + 8 * (l >> 4) + e;
+  blob[i] = (__bf16)master_value(w, seg, r, c);
+}
+
+struct BpttArgs {
+  int B, Bp, L, T, P;
+  int64_t M;                 // L Bp point-steps
+  const int32_t* idx;        // [B] path of each batch row
+  const float* feat;         // [P][T][8]: d_eps / eps_scale (6), rate feature, unused
+  const float* eps;          // [P][T][6]: strain / eps_scale
+  const float* target;       // [P][T][6]: stress / stress_scale
+  const float* pw;           // [P] path weights
+  float Cs[36];              // stiffness eps_scale / stress_scale, row-major
+  float cstep, inv_count;    // inelastic step; 1 / (B L 6)
+  const __bf16* blob;        // training weight blob
+  const float* bias;         // b1..b4 [256] | b5 [12]
+  __bf16* xT;                // [16][M]
+  float* q;                  // [M][6]
+  __bf16* aT;                // [4][256][M]: hidden activations h1..h4
+  __bf16* dzT;               // [4][256][M]: dL/dz1..dz4
+  __bf16* doT;               // [16][M]: dL/do
+  float* loss_part;          // [tiles]
+};
+
+// A-operand fragments of the 64 rows 64 wave .. 64 wave + 63 of a layer: 4 row
+// blocks j of 16 rows, KS k-steps of 32.
+template <int KS>
+struct Frags {
+  bf16x8 f[KS][4];
+};
+
+// The weight blob is read through a buffer resource: one per-lane byte offset for
+// every fragment of a layer, the layer, row-block and k-step offsets in scalar
+// registers (flat addressing needed a 64-bit address per fragment).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t blob_rsrc(const __bf16* blob) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)blob, (short)0, (int)(kTrainBlob * 2), 0x00020000);
+}
+
+__device__ __forceinline__ bf16x8 blob_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+// Fragments of the wave's 64 rows of the fragment-ordered matrix at blob element offset
+// base with ld columns: one lane offset for all of them, the rest in scalar registers.
+template <int KS>
+__device__ __forceinline__ void load_frags(Frags<KS>& w, __amdgpu_buffer_rsrc_t r, int64_t base,
+                                           int ld, int wave, int lane) {
+  const int kst = ld / 32;
+  // an opaque zero keeps the scalar offsets from being hoisted out of the step loop
+  // (192 loop-invariant offsets would not fit the scalar registers)
+  int z0;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+  const int b = z0 + (int)(base * 2) + 4 * wave * kst * 1024;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w.f[ks][j] = blob_load(r, lane * 16, b + (j * kst + ks) * 1024);
+}
+
+// acc[j][r] = (W act^T)[64 wave + 16 j + 4 (lane / 16) + r][lane % 16] over the 16
+// points whose LD-strided bf16 rows are `in`.
+template <int KS, int LD>
+__device__ __forceinline__ void tile_mm(const Frags<KS>& w, const __bf16* in, int lane,
+                                        f32x4 (&acc)[4]) {
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const bf16x8 af = *(const bf16x8*)&in[lr * LD + 32 * ks + lk];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.f[ks][j], af, acc[j], 0, 0, 0);
+  }
+}
+
+// 16-row layer [16][256] at blob offset base: k-steps 2 wave and 2 wave + 1; the
+// wave's partial sums go to red[wave][row][point].
+__device__ __forceinline__ void tile_mm16(const bf16x8 (&wf)[2], const __bf16* in, float* red,
+                                          int wave, int lane) {
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  f32x4 o = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const bf16x8 af = *(const bf16x8*)&in[lr * TLD + 32 * (2 * wave + s) + lk];
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[s], af, o, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) red[(wave * 16 + 4 * (lane >> 4) + r) * 16 + lr] = o[r];
+}
+
+__device__ __forceinline__ void load_frags16(bf16x8 (&wf)[2], __amdgpu_buffer_rsrc_t r,
+                                             int64_t base, int wave, int lane) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+    wf[s] = blob_load(r, lane * 16, (int)((base + (int64_t)(2 * wave + s) * 512) * 2));
+}
+
+__device__ __forceinline__ void gelu_and_grad(float x, float& h, float& d) {
+  const float v = GELU_2K * x * (1.0f + GELU_C * x * x);
+  const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-v));
+  h = x * sg;
+  d = sg + x * sg * (1.0f - sg) * GELU_2K * (1.0f + 3.0f * GELU_C * x * x);
+}
+
+// Forward epilogue: GELU(acc + b) -> bf16 rows of `out`.
+__device__ __forceinline__ void epi_fwd(const f32x4 (&acc)[4], const float* __restrict__ bias,
+                                        __bf16* out, int wave, int lane) {
+  const int lr = lane & 15;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n0 = 64 * wave + 16 * j + 4 * (lane >> 4);
+    bf16x4 o;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = (__bf16)gelu(acc[j][r] + bias[n0 + r]);
+    *(bf16x4*)&out[lr * TLD + n0] = o;
+  }
+}
+
+// Recompute epilogue: as epi_fwd; GELU'(z) goes to the lane's slots of gd (LDS).
+__device__ __forceinline__ void epi_rec(const f32x4 (&acc)[4], const float* __restrict__ bias,
+                                        __bf16* out, f32x4* gd, int wave, int lane) {
+  const int lr = lane & 15;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n0 = 64 * wave + 16 * j + 4 * (lane >> 4);
+    bf16x4 o;
+    f32x4 d;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float h, g;
+      gelu_and_grad(acc[j][r] + bias[n0 + r], h, g);
+      d[r] = g;
+      o[r] = (__bf16)h;
+    }
+    gd[(4 * wave + j) * 64 + lane] = d;
+    *(bf16x4*)&out[lr * TLD + n0] = o;
+  }
+}
+
+// Backward epilogue: dz = acc GELU'(z) -> bf16 rows of `out`.
+__device__ __forceinline__ void epi_bwd(const f32x4 (&acc)[4], const f32x4* gd, __bf16* out,
+                                        int wave, int lane) {
+  const int lr = lane & 15;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n0 = 64 * wave + 16 * j + 4 * (lane >> 4);
+    const f32x4 d = gd[(4 * wave + j) * 64 + lane];
+    bf16x4 o;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = (__bf16)(acc[j][r] * d[r]);
+    *(bf16x4*)&out[lr * TLD + n0] = o;
+  }
+}
+
+// LDS tile [point][feature] of the 16 points -> dst [256][M] at columns mrow ..
+// mrow + 15: each 16-byte store carries 8 points of one feature.
+__device__ __forceinline__ void copy_out_t(const __bf16* buf, __bf16* __restrict__ dst, int64_t M,
+                                           int64_t mrow, int tid) {
+#pragma unroll
+  for (int e = tid; e < 512; e += kTrainThreads) {
+    const int n = e >> 1, h = e & 1;
+    bf16x8 v;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = buf[(8 * h + k) * TLD + n];
+    *(bf16x8*)&dst[(int64_t)n * M + mrow + 8 * h] = v;
+  }
+}
+
+// Workgroup barrier that orders LDS only: __syncthreads() also fences global memory,
+// which makes every barrier wait for all of the wave's outstanding global loads (the
+// weight fragments prefetched for the next layer).  No thread of these kernels reads
+// global data another thread of the same launch writes.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Forward through the window: one workgroup (4 waves, one per SIMD, each owning 64
+// rows of every layer) per 16 batch rows.  Row b's state (latent, inelastic strain,
+// loss) lives in thread b of wave 0, which loads its next inputs a step ahead.  The
+// big layers' fragments use two register sets: X takes W2 at the step start and W4
+// after its product, Y takes W3 during the first layer.  Biases live in LDS, so no
+// epilogue waits on a global load issued after a fragment prefetch.
+__global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+    k_bptt_fwd(BpttArgs a) {
+  constexpr int TP = 16;
+  __shared__ __attribute__((aligned(16))) __bf16 act[2][TP * TLD];
+  __shared__ float red[4 * 16 * TP];
+  __shared__ float bias[4 * FH + 16];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar blob offsets
+  const int b0 = blockIdx.x * TP;
+  for (int i = tid; i < 4 * FH + FOUT; i += kTrainThreads) bias[i] = a.bias[i];
+  float z[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, ein[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float lacc = 0.f, wgt = 0.f;
+  int path = 0;
+  bool valid = tid < TP && b0 + tid < a.B;
+  if (valid) {
+    path = a.idx[b0 + tid];
+    valid = path >= 0 && path < a.P;  // rejected by the host wrapper; never read out of range
+    if (valid) wgt = a.pw[path];
+  }
+  const int64_t prow = (int64_t)path * a.T;
+  const __amdgpu_buffer_rsrc_t rs = blob_rsrc(a.blob);
+  Frags<1> w0;
+  load_frags<1>(w0, rs, oF0, 32, wave, lane);
+  bf16x8 w4[2];
+  load_frags16(w4, rs, oF4, wave, lane);
+  Frags<8> X, Y;
+  float f_nx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) f_nx[c] = a.feat[prow * 8 + c];
+  }
+  for (int t = 0; t < a.L; ++t) {
+    const int64_t mrow = (int64_t)t * a.Bp + b0;
+    float e_t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, g_t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    load_frags<8>(X, rs, oF1, 256, wave, lane);
+    if (tid < TP) {
+      float x[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) x[c] = 0.f;
+      if (valid) {
+#pragma unroll
+        for (int c = 0; c < 7; ++c) x[c] = f_nx[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) x[7 + c] = z[c];
+        // this step's strain and target, the next step's features: consumed after
+        // the layers
+        const float* e = a.eps + (prow + t) * 6;
+        const float* g = a.target + (prow + t) * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          e_t[c] = e[c];
+          g_t[c] = g[c];
+        }
+        if (t + 1 < a.L) {
+#pragma unroll
+          for (int c = 0; c < 7; ++c) f_nx[c] = a.feat[(prow + t + 1) * 8 + c];
+        }
+      }
+      __bf16* row = act[0] + tid * TLD;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const __bf16 h = (__bf16)x[c];
+        row[c] = h;
+        a.xT[(int64_t)c * a.M + mrow + tid] = h;
+      }
+#pragma unroll
+      for (int c = 16; c < 32; ++c) row[c] = (__bf16)0.0f;
+    }
+    lds_barrier();
+    f32x4 acc[4];
+    load_frags<8>(Y, rs, oF2, 256, wave, lane);
+    tile_mm<1, TLD>(w0, act[0], lane, acc);
+    epi_fwd(acc, bias, act[1], wave, lane);
+    lds_barrier();
+    tile_mm<8, TLD>(X, act[1], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);  // the refill reuses the registers just read
+    load_frags<8>(X, rs, oF3, 256, wave, lane);
+    epi_fwd(acc, bias + FH, act[0], wave, lane);
+    lds_barrier();
+    tile_mm<8, TLD>(Y, act[0], lane, acc);
+    epi_fwd(acc, bias + 2 * FH, act[1], wave, lane);
+    lds_barrier();
+    tile_mm<8, TLD>(X, act[1], lane, acc);
+    epi_fwd(acc, bias + 3 * FH, act[0], wave, lane);
+    lds_barrier();
+    tile_mm16(w4, act[0], red, wave, lane);
+    lds_barrier();
+    if (tid < TP) {
+      float o[12];
+#pragma unroll
+      for (int n = 0; n < 12; ++n) {
+        float sum = bias[4 * FH + n];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) sum += red[(w * 16 + n) * TP + tid];
+        o[n] = sum;
+      }
+      float qv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (valid) {
+        float err[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ein[c] += a.cstep * o[c];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+          float sum = 0.f;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) sum += a.Cs[6 * r + c] * (e_t[c] - ein[c]);
+          err[r] = sum - g_t[r];
+          lacc += wgt * err[r] * err[r];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) z[c] += o[6 + c];
+        const float sc = 2.0f * wgt * a.inv_count;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          float sum = 0.f;
+#pragma unroll
+          for (int r = 0; r < 6; ++r) sum += a.Cs[6 * r + c] * err[r];
+          qv[c] = sc * sum;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.q[(mrow + tid) * 6 + c] = qv[c];
+    }
+  }
+  // tile loss partial: threads 0 .. TP - 1 are lanes of wave 0
+  if (wave == 0) {
+    float v = lane < TP ? lacc : 0.f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) a.loss_part[blockIdx.x] = v * a.inv_count;
+  }
+}
+
+// Backward through the window, same tiles, reverse time.  The big layers' fragments
+// rotate through P and Q, each refilled one layer ahead (W2 in Q on entry and exit);
+// GELU' of the recomputed layers is kept in LDS; the next step's q_t and x_t are
+// loaded a step ahead.
+__global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+    k_bptt_bwd(BpttArgs a) {
+  constexpr int TP = 16;
+  __shared__ __attribute__((aligned(16))) __bf16 act[2][TP * TLD];
+  __shared__ __attribute__((aligned(16))) __bf16 dos[TP * DLD];
+  __shared__ float red[4 * 16 * TP];
+  __shared__ f32x4 gds[4 * 16 * 64];  // [layer][wave][j][lane]
+  __shared__ float bias[4 * FH];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar blob offsets
+  const int b0 = blockIdx.x * TP;
+  for (int i = tid; i < 4 * FH; i += kTrainThreads) bias[i] = a.bias[i];
+  const int64_t M = a.M;
+  float G[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gz[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const __amdgpu_buffer_rsrc_t rs = blob_rsrc(a.blob);
+  Frags<1> w0, t4;
+  load_frags<1>(w0, rs, oF0, 32, wave, lane);
+  load_frags<1>(t4, rs, oT4, 32, wave, lane);
+  bf16x8 t0[2];
+  load_frags16(t0, rs, oT0, wave, lane);
+  Frags<8> P, Q;
+  load_frags<8>(Q, rs, oF1, 256, wave, lane);
+  // x_t element of this thread: feature xc = tid / 16 (16 features), point xp
+  const int xc = tid >> 4, xp = tid & 15;
+  int64_t mrow = (int64_t)(a.L - 1) * a.Bp + b0;
+  float q_nx[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (tid < TP) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) q_nx[c] = a.q[(mrow + tid) * 6 + c];
+  }
+  __bf16 x_nx = a.xT[(int64_t)xc * M + mrow + xp];
+  for (int t = a.L - 1; t >= 0; --t) {
+    mrow = (int64_t)t * a.Bp + b0;
+    if (tid < TP) {
+      float d[16];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        G[c] -= q_nx[c];
+        d[c] = a.cstep * G[c];
+        d[6 + c] = gz[c];
+      }
+#pragma unroll
+      for (int c = 12; c < 16; ++c) d[c] = 0.f;
+      __bf16* row = dos + tid * DLD;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const __bf16 h = (__bf16)d[c];
+        row[c] = h;
+        a.doT[(int64_t)c * M + mrow + tid] = h;
+      }
+#pragma unroll
+      for (int c = 16; c < 32; ++c) row[c] = (__bf16)0.0f;
+      if (t > 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) q_nx[c] = a.q[(mrow - a.Bp + tid) * 6 + c];
+      }
+    }
+    act[0][xp * TLD + xc] = x_nx;
+    act[0][xp * TLD + 16 + xc] = (__bf16)0.0f;
+    if (t > 0) x_nx = a.xT[(int64_t)xc * M + mrow - a.Bp + xp];
+    lds_barrier();
+    f32x4 acc[4];
+    // recompute h1..h4; each tile also goes to HBM transposed once the barrier
+    // after its epilogue has passed (the next layer only reads it)
+    load_frags<8>(P, rs, oF2, 256, wave, lane);
+    tile_mm<1, TLD>(w0, act[0], lane, acc);
+    epi_rec(acc, bias, act[1], gds, wave, lane);
+    lds_barrier();
+    copy_out_t(act[1], a.aT, M, mrow, tid);
+    tile_mm<8, TLD>(Q, act[1], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);  // the refill reuses the registers just read
+    load_frags<8>(Q, rs, oF3, 256, wave, lane);
+    epi_rec(acc, bias + FH, act[0], gds + 1024, wave, lane);
+    lds_barrier();
+    copy_out_t(act[0], a.aT + 256 * M, M, mrow, tid);
+    tile_mm<8, TLD>(P, act[0], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    load_frags<8>(P, rs, oT3, 256, wave, lane);
+    epi_rec(acc, bias + 2 * FH, act[1], gds + 2048, wave, lane);
+    lds_barrier();
+    copy_out_t(act[1], a.aT + 512 * M, M, mrow, tid);
+    tile_mm<8, TLD>(Q, act[1], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    load_frags<8>(Q, rs, oT2, 256, wave, lane);
+    epi_rec(acc, bias + 3 * FH, act[0], gds + 3072, wave, lane);
+    lds_barrier();
+    copy_out_t(act[0], a.aT + 768 * M, M, mrow, tid);
+    // back-propagate: dL/dh4 = W5^T do, dz_k = dL/dh_k GELU'(z_k), dL/dh_{k-1} = W_k^T dz_k
+    tile_mm<1, DLD>(t4, dos, lane, acc);
+    epi_bwd(acc, gds + 3072, act[1], wave, lane);
+    lds_barrier();
+    copy_out_t(act[1], a.dzT + 768 * M, M, mrow, tid);
+    tile_mm<8, TLD>(P, act[1], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    load_frags<8>(P, rs, oT1, 256, wave, lane);
+    epi_bwd(acc, gds + 2048, act[0], wave, lane);
+    lds_barrier();
+    copy_out_t(act[0], a.dzT + 512 * M, M, mrow, tid);
+    tile_mm<8, TLD>(Q, act[0], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    load_frags<8>(Q, rs, oF1, 256, wave, lane);  // the next step's first big layer
+    epi_bwd(acc, gds + 1024, act[1], wave, lane);
+    lds_barrier();
+    copy_out_t(act[1], a.dzT + 256 * M, M, mrow, tid);
+    tile_mm<8, TLD>(P, act[1], lane, acc);
+    epi_bwd(acc, gds, act[0], wave, lane);
+    lds_barrier();
+    copy_out_t(act[0], a.dzT, M, mrow, tid);
+    // dL/dx = W1^T dz1; its latent part feeds gz
+    tile_mm16(t0, act[0], red, wave, lane);
+    lds_barrier();
+    if (tid < TP) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        float sum = 0.f;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) sum += red[(w * 16 + 7 + c) * TP + tid];
+        gz[c] += sum;
+      }
+    }
+  }
+}
+
+// part[s][n][k] = sum over point-steps m of split s of A[n][m] Bm[k][m] (k < K) and
+// part[s][n][K] = sum of A[n][m] (the bias gradient); A [N][M], Bm [K][M] bf16 with row
+// stride M (a multiple of 16).  4 waves in 2 x 2, each a (16 WN) x (16 WK) tile.
+template <int WN, int WK>
+__global__ void __launch_bounds__(256) k_wgrad_nt(int64_t M, int N, int K,
+                                                  const __bf16* __restrict__ A,
+                                                  const __bf16* __restrict__ Bm, int64_t chunk,
+                                                  float* __restrict__ part) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = (lane >> 4) * 8;
+  const int tilesN = (N + 32 * WN - 1) / (32 * WN);
+  const int tn = blockIdx.x % tilesN, tk = blockIdx.x / tilesN;
+  const int n0 = (2 * tn + (wave >> 1)) * 16 * WN, k0 = (2 * tk + (wave & 1)) * 16 * WK;
+  if (n0 >= N || k0 >= K) return;
+  const int64_t mb = (int64_t)blockIdx.y * chunk;
+  const int64_t me = mb + chunk < M ? mb + chunk : M;
+  f32x4 acc[WN][WK];
+#pragma unroll
+  for (int i = 0; i < WN; ++i)
+#pragma unroll
+    for (int j = 0; j < WK; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  float rs[WN];
+#pragma unroll
+  for (int i = 0; i < WN; ++i) rs[i] = 0.f;
+  const bool with_bias = k0 == 0;
+  bf16x8 zero8;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) zero8[e] = (__bf16)0.0f;
+  for (int64_t m0 = mb; m0 < me; m0 += 32) {
+    const bool ok = m0 + lk + 8 <= me;
+    bf16x8 af[WN], bfr[WK];
+#pragma unroll
+    for (int i = 0; i < WN; ++i)
+      af[i] = ok ? *(const bf16x8*)&A[(int64_t)(n0 + 16 * i + lr) * M + m0 + lk] : zero8;
+#pragma unroll
+    for (int j = 0; j < WK; ++j)
+      bfr[j] = ok ? *(const bf16x8*)&Bm[(int64_t)(k0 + 16 * j + lr) * M + m0 + lk] : zero8;
+#pragma unroll
+    for (int i = 0; i < WN; ++i)
+#pragma unroll
+      for (int j = 0; j < WK; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    if (with_bias) {
+#pragma unroll
+      for (int i = 0; i < WN; ++i)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) rs[i] += (float)af[i][e];
+    }
+  }
+  const int Kx = K + 1;
+  float* P = part + (int64_t)blockIdx.y * N * Kx;
+#pragma unroll
+  for (int i = 0; i < WN; ++i)
+#pragma unroll
+    for (int j = 0; j < WK; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        P[(int64_t)(n0 + 16 * i + 4 * (lane >> 4) + r) * Kx + k0 + 16 * j + lr] = acc[i][j][r];
+  if (with_bias) {
+#pragma unroll
+    for (int i = 0; i < WN; ++i) {
+      float v = rs[i];
+      v += __shfl_xor(v, 16);
+      v += __shfl_xor(v, 32);
+      if (lane < 16) P[(int64_t)(n0 + 16 * i + lane) * Kx + K] = v;
+    }
+  }
+}
+
+struct WgradLayer {
+  const float* part;  // [S][N][K + 1]
+  int S, N, K;        // padded partial dimensions
+  int Nr, Kr;         // real weight shape
+  int64_t w_off, b_off;
+};
+struct WgradLayers {
+  WgradLayer l[5];
+};
+
+// Gradient blobs from the split partials (fixed summation order); the loss from the
+// tile partials.
+__global__ void k_bptt_reduce(WgradLayers L, float* __restrict__ gw, float* __restrict__ gb,
+                              const float* __restrict__ loss_part, int tiles,
+                              float* __restrict__ loss) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    float s = 0.f;
+    for (int i = 0; i < tiles; ++i) s += loss_part[i];
+    *loss = s;
+  }
+#pragma unroll
+  for (int li = 0; li < 5; ++li) {
+    const WgradLayer& w = L.l[li];
+    const int64_t cnt = (int64_t)w.Nr * (w.Kr + 1);
+    if (e < cnt) {
+      const int n = (int)(e / (w.Kr + 1)), k = (int)(e % (w.Kr + 1));
+      const int kk = k < w.Kr ? k : w.K;
+      const int64_t per = (int64_t)w.N * (w.K + 1);
+      float s = 0.f;
+      for (int sp = 0; sp < w.S; ++sp) s += w.part[sp * per + (int64_t)n * (w.K + 1) + kk];
+      if (k < w.Kr)
+        gw[w.w_off + (int64_t)n * w.Kr + k] = s;
+      else
+        gb[w.b_off + n] = s;
+      return;
+    }
+    e -= cnt;
+  }
+}
+
 }  // namespace cia
 
 extern "C" {
@@ -746,6 +1389,129 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
                      next_anchor_strain, next_anchor_inelastic, next_latent, next_age,
                      next_fallback, (unsigned int*)trip_counter, inelastic, next_inelastic,
                      free_energy, dissipation);
+  CIA_HIP(hipGetLastError());
+  return CIA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Fused BPTT training step: workspace layout and entry points.
+namespace {
+
+struct BpttLayout {
+  int tp, Bp, tiles;
+  int64_t M, chunk, S;
+  size_t xT, q, aT, dzT, doT, blob, loss, part[5], total;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+BpttLayout bptt_layout(int64_t paths, int64_t window) {
+  BpttLayout L{};
+  L.tp = 16;
+  L.Bp = (int)((paths + L.tp - 1) / L.tp) * L.tp;
+  L.tiles = L.Bp / L.tp;
+  L.M = window * L.Bp;
+  L.chunk = std::max<int64_t>(256, ((L.M + 63) / 64 + 31) / 32 * 32);
+  L.S = (L.M + L.chunk - 1) / L.chunk;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += align256(bytes);
+    return at;
+  };
+  L.xT = take((size_t)16 * L.M * 2);
+  L.q = take((size_t)L.M * 6 * 4);
+  L.aT = take((size_t)4 * 256 * L.M * 2);
+  L.dzT = take((size_t)4 * 256 * L.M * 2);
+  L.doT = take((size_t)16 * L.M * 2);
+  L.blob = take((size_t)cia::kTrainBlob * 2);
+  L.loss = take((size_t)L.tiles * 4);
+  const int pn[5] = {256, 256, 256, 256, 16}, pk[5] = {16, 256, 256, 256, 256};
+  for (int l = 0; l < 5; ++l) L.part[l] = take((size_t)L.S * pn[l] * (pk[l] + 1) * 4);
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t cia_surrogate_bptt_workspace(int64_t paths, int64_t window) {
+  if (paths <= 0 || window <= 0) return 0;
+  return (int64_t)bptt_layout(paths, window).total;
+}
+
+cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length, int64_t paths,
+                              int64_t window, const int32_t* path_index, const float* features,
+                              const float* strain, const float* target, const float* path_weight,
+                              const float* stiffness, float inelastic_step, const float* weights,
+                              const float* biases, float* grad_weights, float* grad_biases,
+                              float* loss, void* workspace, int64_t workspace_bytes) {
+  using namespace cia;
+  CIA_ARG(path_index && features && strain && target && path_weight && stiffness && weights &&
+              biases && grad_weights && grad_biases && loss && workspace,
+          "invalid arguments to cia_surrogate_bptt");
+  CIA_ARG(paths > 0 && n_paths > 0 && window > 0 && window <= path_length && paths < (1 << 30) &&
+              n_paths < (1 << 30) && path_length < (1 << 30),
+          "cia_surrogate_bptt: need 0 < window <= path_length, paths > 0 and n_paths > 0");
+  const BpttLayout L = bptt_layout(paths, window);
+  CIA_ARG(workspace_bytes >= (int64_t)L.total,
+          "cia_surrogate_bptt: workspace smaller than cia_surrogate_bptt_workspace()");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  BpttArgs a{};
+  a.B = (int)paths;
+  a.Bp = L.Bp;
+  a.L = (int)window;
+  a.T = (int)path_length;
+  a.P = (int)n_paths;
+  a.M = L.M;
+  a.idx = path_index;
+  a.feat = features;
+  a.eps = strain;
+  a.target = target;
+  a.pw = path_weight;
+  for (int i = 0; i < 36; ++i) a.Cs[i] = stiffness[i];
+  a.cstep = inelastic_step;
+  a.inv_count = (float)(1.0 / ((double)paths * (double)window * 6.0));
+  a.blob = (const __bf16*)(ws + L.blob);
+  a.bias = biases;
+  a.xT = (__bf16*)(ws + L.xT);
+  a.q = (float*)(ws + L.q);
+  a.aT = (__bf16*)(ws + L.aT);
+  a.dzT = (__bf16*)(ws + L.dzT);
+  a.doT = (__bf16*)(ws + L.doT);
+  a.loss_part = (float*)(ws + L.loss);
+  hipLaunchKernelGGL(k_bptt_prep, dim3(grid_for(kTrainBlob)), dim3(kBlock), 0, st, weights,
+                     (__bf16*)(ws + L.blob));
+  hipLaunchKernelGGL(k_bptt_fwd, dim3(L.tiles), dim3(kTrainThreads), 0, st, a);
+  hipLaunchKernelGGL(k_bptt_bwd, dim3(L.tiles), dim3(kTrainThreads), 0, st, a);
+  CIA_HIP(hipGetLastError());
+  const int64_t M = L.M, ch = L.chunk;
+  float* part[5];
+  for (int l = 0; l < 5; ++l) part[l] = (float*)(ws + L.part[l]);
+  // layer 1: dz1 [256][M] x inputs [16][M]
+  hipLaunchKernelGGL((k_wgrad_nt<4, 1>), dim3(2, (unsigned)L.S), dim3(256), 0, st, M, 256, 16,
+                     a.dzT, a.xT, ch, part[0]);
+  for (int l = 1; l < 4; ++l)
+    hipLaunchKernelGGL((k_wgrad_nt<4, 4>), dim3(4, (unsigned)L.S), dim3(256), 0, st, M, 256, 256,
+                       a.dzT + (int64_t)l * 256 * M, a.aT + (int64_t)(l - 1) * 256 * M, ch,
+                       part[l]);
+  hipLaunchKernelGGL((k_wgrad_nt<1, 4>), dim3(2, (unsigned)L.S), dim3(256), 0, st, M, 16, 256,
+                     a.doT, a.aT + (int64_t)3 * 256 * M, ch, part[4]);
+  WgradLayers wl{};
+  const int Ns[5] = {256, 256, 256, 256, 16}, Ks[5] = {16, 256, 256, 256, 256};
+  const int Nr[5] = {256, 256, 256, 256, FOUT}, Kr[5] = {FIN, 256, 256, 256, 256};
+  const int64_t woff[5] = {mW1, mW2, mW3, mW4, mW5};
+  int64_t total = 0;
+  for (int l = 0; l < 5; ++l) {
+    wl.l[l] = WgradLayer{part[l], (int)L.S, Ns[l], Ks[l], Nr[l], Kr[l], woff[l], (int64_t)l * FH};
+    total += (int64_t)Nr[l] * (Kr[l] + 1);
+  }
+  hipLaunchKernelGGL(k_bptt_reduce, dim3(grid_for(total)), dim3(kBlock), 0, st, wl, grad_weights,
+                     grad_biases, (const float*)(ws + L.loss), L.tiles, loss);
   CIA_HIP(hipGetLastError());
   return CIA_OK;
 }

@@ -310,39 +310,101 @@ def rollout(params: SurrogateParams, scales: SurrogateScales, d_eps: torch.Tenso
     return (sig if normalized else sig * scales.stress_scale), torch.stack(lat, dim=1)
 
 
+class BpttData:
+    """A training set on the device in the layout of ``cia_surrogate_bptt``:
+    features [P][T][8] (d_eps / eps_scale, rate feature, 0), cumulative strain / eps_scale
+    [P][T][6] and target stress / stress_scale [P][T][6], fp32."""
+
+    def __init__(self, features, strain, target, stiffness_scaled):
+        dev = device()
+        self.feat = torch.as_tensor(np.asarray(features), dtype=torch.float32, device=dev).contiguous()
+        self.strain = torch.as_tensor(np.asarray(strain), dtype=torch.float32, device=dev).contiguous()
+        self.target = torch.as_tensor(np.asarray(target), dtype=torch.float32, device=dev).contiguous()
+        self.P, self.T = int(self.feat.shape[0]), int(self.feat.shape[1])
+        assert self.feat.shape == (self.P, self.T, 8)
+        assert self.strain.shape == self.target.shape == (self.P, self.T, 6)
+        cs = np.asarray(stiffness_scaled, dtype=np.float64).reshape(36)
+        self.cs = (C.c_float * 36)(*[float(v) for v in cs])
+
+    @classmethod
+    def from_paths(cls, scales: SurrogateScales, d_eps, rate, stress):
+        """Features, strains and targets of physical paths (d_eps [P][T][6], rate [P][T],
+        stress [P][T][6]) in the scaled units of ``scales``."""
+        d_eps = np.asarray(d_eps, dtype=np.float64)
+        P, T, _ = d_eps.shape
+        feat = np.zeros((P, T, 8), dtype=np.float32)
+        feat[..., :6] = d_eps / scales.eps_scale
+        feat[..., 6] = scales.rate_feature(np.asarray(rate, dtype=np.float64))
+        strain = np.cumsum(d_eps / scales.eps_scale, axis=1)
+        target = np.asarray(stress, dtype=np.float64) / scales.stress_scale
+        return cls(feat, strain, target,
+                   scales.stiffness_matrix() * (scales.eps_scale / scales.stress_scale))
+
+    def path_weights(self, window: int) -> torch.Tensor:
+        """1 / (peak |target| over the first ``window`` steps)^2 per path (floor 1e-3):
+        the path normalisation of the loss over the steps it is trained on."""
+        peak = self.target[:, :window].abs().amax(dim=(1, 2))
+        return (1.0 / torch.clamp(peak, min=1e-3) ** 2).contiguous()
+
+
+class BpttStep:
+    """Loss and gradients of one batch through ``cia_surrogate_bptt`` (workspace,
+    gradient blobs and the device loss held across calls)."""
+
+    def __init__(self, params: SurrogateParams, paths: int, window: int):
+        lib = _lib.lib()
+        dev = device()
+        self.paths, self.window = int(paths), int(window)
+        self.ws_bytes = int(lib.cia_surrogate_bptt_workspace(self.paths, self.window))
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.gw = torch.zeros_like(params.w)
+        self.gb = torch.zeros_like(params.b)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    def __call__(self, params: SurrogateParams, data: BpttData, idx: torch.Tensor,
+                 path_w: torch.Tensor):
+        assert idx.dtype == torch.int32 and idx.numel() == self.paths and idx.is_contiguous()
+        _check(_lib.lib().cia_surrogate_bptt(
+            stream_handle(), data.P, data.T, self.paths, self.window, _p(idx), _p(data.feat),
+            _p(data.strain), _p(data.target), _p(path_w), data.cs, INELASTIC_STEP, _p(params.w),
+            _p(params.b), _p(self.gw), _p(self.gb), _p(self.loss), _p(self.ws), self.ws_bytes),
+            "cia_surrogate_bptt")
+        return self.loss
+
+
 def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress, iterations: int,
           batch_paths: int, window: int, lr: float = 1e-3, seed: int = 0, lr_final: float = 1e-5,
-          log_every: int = 0, progress=None):
+          log_every: int = 0, progress=None, data: BpttData | None = None):
     """BPTT on random windows of the training paths (``train_rve_rno``,
     ``nn/training/rve_rno.py:200-415``).  The loss is the path-normalised stress MSE
     of ``_batch_loss`` (``:540-638``): each path's squared errors are divided by its
-    peak squared stress, so low-amplitude paths weigh as much as high ones.  Windows
-    start at step 0 so the latent starts from zero as in deployment.  Cosine
-    learning-rate decay from ``lr`` to ``lr_final``."""
-    dev = device()
-    de = torch.as_tensor(d_eps, dtype=torch.float32, device=dev)
-    rt = torch.as_tensor(rate, dtype=torch.float32, device=dev)
-    sg = torch.as_tensor(stress, dtype=torch.float32, device=dev) / scales.stress_scale
-    P, T, _ = de.shape
-    path_w = 1.0 / torch.clamp(sg.abs().amax(dim=(1, 2)), min=1e-3) ** 2   # [P]
+    peak squared stress over the trained window, so low-amplitude paths weigh as much
+    as high ones.  Windows start at step 0 so the latent starts from zero as in
+    deployment.  Cosine learning-rate decay from ``lr`` to ``lr_final``.  Every
+    iteration is one ``cia_surrogate_bptt`` call (forward and backward through time,
+    weight gradients) and one Adam step; batch indices are drawn on the host in
+    blocks of 64 iterations."""
+    if data is None:
+        data = BpttData.from_paths(scales, d_eps, rate, stress)
+    P, T = data.P, data.T
+    L = min(window, T)
+    B = min(batch_paths, P)
+    path_w = data.path_weights(L)
+    step = BpttStep(params, B, L)
     rng = np.random.default_rng(seed)
-    w_leaf = params.w.detach().requires_grad_(True)
-    b_leaf = params.b.detach().requires_grad_(True)
+    dev = device()
     history = []
+    block = None
     for it in range(iterations):
-        idx = torch.as_tensor(rng.choice(P, size=min(batch_paths, P), replace=P < batch_paths),
-                              device=dev)
-        L = min(window, T)
-        pred, _ = rollout(params, scales, de[idx, :L], rt[idx, :L], False, w_leaf, b_leaf,
-                          normalized=True)
-        loss = torch.mean(((pred - sg[idx, :L]) ** 2).mean(dim=(1, 2)) * path_w[idx])
-        w_leaf.grad = None
-        b_leaf.grad = None
-        loss.backward()
+        if it % 64 == 0:
+            n = min(64, iterations - it)
+            draws = np.stack([rng.choice(P, size=B, replace=P < batch_paths) for _ in range(n)])
+            block = torch.as_tensor(draws, dtype=torch.int32, device=dev)
+        loss = step(params, data, block[it % 64], path_w)
         cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / max(iterations - 1, 1)))
-        params.adam(w_leaf.grad, b_leaf.grad, cur)
+        params.adam(step.gw, step.gb, cur)
         if log_every and (it % log_every == 0 or it == iterations - 1):
-            history.append((it, float(loss.detach())))
+            history.append((it, float(loss)))
             if progress is not None:
                 print(f"surrogate training: iteration {it} loss {history[-1][1]:.3e}", file=progress,
                       flush=True)

@@ -572,6 +572,35 @@ cia_status cia_f32_to_bf16(void* stream, int64_t n, const float* x, void* y);
 cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const void* weights,
                                  const float* biases, float* Y);
 
+/* Fused BPTT training step of the surrogate (the batch step of train_rve_rno,
+ * src/concrete_impact/nn/training/rve_rno.py:200-415, and its path-normalised loss,
+ * :540-638, for this build's network): loss and gradients of a batch of `paths`
+ * rows over a window of `window` steps from step 0, in one call of six kernels.
+ * The training set holds n_paths paths of path_length steps; batch row b uses path
+ * p = path_index[b] (int32, 0 <= p < n_paths; a row with an index outside that range
+ * contributes nothing), which supplies at step t:
+ *   features [P][path_length][8] fp32: d_eps / eps_scale (6), rate feature, unused;
+ *   strain   [P][path_length][6] fp32: cumulative strain / eps_scale;
+ *   target   [P][path_length][6] fp32: stress / stress_scale;
+ *   path_weight [P] fp32.
+ * Recurrence (the rollout of the surrogate material): x_t = [features_t[0:7], z_t],
+ * o_t = MLP(x_t), ein_t = ein_{t-1} + inelastic_step o_t[0:6], z_{t+1} = z_t + o_t[6:12],
+ * sigma_t = Cs (strain_t - ein_t), Cs = stiffness (host, 36 floats row-major, the
+ * stiffness in scaled units); loss = sum_b w_p sum_{t,i} (sigma_t - target_t)_i^2 /
+ * (paths window 6).  weights / biases: the fp32 master blobs (layout of
+ * cia_mlp_forward_fused); grad_weights / grad_biases receive the gradients (same
+ * layouts, overwritten) and loss (device, 1 float) the loss.  Network inputs and
+ * activations are bf16, products accumulate in fp32 (as the layer-wise entry
+ * points).  workspace: at least cia_surrogate_bptt_workspace(paths, window) bytes of
+ * device memory. */
+int64_t cia_surrogate_bptt_workspace(int64_t paths, int64_t window);
+cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length, int64_t paths,
+                              int64_t window, const int32_t* path_index, const float* features,
+                              const float* strain, const float* target, const float* path_weight,
+                              const float* stiffness, float inelastic_step, const float* weights,
+                              const float* biases, float* grad_weights, float* grad_biases,
+                              float* loss, void* workspace, int64_t workspace_bytes);
+
 /* Surrogate material-point update: the material protocol of
  * RVERNOMaterialAdapter.update (src/concrete_impact/nn/deployment.py:97-165) for this
  * build's network, one kernel per call (features, the fused network, stress and

@@ -99,19 +99,23 @@ def rollout(params, d_eps, rate, C):
     return sig, np.stack(lat, axis=1), caches
 
 
-def loss_and_grad(params, d_eps, rate, target, C):
-    """Mean squared stress error of the rollout and its gradient by BPTT."""
+def loss_and_grad(params, d_eps, rate, target, C, path_weight=None):
+    """Mean squared stress error of the rollout and its gradient by BPTT.  With
+    ``path_weight`` (B,) each path's squared errors are multiplied by its weight
+    (loss = sum_b w_b sum_{t,i} diff^2 / (B T 6), the path-normalised loss of
+    ``nn/training/rve_rno.py:540-638`` when w_b = 1 / peak_b^2)."""
     B, T, _ = d_eps.shape
+    w_b = np.ones(B) if path_weight is None else np.asarray(path_weight, dtype=np.float64)
     sig, _, caches = rollout(params, d_eps, rate, C)
     n = B * T * 6
     diff = sig - target
-    loss = float(np.sum(diff * diff) / n)
+    loss = float(np.sum(w_b[:, None, None] * diff * diff) / n)
     grads = [(np.zeros_like(w), np.zeros_like(b)) for w, b in params]
     gz = np.zeros((B, 6))   # dL/dz_{t+1} flowing back
     ge = np.zeros((B, 6))   # dL/dein_t (ein_{t+1} = ein_t + c o[0:6] carries it back)
     for t in range(T - 1, -1, -1):
         go = np.zeros((B, 12))
-        ge = ge - (2.0 * diff[:, t] / n) @ C    # sigma_t = C (eps_t - ein_t)
+        ge = ge - (2.0 * w_b[:, None] * diff[:, t] / n) @ C    # sigma_t = C (eps_t - ein_t)
         go[:, :6] = INELASTIC_STEP * ge
         go[:, 6:] = gz                          # z_{t+1} = z_t + o[6:12]
         cache = caches[t]

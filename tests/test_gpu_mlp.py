@@ -187,6 +187,105 @@ def test_bptt_gradient_matches_oracle(gpu):
     del scales
 
 
+def _fused_bptt(params, d_eps, rate, target, Cs, idx, window, path_w):
+    """Loss and gradients of cia_surrogate_bptt with unit scales (the raw rate is the
+    rate feature, as in the oracle)."""
+    from concrete_impact_ai_amd import surrogate as S
+
+    P, T, _ = d_eps.shape
+    feat = np.zeros((P, T, 8))
+    feat[..., :6] = d_eps
+    feat[..., 6] = rate
+    data = S.BpttData(feat, np.cumsum(d_eps, axis=1), target, Cs)
+    step = S.BpttStep(params, len(idx), window)
+    loss = step(params, data, torch.as_tensor(np.asarray(idx), dtype=torch.int32, device="cuda"),
+                torch.as_tensor(path_w, dtype=torch.float32, device="cuda"))
+    torch.cuda.synchronize()
+    return float(loss), step.gw.double().cpu().numpy(), step.gb.double().cpu().numpy()
+
+
+def _spd(seed):
+    a = np.random.default_rng(seed).standard_normal((6, 6))
+    return a @ a.T / 6.0 + np.eye(6)
+
+
+def test_fused_bptt_matches_oracle_and_layerwise(gpu):
+    """cia_surrogate_bptt (fused forward / backward through time, weight-gradient
+    GEMMs) against the FP64 oracle's loss_and_grad at the bf16 tolerance of the
+    layer-wise test, and against the layer-wise kernels driven by autograd (same bf16
+    roundings) at 1e-2 of the gradient norm."""
+    from concrete_impact_ai_amd import surrogate as S
+
+    rng = np.random.default_rng(1)
+    layers = OM.init_params(5)
+    B, T = 64, 8
+    d_eps = rng.standard_normal((B, T, 6)) * 0.5
+    rate = rng.standard_normal((B, T)) * 0.3
+    target = rng.standard_normal((B, T, 6)) * 0.2
+    Cs = _spd(11)
+    loss_ref, g_ref = OM.loss_and_grad(layers, d_eps, rate, target, Cs)
+    params = S.SurrogateParams(layers)
+    loss, gw, gb = _fused_bptt(params, d_eps, rate, target, Cs, np.arange(B), T, np.ones(B))
+    rw = np.concatenate([g[0].ravel() for g in g_ref])
+    rb = np.concatenate([g[1].ravel() for g in g_ref])
+    assert abs(loss - loss_ref) <= 2e-2 * loss_ref
+    for got, ref in ((gw, rw), (gb, rb)):
+        cos = got @ ref / (np.linalg.norm(got) * np.linalg.norm(ref))
+        assert cos > 0.998
+        assert np.linalg.norm(got - ref) <= 5e-2 * np.linalg.norm(ref)
+    # layer-wise kernels + autograd on the same batch
+    w_leaf = params.w.detach().requires_grad_(True)
+    b_leaf = params.b.detach().requires_grad_(True)
+    de = _dev(d_eps, torch.float32)
+    rt = _dev(rate, torch.float32)
+    z = torch.zeros((B, 6), dtype=torch.float32, device="cuda")
+    ein = torch.zeros((B, 6), dtype=torch.float32, device="cuda")
+    eps = torch.cumsum(de, dim=1)
+    Ct = _dev(Cs, torch.float32)
+    tg = _dev(target, torch.float32)
+    lw = 0.0
+    for t in range(T):
+        o = S.mlp(torch.cat([de[:, t], rt[:, t, None], z], dim=1), params, w_leaf, b_leaf)
+        ein = ein + S.INELASTIC_STEP * o[:, :6]
+        lw = lw + (((eps[:, t] - ein) @ Ct.T - tg[:, t]) ** 2).sum()
+        z = z + o[:, 6:]
+    lw = lw / (B * T * 6)
+    lw.backward()
+    assert abs(loss - float(lw.detach())) <= 2e-3 * loss
+    lgw = w_leaf.grad.double().cpu().numpy()
+    lgb = b_leaf.grad.double().cpu().numpy()
+    assert np.linalg.norm(gw - lgw) <= 1e-2 * np.linalg.norm(lgw)
+    assert np.linalg.norm(gb - lgb) <= 1e-2 * np.linalg.norm(lgb)
+
+
+def test_fused_bptt_ragged_batch_window_and_path_weights(gpu):
+    """A batch of 37 rows (not a multiple of the 16-row tile) drawn with repeats from 20
+    paths of 9 steps, a window of 6 steps and unequal path weights: loss and gradients
+    equal the oracle's path-weighted loss_and_grad on the gathered windows."""
+    from concrete_impact_ai_amd import surrogate as S
+
+    rng = np.random.default_rng(4)
+    layers = OM.init_params(6)
+    P, T, L = 20, 9, 6
+    d_eps = rng.standard_normal((P, T, 6)) * 0.5
+    rate = rng.standard_normal((P, T)) * 0.3
+    target = rng.standard_normal((P, T, 6)) * 0.2
+    pw = rng.uniform(0.5, 2.0, P)
+    idx = rng.integers(0, P, 37)
+    Cs = _spd(12)
+    loss_ref, g_ref = OM.loss_and_grad(layers, d_eps[idx, :L], rate[idx, :L], target[idx, :L], Cs,
+                                       path_weight=pw[idx])
+    params = S.SurrogateParams(layers)
+    loss, gw, gb = _fused_bptt(params, d_eps, rate, target, Cs, idx, L, pw)
+    rw = np.concatenate([g[0].ravel() for g in g_ref])
+    rb = np.concatenate([g[1].ravel() for g in g_ref])
+    assert abs(loss - loss_ref) <= 2e-2 * loss_ref
+    for got, ref in ((gw, rw), (gb, rb)):
+        cos = got @ ref / (np.linalg.norm(got) * np.linalg.norm(ref))
+        assert cos > 0.998
+        assert np.linalg.norm(got - ref) <= 5e-2 * np.linalg.norm(ref)
+
+
 def test_training_reduces_rollout_loss(gpu):
     """Fit a viscoelastic-like recurrence sigma_t = 0.8 sigma_{t-1} + 1e10 d_eps_t
     (elastic stiffness 1e10 I: the inelastic strain takes up the relaxation)."""

@@ -56,6 +56,33 @@ def test_bptt_gradient_matches_central_differences():
         assert abs((lp - lm) / (2 * h) - grads[li][1][k]) <= 1e-7
 
 
+def test_path_weighted_loss_equals_repeated_paths_and_central_differences():
+    """Integer path weights equal repeating the paths: weights (2, 1, 3) on 3 paths give
+    twice the unweighted loss and gradients of the 6-path batch with each path repeated
+    its weight's times (the normaliser B T 6 halves); the weighted gradient also matches
+    central differences."""
+    params = OM.init_params(7, SMALL)
+    d_eps, rate, target = _data(8, B=3, T=4)
+    w = np.array([2.0, 1.0, 3.0])
+    loss_w, g_w = OM.loss_and_grad(params, d_eps, rate, target, C0, path_weight=w)
+    rep = np.repeat(np.arange(3), w.astype(int))
+    loss_r, g_r = OM.loss_and_grad(params, d_eps[rep], rate[rep], target[rep], C0)
+    assert abs(loss_w - 2.0 * loss_r) <= 1e-13 * loss_w
+    for (a, b), (c, d) in zip(g_w, g_r):
+        np.testing.assert_allclose(a, 2.0 * c, rtol=1e-11, atol=1e-15)
+        np.testing.assert_allclose(b, 2.0 * d, rtol=1e-11, atol=1e-15)
+    h = 1e-6
+    wt, _ = params[2]
+    for i, j in ((0, 0), (5, 11), (15, 3)):
+        old = wt[i, j]
+        wt[i, j] = old + h
+        lp, _ = OM.loss_and_grad(params, d_eps, rate, target, C0, path_weight=w)
+        wt[i, j] = old - h
+        lm, _ = OM.loss_and_grad(params, d_eps, rate, target, C0, path_weight=w)
+        wt[i, j] = old
+        assert abs((lp - lm) / (2 * h) - g_w[2][0][i, j]) <= 1e-7 * max(1.0, abs(g_w[2][0][i, j]))
+
+
 def test_bptt_gradient_matches_torch_autograd():
     params = OM.init_params(2, SMALL)
     d_eps, rate, target = _data(3, B=4, T=6)
