@@ -607,11 +607,11 @@ struct BpttArgs {
   float cstep, inv_count;    // inelastic step; 1 / (B L 6)
   const __bf16* blob;        // training weight blob
   const float* bias;         // b1..b4 [256] | b5 [12]
-  __bf16* xT;                // [16][M]
+  __bf16* xT;                // inputs x_t, 16 features, tiled transposed (tt_index)
   float* q;                  // [M][6]
-  __bf16* aT;                // [4][256][M]: hidden activations h1..h4
-  __bf16* dzT;               // [4][256][M]: dL/dz1..dz4
-  __bf16* doT;               // [16][M]: dL/do
+  __bf16* aT;                // 4 x 256 M: hidden activations h1..h4, tiled transposed
+  __bf16* dzT;               // 4 x 256 M: dL/dz1..dz4, tiled transposed
+  __bf16* doT;               // dL/do, 16 features, tiled transposed
   float* loss_part;          // [tiles]
 };
 
@@ -650,21 +650,41 @@ __device__ __forceinline__ void load_frags(Frags<KS>& w, __amdgpu_buffer_rsrc_t 
     for (int j = 0; j < 4; ++j) w.f[ks][j] = blob_load(r, lane * 16, b + (j * kst + ks) * 1024);
 }
 
+// Fragments loaded once per kernel are passed through an empty asm statement: the
+// compiler then keeps them in registers instead of re-issuing their loads at every
+// use inside the step loop (where the load latency would be exposed).
+template <int KS>
+__device__ __forceinline__ void pin(Frags<KS>& w) {
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) asm volatile("" : "+v"(w.f[ks][j]));
+}
+
+__device__ __forceinline__ void pin(bf16x8 (&w)[2]) {
+  asm volatile("" : "+v"(w[0]));
+  asm volatile("" : "+v"(w[1]));
+}
+
 // acc[j][r] = (W act^T)[64 wave + 16 j + 4 (lane / 16) + r][lane % 16] over the 16
 // points whose LD-strided bf16 rows are `in`.
 template <int KS, int LD>
 __device__ __forceinline__ void tile_mm(const Frags<KS>& w, const __bf16* in, int lane,
                                         f32x4 (&acc)[4]) {
   const int lr = lane & 15, lk = (lane >> 4) * 8;
+  // all B fragments are read before the products: with one wave per SIMD nothing
+  // else hides an LDS round trip per k-step
+  bf16x8 af[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) af[ks] = *(const bf16x8*)&in[lr * LD + 32 * ks + lk];
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int ks = 0; ks < KS; ++ks) {
-    const bf16x8 af = *(const bf16x8*)&in[lr * LD + 32 * ks + lk];
+  for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.f[ks][j], af, acc[j], 0, 0, 0);
-  }
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w.f[ks][j], af[ks], acc[j], 0, 0, 0);
 }
 
 // 16-row layer [16][256] at blob offset base: k-steps 2 wave and 2 wave + 1; the
@@ -746,18 +766,34 @@ __device__ __forceinline__ void epi_bwd(const f32x4 (&acc)[4], const f32x4* gd, 
   }
 }
 
-// LDS tile [point][feature] of the 16 points -> dst [256][M] at columns mrow ..
-// mrow + 15: each 16-byte store carries 8 points of one feature.
-__device__ __forceinline__ void copy_out_t(const __bf16* buf, __bf16* __restrict__ dst, int64_t M,
+// Point-step arrays of the weight gradients are "tiled transposed": element (feature
+// n, point-step m) of an array of NF features sits at ((m / 16) NF + n) 16 + m % 16,
+// so the 16 point-steps of one tile and step form one contiguous block of NF x 16
+// (written with coalesced stores) and 8 consecutive point-steps of one feature are one
+// 16-byte load (the MFMA operand of the weight-gradient GEMM).
+__device__ __forceinline__ int64_t tt_index(int64_t m, int n, int NF) {
+  return ((m >> 4) * NF + n) * 16 + (m & 15);
+}
+
+// LDS tile [point][feature] of the 16 points -> the step's block of dst (256 features):
+// thread t gathers feature pair 2 (t % 128), 2 (t % 128) + 1 for points 8 (t / 128) ..
+// + 7 with eight 4-byte LDS reads and writes two 16-byte chunks.
+__device__ __forceinline__ void copy_out_t(const __bf16* buf, __bf16* __restrict__ dst,
                                            int64_t mrow, int tid) {
+  const int np = tid & 127, h = tid >> 7;
+  const uint32_t* b32 = (const uint32_t*)buf;
+  uint32_t w[8];
 #pragma unroll
-  for (int e = tid; e < 512; e += kTrainThreads) {
-    const int n = e >> 1, h = e & 1;
-    bf16x8 v;
+  for (int k = 0; k < 8; ++k) w[k] = b32[(8 * h + k) * (TLD / 2) + np];
+  uint32_t lo[4], hi[4];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = buf[(8 * h + k) * TLD + n];
-    *(bf16x8*)&dst[(int64_t)n * M + mrow + 8 * h] = v;
+  for (int i = 0; i < 4; ++i) {
+    lo[i] = (w[2 * i] & 0xffffu) | (w[2 * i + 1] << 16);
+    hi[i] = (w[2 * i] >> 16) | (w[2 * i + 1] & 0xffff0000u);
   }
+  __bf16* blk = dst + (mrow >> 4) * (256 * 16);
+  *(uint4*)&blk[(2 * np) * 16 + 8 * h] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+  *(uint4*)&blk[(2 * np + 1) * 16 + 8 * h] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
 }
 
 // Workgroup barrier that orders LDS only: __syncthreads() also fences global memory,
@@ -770,22 +806,59 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// acc[j][r] as tile_mm, with the A fragments of an 8-k-step layer read from LDS in
+// fragment order (lds: [16-row block][k-step][lane][8]): conflict-free 16-byte reads.
+__device__ __forceinline__ void tile_mm_lds(const __bf16* wl, const __bf16* in, int wave,
+                                            int lane, f32x4 (&acc)[4]) {
+  const int lr = lane & 15, lk = (lane >> 4) * 8;
+  bf16x8 af[8];
+#pragma unroll
+  for (int ks = 0; ks < 8; ++ks) af[ks] = *(const bf16x8*)&in[lr * TLD + 32 * ks + lk];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  // A fragments two k-steps ahead of their products
+  bf16x8 wf[3][4];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wf[p][j] = *(const bf16x8*)&wl[(((4 * wave + j) * 8 + p) * 64 + lane) * 8];
+#pragma unroll
+  for (int ks = 0; ks < 8; ++ks) {
+    if (ks + 2 < 8) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        wf[(ks + 2) % 3][j] = *(const bf16x8*)&wl[(((4 * wave + j) * 8 + ks + 2) * 64 + lane) * 8];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[ks % 3][j], af[ks], acc[j], 0, 0, 0);
+  }
+}
+
+constexpr size_t kFwdLds = (size_t)65536 * 2 + 2 * 16 * TLD * 2 + 4 * 16 * 16 * 4 + (4 * FH + 16) * 4;
+
 // Forward through the window: one workgroup (4 waves, one per SIMD, each owning 64
 // rows of every layer) per 16 batch rows.  Row b's state (latent, inelastic strain,
-// loss) lives in thread b of wave 0, which loads its next inputs a step ahead.  The
-// big layers' fragments use two register sets: X takes W2 at the step start and W4
-// after its product, Y takes W3 during the first layer.  Biases live in LDS, so no
-// epilogue waits on a global load issued after a fragment prefetch.
+// loss) lives in thread b of wave 0, which loads its next inputs a step ahead.
+// Weights: W1 and W5 fragments stay in registers, W2 in registers too, W3 in LDS for
+// the whole window (fragment order), W4 streams from the L2-resident blob into a
+// register set refilled right after its product, a step ahead of its next use.
+// Biases live in LDS, so no epilogue waits on a global load.
 __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per_eu(1, 1)))
     k_bptt_fwd(BpttArgs a) {
   constexpr int TP = 16;
-  __shared__ __attribute__((aligned(16))) __bf16 act[2][TP * TLD];
-  __shared__ float red[4 * 16 * TP];
-  __shared__ float bias[4 * FH + 16];
+  extern __shared__ __attribute__((aligned(16))) __bf16 fwd_sm[];
+  __bf16* w3l = fwd_sm;                                  // W3 [65536], fragment order
+  __bf16 (*act)[TP * TLD] = (__bf16 (*)[TP * TLD])(fwd_sm + 65536);
+  float* red = (float*)(fwd_sm + 65536 + 2 * TP * TLD);  // [4][16][16]
+  float* bias = red + 4 * 16 * TP;                       // [4 FH + 16]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar blob offsets
   const int b0 = blockIdx.x * TP;
   for (int i = tid; i < 4 * FH + FOUT; i += kTrainThreads) bias[i] = a.bias[i];
+  for (int i = tid; i < 65536 / 8; i += kTrainThreads)
+    *(bf16x8*)&w3l[8 * i] = *(const bf16x8*)&a.blob[oF2 + 8 * i];
   float z[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, ein[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float lacc = 0.f, wgt = 0.f;
   int path = 0;
@@ -801,7 +874,12 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
   load_frags<1>(w0, rs, oF0, 32, wave, lane);
   bf16x8 w4[2];
   load_frags16(w4, rs, oF4, wave, lane);
-  Frags<8> X, Y;
+  Frags<8> W2r, W4s;
+  load_frags<8>(W2r, rs, oF1, 256, wave, lane);
+  load_frags<8>(W4s, rs, oF3, 256, wave, lane);
+  pin(w0);
+  pin(w4);
+  pin(W2r);
   float f_nx[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (valid) {
 #pragma unroll
@@ -810,7 +888,6 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
   for (int t = 0; t < a.L; ++t) {
     const int64_t mrow = (int64_t)t * a.Bp + b0;
     float e_t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, g_t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    load_frags<8>(X, rs, oF1, 256, wave, lane);
     if (tid < TP) {
       float x[16];
 #pragma unroll
@@ -839,26 +916,25 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
       for (int c = 0; c < 16; ++c) {
         const __bf16 h = (__bf16)x[c];
         row[c] = h;
-        a.xT[(int64_t)c * a.M + mrow + tid] = h;
+        a.xT[tt_index(mrow + tid, c, 16)] = h;
       }
 #pragma unroll
       for (int c = 16; c < 32; ++c) row[c] = (__bf16)0.0f;
     }
     lds_barrier();
     f32x4 acc[4];
-    load_frags<8>(Y, rs, oF2, 256, wave, lane);
     tile_mm<1, TLD>(w0, act[0], lane, acc);
     epi_fwd(acc, bias, act[1], wave, lane);
     lds_barrier();
-    tile_mm<8, TLD>(X, act[1], lane, acc);
-    __builtin_amdgcn_sched_barrier(0);  // the refill reuses the registers just read
-    load_frags<8>(X, rs, oF3, 256, wave, lane);
+    tile_mm<8, TLD>(W2r, act[1], lane, acc);
     epi_fwd(acc, bias + FH, act[0], wave, lane);
     lds_barrier();
-    tile_mm<8, TLD>(Y, act[0], lane, acc);
+    tile_mm_lds(w3l, act[0], wave, lane, acc);
     epi_fwd(acc, bias + 2 * FH, act[1], wave, lane);
     lds_barrier();
-    tile_mm<8, TLD>(X, act[1], lane, acc);
+    tile_mm<8, TLD>(W4s, act[1], lane, acc);
+    __builtin_amdgcn_sched_barrier(0);  // the refill reuses the registers just read
+    if (t + 1 < a.L) load_frags<8>(W4s, rs, oF3, 256, wave, lane);
     epi_fwd(acc, bias + 3 * FH, act[0], wave, lane);
     lds_barrier();
     tile_mm16(w4, act[0], red, wave, lane);
@@ -933,6 +1009,9 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
   load_frags<1>(t4, rs, oT4, 32, wave, lane);
   bf16x8 t0[2];
   load_frags16(t0, rs, oT0, wave, lane);
+  pin(w0);
+  pin(t4);
+  pin(t0);
   Frags<8> P, Q;
   load_frags<8>(Q, rs, oF1, 256, wave, lane);
   // x_t element of this thread: feature xc = tid / 16 (16 features), point xp
@@ -943,7 +1022,7 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
     for (int c = 0; c < 6; ++c) q_nx[c] = a.q[(mrow + tid) * 6 + c];
   }
-  __bf16 x_nx = a.xT[(int64_t)xc * M + mrow + xp];
+  __bf16 x_nx = a.xT[tt_index(mrow + xp, xc, 16)];
   for (int t = a.L - 1; t >= 0; --t) {
     mrow = (int64_t)t * a.Bp + b0;
     if (tid < TP) {
@@ -961,7 +1040,7 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
       for (int c = 0; c < 16; ++c) {
         const __bf16 h = (__bf16)d[c];
         row[c] = h;
-        a.doT[(int64_t)c * M + mrow + tid] = h;
+        a.doT[tt_index(mrow + tid, c, 16)] = h;
       }
 #pragma unroll
       for (int c = 16; c < 32; ++c) row[c] = (__bf16)0.0f;
@@ -972,55 +1051,55 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
     }
     act[0][xp * TLD + xc] = x_nx;
     act[0][xp * TLD + 16 + xc] = (__bf16)0.0f;
-    if (t > 0) x_nx = a.xT[(int64_t)xc * M + mrow - a.Bp + xp];
+    if (t > 0) x_nx = a.xT[tt_index(mrow - a.Bp + xp, xc, 16)];
     lds_barrier();
     f32x4 acc[4];
-    // recompute h1..h4; each tile also goes to HBM transposed once the barrier
-    // after its epilogue has passed (the next layer only reads it)
+    // recompute h1..h4; each tile also goes to HBM (tiled transposed) once the
+    // barrier after its epilogue has passed (the next layer only reads it)
     load_frags<8>(P, rs, oF2, 256, wave, lane);
     tile_mm<1, TLD>(w0, act[0], lane, acc);
     epi_rec(acc, bias, act[1], gds, wave, lane);
     lds_barrier();
-    copy_out_t(act[1], a.aT, M, mrow, tid);
+    copy_out_t(act[1], a.aT, mrow, tid);
     tile_mm<8, TLD>(Q, act[1], lane, acc);
     __builtin_amdgcn_sched_barrier(0);  // the refill reuses the registers just read
     load_frags<8>(Q, rs, oF3, 256, wave, lane);
     epi_rec(acc, bias + FH, act[0], gds + 1024, wave, lane);
     lds_barrier();
-    copy_out_t(act[0], a.aT + 256 * M, M, mrow, tid);
+    copy_out_t(act[0], a.aT + 256 * M, mrow, tid);
     tile_mm<8, TLD>(P, act[0], lane, acc);
     __builtin_amdgcn_sched_barrier(0);
     load_frags<8>(P, rs, oT3, 256, wave, lane);
     epi_rec(acc, bias + 2 * FH, act[1], gds + 2048, wave, lane);
     lds_barrier();
-    copy_out_t(act[1], a.aT + 512 * M, M, mrow, tid);
+    copy_out_t(act[1], a.aT + 512 * M, mrow, tid);
     tile_mm<8, TLD>(Q, act[1], lane, acc);
     __builtin_amdgcn_sched_barrier(0);
     load_frags<8>(Q, rs, oT2, 256, wave, lane);
     epi_rec(acc, bias + 3 * FH, act[0], gds + 3072, wave, lane);
     lds_barrier();
-    copy_out_t(act[0], a.aT + 768 * M, M, mrow, tid);
+    copy_out_t(act[0], a.aT + 768 * M, mrow, tid);
     // back-propagate: dL/dh4 = W5^T do, dz_k = dL/dh_k GELU'(z_k), dL/dh_{k-1} = W_k^T dz_k
     tile_mm<1, DLD>(t4, dos, lane, acc);
     epi_bwd(acc, gds + 3072, act[1], wave, lane);
     lds_barrier();
-    copy_out_t(act[1], a.dzT + 768 * M, M, mrow, tid);
+    copy_out_t(act[1], a.dzT + 768 * M, mrow, tid);
     tile_mm<8, TLD>(P, act[1], lane, acc);
     __builtin_amdgcn_sched_barrier(0);
     load_frags<8>(P, rs, oT1, 256, wave, lane);
     epi_bwd(acc, gds + 2048, act[0], wave, lane);
     lds_barrier();
-    copy_out_t(act[0], a.dzT + 512 * M, M, mrow, tid);
+    copy_out_t(act[0], a.dzT + 512 * M, mrow, tid);
     tile_mm<8, TLD>(Q, act[0], lane, acc);
     __builtin_amdgcn_sched_barrier(0);
     load_frags<8>(Q, rs, oF1, 256, wave, lane);  // the next step's first big layer
     epi_bwd(acc, gds + 1024, act[1], wave, lane);
     lds_barrier();
-    copy_out_t(act[1], a.dzT + 256 * M, M, mrow, tid);
+    copy_out_t(act[1], a.dzT + 256 * M, mrow, tid);
     tile_mm<8, TLD>(P, act[1], lane, acc);
     epi_bwd(acc, gds, act[0], wave, lane);
     lds_barrier();
-    copy_out_t(act[0], a.dzT, M, mrow, tid);
+    copy_out_t(act[0], a.dzT, mrow, tid);
     // dL/dx = W1^T dz1; its latent part feeds gz
     tile_mm16(t0, act[0], red, wave, lane);
     lds_barrier();
@@ -1036,18 +1115,20 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
   }
 }
 
-// part[s][n][k] = sum over point-steps m of split s of A[n][m] Bm[k][m] (k < K) and
-// part[s][n][K] = sum of A[n][m] (the bias gradient); A [N][M], Bm [K][M] bf16 with row
-// stride M (a multiple of 16).  4 waves in 2 x 2, each a (16 WN) x (16 WK) tile.
-template <int WN, int WK>
+// part[s][n][k] = sum over point-steps m of split s of A(n, m) Bm(k, m) (k < K) and
+// part[s][n][K] = sum of A(n, m) (the bias gradient); A (N features) and Bm (K
+// features) bf16 point-step arrays in the tiled transposed layout of tt_index (M a
+// multiple of 16).  4 waves in AN x AK, each a (16 WN) x (16 WK) tile.
+template <int WN, int WK, int AN, int AK>
 __global__ void __launch_bounds__(256) k_wgrad_nt(int64_t M, int N, int K,
                                                   const __bf16* __restrict__ A,
                                                   const __bf16* __restrict__ Bm, int64_t chunk,
                                                   float* __restrict__ part) {
+  static_assert(AN * AK == 4, "four waves per workgroup");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = (lane >> 4) * 8;
-  const int tilesN = (N + 32 * WN - 1) / (32 * WN);
+  const int tilesN = (N + 16 * AN * WN - 1) / (16 * AN * WN);
   const int tn = blockIdx.x % tilesN, tk = blockIdx.x / tilesN;
-  const int n0 = (2 * tn + (wave >> 1)) * 16 * WN, k0 = (2 * tk + (wave & 1)) * 16 * WK;
+  const int n0 = (AN * tn + wave / AK) * 16 * WN, k0 = (AK * tk + wave % AK) * 16 * WK;
   if (n0 >= N || k0 >= K) return;
   const int64_t mb = (int64_t)blockIdx.y * chunk;
   const int64_t me = mb + chunk < M ? mb + chunk : M;
@@ -1068,10 +1149,10 @@ __global__ void __launch_bounds__(256) k_wgrad_nt(int64_t M, int N, int K,
     bf16x8 af[WN], bfr[WK];
 #pragma unroll
     for (int i = 0; i < WN; ++i)
-      af[i] = ok ? *(const bf16x8*)&A[(int64_t)(n0 + 16 * i + lr) * M + m0 + lk] : zero8;
+      af[i] = ok ? *(const bf16x8*)&A[tt_index(m0 + lk, n0 + 16 * i + lr, N)] : zero8;
 #pragma unroll
     for (int j = 0; j < WK; ++j)
-      bfr[j] = ok ? *(const bf16x8*)&Bm[(int64_t)(k0 + 16 * j + lr) * M + m0 + lk] : zero8;
+      bfr[j] = ok ? *(const bf16x8*)&Bm[tt_index(m0 + lk, k0 + 16 * j + lr, K)] : zero8;
 #pragma unroll
     for (int i = 0; i < WN; ++i)
 #pragma unroll
@@ -1115,33 +1196,50 @@ struct WgradLayers {
 };
 
 // Gradient blobs from the split partials (fixed summation order); the loss from the
-// tile partials.
-__global__ void k_bptt_reduce(WgradLayers L, float* __restrict__ gw, float* __restrict__ gb,
-                              const float* __restrict__ loss_part, int tiles,
-                              float* __restrict__ loss) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e == 0) {
+// tile partials.  A workgroup reduces 16 consecutive outputs: thread (r, o) sums splits
+// r, r + 16, ... of output o (16 outputs read as one 64-byte segment per split), and
+// the 16 partial sums of each output are added in order through LDS.
+__global__ void __launch_bounds__(256) k_bptt_reduce(WgradLayers L, float* __restrict__ gw,
+                                                     float* __restrict__ gb,
+                                                     const float* __restrict__ loss_part,
+                                                     int tiles, float* __restrict__ loss,
+                                                     int64_t total) {
+  __shared__ float red[16][17];
+  const int ol = threadIdx.x & 15, r = threadIdx.x >> 4;
+  int64_t e = (int64_t)blockIdx.x * 16 + ol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     float s = 0.f;
     for (int i = 0; i < tiles; ++i) s += loss_part[i];
     *loss = s;
   }
-#pragma unroll
-  for (int li = 0; li < 5; ++li) {
-    const WgradLayer& w = L.l[li];
-    const int64_t cnt = (int64_t)w.Nr * (w.Kr + 1);
-    if (e < cnt) {
-      const int n = (int)(e / (w.Kr + 1)), k = (int)(e % (w.Kr + 1));
-      const int kk = k < w.Kr ? k : w.K;
-      const int64_t per = (int64_t)w.N * (w.K + 1);
-      float s = 0.f;
-      for (int sp = 0; sp < w.S; ++sp) s += w.part[sp * per + (int64_t)n * (w.K + 1) + kk];
-      if (k < w.Kr)
-        gw[w.w_off + (int64_t)n * w.Kr + k] = s;
-      else
-        gb[w.b_off + n] = s;
-      return;
+  int li = 0;
+  float acc = 0.f;
+  int n = 0, k = 0;
+  const bool valid = e < total;
+  if (valid) {
+    while (li < 4 && e >= (int64_t)L.l[li].Nr * (L.l[li].Kr + 1)) {
+      e -= (int64_t)L.l[li].Nr * (L.l[li].Kr + 1);
+      ++li;
     }
-    e -= cnt;
+    const WgradLayer& w = L.l[li];
+    n = (int)(e / (w.Kr + 1));
+    k = (int)(e % (w.Kr + 1));
+    const int kk = k < w.Kr ? k : w.K;
+    const int64_t per = (int64_t)w.N * (w.K + 1);
+    const float* src = w.part + (int64_t)n * (w.K + 1) + kk;
+    for (int sp = r; sp < w.S; sp += 16) acc += src[sp * per];
+  }
+  red[r][ol] = acc;
+  __syncthreads();
+  if (r == 0 && valid) {
+    float sum = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) sum += red[i][ol];
+    const WgradLayer& w = L.l[li];
+    if (k < w.Kr)
+      gw[w.w_off + (int64_t)n * w.Kr + k] = sum;
+    else
+      gb[w.b_off + n] = sum;
   }
 }
 
@@ -1401,7 +1499,7 @@ namespace {
 
 struct BpttLayout {
   int tp, Bp, tiles;
-  int64_t M, chunk, S;
+  int64_t M, chunk[5], S[5];
   size_t xT, q, aT, dzT, doT, blob, loss, part[5], total;
 };
 
@@ -1413,8 +1511,13 @@ BpttLayout bptt_layout(int64_t paths, int64_t window) {
   L.Bp = (int)((paths + L.tp - 1) / L.tp) * L.tp;
   L.tiles = L.Bp / L.tp;
   L.M = window * L.Bp;
-  L.chunk = std::max<int64_t>(256, ((L.M + 63) / 64 + 31) / 32 * 32);
-  L.S = (L.M + L.chunk - 1) / L.chunk;
+  // point-step splits of the weight gradients: 64 for the 256 x 256 layers (4
+  // workgroups each), 256 for the thin input and output layers (2 workgroups each)
+  const int64_t target[5] = {256, 64, 64, 64, 256};
+  for (int l = 0; l < 5; ++l) {
+    L.chunk[l] = std::max<int64_t>(256, ((L.M + target[l] - 1) / target[l] + 31) / 32 * 32);
+    L.S[l] = (L.M + L.chunk[l] - 1) / L.chunk[l];
+  }
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -1429,7 +1532,7 @@ BpttLayout bptt_layout(int64_t paths, int64_t window) {
   L.blob = take((size_t)cia::kTrainBlob * 2);
   L.loss = take((size_t)L.tiles * 4);
   const int pn[5] = {256, 256, 256, 256, 16}, pk[5] = {16, 256, 256, 256, 256};
-  for (int l = 0; l < 5; ++l) L.part[l] = take((size_t)L.S * pn[l] * (pk[l] + 1) * 4);
+  for (int l = 0; l < 5; ++l) L.part[l] = take((size_t)L.S[l] * pn[l] * (pk[l] + 1) * 4);
   L.total = o;
   return L;
 }
@@ -1457,6 +1560,8 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
               n_paths < (1 << 30) && path_length < (1 << 30),
           "cia_surrogate_bptt: need 0 < window <= path_length, paths > 0 and n_paths > 0");
   const BpttLayout L = bptt_layout(paths, window);
+  CIA_ARG(L.M <= ((int64_t)1 << 22),
+          "cia_surrogate_bptt: paths x window (rounded up to 16 paths) must not exceed 4194304");
   CIA_ARG(workspace_bytes >= (int64_t)L.total,
           "cia_surrogate_bptt: workspace smaller than cia_surrogate_bptt_workspace()");
   hipStream_t st = (hipStream_t)stream;
@@ -1486,32 +1591,39 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
   a.loss_part = (float*)(ws + L.loss);
   hipLaunchKernelGGL(k_bptt_prep, dim3(grid_for(kTrainBlob)), dim3(kBlock), 0, st, weights,
                      (__bf16*)(ws + L.blob));
-  hipLaunchKernelGGL(k_bptt_fwd, dim3(L.tiles), dim3(kTrainThreads), 0, st, a);
+  static bool fwd_attr = false;
+  if (!fwd_attr) {
+    CIA_HIP(hipFuncSetAttribute((const void*)k_bptt_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kFwdLds));
+    fwd_attr = true;
+  }
+  hipLaunchKernelGGL(k_bptt_fwd, dim3(L.tiles), dim3(kTrainThreads), kFwdLds, st, a);
   hipLaunchKernelGGL(k_bptt_bwd, dim3(L.tiles), dim3(kTrainThreads), 0, st, a);
   CIA_HIP(hipGetLastError());
-  const int64_t M = L.M, ch = L.chunk;
+  const int64_t M = L.M;
   float* part[5];
   for (int l = 0; l < 5; ++l) part[l] = (float*)(ws + L.part[l]);
-  // layer 1: dz1 [256][M] x inputs [16][M]
-  hipLaunchKernelGGL((k_wgrad_nt<4, 1>), dim3(2, (unsigned)L.S), dim3(256), 0, st, M, 256, 16,
-                     a.dzT, a.xT, ch, part[0]);
+  // layer 1: dz1 (256 features) x inputs (16): 4 x 1 waves of 64 x 16
+  hipLaunchKernelGGL((k_wgrad_nt<4, 1, 4, 1>), dim3(1, (unsigned)L.S[0]), dim3(256), 0, st, M,
+                     256, 16, a.dzT, a.xT, L.chunk[0], part[0]);
   for (int l = 1; l < 4; ++l)
-    hipLaunchKernelGGL((k_wgrad_nt<4, 4>), dim3(4, (unsigned)L.S), dim3(256), 0, st, M, 256, 256,
-                       a.dzT + (int64_t)l * 256 * M, a.aT + (int64_t)(l - 1) * 256 * M, ch,
-                       part[l]);
-  hipLaunchKernelGGL((k_wgrad_nt<1, 4>), dim3(2, (unsigned)L.S), dim3(256), 0, st, M, 16, 256,
-                     a.doT, a.aT + (int64_t)3 * 256 * M, ch, part[4]);
+    hipLaunchKernelGGL((k_wgrad_nt<4, 4, 2, 2>), dim3(4, (unsigned)L.S[l]), dim3(256), 0, st, M,
+                       256, 256, a.dzT + (int64_t)l * 256 * M,
+                       a.aT + (int64_t)(l - 1) * 256 * M, L.chunk[l], part[l]);
+  // output layer: dL/do (16) x h4 (256): 1 x 4 waves of 16 x 64
+  hipLaunchKernelGGL((k_wgrad_nt<1, 4, 1, 4>), dim3(1, (unsigned)L.S[4]), dim3(256), 0, st, M, 16,
+                     256, a.doT, a.aT + (int64_t)3 * 256 * M, L.chunk[4], part[4]);
   WgradLayers wl{};
   const int Ns[5] = {256, 256, 256, 256, 16}, Ks[5] = {16, 256, 256, 256, 256};
   const int Nr[5] = {256, 256, 256, 256, FOUT}, Kr[5] = {FIN, 256, 256, 256, 256};
   const int64_t woff[5] = {mW1, mW2, mW3, mW4, mW5};
   int64_t total = 0;
   for (int l = 0; l < 5; ++l) {
-    wl.l[l] = WgradLayer{part[l], (int)L.S, Ns[l], Ks[l], Nr[l], Kr[l], woff[l], (int64_t)l * FH};
+    wl.l[l] = WgradLayer{part[l], (int)L.S[l], Ns[l], Ks[l], Nr[l], Kr[l], woff[l], (int64_t)l * FH};
     total += (int64_t)Nr[l] * (Kr[l] + 1);
   }
-  hipLaunchKernelGGL(k_bptt_reduce, dim3(grid_for(total)), dim3(kBlock), 0, st, wl, grad_weights,
-                     grad_biases, (const float*)(ws + L.loss), L.tiles, loss);
+  hipLaunchKernelGGL(k_bptt_reduce, dim3((unsigned)((total + 15) / 16)), dim3(256), 0, st, wl,
+                     grad_weights, grad_biases, (const float*)(ws + L.loss), L.tiles, loss, total);
   CIA_HIP(hipGetLastError());
   return CIA_OK;
 }
