@@ -596,7 +596,7 @@ __global__ void k_bptt_prep(const float* __restrict__ w, __bf16* __restrict__ bl
 }
 
 struct BpttArgs {
-  int B, Bp, L, T, P;
+  int B, Bp, L, T, P, s0;    // ..., window start step
   int64_t M;                 // L Bp point-steps
   const int32_t* idx;        // [B] path of each batch row
   const float* feat;         // [P][T][8]: d_eps / eps_scale (6), rate feature, unused
@@ -613,6 +613,8 @@ struct BpttArgs {
   __bf16* dzT;               // 4 x 256 M: dL/dz1..dz4, tiled transposed
   __bf16* doT;               // dL/do, 16 features, tiled transposed
   float* loss_part;          // [tiles]
+  const float* state_in;     // [B][12] latent and inelastic strain at the window start, or null
+  float* state_out;          // [B][12] the same at the window end, or null
 };
 
 // A-operand fragments of the 64 rows 64 wave .. 64 wave + 63 of a layer: 4 row
@@ -868,7 +870,14 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
     valid = path >= 0 && path < a.P;  // rejected by the host wrapper; never read out of range
     if (valid) wgt = a.pw[path];
   }
-  const int64_t prow = (int64_t)path * a.T;
+  const int64_t prow = (int64_t)path * a.T + a.s0;  // the window's first step of the path
+  if (valid && a.state_in) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      z[c] = a.state_in[(int64_t)(b0 + tid) * 12 + c];
+      ein[c] = a.state_in[(int64_t)(b0 + tid) * 12 + 6 + c];
+    }
+  }
   const __amdgpu_buffer_rsrc_t rs = blob_rsrc(a.blob);
   Frags<1> w0;
   load_frags<1>(w0, rs, oF0, 32, wave, lane);
@@ -974,6 +983,13 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
       }
 #pragma unroll
       for (int c = 0; c < 6; ++c) a.q[(mrow + tid) * 6 + c] = qv[c];
+    }
+  }
+  if (tid < TP && b0 + tid < a.B && a.state_out) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      a.state_out[(int64_t)(b0 + tid) * 12 + c] = z[c];
+      a.state_out[(int64_t)(b0 + tid) * 12 + 6 + c] = ein[c];
     }
   }
   // tile loss partial: threads 0 .. TP - 1 are lanes of wave 0
@@ -1547,18 +1563,24 @@ int64_t cia_surrogate_bptt_workspace(int64_t paths, int64_t window) {
 }
 
 cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length, int64_t paths,
-                              int64_t window, const int32_t* path_index, const float* features,
-                              const float* strain, const float* target, const float* path_weight,
-                              const float* stiffness, float inelastic_step, const float* weights,
+                              int64_t window_start, int64_t window, const int32_t* path_index,
+                              const float* features, const float* strain, const float* target,
+                              const float* path_weight, const float* stiffness,
+                              float inelastic_step, const float* weights,
                               const float* biases, float* grad_weights, float* grad_biases,
-                              float* loss, void* workspace, int64_t workspace_bytes) {
+                              float* loss, const float* state_in, float* state_out,
+                              void* workspace, int64_t workspace_bytes) {
   using namespace cia;
   CIA_ARG(path_index && features && strain && target && path_weight && stiffness && weights &&
               biases && grad_weights && grad_biases && loss && workspace,
           "invalid arguments to cia_surrogate_bptt");
-  CIA_ARG(paths > 0 && n_paths > 0 && window > 0 && window <= path_length && paths < (1 << 30) &&
-              n_paths < (1 << 30) && path_length < (1 << 30),
-          "cia_surrogate_bptt: need 0 < window <= path_length, paths > 0 and n_paths > 0");
+  CIA_ARG(paths > 0 && n_paths > 0 && window > 0 && window_start >= 0 &&
+              window_start + window <= path_length && paths < (1 << 30) && n_paths < (1 << 30) &&
+              path_length < (1 << 30),
+          "cia_surrogate_bptt: need 0 <= window_start, 0 < window, window_start + window <= "
+          "path_length, paths > 0 and n_paths > 0");
+  CIA_ARG(!state_out || state_out != state_in,
+          "cia_surrogate_bptt: state_out must not alias state_in");
   const BpttLayout L = bptt_layout(paths, window);
   CIA_ARG(L.M <= ((int64_t)1 << 22),
           "cia_surrogate_bptt: paths x window (rounded up to 16 paths) must not exceed 4194304");
@@ -1572,6 +1594,9 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
   a.L = (int)window;
   a.T = (int)path_length;
   a.P = (int)n_paths;
+  a.s0 = (int)window_start;
+  a.state_in = state_in;
+  a.state_out = state_out;
   a.M = L.M;
   a.idx = path_index;
   a.feat = features;

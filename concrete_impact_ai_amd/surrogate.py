@@ -362,47 +362,73 @@ class BpttStep:
         self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
 
     def __call__(self, params: SurrogateParams, data: BpttData, idx: torch.Tensor,
-                 path_w: torch.Tensor):
+                 path_w: torch.Tensor, window_start: int = 0, state_in: torch.Tensor | None = None,
+                 state_out: torch.Tensor | None = None):
+        """Loss (device, 1 float) of the batch over steps window_start ..
+        window_start + window - 1; gradients in ``gw`` / ``gb``.  ``state_in`` /
+        ``state_out``: [paths][12] fp32 recurrence state (latent, inelastic strain) at
+        the window start / end (None = zeros / not wanted)."""
         assert idx.dtype == torch.int32 and idx.numel() == self.paths and idx.is_contiguous()
+        for st in (state_in, state_out):
+            assert st is None or (st.dtype == torch.float32 and st.shape == (self.paths, 12)
+                                  and st.is_contiguous())
         _check(_lib.lib().cia_surrogate_bptt(
-            stream_handle(), data.P, data.T, self.paths, self.window, _p(idx), _p(data.feat),
-            _p(data.strain), _p(data.target), _p(path_w), data.cs, INELASTIC_STEP, _p(params.w),
-            _p(params.b), _p(self.gw), _p(self.gb), _p(self.loss), _p(self.ws), self.ws_bytes),
-            "cia_surrogate_bptt")
+            stream_handle(), data.P, data.T, self.paths, int(window_start), self.window, _p(idx),
+            _p(data.feat), _p(data.strain), _p(data.target), _p(path_w), data.cs, INELASTIC_STEP,
+            _p(params.w), _p(params.b), _p(self.gw), _p(self.gb), _p(self.loss), _p(state_in),
+            _p(state_out), _p(self.ws), self.ws_bytes), "cia_surrogate_bptt")
         return self.loss
 
 
 def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress, iterations: int,
           batch_paths: int, window: int, lr: float = 1e-3, seed: int = 0, lr_final: float = 1e-5,
-          log_every: int = 0, progress=None, data: BpttData | None = None):
-    """BPTT on random windows of the training paths (``train_rve_rno``,
-    ``nn/training/rve_rno.py:200-415``).  The loss is the path-normalised stress MSE
-    of ``_batch_loss`` (``:540-638``): each path's squared errors are divided by its
-    peak squared stress over the trained window, so low-amplitude paths weigh as much
-    as high ones.  Windows start at step 0 so the latent starts from zero as in
-    deployment.  Cosine learning-rate decay from ``lr`` to ``lr_final``.  Every
+          log_every: int = 0, progress=None, data: BpttData | None = None, carry: bool = False,
+          first_window_iterations: int = 0):
+    """BPTT on the training paths (``train_rve_rno``, ``nn/training/rve_rno.py:200-415``).
+    The loss is the path-normalised stress MSE of ``_batch_loss`` (``:540-638``): each
+    path's squared errors are divided by its peak squared stress (over the trained
+    window, or over the whole path when ``carry``), so low-amplitude paths weigh as much
+    as high ones.  Cosine learning-rate decay from ``lr`` to ``lr_final``.  Every
     iteration is one ``cia_surrogate_bptt`` call (forward and backward through time,
-    weight gradients) and one Adam step; batch indices are drawn on the host in
-    blocks of 64 iterations."""
+    weight gradients) and one Adam step; batch indices are drawn on the host.
+
+    ``carry=False``: every iteration trains the first ``window`` steps of a new batch
+    from the zero state (as in deployment).  ``carry=True``: truncated BPTT over whole
+    paths -- a batch is trained window by window (steps 0 .. window - 1, window .. 2
+    window - 1, ...), the recurrence state carried from one window to the next and the
+    gradient truncated at window starts; the first ``first_window_iterations``
+    iterations train first windows only (an untrained recurrence drifts over later
+    windows)."""
     if data is None:
         data = BpttData.from_paths(scales, d_eps, rate, stress)
     P, T = data.P, data.T
     L = min(window, T)
     B = min(batch_paths, P)
-    path_w = data.path_weights(L)
+    path_w = data.path_weights(T if carry else L)
     step = BpttStep(params, B, L)
+    tail = T - (T // L) * L
+    step_tail = BpttStep(params, B, tail) if carry and tail else None
     rng = np.random.default_rng(seed)
     dev = device()
+    states = [torch.zeros((B, 12), dtype=torch.float32, device=dev) for _ in range(2)]
     history = []
-    block = None
+    idx = None
+    s0 = 0
     for it in range(iterations):
-        if it % 64 == 0:
-            n = min(64, iterations - it)
-            draws = np.stack([rng.choice(P, size=B, replace=P < batch_paths) for _ in range(n)])
-            block = torch.as_tensor(draws, dtype=torch.int32, device=dev)
-        loss = step(params, data, block[it % 64], path_w)
+        if not carry or it < first_window_iterations or s0 >= T:
+            s0 = 0
+        if s0 == 0:
+            draw = rng.choice(P, size=B, replace=P < batch_paths)
+            idx = torch.as_tensor(draw, dtype=torch.int32, device=dev)
+        st = step if s0 + L <= T else step_tail
+        if carry:
+            loss = st(params, data, idx, path_w, s0, None if s0 == 0 else states[0], states[1])
+            states.reverse()
+            s0 += st.window
+        else:
+            loss = st(params, data, idx, path_w)
         cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / max(iterations - 1, 1)))
-        params.adam(step.gw, step.gb, cur)
+        params.adam(st.gw, st.gb, cur)
         if log_every and (it % log_every == 0 or it == iterations - 1):
             history.append((it, float(loss)))
             if progress is not None:

@@ -575,7 +575,11 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
 /* Fused BPTT training step of the surrogate (the batch step of train_rve_rno,
  * src/concrete_impact/nn/training/rve_rno.py:200-415, and its path-normalised loss,
  * :540-638, for this build's network): loss and gradients of a batch of `paths`
- * rows over a window of `window` steps from step 0, in one call of six kernels.
+ * rows over the steps window_start .. window_start + window - 1, in one call of six
+ * kernels.  The recurrence starts from state_in [paths][12] (latent 6, inelastic
+ * strain 6 per row; NULL = zeros, the state at step 0) and, when state_out is not
+ * NULL, leaves the state after the window there, so that consecutive windows of a
+ * path are trained by truncated BPTT with the state carried between them.
  * The training set holds n_paths paths of path_length steps; batch row b uses path
  * p = path_index[b] (int32, 0 <= p < n_paths; a row with an index outside that range
  * contributes nothing), which supplies at step t:
@@ -583,7 +587,8 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
  *   strain   [P][path_length][6] fp32: cumulative strain / eps_scale;
  *   target   [P][path_length][6] fp32: stress / stress_scale;
  *   path_weight [P] fp32.
- * Recurrence (the rollout of the surrogate material): x_t = [features_t[0:7], z_t],
+ * Recurrence (the rollout of the surrogate material, t from window_start):
+ * x_t = [features_t[0:7], z_t],
  * o_t = MLP(x_t), ein_t = ein_{t-1} + inelastic_step o_t[0:6], z_{t+1} = z_t + o_t[6:12],
  * sigma_t = Cs (strain_t - ein_t), Cs = stiffness (host, 36 floats row-major, the
  * stiffness in scaled units); loss = sum_b w_p sum_{t,i} (sigma_t - target_t)_i^2 /
@@ -595,11 +600,13 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
  * device memory. */
 int64_t cia_surrogate_bptt_workspace(int64_t paths, int64_t window);
 cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length, int64_t paths,
-                              int64_t window, const int32_t* path_index, const float* features,
-                              const float* strain, const float* target, const float* path_weight,
-                              const float* stiffness, float inelastic_step, const float* weights,
+                              int64_t window_start, int64_t window, const int32_t* path_index,
+                              const float* features, const float* strain, const float* target,
+                              const float* path_weight, const float* stiffness,
+                              float inelastic_step, const float* weights,
                               const float* biases, float* grad_weights, float* grad_biases,
-                              float* loss, void* workspace, int64_t workspace_bytes);
+                              float* loss, const float* state_in, float* state_out,
+                              void* workspace, int64_t workspace_bytes);
 
 /* Surrogate material-point update: the material protocol of
  * RVERNOMaterialAdapter.update (src/concrete_impact/nn/deployment.py:97-165) for this

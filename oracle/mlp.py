@@ -77,13 +77,15 @@ def mlp_forward(params, x):
     return a, cache
 
 
-def rollout(params, d_eps, rate, C):
+def rollout(params, d_eps, rate, C, state=None):
     """d_eps (B, T, 6), rate (B, T), stiffness C (6, 6), all scaled -> stresses
-    (B, T, 6), latents (B, T + 1, 6), caches."""
+    (B, T, 6), latents (B, T + 1, 6), caches.  ``state``: (z, ein, eps) at the start,
+    each (B, 6) (default zeros: the start of a path)."""
     B, T, _ = d_eps.shape
-    z = np.zeros((B, 6))
-    eps = np.zeros((B, 6))
-    ein = np.zeros((B, 6))
+    if state is None:
+        z, ein, eps = np.zeros((B, 6)), np.zeros((B, 6)), np.zeros((B, 6))
+    else:
+        z, ein, eps = (np.array(v, dtype=np.float64) for v in state)
     sig = np.zeros((B, T, 6))
     lat = [z]
     caches = []
@@ -99,14 +101,25 @@ def rollout(params, d_eps, rate, C):
     return sig, np.stack(lat, axis=1), caches
 
 
-def loss_and_grad(params, d_eps, rate, target, C, path_weight=None):
+def final_state(params, d_eps, rate, C, state=None):
+    """(z, ein, eps) after rolling the recurrence over d_eps / rate from ``state``."""
+    B, T, _ = d_eps.shape
+    sig, lat, _ = rollout(params, d_eps, rate, C, state)
+    eps = (np.zeros((B, 6)) if state is None else np.asarray(state[2])) + d_eps.sum(axis=1)
+    ein = eps - sig[:, -1] @ np.linalg.inv(C).T        # sigma_T = C (eps_T - ein_T)
+    return lat[:, -1], ein, eps
+
+
+def loss_and_grad(params, d_eps, rate, target, C, path_weight=None, state=None):
     """Mean squared stress error of the rollout and its gradient by BPTT.  With
     ``path_weight`` (B,) each path's squared errors are multiplied by its weight
     (loss = sum_b w_b sum_{t,i} diff^2 / (B T 6), the path-normalised loss of
-    ``nn/training/rve_rno.py:540-638`` when w_b = 1 / peak_b^2)."""
+    ``nn/training/rve_rno.py:540-638`` when w_b = 1 / peak_b^2).  ``state`` as for
+    ``rollout``: the gradient of a window that starts from a carried state, truncated at
+    the window start (truncated BPTT)."""
     B, T, _ = d_eps.shape
     w_b = np.ones(B) if path_weight is None else np.asarray(path_weight, dtype=np.float64)
-    sig, _, caches = rollout(params, d_eps, rate, C)
+    sig, _, caches = rollout(params, d_eps, rate, C, state)
     n = B * T * 6
     diff = sig - target
     loss = float(np.sum(w_b[:, None, None] * diff * diff) / n)

@@ -83,6 +83,32 @@ def test_path_weighted_loss_equals_repeated_paths_and_central_differences():
         assert abs((lp - lm) / (2 * h) - g_w[2][0][i, j]) <= 1e-7 * max(1.0, abs(g_w[2][0][i, j]))
 
 
+def test_carried_window_continues_the_rollout_and_its_truncated_gradient():
+    """Rolling [0, s0) and then [s0, T) from final_state reproduces the full rollout's
+    stresses and latents; the window's loss gradient from a fixed carried state (the
+    truncated-BPTT gradient) matches central differences with that state held fixed."""
+    params = OM.init_params(9, SMALL)
+    d_eps, rate, target = _data(10, B=3, T=7)
+    s0 = 3
+    full, lat_full, _ = OM.rollout(params, d_eps, rate, C0)
+    st = OM.final_state(params, d_eps[:, :s0], rate[:, :s0], C0)
+    np.testing.assert_allclose(st[0], lat_full[:, s0], rtol=1e-12, atol=1e-14)
+    tail, lat_tail, _ = OM.rollout(params, d_eps[:, s0:], rate[:, s0:], C0, st)
+    np.testing.assert_allclose(tail, full[:, s0:], rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(lat_tail, lat_full[:, s0:], rtol=1e-12, atol=1e-14)
+    _, g = OM.loss_and_grad(params, d_eps[:, s0:], rate[:, s0:], target[:, s0:], C0, state=st)
+    h = 1e-6
+    w1, _ = params[1]
+    for i, j in ((0, 0), (7, 3), (15, 15)):
+        old = w1[i, j]
+        w1[i, j] = old + h
+        lp, _ = OM.loss_and_grad(params, d_eps[:, s0:], rate[:, s0:], target[:, s0:], C0, state=st)
+        w1[i, j] = old - h
+        lm, _ = OM.loss_and_grad(params, d_eps[:, s0:], rate[:, s0:], target[:, s0:], C0, state=st)
+        w1[i, j] = old
+        assert abs((lp - lm) / (2 * h) - g[1][0][i, j]) <= 1e-7 * max(1.0, abs(g[1][0][i, j]))
+
+
 def test_bptt_gradient_matches_torch_autograd():
     params = OM.init_params(2, SMALL)
     d_eps, rate, target = _data(3, B=4, T=6)
