@@ -357,8 +357,8 @@ _SIGNATURES.update({
     "cia_surrogate_bptt_workspace": (C.c_int64, [C.c_int64, C.c_int64]),
     "cia_surrogate_bptt": (
         C.c_int32, [_vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64, _vp, _vp, _vp,
-                    _vp, _vp, C.POINTER(C.c_float), C.c_float, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                    _vp, C.c_int64]),
+                    _vp, _vp, C.POINTER(C.c_float), C.c_float, _vp, _vp, _vp, _vp, _vp, _vp,
+                    C.c_float, _vp, _vp, _vp, C.c_int64]),
     "cia_surrogate_update": (
         C.c_int32, [_vp, C.c_int64, _vp, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -6,7 +6,7 @@
 // network (EnergyDissipationRVERNO, src/concrete_impact/nn/models/rve_rno.py:162-476,
 // trained by BPTT in src/concrete_impact/nn/training/rve_rno.py:113-415) with the
 // architecture 13 -> 256 -> 256 -> 256 -> 256 -> 12 (GELU): input = strain
-// increment (6), log strain-rate feature (1), latent state (6); output = stress (6),
+// increment (6), log strain-rate feature (1), tanh of the latent state (6); output = stress (6),
 // next latent state (6).
 //
 // Matrix cores: v_mfma_f32_16x16x32_bf16.  Operand layout per 64-lane wave:
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(256, 2) k_mlp_fused(int64_t M, const float* __
 // its last committed network step.  Every call evaluates the network once from the
 // anchor,
 //   d = eps - eps_a,  tau = age + dt,  rate = |d|_2 / tau,
-//   x = [d / eps_scale, clamp_lo(log10(rate / rate_ref) / rate_scale), z_a],
+//   x = [d / eps_scale, clamp_lo(log10(rate / rate_ref) / rate_scale), tanh(z_a)],
 //   ein' = ein_a + inelastic_step eps_scale o[0:6],  sigma = C (eps - ein'),
 //   z' = z_a + o[6:12],
 // and commits (anchor <- eps, ein', z', age 0) only when |d|_inf reaches
@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(256, 2)
       for (int c = 0; c < 6; ++c) row[c] = (__bf16)(float)(p.d[c] / g.eps_scale);
       row[6] = (__bf16)(float)p.feat;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) row[7 + c] = (__bf16)a_lat[c * n + m];
+      for (int c = 0; c < 6; ++c) row[7 + c] = (__bf16)tanhf(a_lat[c * n + m]);
     } else {
 #pragma unroll
       for (int c = 0; c < FIN; ++c) row[c] = (__bf16)0.0f;
@@ -527,11 +527,11 @@ __global__ void __launch_bounds__(256, 2)
 // Point-step index m = t Bp + b (Bp = B rounded up to the tile).  Padding paths
 // have zero inputs and zero loss weight, so they contribute nothing.
 //
-// Recurrence (scaled units): x_t = [f_t (6), r_t, z_t (6)], o_t = MLP(x_t),
+// Recurrence (scaled units): x_t = [f_t (6), r_t, tanh(z_t) (6)], o_t = MLP(x_t),
 // ein_t = ein_{t-1} + c o_t[0:6], sigma_t = Cs (eps_t - ein_t), z_{t+1} = z_t + o_t[6:12],
 // loss = sum_b w_b sum_{t,i} (sigma_t - target_t)_i^2 / (B L 6).
 // Adjoints in reverse time: G_t = G_{t+1} - q_t (dL/dein_t), do_t = [c G_t, gz_{t+1}],
-// gz_t = gz_{t+1} + dL/dx_t[7:13] (dL/dz_t).
+// gz_t = gz_{t+1} + dL/dx_t[7:13] (1 - tanh^2 z_t) (dL/dz_t).
 //
 // Every layer is the transposed product out^T = W act^T on the matrix cores: the A
 // operand is 16 weight rows (16-byte loads from the L2-resident blob, issued one
@@ -608,7 +608,7 @@ struct BpttArgs {
   const __bf16* blob;        // training weight blob
   const float* bias;         // b1..b4 [256] | b5 [12]
   __bf16* xT;                // inputs x_t, 16 features, tiled transposed (tt_index)
-  float* q;                  // [M][6]
+  float* q;                  // [M][12]: Cs^T dL/dsigma_t (6), tanh(z_t) (6)
   __bf16* aT;                // 4 x 256 M: hidden activations h1..h4, tiled transposed
   __bf16* dzT;               // 4 x 256 M: dL/dz1..dz4, tiled transposed
   __bf16* doT;               // dL/do, 16 features, tiled transposed
@@ -905,7 +905,10 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int c = 0; c < 7; ++c) x[c] = f_nx[c];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) x[7 + c] = z[c];
+        for (int c = 0; c < 6; ++c) {
+          x[7 + c] = tanhf(z[c]);  // the network sees the latent through tanh
+          a.q[(mrow + tid) * 12 + 6 + c] = x[7 + c];
+        }
         // this step's strain and target, the next step's features: consumed after
         // the layers
         const float* e = a.eps + (prow + t) * 6;
@@ -982,7 +985,7 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
         }
       }
 #pragma unroll
-      for (int c = 0; c < 6; ++c) a.q[(mrow + tid) * 6 + c] = qv[c];
+      for (int c = 0; c < 6; ++c) a.q[(mrow + tid) * 12 + c] = qv[c];
     }
   }
   if (tid < TP && b0 + tid < a.B && a.state_out) {
@@ -1033,18 +1036,20 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
   // x_t element of this thread: feature xc = tid / 16 (16 features), point xp
   const int xc = tid >> 4, xp = tid & 15;
   int64_t mrow = (int64_t)(a.L - 1) * a.Bp + b0;
-  float q_nx[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float q_nx[12] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   if (tid < TP) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) q_nx[c] = a.q[(mrow + tid) * 6 + c];
+    for (int c = 0; c < 12; ++c) q_nx[c] = a.q[(mrow + tid) * 12 + c];
   }
   __bf16 x_nx = a.xT[tt_index(mrow + xp, xc, 16)];
+  float th[6];  // tanh(z_t), the latent input of step t
   for (int t = a.L - 1; t >= 0; --t) {
     mrow = (int64_t)t * a.Bp + b0;
     if (tid < TP) {
       float d[16];
 #pragma unroll
       for (int c = 0; c < 6; ++c) {
+        th[c] = q_nx[6 + c];
         G[c] -= q_nx[c];
         d[c] = a.cstep * G[c];
         d[6 + c] = gz[c];
@@ -1062,7 +1067,7 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
       for (int c = 16; c < 32; ++c) row[c] = (__bf16)0.0f;
       if (t > 0) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) q_nx[c] = a.q[(mrow - a.Bp + tid) * 6 + c];
+        for (int c = 0; c < 12; ++c) q_nx[c] = a.q[(mrow - a.Bp + tid) * 12 + c];
       }
     }
     act[0][xp * TLD + xc] = x_nx;
@@ -1125,7 +1130,7 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
         float sum = 0.f;
 #pragma unroll
         for (int w = 0; w < 4; ++w) sum += red[(w * 16 + 7 + c) * TP + tid];
-        gz[c] += sum;
+        gz[c] += sum * (1.0f - th[c] * th[c]);  // through the tanh of the latent input
       }
     }
   }
@@ -1219,7 +1224,7 @@ __global__ void __launch_bounds__(256) k_bptt_reduce(WgradLayers L, float* __res
                                                      float* __restrict__ gb,
                                                      const float* __restrict__ loss_part,
                                                      int tiles, float* __restrict__ loss,
-                                                     int64_t total) {
+                                                     int64_t total, float* __restrict__ sq_part) {
   __shared__ float red[16][17];
   const int ol = threadIdx.x & 15, r = threadIdx.x >> 4;
   int64_t e = (int64_t)blockIdx.x * 16 + ol;
@@ -1247,6 +1252,7 @@ __global__ void __launch_bounds__(256) k_bptt_reduce(WgradLayers L, float* __res
   }
   red[r][ol] = acc;
   __syncthreads();
+  float sq = 0.f;
   if (r == 0 && valid) {
     float sum = 0.f;
 #pragma unroll
@@ -1256,7 +1262,52 @@ __global__ void __launch_bounds__(256) k_bptt_reduce(WgradLayers L, float* __res
       gw[w.w_off + (int64_t)n * w.Kr + k] = sum;
     else
       gb[w.b_off + n] = sum;
+    sq = sum * sum;
   }
+  // the block's sum of squares (the 16 outputs sit in lanes 0 .. 15 of wave 0)
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) sq += __shfl_xor(sq, off);
+    if (threadIdx.x == 0) sq_part[blockIdx.x] = sq;
+  }
+}
+
+// Global gradient norm from the block sums of squares (fixed order) and the clip scale:
+// min(1, max_norm / norm) when max_norm > 0, 0 when the norm is not finite (the step
+// is skipped), 1 otherwise.  norm_scale[0] = norm, norm_scale[1] = scale.
+__global__ void __launch_bounds__(256) k_grad_norm(const float* __restrict__ sq_part, int n,
+                                                   float max_norm, float* __restrict__ norm_scale,
+                                                   float* __restrict__ norm_out) {
+  __shared__ float red[256];
+  float v = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) v += sq_part[i];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float norm = sqrtf(red[0]);
+    float scale = 1.f;
+    if (!isfinite(norm))
+      scale = 0.f;
+    else if (max_norm > 0.f && norm > max_norm)
+      scale = max_norm / norm;
+    norm_scale[0] = norm;
+    norm_scale[1] = scale;
+    if (norm_out) *norm_out = norm;
+  }
+}
+
+__global__ void k_grad_scale(int64_t nw, float* __restrict__ gw, int64_t nb,
+                             float* __restrict__ gb, const float* __restrict__ norm_scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const float s = norm_scale[1];
+  if (s == 1.f) return;
+  // a zero scale (non-finite norm) writes zeros rather than 0 * inf
+  if (i < nw) gw[i] = s == 0.f ? 0.f : gw[i] * s;
+  else if (i < nw + nb) gb[i - nw] = s == 0.f ? 0.f : gb[i - nw] * s;
 }
 
 }  // namespace cia
@@ -1513,10 +1564,13 @@ cia_status cia_surrogate_update(void* stream, int64_t n, const double* strain, d
 // Fused BPTT training step: workspace layout and entry points.
 namespace {
 
+// gradient entries of the network: weights and biases
+constexpr int64_t kGradTotal = cia::mW5 + 12 * 256 + 4 * cia::FH + cia::FOUT;
+
 struct BpttLayout {
   int tp, Bp, tiles;
   int64_t M, chunk[5], S[5];
-  size_t xT, q, aT, dzT, doT, blob, loss, part[5], total;
+  size_t xT, q, aT, dzT, doT, blob, loss, part[5], sq, norm, total;
 };
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -1541,7 +1595,7 @@ BpttLayout bptt_layout(int64_t paths, int64_t window) {
     return at;
   };
   L.xT = take((size_t)16 * L.M * 2);
-  L.q = take((size_t)L.M * 6 * 4);
+  L.q = take((size_t)L.M * 12 * 4);
   L.aT = take((size_t)4 * 256 * L.M * 2);
   L.dzT = take((size_t)4 * 256 * L.M * 2);
   L.doT = take((size_t)16 * L.M * 2);
@@ -1549,6 +1603,8 @@ BpttLayout bptt_layout(int64_t paths, int64_t window) {
   L.loss = take((size_t)L.tiles * 4);
   const int pn[5] = {256, 256, 256, 256, 16}, pk[5] = {16, 256, 256, 256, 256};
   for (int l = 0; l < 5; ++l) L.part[l] = take((size_t)L.S[l] * pn[l] * (pk[l] + 1) * 4);
+  L.sq = take((size_t)(kGradTotal + 15) / 16 * 4);
+  L.norm = take(2 * 4);
   L.total = o;
   return L;
 }
@@ -1568,8 +1624,9 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
                               const float* path_weight, const float* stiffness,
                               float inelastic_step, const float* weights,
                               const float* biases, float* grad_weights, float* grad_biases,
-                              float* loss, const float* state_in, float* state_out,
-                              void* workspace, int64_t workspace_bytes) {
+                              float* loss, float* grad_norm, float max_grad_norm,
+                              const float* state_in, float* state_out, void* workspace,
+                              int64_t workspace_bytes) {
   using namespace cia;
   CIA_ARG(path_index && features && strain && target && path_weight && stiffness && weights &&
               biases && grad_weights && grad_biases && loss && workspace,
@@ -1642,13 +1699,20 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
   const int Ns[5] = {256, 256, 256, 256, 16}, Ks[5] = {16, 256, 256, 256, 256};
   const int Nr[5] = {256, 256, 256, 256, FOUT}, Kr[5] = {FIN, 256, 256, 256, 256};
   const int64_t woff[5] = {mW1, mW2, mW3, mW4, mW5};
-  int64_t total = 0;
+  int64_t total = 0;  // == kGradTotal
   for (int l = 0; l < 5; ++l) {
     wl.l[l] = WgradLayer{part[l], (int)L.S[l], Ns[l], Ks[l], Nr[l], Kr[l], woff[l], (int64_t)l * FH};
     total += (int64_t)Nr[l] * (Kr[l] + 1);
   }
-  hipLaunchKernelGGL(k_bptt_reduce, dim3((unsigned)((total + 15) / 16)), dim3(256), 0, st, wl,
-                     grad_weights, grad_biases, (const float*)(ws + L.loss), L.tiles, loss, total);
+  const unsigned nblk = (unsigned)((total + 15) / 16);
+  hipLaunchKernelGGL(k_bptt_reduce, dim3(nblk), dim3(256), 0, st, wl, grad_weights, grad_biases,
+                     (const float*)(ws + L.loss), L.tiles, loss, total, (float*)(ws + L.sq));
+  hipLaunchKernelGGL(k_grad_norm, dim3(1), dim3(256), 0, st, (const float*)(ws + L.sq), (int)nblk,
+                     max_grad_norm, (float*)(ws + L.norm), grad_norm);
+  if (max_grad_norm > 0.f)
+    hipLaunchKernelGGL(k_grad_scale, dim3(grid_for(kGradTotal)), dim3(kBlock), 0, st, mW5 + 12 * 256,
+                       grad_weights, (int64_t)(4 * FH + FOUT), grad_biases,
+                       (const float*)(ws + L.norm));
   CIA_HIP(hipGetLastError());
   return CIA_OK;
 }

@@ -6,12 +6,14 @@ protocol (``EnergyDissipationRVERNO``, ``src/concrete_impact/nn/models/rve_rno.p
 through time on RVE responses (``nn/training/rve_rno.py:113-415``).  This network is
 the build's own (13 -> 256 -> 256 -> 256 -> 256 -> 12, GELU):
 
-    x_t = [d_eps_t / eps_scale (6), log10(rate_t / rate_ref) / rate_scale (1), z_t (6)]
+    x_t = [d_eps_t / eps_scale (6), log10(rate_t / rate_ref) / rate_scale (1), tanh(z_t) (6)]
     o_t = MLP(x_t)
     ein_t = ein_{t-1} + c eps_scale o_t[0:6],   z_{t+1} = z_t + o_t[6:12]
     sigma_t = C (eps_t - ein_t)
 
-with c = INELASTIC_STEP, ein_{-1} = z_0 = 0 and C the homogenised elastic stiffness of
+with c = INELASTIC_STEP, ein_{-1} = z_0 = 0 (the network sees the latent through tanh, a
+bounded input, so that an untrained recurrence cannot feed back on itself without
+bound over long paths) and C the homogenised elastic stiffness of
 the RVE (its effective tangent at a small strain): the network predicts inelastic
 strain increments, so an elastic increment, loading or unloading, needs no network
 output, and the six internal state variables carry only the inelastic state.  On
@@ -300,7 +302,7 @@ def rollout(params: SurrogateParams, scales: SurrogateScales, d_eps: torch.Tenso
     out = []
     lat = [z]
     for t in range(T):
-        x = torch.cat([feats_e[:, t], feats_r[:, t, None], z], dim=1)
+        x = torch.cat([feats_e[:, t], feats_r[:, t, None], torch.tanh(z)], dim=1)
         o = forward_fused(x, params) if fused else mlp(x, params, w_leaf, b_leaf)
         ein = ein + INELASTIC_STEP * o[:, :6]
         out.append((eps[:, t] - ein) @ Cs.T)
@@ -360,6 +362,8 @@ class BpttStep:
         self.gw = torch.zeros_like(params.w)
         self.gb = torch.zeros_like(params.b)
         self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.grad_norm = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.max_grad_norm = 0.0   # > 0: clip the global gradient norm (see cia_surrogate_bptt)
 
     def __call__(self, params: SurrogateParams, data: BpttData, idx: torch.Tensor,
                  path_w: torch.Tensor, window_start: int = 0, state_in: torch.Tensor | None = None,
@@ -375,15 +379,16 @@ class BpttStep:
         _check(_lib.lib().cia_surrogate_bptt(
             stream_handle(), data.P, data.T, self.paths, int(window_start), self.window, _p(idx),
             _p(data.feat), _p(data.strain), _p(data.target), _p(path_w), data.cs, INELASTIC_STEP,
-            _p(params.w), _p(params.b), _p(self.gw), _p(self.gb), _p(self.loss), _p(state_in),
-            _p(state_out), _p(self.ws), self.ws_bytes), "cia_surrogate_bptt")
+            _p(params.w), _p(params.b), _p(self.gw), _p(self.gb), _p(self.loss),
+            _p(self.grad_norm), float(self.max_grad_norm), _p(state_in), _p(state_out),
+            _p(self.ws), self.ws_bytes), "cia_surrogate_bptt")
         return self.loss
 
 
 def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress, iterations: int,
           batch_paths: int, window: int, lr: float = 1e-3, seed: int = 0, lr_final: float = 1e-5,
           log_every: int = 0, progress=None, data: BpttData | None = None, carry: bool = False,
-          first_window_iterations: int = 0):
+          first_window_iterations: int = 0, max_grad_norm: float = 1.0):
     """BPTT on the training paths (``train_rve_rno``, ``nn/training/rve_rno.py:200-415``).
     The loss is the path-normalised stress MSE of ``_batch_loss`` (``:540-638``): each
     path's squared errors are divided by its peak squared stress (over the trained
@@ -398,7 +403,12 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
     window - 1, ...), the recurrence state carried from one window to the next and the
     gradient truncated at window starts; the first ``first_window_iterations``
     iterations train first windows only (an untrained recurrence drifts over later
-    windows)."""
+    windows).
+
+    Every step's global gradient norm is clipped to ``max_grad_norm`` (0 disables), and
+    a step whose gradient is not finite is skipped (its gradient zeroed); the number of
+    skipped steps is returned as the ``skipped_steps`` attribute of the history list --
+    the reference raises instead (``_require_finite_gradients``, ``:1040-1067``)."""
     if data is None:
         data = BpttData.from_paths(scales, d_eps, rate, stress)
     P, T = data.P, data.T
@@ -408,6 +418,10 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
     step = BpttStep(params, B, L)
     tail = T - (T // L) * L
     step_tail = BpttStep(params, B, tail) if carry and tail else None
+    for st in (step, step_tail):
+        if st is not None:
+            st.max_grad_norm = max_grad_norm
+    skipped = torch.zeros(1, dtype=torch.int32, device=device())
     rng = np.random.default_rng(seed)
     dev = device()
     states = [torch.zeros((B, 12), dtype=torch.float32, device=dev) for _ in range(2)]
@@ -429,12 +443,21 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
             loss = st(params, data, idx, path_w)
         cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * it / max(iterations - 1, 1)))
         params.adam(st.gw, st.gb, cur)
+        skipped += (~torch.isfinite(st.grad_norm)).int()
         if log_every and (it % log_every == 0 or it == iterations - 1):
             history.append((it, float(loss)))
             if progress is not None:
                 print(f"surrogate training: iteration {it} loss {history[-1][1]:.3e}", file=progress,
                       flush=True)
+    history = _History(history)
+    history.skipped_steps = int(skipped)
     return history
+
+
+class _History(list):
+    """Training history [(iteration, loss)] with the count of skipped steps."""
+
+    skipped_steps = 0
 
 
 def envelope(params, scales, d_eps, rate):
@@ -597,7 +620,8 @@ class SurrogateMaterial:
         rate = nrm / tau
         feat_raw = torch.log10(torch.clamp(rate, min=1e-30) / sc.rate_ref) / sc.rate_scale
         feat = torch.clamp(feat_raw, min=sc.rate_lo)
-        x = torch.cat([d / sc.eps_scale, feat[:, None], st.latent.t().double()], dim=1).float()
+        x = torch.cat([d / sc.eps_scale, feat[:, None], torch.tanh(st.latent.t().double())],
+                      dim=1).float()
         x = x.detach().requires_grad_(True)
         o = mlp(x, self.params)
         jac = torch.empty((n, 6, 13), dtype=torch.float64, device=strain.device)

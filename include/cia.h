@@ -588,13 +588,17 @@ cia_status cia_mlp_forward_fused(void* stream, int64_t M, const float* X, const 
  *   target   [P][path_length][6] fp32: stress / stress_scale;
  *   path_weight [P] fp32.
  * Recurrence (the rollout of the surrogate material, t from window_start):
- * x_t = [features_t[0:7], z_t],
+ * x_t = [features_t[0:7], tanh(z_t)],
  * o_t = MLP(x_t), ein_t = ein_{t-1} + inelastic_step o_t[0:6], z_{t+1} = z_t + o_t[6:12],
  * sigma_t = Cs (strain_t - ein_t), Cs = stiffness (host, 36 floats row-major, the
  * stiffness in scaled units); loss = sum_b w_p sum_{t,i} (sigma_t - target_t)_i^2 /
  * (paths window 6).  weights / biases: the fp32 master blobs (layout of
  * cia_mlp_forward_fused); grad_weights / grad_biases receive the gradients (same
- * layouts, overwritten) and loss (device, 1 float) the loss.  Network inputs and
+ * layouts, overwritten) and loss (device, 1 float) the loss.  grad_norm (device, 1
+ * float, or NULL) receives the global norm of the gradients as computed; with
+ * max_grad_norm > 0 the gradients are then scaled by min(1, max_grad_norm / norm), and
+ * set to zero when the norm is not finite (no reference counterpart: the reference
+ * rejects such a step, nn/training/rve_rno.py:1040-1067).  Network inputs and
  * activations are bf16, products accumulate in fp32 (as the layer-wise entry
  * points).  workspace: at least cia_surrogate_bptt_workspace(paths, window) bytes of
  * device memory. */
@@ -605,8 +609,9 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
                               const float* path_weight, const float* stiffness,
                               float inelastic_step, const float* weights,
                               const float* biases, float* grad_weights, float* grad_biases,
-                              float* loss, const float* state_in, float* state_out,
-                              void* workspace, int64_t workspace_bytes);
+                              float* loss, float* grad_norm, float max_grad_norm,
+                              const float* state_in, float* state_out, void* workspace,
+                              int64_t workspace_bytes);
 
 /* Surrogate material-point update: the material protocol of
  * RVERNOMaterialAdapter.update (src/concrete_impact/nn/deployment.py:97-165) for this
@@ -616,7 +621,7 @@ cia_status cia_surrogate_bptt(void* stream, int64_t n_paths, int64_t path_length
  * an anchor (strain, inelastic strain, latent, age = time since the anchor); the
  * network is evaluated once from the anchor,
  *   d = strain - anchor_strain, tau = age + time_step, rate = |d|_2 / tau,
- *   x = [d / eps_scale, max(log10(rate / rate_ref) / rate_scale, rate_lo), latent],
+ *   x = [d / eps_scale, max(log10(rate / rate_ref) / rate_scale, rate_lo), tanh(latent)],
  *   ein' = anchor_inelastic + inelastic_step eps_scale o[0:6],
  *   stress = stiffness (strain - ein'),  z' = latent + o[6:12],
  * and the anchor moves to (strain, ein', z', 0) only when |d|_inf >= commit_strain

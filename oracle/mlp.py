@@ -7,7 +7,7 @@ The reference's surrogate (``EnergyDissipationRVERNO``,
 build's own network restated in NumPy float64: per step, in scaled units (strain
 in units of the feature scale, stress in units of the stress scale),
 
-    x_t = [d_eps_t (6), rate_t (1), z_t (6)]          (13 features)
+    x_t = [d_eps_t (6), rate_t (1), tanh(z_t) (6)]    (13 features)
     h   = GELU(W4 GELU(W3 GELU(W2 GELU(W1 x + b1) + b2) + b3) + b4)
     o   = W5 h + b5                                   (12 outputs)
     eps_t = eps_{t-1} + d_eps_t,   ein_t = ein_{t-1} + c o[0:6]
@@ -15,7 +15,9 @@ in units of the feature scale, stress in units of the stress scale),
 
 (c = INELASTIC_STEP; C = the homogenised elastic stiffness of the RVE: the network
 predicts inelastic strain increments, so an elastic increment needs no network
-output at all and the latent carries only the inelastic state)
+output at all and the latent carries only the inelastic state; the network sees the
+latent through tanh, a bounded input, so that an untrained recurrence cannot feed
+back on itself without bound over long paths)
 
 with GELU in its tanh form, 0.5 v (1 + tanh(sqrt(2/pi) (v + 0.044715 v^3))), and loss = mean over paths, steps and the six
 components of (sigma_t - target_t)^2.  ``loss_and_grad`` is a hand-written
@@ -90,7 +92,7 @@ def rollout(params, d_eps, rate, C, state=None):
     lat = [z]
     caches = []
     for t in range(T):
-        x = np.concatenate([d_eps[:, t], rate[:, t, None], z], axis=1)
+        x = np.concatenate([d_eps[:, t], rate[:, t, None], np.tanh(z)], axis=1)
         o, cache = mlp_forward(params, x)
         caches.append(cache)
         eps = eps + d_eps[:, t]
@@ -119,7 +121,7 @@ def loss_and_grad(params, d_eps, rate, target, C, path_weight=None, state=None):
     the window start (truncated BPTT)."""
     B, T, _ = d_eps.shape
     w_b = np.ones(B) if path_weight is None else np.asarray(path_weight, dtype=np.float64)
-    sig, _, caches = rollout(params, d_eps, rate, C, state)
+    sig, lat, caches = rollout(params, d_eps, rate, C, state)
     n = B * T * 6
     diff = sig - target
     loss = float(np.sum(w_b[:, None, None] * diff * diff) / n)
@@ -141,7 +143,8 @@ def loss_and_grad(params, d_eps, rate, target, C, path_weight=None, state=None):
             gw += g.T @ a
             gb += g.sum(axis=0)
             g = g @ params[li][0]
-        gz = gz + g[:, 7:13]                    # dL/dz_t = dL/dz_{t+1} + dL/dx_t[7:13]
+        # dL/dz_t = dL/dz_{t+1} + dL/dx_t[7:13] tanh'(z_t) (the network sees tanh(z_t))
+        gz = gz + g[:, 7:13] * (1.0 - np.tanh(lat[:, t]) ** 2)
     return loss, grads
 
 
@@ -197,7 +200,7 @@ def material_update(params, gate, strain, dt, anchor):
     rate = np.linalg.norm(d, axis=1) / tau
     feat_raw = np.log10(np.maximum(rate, 1e-30) / g["rate_ref"]) / g["rate_scale"]
     feat = np.maximum(feat_raw, g["rate_lo"])
-    x = np.concatenate([d / g["eps_scale"], feat[:, None], anchor["latent"]], axis=1)
+    x = np.concatenate([d / g["eps_scale"], feat[:, None], np.tanh(anchor["latent"])], axis=1)
     o, _ = mlp_forward(params, x)
     ein = anchor["inelastic"] + g["inelastic_step"] * g["eps_scale"] * o[:, :6]
     stress = (strain - ein) @ C.T

@@ -121,7 +121,7 @@ def test_bptt_gradient_matches_torch_autograd():
     eps = torch.cumsum(de, dim=1)
     tl = 0.0
     for t in range(de.shape[1]):
-        a = torch.cat([de[:, t], rt[:, t, None], z], dim=1)
+        a = torch.cat([de[:, t], rt[:, t, None], torch.tanh(z)], dim=1)
         for li, (w, b) in enumerate(tp):
             a = a @ w.T + b
             if li < len(tp) - 1:
@@ -197,7 +197,7 @@ def test_planted_mistake_in_stress_recurrence_is_caught():
     s = np.zeros((B, 6))
     eps = np.cumsum(d_eps, axis=1)
     for t in range(T):
-        o, _ = OM.mlp_forward(params, np.concatenate([d_eps[:, t], rate[:, t, None], z], axis=1))
+        o, _ = OM.mlp_forward(params, np.concatenate([d_eps[:, t], rate[:, t, None], np.tanh(z)], axis=1))
         s = s + OM.INELASTIC_STEP * o[:, :6] @ C0.T
         no_elastic[:, t] = -s
         no_accum[:, t] = (eps[:, t] - OM.INELASTIC_STEP * o[:, :6]) @ C0.T
