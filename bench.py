@@ -141,43 +141,53 @@ MLP_FLOP_PER_POINT = 2 * (13 * 256 + 3 * 256 * 256 + 256 * 12)
 
 
 def surrogate_section(args) -> dict:
-    """Surrogate vs plain FE²: train the recurrent surrogate on FE² responses of a
-    two-phase RVE (edge ``--surrogate-rve``), report its stress error on held-out
-    paths, the FE² rate that produced the data (RVE solves/s, concurrent streams), the
-    fused-inference rate at 2^20 points with its matrix-core roofline, and the BPTT
-    iteration time at 4096 paths x 32 steps."""
+    """Surrogate vs plain FE² (the recurrent material-point surrogate at its stated
+    scale): FE² data of ``--surrogate-paths`` training + ``--surrogate-test-paths``
+    held-out paths of ``--surrogate-steps`` steps on a ``--surrogate-rve``^3 two-phase
+    RVE (FP64, the batched engine), training by the fused BPTT kernels, the held-out
+    stress error, the fused-inference rate at 2^20 points with its matrix-core
+    roofline, and one BPTT iteration at 4096 paths x 32 steps with its roofline.
+
+    Training schedule (every training path in every iteration): the first 64 steps
+    (``--surrogate-iters``, lr 2e-3), then whole paths (``--surrogate-full-iters``, lr
+    3e-4).  Training stages on 32-step windows overfit the first steps of the 200-step
+    paths (held-out error 6.7 % RMS against 0.9 % on the same budget), so the 4096 x 32
+    shape is the measured BPTT step, not a training stage."""
     import torch
 
     from concrete_impact_ai_amd import surrogate as S
     from concrete_impact_ai_amd import surrogate_data as SD
 
     n, T = args.surrogate_rve, args.surrogate_steps
-    p_train, p_test = args.surrogate_paths, max(8, args.surrogate_paths // 5)
+    p_train, p_test = args.surrogate_paths, args.surrogate_test_paths
     model = SD.two_phase_rve(n, seed=args.seed)
     times, strains = SD.training_paths(p_train + p_test, T, seed=1000 + args.seed,
                                        production_fraction=args.surrogate_production_fraction)
     print(f"surrogate: FE² data {p_train + p_test} paths x {T} steps on {n}^3 RVEs",
           file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
     d_eps, rate, stress, fe2_rate = SD.rve_responses_batched(model, times, strains, progress=sys.stderr)
+    data_s = time.perf_counter() - t0
     tr = slice(0, p_train)
     te = slice(p_train, p_train + p_test)
     scales = S.SurrogateScales.fit(d_eps[tr], rate[tr], stress[tr], SD.homogenised_stiffness(model))
     params = S.SurrogateParams(seed=args.seed)
-    print(f"surrogate: training {args.surrogate_iters} BPTT iterations", file=sys.stderr, flush=True)
+    data = S.BpttData.from_paths(scales, d_eps[tr], rate[tr], stress[tr])
+    stages = [(args.surrogate_iters, min(64, T), p_train, 2e-3),
+              (args.surrogate_full_iters, T, p_train, 3e-4)]
+    print(f"surrogate: training {stages[0][0]} + {stages[1][0]} BPTT iterations", file=sys.stderr,
+          flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    # paths longer than 64 steps: train on their first 64 steps first, then on the
-    # whole path at a tenth of the learning rate (the recurrence of an untrained
-    # network diverges over ~100 commits, and full-length BPTT at the first stage's
-    # rate diverges too)
-    hist = []
-    stages = [(64, args.surrogate_iters // 2, 2e-3),
-              (T, args.surrogate_iters - args.surrogate_iters // 2, 2e-4)] \
-        if T > 64 else [(T, args.surrogate_iters, 2e-3)]
-    for window, iters, lr in stages:
-        hist += S.train(params, scales, d_eps[tr], rate[tr], stress[tr], iterations=iters,
-                        batch_paths=min(96, p_train), window=window, lr=lr, seed=args.seed,
-                        log_every=max(1, iters // 10), progress=sys.stderr)
+    hist, skipped = [], 0
+    for k, (iters, window, batch, lr) in enumerate(stages):
+        if iters <= 0:
+            continue
+        h = S.train(params, scales, None, None, None, iterations=iters, batch_paths=batch,
+                    window=window, lr=lr, seed=args.seed + k, log_every=max(1, iters // 5),
+                    progress=sys.stderr, data=data)
+        hist += [(it + sum(s[0] for s in stages[:k]), loss) for it, loss in h]
+        skipped += h.skipped_steps
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     S.envelope(params, scales, d_eps[tr], rate[tr])
@@ -187,9 +197,9 @@ def surrogate_section(args) -> dict:
     errs = SD.stress_errors(pred.double().cpu().numpy(), stress[te])
     rms = SD.stress_rms_errors(pred.double().cpu().numpy(), stress[te])
     train_errs = SD.stress_errors(
-        S.rollout(params, scales, torch.as_tensor(d_eps[tr][:32], dtype=torch.float32, device=dev),
-                  torch.as_tensor(rate[tr][:32], dtype=torch.float32, device=dev),
-                  fused=True)[0].double().cpu().numpy(), stress[tr][:32])
+        S.rollout(params, scales, torch.as_tensor(d_eps[tr][:64], dtype=torch.float32, device=dev),
+                  torch.as_tensor(rate[tr][:64], dtype=torch.float32, device=dev),
+                  fused=True)[0].double().cpu().numpy(), stress[tr][:64])
     # fused inference at 2^20 points
     M = 1 << 20
     X = torch.randn((M, 13), dtype=torch.float32, device=dev) * 0.5
@@ -204,27 +214,36 @@ def surrogate_section(args) -> dict:
     torch.cuda.synchronize()
     inf_ms = e0.elapsed_time(e1) / reps
     tflops = MLP_FLOP_PER_POINT * M / (inf_ms * 1e-3) / 1e12
-    # one BPTT iteration at 4096 paths x 32 steps (synthetic inputs)
+    # one BPTT iteration (cia_surrogate_bptt: forward and backward through time, weight
+    # gradients) at 4096 paths x 32 steps of the training data, HIP events on its stream
     Bp, W = 4096, 32
-    de = np.random.default_rng(0).standard_normal((Bp, W, 6)) * scales.eps_scale * 0.3
-    rt = np.full((Bp, W), scales.rate_ref)
-    sg = np.random.default_rng(1).standard_normal((Bp, W, 6)) * scales.stress_scale * 0.3
     tmp = S.SurrogateParams(seed=1)
-    S.train(tmp, scales, de, rt, sg, iterations=2, batch_paths=Bp, window=W, lr=1e-4)
+    bstep = S.BpttStep(tmp, Bp, W)
+    idx = torch.as_tensor(np.random.default_rng(3).choice(p_train, Bp), dtype=torch.int32, device=dev)
+    pw = data.path_weights(W)
+    for _ in range(3):
+        bstep(tmp, data, idx, pw)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        bstep(tmp, data, idx, pw)
+    e1.record()
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    S.train(tmp, scales, de, rt, sg, iterations=5, batch_paths=Bp, window=W, lr=1e-4)
-    torch.cuda.synchronize()
-    bptt_ms = (time.perf_counter() - t1) / 5 * 1e3
+    bptt_ms = e0.elapsed_time(e1) / reps
+    bptt_tf = 3 * MLP_FLOP_PER_POINT * Bp * W / (bptt_ms * 1e-3) / 1e12
     pts_per_s = M / (inf_ms * 1e-3)
     return {
         "rve_edge": n,
         "fe2_rve_solves_per_s": fe2_rate,
-        "fe2_data": f"{p_train + p_test} paths x {T} steps, one batched FE² solve per step",
+        "fe2_data": f"{p_train + p_test} paths x {T} steps, one batched FP64 FE² solve per step",
+        "fe2_data_seconds": data_s,
         "train_paths": p_train,
         "test_paths": p_test,
-        "train_iterations": args.surrogate_iters,
+        "train_schedule": [{"iterations": i, "window": w, "batch_paths": b, "lr": lr}
+                           for i, w, b, lr in stages],
+        "train_iterations": sum(s[0] for s in stages),
         "train_seconds": train_s,
+        "train_skipped_steps": skipped,
         "loss_history": hist,
         "test_stress_error_median": float(np.median(errs)),
         "test_stress_error_p95": float(np.percentile(errs, 95)),
@@ -240,6 +259,11 @@ def surrogate_section(args) -> dict:
         "inference_roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
                                "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS},
         "bptt_ms_4096x32": bptt_ms,
+        "bptt_roofline": {"kernel": "cia_surrogate_bptt (k_bptt_prep, k_bptt_fwd, k_bptt_bwd, "
+                                    "k_wgrad_nt x5, k_bptt_reduce, k_grad_norm)",
+                          "bound": "mfma", "achieved": bptt_tf, "peak": BF16_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": bptt_tf / BF16_PEAK_TFLOPS,
+                          "flop_per_iteration": 3 * MLP_FLOP_PER_POINT * Bp * W},
         "speedup_vs_fe2_per_point_update": pts_per_s / fe2_rate,
         "meets_2pct_rms_median": bool(np.median(rms) <= 0.02),
         "meets_2pct_max_p95": bool(np.percentile(errs, 95) <= 0.02),
@@ -263,19 +287,24 @@ def _impact_slab(divisions, material, patch_elems, h=0.01):
 
 
 def macro_surrogate_section(args, ctx) -> dict:
-    """Surrogate-driven explicit impact step against the plain FE² step.
+    """Surrogate-driven explicit impact step against the plain FE² step (config 5).
 
     The macro slab of the plain-J2 line (128 x 128 x 64 Hex8, h = 10 mm, 50 MPa
     half-sine over 0.2 ms on the centred 20 x 20-element top patch, bottom fixed)
-    with the trained surrogate at every Gauss point and the gated FE² fallback (the
-    training RVE, FP64, batched device solves).  The time step is the CFL-0.5 step of
+    with the trained surrogate at every Gauss point, ``--macro-surrogate-steps`` steps,
+    and the gated FE² fallback on ``--fallback-rve``^3 two-phase RVEs (FP64, batched
+    device solves): ``--macro-monitors`` monitor points under the patch carry their own
+    fallback RVEs and trip the error gate when their surrogate stress leaves the FE²
+    stress by more than 5 % of the stress scale.  The time step is the CFL-0.5 step of
     the stiffest phase's P-wave speed.  Reported:
     * surrogate macro steps/s (host clock between material calls, after warm-up);
-    * the plain FE² step rate of the same slab: every Gauss point one RVE solve per
-      step, at the FE² RVE solves/s measured on the training-data batch (n_gp /
-      rate), and an end-to-end FE² macro step measured on a small slab;
+    * the plain FE² step: end-to-end FE² macro steps of the 8 x 8 x 4 sample slab
+      (2048 fallback-RVE solves per step) measured, and its Gauss-point rate scaled to
+      the full slab's Gauss points;
     * the stress error at probe Gauss points under the patch against the FP64 FE²
-      oracle, which drives one RVE per probe along the probe's strain history."""
+      oracle on the training RVE, driven along each probe's strain history;
+    * a gate-trip leg: the sample slab with every point monitored and a 1 % error
+      bound, so that points trip and continue on their FE² RVEs."""
     import torch
 
     from concrete_impact_ai_amd import dynamics as D
@@ -289,24 +318,32 @@ def macro_surrogate_section(args, ctx) -> dict:
     settings = M.MaterialUpdateSettings(50, 1e-10, 1e-12, 1e-10)
     nx, ny, nz = args.macro_divisions
     h = 0.01
-    fb = fe2.RVEHomogenizedMaterial(model, "fallback", fe2.RVEExecutionSettings("batched"))
+    fb_model = SD.two_phase_rve(args.fallback_rve, seed=args.seed + 1)
+    batched = fe2.RVEExecutionSettings("batched")
+    fb = fe2.RVEHomogenizedMaterial(fb_model, "fallback", batched)
     cp = fb.maximum_wave_speed  # the stiffest phase's P-wave speed, as the FE² material states it
     dt = 0.5 * h / cp
     n_gp = 8 * nx * ny * nz
     rng = np.random.default_rng(args.seed)
-    P = args.macro_probes
     c0 = ((nx - 20) // 2, (ny - 20) // 2)
-    pi = rng.integers(c0[0] - 4, c0[0] + 24, P)
-    pj = rng.integers(c0[1] - 4, c0[1] + 24, P)
-    pk = rng.integers(nz // 2, nz, P)
-    probes = 8 * (pi + nx * (pj + ny * pk)) + rng.integers(0, 8, P)
 
-    def surrogate_material(probe_points=None, fallback=None):
-        return S.SurrogateMaterial("concrete_surrogate", params, scales, 2400.0, cp, fallback=fallback,
-                                   probe_points=probe_points)
+    def under_patch(count):
+        pi = rng.integers(c0[0] - 4, c0[0] + 24, count)
+        pj = rng.integers(c0[1] - 4, c0[1] + 24, count)
+        pk = rng.integers(nz // 2, nz, count)
+        return 8 * (pi + nx * (pj + ny * pk)) + rng.integers(0, 8, count)
+
+    P = args.macro_probes
+    probes = under_patch(P)
+    monitors = under_patch(args.macro_monitors)
+
+    def surrogate_material(probe_points=None, monitor_points=None, error_bound=0.05):
+        return S.SurrogateMaterial("concrete_surrogate", params, scales, 2400.0, cp, fallback=fb,
+                                   probe_points=probe_points, monitor_points=monitor_points,
+                                   check_interval=20, error_bound=error_bound)
 
     try:
-        sm = surrogate_material(probes, fb)
+        sm = surrogate_material(probes, monitors)
         bundle, load, dirichlet, _ = _impact_slab((nx, ny, nz), sm, 20, h)
         n = bundle.mesh_info.dof_map.size
         steps = args.macro_surrogate_steps
@@ -319,7 +356,8 @@ def macro_surrogate_section(args, ctx) -> dict:
         ct = sm.call_times
         per_step = (ct[-1] - ct[w]) / (len(ct) - 1 - w)
         strain, stress, dts = sm.probe_history()      # (T, P, 6): calls 0..steps
-        # FP64 FE² oracle along the probe strain histories (one RVE per probe)
+        stats = dict(sm.stats)
+        # FP64 FE² oracle on the training RVE along the probe strain histories
         times = np.concatenate([[0.0], np.cumsum(dts[1:])])
         hist = np.concatenate([np.zeros((1, P, 6)), strain[1:]], axis=0).transpose(1, 0, 2)
         print(f"macro surrogate: FP64 FE² oracle on {P} probe paths", file=sys.stderr, flush=True)
@@ -329,13 +367,13 @@ def macro_surrogate_section(args, ctx) -> dict:
         keep = peak >= 0.05 * np.max(peak)
         rms = SD.stress_rms_errors(sur[keep], ref[keep])
         mx = SD.stress_errors(sur[keep], ref[keep])
-        stats = dict(sm.stats)
         print(f"macro surrogate: {1e3 * per_step:.2f} ms/step, {stats['gate_trips']} gate trips, "
               f"stress rms error median {np.median(rms):.4f} p95 {np.percentile(rms, 95):.4f}, "
               f"max error median {np.median(mx):.4f}", file=sys.stderr, flush=True)
-        # end-to-end FE² macro steps on a small slab, and the surrogate on the same slab
+        # end-to-end FE² macro steps on the sample slab, and the surrogate on the same slab
         small = tuple(args.macro_small)
         small_steps = 6
+        n_small = 8 * small[0] * small[1] * small[2]
 
         def run_small(material, k_steps):
             b, ld, dr, _ = _impact_slab(small, material, max(2, small[0] // 2), h)
@@ -349,7 +387,7 @@ def macro_surrogate_section(args, ctx) -> dict:
             return time.perf_counter() - t0
 
         print(f"macro surrogate: end-to-end FE² on a {small} slab", file=sys.stderr, flush=True)
-        fe2_small = fe2.RVEHomogenizedMaterial(model, "plain_fe2", fe2.RVEExecutionSettings("batched"))
+        fe2_small = fe2.RVEHomogenizedMaterial(fb_model, "plain_fe2", batched)
         try:
             run_small(fe2_small, 2)                     # batch set-up and autotuning
             ta = run_small(fe2_small, 2)
@@ -357,37 +395,55 @@ def macro_surrogate_section(args, ctx) -> dict:
         finally:
             fe2_small.close()
         fe2_small_step = (tb - ta) / small_steps
-        sm_small = surrogate_material(None, fb)
+        sm_small = surrogate_material()
         run_small(sm_small, 2)
         ta = run_small(sm_small, 2)
         tb = run_small(sm_small, 2 + small_steps)
         sur_small_step = (tb - ta) / small_steps
+        # gate-trip leg: every point of the sample slab monitored, 1 % error bound
+        print("macro surrogate: gate-trip leg", file=sys.stderr, flush=True)
+        sm_trip = surrogate_material(None, np.arange(n_small), error_bound=0.01)
+        trip_s = run_small(sm_trip, 40)
+        trip_stats = dict(sm_trip.stats)
     finally:
         fb.close()
-    n_small = 8 * small[0] * small[1] * small[2]
-    fe2_pts = max(ctx["fe2_rate"], n_small / fe2_small_step)
+    fe2_pts = n_small / fe2_small_step
     fe2_full_step = n_gp / fe2_pts
     return {
         "workload": f"explicit velocity Verlet, {nx}x{ny}x{nz} Hex8 slab, surrogate at every Gauss point "
-                    f"with gated FE² fallback ({args.surrogate_rve}^3 two-phase RVE, FP64), 50 MPa / "
-                    f"0.2 ms half-sine on a 20x20 top patch, dt {dt:.4e} s (CFL 0.5)",
+                    f"(trained on {args.surrogate_rve}^3 FE² data) with FE²-judged error gate and "
+                    f"fallback on {args.fallback_rve}^3 two-phase RVEs (FP64), 50 MPa / 0.2 ms "
+                    f"half-sine on a 20x20 top patch, dt {dt:.4e} s (CFL 0.5)",
         "gauss_points": n_gp,
         "steps": steps,
         "surrogate_macro_steps_per_s": 1.0 / per_step,
         "surrogate_ms_per_step": per_step * 1e3,
         "surrogate_gp_updates_per_s": n_gp / per_step,
+        "monitor_points": int(args.macro_monitors),
+        "error_checks": stats["error_checks"],
         "gate_trips": stats["gate_trips"],
+        "error_trips": stats["error_trips"],
+        "judged_error_max": stats["judged_error_max"],
         "fallback_point_steps": stats["fallback_points"],
-        "plain_fe2_rve_solves_per_s": fe2_pts,
+        "plain_fe2_sample": {"divisions": list(small), "gauss_points": n_small,
+                             "steps_timed": small_steps,
+                             "ms_per_step": fe2_small_step * 1e3,
+                             "rve_solves_per_s": fe2_pts},
         "plain_fe2_ms_per_step": fe2_full_step * 1e3,
         "speedup_vs_plain_fe2_step": fe2_full_step / per_step,
-        "fe2_step_note": "plain FE² on this slab = one FP64 RVE solve per Gauss point per step at the "
-                         "best measured FE² RVE rate (batched device solves); a full-slab FE² step is not "
-                         "run (8.4 M RVEs per step)",
+        "fe2_step_note": "plain FE² step of this slab = its Gauss points at the RVE-solve rate measured "
+                         "end to end on the sample slab (one fallback RVE per Gauss point, batched); the "
+                         "full-slab FE² step (8.4 M RVE solves) is not run",
         "small_slab": {"divisions": list(small), "steps_timed": small_steps,
                        "plain_fe2_ms_per_step": fe2_small_step * 1e3,
                        "surrogate_ms_per_step": sur_small_step * 1e3,
                        "speedup": fe2_small_step / sur_small_step},
+        "gate_trip_leg": {"divisions": list(small), "steps": 40, "monitor_points": n_small,
+                          "error_bound": 0.01, "seconds": trip_s,
+                          "error_checks": trip_stats["error_checks"],
+                          "gate_trips": trip_stats["gate_trips"],
+                          "error_trips": trip_stats["error_trips"],
+                          "fallback_point_steps": trip_stats["fallback_points"]},
         "probes": P,
         "probes_scored": int(np.sum(keep)),
         "probe_peak_stress_max": float(np.max(peak)),
@@ -398,8 +454,10 @@ def macro_surrogate_section(args, ctx) -> dict:
         "oracle_rve_solves_per_s": oracle_rate,
         "error_definition": "per probe Gauss point under the patch (peak stress >= 5 % of the largest): "
                             "rms / max over the run of |sigma_sur - sigma_FE2|_inf, divided by the probe's "
-                            "peak |sigma_FE2|_inf; FE2 = FP64 RVE driven along the probe's strain history",
+                            f"peak |sigma_FE2|_inf; FE2 = FP64 {args.surrogate_rve}^3 training RVE driven "
+                            "along the probe's strain history",
         "final_dissipated_energy": float(sol.dissipated_energy[-1]),
+        "final_free_energy": float(sol.free_energy[-1]),
         "meets_50x": bool(fe2_full_step / per_step >= 50.0),
         "meets_2pct_rms_median": bool(np.median(rms) <= 0.02),
     }
@@ -738,16 +796,23 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=8)
     ap.add_argument("--batch-sample", type=int, default=0,
                     help="RVEs of the single-RVE comparison (0: all)")
-    ap.add_argument("--surrogate-rve", type=int, default=12)
-    ap.add_argument("--surrogate-steps", type=int, default=64)
-    ap.add_argument("--surrogate-paths", type=int, default=1280)
-    ap.add_argument("--surrogate-iters", type=int, default=4000)
+    ap.add_argument("--surrogate-rve", type=int, default=24)
+    ap.add_argument("--surrogate-steps", type=int, default=200)
+    ap.add_argument("--surrogate-paths", type=int, default=2000)
+    ap.add_argument("--surrogate-test-paths", type=int, default=400)
+    ap.add_argument("--surrogate-iters", type=int, default=3000,
+                    help="BPTT iterations on 4096 x 64-step windows")
+    ap.add_argument("--surrogate-full-iters", type=int, default=6000,
+                    help="BPTT iterations on whole paths (every training path)")
+    ap.add_argument("--fallback-rve", type=int, default=16,
+                    help="edge of the FE² fallback / monitor RVE of the macro impact run")
     ap.add_argument("--surrogate-production-fraction", type=float, default=0.5,
                     help="share of training paths from the production path design")
     ap.add_argument("--no-macro-surrogate", action="store_true",
                     help="skip the surrogate-driven macro impact run")
-    ap.add_argument("--macro-surrogate-steps", type=int, default=300)
-    ap.add_argument("--macro-probes", type=int, default=128)
+    ap.add_argument("--macro-surrogate-steps", type=int, default=2000)
+    ap.add_argument("--macro-probes", type=int, default=64)
+    ap.add_argument("--macro-monitors", type=int, default=64)
     ap.add_argument("--macro-small", type=int, nargs=3, default=(8, 8, 4))
     args = ap.parse_args()
     if args.gpus != 1:

@@ -252,6 +252,10 @@ __global__ void __launch_bounds__(kBlock)
     for (int m = 0; m < 6; ++m) state_t[m * np + p] = o.epsp[m];
     state_t[6 * np + p] = o.qbar;
     state_t[7 * np + p] = o.gam;
+  } else {
+    // elastic points carry no internal state: every row of the output is written
+#pragma unroll
+    for (int m = 0; m < 8; ++m) state_t[m * np + p] = (T)0;
   }
   tan[0 * np + p] = o.tan.d;
   tan[1 * np + p] = o.tan.r;

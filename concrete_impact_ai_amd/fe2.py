@@ -154,7 +154,7 @@ class RVEHomogenizedMaterial:
         self.execution_settings = execution_settings or RVEExecutionSettings()
         es = self.execution_settings
         self._batched = es.backend == "batched"
-        self._batches = {}  # number of macro points -> batched device operator
+        self._batches = {}  # RVEs per batched solve -> batched device operator
         self._workers = 1 if es.backend in ("serial", "batched") else es.workers
         self.operator_cache = RVEOperatorCache(model)
         self._caches = [self.operator_cache] + [RVEOperatorCache(model)
@@ -210,28 +210,53 @@ class RVEHomogenizedMaterial:
                              torch.as_tensor(w, device=dev),
                              np.asarray(state.variables["rve_macro_strain"]), c)
 
+    # Gauss points of all RVEs of one batched solve: larger point sets are solved in
+    # chunks, so that a fallback set of any size fits device memory
+    BATCH_GAUSS_POINTS = 1 << 28
+
+    def _batch_for(self, m: int):
+        from concrete_impact_ai_amd import batch as B
+
+        bt = self._batches.get(m)
+        if bt is None:
+            if len(self._batches) >= 2:  # keep the operators of the last two chunk sizes
+                self._batches.pop(next(iter(self._batches))).close()
+            bt = self._batches[m] = B.RVEBatch([self.model] * m)
+        return bt
+
     def _solve_batched(self, strains: np.ndarray, time_step, settings, committed: RVEBatchState):
-        """All macro points' RVEs in one batched solve: the per-point state layout
-        [n][8][n_micro] is permuted to the batch arena's [8][n n_micro] and back."""
+        """All macro points' RVEs in batched solves of at most BATCH_GAUSS_POINTS Gauss
+        points: the per-point state layout [n][8][n_micro] is permuted to the batch
+        arena's [8][m n_micro] and back.  The last chunk is padded to a power of two up to
+        64 RVEs, beyond that to a multiple of 64 (copies of its last point, results
+        discarded), so that few distinct batch sizes are ever built."""
         from concrete_impact_ai_amd import batch as B
 
         n = strains.shape[0]
         c = self.operator_cache
-        bt = self._batches.get(n)
-        if bt is None:
-            if len(self._batches) >= 4:  # keep the operators of the last few point counts
-                self._batches.pop(next(iter(self._batches))).close()
-            bt = self._batches[n] = B.RVEBatch([self.model] * n)
-        packed = committed.packed.permute(1, 0, 2).reshape(8, n * c.n_points)
-        steps = np.broadcast_to(np.asarray(time_step, dtype=np.float64), (n,))
-        status, res, nxt_p, nxt_w = B.solve_batch_raw(bt, packed, committed.w.reshape(-1),
-                                                      committed.macro, strains, steps, settings)
-        if status == _lib.CIA_ERR_RVE_FAILURE:
-            i = next(k for k, r in enumerate(res) if r.reason != 0)
-            B.raise_failure(bt, i, res[i], strains[i], float(steps[i]), settings)
-        nxt_packed = nxt_p.reshape(8, n, c.n_points).permute(1, 0, 2).contiguous()
-        return list(res), RVEBatchState(nxt_packed, nxt_w.reshape(n, 3 * c.n_classes),
-                                        strains.copy(), c)
+        cap = max(1, self.BATCH_GAUSS_POINTS // c.n_points)
+        steps_all = np.broadcast_to(np.asarray(time_step, dtype=np.float64), (n,))
+        results: list = []
+        packs, ws = [], []
+        for s0 in range(0, n, cap):
+            m = min(cap, n - s0)
+            mb = cap if m == cap else min(cap, 1 << (m - 1).bit_length() if m <= 64 else -(-m // 64) * 64)
+            rows = np.minimum(np.arange(s0, s0 + mb), s0 + m - 1)
+            rows_t = torch.as_tensor(rows, device=committed.packed.device)
+            bt = self._batch_for(mb)
+            packed = committed.packed[rows_t].permute(1, 0, 2).reshape(8, mb * c.n_points)
+            status, res, nxt_p, nxt_w = B.solve_batch_raw(
+                bt, packed.contiguous(), committed.w[rows_t].reshape(-1).contiguous(),
+                committed.macro[rows], strains[rows], steps_all[rows], settings)
+            if status == _lib.CIA_ERR_RVE_FAILURE:
+                i = next(k for k, r in enumerate(res[:m]) if r.reason != 0)
+                B.raise_failure(bt, i, res[i], strains[s0 + i], float(steps_all[s0 + i]), settings)
+            results.extend(list(res)[:m])
+            packs.append(nxt_p.reshape(8, mb, c.n_points)[:, :m].permute(1, 0, 2))
+            ws.append(nxt_w.reshape(mb, 3 * c.n_classes)[:m])
+        nxt_packed = torch.cat(packs).contiguous() if len(packs) > 1 else packs[0].contiguous()
+        nxt_w = torch.cat(ws) if len(ws) > 1 else ws[0]
+        return results, RVEBatchState(nxt_packed, nxt_w.contiguous(), strains.copy(), c)
 
     def _solve_all(self, strains: np.ndarray, time_step: float, settings, need_tangent: bool,
                    committed: RVEBatchState):

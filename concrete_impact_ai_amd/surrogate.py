@@ -432,7 +432,7 @@ def train(params: SurrogateParams, scales: SurrogateScales, d_eps, rate, stress,
         if not carry or it < first_window_iterations or s0 >= T:
             s0 = 0
         if s0 == 0:
-            draw = rng.choice(P, size=B, replace=P < batch_paths)
+            draw = rng.choice(P, size=B, replace=False)   # B <= P: distinct paths
             idx = torch.as_tensor(draw, dtype=torch.int32, device=dev)
         st = step if s0 + L <= T else step_tail
         if carry:

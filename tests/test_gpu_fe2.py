@@ -256,3 +256,75 @@ def test_fe2_batched_backend_equals_serial(gpu):
     finally:
         serial.close()
         bat.close()
+
+
+def test_fe2_batched_backend_chunks_large_point_sets(gpu):
+    """With a Gauss-point budget of 4 RVEs per batched solve, 10 macro points run in
+    chunks of 4, 4 and 2: stresses, dissipation and the next states equal the solve of
+    all 10 points in one batch (padded to 16 RVEs; each RVE's solve depends only on its
+    own data), and only batches of 4 and 2 RVEs are built."""
+    from concrete_impact_ai_amd import fe2
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import solvers as S
+
+    rng = np.random.default_rng(17)
+    npts = 10
+    _, one = _two_phase_adapter(M, R, S, fe2, "batched", 1)
+    _, chunked = _two_phase_adapter(M, R, S, fe2, "batched", 1)
+    chunked.BATCH_GAUSS_POINTS = 4 * chunked.operator_cache.n_points
+    req = M.MaterialResponseRequirements(tangent=False, free_energy=True, dissipation=True)
+    try:
+        st_a, st_b = one.initialize_state(npts), chunked.initialize_state(npts)
+        for step in range(3):
+            strains = rng.uniform(-1.0, 1.0, (npts, 6)) * 1.5e-3 * (step + 1)
+            ra = one.update(_request(M, strains), st_a, req)
+            rb = chunked.update(_request(M, strains), st_b, req)
+            scale = np.max(np.abs(ra.stresses))
+            assert np.max(np.abs(rb.stresses - ra.stresses)) <= 1e-12 * scale
+            np.testing.assert_allclose(rb.dissipation, ra.dissipation, rtol=1e-10,
+                                       atol=1e-12 * np.max(np.abs(ra.free_energy)))
+            for k, v in ra.state.variables.items():
+                np.testing.assert_allclose(rb.state.variables[k], v, rtol=0,
+                                           atol=1e-12 * max(np.max(np.abs(v)), 1e-3), err_msg=k)
+            st_a, st_b = ra.state, rb.state
+        assert sorted(chunked._batches) == [2, 4]
+    finally:
+        one.close()
+        chunked.close()
+
+
+def test_fe2_tangent_step_writes_the_whole_next_state(gpu):
+    """A step with a tangent request (per-point device solves) from a batched state
+    returns the same next RVE states as the batched step without tangent, elastic
+    points' rows included (all zero): the single-RVE assembly writes every row of its
+    next-state output, which the adapter allocates uninitialised."""
+    import torch
+
+    from concrete_impact_ai_amd import fe2
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd import solvers as S
+
+    rng = np.random.default_rng(23)
+    npts = 6
+    model, bat = _two_phase_adapter(M, R, S, fe2, "batched", 1)
+    try:
+        st = bat.initialize_state(npts)
+        for step in range(3):
+            strains = rng.uniform(-1.0, 1.0, (npts, 6)) * 2e-3 * (step + 1)
+            _, st = bat.solve_batch(strains, np.full(npts, 1e-5), _request(M, strains).update_settings, st)
+        # poison free blocks of the output's size so that unwritten output would show
+        junk = [torch.full_like(st.packed, 7.0) for _ in range(8)]
+        del junk
+        strains = rng.uniform(-1.0, 1.0, (npts, 6)) * 8e-3
+        settings = _request(M, strains).update_settings
+        _, s_b = bat.solve_batch(strains, np.full(npts, 1e-5), settings, st, False)
+        _, s_t = bat.solve_batch(strains, np.full(npts, 1e-5), settings, st, True)
+        pb, pt = s_b.packed.cpu().numpy(), s_t.packed.cpu().numpy()
+        scale = max(np.max(np.abs(pb)), 1e-30)
+        assert np.max(np.abs(pt - pb)) <= 1e-10 * scale
+        elastic = np.repeat(np.asarray(model.element_phase_ids) == 1, 8)
+        assert not pt[:, :, elastic].any()
+    finally:
+        bat.close()
