@@ -87,15 +87,22 @@ def build_model(n: int, seed: int, precision: str, pcg_rtol: float, forcing: boo
     return model, eph
 
 
-def cpu_baseline(n: int, seed: int, newton_per_step: float, backtracks_per_step: float) -> dict:
-    """Oracle (FP64 NumPy/SciPy port of the reference algorithm) on the host.
+def cpu_baseline(n: int, seed: int, n_cpu: int = 12, k: int = 3) -> dict:
+    """The reference algorithm on the host, timed in full: the oracle (FP64 NumPy port
+    of the reference's point updates, Hex8 assembly and Newton / Armijo with SciPy
+    ``spsolve`` = SuperLU, ``src/fem/rve/solver.py:62-227``, ``src/fem/solvers/
+    linear.py:15-22``) on one core (BLAS limited to one thread; SuperLU is serial).
 
-    Sample: one full Newton assembly (strain fields, J2/elastic point updates,
-    internal force, tangent COO -> CSR) of the same 48^3 RVE.  A reference
-    increment needs at least 1 + newton (1 + candidates) such assemblies plus one
-    SuperLU factorisation per Newton iteration; the SuperLU time is excluded,
-    so the reported rate is an upper bound on the CPU RVE solves/s.
-    """
+    SuperLU's factorisation of the 3-D periodic stiffness grows roughly as DOF^2
+    (2.1 s per solve at 12^3 with 5 181 DOFs, ~10 s at 16^3 on the build host), so one
+    increment of the 48^3 RVE (331 773 DOFs) would take hours.  The timed sample is
+    the same workload -- microstructure generator, seed and volume fraction, phases,
+    the seeded 200-step path, Newton and Armijo settings -- on an ``n_cpu``^3 RVE: ``k``
+    increments (3 .. 2 + k of the path), each timed in full, median and MAD.  Beside it:
+    SuperLU solve times at 8^3 and ``n_cpu``^3 and the sample's assembly time
+    (measured), and the 48^3 increment extrapolated from them (marked as such)."""
+    import os
+
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:  # pragma: no cover
@@ -104,35 +111,85 @@ def cpu_baseline(n: int, seed: int, newton_per_step: float, backtracks_per_step:
     from oracle import j2vp as OJ
     from oracle import rve as OR
 
-    eph = spherical_aggregates((n, n, n), 0.40, (2.0, 6.0), seed=seed)
-    ctx = threadpool_limits(limits=1) if threadpool_limits else None
-    try:
-        model = OR.Model((1.0, 1.0, 1.0), (n, n, n),
-                         [("mortar", OJ.J2VPParams(25e9, 0.2, 20e6, 0.5e9, 1e-3, 20e6, 1.0)),
-                          ("aggregate", OJ.ElasticParams(50e9, 0.2))],
-                         OR.Newton(25, 2e-3, 1e-10, 1e-12, 1e-10, OR.Armijo(True)),
-                         element_phase_ids=eph)
-        times, strains = random_strain_path(200, 8, 3.0e-3, (10.0, 1000.0), seed=seed)
-        st = model.initial_state()
-        u = model.constraint.affine @ strains[2]
-        up = model.constraint.affine @ strains[1]
+    def model_for(edge):
+        eph = spherical_aggregates((edge, edge, edge), 0.40, (2.0, 6.0), seed=seed)
+        return OR.Model((1.0, 1.0, 1.0), (edge, edge, edge),
+                        [("mortar", OJ.J2VPParams(25e9, 0.2, 20e6, 0.5e9, 1e-3, 20e6, 1.0)),
+                         ("aggregate", OJ.ElasticParams(50e9, 0.2))],
+                        OR.Newton(25, 2e-3, 1e-10, 1e-12, 1e-10, OR.Armijo(True)),
+                        element_phase_ids=eph)
+
+    times, strains = random_strain_path(200, 8, 3.0e-3, (10.0, 1000.0), seed=seed)
+    solve_times = []
+    real_spsolve = OR.spsolve
+
+    def timed_spsolve(a, b):
         t0 = time.perf_counter()
-        model.assemble(u, up, times[2] - times[1], OJ.UpdateSettings(), st["material_state"])
-        t_asm = time.perf_counter() - t0
+        x = real_spsolve(a, b)
+        solve_times.append(time.perf_counter() - t0)
+        return x
+
+    ctx = threadpool_limits(limits=1) if threadpool_limits else None
+    OR.spsolve = timed_spsolve
+    try:
+        def run(edge, incs):
+            model = model_for(edge)
+            st = model.initial_state()
+            out = []
+            for j in range(1, 3 + incs):
+                del solve_times[:]
+                t0 = time.perf_counter()
+                r = OR.solve_increment(model, strains[j], times[j] - times[j - 1], OJ.UpdateSettings(),
+                                       st, need_tangent=False)
+                dt_inc = time.perf_counter() - t0
+                st = r["state"]
+                if j >= 3:
+                    out.append((dt_inc, r["diagnostics"]["newton_iterations"],
+                                r["diagnostics"]["armijo_backtracks"], list(solve_times)))
+            return out
+
+        small = run(8, 2)
+        sample = run(n_cpu, k)
     finally:
+        OR.spsolve = real_spsolve
         if ctx is not None:
             ctx.__exit__(None, None, None)
-    # iterations 0..N each assemble once; each of the N Armijo searches evaluates 1 + bt candidates
-    assemblies = 1.0 + 2.0 * newton_per_step + backtracks_per_step
-    per_step = assemblies * t_asm
+    inc = np.asarray([s[0] for s in sample])
+    med = float(np.median(inc))
+    mad = float(np.median(np.abs(inc - med)))
+    lu8 = float(np.median([t for s in small for t in s[3]]))
+    lu = float(np.median([t for s in sample for t in s[3]]))
+    dof = lambda e: 3 * e ** 3 - 3  # noqa: E731  (reduced periodic DOFs)
+    expo = np.log(lu / lu8) / np.log(dof(n_cpu) / dof(8))
+    lu48 = lu * (dof(n) / dof(n_cpu)) ** expo
+    newton = float(np.mean([s[1] for s in sample]))
+    backtracks = float(np.mean([s[2] for s in sample]))
+    # one full assembly of the sample (the increment less its SuperLU solves, over its
+    # 1 + 2 newton + backtracks assemblies), scaled by the number of Gauss points
+    per_asm = float(np.median([(s[0] - sum(s[3])) / (1.0 + 2.0 * s[1] + s[2]) for s in sample]))
+    t_asm48 = per_asm * (n / n_cpu) ** 3
+    inc48 = (1.0 + 2.0 * newton + backtracks) * t_asm48 + newton * lu48
     return {
-        "value": 1.0 / per_step,
-        "unit": "RVE solves/s (upper bound)",
+        "value": 1.0 / med,
+        "unit": f"RVE solves/s ({n_cpu}^3 sample of the {n}^3 workload)",
         "cores": 1,
+        "host_cores_available": len(os.sched_getaffinity(0)),
         "kind": "port",
-        "sample": (f"oracle FP64 assembly of the same {n}^3 RVE timed once ({t_asm:.2f} s); "
-                   f"x {assemblies:.2f} assemblies per increment at the GPU's Newton/Armijo "
-                   "counts; SuperLU factorisations excluded (lower bound on CPU time)"),
+        "sample": (f"oracle FP64 (reference algorithm, SuperLU) on the {n_cpu}^3 RVE of the same "
+                   f"generator, seed, phases, path and Newton/Armijo settings: increments 3-{2 + k} "
+                   f"timed in full, median {med:.2f} s +- MAD {mad:.2f} s, one core"),
+        "increment_seconds": inc.tolist(),
+        "newton_iterations_per_increment": newton,
+        "superlu_seconds_per_solve": {"8": lu8, str(n_cpu): lu},
+        "assembly_seconds": {str(n_cpu): per_asm},
+        "extrapolated_48": {"increment_seconds": inc48, "rve_solves_per_s": 1.0 / inc48,
+                            "assembly_seconds": t_asm48,
+                            "superlu_seconds_per_solve": lu48,
+                            "superlu_exponent_in_dofs": float(expo),
+                            "note": f"{n_cpu}^3 assembly time x Gauss-point ratio, at the sample's "
+                                    "Newton/Armijo counts, + Newton solves x the SuperLU time "
+                                    f"extrapolated from 8^3 and {n_cpu}^3 in the DOF count "
+                                    "(extrapolated, not timed)"},
     }
 
 
@@ -367,6 +424,14 @@ def macro_surrogate_section(args, ctx) -> dict:
         keep = peak >= 0.05 * np.max(peak)
         rms = SD.stress_rms_errors(sur[keep], ref[keep])
         mx = SD.stress_errors(sur[keep], ref[keep])
+        # the same error over the first K steps of the run (the network's drift grows with
+        # the number of its steps per point beyond the 200-step training paths)
+        horizon = {}
+        for K in (300, 1000):
+            if K < steps:
+                e_k = SD.stress_rms_errors(sur[keep][:, :K], ref[keep][:, :K])
+                horizon[str(K)] = {"rms_median": float(np.median(e_k)),
+                                   "rms_p95": float(np.percentile(e_k, 95))}
         print(f"macro surrogate: {1e3 * per_step:.2f} ms/step, {stats['gate_trips']} gate trips, "
               f"stress rms error median {np.median(rms):.4f} p95 {np.percentile(rms, 95):.4f}, "
               f"max error median {np.median(mx):.4f}", file=sys.stderr, flush=True)
@@ -451,6 +516,7 @@ def macro_surrogate_section(args, ctx) -> dict:
         "stress_rms_error_p95": float(np.percentile(rms, 95)),
         "stress_max_error_median": float(np.median(mx)),
         "stress_max_error_p95": float(np.percentile(mx, 95)),
+        "stress_rms_error_first_steps": horizon,
         "oracle_rve_solves_per_s": oracle_rate,
         "error_definition": "per probe Gauss point under the patch (peak stress >= 5 % of the largest): "
                             "rms / max over the run of |sigma_sur - sigma_FE2|_inf, divided by the probe's "
@@ -1029,7 +1095,7 @@ def main() -> int:
         c32.close()
     if not args.no_cpu_baseline:
         print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(n, args.seed, newton / K, backtracks / K)
+        line["cpu_baseline"] = cpu_baseline(n, args.seed)
     if not args.no_macro:
         print("macro explicit slab ...", file=sys.stderr, flush=True)
         line["macro_explicit"] = macro_explicit_section(args)
