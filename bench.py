@@ -565,6 +565,25 @@ def batched_models(n_rves: int, seed: int, precision: str, newton_rtol: float):
     return models, np.asarray(times), np.asarray(strains), edges
 
 
+def assemble_roofline(models, tuned) -> dict:
+    """Roofline of the batched Newton assembly (k_b_assemble, FP32 point data) from
+    the autotuner's HIP-event time of its all-active launch (every RVE forced active,
+    at the first increment).  Algorithmic bytes: per Gauss point its stress (6) and
+    tangent coefficients d, r (2) written; per mortar (J2) point its committed state
+    (8) read and its next state (8) written; per element its corner forces (24)
+    written, its corner fluctuation (3 FP64, one node per element on the periodic
+    grid) read and its phase byte.  The plastic-flow rows of the tangent (6 more per
+    plastic point) are not counted: the first increment is elastic."""
+    n_el = sum(int(np.asarray(m.element_phase_ids).size) for m in models)
+    n_j2 = sum(int(np.sum(np.asarray(m.element_phase_ids) == 0)) for m in models)
+    alg = 8 * n_el * 8 * 4 + 8 * n_j2 * 16 * 4 + n_el * (24 * 4 + 3 * 8 + 1)
+    us = float(tuned["assemble"]["us"])
+    gbs = alg / (us * 1e-6) / 1e9
+    return {"kernel": "k_b_assemble<float> (all RVEs active, autotuned grid)", "bound": "hbm",
+            "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "alg_bytes_per_launch": alg, "us": us, "mortar_element_fraction": n_j2 / n_el}
+
+
 def batched_section(args) -> dict:
     """Batched FE²: 256 uneven RVEs advanced together (FP32 element work, FP64
     residuals / dots / multiplier solve, Newton relative tolerance 1e-6; once from
@@ -634,6 +653,7 @@ def batched_section(args) -> dict:
         out[tag]["counters_timed"] = {k: cnt1[k] - cnt0[k] for k in cnt1}
         if tag == "fp32":
             out["autotuned_grids"] = batch.tuning()
+            out["assemble_roofline"] = assemble_roofline(models32, out["autotuned_grids"])
         del batch, st
         torch.cuda.empty_cache()
     den = np.maximum(np.max(np.abs(stresses["fp64"]), axis=2), 1e-300)

@@ -518,10 +518,6 @@ def diagnostics_dict(phases, point_phase_ids, diag: np.ndarray, has_rates: bool,
     active_row = diag[_lib.DIAG_NAMES.index("viscoplastic_active")]
     if not np.any(j2_points & (active_row > 0.5)):
         names.discard("viscoplastic_hv")
-        if phased and np.any(j2_points):
-            pass
-    if not phased and "viscoplastic_hv" in names:
-        pass
     ordered = sorted(names) if phased else [k for k in _lib.DIAG_NAMES if k in names]
     return {k: diag[_lib.DIAG_NAMES.index(k)].copy() for k in ordered}
 
