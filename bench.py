@@ -37,6 +37,10 @@ KERNEL_NAMES = {"pcg_elem": "k_sp_elem", "pcg_node": "k_sp_node", "pcg_update": 
                 "precond": "spectral preconditioner slot: k_dft_strided (forward y) + k_sp_zcol + "
                            "k_dft_strided (inverse y) + k_sp_xinv",
                 "gather": "k_gather + k_finalize"}
+# RVE edges of 16, 32, 48 and 64 run the row-lane spectral passes (same bytes per class)
+KERNEL_NAMES_ROW_LANE = dict(KERNEL_NAMES, pcg_update="k_sp_xlines_fwd",
+                             precond="spectral preconditioner slot: k_sp_ylines (forward y) + "
+                                     "k_sp_zlines + k_sp_ylines (inverse y) + k_sp_xlines_inv")
 
 # HBM bytes per launch of a kernel from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # this script (--copy-calibration), each counter scaled by the ratio of the bytes a plain
@@ -1000,8 +1004,8 @@ def main() -> int:
     #  assemble: committed state read + trial state written (J2 points), stress and
     #            compact tangent written, w read once per class, force rows written.
     #  precond: one application of the spectral preconditioner after the x-DFT
-    #            (k_dft_strided forward y, k_sp_zcol, k_dft_strided inverse y,
-    #            k_sp_xinv): the three components' spectra (16 B each) read and
+    #            (forward y pass, z pass with the symbol solve, inverse y pass,
+    #            inverse x pass): the three components' spectra (16 B each) read and
     #            written by each y pass (96 B per class), by the z-column pass
     #            (96 B per mode) together with the FP32 symbol (48 B per mode),
     #            then read once more by the inverse x pass, which writes the real z
@@ -1033,7 +1037,7 @@ def main() -> int:
     ach = kernels[dom]["achieved_GBs"]
     traffic = MEASURED_TRAFFIC.get((dom, args.size, args.precision))
     roofline = {
-        "kernel": KERNEL_NAMES[dom],
+        "kernel": (KERNEL_NAMES_ROW_LANE if n % 16 == 0 and n <= 64 else KERNEL_NAMES)[dom],
         "bound": "hbm",
         "achieved": ach,
         "peak": HBM_PEAK_GBS,

@@ -37,6 +37,7 @@
 #include "dft.hpp"
 #include <hip/hip_complex.h>
 #include "hex8.hpp"
+#include "lanefft.hpp"
 #include "spectral.hpp"
 #include "symbol.hpp"
 
@@ -791,6 +792,313 @@ __global__ void __launch_bounds__(kBlock)
   xlines_inv(nx, l0, nl, S, C, zout, ax, dft_sm, wx);
 }
 
+// Row-lane passes (lanefft.hpp) for y and z edges of 16, 32, 48 or 64: a 16-lane row
+// holds one line of one x mode of one component, a wave the lines of 4 consecutive
+// x modes of one component, and a block of kLaneBlock threads (6 waves) those of 8
+// x modes (128-byte rows per load) for the three components.  No LDS staging and
+// no barrier inside a transform.
+constexpr int kLaneWaves = 6;
+constexpr int kLaneBlock = 64 * kLaneWaves;
+constexpr int kLaneKx = 8;
+
+__host__ __device__ inline int lane_tiles_x(int nx) { return (nx + kLaneKx - 1) / kLaneKx; }
+constexpr int kZlineKx = 16;  // x modes per block of the z pass
+
+// Position of this thread in a row-lane block: component, x mode, lane in the row.
+struct LanePos {
+  int m, kx, l;
+};
+
+__device__ __forceinline__ LanePos lane_pos(int xt) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  return {wave % 3, xt * kLaneKx + 4 * (wave / 3) + (lane >> 4), lane & 15};
+}
+
+// y pass (INV: inverse).  The forward pass stores its modes in natural order (lane l,
+// register k1 -> mode k1 + M bitrev4(l)); the inverse pass reads them back the same
+// way.  The forward launch has one block more: it sums the update partials of
+// k_sp_xfwd once into sc (r.r, sum r per component) and sets the PCG tolerance on the
+// first application.
+template <int M, bool INV>
+__global__ void __launch_bounds__(kLaneBlock)
+    k_sp_ylines(int nx, int ny, int nz, int64_t C, DftAxis ay, double2* __restrict__ S,
+                Scalars* __restrict__ sc, const double* __restrict__ part_upd, int P, int first,
+                double rtol, double atol) {
+  const int tiles_x = lane_tiles_x(nx);
+  const int nb = tiles_x * nz;
+  if (!INV && (int)blockIdx.x == nb) {
+    if (sc->done) return;
+    __shared__ double ws[kLaneWaves][4];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < P; b += kLaneBlock) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) a[s] += part_upd[s * P + b];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double w = wave_sum(a[s]);
+      if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][s] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double u[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kLaneWaves; ++w) t += ws[w][s];
+        u[s] = t;
+      }
+      sc->rr = u[0];
+      sc->sumr[0] = u[1];
+      sc->sumr[1] = u[2];
+      sc->sumr[2] = u[3];
+      if (first) {
+        const double tol = fmax(rtol * sqrt(u[0]), atol);
+        sc->tol2 = tol * tol;
+      }
+    }
+    return;
+  }
+  const int blk = (int)blockIdx.x;
+  const LanePos lp = lane_pos(blk % tiles_x);
+  const int z = blk / tiles_x, l = lp.l;
+  const bool on = lp.kx < nx;
+  double2* line = S + lp.m * C + lp.kx + (int64_t)nx * ny * z;
+  const int br = bitrev4(l);
+  double2 v[1][M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const int y = INV ? j + M * br : l + 16 * j;
+    v[0][j] = on ? line[(int64_t)nx * y] : make_double2(0.0, 0.0);
+  }
+  LaneTw<M> tw;
+  lane_twiddles<M>(ay, l, tw);
+  // the converged flag is read after the loads (one memory round trip)
+  if (sc->done) return;
+  if (INV)
+    lane_dft_inv<M, 1>(v, tw, l);
+  else
+    lane_dft_fwd<M, 1>(v, tw, l);
+  if (on) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int y = INV ? l + 16 * j : j + M * br;
+      line[(int64_t)nx * y] = v[0][j];
+    }
+  }
+}
+
+// Forward x pass with the PCG vector update for x edges of 16, 32, 48 or 64 (the
+// row-lane form of k_sp_xfwd): a 16-lane row holds one (y, z) line for the three
+// components (24 contiguous bytes per lane and position), a block of 256 threads
+// 16 lines.  INIT: r = b, x = 0, p_old = 0; otherwise alpha from the p.y partials,
+// x += alpha p, r -= alpha y.  Then the x-DFT of r into S (modes in natural order)
+// and the block partials (r.r, sum r per component).  Every vector load is issued
+// before the wait for the solver's flag and partials.
+template <int M, bool INIT>
+__global__ void __launch_bounds__(256)
+    k_sp_xlines_fwd(int nx, int64_t n_lines, DftAxis ax, const double* __restrict__ b,
+                    const double* __restrict__ p, const double* __restrict__ y,
+                    double* __restrict__ x, double* __restrict__ r, double* __restrict__ p_old,
+                    double2* __restrict__ S, int64_t C, const double* __restrict__ part_pq, int P,
+                    double* __restrict__ part_upd, Scalars* __restrict__ sc) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t line = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int l = threadIdx.x & 15;
+  const bool on = line < n_lines;
+  const int64_t c0 = line * nx;
+  double a0[M][3], a1[M][3], a2[M][3], a3[M][3];  // INIT: b; otherwise p, y, r, x
+#pragma unroll
+  for (int j = 0; j < M; ++j)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * (c0 + l + 16 * j) + m;
+      if (INIT) {
+        a0[j][m] = on ? b[d] : 0.0;
+      } else {
+        a0[j][m] = on ? p[d] : 0.0;
+        a1[j][m] = on ? y[d] : 0.0;
+        a2[j][m] = on ? r[d] : 0.0;
+        a3[j][m] = on ? x[d] : 0.0;
+      }
+    }
+  LaneTw<M> tw;
+  lane_twiddles<M>(ax, l, tw);
+  double alpha = 0.0;
+  int it = 0;
+  if (!INIT) {
+    if (sc->done) return;
+    double pq[1];
+    reduce_partials<1>(part_pq, P, pq, lds);
+    if (!(pq[0] > 0.0) || !isfinite(pq[0])) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
+      return;
+    }
+    it = sc->it_cur;
+    alpha = sc->rzr[it & 1] / pq[0];
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double2 v[3][M];
+#pragma unroll
+  for (int j = 0; j < M; ++j)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t d = 3 * (c0 + l + 16 * j) + m;
+      double rv;
+      if (INIT) {
+        rv = a0[j][m];
+        if (on) {
+          x[d] = 0.0;
+          p_old[d] = 0.0;
+        }
+      } else {
+        rv = a2[j][m] - alpha * a1[j][m];
+        if (on) x[d] = a3[j][m] + alpha * a0[j][m];
+      }
+      if (on) r[d] = rv;
+      acc[0] += rv * rv;
+      acc[1 + m] += rv;
+      v[m][j] = make_double2(rv, 0.0);
+    }
+  lane_dft_fwd<M, 3>(v, tw, l);
+  if (on) {
+    const int br = bitrev4(l);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int k = 0; k < M; ++k) S[m * C + c0 + k + M * br] = v[m][k];
+  }
+  write_partials<4>(acc, part_upd, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (INIT) {
+      sc->done = 0;
+      sc->iters = 0;
+    } else {
+      sc->iters = it + 1;
+      sc->alpha = alpha;
+    }
+  }
+}
+
+// Inverse x pass for x edges of 16, 32, 48 or 64: a 16-lane row holds one (y, z) line
+// for the three components, a block of 256 threads 16 lines; z = Re(inverse x-DFT),
+// written interleaved per class (24 contiguous bytes per lane and position).
+template <int M>
+__global__ void __launch_bounds__(256)
+    k_sp_xlines_inv(int nx, int64_t n_lines, int64_t C, DftAxis ax, const double2* __restrict__ S,
+                    double* __restrict__ zout, const Scalars* __restrict__ sc) {
+  const int64_t line = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+  const int l = threadIdx.x & 15;
+  const bool on = line < n_lines;
+  const int br = bitrev4(l);
+  const int64_t c0 = line * nx;
+  double2 v[3][M];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int j = 0; j < M; ++j) v[m][j] = on ? S[m * C + c0 + j + M * br] : make_double2(0.0, 0.0);
+  LaneTw<M> tw;
+  lane_twiddles<M>(ax, l, tw);
+  // the converged flag is read after the loads (one memory round trip)
+  if (sc->done) return;
+  lane_dft_inv<M, 3>(v, tw, l);
+  if (on) {
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) zout[3 * (c0 + l + 16 * j) + m] = v[m][j].x;
+  }
+}
+
+// z pass with the symbol solve: convergence test on the sums left by the forward y
+// pass, forward z-DFT, Z = Khat^-1 (R - sum r) / N per mode (the lane's modes
+// k1 + M bitrev4(l)), r.z by Parseval, inverse z-DFT, in place; one r.z partial per
+// block.  Here a 16-lane row holds one column of one x mode for all three
+// components (the symbol solve stays in registers), a wave 4 consecutive x modes
+// and a block of 16 KX threads KX x modes.
+template <int M, int KX>
+__global__ void __launch_bounds__(16 * KX)
+    k_sp_zlines(int nx, int ny, int nz, int64_t C, double inv_n, DftAxis az,
+                const hipFloatComplex* __restrict__ kinv, double2* __restrict__ S,
+                double* __restrict__ part_rz, Scalars* __restrict__ sc) {
+  constexpr int NW = KX / 4;
+  __shared__ double ws[NW];
+  const int tiles_x = (nx + KX - 1) / KX;
+  const int blk = (int)blockIdx.x;
+  const int xt = blk % tiles_x, ky = blk / tiles_x;
+  const int l = threadIdx.x & 15;
+  const int kx = xt * KX + (threadIdx.x >> 4);
+  const bool on = kx < nx;
+  const int64_t plane = (int64_t)nx * ny;
+  const int64_t col = kx + (int64_t)nx * ky;
+  const int br = bitrev4(l);
+  // every global load is issued before the wait for the PCG scalars
+  double2 v[3][M];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+      v[m][j] = on ? S[m * C + col + plane * (l + 16 * j)] : make_double2(0.0, 0.0);
+  float4 q[M][3];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const float4* src = reinterpret_cast<const float4*>(kinv + 6 * (col + plane * (k + M * br)));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[k][c] = on ? src[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  LaneTw<M> tw;
+  lane_twiddles<M>(az, l, tw);
+  // the solver's flag and scalars are read after the loads (one memory round trip)
+  if (sc->done) return;
+  const double rr = sc->rr, tol2 = sc->tol2;
+  if (rr <= tol2 || !isfinite(rr)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = isfinite(rr) ? 1 : 2;
+    return;
+  }
+  const double sr[3] = {sc->sumr[0], sc->sumr[1], sc->sumr[2]};
+  lane_dft_fwd<M, 3>(v, tw, l);
+  double rz = 0.0;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const hipFloatComplex qq[6] = {
+        make_hipFloatComplex(q[k][0].x, q[k][0].y), make_hipFloatComplex(q[k][0].z, q[k][0].w),
+        make_hipFloatComplex(q[k][1].x, q[k][1].y), make_hipFloatComplex(q[k][1].z, q[k][1].w),
+        make_hipFloatComplex(q[k][2].x, q[k][2].y), make_hipFloatComplex(q[k][2].z, q[k][2].w)};
+    double ar[3], ai[3], zr[3], zi[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      ar[m] = v[m][k].x - sr[m];
+      ai[m] = v[m][k].y;
+    }
+    spec_solve(qq, false, ar, ai, zr, zi);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      zr[m] *= inv_n;
+      zi[m] *= inv_n;
+      rz += ar[m] * zr[m] + ai[m] * zi[m];
+      v[m][k] = make_double2(zr[m], zi[m]);
+    }
+  }
+  lane_dft_inv<M, 3>(v, tw, l);
+  if (on) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < M; ++j) S[m * C + col + plane * (l + 16 * j)] = v[m][j];
+  }
+  // out-of-range lanes hold zero symbols and add nothing
+  const double w = wave_sum(rz);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < NW; ++u) t += ws[u];
+    part_rz[blockIdx.x] = t;
+  }
+}
+
 // y = gather(F) (class 0 zeroed); p.y partials.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -1428,10 +1736,20 @@ struct Ops {
     const int nx = R->nx, ny = R->ny, nz = R->nz;
     const int64_t n_lines = (int64_t)ny * nz;
     const unsigned g_xl = (unsigned)((n_lines + xline_group(nx) - 1) / xline_group(nx));
-    // z-column tiles: narrower tiles until the pass has enough workgroups
+    // y and z edges of 16, 32, 48 or 64 take the row-lane passes; other edges the
+    // LDS passes (z-column tiles narrowed until the pass has enough workgroups)
+    const int My = (ny % 16 == 0 && ny <= 64) ? ny / 16 : 0;
+    const int Mz = (nz % 16 == 0 && nz <= 64) ? nz / 16 : 0;
+    const int Mx = (nx % 16 == 0 && nx <= 64) ? nx / 16 : 0;
+    const bool lanes = My > 0 && Mz > 0;
+    const bool xrow = lanes && Mx > 0;  // x passes in row-lane form too
+    const unsigned g_xr = (unsigned)((n_lines + 15) / 16);
+    const int p_upd = xrow ? (int)g_xr : (int)g_xl;  // update partials per row
     int tile_w = kZTile;
     while (tile_w > 2 && (int64_t)((nx + tile_w - 1) / tile_w) * ny < 512) tile_w /= 2;
-    const unsigned g_zc = (unsigned)(((nx + tile_w - 1) / tile_w) * ny);
+    const unsigned g_zc = lanes ? (unsigned)(((nx + kZlineKx - 1) / kZlineKx) * ny)
+                                : (unsigned)(((nx + tile_w - 1) / tile_w) * ny);
+    const unsigned g_yl = (unsigned)(lane_tiles_x(nx) * nz);
     const double inv_n = 1.0 / (double)C;
     // block partials: p.y (node pass), (r.r, sum r) (x-line pass), r.z (z-column pass)
     double* part_pq = R->partials;
@@ -1442,7 +1760,49 @@ struct Ops {
     const dim3 g_y((unsigned)((nx + kDftTile - 1) / kDftTile), (unsigned)nz, 3);
     double2* S = (double2*)R->uab;
     const int32_t* dflag = &R->sc->done;
+    const double atol = R->lin.absolute_tolerance;
+    auto lane_passes = [&](auto my, auto mz, hipStream_t s, int first) {
+      constexpr int MY = decltype(my)::value, MZ = decltype(mz)::value;
+      hipLaunchKernelGGL((k_sp_ylines<MY, false>), dim3(g_yl + 1), dim3(kLaneBlock), 0, s, nx, ny,
+                         nz, C, R->ay, S, R->sc, (const double*)part_upd, p_upd, first, rtol,
+                         atol);
+      hipLaunchKernelGGL((k_sp_zlines<MZ, kZlineKx>), dim3(g_zc), dim3(16 * kZlineKx), 0, s, nx,
+                         ny, nz, C,
+                         inv_n, R->az, (const hipFloatComplex*)R->kinv, S, part_rz, R->sc);
+      hipLaunchKernelGGL((k_sp_ylines<MY, true>), dim3(g_yl), dim3(kLaneBlock), 0, s, nx, ny, nz,
+                         C, R->ay, S, R->sc, (const double*)nullptr, 0, 0, rtol, atol);
+    };
+    auto lane_z = [&](auto my, hipStream_t s, int first) {
+      switch (Mz) {
+        case 1: lane_passes(my, std::integral_constant<int, 1>{}, s, first); break;
+        case 2: lane_passes(my, std::integral_constant<int, 2>{}, s, first); break;
+        case 3: lane_passes(my, std::integral_constant<int, 3>{}, s, first); break;
+        default: lane_passes(my, std::integral_constant<int, 4>{}, s, first); break;
+      }
+    };
     auto precond = [&](hipStream_t s, int first) {
+      if (lanes) {
+        switch (My) {
+          case 1: lane_z(std::integral_constant<int, 1>{}, s, first); break;
+          case 2: lane_z(std::integral_constant<int, 2>{}, s, first); break;
+          case 3: lane_z(std::integral_constant<int, 3>{}, s, first); break;
+          default: lane_z(std::integral_constant<int, 4>{}, s, first); break;
+        }
+        switch (xrow ? Mx : 0) {
+          case 1: hipLaunchKernelGGL(k_sp_xlines_inv<1>, dim3(g_xr), dim3(256), 0, s, nx, n_lines, C,
+                                     R->ax, (const double2*)S, R->pz, (const Scalars*)R->sc); break;
+          case 2: hipLaunchKernelGGL(k_sp_xlines_inv<2>, dim3(g_xr), dim3(256), 0, s, nx, n_lines, C,
+                                     R->ax, (const double2*)S, R->pz, (const Scalars*)R->sc); break;
+          case 3: hipLaunchKernelGGL(k_sp_xlines_inv<3>, dim3(g_xr), dim3(256), 0, s, nx, n_lines, C,
+                                     R->ax, (const double2*)S, R->pz, (const Scalars*)R->sc); break;
+          case 4: hipLaunchKernelGGL(k_sp_xlines_inv<4>, dim3(g_xr), dim3(256), 0, s, nx, n_lines, C,
+                                     R->ax, (const double2*)S, R->pz, (const Scalars*)R->sc); break;
+          default:
+            hipLaunchKernelGGL(k_sp_xinv, dim3(g_xl), dim3(kBlock), lds_xl, s, nx, n_lines, R->ax,
+                               (const double2*)S, C, R->pz, (const Scalars*)R->sc);
+        }
+        return;
+      }
       hipLaunchKernelGGL((k_dft_strided<false, false>), g_y, dim3(256), lds_y, s, nx, ny, nz, C,
                          R->ay, S, dflag);
       hipLaunchKernelGGL(k_sp_zcol, dim3(g_zc), dim3(kBlock), lds_zc, s, nx, ny, nz, tile_w, inv_n,
@@ -1453,10 +1813,29 @@ struct Ops {
       hipLaunchKernelGGL(k_sp_xinv, dim3(g_xl), dim3(kBlock), lds_xl, s, nx, n_lines, R->ax,
                          (const double2*)S, C, R->pz, (const Scalars*)R->sc);
     };
+    // forward x pass with the vector update (INIT: r = b, x = 0, p_old = 0)
+    auto xfwd = [&](auto init, hipStream_t s, const double* p_new, double* p_old) {
+      constexpr bool I = decltype(init)::value;
+      const double* bb = I ? b : nullptr;
+      const double* yy = I ? nullptr : (const double*)R->py;
+#define CIA_XROW(MM)                                                                            \
+  hipLaunchKernelGGL((k_sp_xlines_fwd<MM, I>), dim3(g_xr), dim3(256), 0, s, nx, n_lines, R->ax,  \
+                     bb, p_new, yy, x, R->pr, p_old, S, C, (const double*)part_pq, (int)gn,      \
+                     part_upd, R->sc)
+      switch (xrow ? Mx : 0) {
+        case 1: CIA_XROW(1); break;
+        case 2: CIA_XROW(2); break;
+        case 3: CIA_XROW(3); break;
+        case 4: CIA_XROW(4); break;
+        default:
+          hipLaunchKernelGGL(k_sp_xfwd<I>, dim3(g_xl), dim3(kBlock), lds_xl, s, nx, n_lines, R->ax,
+                             bb, p_new, yy, x, R->pr, p_old, S, C, (const double*)part_pq, (int)gn,
+                             part_upd, R->sc);
+      }
+#undef CIA_XROW
+    };
     prof::begin(prof::PRECOND, st);
-    hipLaunchKernelGGL(k_sp_xfwd<true>, dim3(g_xl), dim3(kBlock), lds_xl, st, nx, n_lines, R->ax, b,
-                       (const double*)nullptr, (const double*)nullptr, x, R->pr, R->pp[1], S, C,
-                       (const double*)part_pq, (int)gn, part_upd, R->sc);
+    xfwd(std::true_type{}, st, (const double*)nullptr, R->pp[1]);
     precond(st, 1);
     prof::end(st);
     CIA_HIP(hipGetLastError());
@@ -1477,10 +1856,7 @@ struct Ops {
                          R->py, part_pq, R->sc);
       prof::end(s);
       prof::begin(prof::PCG_UPDATE, s, tag);
-      hipLaunchKernelGGL(k_sp_xfwd<false>, dim3(g_xl), dim3(kBlock), lds_xl, s, nx, n_lines, R->ax,
-                         (const double*)nullptr, (const double*)p_new, (const double*)R->py, x,
-                         R->pr, (double*)nullptr, S, C, (const double*)part_pq, (int)gn, part_upd,
-                         R->sc);
+      xfwd(std::false_type{}, s, (const double*)p_new, (double*)nullptr);
       prof::end(s);
       prof::begin(prof::PRECOND, s, tag);
       precond(s, 0);

@@ -23,7 +23,10 @@
 // Five launches per application (with the update) instead of eight: the single
 // RVE's PCG iteration is bound by launch latency, not by bytes.  The 1-D
 // transforms are dft_stage (one Cooley-Tukey level, twiddles in LDS); edges up to
-// 64 elements.
+// 64 elements.  Grids whose edges are all 16, 32, 48 or 64 (y and z edges for the
+// y and z passes) run the row-lane forms of these passes instead (k_sp_xlines_fwd,
+// k_sp_ylines, k_sp_zlines, k_sp_xlines_inv: lanefft.hpp transforms in registers,
+// no LDS staging).
 #pragma once
 
 #include <hip/hip_complex.h>

@@ -36,6 +36,19 @@ CASES = {
     "mortar_soft_m12": ((4, 3, 3), (1.0, 1.0, 1.0), [0, 1, 0, 0], lambda: [mortar(1.0), soft(2.0)]),
 }
 
+# Device-only cases whose y and z edges (16, 32, 48, 64) take the single-RVE solver's
+# row-lane spectral passes (all three edges for the last: the x passes too); held to
+# the laminate recurrence alone (their SuperLU oracle path would take minutes).
+LANE_CASES = {
+    "lanes_mortar_aggregate_m1_16x48": ((5, 16, 48), (1.0, 0.8, 1.2), [0, 1, 1, 0, 0],
+                                        lambda: [mortar(1.0), aggregate()]),
+    "lanes_mortar_soft_m12_32x64": ((4, 32, 64), (1.0, 1.0, 1.0), [0, 1, 0, 0],
+                                    lambda: [mortar(1.0), soft(2.0)]),
+    "lanes_mortar_aggregate_m2_16x16x32": ((16, 16, 32), (1.0, 0.9, 1.1),
+                                           [0, 0, 1, 0, 1, 1, 0, 1, 0, 0, 1, 0, 1, 1, 0, 1],
+                                           lambda: [mortar(2.0), aggregate()]),
+}
+
 
 def load_unload_path(seed=11, steps=20, peak=3.0e-3, rate=100.0):
     """Times (steps+1) and macro strains (steps+1, 6): loading over 12 steps to
@@ -51,7 +64,7 @@ def load_unload_path(seed=11, steps=20, peak=3.0e-3, rate=100.0):
 
 
 def layer_setup(name):
-    div, lengths, layers, factory = CASES[name]
+    div, lengths, layers, factory = CASES[name] if name in CASES else LANE_CASES[name]
     phases = factory()
     layers = np.asarray(layers)
     fractions = np.asarray([np.count_nonzero(layers == p) for p in range(len(phases))], float) / div[0]

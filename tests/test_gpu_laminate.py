@@ -109,6 +109,30 @@ def test_rve_path_equals_laminate_recurrence(gpu, name):
     assert max(r["diagnostics"]["viscoplastic_active_volume_fraction"] for r in opath) > 0.3
 
 
+@pytest.mark.parametrize("name", sorted(LC.LANE_CASES))
+def test_row_lane_spectral_passes_equal_laminate_recurrence(gpu, name):
+    """y and z edges of 16, 32, 48 and 64 run the single-RVE solver's row-lane DFT
+    passes (16-lane DPP transforms, symbol solve in registers): the plastic laminate
+    path still equals the per-phase recurrence at 1e-9."""
+    from concrete_impact_ai_amd import rve as R
+
+    model, _, phases, fractions, eph = _gpu_model(name)
+    times, strains = LC.load_unload_path()
+    lam = OL.laminate_path(phases, fractions, times, strains, SET)
+    pid = LC.point_phase_rows(eph)
+    steps = list(R.iterate_prescribed_rve_path(model, times, strains, _settings(),
+                                               requirements=R.MaterialResponseRequirements(tangent=False)))
+    assert len(steps) == len(times) - 1
+    for s in steps:
+        ref = lam[s.step_id - 1]
+        resp = s.response
+        assert _rel(resp.macro_stress, ref["macro_stress"]) < 1e-9, s.step_id
+        assert _rel(resp.micro_strains, ref["phase_strain"][pid]) < 1e-9, s.step_id
+        assert _rel(resp.micro_stresses, ref["phase_stress"][pid]) < 1e-9, s.step_id
+        _check_state(resp.state.material_state.variables, phases, ref, 1e-9)
+    assert max(s.response.diagnostics["viscoplastic_active_volume_fraction"] for s in steps) > 0.3
+
+
 def test_batched_rves_equal_laminate_recurrences(gpu):
     """Every laminate case in one batch, each on its own path."""
     from concrete_impact_ai_amd import batch as B

@@ -103,6 +103,7 @@ def _rel(a, b):
 @pytest.mark.parametrize("n,m,linear", [
     ((4, 4, 4), 1.0, "spectral"), ((5, 3, 4), 1.0, "spectral"), ((4, 4, 4), 2.0, "spectral"),
     ((4, 4, 4), 1.0, "block_jacobi"), ((6, 5, 4), 1.0, "inexact"), ((7, 4, 5), 2.0, "inexact"),
+    ((3, 16, 16), 1.0, "spectral"), ((2, 48, 32), 2.0, "inexact"),
 ])
 def test_two_phase_rve_path_matches_oracle(gpu, n, m, linear):
     from concrete_impact_ai_amd import rve as R
@@ -193,7 +194,8 @@ def test_fp32_refined_rve_error_against_fp64_oracle(gpu):
 
 
 @pytest.mark.parametrize("n", [(2, 3, 4), (5, 5, 5), (6, 7, 8), (12, 13, 16), (17, 24, 31), (32, 9, 11),
-                               (48, 16, 24), (12, 48, 32)])
+                               (48, 16, 24), (12, 48, 32), (16, 16, 16), (5, 64, 48), (3, 32, 64),
+                               (48, 48, 48)])
 def test_spectral_preconditioner_is_exact_for_the_reference_medium(gpu, n):
     """On a homogeneous elastic RVE whose material is the reference medium the spectral
     preconditioner is the exact inverse of the stiffness (up to its FP32 storage).
