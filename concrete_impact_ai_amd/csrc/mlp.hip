@@ -905,10 +905,7 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int c = 0; c < 7; ++c) x[c] = f_nx[c];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          x[7 + c] = tanhf(z[c]);  // the network sees the latent through tanh
-          a.q[(mrow + tid) * 12 + 6 + c] = x[7 + c];
-        }
+        for (int c = 0; c < 6; ++c) x[7 + c] = tanhf(z[c]);  // the latent through tanh
         // this step's strain and target, the next step's features: consumed after
         // the layers
         const float* e = a.eps + (prow + t) * 6;
@@ -923,6 +920,10 @@ __global__ void __launch_bounds__(kTrainThreads) __attribute__((amdgpu_waves_per
           for (int c = 0; c < 7; ++c) f_nx[c] = a.feat[(prow + t + 1) * 8 + c];
         }
       }
+      // tanh(z) for the backward pass, zeros on rows past the batch (the backward
+      // kernel reads every row of the tile)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.q[(mrow + tid) * 12 + 6 + c] = x[7 + c];
       __bf16* row = act[0] + tid * TLD;
 #pragma unroll
       for (int c = 0; c < 16; ++c) {

@@ -113,7 +113,7 @@ def production_strain_path(family, direction, amplitude, target_rate, n_steps, s
 
     ``direction`` is a name of ``TENSOR_DIRECTIONS`` or a 6-vector; it is normalised
     and perturbed by ``perturbation`` times a seeded unit direction orthogonal to it
-    (in the engineering-tensor inner product).  ``family`` is one of
+    (in the engineering-tensor inner product; drawn first, whatever the magnitude).  ``family`` is one of
     ``FAMILY_CONTROLS`` (scalar knots along the direction) or ``nonproportional``
     (knots 0, d, d + 0.75 r with r the direction turned by ``turning_angle`` towards
     a seeded orthogonal one).  The knots are scaled to unit peak norm, interpolated
@@ -130,9 +130,10 @@ def production_strain_path(family, direction, amplitude, target_rate, n_steps, s
         o -= np.sum(o * d * TENSOR_WEIGHTS) * d
         return o / tensor_norm(o)
 
-    if perturbation:
-        d = d + perturbation * orthogonal()
-        d = d / tensor_norm(d)
+    # the perturbation direction is drawn even at zero magnitude, so the seeded
+    # stream (and a nonproportional path's turn) does not depend on it
+    d = d + perturbation * orthogonal()
+    d = d / tensor_norm(d)
     if family == "nonproportional":
         if turning_angle is None:
             raise ValueError("a nonproportional path needs a turning angle")

@@ -261,7 +261,9 @@ def test_fused_bptt_matches_oracle_and_layerwise(gpu):
 def test_fused_bptt_ragged_batch_window_and_path_weights(gpu):
     """A batch of 37 rows (not a multiple of the 16-row tile) drawn with repeats from 20
     paths of 9 steps, a window of 6 steps and unequal path weights: loss and gradients
-    equal the oracle's path-weighted loss_and_grad on the gathered windows."""
+    equal the oracle's path-weighted loss_and_grad on the gathered windows, with the
+    workspace taken from NaN-filled memory (the tile's rows past the batch are read by
+    the backward kernel)."""
     from concrete_impact_ai_amd import surrogate as S
 
     rng = np.random.default_rng(4)
@@ -276,6 +278,13 @@ def test_fused_bptt_ragged_batch_window_and_path_weights(gpu):
     loss_ref, g_ref = OM.loss_and_grad(layers, d_eps[idx, :L], rate[idx, :L], target[idx, :L], Cs,
                                        path_weight=pw[idx])
     params = S.SurrogateParams(layers)
+    # the workspace comes from blocks filled with NaN bit patterns: rows past the
+    # batch must be written by the kernels before anything reads them
+    from concrete_impact_ai_amd import _lib
+
+    ws = int(_lib.lib().cia_surrogate_bptt_workspace(len(idx), L))
+    for _ in range(4):
+        torch.full((ws,), 255, dtype=torch.uint8, device="cuda")
     loss, gw, gb = _fused_bptt(params, d_eps, rate, target, Cs, idx, L, pw)
     rw = np.concatenate([g[0].ravel() for g in g_ref])
     rb = np.concatenate([g[1].ravel() for g in g_ref])
