@@ -47,7 +47,7 @@ KERNEL_NAMES_ROW_LANE = dict(KERNEL_NAMES, pcg_update="k_sp_xlines_fwd",
 # copy kernel of known size moves to what the same counter reads for it.  Keyed by
 # (slot, RVE edge, precision); absent keys report null.  Source summary:
 # profiles/k_sp_elem_rve48_fp64_hbm_traffic.md, profiles/spectral_precond_rve48_fp64_hbm_traffic.md
-MEASURED_TRAFFIC: dict = {("pcg_elem", 48, "fp64"): 77_042_165, ("precond", 48, "fp64"): 46_298_877}
+MEASURED_TRAFFIC: dict = {("pcg_elem", 48, "fp64"): 77_042_165, ("precond", 48, "fp64"): 45_494_701}
 
 
 def copy_calibration(torch) -> None:
@@ -195,6 +195,155 @@ def cpu_baseline(n: int, seed: int, n_cpu: int = 12, k: int = 3) -> dict:
                                     f"extrapolated from 8^3 and {n_cpu}^3 in the DOF count "
                                     "(extrapolated, not timed)"},
     }
+
+
+def plan_section(args) -> dict:
+    """The reference's fixed-cell training-data plan produced on the device: the 2000
+    tasks of ``configs/experiments/rve_rno_c48_v2_2000.yaml`` (the copy under
+    tests/golden/configs), their production strain paths (64-192 increments), the macro
+    responses from the batched FP64 FE² engine on a ``--plan-rve``^3 voxel cell with
+    the plan's phases (tasks grouped by increment count, one batched solve per
+    increment; a structured stand-in for the reference's O-grid cell)."""
+    import torch
+
+    from concrete_impact_ai_amd import data_plan as DP
+
+    plan = DP.load_rve_data_plan(Path(__file__).resolve().parent / "tests" / "golden" / "configs"
+                                 / "rve_rno_c48_v2_2000.yaml")
+    src = plan.sources[0]
+    tasks = DP.build_rve_data_task_manifest(plan)
+    model, eph = DP.fixed_cell_voxel_model(src, args.plan_rve)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = DP.produce_plan_responses(src, tasks, model)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    solves = sum(t.accepted_increment_count for t in tasks)
+    act = np.asarray([np.max(out[t.task_id]["viscoplastic_active_volume_fraction"]) for t in tasks])
+    return {"workload": f"{len(tasks)} plan tasks (5 families x 5 amplitude x 4 rate bands x 20), "
+                        f"64-192 increments, FP64 batched FE2 on a {args.plan_rve}^3 voxel cell",
+            "paths": len(tasks), "rve_solves": solves, "seconds": el, "rve_solves_per_s": solves / el,
+            "inclusion_fraction": float(np.mean(eph == 1)),
+            "yielding_paths": int(np.sum(act > 0.0)),
+            "splits": {k: sum(t.preassigned_split == k for t in tasks)
+                       for k in ("train", "validation", "test")}}
+
+
+def c1_section() -> dict:
+    """The plain-method RVE on the device: the 8^3 homogeneous J2 Perzyna RVE in SI
+    units (E 30 GPa, nu 0.2, sigma_y 30 MPa, H 1 GPa, eta 1e-3 s, m 1), uniaxial strain
+    ramped to 2e-3 at 100 1/s over 100 steps through ``iterate_prescribed_rve_path``,
+    Newton relative tolerance 1e-10 with the absolute tolerance scaled to the forces.
+    One untimed pass, then the whole path timed (median of three)."""
+    import torch
+
+    from concrete_impact_ai_amd import materials as M
+    from concrete_impact_ai_amd import rve as R
+    from concrete_impact_ai_amd.materials import MaterialResponseRequirements
+    from concrete_impact_ai_amd.solvers import ArmijoSettings, NonlinearNewtonSettings
+
+    E, nu, sy, H, tau = 30.0e9, 0.2, 30.0e6, 1.0e9, 1.0e-3
+    mat = M.J2ViscoplasticMaterial("c1_matrix", 2400.0, E, nu, sy, H, tau, sy, 1.0)
+    div = (8, 8, 8)
+    nodes, elements = R.build_structured_hex8_box((1.0, 1.0, 1.0), div)
+    newton = NonlinearNewtonSettings(20, 1e-10 * sy, 1e-10, 1e-12, 1e-10, ArmijoSettings(True))
+    model = R.StructuredHex8RVE(nodes, elements, div, mat,
+                                R.build_structured_hex8_periodic_constraint(nodes, div), newton)
+    steps, dt = 100, 2.0e-7
+    times = dt * np.arange(steps + 1)
+    strains = np.zeros((steps + 1, 6))
+    strains[:, 0] = 2.0e-3 * np.arange(steps + 1) / steps
+    settings = M.MaterialUpdateSettings(50, 1.0e-10, 1.0e-12, 1.0e-10)
+    req = MaterialResponseRequirements(tangent=False)
+    runs = []
+    last = None
+    for rep in range(4):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in R.iterate_prescribed_rve_path(model, times, strains, settings, requirements=req):
+            last = s_
+        torch.cuda.synchronize()
+        if rep:
+            runs.append(time.perf_counter() - t0)
+    med = float(np.median(runs))
+    return {"workload": "8^3 homogeneous J2 Perzyna RVE (SI), 100-step uniaxial ramp to 2e-3 at "
+                        "100 1/s, FP64, Newton 1e-10",
+            "path_seconds": runs, "rve_solves_per_s": steps / med,
+            "final_stress_xx": float(last.response.macro_stress[0])}
+
+
+def cpu_baseline_paths(seed: int) -> dict:
+    """More of the reference algorithm on the host, one core (BLAS limited to one
+    thread): the plain-method RVE (8^3 homogeneous J2 Perzyna RVE in SI units,
+    uniaxial strain ramped to 2e-3 at 100 1/s over 100 steps, Newton 1e-10) run in
+    full with the vectorised oracle, three times (median and MAD), and its first
+    three increments with the reference's literal per-point and per-element loops
+    (``oracle/loops.py``, the form of ``plasticity.py:343-431`` and
+    ``nonlinear_solid.py:460-493``); and five explicit velocity-Verlet steps of a
+    32 x 32 x 16 sample of the macro slab with the oracle's dynamics (plain J2,
+    ``src/fem/dynamics/material.py:89-297``), with the 128 x 128 x 64 slab step
+    scaled from it by the element count (marked as extrapolated)."""
+    import os
+
+    from threadpoolctl import threadpool_limits
+
+    from oracle import dynamics as OD
+    from oracle import j2vp as OJ
+    from oracle import loops as OLP
+    from oracle import rve as OR
+
+    out = {"cores": 1, "blas_threads": 1, "host_cores_available": len(os.sched_getaffinity(0))}
+    with threadpool_limits(limits=1):
+        E, nu, sy, H, tau = 30.0e9, 0.2, 30.0e6, 1.0e9, 1.0e-3
+        mat = OJ.J2VPParams(E, nu, sy, H, tau, sy, 1.0)
+        kw = dict(lengths=(1.0, 1.0, 1.0), divisions=(8, 8, 8), phases=[("c1", mat)],
+                  newton=OR.Newton(20, 1e-10 * sy, 1e-10, 1e-12, 1e-10, OR.Armijo(True)))
+        steps, dt = 100, 2.0e-7
+        times = dt * np.arange(steps + 1)
+        strains = np.zeros((steps + 1, 6))
+        strains[:, 0] = 2.0e-3 * np.arange(steps + 1) / steps
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            OR.solve_path(OR.Model(**kw), times, strains, OJ.UpdateSettings(), need_tangent=False)
+            runs.append(time.perf_counter() - t0)
+        med = float(np.median(runs))
+        out["c1_vectorised"] = {
+            "path_seconds": runs, "median_seconds": med,
+            "mad_seconds": float(np.median(np.abs(np.asarray(runs) - med))),
+            "rve_solves_per_s": steps / med,
+            "sample": "8^3 SI RVE, 100-step uniaxial ramp, vectorised oracle + SuperLU, whole path"}
+        model = OLP.LoopModel(**kw)
+        st = model.initial_state()
+        per = []
+        for k in range(1, 4):
+            t0 = time.perf_counter()
+            st = OR.solve_increment(model, strains[k], dt, OJ.UpdateSettings(), st,
+                                    need_tangent=False)["state"]
+            per.append(time.perf_counter() - t0)
+        mp = float(np.median(per))
+        out["c1_literal_loops"] = {
+            "increment_seconds": per, "rve_solves_per_s": 1.0 / mp,
+            "path_seconds_extrapolated": steps * mp,
+            "sample": "the same RVE, increments 1-3, per-point and per-element Python loops "
+                      "(the 100-step path time is 100 x the median increment: extrapolated)"}
+        nx, ny, nz, h = 32, 32, 16, 0.01
+        slab = OD.Box((nx * h, ny * h, nz * h), (nx, ny, nz),
+                      [("slab", OJ.J2VPParams(30e9, 0.2, 30e6, 1e9, 1e-3, 30e6, 1.0, 2400.0))])
+        f_unit = OD.top_patch_unit_pressure(slab.lengths, (nx, ny, nz), ((11, 21), (11, 21)))
+        bottom = np.flatnonzero(np.isclose(slab.nodes[:, 2], 0.0))
+        fixed = (3 * bottom[:, None] + np.arange(3)).ravel()
+        dt_m = 0.5 * h / float(np.sqrt(30e9 * 0.8 / (1.2 * 0.6) / 2400.0))
+        t0 = time.perf_counter()
+        OD.explicit(slab, dt_m, 5, fixed, f_unit, OD.half_sine(50e6, 2e-4), OJ.UpdateSettings())
+        per_step = (time.perf_counter() - t0) / 5.0
+        out["macro_explicit_sample"] = {
+            "seconds_per_step": per_step, "elements": nx * ny * nz,
+            "slab_128x128x64_seconds_per_step_extrapolated": per_step * 64.0,
+            "slab_128x128x64_steps_per_s_extrapolated": 1.0 / (per_step * 64.0),
+            "sample": "32x32x16 slab, 5 explicit steps (initial assembly included), vectorised "
+                      "oracle; the 128x128x64 figure scales by the element count"}
+    return out
 
 
 BF16_PEAK_TFLOPS = 2500.0
@@ -904,6 +1053,8 @@ def main() -> int:
     ap.add_argument("--macro-probes", type=int, default=64)
     ap.add_argument("--macro-monitors", type=int, default=64)
     ap.add_argument("--macro-small", type=int, nargs=3, default=(8, 8, 4))
+    ap.add_argument("--no-plan", action="store_true", help="skip the data-plan production section")
+    ap.add_argument("--plan-rve", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -1117,9 +1268,15 @@ def main() -> int:
             "error_definition": "per increment |sigma_fp32 - sigma_fp64|_inf / |sigma_fp64|_inf",
         }
         c32.close()
+    print("plain-method RVE (8^3, 100 steps) ...", file=sys.stderr, flush=True)
+    line["c1_rve"] = c1_section()
+    if not args.no_plan:
+        print("data plan production ...", file=sys.stderr, flush=True)
+        line["plan_production"] = plan_section(args)
     if not args.no_cpu_baseline:
         print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(n, args.seed)
+        line["cpu_baseline"]["other_paths"] = cpu_baseline_paths(args.seed)
     if not args.no_macro:
         print("macro explicit slab ...", file=sys.stderr, flush=True)
         line["macro_explicit"] = macro_explicit_section(args)

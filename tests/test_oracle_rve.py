@@ -161,3 +161,27 @@ def test_armijo_reference_cases():
         OR.armijo_search(np.asarray([1.0]), lambda a: np.asarray([1.0]), OR.Armijo(True, max_backtracks=2))
     assert info.value.diagnostics["max_backtracks"] == 2
     assert info.value.diagnostics["last_candidate_residual_norm"] == 1.0
+
+
+def test_literal_loop_model_equals_vectorised():
+    """The reference's literal per-point / per-element loops (oracle/loops.py, timed by
+    the CPU baseline) give the vectorised model's increments to round-off."""
+    from oracle import loops as OLP
+
+    mat = OJ.J2VPParams(30e9, 0.2, 30e6, 1e9, 1e-3, 30e6, 1.0)
+    kw = dict(lengths=(1.0, 1.0, 1.0), divisions=(2, 2, 3), phases=[("m", mat)],
+              newton=OR.Newton(20, 1e-10 * 30e6, 1e-10, 1e-12, 1e-10, OR.Armijo(True)))
+    vec, loop = OR.Model(**kw), OLP.LoopModel(**kw)
+    sv, sl = vec.initial_state(), loop.initial_state()
+    dt = 2e-7
+    for k in range(1, 4):
+        eps = np.asarray([1.2e-3 * k, -2e-4 * k, 1e-4, 3e-4 * k, 0.0, -1e-4 * k])
+        rv = OR.solve_increment(vec, eps, dt, OJ.UpdateSettings(), sv)
+        rl = OR.solve_increment(loop, eps, dt, OJ.UpdateSettings(), sl)
+        np.testing.assert_allclose(rl["macro_stress"], rv["macro_stress"], rtol=1e-12, atol=1e-3)
+        np.testing.assert_allclose(rl["effective_tangent"], rv["effective_tangent"], rtol=1e-10,
+                                   atol=1e-3)
+        for key, v in rv["state"]["material_state"].items():
+            np.testing.assert_allclose(rl["state"]["material_state"][key], v, rtol=1e-12, atol=1e-18)
+        assert rl["diagnostics"]["newton_iterations"] == rv["diagnostics"]["newton_iterations"]
+        sv, sl = rv["state"], rl["state"]
