@@ -816,9 +816,10 @@ __device__ __forceinline__ LanePos lane_pos(int xt) {
 
 // y pass (INV: inverse).  The forward pass stores its modes in natural order (lane l,
 // register k1 -> mode k1 + M bitrev4(l)); the inverse pass reads them back the same
-// way.  The forward launch has one block more: it sums the update partials of
-// k_sp_xfwd once into sc (r.r, sum r per component) and sets the PCG tolerance on the
-// first application.
+// way.  Each launch has one block more: the forward one sums the update partials of
+// the x pass once into sc (r.r, sum r per component) and sets the PCG tolerance on
+// the first application; the inverse one sums the z pass's r.z partials and does the
+// iteration's r.z bookkeeping and beta, which the next element pass reads.
 template <int M, bool INV>
 __global__ void __launch_bounds__(kLaneBlock)
     k_sp_ylines(int nx, int ny, int nz, int64_t C, DftAxis ay, double2* __restrict__ S,
@@ -826,6 +827,33 @@ __global__ void __launch_bounds__(kLaneBlock)
                 double rtol, double atol) {
   const int tiles_x = lane_tiles_x(nx);
   const int nb = tiles_x * nz;
+  if (INV && (int)blockIdx.x == nb) {
+    // r.z from the z pass's block partials (P of them in part_upd), once; the next
+    // element pass reads beta
+    if (sc->done) return;
+    __shared__ double wr[kLaneWaves];
+    double a = 0.0;
+    for (int b = threadIdx.x; b < P; b += kLaneBlock) a += part_upd[b];
+    const double w = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) wr[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double rz = 0.0;
+#pragma unroll
+      for (int u = 0; u < kLaneWaves; ++u) rz += wr[u];
+      if (!(rz > 0.0) || !isfinite(rz)) {
+        sc->done = 2;
+      } else {
+        const int it = sc->iters;  // completed iterations: written only by the x pass
+        const double beta = it == 0 ? 0.0 : rz / sc->rzr[(it + 1) & 1];
+        sc->it_cur = it;
+        sc->rzr[it & 1] = rz;
+        sc->rz = rz;
+        sc->beta = beta;
+      }
+    }
+    return;
+  }
   if (!INV && (int)blockIdx.x == nb) {
     if (sc->done) return;
     __shared__ double ws[kLaneWaves][4];
@@ -1138,7 +1166,10 @@ __global__ void __launch_bounds__(kBlock)
 // corner exchange and no cross-lane force reduction.  At least two waves per SIMD:
 // in FP64 the pass needs ~280 registers per lane, and a second resident wave
 // (a few spilled values) beats one wave without spills (21.9 -> 19.5 us at 48^3).
-template <class T>
+//
+// PRE: beta and the r.z bookkeeping were done once by the extra block of the inverse
+// y launch (row-lane passes); otherwise every block reduces the r.z partials itself.
+template <class T, bool PRE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
     k_sp_elem(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
               const T* __restrict__ tan, const double* __restrict__ zr,
@@ -1146,19 +1177,24 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
               const double* __restrict__ part_rz, int P, Scalars* __restrict__ sc) {
   __shared__ double lds[kBlock / 64];
   if (sc->done) return;
-  double rz[1];
-  reduce_partials<1>(part_rz, P, rz, lds);
-  if (!(rz[0] > 0.0) || !isfinite(rz[0])) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
-    return;
-  }
-  const int it = sc->iters;  // completed iterations: written only by k_sp_update
-  const double beta = it == 0 ? 0.0 : rz[0] / sc->rzr[(it + 1) & 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->it_cur = it;
-    sc->rzr[it & 1] = rz[0];
-    sc->rz = rz[0];
-    sc->beta = beta;
+  double beta;
+  if (PRE) {
+    beta = sc->beta;
+  } else {
+    double rz[1];
+    reduce_partials<1>(part_rz, P, rz, lds);
+    if (!(rz[0] > 0.0) || !isfinite(rz[0])) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) sc->done = 2;
+      return;
+    }
+    const int it = sc->iters;  // completed iterations: written only by k_sp_update
+    beta = it == 0 ? 0.0 : rz[0] / sc->rzr[(it + 1) & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->it_cur = it;
+      sc->rzr[it & 1] = rz[0];
+      sc->rz = rz[0];
+      sc->beta = beta;
+    }
   }
   const int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (e >= G.E) return;
@@ -1769,8 +1805,8 @@ struct Ops {
       hipLaunchKernelGGL((k_sp_zlines<MZ, kZlineKx>), dim3(g_zc), dim3(16 * kZlineKx), 0, s, nx,
                          ny, nz, C,
                          inv_n, R->az, (const hipFloatComplex*)R->kinv, S, part_rz, R->sc);
-      hipLaunchKernelGGL((k_sp_ylines<MY, true>), dim3(g_yl), dim3(kLaneBlock), 0, s, nx, ny, nz,
-                         C, R->ay, S, R->sc, (const double*)nullptr, 0, 0, rtol, atol);
+      hipLaunchKernelGGL((k_sp_ylines<MY, true>), dim3(g_yl + 1), dim3(kLaneBlock), 0, s, nx, ny,
+                         nz, C, R->ay, S, R->sc, (const double*)part_rz, (int)g_zc, 0, rtol, atol);
     };
     auto lane_z = [&](auto my, hipStream_t s, int first) {
       switch (Mz) {
@@ -1847,9 +1883,14 @@ struct Ops {
       double* p_old = R->pp[1 - pi];
       double* p_new = R->pp[pi];
       prof::begin(prof::PCG_ELEM, s, tag);
-      hipLaunchKernelGGL(k_sp_elem<T>, dim3(gel), dim3(kBlock), 0, s, R->G, R->d_tab, R->d_ephase,
-                         (const T*)R->tan[slot], (const double*)R->pz, p_old, p_new,
-                         (T*)R->F, (const double*)part_rz, (int)g_zc, R->sc);
+      if (lanes)
+        hipLaunchKernelGGL((k_sp_elem<T, true>), dim3(gel), dim3(kBlock), 0, s, R->G, R->d_tab,
+                           R->d_ephase, (const T*)R->tan[slot], (const double*)R->pz, p_old, p_new,
+                           (T*)R->F, (const double*)part_rz, (int)g_zc, R->sc);
+      else
+        hipLaunchKernelGGL((k_sp_elem<T, false>), dim3(gel), dim3(kBlock), 0, s, R->G, R->d_tab,
+                           R->d_ephase, (const T*)R->tan[slot], (const double*)R->pz, p_old, p_new,
+                           (T*)R->F, (const double*)part_rz, (int)g_zc, R->sc);
       prof::end(s);
       prof::begin(prof::PCG_NODE, s, tag);
       hipLaunchKernelGGL(k_sp_node<T>, dim3(gn), dim3(kBlock), 0, s, R->G, (const T*)R->F, p_new,
