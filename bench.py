@@ -282,7 +282,8 @@ def cpu_baseline_paths(seed: int) -> dict:
     ``nonlinear_solid.py:460-493``); and five explicit velocity-Verlet steps of a
     32 x 32 x 16 sample of the macro slab with the oracle's dynamics (plain J2,
     ``src/fem/dynamics/material.py:89-297``), with the 128 x 128 x 64 slab step
-    scaled from it by the element count (marked as extrapolated)."""
+    scaled from it by the element count (marked as extrapolated); and the surrogate
+    network's forward pass in torch on the host (FP32, every host core)."""
     import os
 
     from threadpoolctl import threadpool_limits
@@ -343,6 +344,40 @@ def cpu_baseline_paths(seed: int) -> dict:
             "slab_128x128x64_steps_per_s_extrapolated": 1.0 / (per_step * 64.0),
             "sample": "32x32x16 slab, 5 explicit steps (initial assembly included), vectorised "
                       "oracle; the 128x128x64 figure scales by the element count"}
+    # the surrogate network (13 -> 256^4 -> 12, GELU) in torch on the host, FP32, all cores
+    import torch
+
+    n_thr = len(os.sched_getaffinity(0))
+    prev = torch.get_num_threads()
+    torch.set_num_threads(n_thr)
+    try:
+        g = torch.Generator().manual_seed(seed)
+        dims = [13, 256, 256, 256, 256, 12]
+        ws = [torch.randn(dims[i + 1], dims[i], generator=g) / dims[i] ** 0.5 for i in range(5)]
+        bs = [torch.zeros(dims[i + 1]) for i in range(5)]
+        x = torch.randn(1 << 16, 13, generator=g)
+
+        def net(v):
+            for i in range(5):
+                v = torch.nn.functional.linear(v, ws[i], bs[i])
+                if i < 4:
+                    v = torch.nn.functional.gelu(v, approximate="tanh")
+            return v
+
+        with torch.no_grad():
+            net(x)
+            reps = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                net(x)
+                reps.append(time.perf_counter() - t0)
+        med = float(np.median(reps))
+        out["surrogate_inference_torch_cpu"] = {
+            "points_per_s": x.shape[0] / med, "seconds_per_2^16_points": reps, "threads": n_thr,
+            "sample": "the surrogate network's forward pass at 2^16 points, torch FP32 on the "
+                      "host, every host core (median of three after one warm-up)"}
+    finally:
+        torch.set_num_threads(prev)
     return out
 
 
