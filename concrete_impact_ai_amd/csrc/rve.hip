@@ -823,17 +823,18 @@ __device__ __forceinline__ LanePos lane_pos(int xt) {
 template <int M, bool INV>
 __global__ void __launch_bounds__(kLaneBlock)
     k_sp_ylines(int nx, int ny, int nz, int64_t C, DftAxis ay, double2* __restrict__ S,
-                Scalars* __restrict__ sc, const double* __restrict__ part_upd, int P, int first,
+                Scalars* __restrict__ sc, const double* __restrict__ part, int P, int first,
                 double rtol, double atol) {
+  // part: the forward launch's 4 rows of P update partials (r.r, sum r per component)
+  // from the x pass, or the inverse launch's P r.z partials from the z pass
   const int tiles_x = lane_tiles_x(nx);
   const int nb = tiles_x * nz;
   if (INV && (int)blockIdx.x == nb) {
-    // r.z from the z pass's block partials (P of them in part_upd), once; the next
-    // element pass reads beta
+    // r.z from the z pass's block partials, once; the next element pass reads beta
     if (sc->done) return;
     __shared__ double wr[kLaneWaves];
     double a = 0.0;
-    for (int b = threadIdx.x; b < P; b += kLaneBlock) a += part_upd[b];
+    for (int b = threadIdx.x; b < P; b += kLaneBlock) a += part[b];
     const double w = wave_sum(a);
     if ((threadIdx.x & 63) == 0) wr[threadIdx.x >> 6] = w;
     __syncthreads();
@@ -860,7 +861,7 @@ __global__ void __launch_bounds__(kLaneBlock)
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < P; b += kLaneBlock) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) a[s] += part_upd[s * P + b];
+      for (int s = 0; s < 4; ++s) a[s] += part[s * P + b];
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {

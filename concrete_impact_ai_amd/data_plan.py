@@ -39,7 +39,7 @@ import numpy as np
 
 from concrete_impact_ai_amd.config import load_yaml_config
 from concrete_impact_ai_amd.microstructure import (FAMILY_CONTROLS, TENSOR_DIRECTIONS,
-                                                   TENSOR_WEIGHTS, production_strain_path)
+                                                   production_strain_path)
 
 __all__ = ["AmplitudeBand", "RateBand", "ProductionDesign", "LoadCase", "DataSource",
            "ParallelPlan", "RVEDataPlan", "RVEDataTask", "GeneratedControlPath",
@@ -726,6 +726,3 @@ def produce_plan_responses(source: DataSource, tasks, model, *, progress=None) -
                               **{k: np.stack(v)[:, i] for k, v in rows.items()}}
     return out
 
-
-# the engineering-tensor weights of the production paths (re-exported for tests)
-TENSOR_NORM_WEIGHTS = TENSOR_WEIGHTS

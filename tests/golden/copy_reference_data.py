@@ -1,5 +1,5 @@
 """Copy the reference's data files that the tests read into tests/golden/: the
-benchmark configs that the GPU config tests run and the fixed-cell data plan
+benchmark configs that the GPU config tests run and the data plans
 (tests/golden/configs/), and the
 reference's sample RVE response shard (tests/golden/rve_rno_c48_sample_response.h5,
 from data/rve_rno_c48_v2_2000/sample/response.h5).  Input data only: the GPU box
@@ -19,7 +19,8 @@ FILES = [
     "plasticity/dynamic_j2_viscoplastic_pressure_pulse_3d.yaml",
 ]
 
-PLANS = ["rve_rno_c48_v2_2000.yaml"]  # configs/experiments: the fixed-cell data plan
+PLANS = ["rve_rno_c48_v2_2000.yaml", "rve_rno_fixed_cell_v1.yaml", "rve_rno_data_plan.yaml",
+         "rve_rno_data_pilot.yaml", "rve_rno_fe2_preflight.yaml"]  # configs/experiments: data plans
 
 SAMPLE = Path("/root/reference/data/rve_rno_c48_v2_2000/sample/response.h5")
 

@@ -182,3 +182,58 @@ def test_voxel_cell_has_the_plan_phases_and_inclusion_fraction(plan):
     phases = model.material.phases
     assert phases[0].material.yield_stress == 10.0 and phases[0].material.rate_exponent == 2.0
     assert phases[1].material.young_modulus == pytest.approx(6666.666666666667)
+
+
+def test_fixed_cell_v1_plan_has_eighty_balanced_unique_paths():
+    """``test_fixed_cell_rve_plan.py:93-110``: 40 elastic and 40 viscoplastic paths,
+    five families balanced per load case, unique task and lineage ids; its parallel
+    budget fits 32 CPUs."""
+    plan = DP.load_rve_data_plan(GOLDEN / "configs" / "rve_rno_fixed_cell_v1.yaml")
+    tasks = DP.build_rve_data_task_manifest(plan)
+    assert len(tasks) == 80
+    assert len({t.task_id for t in tasks}) == 80 and len({t.lineage_id for t in tasks}) == 80
+    for lc in plan.sources[0].load_cases:
+        assert lc.path_count == 40
+        counts = {}
+        for i in range(lc.path_count):
+            f = lc.path_families[i % len(lc.path_families)]
+            counts[f] = counts.get(f, 0) + 1
+        assert set(counts.values()) == {8}
+    DP.validate_parallel_resources(plan, 32)
+    # legacy strain families along a seeded unit direction, peak_scale at the control peak
+    src = plan.sources[0]
+    for t in tasks[:10]:
+        path = DP.generate_data_control_path(src, t)
+        lc = next(c for c in src.load_cases if c.name == t.load_case_name)
+        assert path.values.shape == (lc.time_points, 6) and path.times[-1] == lc.duration
+        np.testing.assert_allclose(path.values[0], 0.0, atol=1e-15)
+
+
+@pytest.mark.parametrize("name,purpose,count", [
+    ("rve_rno_data_plan.yaml", "complete_four_source", 3392),
+    ("rve_rno_data_pilot.yaml", "complete_four_source", 40),
+    ("rve_rno_fe2_preflight.yaml", "direct_fe2_preflight", 8),
+])
+def test_four_source_and_preflight_plans(name, purpose, count):
+    """The reference's other data plans load and expand: every canonical source kind,
+    impact tasks sharing one macro-run lineage per load case, the impact pressure a
+    half-sine of the load case's peak over its duration."""
+    plan = DP.load_rve_data_plan(GOLDEN / "configs" / name)
+    assert plan.purpose == purpose
+    tasks = DP.build_rve_data_task_manifest(plan)
+    assert len(tasks) == count and len({t.task_id for t in tasks}) == count
+    for src in plan.sources:
+        mine = [t for t in tasks if t.source_name == src.name]
+        assert len(mine) == sum(lc.path_count for lc in src.load_cases)
+        t = mine[0]
+        path = DP.generate_data_control_path(src, t)
+        lc = next(c for c in src.load_cases if c.name == t.load_case_name)
+        if src.kind == "impact_path":
+            assert path.control_kind == "pressure"
+            assert {x.lineage_id for x in mine if x.load_case_name == lc.name} == {
+                f"{src.name}__{lc.name}__macro_run"}
+            np.testing.assert_allclose(path.values[:, 0],
+                                       lc.peak_scale * np.sin(np.pi * path.times / lc.duration),
+                                       rtol=0, atol=1e-12 * lc.peak_scale)
+        else:
+            assert path.control_kind == "macro_strain" and path.values.shape == (lc.time_points, 6)
