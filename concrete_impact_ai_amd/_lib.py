@@ -280,6 +280,7 @@ _SIGNATURES = {
         [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
          C.POINTER(C.c_double), C.POINTER(UpdateSettings), _vp, _vp, C.POINTER(RVEResult)],
     ),
+    "cia_rve_batch_effective_tangent": (C.c_int32, [_vp, _vp, _vp, _vp]),
     "cia_rve_batch_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                          C.c_int32]),
     "cia_rve_batch_set_tuning": (C.c_int32, [_vp, C.POINTER(C.c_int32), C.c_int32]),

@@ -190,6 +190,7 @@ class RVEBatchResponse:
     diagnostics: dict                 # name -> (n,) array (BATCH_DIAGNOSTICS)
     reasons: tuple                    # per RVE: "ok" or the reference failure reason
     state: RVEBatchState
+    effective_tangent: np.ndarray | None = None  # (n, 6, 6) when asked for, NaN where failed
 
     @property
     def converged(self) -> np.ndarray:
@@ -199,13 +200,16 @@ class RVEBatchResponse:
 def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
                     update_settings: MaterialUpdateSettings, committed: RVEBatchState,
                     raise_on_failure: bool = True,
-                    initial_w: torch.Tensor | None = None) -> RVEBatchResponse:
+                    initial_w: torch.Tensor | None = None,
+                    need_tangent: bool = False) -> RVEBatchResponse:
     """One strain-driven increment of every RVE of the batch.
 
     ``macro_strains`` (n, 6), ``time_steps`` (n,).  ``initial_w`` (device, shaped
     like ``committed.w``) starts Newton from a predicted fluctuation instead of the
     committed one (the reference's start); the converged state meets the same
-    tolerances.  With ``raise_on_failure`` the
+    tolerances.  ``need_tangent`` also returns every converged RVE's effective
+    tangent (``solver.py:438-461``, batched by ``effective_tangents_raw``); its
+    six linear solves count in ``linear_iterations``.  With ``raise_on_failure`` the
     lowest-index failed RVE raises the reference's exception
     (``RVEEquilibriumError`` or ``MaterialPointConvergenceError``) with
     ``rve_index`` in its diagnostics; otherwise failures are reported in
@@ -223,6 +227,11 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
     status, res, next_packed, next_w = solve_batch_raw(
         batch, committed.packed, committed.w, committed.macro, macro, dts, update_settings,
         initial_w)
+    tangents = None
+    if need_tangent:
+        tstatus, tangents = effective_tangents_raw(batch, res)
+        if tstatus == _lib.CIA_ERR_RVE_FAILURE:
+            status = tstatus
     a = np.frombuffer(res, dtype=_RESULT_DTYPE)  # the cia_rve_result array as columns
     reasons = tuple(_REASONS[a["reason"]])
     if status == _lib.CIA_ERR_RVE_FAILURE and raise_on_failure:
@@ -236,6 +245,7 @@ def solve_rve_batch(batch: RVEBatch, macro_strains, time_steps,
         diagnostics=diag,
         reasons=reasons,
         state=RVEBatchState(batch, next_packed, next_w, macro.copy()),
+        effective_tangent=tangents,
     )
 
 
@@ -277,6 +287,37 @@ def solve_batch_raw(batch: RVEBatch, packed: torch.Tensor, w: torch.Tensor, comm
         C.c_void_p(next_packed.data_ptr()), C.c_void_p(next_w.data_ptr()), res)
     _lib.check(status, "cia_rve_batch_solve")
     return status, res, next_packed, next_w
+
+
+_NON_FINITE_TANGENT = _lib.RVE_REASONS.index("non_finite_effective_tangent_sensitivity")
+
+
+def effective_tangents_raw(batch: RVEBatch, res):
+    """Effective tangents of the RVEs that converged in the batch's last solve
+    (``_compute_effective_tangent``, ``src/fem/rve/solver.py:438-461``: the Schur
+    complement of the converged tangent, six linear solves with the Newton systems'
+    preconditioned CG, all RVEs together).  Fills ``effective_tangent`` of each
+    converged entry of ``res`` (the ``cia_rve_result`` array of that solve), adds the
+    six solves' iterations to its ``linear_iterations`` and marks an RVE whose
+    sensitivity came out non-finite with that failure reason; returns the status and
+    the (n, 6, 6) tangents (NaN for RVEs that did not converge)."""
+    n = batch.n_rves
+    tan = np.empty((n, 36), dtype=np.float64)
+    its = np.zeros(n, dtype=np.int32)
+    status = _lib.lib().cia_rve_batch_effective_tangent(
+        batch.handle, stream_handle(), tan.ctypes.data_as(C.c_void_p),
+        its.ctypes.data_as(C.c_void_p))
+    _lib.check(status, "cia_rve_batch_effective_tangent")
+    for i in range(n):
+        r = res[i]
+        if r.reason != 0:
+            continue
+        if not np.all(np.isfinite(tan[i])):
+            r.reason = _NON_FINITE_TANGENT
+            continue
+        C.memmove(r.effective_tangent, tan[i].ctypes.data, 36 * 8)
+        r.linear_iterations += int(its[i])
+    return status, tan.reshape(n, 6, 6)
 
 
 def raise_failure(batch: RVEBatch, i: int, res, macro, dt, update_settings):

@@ -1151,6 +1151,176 @@ __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
 }
 
 // ---------------------------------------------------------------------------
+// Effective tangent of the converged RVEs of the last solve (the Schur complement
+// of _compute_effective_tangent, src/fem/rve/solver.py:438-461, as the single-RVE
+// solver forms it): for every macro strain direction e_j the affine force rows of
+// e_j through the converged compact tangents, gathered into the Newton residual
+// slot; the batched PCG solves K dw = -T^T K A e_j; then column j of
+// C_eff = (1/V) sum_q w C_q (e_j + B_q dw).  RVEs in this phase are in S_PCG with
+// Ctl::pad0 = 1 (tangent mode).
+
+__global__ void __launch_bounds__(kBlock) k_b_tan_begin(Arena A, double lin_rtol) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= A.R) return;
+  Ctl& c = A.ctl[r];
+  if (c.state != S_DONE) return;
+  c.state = S_PCG;
+  c.pad0 = 1;
+  c.p_done = 0;
+  c.p_iters = 0;
+  c.lin_rtol = lin_rtol;
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_tan_end(Arena A) {
+  const int r = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (r >= A.R) return;
+  Ctl& c = A.ctl[r];
+  if (c.pad0 != 1) return;
+  if (c.state == S_PCG) c.state = S_DONE;
+  c.pad0 = 0;
+}
+
+// Affine force rows of e_j (one lane per element), tangent-mode RVEs.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_b_affine(Arena A, int jdir) {
+  const int64_t NPT = A.NPT, ET = A.ET;
+  CIA_ACTIVE_TILES(F_CL) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (c.state != S_PCG || c.pad0 != 1) continue;
+    const int64_t e = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    if (e >= D.E) continue;
+    const int64_t ge = D.e0 + e;
+    const T* tan = (const T*)A.tan[c.cur];
+    Tangent<T> tc;
+    tc.K = (T)A.tabs[r].p[A.ephase[ge]].K;
+    T f[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) f[a][0] = f[a][1] = f[a][2] = (T)0;
+    T ej[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) ej[m] = m == jdir ? (T)1 : (T)0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t p = 8 * ge + q;
+      tc.d = tan[p];
+      tc.r = tan[NPT + p];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != (T)0 ? tan[(2 + m) * NPT + p] : (T)0;
+      T sg[6];
+      tangent_apply(tc, ej, sg);
+      switch (q) {
+        case 0: force_at<0>(sg, D.g, f); break;
+        case 1: force_at<1>(sg, D.g, f); break;
+        case 2: force_at<2>(sg, D.g, f); break;
+        case 3: force_at<3>(sg, D.g, f); break;
+        case 4: force_at<4>(sg, D.g, f); break;
+        case 5: force_at<5>(sg, D.g, f); break;
+        case 6: force_at<6>(sg, D.g, f); break;
+        default: force_at<7>(sg, D.g, f); break;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) ((T*)A.F)[(3 * a + m) * ET + ge] = f[a][m];
+  }
+}
+
+// Tile partials of sum_q C_q (e_j + B_q dw) (rows P_RESP .. P_RESP + 5; the response
+// rows of the final pass are read before this phase), every tile of the
+// tangent-mode RVEs.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_b_ceff(Arena A, int jdir) {
+  __shared__ double red[kBlock / 64][6];
+  const int64_t NPT = A.NPT;
+  CIA_TILES(A.T[F_CL]) {
+    const int r = A.t2r[F_CL][t];
+    const Desc& D = A.desc[r];
+    const Ctl& c = A.ctl[r];
+    if (c.state != S_PCG || c.pad0 != 1) continue;
+    const int64_t e = (int64_t)(t - D.t0[F_CL]) * kBlock + threadIdx.x;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (e < D.E) {
+      const int64_t ge = D.e0 + e;
+      int64_t cq[8];
+      int i, j, k;
+      periodic_corners(e, D.nx, D.ny, D.nz, cq, i, j, k);
+      double se[8][3];
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) se[a][m] = A.dw[3 * (D.e0 + cq[a]) + m];
+      const T* tan = (const T*)A.tan[c.cur];
+      Tangent<double> tc;
+      tc.K = A.tabs[r].p[A.ephase[ge]].K;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int64_t p = 8 * ge + q;
+        tc.d = (double)tan[p];
+        tc.r = (double)tan[NPT + p];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) tc.n[m] = tc.r != 0.0 ? (double)tan[(2 + m) * NPT + p] : 0.0;
+        double eps[6], sg[6];
+        switch (q) {
+          case 0: strain_at<0>(se, D.g, eps); break;
+          case 1: strain_at<1>(se, D.g, eps); break;
+          case 2: strain_at<2>(se, D.g, eps); break;
+          case 3: strain_at<3>(se, D.g, eps); break;
+          case 4: strain_at<4>(se, D.g, eps); break;
+          case 5: strain_at<5>(se, D.g, eps); break;
+          case 6: strain_at<6>(se, D.g, eps); break;
+          default: strain_at<7>(se, D.g, eps); break;
+        }
+        eps[jdir] += 1.0;
+        tangent_apply(tc, eps, sg);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc[m] += sg[m];
+      }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const double a = wave_sum(acc[m]);
+      if (lane == 0) red[wid][m] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      double a = 0.0;
+#pragma unroll
+      for (int u = 0; u < kBlock / 64; ++u) a += red[u][threadIdx.x];
+      A.part[(P_RESP + threadIdx.x) * A.Tmax + t] = a;
+    }
+    __syncthreads();
+  }
+}
+
+// Column j of every tangent-mode RVE's C_eff (out [R][36], row-major), its PCG
+// iterations counted, the PCG reset for the next direction.
+__global__ void __launch_bounds__(kBlock) k_b_ceff_sum(Arena A, int jdir, double* __restrict__ out) {
+  CIA_RVE_WAVE();
+  if (c.state != S_PCG || c.pad0 != 1) return;
+  double col[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) col[m] = rve_sum(A, P_RESP + m, D.t0[F_CL], D.nt[F_CL]);
+  if (lane != 0) return;
+  bool fin = true;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    out[36 * r + 6 * m + jdir] = col[m] * D.g.jw / D.volume;
+    fin = fin && isfinite(col[m]);
+  }
+  c.pcg_its += c.p_iters;
+  c.p_done = 0;
+  c.p_iters = 0;
+  if (!fin) {
+    c.state = S_FAILED;
+    c.reason = CIA_RVE_NON_FINITE_EFFECTIVE_TANGENT_SENSITIVITY;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Newton control (one wavefront per RVE).
 
 __device__ void newton_test(Ctl& c, const NewtonP& N) {
@@ -1424,6 +1594,8 @@ struct cia_rve_batch {
   Ctl* ctl = nullptr;
   Req* req = nullptr;
   double* resp = nullptr;
+  double* tan_out = nullptr;  // [R][36] effective tangents (allocated on first use)
+  bool solved = false;        // a solve has run: its converged states can give tangents
   int* cnt = nullptr;
   cia_point_failure* rec = nullptr;
   // pinned host staging
@@ -1759,6 +1931,7 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
   CIA_HIP(hipStreamSynchronize(st));
   bool all_ok = true;
   B->stats[0] += 1;
+  B->solved = true;
   for (int r = 0; r < R; ++r) B->stats[3] += hc[r].pcg_its;
   for (int r = 0; r < R; ++r) {
     cia_rve_result& o = res[r];
@@ -1825,6 +1998,69 @@ cia_status solve_batch(cia_rve_batch* B, hipStream_t st, const void* state_c, co
     }
   }
   return all_ok ? CIA_OK : CIA_ERR_RVE_FAILURE;
+}
+
+// Effective tangents of the RVEs that converged in the last solve: six batched PCG
+// solves with the converged tangents, one per macro strain direction (kernels
+// k_b_tan_begin .. k_b_tan_end above).  tangents: host [R][36], row-major; RVEs that
+// did not converge read NaN; linear_iterations (if given, host [R]) receives each
+// RVE's PCG iterations of the six solves.
+template <class T>
+cia_status tangent_batch(cia_rve_batch* B, hipStream_t st, double* tangents, int32_t* its) {
+  const int R = B->R;
+  if (!B->tan_out) CIA_TRY(balloc(B, (void**)&B->tan_out, (size_t)R * 36 * sizeof(double)));
+  const unsigned gr = (unsigned)((R + kBlock - 1) / kBlock), gw = (unsigned)((R + 3) / 4);
+  std::vector<Ctl> h0(R);
+  CIA_HIP(hipMemcpyAsync(h0.data(), B->ctl, R * sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  hipLaunchKernelGGL(k_b_tan_begin, dim3(gr), dim3(kBlock), 0, st, B->A, B->lin.relative_tolerance);
+  CIA_HIP(hipGetLastError());
+  int cnt[7];
+  CIA_TRY(read_counts(B, st, cnt));
+  const int interval = std::max(1, B->lin.check_interval);
+  for (int j = 0; j < 6 && cnt[S_PCG] > 0; ++j) {
+    BOps<T>::compact(B, st);
+    hipLaunchKernelGGL(k_b_affine<T>, dim3(kgrid(B, K_ELEM, F_CL)), dim3(kBlock), 0, st, B->A, j);
+    BOps<T>::gather(B, st, S_PCG);
+    BOps<T>::pcg_start(B, st);
+    int it = 0;
+    while (it < B->lin.max_iterations) {
+      const int n = std::min(interval, B->lin.max_iterations - it);
+      if (B->pair) {
+        const int pairs = std::min((n + 1) / 2, (B->lin.max_iterations - it) / 2);
+        for (int s2 = 0; s2 < pairs; ++s2, it += 2) CIA_HIP(hipGraphLaunch(B->pair, st));
+        if (pairs == 0) {
+          BOps<T>::iteration(B, st, it & 1);
+          ++it;
+        }
+      } else {
+        for (int s2 = 0; s2 < n; ++s2, ++it) BOps<T>::iteration(B, st, it & 1);
+      }
+      CIA_HIP(hipGetLastError());
+      CIA_TRY(read_counts(B, st, cnt));
+      if (cnt[5] == 0) break;
+    }
+    B->stats[2] += it;
+    hipLaunchKernelGGL(k_b_ceff<T>, dim3(kgrid(B, K_UPD, F_CL)), dim3(kBlock), 0, st, B->A, j);
+    hipLaunchKernelGGL(k_b_ceff_sum, dim3(gw), dim3(kBlock), 0, st, B->A, j, B->tan_out);
+    CIA_HIP(hipGetLastError());
+    if (j < 5) CIA_TRY(read_counts(B, st, cnt));
+  }
+  hipLaunchKernelGGL(k_b_tan_end, dim3(gr), dim3(kBlock), 0, st, B->A);
+  CIA_HIP(hipGetLastError());
+  std::vector<Ctl> hc(R);
+  std::vector<double> ht((size_t)R * 36);
+  CIA_HIP(hipMemcpyAsync(hc.data(), B->ctl, R * sizeof(Ctl), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipMemcpyAsync(ht.data(), B->tan_out, ht.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+  CIA_HIP(hipStreamSynchronize(st));
+  bool ok = true;
+  for (int r = 0; r < R; ++r) {
+    const bool done = h0[r].state == S_DONE && hc[r].state == S_DONE;
+    ok = ok && done;
+    for (int k = 0; k < 36; ++k) tangents[36 * r + k] = done ? ht[36 * r + k] : NAN;
+    if (its) its[r] = hc[r].pcg_its - h0[r].pcg_its;
+  }
+  return ok ? CIA_OK : CIA_ERR_RVE_FAILURE;
 }
 
 }  // namespace
@@ -2174,6 +2410,15 @@ cia_status cia_rve_batch_solve(cia_rve_batch* B, void* stream, const void* commi
                                macro_strain, time_step, S, next_state, next_w, results);
   return solve_batch<float>(B, st, committed_state, committed_w, initial_w, committed_macro,
                             macro_strain, time_step, S, next_state, next_w, results);
+}
+
+cia_status cia_rve_batch_effective_tangent(cia_rve_batch* B, void* stream, double* tangents,
+                                           int32_t* linear_iterations) {
+  CIA_ARG(B && tangents, "NULL argument to cia_rve_batch_effective_tangent");
+  CIA_ARG(B->solved, "cia_rve_batch_effective_tangent needs a preceding cia_rve_batch_solve");
+  hipStream_t st = (hipStream_t)stream;
+  if (B->precision == CIA_PREC_FP64) return tangent_batch<double>(B, st, tangents, linear_iterations);
+  return tangent_batch<float>(B, st, tangents, linear_iterations);
 }
 
 cia_status cia_rve_batch_tuning(const cia_rve_batch* B, int32_t* grids, double* microseconds,

@@ -224,12 +224,14 @@ class RVEHomogenizedMaterial:
             bt = self._batches[m] = B.RVEBatch([self.model] * m)
         return bt
 
-    def _solve_batched(self, strains: np.ndarray, time_step, settings, committed: RVEBatchState):
+    def _solve_batched(self, strains: np.ndarray, time_step, settings, committed: RVEBatchState,
+                       need_tangent: bool = False):
         """All macro points' RVEs in batched solves of at most BATCH_GAUSS_POINTS Gauss
         points: the per-point state layout [n][8][n_micro] is permuted to the batch
-        arena's [8][m n_micro] and back.  The last chunk is padded to a power of two up to
-        64 RVEs, beyond that to a multiple of 64 (copies of its last point, results
-        discarded), so that few distinct batch sizes are ever built."""
+        arena's [8][m n_micro] and back.  Effective tangents, when asked for, come from
+        the batched Schur-complement solves of each chunk.  The last chunk is padded to a
+        power of two up to 64 RVEs, beyond that to a multiple of 64 (copies of its last
+        point, results discarded), so that few distinct batch sizes are ever built."""
         from concrete_impact_ai_amd import batch as B
 
         n = strains.shape[0]
@@ -248,6 +250,8 @@ class RVEHomogenizedMaterial:
             status, res, nxt_p, nxt_w = B.solve_batch_raw(
                 bt, packed.contiguous(), committed.w[rows_t].reshape(-1).contiguous(),
                 committed.macro[rows], strains[rows], steps_all[rows], settings)
+            if need_tangent and status == _lib.CIA_OK:
+                status, _ = B.effective_tangents_raw(bt, res)
             if status == _lib.CIA_ERR_RVE_FAILURE:
                 i = next(k for k, r in enumerate(res[:m]) if r.reason != 0)
                 B.raise_failure(bt, i, res[i], strains[s0 + i], float(steps_all[s0 + i]), settings)
@@ -261,8 +265,8 @@ class RVEHomogenizedMaterial:
     def _solve_all(self, strains: np.ndarray, time_step: float, settings, need_tangent: bool,
                    committed: RVEBatchState):
         n = strains.shape[0]
-        if self._batched and not need_tangent and n > 0:
-            return self._solve_batched(strains, time_step, settings, committed)
+        if self._batched and n > 0:
+            return self._solve_batched(strains, time_step, settings, committed, need_tangent)
         nxt_packed = torch.empty_like(committed.packed)
         nxt_w = torch.empty_like(committed.w)
         results: list = [None] * n

@@ -387,6 +387,16 @@ cia_status cia_rve_batch_solve(cia_rve_batch* batch, void* stream, const void* c
                                const double* committed_macro, const double* macro_strain,
                                const double* time_step, const cia_update_settings* settings,
                                void* next_state, double* next_w, cia_rve_result* results);
+/* cia_rve_batch_effective_tangent: the effective (consistent, homogenised) tangent
+ * of every RVE that converged in the preceding cia_rve_batch_solve, by the Schur
+ * complement of _compute_effective_tangent (src/fem/rve/solver.py:438-461): six
+ * batched PCG solves K dw_j = -T^T K A e_j with the converged compact tangents, then
+ * C_eff e_j = (1/V) sum_q w C_q (e_j + B_q dw_j).  tangents host [n][36] row-major
+ * (NaN rows for RVEs that did not converge); linear_iterations host [n] or NULL.
+ * Replaces the per-point single-RVE tangent solves of RVEHomogenizedMaterial.update
+ * with requirements.tangent (src/fem/rve/material.py:104-191). */
+cia_status cia_rve_batch_effective_tangent(cia_rve_batch* batch, void* stream, double* tangents,
+                                           int32_t* linear_iterations);
 /* Autotuned grid per batched kernel (0 = one tile per workgroup) and its time
  * in microseconds; order: assemble, gather, PCG element pass, PCG node pass,
  * PCG update, symbol solve, DFT x, y, z passes (CIA_BATCH_N_KERNELS entries).

@@ -5,8 +5,13 @@
   1e-9 relative, state 1e-9) and against the single-RVE device solver
   (iteration counts equal, stress 1e-10)
 * FP32 element work with FP64 residuals: error of the homogenised stress against
-  the FP64 oracle (2e-6, as for the single-RVE solver)
+  the FP64 oracle (2e-6, as for the single-RVE solver) and of the effective
+  tangent (2e-7)
 * Newton failure reasons per RVE equal those of the single-RVE solver
+* batched effective tangents (cia_rve_batch_effective_tangent, the Schur
+  complement of solver.py:438-461) against the oracle's SuperLU tangent (1e-8)
+  and the single-RVE device tangent (1e-9), along loading and unloading paths;
+  NaN for the RVEs whose Newton solve failed
 * the autotuned grids change no bit of the result
 """
 
@@ -127,6 +132,48 @@ def test_batch_matches_oracle_and_single_rve(gpu):
     assert plastic > 0.3
 
 
+def test_batch_effective_tangent_matches_oracle_and_single_rve(gpu):
+    from concrete_impact_ai_amd import batch as B
+    from concrete_impact_ai_amd import rve as R
+
+    models, omodels = _models(SIZES)
+    steps = 4
+    times, strains = _paths(len(models), steps, seed=9)
+    batch = B.RVEBatch(models)
+    bstate = batch.initial_state()
+    caches = [R.build_rve_operator_cache(m) for m in models]
+    sstates = [R.initialize_rve_state(m, c) for m, c in zip(models, caches)]
+    ostates = [om.initial_state() for om in omodels]
+    osettings = OJ.UpdateSettings(50, 1e-10, 1e-12, 1e-10)
+    plastic = 0.0
+    for k in range(1, steps + 1):
+        dts = times[:, k] - times[:, k - 1]
+        plain = B.solve_rve_batch(batch, strains[:, k], dts, _settings(), bstate)
+        resp = B.solve_rve_batch(batch, strains[:, k], dts, _settings(), bstate, need_tangent=True)
+        assert resp.effective_tangent.shape == (len(models), 6, 6)
+        assert np.array_equal(resp.macro_stress, plain.macro_stress)
+        assert np.all(resp.diagnostics["linear_iterations"] > plain.diagnostics["linear_iterations"])
+        for i, (m, c) in enumerate(zip(models, caches)):
+            single = R.solve_rve_microequilibrium(
+                m, R.RVERequest(strains[i, k], float(dts[i]), _settings(),
+                                R.MaterialResponseRequirements(tangent=True)),
+                sstates[i], c, retain_micro_fields=False)
+            ref = OR.solve_increment(omodels[i], strains[i, k], float(dts[i]), osettings, ostates[i],
+                                     need_tangent=True)
+            got = resp.effective_tangent[i]
+            assert _rel(got, ref["effective_tangent"]) < 1e-8, (k, i)
+            assert _rel(got, single.effective_tangent) < 1e-9, (k, i)
+            plastic = max(plastic, ref["diagnostics"]["viscoplastic_active_volume_fraction"])
+            sstates[i], ostates[i] = single.state, ref["state"]
+        bstate = resp.state
+    assert plastic > 0.3
+    # the tangent of the last solve again: the same bits
+    res = (B._lib.RVEResult * len(models))()
+    status, again = B.effective_tangents_raw(batch, res)
+    assert status == B._lib.CIA_OK
+    assert np.array_equal(again, resp.effective_tangent)
+
+
 def test_batch_fp32_error_against_fp64_oracle(gpu):
     from concrete_impact_ai_amd import batch as B
 
@@ -137,17 +184,20 @@ def test_batch_fp32_error_against_fp64_oracle(gpu):
     batch = B.RVEBatch(models)
     st = batch.initial_state()
     ost = [om.initial_state() for om in omodels]
-    worst = 0.0
+    worst = worst_t = 0.0
     for k in range(1, steps + 1):
         dts = times[:, k] - times[:, k - 1]
-        resp = B.solve_rve_batch(batch, strains[:, k], dts, _settings(), st)
+        resp = B.solve_rve_batch(batch, strains[:, k], dts, _settings(), st, need_tangent=True)
         for i, om in enumerate(omodels):
             ref = OR.solve_increment(om, strains[i, k], float(dts[i]), OJ.UpdateSettings(), ost[i],
-                                     need_tangent=False)
+                                     need_tangent=True)
             worst = max(worst, _rel(resp.macro_stress[i], ref["macro_stress"]))
+            worst_t = max(worst_t, _rel(resp.effective_tangent[i], ref["effective_tangent"]))
             ost[i] = ref["state"]
         st = resp.state
+    print(f"fp32 batch: stress {worst:.2e}, effective tangent {worst_t:.2e}", flush=True)
     assert worst < 2e-6, worst
+    assert worst_t < 2e-7, worst_t
 
 
 def test_batch_failure_reasons_match_single_rve(gpu):
@@ -174,6 +224,11 @@ def test_batch_failure_reasons_match_single_rve(gpu):
             expected.append(e.reason)
     assert list(resp.reasons) == expected
     assert "maximum_iterations_exceeded" in expected and "ok" in expected
+    tr = B.solve_rve_batch(batch, eps, np.full(len(models), 1e-5), _settings(),
+                           batch.initial_state(), raise_on_failure=False, need_tangent=True)
+    assert list(tr.reasons) == expected
+    for i, why in enumerate(expected):
+        assert np.all(np.isfinite(tr.effective_tangent[i])) == (why == "ok"), i
     with pytest.raises(R.RVEEquilibriumError) as err:
         B.solve_rve_batch(batch, eps, np.full(len(models), 1e-5), _settings(), batch.initial_state())
     assert err.value.reason == "maximum_iterations_exceeded"

@@ -211,7 +211,8 @@ def test_fe2_host_state_codec_round_trip(gpu):
 def test_fe2_batched_backend_equals_serial(gpu):
     """The batched backend advances all macro points' RVEs in one batched solve;
     stresses, energies, iteration counts and the macro diagnostics equal the serial
-    per-point solves to the Newton tolerance.  Tangent requests run per point."""
+    per-point solves to the Newton tolerance; so do the effective tangents of a step
+    with a tangent request (batched Schur-complement solves, 1e-8)."""
     from concrete_impact_ai_amd import fe2
     from concrete_impact_ai_amd import materials as M
     from concrete_impact_ai_amd import rve as R
@@ -295,10 +296,11 @@ def test_fe2_batched_backend_chunks_large_point_sets(gpu):
 
 
 def test_fe2_tangent_step_writes_the_whole_next_state(gpu):
-    """A step with a tangent request (per-point device solves) from a batched state
-    returns the same next RVE states as the batched step without tangent, elastic
-    points' rows included (all zero): the single-RVE assembly writes every row of its
-    next-state output, which the adapter allocates uninitialised."""
+    """A step with a tangent request from a batched state returns the same next RVE
+    states as the batched step without tangent, in the batched backend and in the
+    per-point device solves of the serial backend, elastic points' rows included (all
+    zero): the single-RVE assembly writes every row of its next-state output, which
+    the adapter allocates uninitialised."""
     import torch
 
     from concrete_impact_ai_amd import fe2
@@ -309,6 +311,7 @@ def test_fe2_tangent_step_writes_the_whole_next_state(gpu):
     rng = np.random.default_rng(23)
     npts = 6
     model, bat = _two_phase_adapter(M, R, S, fe2, "batched", 1)
+    _, ser = _two_phase_adapter(M, R, S, fe2, "serial", 1)
     try:
         st = bat.initialize_state(npts)
         for step in range(3):
@@ -320,11 +323,14 @@ def test_fe2_tangent_step_writes_the_whole_next_state(gpu):
         strains = rng.uniform(-1.0, 1.0, (npts, 6)) * 8e-3
         settings = _request(M, strains).update_settings
         _, s_b = bat.solve_batch(strains, np.full(npts, 1e-5), settings, st, False)
-        _, s_t = bat.solve_batch(strains, np.full(npts, 1e-5), settings, st, True)
-        pb, pt = s_b.packed.cpu().numpy(), s_t.packed.cpu().numpy()
+        pb = s_b.packed.cpu().numpy()
         scale = max(np.max(np.abs(pb)), 1e-30)
-        assert np.max(np.abs(pt - pb)) <= 1e-10 * scale
         elastic = np.repeat(np.asarray(model.element_phase_ids) == 1, 8)
-        assert not pt[:, :, elastic].any()
+        for mat in (bat, ser):
+            _, s_t = mat.solve_batch(strains, np.full(npts, 1e-5), settings, st, True)
+            pt = s_t.packed.cpu().numpy()
+            assert np.max(np.abs(pt - pb)) <= 1e-10 * scale
+            assert not pt[:, :, elastic].any()
     finally:
         bat.close()
+        ser.close()
