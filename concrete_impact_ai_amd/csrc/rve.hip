@@ -1374,6 +1374,10 @@ enum {
 };
 
 // NPH >= number of phases: the per-phase sums use compile-time register indices.
+// One lane per Gauss point, so that every point row is read in coalesced lines: a
+// block takes the 8 kBlock points of the elements [kBlock b, kBlock (b + 1)) in 8
+// rounds (one partial column per block, as for the element-per-lane kernels), and
+// the lane of point q of element e also adds corner a = q of the virtual-work sum.
 template <class T, int NPH>
 __global__ void __launch_bounds__(kBlock)
     k_response(Grid G, const uint8_t* __restrict__ ephase, double dt, const T* __restrict__ sig,
@@ -1383,64 +1387,65 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int RN = R_PHASE + 9 * NPH;
   __shared__ double lds[kBlock / 64];
   __shared__ double red[kBlock / 64][RN];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t np = G.E * 8;
   double v[RN];
 #pragma unroll
   for (int s = 0; s < RN; ++s) v[s] = 0.0;
   double mn = INFINITY, mx = -INFINITY;
-  if (e < G.E) {
+  const double hx = 1.0 / G.g.ihx, hy = 1.0 / G.g.ihy, hz = 1.0 / G.g.ihz;
+  const int64_t p0 = (int64_t)blockIdx.x * (8 * kBlock) + threadIdx.x;
+  for (int rd = 0; rd < 8; ++rd) {
+    const int64_t p = p0 + rd * kBlock;
+    if (p >= np) break;
+    const int64_t e = p >> 3;
+    const int a = (int)(p & 7);
     const int ph = ephase[e];
-    double pe[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // this element's phase sums
-    for (int q = 0; q < 8; ++q) {
-      const int64_t p = 8 * e + q;
-      double mw = 0.0;
+    double sg[6], mw = 0.0;
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const double sg = (double)sig[m * np + p];
-        v[R_SIG + m] += sg;
-        pe[1 + m] += sg;
-        mw += sg * (dt * (double)rate[m * np + p]);
-      }
-      v[R_MICRO_WORK] += mw;
-      v[R_PSI] += (double)psi[p];
-      v[R_DISS] += (double)diss[p];
-      pe[0] += 1.0;
-      pe[7] += (double)psi[p];
-      pe[8] += (double)diss[p];
-      const double pw = (double)diag[CIA_D_INCREMENTAL_WORK_DENSITY * np + p];
-      v[R_POS_WORK] += fmax(pw, 0.0);
-      v[R_NEG_WORK] += fmin(pw, 0.0);
-      v[R_UNLOAD] += pw < 0.0 ? 1.0 : 0.0;
-      v[R_ACTIVE] += (double)diag[CIA_D_VISCOPLASTIC_ACTIVE * np + p] > 0.5 ? 1.0 : 0.0;
-      const double mg = (double)diag[CIA_D_YIELD_ACTIVATION_MARGIN * np + p];
-      if (isfinite(mg)) {
-        v[R_MARGIN_N] += 1.0;
-        mn = fmin(mn, mg);
-        mx = fmax(mx, mg);
-      }
+    for (int m = 0; m < 6; ++m) {
+      sg[m] = (double)sig[m * np + p];
+      v[R_SIG + m] += sg[m];
+      mw += sg[m] * (dt * (double)rate[m * np + p]);
+    }
+    v[R_MICRO_WORK] += mw;
+    const double ps = (double)psi[p], ds = (double)diss[p];
+    v[R_PSI] += ps;
+    v[R_DISS] += ds;
+    const double pw = (double)diag[CIA_D_INCREMENTAL_WORK_DENSITY * np + p];
+    v[R_POS_WORK] += fmax(pw, 0.0);
+    v[R_NEG_WORK] += fmin(pw, 0.0);
+    v[R_UNLOAD] += pw < 0.0 ? 1.0 : 0.0;
+    v[R_ACTIVE] += (double)diag[CIA_D_VISCOPLASTIC_ACTIVE * np + p] > 0.5 ? 1.0 : 0.0;
+    const double mg = (double)diag[CIA_D_YIELD_ACTIVATION_MARGIN * np + p];
+    if (isfinite(mg)) {
+      v[R_MARGIN_N] += 1.0;
+      mn = fmin(mn, mg);
+      mx = fmax(mx, mg);
     }
 #pragma unroll
     for (int P = 0; P < NPH; ++P) {
       if (ph == P) {
+        v[R_PHASE + 9 * P] += 1.0;
 #pragma unroll
-        for (int m = 0; m < 9; ++m) v[R_PHASE + 9 * P + m] += pe[m];
+        for (int m = 0; m < 6; ++m) v[R_PHASE + 9 * P + 1 + m] += sg[m];
+        v[R_PHASE + 9 * P + 7] += ps;
+        v[R_PHASE + 9 * P + 8] += ds;
       }
     }
-    // virtual work stress: sum_a A(x_a)^T F_a with x_a the unwrapped node position
-    const int i = (int)(e % G.nx), j = (int)((e / G.nx) % G.ny), k = (int)(e / ((int64_t)G.nx * G.ny));
-    const double hx = 1.0 / G.g.ihx, hy = 1.0 / G.g.ihy, hz = 1.0 / G.g.ihz;
-    for (int a = 0; a < 8; ++a) {
-      const double x = (i + (a & 1)) * hx, y = (j + ((a >> 1) & 1)) * hy, z = (k + ((a >> 2) & 1)) * hz;
-      const double fx = (double)F[(3 * a + 0) * G.E + e], fy = (double)F[(3 * a + 1) * G.E + e],
-                   fz = (double)F[(3 * a + 2) * G.E + e];
-      v[R_VIRTUAL + 0] += x * fx;
-      v[R_VIRTUAL + 1] += y * fy;
-      v[R_VIRTUAL + 2] += z * fz;
-      v[R_VIRTUAL + 3] += 0.5 * z * fy + 0.5 * y * fz;
-      v[R_VIRTUAL + 4] += 0.5 * z * fx + 0.5 * x * fz;
-      v[R_VIRTUAL + 5] += 0.5 * y * fx + 0.5 * x * fy;
-    }
+    // virtual work stress: A(x_a)^T F_a with x_a the unwrapped position of corner a
+    const unsigned eu = (unsigned)e, rw = eu / (unsigned)G.nx;
+    const int i = (int)(eu - rw * (unsigned)G.nx);
+    const int k = (int)(rw / (unsigned)G.ny);
+    const int j = (int)(rw - (unsigned)k * (unsigned)G.ny);
+    const double x = (i + (a & 1)) * hx, y = (j + ((a >> 1) & 1)) * hy, z = (k + ((a >> 2) & 1)) * hz;
+    const double fx = (double)F[(3 * a + 0) * G.E + e], fy = (double)F[(3 * a + 1) * G.E + e],
+                 fz = (double)F[(3 * a + 2) * G.E + e];
+    v[R_VIRTUAL + 0] += x * fx;
+    v[R_VIRTUAL + 1] += y * fy;
+    v[R_VIRTUAL + 2] += z * fz;
+    v[R_VIRTUAL + 3] += 0.5 * z * fy + 0.5 * y * fz;
+    v[R_VIRTUAL + 4] += 0.5 * z * fx + 0.5 * x * fz;
+    v[R_VIRTUAL + 5] += 0.5 * y * fx + 0.5 * x * fy;
   }
   // one pass of wave sums, then a single LDS exchange
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1546,6 +1551,48 @@ __global__ void __launch_bounds__(kBlock)
   }
   const double m = block_max<kBlock>(nrm, lds);
   if (threadIdx.x == 0) max_partials[blockIdx.x] = m;
+}
+
+// The response partials reduced on the device, one block per output (fixed order):
+// out[0, nrows) row sums of `part` (R_N rows of ge columns; out[nrows, R_N) = 0),
+// out[R_N], out[R_N + 1] min / max of the margin partials, out[R_N + 2 + s] the N_N
+// node sums, then the fluctuation and class-reaction maxima.
+constexpr int kRespOut = R_N + 4 + N_N;
+
+__global__ void __launch_bounds__(kBlock)
+    k_response_rows(const double* __restrict__ part, int ge, int nrows, const double* __restrict__ mm,
+                    const double* __restrict__ np_part, int gfull, const double* __restrict__ np_max,
+                    const double* __restrict__ cl_max, int gc, double* __restrict__ out) {
+  __shared__ double lds[kBlock / 64];
+  const int b = (int)blockIdx.x;
+  const int t = (int)threadIdx.x;
+  if (b < R_N) {
+    double a = 0.0;
+    if (b < nrows)
+      for (int c = t; c < ge; c += kBlock) a += part[(int64_t)b * ge + c];
+    a = block_sum<kBlock>(a, lds);
+    if (t == 0) out[b] = a;
+  } else if (b < R_N + 2) {
+    const int hi = b - R_N;
+    double a = hi ? -INFINITY : INFINITY;
+    for (int c = t; c < ge; c += kBlock) a = hi ? fmax(a, mm[2 * c + 1]) : fmin(a, mm[2 * c]);
+    a = hi ? block_max<kBlock>(a, lds) : block_min<kBlock>(a, lds);
+    if (t == 0) out[b] = a;
+  } else if (b < R_N + 2 + N_N) {
+    const int s = b - R_N - 2;
+    double a = 0.0;
+    for (int c = t; c < gfull; c += kBlock) a += np_part[(int64_t)s * gfull + c];
+    a = block_sum<kBlock>(a, lds);
+    if (t == 0) out[b] = a;
+  } else {
+    const bool fl = b == R_N + 2 + N_N;
+    const double* src = fl ? np_max : cl_max;
+    const int n = fl ? gfull : gc;
+    double a = 0.0;
+    for (int c = t; c < n; c += kBlock) a = fmax(a, src[c]);
+    a = block_max<kBlock>(a, lds);
+    if (t == 0) out[b] = a;
+  }
 }
 
 template <class T>
@@ -1663,6 +1710,13 @@ struct cia_rve {
   cia_point_failure* rec = nullptr;
   double* red = nullptr;  // response partials
   int64_t red_len = 0;
+  // pinned host staging of the read-backs: a pageable destination adds a staged copy
+  // to every host round trip of the Newton and PCG control
+  struct Staging {
+    cia::Scalars sc;
+    unsigned long long ff;
+    double resp[cia::kRespOut];
+  }* hst = nullptr;
   int pcg_total = 0;
   static constexpr int kHints = 32;
   int its_hint[kHints] = {0};  // PCG iterations of Newton iteration k in the last increment
@@ -1686,6 +1740,20 @@ struct cia_rve {
 };
 
 namespace cia {
+
+// The scalar block (and, with ff, the first failing point) read back through the
+// pinned staging copy, with one synchronisation.
+static cia_status fetch_scalars(cia_rve* R, hipStream_t st, Scalars* hs,
+                                unsigned long long* ff = nullptr) {
+  CIA_HIP(hipMemcpyAsync(&R->hst->sc, R->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+  if (ff)
+    CIA_HIP(hipMemcpyAsync(&R->hst->ff, R->first_fail, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipStreamSynchronize(st));
+  if (hs) *hs = R->hst->sc;
+  if (ff) *ff = R->hst->ff;
+  return CIA_OK;
+}
 
 static cia_status dmalloc(void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -1929,15 +1997,13 @@ struct Ops {
       }
       CIA_HIP(hipGetLastError());
       CIA_TRY(norm2_async(R, st, 3 * C, x, 3));  // ||x||^2 read with the convergence flag
-      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-      CIA_HIP(hipStreamSynchronize(st));
+      CIA_TRY(fetch_scalars(R, st, &hs));
       done = hs.done;
       prof::flush(done ? hs.iters : 1 << 30);
       if (done) break;
     }
     if (!done) {
-      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-      CIA_HIP(hipStreamSynchronize(st));
+      CIA_TRY(fetch_scalars(R, st, &hs));
     }
     *iters_out = hs.iters;
     *flag_out = hs.done;
@@ -1992,15 +2058,13 @@ struct Ops {
       }
       CIA_HIP(hipGetLastError());
       it += n;
-      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-      CIA_HIP(hipStreamSynchronize(st));
+      CIA_TRY(fetch_scalars(R, st, &hs));
       prof::flush();
       done = hs.done;
       if (done) break;
     }
     if (!done) {
-      CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-      CIA_HIP(hipStreamSynchronize(st));
+      CIA_TRY(fetch_scalars(R, st, &hs));
     }
     *iters_out = hs.iters;
     *flag_out = hs.done;
@@ -2008,8 +2072,7 @@ struct Ops {
     if (x_norm2) {
       CIA_TRY(norm2_async(R, st, 3 * C, x, 3));
       Scalars h2;
-      CIA_HIP(hipMemcpyAsync(&h2, R->sc, sizeof(h2), hipMemcpyDeviceToHost, st));
-      CIA_HIP(hipStreamSynchronize(st));
+      CIA_TRY(fetch_scalars(R, st, &h2));
       *x_norm2 = h2.norm2[3];
     }
     return CIA_OK;
@@ -2018,8 +2081,7 @@ struct Ops {
 
 static cia_status read_norms(cia_rve* R, hipStream_t st, double out[4]) {
   Scalars hs;
-  CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-  CIA_HIP(hipStreamSynchronize(st));
+  CIA_TRY(fetch_scalars(R, st, &hs));
   prof::flush();
   for (int i = 0; i < 4; ++i) out[i] = hs.norm2[i];
   return CIA_OK;
@@ -2040,9 +2102,7 @@ static cia_status norm2_async(cia_rve* R, hipStream_t st, int64_t n, const doubl
 static cia_status read_norms_fail(cia_rve* R, hipStream_t st, double out[4],
                                   unsigned long long* ff) {
   Scalars hs;
-  CIA_HIP(hipMemcpyAsync(&hs, R->sc, sizeof(hs), hipMemcpyDeviceToHost, st));
-  CIA_HIP(hipMemcpyAsync(ff, R->first_fail, sizeof(*ff), hipMemcpyDeviceToHost, st));
-  CIA_HIP(hipStreamSynchronize(st));
+  CIA_TRY(fetch_scalars(R, st, &hs, ff));
   prof::flush();
   for (int i = 0; i < 4; ++i) out[i] = hs.norm2[i];
   return CIA_OK;
@@ -2062,8 +2122,9 @@ static cia_status norm2_of(cia_rve* R, hipStream_t st, int64_t n, const double* 
 }
 
 static cia_status check_fail(cia_rve* R, hipStream_t st, unsigned long long* ff) {
-  CIA_HIP(hipMemcpyAsync(ff, R->first_fail, sizeof(*ff), hipMemcpyDeviceToHost, st));
+  CIA_HIP(hipMemcpyAsync(&R->hst->ff, R->first_fail, sizeof(*ff), hipMemcpyDeviceToHost, st));
   CIA_HIP(hipStreamSynchronize(st));
+  *ff = R->hst->ff;
   return CIA_OK;
 }
 
@@ -2246,31 +2307,19 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
     hipLaunchKernelGGL(k_class_max<T>, dim3(gc), dim3(kBlock), 0, st, R->G, (const T*)R->F, cl_max);
     CIA_HIP(hipGetLastError());
     const int64_t total = (int64_t)R_N * ge + 2 * ge + (int64_t)N_N * gfull + gfull + gc;
-    std::vector<double> h(total);
-    CIA_HIP(hipMemcpyAsync(h.data(), R->red, total * sizeof(double), hipMemcpyDeviceToHost, st));
+    const int nrows = R_PHASE + 9 * std::min(R->n_phases, CIA_MAX_PHASES);
+    double* dout = R->red + total;
+    hipLaunchKernelGGL(k_response_rows, dim3(kRespOut), dim3(kBlock), 0, st, part, (int)ge, nrows,
+                       mm, np_part, (int)gfull, np_max, cl_max, (int)gc, dout);
+    CIA_HIP(hipGetLastError());
+    const double* h = R->hst->resp;
+    CIA_HIP(hipMemcpyAsync(R->hst->resp, dout, kRespOut * sizeof(double), hipMemcpyDeviceToHost,
+                           st));
     CIA_HIP(hipStreamSynchronize(st));
-    double sums[R_N];
-    for (int s = 0; s < R_N; ++s) {
-      double a = 0.0;
-      for (unsigned b = 0; b < ge; ++b) a += h[(int64_t)s * ge + b];
-      sums[s] = a;
-    }
-    double mn = INFINITY, mx = -INFINITY;
-    for (unsigned b = 0; b < ge; ++b) {
-      mn = std::min(mn, h[(int64_t)R_N * ge + 2 * b]);
-      mx = std::max(mx, h[(int64_t)R_N * ge + 2 * b + 1]);
-    }
-    const double* hn = h.data() + (int64_t)R_N * ge + 2 * ge;
-    double nsum[N_N];
-    for (int s = 0; s < N_N; ++s) {
-      double a = 0.0;
-      for (unsigned b = 0; b < gfull; ++b) a += hn[(int64_t)s * gfull + b];
-      nsum[s] = a;
-    }
-    double flmax = 0.0;
-    for (unsigned b = 0; b < gfull; ++b) flmax = std::max(flmax, hn[(int64_t)N_N * gfull + b]);
-    double clmax = 0.0;
-    for (unsigned b = 0; b < gc; ++b) clmax = std::max(clmax, hn[(int64_t)N_N * gfull + gfull + b]);
+    const double* sums = h;
+    const double mn = h[R_N], mx = h[R_N + 1];
+    const double* nsum = h + R_N + 2;
+    const double flmax = h[R_N + 2 + N_N], clmax = h[R_N + 3 + N_N];
 
     const double jw = R->G.g.jw, V = R->volume;
     for (int m = 0; m < 6; ++m) res->macro_stress[m] = sums[R_SIG + m] * jw / V;
@@ -2464,8 +2513,13 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
   ALLOC(R->class0, 3 * 8);
   ALLOC(R->first_fail, 8);
   ALLOC(R->rec, sizeof(cia_point_failure));
+  if (hipHostMalloc((void**)&R->hst, sizeof(cia_rve::Staging)) != hipSuccess) {
+    set_error_msg("hipHostMalloc of the read-back staging failed");
+    cia_rve_destroy(R);
+    return CIA_ERR_OUT_OF_MEMORY;
+  }
   R->red_len = (int64_t)R_N * grid_for(R->E) + 2 * grid_for(R->E) + (int64_t)N_N * grid_for(R->N) +
-               grid_for(R->N) + grid_for(R->C) + 64;
+               grid_for(R->N) + grid_for(R->C) + kRespOut + 64;
   ALLOC(R->red, R->red_len * 8);
   R->prec = d->linear.preconditioner;
   if (R->prec == CIA_PCG_SPECTRAL && R->C > 1) {
@@ -2541,6 +2595,7 @@ cia_status cia_rve_destroy(cia_rve* R) {
   if (R->uab) (void)hipFree(R->uab);
   if (R->tw) (void)hipFree(R->tw);
   if (R->norm_part) (void)hipFree(R->norm_part);
+  if (R->hst) (void)hipHostFree(R->hst);
   for (auto& g : R->graphs) (void)hipGraphExecDestroy(g.exec);
   if (R->cap_stream) (void)hipStreamDestroy(R->cap_stream);
   delete R;
