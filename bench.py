@@ -1187,8 +1187,10 @@ def main() -> int:
     #            24 element force rows written.
     #  pcg_update: p, y, x, r read and x, r written once per class, then the
     #            x-transformed spectra of the three residual components written.
-    #  assemble: committed state read + trial state written (J2 points), stress and
-    #            compact tangent written, w read once per class, force rows written.
+    #  assemble: a Newton pass (all but one launch per increment): committed state
+    #            read (J2 points), compact tangent written, w read once per class,
+    #            force rows written (the final pass also writes stresses, next state
+    #            and response rows).
     #  precond: one application of the spectral preconditioner after the x-DFT
     #            (forward y pass, z pass with the symbol solve, inverse y pass,
     #            inverse x pass): the three components' spectra (16 B each) read and
@@ -1202,7 +1204,7 @@ def main() -> int:
         "pcg_elem": n_gp * bpe * (2 + 6 * plastic_frac) + n_dof * 8 * 3 + n_el * 24 * bpe,
         "pcg_node": n_el * 24 * bpe + n_dof * 8 * 2,
         "pcg_update": n_dof * 8 * 6 + n_el * 48,
-        "assemble": mortar_gp * bpe * 16 + n_gp * bpe * (6 + 2 + 6 * plastic_frac) + n_dof * 8
+        "assemble": mortar_gp * bpe * 8 + n_gp * bpe * (2 + 6 * plastic_frac) + n_dof * 8
                     + n_el * 24 * bpe,
         "precond": n_el * 408,
         "gather": n_el * 24 * bpe + n_dof * 8,

@@ -176,6 +176,10 @@ struct Macro {
 // Assembly: strain -> constitutive update -> element forces.  One lane per
 // Gauss point (8 lanes per element): every per-point row is read and written
 // by consecutive lanes, and the element force is reduced across the 8 lanes.
+// The Newton passes (EXTRA false) write only what the Newton step reads: the
+// compact tangent and the force rows.  The final pass at the converged
+// fluctuation (EXTRA) evaluates the points once more and writes the stresses,
+// the next state and the response rows.
 template <class T, bool EXTRA>
 __global__ void __launch_bounds__(kBlock)
     k_assemble(Grid G, const PhaseTable* __restrict__ tab, const uint8_t* __restrict__ ephase,
@@ -246,17 +250,19 @@ __global__ void __launch_bounds__(kBlock)
   const int st = material_point<T, EXTRA>(P, S, dt, eps, EXTRA ? rate : nullptr, epsp, qb, ga, o,
                                           EXTRA ? &ex : nullptr, nullptr);
   if (st != CIA_PT_OK) atomicMin(first_fail, (unsigned long long)p);
+  if (EXTRA) {
 #pragma unroll
-  for (int m = 0; m < 6; ++m) sig[m * np + p] = o.sig[m];
-  if (j2) {
+    for (int m = 0; m < 6; ++m) sig[m * np + p] = o.sig[m];
+    if (j2) {
 #pragma unroll
-    for (int m = 0; m < 6; ++m) state_t[m * np + p] = o.epsp[m];
-    state_t[6 * np + p] = o.qbar;
-    state_t[7 * np + p] = o.gam;
-  } else {
-    // elastic points carry no internal state: every row of the output is written
+      for (int m = 0; m < 6; ++m) state_t[m * np + p] = o.epsp[m];
+      state_t[6 * np + p] = o.qbar;
+      state_t[7 * np + p] = o.gam;
+    } else {
+      // elastic points carry no internal state: every row of the output is written
 #pragma unroll
-    for (int m = 0; m < 8; ++m) state_t[m * np + p] = (T)0;
+      for (int m = 0; m < 8; ++m) state_t[m * np + p] = (T)0;
+    }
   }
   tan[0 * np + p] = o.tan.d;
   tan[1 * np + p] = o.tan.r;
@@ -1677,7 +1683,6 @@ struct cia_rve {
   // device
   cia::PhaseTable* d_tab = nullptr;
   uint8_t* d_ephase = nullptr;
-  void* st_trial[2] = {nullptr, nullptr};
   void* sig[2] = {nullptr, nullptr};
   void* tan[2] = {nullptr, nullptr};
   void* F = nullptr;
@@ -1812,8 +1817,8 @@ struct Ops {
                          R->first_fail);
     else
       hipLaunchKernelGGL((k_assemble<T, false>), dim3(grid), dim3(kBlock), 0, st, R->G, R->d_tab,
-                         R->d_ephase, S, dt, w, wc, M, (const T*)state_c, (T*)state_t,
-                         (T*)R->sig[slot], (T*)R->tan[slot], (T*)R->F, nullptr, nullptr, nullptr,
+                         R->d_ephase, S, dt, w, wc, M, (const T*)state_c, (T*)nullptr,
+                         (T*)nullptr, (T*)R->tan[slot], (T*)R->F, nullptr, nullptr, nullptr,
                          nullptr, nullptr, R->first_fail);
     prof::end(st);
     CIA_HIP(hipGetLastError());
@@ -2161,7 +2166,7 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
   };
 
   // assembly at the committed fluctuation (iteration 0); ||w||^2 -> norm2[2]
-  CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, R->st_trial[cur], S, dt));
+  CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, nullptr, S, dt));
   res->gp_updates += R->np;
   CIA_TRY(Ops<T>::gather(R, st, R->r, nullptr, 0));
   CIA_TRY(norm2_async(R, st, C3, R->w, 2));
@@ -2225,7 +2230,7 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
                            R->w_cand);
         CIA_TRY(norm2_async(R, st, C3, R->w_cand, 2));
         CIA_TRY(Ops<T>::assemble(R, st, false, cand, R->w_cand, R->wc_buf, M, state_c,
-                                 R->st_trial[cand], S, dt));
+                                 nullptr, S, dt));
         res->gp_updates += R->np;
         CIA_TRY(Ops<T>::gather(R, st, R->r_cand, nullptr, 1));
         CIA_TRY(read_norms_fail(R, st, nv, &ff));
@@ -2257,8 +2262,7 @@ static cia_status solve_impl(cia_rve* R, hipStream_t st, const void* state_c, co
                          R->w_cand);
       std::swap(R->w, R->w_cand);
       CIA_TRY(norm2_async(R, st, C3, R->w, 2));
-      CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, R->st_trial[cur], S,
-                               dt));
+      CIA_TRY(Ops<T>::assemble(R, st, false, cur, R->w, R->wc_buf, M, state_c, nullptr, S, dt));
       res->gp_updates += R->np;
       CIA_TRY(Ops<T>::gather(R, st, R->r, nullptr, 0));
       CIA_TRY(read_norms_fail(R, st, nv, &ff));
@@ -2477,7 +2481,6 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
   ALLOC(R->d_tab, sizeof(PhaseTable));
   ALLOC(R->d_ephase, R->E);
   for (int k = 0; k < 2; ++k) {
-    ALLOC(R->st_trial[k], 8 * np * ts);
     ALLOC(R->sig[k], 6 * np * ts);
     ALLOC(R->tan[k], 8 * np * ts);
   }
@@ -2584,7 +2587,7 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
 
 cia_status cia_rve_destroy(cia_rve* R) {
   if (!R) return CIA_OK;
-  void* ptrs[] = {R->d_tab, R->d_ephase, R->st_trial[0], R->st_trial[1], R->sig[0], R->sig[1],
+  void* ptrs[] = {R->d_tab, R->d_ephase, R->sig[0], R->sig[1],
                   R->tan[0], R->tan[1], R->F, R->strain_out, R->rate_out, R->psi_out, R->diss_out,
                   R->diag_out, R->w, R->w_cand, R->dw, R->r, R->r_cand, R->wc_buf, R->pr, R->pz,
                   R->pp[0], R->pp[1], R->py, R->sens, R->minv, R->u_full, R->f_full, R->partials,
