@@ -128,8 +128,97 @@ __device__ __forceinline__ void dft_stage_ct(const C2* __restrict__ in, C2* __re
 // STEP_A: out[(j2 n1 + k1)] = W_n^(j2 k1) sum_j1 in[n2 j1 + j2] W_n1^(j1 k1)
 // STEP_B: out[(k1 + n1 k2)] = sum_j2 in[j2 n1 + k1] W_n2^(j2 k2)
 // (j1 k1 mod n1 and j2 k2 mod n2 advance incrementally; j2 k1 < n needs no mod.)
-// w: twiddles of the axis in LDS.  The edges of the benchmark workloads run the
-// compile-time specialisation above (dft_stage_ct).
+// w: twiddles of the axis in LDS.  Edges 12, 16, 24, 32 and 48 run the
+// compile-time specialisation above (dft_stage_ct); a stage of odd length R >= 3
+// (a prime edge's direct DFT, or an odd factor) runs dft_stage_pairs.
+template <class C2>
+struct RealOf;
+template <>
+struct RealOf<float2> {
+  using type = float;
+};
+template <>
+struct RealOf<double2> {
+  using type = double;
+};
+
+// Outputs k and R - k (1 <= k <= (R - 1) / 2) of the R-point DFT (R odd) of
+// x_j = in[j st], from the sums and differences of the inputs j and R - j:
+//   X_k = x_0 + A -/+ i B,  X_(R-k) = x_0 + A +/- i B  (upper signs forward),
+//   A = sum_j (x_j + x_(R-j)) cos(2 pi j k / R),  B = sum_j (x_j - x_(R-j)) sin(2 pi j k / R),
+// j = 1 .. (R - 1) / 2: half the products and a quarter of the twiddle reads of the
+// two direct sums.  wr[m] = W_R^m = (cos, -sin)(2 pi m / R).
+template <bool INV, class C2>
+__device__ __forceinline__ void dft_pair(const C2* __restrict__ in, int st, int R, int k,
+                                         const C2* __restrict__ wr, C2& xk, C2& xrk) {
+  using F = typename RealOf<C2>::type;
+  const C2 x0 = in[0];
+  F ax = 0, ay = 0, bx = 0, by = 0;
+  const int h = (R - 1) >> 1;
+  int m = 0;
+  for (int j = 1; j <= h; ++j) {
+    m += k;
+    if (m >= R) m -= R;
+    const C2 xj = in[j * st], xm = in[(R - j) * st];
+    const C2 wm = wr[m];
+    const F c = wm.x, sn = -wm.y;
+    ax += (xj.x + xm.x) * c;
+    ay += (xj.y + xm.y) * c;
+    bx += (xj.x - xm.x) * sn;
+    by += (xj.y - xm.y) * sn;
+  }
+  // -i B = (by, -bx), +i B = (-by, bx)
+  const F sx = INV ? -by : by, sy = INV ? bx : -bx;
+  xk.x = x0.x + ax + sx;
+  xk.y = x0.y + ay + sy;
+  xrk.x = x0.x + ax - sx;
+  xrk.y = x0.y + ay - sy;
+}
+
+// dft_stage for a stage whose small DFTs have odd length R >= 3: one thread per
+// output pair (k, R - k) or per k = 0 of a small DFT (dft_pair).  Same stage
+// definition as dft_stage.
+template <bool STEP_A, bool INV, class C2>
+__device__ __forceinline__ void dft_stage_pairs(const C2* __restrict__ in, C2* __restrict__ out,
+                                                int n_lines, int ls, int es, const DftAxis& ax,
+                                                const C2* __restrict__ w) {
+  const int n1 = ax.n1, n2 = ax.n2;
+  const int R = STEP_A ? n1 : n2;  // points of one small DFT
+  const int M = STEP_A ? n2 : n1;  // small DFTs per line
+  const int H = (R + 1) >> 1;      // tasks per small DFT: k = 0 and the (R - 1) / 2 pairs
+  const int per_line = M * H;
+  const C2* wr = STEP_A ? w : w + n1;
+  const C2* wn = w + n1 + n2;
+  const int st = (STEP_A ? n2 : n1) * es;
+  const float inv_pl = 1.0f / (float)per_line, inv_h = 1.0f / (float)H;
+  const int total = n_lines * per_line;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int line = fdiv(t, inv_pl), rem = t - line * per_line;
+    const int q = fdiv(rem, inv_h), k = rem - q * H;
+    const C2* src = in + line * ls + q * es;  // STEP_A: j2 = q; STEP_B: k1 = q
+    C2* dst = out + line * ls;
+    if (k == 0) {
+      C2 acc = src[0];
+      for (int j = 1; j < R; ++j) {
+        const C2 v = src[j * st];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      dst[(STEP_A ? q * n1 : q) * es] = acc;
+    } else {
+      C2 xa, xb;
+      dft_pair<INV>(src, st, R, k, wr, xa, xb);
+      if (STEP_A) {  // outputs k1 = k, R - k of j2 = q, times W_n^(j2 k1)
+        dst[(q * n1 + k) * es] = cmul(xa, wn[q * k], INV);
+        dst[(q * n1 + R - k) * es] = cmul(xb, wn[q * (R - k)], INV);
+      } else {  // outputs k2 = k, R - k of k1 = q
+        dst[(q + n1 * k) * es] = xa;
+        dst[(q + n1 * (R - k)) * es] = xb;
+      }
+    }
+  }
+}
+
 template <bool STEP_A, bool INV, class C2>
 __device__ __forceinline__ void dft_stage(const C2* __restrict__ in, C2* __restrict__ out,
                                           int n_lines, int ls, int es, const DftAxis& ax,
@@ -143,6 +232,10 @@ __device__ __forceinline__ void dft_stage(const C2* __restrict__ in, C2* __restr
     default: break;
   }
   const int n = ax.n, n1 = ax.n1, n2 = ax.n2;
+  if (((STEP_A ? n1 : n2) & 1) && (STEP_A ? n1 : n2) >= 3) {
+    dft_stage_pairs<STEP_A, INV>(in, out, n_lines, ls, es, ax, w);
+    return;
+  }
   const C2* w1 = w;
   const C2* w2 = w + n1;
   const C2* wn = w + n1 + n2;

@@ -195,7 +195,7 @@ def test_fp32_refined_rve_error_against_fp64_oracle(gpu):
 
 @pytest.mark.parametrize("n", [(2, 3, 4), (5, 5, 5), (6, 7, 8), (12, 13, 16), (17, 24, 31), (32, 9, 11),
                                (48, 16, 24), (12, 48, 32), (16, 16, 16), (5, 64, 48), (3, 32, 64),
-                               (48, 48, 48)])
+                               (48, 48, 48), (19, 23, 29), (21, 25, 27)])
 def test_spectral_preconditioner_is_exact_for_the_reference_medium(gpu, n):
     """On a homogeneous elastic RVE whose material is the reference medium the spectral
     preconditioner is the exact inverse of the stiffness (up to its FP32 storage).
