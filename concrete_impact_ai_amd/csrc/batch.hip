@@ -1076,7 +1076,7 @@ __global__ void __launch_bounds__(256) k_b_dft_x(Arena A) {
     const int64_t total_lines = 2 * (int64_t)D.ny * D.nz;
     const int64_t l0 = (int64_t)(t - D.t0[F_DX]) * lines;
     const int nl = (int)min<int64_t>(lines, total_lines - l0);
-    const int ld = nx + 1;
+    const int ld = nx | 1;  // odd row pitch: lanes a row apart fall in distinct LDS banks
     SC* X = dft_sm;
     SC* Y = dft_sm + lines * ld;
     SC* g = (SC*)A.uab + 2 * D.e0 + l0 * nx;
@@ -1115,7 +1115,7 @@ __global__ void __launch_bounds__(256) k_b_dft_strided(Arena A) {
     const Desc& D = A.desc[r];
     if (!pcg_active(A, A.ctl[r])) continue;
     const DftAxis& ax = ZPASS ? D.az : D.ay;
-    const int n = ax.n, ld = n + 1;
+    const int n = ax.n, ld = n | 1;  // odd row pitch (distinct LDS banks a row apart)
     const int tiles_x = (D.nx + kBatchDftTile - 1) / kBatchDftTile;
     const int lt = t - D.t0[FAM];
     const int xt = lt % tiles_x, rest = lt / tiles_x;

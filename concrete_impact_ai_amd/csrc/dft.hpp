@@ -290,7 +290,7 @@ constexpr int kDftMaxTw = 64 + 16 + 64;
 
 // Strided pass (y: stride nx, z: stride nx ny) over 16 consecutive x positions:
 // blockIdx.x = x tile, blockIdx.y = the other grid index (z for the y pass, y for
-// the z pass), blockIdx.z = component.  Dynamic LDS 2 * 16 (n + 1) double2.
+// the z pass), blockIdx.z = component.  Dynamic LDS 2 * 16 (n | 1) double2.
 template <bool INV, bool ZPASS>
 __global__ void __launch_bounds__(256)
     k_dft_strided(int nx, int ny, int nz, int64_t N, DftAxis ax, double2* __restrict__ data,
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(256)
   if (done && *done) return;
   extern __shared__ double2 dft_sm[];
   __shared__ double2 w[kDftMaxTw];
-  const int n = ax.n, ld = n + 1;  // one padded LDS row per line
+  const int n = ax.n, ld = n | 1;  // one LDS row per line, odd pitch (distinct banks a row apart)
   double2* A = dft_sm;
   double2* T = dft_sm + kDftTile * ld;
   const int i0 = blockIdx.x * kDftTile, o = blockIdx.y;

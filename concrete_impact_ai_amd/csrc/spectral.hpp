@@ -70,7 +70,7 @@ __device__ __forceinline__ void zcol_prefetch(int nx, int ny, int nz, int tile_w
                                               const double2* __restrict__ S, int64_t C,
                                               const hipFloatComplex* __restrict__ kinv,
                                               ZcolTile& t) {
-  t.ldz = nz + 1;
+  t.ldz = nz | 1;  // odd row pitch: lanes a row apart fall in distinct LDS banks
   t.TW = tile_w;
   const int i0 = xt * tile_w;
   t.width = min(tile_w, nx - i0);
@@ -106,8 +106,8 @@ __device__ __forceinline__ double zcol_solve(const ZcolTile& t, double2* __restr
                                              const double sr[3], double inv_n, const DftAxis& az,
                                              double2* sm, const double2* wz) {
   const int ldz = t.ldz, TW = t.TW, width = t.width, per = t.per;
-  double2* A = sm;                   // [3][TW][nz + 1]
-  double2* T = sm + 3 * TW * ldz;    // [3][TW][nz + 1]
+  double2* A = sm;                   // [3][TW][ldz]
+  double2* T = sm + 3 * TW * ldz;    // [3][TW][ldz]
 #pragma unroll
   for (int j = 0; j < kZcolPf; ++j) {
     const int u = threadIdx.x + j * blockDim.x;
@@ -186,12 +186,12 @@ __device__ __forceinline__ void xlines_update_fwd(int nx, int64_t l0, int nl, do
                                                   double2* __restrict__ S, int64_t C,
                                                   const DftAxis& ax, double2* sm,
                                                   const double2* wx, double acc[4]) {
-  const int ld = nx + 1, L = xline_group(nx);
+  const int ld = nx | 1, L = xline_group(nx);  // odd row pitch
   const int ncl = nl * nx;
   const int64_t c0 = l0 * nx;
   const float inv_nx = 1.0f / (float)nx;
-  double2* A = sm;               // [3][L][nx + 1]
-  double2* T = sm + 3 * L * ld;  // [3][L][nx + 1]
+  double2* A = sm;               // [3][L][ld]
+  double2* T = sm + 3 * L * ld;  // [3][L][ld]
 #pragma unroll
   for (int s = 0; s < 4; ++s) acc[s] = 0.0;
   const int t = threadIdx.x;
@@ -233,7 +233,7 @@ __device__ __forceinline__ void xlines_inv(int nx, int64_t l0, int nl,
                                            const double2* __restrict__ S, int64_t C,
                                            double* __restrict__ zout, const DftAxis& ax,
                                            double2* sm, const double2* wx) {
-  const int ld = nx + 1, L = xline_group(nx);
+  const int ld = nx | 1, L = xline_group(nx);  // odd row pitch
   const int ncl = nl * nx;
   const int64_t c0 = l0 * nx;
   const float inv_nx = 1.0f / (float)nx;
