@@ -56,7 +56,7 @@ class RVEExecutionSettings:
     build's backends and ``process_pool`` is executed by ``streams``).  ``batched``
     advances every macro point's RVE in one batched device solve
     (``concrete_impact_ai_amd.batch``); updates that request the effective tangent
-    (which the batched response does not carry) run as per-point solves."""
+    add the batched Schur-complement solves of the converged RVEs."""
 
     backend: str = "serial"
     workers: int = 1
