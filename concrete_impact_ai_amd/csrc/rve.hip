@@ -29,6 +29,7 @@
 
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -1716,7 +1717,9 @@ struct cia_rve {
   double* red = nullptr;  // response partials
   int64_t red_len = 0;
   // pinned host staging of the read-backs: a pageable destination adds a staged copy
-  // to every host round trip of the Newton and PCG control
+  // to every host round trip of the Newton and PCG control.  The device scalar block
+  // has the same layout up to `ff` (first_fail lives right after the scalars), so the
+  // scalars and the failure flag come back in one copy.
   struct Staging {
     cia::Scalars sc;
     unsigned long long ff;
@@ -1750,10 +1753,9 @@ namespace cia {
 // pinned staging copy, with one synchronisation.
 static cia_status fetch_scalars(cia_rve* R, hipStream_t st, Scalars* hs,
                                 unsigned long long* ff = nullptr) {
-  CIA_HIP(hipMemcpyAsync(&R->hst->sc, R->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-  if (ff)
-    CIA_HIP(hipMemcpyAsync(&R->hst->ff, R->first_fail, sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, st));
+  const size_t bytes = ff ? offsetof(cia_rve::Staging, ff) + sizeof(unsigned long long)
+                          : sizeof(Scalars);
+  CIA_HIP(hipMemcpyAsync(&R->hst->sc, R->sc, bytes, hipMemcpyDeviceToHost, st));
   CIA_HIP(hipStreamSynchronize(st));
   if (hs) *hs = R->hst->sc;
   if (ff) *ff = R->hst->ff;
@@ -2512,9 +2514,11 @@ cia_status cia_rve_create(const cia_rve_desc* d, cia_rve** out) {
   ALLOC(R->partials, std::max(8 * maxb, sp_parts) * 8);
   ALLOC(R->norm_part, 2 * 1024 * sizeof(double));
   ALLOC(R->counter, 64);
-  ALLOC(R->sc, sizeof(Scalars));
+  // the scalar block, then first_fail at the offset the staging copy has for it
+  ALLOC(R->sc, offsetof(cia_rve::Staging, ff) + sizeof(unsigned long long));
+  R->first_fail = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(R->sc) +
+                                                         offsetof(cia_rve::Staging, ff));
   ALLOC(R->class0, 3 * 8);
-  ALLOC(R->first_fail, 8);
   ALLOC(R->rec, sizeof(cia_point_failure));
   if (hipHostMalloc((void**)&R->hst, sizeof(cia_rve::Staging)) != hipSuccess) {
     set_error_msg("hipHostMalloc of the read-back staging failed");
@@ -2591,7 +2595,7 @@ cia_status cia_rve_destroy(cia_rve* R) {
                   R->tan[0], R->tan[1], R->F, R->strain_out, R->rate_out, R->psi_out, R->diss_out,
                   R->diag_out, R->w, R->w_cand, R->dw, R->r, R->r_cand, R->wc_buf, R->pr, R->pz,
                   R->pp[0], R->pp[1], R->py, R->sens, R->minv, R->u_full, R->f_full, R->partials,
-                  R->counter, R->sc, R->class0, R->first_fail, R->rec, R->red};
+                  R->counter, R->sc, R->class0, R->rec, R->red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (R->kinv) (void)hipFree(R->kinv);
